@@ -1,0 +1,195 @@
+/*
+ * mwdiem_hip.h -- C ABI of libmwdiem_hip.so, the MI355X (gfx950) implementation
+ * of MWDiEM's per-timestep DEM loop for convex polyhedra.
+ *
+ * Conventions
+ *   - Every entry point returns int: MWD_OK (0) or a negative MWD_E_* code.
+ *     mwd_last_error() returns a short text for the last failure on the
+ *     calling thread.
+ *   - Array arguments are DEVICE pointers (e.g. torch.Tensor.data_ptr() of a
+ *     tensor on the GPU), C-contiguous, in the reference's own layouts: [N,3]
+ *     rows for vectors, [N,4] for quaternions (w,x,y,z), [N,3,3] for inertia,
+ *     [N,Vmax,3] for vertices. Floating point is FP64 throughout.
+ *   - `stream` is a hipStream_t (NULL = the default stream). Calls are
+ *     stream-ordered and asynchronous unless stated otherwise.
+ *   - Units follow the reference: mm, kg, s.
+ *
+ * Reference interfaces replaced (kgillemot/MWDiEM):
+ *   mwd_verlet ............ __global__ Verlet, LIBRARIES/gpu_Verlet.cu:85-431,
+ *                           launched by gpu_Verlett, LIBRARIES/gpu_Verlet.py:18-50
+ *   mwd_contact_pairs ..... __global__ contact_detection,
+ *                           LIBRARIES/gpu_fine_contact_detection.cu:874-1322,
+ *                           launched by gpu_contact_detection,
+ *                           LIBRARIES/gpu_fine_contact_detection.py:18-73
+ *   mwd_cell_keys ......... cell keys, MWDiem.py:813-816
+ *   mwd_candidates ........ wall slab test MWDiem.py:807-811 followed by
+ *                           Linked_cell_hash,
+ *                           LIBRARIES/LIBRARY_Coarse_contact_detection.py:84-155
+ *   mwd_step .............. one iteration of the time loop, MWDiem.py:805-1001
+ *                           (candidates, gather, contact_detection, force
+ *                           assembly :977-981, Verlet), device-resident
+ */
+#ifndef MWDIEM_HIP_H
+#define MWDIEM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MWD_OK 0
+#define MWD_E_INVALID (-1)   /* bad argument (null pointer, negative size ...) */
+#define MWD_E_HIP (-2)       /* a HIP runtime call failed */
+#define MWD_E_CAPACITY (-3)  /* a workspace buffer is too small; see mwd_step_status() */
+#define MWD_E_KEYRANGE (-4)  /* a cell key fell outside the scene's key range */
+#define MWD_E_CELL (-5)      /* a cell holds more particles than the set-order emulation supports */
+
+/* Status bits of mwd_step_status().flags */
+#define MWD_F_PAIR_OVERFLOW 1u     /* more candidate pairs than pair_cap */
+#define MWD_F_CONTACT_OVERFLOW 2u  /* more contacts than contact_cap */
+#define MWD_F_KEYRANGE 4u          /* cell key outside [kmin, kmin+kspan) (clamped) */
+#define MWD_F_CELL 8u              /* cell occupancy above MWD_MAX_CELL */
+#define MWD_F_EPA_CAP 16u          /* some EPA hit its 25-expansion cap (.cu:443-447) */
+#define MWD_F_EPA_FACES 32u        /* some EPA ran out of face storage */
+
+#define MWD_EPA_MAX_FACES 128
+#define MWD_MAX_CELL 76            /* largest cell occupancy whose set order is emulated */
+
+/* Contact-law coefficients of Force (.cu:685-701), filled by mwd_law_init in
+ * the reference's operator order. */
+typedef struct mwd_law {
+    double Kn, mu, ln_en, ln_et, A_en, tc, Gn, Kt_num, inv_mu, dt;
+} mwd_law;
+
+/* One scene (one independent bed with its walls, grid and law). Rows
+ * [row0, row0+n_walls) are the walls, rows [row0+n_walls, row0+n_active) the
+ * particles that move; ids inside a scene are row-row0 (the reference's
+ * particle ids). Host struct; the library copies it. */
+typedef struct mwd_scene {
+    int64_t row0, n_walls, n_active;
+    double box[6];          /* syssize x_max, x_min, y_max, y_min, z_max, z_min */
+    double cell[3];         /* cellsize_x, _y, _z (MWDiem.py:578-580) */
+    int64_t ncx, ncy;       /* numcells_x, numcells_y */
+    int64_t kmin, kspan;    /* every key of the scene lies in [kmin, kmin+kspan) */
+    double wall_thr;        /* 6*max_particlesize (MWDiem.py:810) */
+    mwd_law law_p, law_w;   /* particle-particle and wall laws */
+} mwd_scene;
+
+/* Device-resident particle state (all device pointers, rows of all scenes). */
+typedef struct mwd_state {
+    int64_t n_rows;
+    int32_t vmax;
+    double *com, *v, *v12, *w, *wbf, *w12bf;   /* [n,3] */
+    double *q, *q12, *qall;                    /* [n,4] */
+    double *dxall;                             /* [n,3] */
+    double *verts;                             /* [n,vmax,3] world vertices */
+    const int32_t *nverts;                     /* [n] */
+    const double *mass, *I, *Iinv, *isize;     /* [n], [n,3,3], [n,3,3], [n] */
+    const double *fext, *text;                 /* [n,3] external force/torque */
+    const double *wall_n, *wall_c;             /* [n,3] wall normal and centre (rows of walls) */
+} mwd_state;
+
+typedef struct mwd_workspace mwd_workspace;
+
+/* Status of the last mwd_step / mwd_candidates on a workspace (synchronises
+ * the workspace's stream). */
+typedef struct mwd_status {
+    int64_t n_pairs;      /* candidate pairs (wall pairs first) */
+    int64_t n_wall_pairs;
+    int64_t n_contacts;   /* pairs with collision == 1 */
+    int64_t n_cells;
+    uint32_t flags;       /* MWD_F_* */
+} mwd_status;
+
+const char* mwd_last_error(void);
+int mwd_version(void);
+
+/* Coefficients of the normal/tangential law for (Kn, en, et, mu) and dt. */
+int mwd_law_init(mwd_law* out, double Kn, double en, double et, double mu, double dt);
+
+/* Workspace owning every scratch buffer of the step: sort buffers, cell
+ * tables, the candidate pair list (pair_cap pairs), contacts (contact_cap).
+ * Scenes are copied in. Calls on distinct workspaces are independent. */
+int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scene* scenes, int32_t n_scenes,
+                         int64_t n_rows, int32_t vmax, int64_t pair_cap, int64_t contact_cap, double dt,
+                         void* stream);
+int mwd_workspace_destroy(mwd_workspace* ws);
+
+/* Verlet over rows [pmin, pmax) of the reference's arrays
+ * (gpu_Verlet.cu:85-431). nrot: per-row count of vertex slots to rotate, or
+ * NULL to rotate all vmax slots as the reference does. box as in mwd_scene. */
+int mwd_verlet(double* v, double* v12, double* w12bf, double* q12, double* q, double* qall, double* wbf,
+               double* w, const double* F, const double* tau, double* verts, double* com, const double* mass,
+               const double* Iinv, const double* I, double* dxall, int32_t vmax, const int32_t* nrot, double dt,
+               const double* box, int64_t pmin, int64_t pmax, void* stream);
+
+/* Fine contact over P pairs in the reference's per-pair layout
+ * (gpu_fine_contact_detection.cu:874-895, FP64):
+ *   comA/comB/vA/vB/wA/wB [P,3], vertsA/vertsB [P,vmax,3], nA/nB/idA/idB [P],
+ *   dispt [P,4], qA [P,4], isA/isB [P].
+ * Outputs (each may be NULL): collision [P] (-1 undecided, 0 apart, 1
+ * contact), epa_status [P], depth [P], normal/cpA/cpB [P,3], disp [P,4].
+ * force/torque [n_particles,3] receive, per particle, the contact sum taken in
+ * ascending pair order (Fn on A, -Fn on B, tauA on A, tauB on B), added once
+ * to their current contents. Synchronises `stream` before returning. */
+int mwd_contact_pairs(int64_t P, int32_t vmax, const double* comA, const double* comB, const double* vertsA,
+                      const double* vertsB, const double* vA, const double* vB, const double* wA, const double* wB,
+                      const int32_t* nA, const int32_t* nB, const int32_t* idA, const int32_t* idB,
+                      const double* dispt, const double* qA, const double* isA, const double* isB,
+                      const mwd_law* law_p, const mwd_law* law_w, int64_t n_walls, int32_t* collision,
+                      int32_t* epa_status, double* depth, double* normal, double* cpA, double* cpB, double* disp,
+                      double* force, double* torque, int64_t n_particles, void* stream);
+
+/* Cell keys (MWDiem.py:813-816) for rows [0,n): trunc(com/cell) combined as
+ * xc + yc*ncx + zc*ncx*ncy, int64. */
+int mwd_cell_keys(const double* com, int64_t n, const double* cell, int64_t ncx, int64_t ncy, int64_t* keys,
+                  void* stream);
+
+/* Candidate pairs of every scene: wall slab pairs [j,k] (walls k outer,
+ * particles j inner) then Linked_cell_hash pairs (cells in order of their
+ * lowest id, members in CPython set order, the 7 forward offsets). If keys is
+ * NULL they are computed from state->com. Pairs (global rows) go to the
+ * workspace; mwd_pairs_view exposes them. */
+int mwd_candidates(mwd_workspace* ws, const mwd_state* state, const int64_t* keys);
+
+/* Device pointers of the workspace's candidate list [pair_cap,2] int32 and its
+ * count (int64, device). */
+int mwd_pairs_view(mwd_workspace* ws, int32_t** pairs, int64_t** n_pairs);
+
+/* Fine contact over the workspace's candidate list, indexed into the state.
+ * Per-pair outputs stay in the workspace; mwd_contacts_view exposes them. */
+int mwd_fine_contact(mwd_workspace* ws, const mwd_state* state);
+
+/* collision [pair_cap] int32; per contact c < n_contacts: pair index, depth,
+ * normal/cpA/cpB [.,3], Fn/tauA/tauB [.,3], epa status. */
+int mwd_contacts_view(mwd_workspace* ws, int32_t** collision, int32_t** contact_pair, double** depth,
+                      double** normal, double** cpA, double** cpB, double** Fn, double** tauA, double** tauB,
+                      int32_t** epa_status, int64_t** n_contacts);
+
+/* Force assembly and Verlet (MWDiem.py:977-1001): per moving particle the
+ * contact sum (ascending pair order) plus fext/text, then the integrator. */
+int mwd_reduce_verlet(mwd_workspace* ws, mwd_state* state);
+
+/* One full step: mwd_candidates + mwd_fine_contact + mwd_reduce_verlet.
+ * Stream-ordered, no host synchronisation, capturable in a hipGraph. */
+int mwd_step(mwd_workspace* ws, mwd_state* state);
+
+/* Per-stage timing. When enabled, HIP events are recorded on the workspace
+ * stream between the stages of mwd_step; mwd_stage_times synchronises and
+ * returns the accumulated milliseconds of the MWD_N_STAGES stages (walls,
+ * keys and sort; cell tables and emission; GJK; contact compaction; EPA and
+ * contact law; record sort; force assembly and Verlet) and the number of
+ * timed steps, then resets the accumulators. */
+#define MWD_N_STAGES 7
+int mwd_set_timing(mwd_workspace* ws, int enable);
+int mwd_stage_times(mwd_workspace* ws, double* ms, int64_t* steps);
+
+/* Counts and flags of the last step (synchronises the workspace stream). */
+int mwd_step_status(mwd_workspace* ws, mwd_status* out);
+int mwd_clear_flags(mwd_workspace* ws);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

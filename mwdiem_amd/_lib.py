@@ -1,0 +1,149 @@
+"""ctypes binding of libmwdiem_hip.so (the C ABI in include/mwdiem_hip.h).
+
+The library is loaded from this package directory. There is no fallback: if
+the library is missing or a call fails, a MwdError is raised.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libmwdiem_hip.so"
+
+MWD_OK = 0
+MWD_E_INVALID = -1
+MWD_E_HIP = -2
+MWD_E_CAPACITY = -3
+MWD_E_KEYRANGE = -4
+MWD_E_CELL = -5
+
+F_PAIR_OVERFLOW = 1
+F_CONTACT_OVERFLOW = 2
+F_KEYRANGE = 4
+F_CELL = 8
+F_EPA_CAP = 16
+F_EPA_FACES = 32
+FATAL_FLAGS = F_PAIR_OVERFLOW | F_CONTACT_OVERFLOW | F_KEYRANGE | F_CELL
+
+EPA_MAX_FACES = 128
+MAX_CELL = 76
+
+
+class MwdError(RuntimeError):
+    def __init__(self, code: int, where: str, msg: str = ""):
+        super().__init__(f"{where} failed with code {code}: {msg}")
+        self.code = code
+
+
+class Law(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("Kn", "mu", "ln_en", "ln_et", "A_en", "tc", "Gn", "Kt_num", "inv_mu", "dt")]
+
+
+class Scene(C.Structure):
+    _fields_ = [
+        ("row0", C.c_int64), ("n_walls", C.c_int64), ("n_active", C.c_int64),
+        ("box", C.c_double * 6), ("cell", C.c_double * 3),
+        ("ncx", C.c_int64), ("ncy", C.c_int64), ("kmin", C.c_int64), ("kspan", C.c_int64),
+        ("wall_thr", C.c_double), ("law_p", Law), ("law_w", Law),
+    ]
+
+
+_P = C.c_void_p
+
+
+class State(C.Structure):
+    _fields_ = [
+        ("n_rows", C.c_int64), ("vmax", C.c_int32),
+        ("com", _P), ("v", _P), ("v12", _P), ("w", _P), ("wbf", _P), ("w12bf", _P),
+        ("q", _P), ("q12", _P), ("qall", _P), ("dxall", _P), ("verts", _P), ("nverts", _P),
+        ("mass", _P), ("I", _P), ("Iinv", _P), ("isize", _P), ("fext", _P), ("text", _P),
+        ("wall_n", _P), ("wall_c", _P),
+    ]
+
+
+class Status(C.Structure):
+    _fields_ = [("n_pairs", C.c_int64), ("n_wall_pairs", C.c_int64), ("n_contacts", C.c_int64),
+                ("n_cells", C.c_int64), ("flags", C.c_uint32)]
+
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "mwd_last_error": (C.c_char_p, []),
+    "mwd_version": (C.c_int, []),
+    "mwd_law_init": (C.c_int, [C.POINTER(Law), C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]),
+    "mwd_workspace_create": (C.c_int, [C.POINTER(_P), C.c_int, C.POINTER(Scene), C.c_int32, C.c_int64, C.c_int32,
+                                       C.c_int64, C.c_int64, C.c_double, _P]),
+    "mwd_workspace_destroy": (C.c_int, [_P]),
+    "mwd_verlet": (C.c_int, [_P] * 16 + [C.c_int32, _P, C.c_double, _P, C.c_int64, C.c_int64, _P]),
+    "mwd_contact_pairs": (C.c_int, [C.c_int64, C.c_int32] + [_P] * 16 + [_P, _P, C.c_int64] + [_P] * 9
+                          + [C.c_int64, _P]),
+    "mwd_cell_keys": (C.c_int, [_P, C.c_int64, _P, C.c_int64, C.c_int64, _P, _P]),
+    "mwd_candidates": (C.c_int, [_P, C.POINTER(State), _P]),
+    "mwd_pairs_view": (C.c_int, [_P, C.POINTER(_P), C.POINTER(_P)]),
+    "mwd_fine_contact": (C.c_int, [_P, C.POINTER(State)]),
+    "mwd_contacts_view": (C.c_int, [_P] + [C.POINTER(_P)] * 11),
+    "mwd_reduce_verlet": (C.c_int, [_P, C.POINTER(State)]),
+    "mwd_step": (C.c_int, [_P, C.POINTER(State)]),
+    "mwd_step_status": (C.c_int, [_P, C.POINTER(Status)]),
+    "mwd_clear_flags": (C.c_int, [_P]),
+    "mwd_set_timing": (C.c_int, [_P, C.c_int]),
+    "mwd_stage_times": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+}
+
+N_STAGES = 7
+STAGE_NAMES = ("walls+keys+sort", "cells+emit", "gjk", "compact", "epa+law", "record-sort", "reduce+verlet")
+
+EXPORTED = tuple(_SIGS)
+
+_lib = None
+
+
+def load():
+    """Load the library (once). Raises MwdError/OSError when it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    # torch first: its HIP runtime (soname libamdhip64.so.7) then satisfies
+    # the library's dependency, so one runtime serves tensors, streams and kernels
+    import torch  # noqa: F401
+    if not LIB_PATH.exists():
+        raise OSError(f"{LIB_PATH} is not built; run __graft_entry__.build() (or make -C mwdiem_amd/csrc)")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int, where: str) -> None:
+    if rc != MWD_OK:
+        msg = load().mwd_last_error()
+        raise MwdError(rc, where, msg.decode() if msg else "")
+
+
+def law(Kn: float, en: float, et: float, mu: float, dt: float) -> Law:
+    L = Law()
+    check(load().mwd_law_init(C.byref(L), Kn, en, et, mu, dt), "mwd_law_init")
+    return L
+
+
+def ptr(t) -> int | None:
+    """Device address of a torch tensor (None for None)."""
+    if t is None:
+        return None
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return t.data_ptr()
+
+
+def stream_handle(torch_stream=None) -> int | None:
+    import torch
+    s = torch_stream if torch_stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream or None
+
+
+def loaded_path() -> str:
+    return os.fspath(LIB_PATH)

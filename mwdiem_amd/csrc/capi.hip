@@ -1,0 +1,618 @@
+// C ABI of libmwdiem_hip.so (declared in include/mwdiem_hip.h).
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "mwd_internal.h"
+
+#define MWD_API extern "C" __attribute__((visibility("default")))
+
+static void destroy_events(mwd_workspace* ws);
+
+namespace {
+thread_local std::string g_err;
+
+int bits_for(int64_t span) {
+    // number of key bits needed for values in [0, span)
+    int b = 0;
+    uint64_t v = span > 1 ? (uint64_t)(span - 1) : 0;
+    while (v) {
+        ++b;
+        v >>= 1;
+    }
+    return b;
+}
+int passes_for(int64_t span) {
+    const int b = bits_for(span);
+    return b == 0 ? 1 : (b + 7) / 8;
+}
+
+template <typename T>
+int dalloc(T** p, int64_t count) {
+    *p = nullptr;
+    if (count <= 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, sizeof(T) * (size_t)count);
+    if (e != hipSuccess) {
+        mwd_set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+        return MWD_E_HIP;
+    }
+    return MWD_OK;
+}
+
+void free_ws(mwd_workspace* ws) {
+    if (!ws) return;
+    void* ptrs[] = {ws->scenes_dev, ws->plist, ws->scene_of, ws->is_wall, ws->keys, ws->keys_alt, ws->vals,
+                    ws->vals_alt, ws->head, ws->head_off, ws->cell_start, ws->cell_key, ws->cell_nb, ws->row_first,
+                    ws->row_rank, ws->cell_by_rank, ws->cell_cnt, ws->cell_off, ws->perm, ws->wall_flag, ws->wall_off,
+                    ws->pairs, ws->collision, ws->simplex, ws->pair_flag, ws->contact_off, ws->contact_pair,
+                    ws->depth, ws->normal, ws->cpA, ws->cpB, ws->Fn, ws->tA, ws->tB, ws->epa_status, ws->rec_key,
+                    ws->rec_key_alt, ws->rec_val, ws->rec_val_alt, ws->seg_begin, ws->seg_end, ws->hist,
+                    ws->hist_off, ws->scan_tmp, ws->counters, ws->flags, ws->laws, ws->scene_wt_off};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    destroy_events(ws);
+    delete[] ws->scenes_host;
+    delete ws;
+}
+
+BodyView state_view(const mwd_workspace* ws, const mwd_state* st, int side) {
+    BodyView b;
+    b.com = st->com;
+    b.verts = st->verts;
+    b.nverts = st->nverts;
+    b.vel = st->v;
+    b.angvel = st->w;
+    b.quat = st->q;
+    b.isize = st->isize;
+    b.is_wall = ws->is_wall;
+    b.scene = ws->scene_of;
+    b.rows = ws->pairs + side;
+    b.row_stride = 2;
+    b.vmax = st->vmax;
+    return b;
+}
+
+__global__ void k_wall_from_ids(const int32_t* idA, const int32_t* idB, int64_t W, int64_t P, uint8_t* wA,
+                                uint8_t* wB) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    wA[p] = idA[p] < W ? 1 : 0;
+    wB[p] = idB[p] < W ? 1 : 0;
+}
+
+}  // namespace
+
+void mwd_set_error(const std::string& msg) { g_err = msg; }
+
+static inline hipEvent_t& ev_at(mwd_workspace* ws, int slot, int k) {
+    return ws->ev[slot * (MWD_N_STAGES + 1) + k];
+}
+
+static void flush_timing(mwd_workspace* ws) {
+    if (!ws->timing || ws->ring_slot == 0) return;
+    (void)hipEventSynchronize(ev_at(ws, ws->ring_slot - 1, MWD_N_STAGES));
+    for (int sl = 0; sl < ws->ring_slot; ++sl) {
+        for (int k = 0; k < MWD_N_STAGES; ++k) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, ev_at(ws, sl, k), ev_at(ws, sl, k + 1)) == hipSuccess)
+                ws->stage_ms[k] += ms;
+        }
+        ws->stage_calls += 1;
+    }
+    ws->ring_slot = 0;
+}
+
+void mwd_mark(mwd_workspace* ws, int stage) {
+    if (ws->timing) (void)hipEventRecord(ev_at(ws, ws->ring_slot, stage), ws->stream);
+}
+void mwd_mark_end(mwd_workspace* ws) {
+    if (!ws->timing) return;
+    (void)hipEventRecord(ev_at(ws, ws->ring_slot, MWD_N_STAGES), ws->stream);
+    ws->ring_slot += 1;
+    if (ws->ring_slot == MWD_TIMING_RING) flush_timing(ws);
+}
+
+static void destroy_events(mwd_workspace* ws) {
+    if (!ws->ev) return;
+    for (int k = 0; k < MWD_TIMING_RING * (MWD_N_STAGES + 1); ++k) (void)hipEventDestroy(ws->ev[k]);
+    delete[] ws->ev;
+    ws->ev = nullptr;
+}
+
+MWD_API int mwd_set_timing(mwd_workspace* ws, int enable) {
+    if (!ws) return MWD_E_INVALID;
+    if (enable && !ws->ev) {
+        ws->ev = new hipEvent_t[MWD_TIMING_RING * (MWD_N_STAGES + 1)];
+        for (int k = 0; k < MWD_TIMING_RING * (MWD_N_STAGES + 1); ++k) MWD_HIP_CHECK(hipEventCreate(&ws->ev[k]));
+    }
+    if (!enable) {
+        MWD_HIP_CHECK(hipStreamSynchronize(ws->stream));
+        destroy_events(ws);
+    }
+    ws->timing = enable ? 1 : 0;
+    ws->ring_slot = 0;
+    for (int k = 0; k < MWD_N_STAGES; ++k) ws->stage_ms[k] = 0.0;
+    ws->stage_calls = 0;
+    return MWD_OK;
+}
+
+MWD_API int mwd_stage_times(mwd_workspace* ws, double* ms, int64_t* steps) {
+    if (!ws || !ms) return MWD_E_INVALID;
+    MWD_HIP_CHECK(hipStreamSynchronize(ws->stream));
+    flush_timing(ws);
+    for (int k = 0; k < MWD_N_STAGES; ++k) {
+        ms[k] = ws->stage_ms[k];
+        ws->stage_ms[k] = 0.0;
+    }
+    if (steps) *steps = ws->stage_calls;
+    ws->stage_calls = 0;
+    return MWD_OK;
+}
+
+MWD_API const char* mwd_last_error(void) { return g_err.c_str(); }
+MWD_API int mwd_version(void) { return 1; }
+
+MWD_API int mwd_law_init(mwd_law* out, double Kn, double en, double et, double mu, double dt) {
+    if (!out || !(Kn > 0) || !(en > 0) || !(et > 0) || !(mu > 0)) {
+        mwd_set_error("mwd_law_init: Kn, en, et and mu must be positive");
+        return MWD_E_INVALID;
+    }
+    // Force, gpu_fine_contact_detection.cu:685-701, in the reference's operator order.
+    const double pi = 3.141592653589793;
+    const double pi2 = pi * pi;
+    out->Kn = Kn;
+    out->mu = mu;
+    out->ln_en = log(en);
+    out->ln_et = log(et);
+    out->A_en = pi2 + out->ln_en * out->ln_en;
+    out->tc = sqrt((mu * out->A_en) / Kn);
+    out->Gn = ((-2.0 * mu) / out->tc) * out->ln_en;
+    out->Kt_num = ((Kn * (pi2 + out->ln_et * out->ln_et)) / out->A_en) * 1.0;
+    out->inv_mu = 1.0 / mu;
+    out->dt = dt;
+    return MWD_OK;
+}
+
+MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scene* scenes, int32_t n_scenes,
+                                 int64_t n_rows, int32_t vmax, int64_t pair_cap, int64_t contact_cap, double dt,
+                                 void* stream) {
+    if (!out || !scenes || n_scenes <= 0 || n_rows <= 0 || vmax <= 0 || pair_cap <= 0 || contact_cap <= 0) {
+        mwd_set_error("mwd_workspace_create: invalid argument");
+        return MWD_E_INVALID;
+    }
+    *out = nullptr;
+    MWD_HIP_CHECK(hipSetDevice(device));
+    mwd_workspace* ws = new mwd_workspace();
+    memset((void*)ws, 0, sizeof(*ws));
+    ws->device = device;
+    ws->stream = (hipStream_t)stream;
+    ws->n_scenes = n_scenes;
+    ws->n_rows = n_rows;
+    ws->vmax = vmax;
+    ws->pair_cap = pair_cap;
+    ws->contact_cap = contact_cap;
+    ws->dt = dt;
+    ws->scenes_host = new mwd_scene[n_scenes];
+    memcpy(ws->scenes_host, scenes, sizeof(mwd_scene) * n_scenes);
+
+    std::vector<SceneDev> sd(n_scenes);
+    std::vector<int32_t> scene_of(n_rows, 0);
+    std::vector<uint8_t> is_wall(n_rows, 0);
+    std::vector<int32_t> plist;
+    std::vector<int64_t> wt(n_scenes + 1, 0);
+    std::vector<mwd_law> laws(2 * n_scenes);
+    int64_t kbase = 0, prev_end = 0;
+    for (int s = 0; s < n_scenes; ++s) {
+        const mwd_scene& sc = scenes[s];
+        if (sc.row0 < prev_end || sc.n_walls < 0 || sc.n_active < sc.n_walls || sc.row0 + sc.n_active > n_rows ||
+            sc.kspan <= 0 || sc.ncx <= 0 || sc.ncy <= 0) {
+            mwd_set_error("mwd_workspace_create: scene " + std::to_string(s) + " is inconsistent");
+            free_ws(ws);
+            return MWD_E_INVALID;
+        }
+        const int64_t row_end = (s + 1 < n_scenes) ? scenes[s + 1].row0 : n_rows;
+        for (int64_t r = sc.row0; r < row_end; ++r) scene_of[r] = s;
+        for (int64_t r = sc.row0; r < sc.row0 + sc.n_walls; ++r) is_wall[r] = 1;
+        SceneDev& d = sd[s];
+        d.row0 = sc.row0;
+        d.n_walls = sc.n_walls;
+        d.n_active = sc.n_active;
+        memcpy(d.box, sc.box, sizeof(d.box));
+        memcpy(d.cell, sc.cell, sizeof(d.cell));
+        d.ncx = sc.ncx;
+        d.ncy = sc.ncy;
+        d.kmin = sc.kmin;
+        d.kspan = sc.kspan;
+        d.kbase = kbase;
+        d.wt_off = wt[s];
+        d.plist_off = (int64_t)plist.size();
+        d.wall_thr = sc.wall_thr;
+        d.law_p = sc.law_p;
+        d.law_w = sc.law_w;
+        laws[2 * s] = sc.law_p;
+        laws[2 * s + 1] = sc.law_w;
+        kbase += sc.kspan;
+        wt[s + 1] = wt[s] + sc.n_walls * (sc.n_active - sc.n_walls);
+        for (int64_t r = sc.row0 + sc.n_walls; r < sc.row0 + sc.n_active; ++r) plist.push_back((int32_t)r);
+        prev_end = sc.row0 + sc.n_active;
+    }
+    if (kbase > (int64_t)0xFFFFFFFFll) {
+        mwd_set_error("mwd_workspace_create: total key span exceeds 32 bits");
+        free_ws(ws);
+        return MWD_E_KEYRANGE;
+    }
+    const int64_t M = (int64_t)plist.size();
+    ws->n_moving = M;
+    ws->n_wall_tests = wt[n_scenes];
+    ws->key_span_total = kbase;
+    ws->key_passes = passes_for(kbase);
+    ws->row_passes = passes_for(n_rows);
+    const int64_t rec_cap = 2 * contact_cap;
+    const int64_t max_sort = std::max(M, rec_cap);
+    const int64_t tiles = mwd_sort_tiles(max_sort);
+    const int64_t max_scan = std::max({M, n_rows, pair_cap, ws->n_wall_tests, 256 * tiles});
+    ws->scan_tmp_len = mwd_scan_tmp_len(max_scan);
+
+    int rc = 0;
+#define A(ptr, n)                                  \
+    if ((rc = dalloc(&ws->ptr, (n))) != MWD_OK) {  \
+        free_ws(ws);                               \
+        return rc;                                 \
+    }
+    A(scenes_dev, n_scenes);
+    A(plist, M);
+    A(scene_of, n_rows);
+    A(is_wall, n_rows);
+    A(keys, M);
+    A(keys_alt, M);
+    A(vals, M);
+    A(vals_alt, M);
+    A(head, M);
+    A(head_off, M);
+    A(cell_start, M + 1);
+    A(cell_key, M);
+    A(cell_nb, 7 * M);
+    A(row_first, n_rows);
+    A(row_rank, n_rows);
+    A(cell_by_rank, M);
+    A(cell_cnt, M);
+    A(cell_off, M);
+    A(perm, M);
+    A(wall_flag, ws->n_wall_tests);
+    A(wall_off, ws->n_wall_tests);
+    A(pairs, 2 * pair_cap);
+    A(collision, pair_cap);
+    A(simplex, pair_cap);
+    A(pair_flag, pair_cap);
+    A(contact_off, pair_cap);
+    A(contact_pair, contact_cap);
+    A(depth, contact_cap);
+    A(normal, 3 * contact_cap);
+    A(cpA, 3 * contact_cap);
+    A(cpB, 3 * contact_cap);
+    A(Fn, 3 * contact_cap);
+    A(tA, 3 * contact_cap);
+    A(tB, 3 * contact_cap);
+    A(epa_status, contact_cap);
+    A(rec_key, rec_cap);
+    A(rec_key_alt, rec_cap);
+    A(rec_val, rec_cap);
+    A(rec_val_alt, rec_cap);
+    A(seg_begin, n_rows);
+    A(seg_end, n_rows);
+    A(hist, 256 * tiles);
+    A(hist_off, 256 * tiles);
+    A(scan_tmp, ws->scan_tmp_len);
+    A(counters, 8);
+    A(flags, 1);
+    A(laws, 2 * n_scenes);
+    A(scene_wt_off, n_scenes + 1);
+#undef A
+    hipStream_t s = ws->stream;
+#define H2D(dst, src, n)                                                                                  \
+    if (hipMemcpyAsync(ws->dst, (src), sizeof(*ws->dst) * (size_t)(n), hipMemcpyHostToDevice, s) !=     \
+        hipSuccess) {                                                                                     \
+        mwd_set_error("mwd_workspace_create: upload failed");                                            \
+        free_ws(ws);                                                                                      \
+        return MWD_E_HIP;                                                                                 \
+    }
+    H2D(scenes_dev, sd.data(), n_scenes);
+    if (M) H2D(plist, plist.data(), M);
+    H2D(scene_of, scene_of.data(), n_rows);
+    H2D(is_wall, is_wall.data(), n_rows);
+    H2D(laws, laws.data(), 2 * n_scenes);
+    H2D(scene_wt_off, wt.data(), n_scenes + 1);
+#undef H2D
+    if (hipMemsetAsync(ws->counters, 0, sizeof(int64_t) * 8, s) != hipSuccess ||
+        hipMemsetAsync(ws->flags, 0, sizeof(uint32_t), s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        mwd_set_error("mwd_workspace_create: initialisation failed");
+        free_ws(ws);
+        return MWD_E_HIP;
+    }
+    ws->keys_sorted = ws->keys;
+    ws->vals_sorted = ws->vals;
+    ws->rec_key_sorted = ws->rec_key;
+    ws->rec_val_sorted = ws->rec_val;
+    *out = ws;
+    return MWD_OK;
+}
+
+MWD_API int mwd_workspace_destroy(mwd_workspace* ws) {
+    if (!ws) return MWD_OK;
+    (void)hipStreamSynchronize(ws->stream);
+    free_ws(ws);
+    return MWD_OK;
+}
+
+MWD_API int mwd_verlet(double* v, double* v12, double* w12bf, double* q12, double* q, double* qall, double* wbf,
+                       double* w, const double* F, const double* tau, double* verts, double* com, const double* mass,
+                       const double* Iinv, const double* I, double* dxall, int32_t vmax, const int32_t* nrot,
+                       double dt, const double* box, int64_t pmin, int64_t pmax, void* stream) {
+    if (!v || !v12 || !w12bf || !q12 || !q || !qall || !wbf || !w || !F || !tau || !verts || !com || !mass ||
+        !Iinv || !I || !dxall || !box || vmax <= 0 || pmin < 0 || pmax < pmin) {
+        mwd_set_error("mwd_verlet: invalid argument");
+        return MWD_E_INVALID;
+    }
+    return mwd_launch_verlet(v, v12, w12bf, q12, q, qall, wbf, w, F, tau, verts, com, mass, Iinv, I, dxall, vmax,
+                             nrot, dt, box, pmin, pmax, (hipStream_t)stream);
+}
+
+MWD_API int mwd_cell_keys(const double* com, int64_t n, const double* cell, int64_t ncx, int64_t ncy, int64_t* keys,
+                          void* stream) {
+    if (!com || !cell || !keys || n < 0) {
+        mwd_set_error("mwd_cell_keys: invalid argument");
+        return MWD_E_INVALID;
+    }
+    return mwd_launch_cell_keys(com, n, cell, ncx, ncy, keys, (hipStream_t)stream);
+}
+
+MWD_API int mwd_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t* keys) {
+    if (!ws || !st || (!st->com && !keys)) {
+        mwd_set_error("mwd_candidates: invalid argument");
+        return MWD_E_INVALID;
+    }
+    return mwd_launch_candidates(ws, st, keys);
+}
+
+MWD_API int mwd_pairs_view(mwd_workspace* ws, int32_t** pairs, int64_t** n_pairs) {
+    if (!ws) return MWD_E_INVALID;
+    if (pairs) *pairs = ws->pairs;
+    if (n_pairs) *n_pairs = ws->counters + 3;
+    return MWD_OK;
+}
+
+MWD_API int mwd_fine_contact(mwd_workspace* ws, const mwd_state* st) {
+    if (!ws || !st || st->vmax != ws->vmax) {
+        mwd_set_error("mwd_fine_contact: invalid argument");
+        return MWD_E_INVALID;
+    }
+    hipStream_t s = ws->stream;
+    const BodyView A = state_view(ws, st, 0), B = state_view(ws, st, 1);
+    int64_t* ctr = ws->counters;
+    mwd_mark(ws, 2);
+    int rc = mwd_launch_gjk(A, B, ws->pair_cap, ctr + 3, ws->collision, ws->simplex, s);
+    if (rc) return rc;
+    mwd_mark(ws, 3);
+    rc = mwd_launch_compact_contacts(ws->collision, ws->pair_cap, ctr + 3, ws->pair_flag, ws->contact_off,
+                                     ws->contact_pair, ws->contact_cap, ctr + 4, ws->flags, ws->scan_tmp,
+                                     ws->scan_tmp_len, s);
+    if (rc) return rc;
+    mwd_mark(ws, 4);
+    rc = mwd_launch_epa_force(A, B, ws->laws, ws->contact_pair, ws->simplex, ws->contact_cap, ctr + 4, nullptr,
+                              nullptr, nullptr, ws->depth, ws->normal, ws->cpA, ws->cpB, ws->Fn, ws->tA, ws->tB,
+                              nullptr, ws->epa_status, ws->rec_key, ws->rec_val, ctr + 5, ws->flags, s);
+    if (rc) return rc;
+    mwd_mark(ws, 5);
+    uint32_t* ks;
+    int32_t* vs;
+    rc = mwd_radix_sort(ws->rec_key, ws->rec_val, ws->rec_key_alt, ws->rec_val_alt, 2 * ws->contact_cap, ctr + 5,
+                        ws->row_passes, ws->hist, ws->hist_off, ws->scan_tmp, ws->scan_tmp_len, s, &ks, &vs);
+    if (rc) return rc;
+    ws->rec_key_sorted = ks;
+    ws->rec_val_sorted = vs;
+    rc = mwd_fill_i32(ws->seg_begin, 0, ws->n_rows, s);
+    if (rc) return rc;
+    rc = mwd_fill_i32(ws->seg_end, 0, ws->n_rows, s);
+    if (rc) return rc;
+    return mwd_launch_segments(ks, 2 * ws->contact_cap, ctr + 5, ws->seg_begin, ws->seg_end, s);
+}
+
+MWD_API int mwd_contacts_view(mwd_workspace* ws, int32_t** collision, int32_t** contact_pair, double** depth,
+                              double** normal, double** cpA, double** cpB, double** Fn, double** tauA, double** tauB,
+                              int32_t** epa_status, int64_t** n_contacts) {
+    if (!ws) return MWD_E_INVALID;
+    if (collision) *collision = ws->collision;
+    if (contact_pair) *contact_pair = ws->contact_pair;
+    if (depth) *depth = ws->depth;
+    if (normal) *normal = ws->normal;
+    if (cpA) *cpA = ws->cpA;
+    if (cpB) *cpB = ws->cpB;
+    if (Fn) *Fn = ws->Fn;
+    if (tauA) *tauA = ws->tA;
+    if (tauB) *tauB = ws->tB;
+    if (epa_status) *epa_status = ws->epa_status;
+    if (n_contacts) *n_contacts = ws->counters + 4;
+    return MWD_OK;
+}
+
+MWD_API int mwd_reduce_verlet(mwd_workspace* ws, mwd_state* st) {
+    if (!ws || !st) {
+        mwd_set_error("mwd_reduce_verlet: invalid argument");
+        return MWD_E_INVALID;
+    }
+    // the fused kernel reads the sorted record values
+    int32_t* saved = ws->rec_val;
+    ws->rec_val = ws->rec_val_sorted;
+    mwd_mark(ws, 6);
+    int rc = mwd_launch_reduce_verlet(ws, st);
+    ws->rec_val = saved;
+    return rc;
+}
+
+MWD_API int mwd_step(mwd_workspace* ws, mwd_state* st) {
+    int rc = mwd_candidates(ws, st, nullptr);
+    if (rc) return rc;
+    rc = mwd_fine_contact(ws, st);
+    if (rc) return rc;
+    rc = mwd_reduce_verlet(ws, st);
+    if (rc) return rc;
+    mwd_mark_end(ws);
+    return MWD_OK;
+}
+
+MWD_API int mwd_step_status(mwd_workspace* ws, mwd_status* out) {
+    if (!ws || !out) return MWD_E_INVALID;
+    int64_t c[8];
+    uint32_t f = 0;
+    MWD_HIP_CHECK(hipMemcpyAsync(c, ws->counters, sizeof(c), hipMemcpyDeviceToHost, ws->stream));
+    MWD_HIP_CHECK(hipMemcpyAsync(&f, ws->flags, sizeof(f), hipMemcpyDeviceToHost, ws->stream));
+    MWD_HIP_CHECK(hipStreamSynchronize(ws->stream));
+    out->n_cells = c[0];
+    out->n_wall_pairs = c[1];
+    out->n_pairs = c[7];
+    out->n_contacts = c[4];
+    out->flags = f;
+    return MWD_OK;
+}
+
+MWD_API int mwd_clear_flags(mwd_workspace* ws) {
+    if (!ws) return MWD_E_INVALID;
+    MWD_HIP_CHECK(hipMemsetAsync(ws->flags, 0, sizeof(uint32_t), ws->stream));
+    return MWD_OK;
+}
+
+MWD_API int mwd_contact_pairs(int64_t P, int32_t vmax, const double* comA, const double* comB, const double* vertsA,
+                              const double* vertsB, const double* vA, const double* vB, const double* wA,
+                              const double* wB, const int32_t* nA, const int32_t* nB, const int32_t* idA,
+                              const int32_t* idB, const double* dispt, const double* qA, const double* isA,
+                              const double* isB, const mwd_law* law_p, const mwd_law* law_w, int64_t n_walls,
+                              int32_t* collision, int32_t* epa_status, double* depth, double* normal, double* cpA,
+                              double* cpB, double* disp, double* force, double* torque, int64_t n_particles,
+                              void* stream) {
+    if (P < 0 || vmax <= 0 || !law_p || !law_w || n_particles < 0) {
+        mwd_set_error("mwd_contact_pairs: invalid argument");
+        return MWD_E_INVALID;
+    }
+    if (P == 0) return MWD_OK;
+    if (!comA || !comB || !vertsA || !vertsB || !vA || !vB || !wA || !wB || !nA || !nB || !idA || !idB || !qA ||
+        !isA || !isB || ((force || torque) && (!force || !torque || n_particles <= 0))) {
+        mwd_set_error("mwd_contact_pairs: null array");
+        return MWD_E_INVALID;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    // temporaries of this call
+    int32_t *coll = nullptr, *flag = nullptr, *cpair = nullptr, *est = nullptr, *rval = nullptr, *rval2 = nullptr;
+    int32_t *sb = nullptr, *se = nullptr, *hist = nullptr;
+    uint8_t *wallA = nullptr, *wallB = nullptr;
+    Simplex* sx = nullptr;
+    int64_t *off = nullptr, *ctr = nullptr, *hist_off = nullptr, *tmp = nullptr;
+    double *dep = nullptr, *nrm = nullptr, *ca = nullptr, *cb = nullptr, *fn = nullptr, *ta = nullptr, *tb = nullptr;
+    uint32_t *rkey = nullptr, *rkey2 = nullptr, *flags = nullptr;
+    mwd_law* laws = nullptr;
+    const int64_t rec = 2 * P;
+    const int64_t tiles = mwd_sort_tiles(rec);
+    const int64_t tmp_len = mwd_scan_tmp_len(std::max({P, rec, 256 * tiles, n_particles}));
+    int rc = MWD_OK;
+    std::vector<void*> owned;
+    auto al = [&](auto** p, int64_t n) {
+        if (rc) return;
+        rc = dalloc(p, n);
+        if (!rc) owned.push_back((void*)*p);
+    };
+    al(&coll, P);
+    al(&flag, P);
+    al(&cpair, P);
+    al(&est, P);
+    al(&rval, rec);
+    al(&rval2, rec);
+    al(&rkey, rec);
+    al(&rkey2, rec);
+    al(&wallA, P);
+    al(&wallB, P);
+    al(&sx, P);
+    al(&off, P);
+    al(&ctr, 8);
+    al(&hist, 256 * tiles);
+    al(&hist_off, 256 * tiles);
+    al(&tmp, tmp_len);
+    al(&dep, P);
+    al(&nrm, 3 * P);
+    al(&ca, 3 * P);
+    al(&cb, 3 * P);
+    al(&fn, 3 * P);
+    al(&ta, 3 * P);
+    al(&tb, 3 * P);
+    al(&flags, 1);
+    al(&laws, 2);
+    if (force) {
+        al(&sb, n_particles);
+        al(&se, n_particles);
+    }
+    auto cleanup = [&]() {
+        (void)hipStreamSynchronize(s);
+        for (void* p : owned) (void)hipFree(p);
+    };
+    if (rc) {
+        cleanup();
+        return rc;
+    }
+    mwd_law lw[2] = {*law_p, *law_w};
+    do {
+        if (hipMemcpyAsync(laws, lw, sizeof(lw), hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipMemsetAsync(ctr, 0, 8 * sizeof(int64_t), s) != hipSuccess ||
+            hipMemsetAsync(flags, 0, sizeof(uint32_t), s) != hipSuccess) {
+            mwd_set_error("mwd_contact_pairs: upload failed");
+            rc = MWD_E_HIP;
+            break;
+        }
+        hipLaunchKernelGGL(k_wall_from_ids, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, idA, idB, n_walls, P,
+                           wallA, wallB);
+        if (hipGetLastError() != hipSuccess) {
+            mwd_set_error("mwd_contact_pairs: launch failed");
+            rc = MWD_E_HIP;
+            break;
+        }
+        BodyView A{comA, vertsA, nA, vA, wA, qA, isA, wallA, nullptr, nullptr, 1, vmax};
+        BodyView B{comB, vertsB, nB, vB, wB, qA, isB, wallB, nullptr, nullptr, 1, vmax};
+        if ((rc = mwd_set_i64(ctr + 3, P, s))) break;
+        if ((rc = mwd_launch_zero_pairs(P, depth, normal, cpA, cpB, epa_status, disp, s))) break;
+        if ((rc = mwd_launch_gjk(A, B, P, ctr + 3, coll, sx, s))) break;
+        if ((rc = mwd_launch_compact_contacts(coll, P, ctr + 3, flag, off, cpair, P, ctr + 4, flags, tmp, tmp_len,
+                                              s)))
+            break;
+        if ((rc = mwd_launch_epa_force(A, B, laws, cpair, sx, P, ctr + 4, dispt, idA, idB, dep, nrm, ca, cb, fn, ta,
+                                       tb, disp, est, rkey, rval, ctr + 5, flags, s)))
+            break;
+        // per-pair outputs scattered back from contact order
+        if (collision &&
+            hipMemcpyAsync(collision, coll, sizeof(int32_t) * P, hipMemcpyDeviceToDevice, s) != hipSuccess) {
+            mwd_set_error("mwd_contact_pairs: copy failed");
+            rc = MWD_E_HIP;
+            break;
+        }
+        if ((rc = mwd_launch_scatter_contacts(cpair, P, ctr + 4, dep, nrm, ca, cb, est, P, depth, normal, cpA, cpB,
+                                              epa_status, disp, s)))
+            break;
+        if (force) {
+            uint32_t* ks;
+            int32_t* vs;
+            if ((rc = mwd_radix_sort(rkey, rval, rkey2, rval2, rec, ctr + 5, passes_for(n_particles), hist, hist_off,
+                                     tmp, tmp_len, s, &ks, &vs)))
+                break;
+            if ((rc = mwd_fill_i32(sb, 0, n_particles, s))) break;
+            if ((rc = mwd_fill_i32(se, 0, n_particles, s))) break;
+            if ((rc = mwd_launch_segments(ks, rec, ctr + 5, sb, se, s))) break;
+            if ((rc = mwd_launch_add_sums(sb, se, vs, fn, ta, tb, force, torque, n_particles, s))) break;
+        }
+    } while (0);
+    if (hipStreamSynchronize(s) != hipSuccess && !rc) {
+        mwd_set_error("mwd_contact_pairs: kernel failure");
+        rc = MWD_E_HIP;
+    }
+    cleanup();
+    return rc;
+}

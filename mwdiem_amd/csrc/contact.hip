@@ -1,0 +1,698 @@
+// Fine contact: GJK, EPA and the normal/tangential contact law (FP64), one
+// pair per lane.
+//
+// Follows the control flow of the reference's contact_detection kernel
+// (LIBRARIES/gpu_fine_contact_detection.cu:874-1322) and its helpers
+// (:41-871): the same support-map tie-breaking (first strict maximum), the
+// same Voronoi tests and simplex reduction, the same EPA face order,
+// termination constants (1e-5 relative, 25 expansions) and contact-point
+// construction, and the same law (including the reference's quirks: contact
+// point (cpA-cpB)/2, mu in the damping time, normal force only on the linear
+// momentum). Evaluated in FP64 with a fixed operator order.
+//
+// The stage is split in two kernels so that pairs which separate early do not
+// hold a wavefront hostage to EPA: k_gjk decides every candidate pair and
+// stores the final simplex of colliding ones; after compaction k_epa_force
+// runs EPA and the law over contacts only, writing per-contact force/torque
+// and two reduction records (row of A, row of B) per contact.
+#include "mwd_internal.h"
+
+namespace {
+
+constexpr int EPA_FACES = MWD_EPA_MAX_FACES;
+constexpr int EPA_VERTS = 32;
+
+struct Side {
+    const double* v;  // vertices [n][3]
+    int n;
+};
+
+__device__ __forceinline__ int64_t view_row(const BodyView& b, int64_t p) {
+    return b.rows ? (int64_t)b.rows[p * b.row_stride] : p;
+}
+
+// Support map (Support_funtion, .cu:55-101): first index of the strict maximum
+// of d.a over A and of (-d).b over B; returns a-b.
+__device__ __forceinline__ d3 support(const Side& A, const Side& B, d3 d, int* ia, int* ib) {
+    double best = -1e30;
+    int i1 = 0;
+    for (int i = 0; i < A.n; ++i) {
+        const double t = dot3(d, ld3(A.v + 3 * i));
+        if (t > best) {
+            best = t;
+            i1 = i;
+        }
+    }
+    const d3 nd = neg3(d);
+    double best2 = -1e30;
+    int i2 = 0;
+    for (int i = 0; i < B.n; ++i) {
+        const double t = dot3(nd, ld3(B.v + 3 * i));
+        if (t > best2) {
+            best2 = t;
+            i2 = i;
+        }
+    }
+    *ia = i1;
+    *ib = i2;
+    return ld3(A.v + 3 * i1) - ld3(B.v + 3 * i2);
+}
+
+// Check_linesegment, .cu:103-124
+__device__ __forceinline__ void check_segment(d3 P1, d3 P2, int* coll, d3* d) {
+    *coll = -1;
+    *d = mk3(0.0, 0.0, 0.0);
+    const d3 V21 = P2 - P1;
+    if (dot3(V21, P2) < 0.0) {
+        *coll = 0;
+        return;
+    }
+    if (zero3(cross3(V21, P2))) {
+        *coll = 1;
+        return;
+    }
+    const d3 Vt = cross3(V21, neg3(P2));
+    *d = cross3(Vt, V21);
+}
+
+// Check_triangle, .cu:127-186
+__device__ __forceinline__ void check_triangle(d3 P1, d3 P2, d3 P3, int* coll, d3* d) {
+    *coll = -1;
+    *d = mk3(0.0, 0.0, 0.0);
+    int c2, c3;
+    check_segment(P2, P3, &c2, d);
+    if (c2 == 1) {
+        *coll = 1;
+        return;
+    }
+    check_segment(P1, P3, &c3, d);
+    if (c3 == 1) {
+        *coll = 1;
+        return;
+    }
+    if (c2 + c3 > -1) {
+        *coll = 0;
+        return;
+    }
+    const d3 V32 = P3 - P2;
+    const d3 V13 = P1 - P3;
+    const d3 n = cross3(V32, V13);
+    const d3 n2 = cross3(V32, n);
+    const d3 n3 = cross3(V13, n);
+    if (dot3(n2, P2) < 0.0) {
+        *d = n2;
+        return;
+    }
+    if (dot3(n3, P3) < 0.0) {
+        *d = n3;
+        return;
+    }
+    if (dot3(P3, n) == 0.0) {
+        *coll = 1;
+        return;
+    }
+    *d = (dot3(n, P3) > 0.0) ? neg3(n) : n;
+}
+
+// Check_Tetrahedron, .cu:188-254
+__device__ __forceinline__ void check_tetra(d3 P1, d3 P2, d3 P3, d3 P4, int* coll, d3* d, int* todel) {
+    *coll = -1;
+    *d = mk3(0.0, 0.0, 0.0);
+    *todel = 0;
+    int c1, c2, c3;
+    check_triangle(P1, P2, P4, &c1, d);
+    if (c1 == 1) {
+        *coll = 1;
+        return;
+    }
+    check_triangle(P2, P3, P4, &c2, d);
+    if (c2 == 1) {
+        *coll = 1;
+        return;
+    }
+    check_triangle(P3, P1, P4, &c3, d);
+    if (c3 == 1) {
+        *coll = 1;
+        return;
+    }
+    if (c1 + c2 + c3 > -2) {
+        *coll = 0;
+        return;
+    }
+    const d3 V13 = P1 - P3, V21 = P2 - P1, V32 = P3 - P2, V41 = P4 - P1;
+    const d3 n2 = cross3(V21, V41);
+    const d3 V42 = P4 - P2;
+    const d3 n3 = cross3(V32, V42);
+    const d3 V43 = P4 - P3;
+    const d3 n4 = cross3(V13, V43);
+    if (dot3(n2, P4) < 0.0) {
+        *d = n2;
+        *todel = 3;
+    } else if (dot3(n3, P4) < 0.0) {
+        *d = n3;
+        *todel = 1;
+    } else if (dot3(n4, P4) < 0.0) {
+        *d = n4;
+        *todel = 2;
+    } else {
+        *coll = 1;
+    }
+}
+
+// CorrectFlat, .cu:256-268
+__device__ __forceinline__ bool correct_flat(d3 S1, d3 S2, d3 S3, d3 S4) {
+    const double det1 = S1.x * (S2.y * S3.z - S3.y * S2.z) - S2.x * (S1.y * S3.z - S3.y * S1.z) +
+                        S3.x * (S1.y * S2.z - S2.y * S1.z);
+    if (det1 < 0.000001 && det1 > -0.000001) {
+        const double det2 = S1.x * (S2.y * S4.z - S4.y * S2.z) - S2.x * (S1.y * S4.z - S4.y * S1.z) +
+                            S4.x * (S1.y * S2.z - S2.y * S1.z);
+        if (det2 < 0.000001 && det2 > -0.000001) return true;
+    }
+    return false;
+}
+
+// GJK of one pair (.cu:1009-1288). Returns -1 (tetrahedron cap), 0 or 1; on
+// 1 the simplex handed to EPA (after CorrectFlat) is in S/ia/ib.
+__device__ int gjk_pair(const Side& A, const Side& B, d3 comA, d3 comB, d3* S, int* ia, int* ib) {
+    const d3 d1 = comB - comA;
+    S[0] = support(A, B, d1, &ia[0], &ib[0]);
+    if (dot3(d1, S[0]) < 0.0) return 0;
+    const d3 d2 = neg3(d1);
+    S[1] = support(A, B, d2, &ia[1], &ib[1]);
+    if (dot3(d2, S[1]) < 0.0) return 0;
+    int coll;
+    d3 d3v;
+    check_segment(S[0], S[1], &coll, &d3v);
+    if (coll == 0) return 0;
+    S[2] = support(A, B, d3v, &ia[2], &ib[2]);
+    if (dot3(d3v, S[2]) < 0.0) return 0;
+    d3 d4;
+    check_triangle(S[0], S[1], S[2], &coll, &d4);
+    if (coll == 0) return 0;
+    for (int zz = 0; zz < 50; ++zz) {
+        S[3] = support(A, B, d4, &ia[3], &ib[3]);
+        if (dot3(d4, S[3]) < 0.0) return 0;
+        bool hit = zero3(S[3]);
+        if (!hit) {
+            int todel;
+            check_tetra(S[0], S[1], S[2], S[3], &coll, &d4, &todel);
+            if (coll == 0) return 0;
+            if (coll == 1) {
+                hit = true;
+            } else {
+                d3 n1, n2;
+                int a1, b1, a2, b2;
+                if (todel == 1) {
+                    n1 = S[1]; a1 = ia[1]; b1 = ib[1];
+                    n2 = S[2]; a2 = ia[2]; b2 = ib[2];
+                } else if (todel == 2) {
+                    n1 = S[2]; a1 = ia[2]; b1 = ib[2];
+                    n2 = S[0]; a2 = ia[0]; b2 = ib[0];
+                } else {
+                    n1 = S[0]; a1 = ia[0]; b1 = ib[0];
+                    n2 = S[1]; a2 = ia[1]; b2 = ib[1];
+                }
+                S[0] = n1; ia[0] = a1; ib[0] = b1;
+                S[1] = n2; ia[1] = a2; ib[1] = b2;
+                S[2] = S[3]; ia[2] = ia[3]; ib[2] = ib[3];
+            }
+        }
+        if (hit) {
+            if (correct_flat(S[0], S[1], S[2], S[3])) {
+                d4 = cross3(S[0], S[1]);
+                S[3] = support(A, B, d4, &ia[3], &ib[3]);
+            }
+            return 1;
+        }
+    }
+    return -1;
+}
+
+__global__ void __launch_bounds__(256) k_gjk(BodyView A, BodyView B, int64_t n_cap, const int64_t* n_dev,
+                                             int32_t* __restrict__ collision, Simplex* __restrict__ simplex) {
+    int64_t n = n_dev ? *n_dev : n_cap;
+    if (n > n_cap) n = n_cap;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+        const int64_t ra = view_row(A, p), rb = view_row(B, p);
+        const Side sa{A.verts + (int64_t)A.vmax * 3 * ra, A.nverts[ra]};
+        const Side sb{B.verts + (int64_t)B.vmax * 3 * rb, B.nverts[rb]};
+        d3 S[4];
+        int ia[4] = {0, 0, 0, 0}, ib[4] = {0, 0, 0, 0};
+        const int c = gjk_pair(sa, sb, ld3(A.com + 3 * ra), ld3(B.com + 3 * rb), S, ia, ib);
+        collision[p] = c;
+        if (c == 1) {
+            Simplex& x = simplex[p];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                x.s[k][0] = S[k].x;
+                x.s[k][1] = S[k].y;
+                x.s[k][2] = S[k].z;
+                x.ia[k] = ia[k];
+                x.ib[k] = ib[k];
+            }
+        }
+    }
+}
+
+__global__ void k_contact_flags(const int32_t* __restrict__ collision, int64_t n_cap, const int64_t* n_dev,
+                                int32_t* __restrict__ flag) {
+    int64_t n = n_dev ? *n_dev : n_cap;
+    if (n > n_cap) n = n_cap;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride)
+        flag[p] = collision[p] == 1 ? 1 : 0;
+}
+
+__global__ void k_contact_scatter(const int32_t* __restrict__ collision, const int64_t* __restrict__ off,
+                                  int64_t n_cap, const int64_t* n_dev, int32_t* __restrict__ contact_pair,
+                                  int64_t contact_cap, int64_t* n_contacts, uint32_t* flags) {
+    int64_t n = n_dev ? *n_dev : n_cap;
+    if (n > n_cap) n = n_cap;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid == 0 && *n_contacts > contact_cap) {
+        atomicOr(flags, MWD_F_CONTACT_OVERFLOW);
+    }
+    for (int64_t p = gid; p < n; p += stride) {
+        if (collision[p] == 1) {
+            const int64_t c = off[p];
+            if (c < contact_cap) contact_pair[c] = (int32_t)p;
+        }
+    }
+}
+
+// ------------------------------------------------------------------- EPA
+struct Poly {
+    d3 V[EPA_VERTS];
+    d3 n[EPA_FACES];
+    double dist[EPA_FACES];
+    uint8_t tr[EPA_FACES][3];
+    int8_t valid[EPA_FACES];
+    uint8_t opA[EPA_VERTS], opB[EPA_VERTS];
+    uint8_t vis[EPA_FACES];
+};
+
+// AddTriangle, .cu:270-325
+__device__ __forceinline__ void add_triangle(Poly& P, int e0, int e1, int e2, int id) {
+    d3 nn = cross3(P.V[e1] - P.V[e0], P.V[e2] - P.V[e1]);
+    const double len = sqrt(nrm2(nn));
+    nn = nn / len;
+    double dd = dot3(nn, P.V[e0]);
+    if (dd < 0.0) {
+        const int o = e2;
+        e2 = e1;
+        e1 = o;
+        dd = dd * -1.0;
+        nn = nn * -1.0;
+    }
+    P.tr[id][0] = (uint8_t)e0;
+    P.tr[id][1] = (uint8_t)e1;
+    P.tr[id][2] = (uint8_t)e2;
+    const d3 a = P.V[e0], b = P.V[e1], c = P.V[e2];
+    const bool inside = dot3(cross3(b - a, nn), neg3(a)) < 0.0 && dot3(cross3(c - b, nn), neg3(b)) < 0.0 &&
+                        dot3(cross3(a - c, nn), neg3(c)) < 0.0;
+    P.valid[id] = inside ? 1 : -2;
+    if (!inside) dd = 1e30;
+    P.n[id] = nn;
+    P.dist[id] = dd;
+}
+
+struct EpaResult {
+    double depth;
+    d3 n, cpA, cpB;
+    int status;
+};
+
+// EPA, .cu:328-603. status 1: expansion cap; 2: face storage exhausted.
+__device__ EpaResult epa(const Simplex& sx, const Side& A, const Side& B) {
+    Poly P;
+    double xmin_old = 1e29;
+    double proj = 2.0 * xmin_old + 1.0;
+    int counter = 0, nv = 4, nt = 0, indmin = 0;
+    double xmin_new = 0.0;
+    int st = 0;
+    for (int k = 0; k < 4; ++k) {
+        P.V[k] = ld3(sx.s[k]);
+        P.opA[k] = (uint8_t)sx.ia[k];
+        P.opB[k] = (uint8_t)sx.ib[k];
+    }
+    add_triangle(P, 0, 1, 2, 0);
+    add_triangle(P, 1, 2, 3, 1);
+    add_triangle(P, 2, 0, 3, 2);
+    add_triangle(P, 0, 1, 3, 3);
+    nt = 4;
+    while (xmin_old < proj) {
+        indmin = 0;
+        double xm = P.dist[0];
+        for (int k = 0; k < nt; ++k)
+            if (P.dist[k] < xm) {
+                xm = P.dist[k];
+                indmin = k;
+            }
+        xmin_new = P.dist[indmin];
+        xmin_old = xmin_new;
+        int a, b;
+        const d3 Suj = support(A, B, P.n[indmin], &a, &b);
+        proj = dot3(P.n[indmin], Suj);
+        P.opA[nv] = (uint8_t)a;
+        P.opB[nv] = (uint8_t)b;
+        if (xmin_old > proj || (xmin_old * 1.00001 > proj && xmin_old * 0.99999 < proj)) break;
+        if (counter > 49) {
+            st |= 1;
+            break;
+        }
+        counter = counter + 1;
+        P.V[nv] = Suj;
+        nv = nv + 1;
+        // faces that see the new point
+        int nvis = 0;
+        for (int i = 0; i < nt; ++i) {
+            if ((P.valid[i] == 1 || P.valid[i] == -2) && dot3(P.V[nv - 1] - P.V[P.tr[i][0]], P.n[i]) >= 0.0) {
+                P.valid[i] = -1;
+                P.vis[nvis++] = (uint8_t)i;
+            }
+        }
+        // horizon: an edge of a visible face that no other visible face holds
+        // (the reference's delpair count < 2, .cu:470-540)
+        bool overflow = false;
+        const int nt0 = nt;
+        for (int vi = 0; vi < nvis && !overflow; ++vi) {
+            const int i = P.vis[vi];
+            for (int e = 0; e < 3; ++e) {
+                const int u = P.tr[i][e], v = P.tr[i][e == 2 ? 0 : e + 1];
+                int cnt = 0;
+                for (int vj = 0; vj < nvis; ++vj) {
+                    const int j = P.vis[vj];
+                    const int x0 = P.tr[j][0], x1 = P.tr[j][1], x2 = P.tr[j][2];
+                    cnt += ((x0 == u && x1 == v) || (x0 == v && x1 == u));
+                    cnt += ((x1 == u && x2 == v) || (x1 == v && x2 == u));
+                    cnt += ((x2 == u && x0 == v) || (x2 == v && x0 == u));
+                }
+                if (cnt < 2) {
+                    if (nt == EPA_FACES) {
+                        overflow = true;
+                        break;
+                    }
+                    add_triangle(P, u, v, nv - 1, nt);
+                    nt = nt + 1;
+                }
+            }
+        }
+        (void)nt0;
+        if (overflow) {
+            st |= 2;
+            break;
+        }
+        for (int vi = 0; vi < nvis; ++vi) {
+            const int i = P.vis[vi];
+            P.valid[i] = 0;
+            P.dist[i] = 3e20;
+        }
+        counter = counter + 1;
+    }
+    EpaResult r;
+    r.depth = xmin_new;
+    const int a = P.tr[indmin][0], b = P.tr[indmin][1], c = P.tr[indmin][2];
+    const d3 nrm = P.n[indmin];
+    const d3 Proj = nrm * xmin_new;
+    const d3 Aa = P.V[a], Bb = P.V[b], Cc = P.V[c];
+    d3 v0 = Bb - Aa, v1 = Cc - Aa;
+    const d3 v2 = Proj - Aa;
+    if (dot3(cross3(v0, v1), nrm) < 0.0) {
+        v0 = Cc - Aa;
+        v1 = Bb - Aa;
+    }
+    const double v00 = dot3(v0, v0), v01 = dot3(v0, v1), v11 = dot3(v1, v1), v20 = dot3(v2, v0),
+                 v21 = dot3(v2, v1);
+    const double denom = v00 * v11 - v01 * v01;
+    const double vv = (v11 * v20 - v01 * v21) / denom;
+    const double ww = (v00 * v21 - v01 * v20) / denom;
+    const double uu = 1.0 - ww - vv;
+    r.cpA = (ld3(A.v + 3 * P.opA[a]) * uu + ld3(A.v + 3 * P.opA[b]) * vv) + ld3(A.v + 3 * P.opA[c]) * ww;
+    r.cpB = (ld3(B.v + 3 * P.opB[a]) * uu + ld3(B.v + 3 * P.opB[b]) * vv) + ld3(B.v + 3 * P.opB[c]) * ww;
+    r.n = nrm;
+    r.status = st;
+    return r;
+}
+
+struct LawOut {
+    d3 Fn, tA, tB, disp;
+};
+
+// Force, .cu:647-871
+__device__ LawOut force_law(d3 comA, d3 comB, d3 vA, d3 vB, d3 wA, d3 wB, d3 dispt, q4 qA, double isA,
+                            double isB, double depth, d3 n, d3 cpA, d3 cpB, const mwd_law& L) {
+    LawOut o;
+    const d3 cp = (cpA - cpB) / 2.0;
+    const d3 dA = cp - comA, dB = cp - comB;
+    const double C = (L.inv_mu + nrm2(dA) / isA) + nrm2(dB) / isB;
+    const double Kt = L.Kt_num / C;
+    const double Gt = (-2.0 / C) * L.ln_et;
+    if (dot3(comB - comA, n) < 0.0) n = n * -1.0;
+    n = n / sqrt(nrm2(n));
+    const d3 disp_n = n * depth;
+    const d3 u = ((vA + cross3(wA, cpA - comA)) - vB) - cross3(wB, cpB - comB);
+    const d3 un = n * dot3(u, n);
+    const d3 ut = u - un;
+    const double utn = sqrt(nrm2(ut));
+    const double dtn = sqrt(nrm2(dispt));
+    d3 t = mk3(0.0, 0.0, 0.0);
+    if (utn != 0.0) t = ut / utn;
+    if (utn == 0.0 && dtn != 0.0) t = dispt / dtn;
+    const d3 Fn = (disp_n * -1.0) * L.Kn - un * L.Gn;
+    const double Fnn = sqrt(nrm2(Fn));
+    d3 Ft = (dispt * -1.0) * Kt - ut * Gt;
+    const double Ftn = sqrt(nrm2(Ft));
+    const double muF = L.mu * Fnn;
+    d3 dnew;
+    if (Ftn > muF) Ft = ((t * Fnn) * -1.0) * L.mu;
+    if (Ftn <= muF) {
+        dnew = qrot(qA, dispt, qconj(qA)) + ut * L.dt;
+    } else {
+        dnew = ((((t * Fnn) * L.mu) * -1.0) - ut * Gt) / Kt;
+    }
+    const d3 Ff = Ft + Fn;
+    o.Fn = Fn;
+    o.tA = cross3(cp - comA, Ff);
+    o.tB = cross3(cp - comB, Ff * -1.0);
+    o.disp = dnew;
+    return o;
+}
+
+__global__ void __launch_bounds__(256) k_epa_force(BodyView A, BodyView B, const mwd_law* __restrict__ laws2,
+                                                   const int32_t* __restrict__ contact_pair,
+                                                   const Simplex* __restrict__ simplex, int64_t contact_cap,
+                                                   const int64_t* n_contacts, const double* __restrict__ dispt,
+                                                   const int32_t* __restrict__ idA, const int32_t* __restrict__ idB,
+                                                   double* __restrict__ depth, double* __restrict__ normal,
+                                                   double* __restrict__ cpA, double* __restrict__ cpB,
+                                                   double* __restrict__ Fn, double* __restrict__ tA,
+                                                   double* __restrict__ tB, double* __restrict__ disp_out,
+                                                   int32_t* __restrict__ epa_status, uint32_t* __restrict__ rec_key,
+                                                   int32_t* __restrict__ rec_val, int64_t* n_records,
+                                                   uint32_t* flags) {
+    int64_t K = *n_contacts;
+    if (K > contact_cap) K = contact_cap;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid == 0 && n_records) *n_records = 2 * K;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    uint32_t fl = 0;
+    for (int64_t c = gid; c < K; c += stride) {
+        const int64_t p = contact_pair[c];
+        const int64_t ra = view_row(A, p), rb = view_row(B, p);
+        const Side sa{A.verts + (int64_t)A.vmax * 3 * ra, A.nverts[ra]};
+        const Side sb{B.verts + (int64_t)B.vmax * 3 * rb, B.nverts[rb]};
+        const EpaResult e = epa(simplex[p], sa, sb);
+        const int wall = (A.is_wall[ra] || B.is_wall[rb]) ? 1 : 0;
+        const int scene = A.scene ? A.scene[ra] : 0;
+        const mwd_law& L = laws2[2 * scene + wall];
+        const d3 dt_in = dispt ? ld3(dispt + 4 * p) : mk3(0.0, 0.0, 0.0);
+        const LawOut f = force_law(ld3(A.com + 3 * ra), ld3(B.com + 3 * rb), ld3(A.vel + 3 * ra), ld3(B.vel + 3 * rb),
+                                   ld3(A.angvel + 3 * ra), ld3(B.angvel + 3 * rb), dt_in, ldq(A.quat + 4 * ra),
+                                   A.isize[ra], B.isize[rb], e.depth, e.n, e.cpA, e.cpB, L);
+        depth[c] = e.depth;
+        st3(normal + 3 * c, e.n);
+        st3(cpA + 3 * c, e.cpA);
+        st3(cpB + 3 * c, e.cpB);
+        st3(Fn + 3 * c, f.Fn);
+        st3(tA + 3 * c, f.tA);
+        st3(tB + 3 * c, f.tB);
+        if (epa_status) epa_status[c] = e.status;
+        if (e.status & 1) fl |= MWD_F_EPA_CAP;
+        if (e.status & 2) fl |= MWD_F_EPA_FACES;
+        if (disp_out) {
+            double* dp = disp_out + 4 * p;
+            dp[0] = f.disp.x;
+            dp[1] = f.disp.y;
+            dp[2] = f.disp.z;
+            dp[3] = 1.0;
+        }
+        rec_key[2 * c] = (uint32_t)(idA ? idA[p] : ra);
+        rec_val[2 * c] = (int32_t)(2 * c);
+        rec_key[2 * c + 1] = (uint32_t)(idB ? idB[p] : rb);
+        rec_val[2 * c + 1] = (int32_t)(2 * c + 1);
+    }
+    if (fl) atomicOr(flags, fl);
+}
+
+__global__ void k_segments(const uint32_t* __restrict__ key, int64_t cap, const int64_t* n_dev,
+                           int32_t* __restrict__ seg_begin, int32_t* __restrict__ seg_end) {
+    int64_t n = *n_dev;
+    if (n > cap) n = cap;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += stride) {
+        const uint32_t k = key[s];
+        if (s == 0 || key[s - 1] != k) seg_begin[k] = (int32_t)s;
+        if (s == n - 1 || key[s + 1] != k) seg_end[k] = (int32_t)(s + 1);
+    }
+}
+
+__global__ void k_add_sums(const int32_t* __restrict__ seg_begin, const int32_t* __restrict__ seg_end,
+                           const int32_t* __restrict__ rec_val, const double* __restrict__ Fn,
+                           const double* __restrict__ tA, const double* __restrict__ tB, double* __restrict__ force,
+                           double* __restrict__ torque, int64_t n_rows) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_rows) return;
+    const int32_t b = seg_begin[i], e = seg_end[i];
+    if (b == e) return;
+    d3 F = mk3(0.0, 0.0, 0.0), T = mk3(0.0, 0.0, 0.0);
+    for (int32_t s = b; s < e; ++s) {
+        const int32_t r = rec_val[s];
+        const int64_t c = r >> 1;
+        if (r & 1) {
+            F = F + neg3(ld3(Fn + 3 * c));
+            T = T + ld3(tB + 3 * c);
+        } else {
+            F = F + ld3(Fn + 3 * c);
+            T = T + ld3(tA + 3 * c);
+        }
+    }
+    st3(force + 3 * i, ld3(force + 3 * i) + F);
+    st3(torque + 3 * i, ld3(torque + 3 * i) + T);
+}
+
+// Per-pair views of per-contact results: non-contact pairs get zeros.
+__global__ void k_zero_pairs(int64_t P, double* depth, double* normal, double* cpA, double* cpB, int32_t* epa_status,
+                             double* disp) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    if (depth) depth[p] = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        if (normal) normal[3 * p + k] = 0.0;
+        if (cpA) cpA[3 * p + k] = 0.0;
+        if (cpB) cpB[3 * p + k] = 0.0;
+    }
+    if (epa_status) epa_status[p] = 0;
+    if (disp)
+        for (int k = 0; k < 4; ++k) disp[4 * p + k] = 0.0;
+}
+
+__global__ void k_scatter_contacts(const int32_t* __restrict__ cpair, int64_t cap, const int64_t* n_contacts,
+                                   const double* __restrict__ dep, const double* __restrict__ nrm,
+                                   const double* __restrict__ ca, const double* __restrict__ cb,
+                                   const int32_t* __restrict__ est, double* depth, double* normal, double* cpA,
+                                   double* cpB, int32_t* epa_status) {
+    int64_t K = *n_contacts;
+    if (K > cap) K = cap;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= K) return;
+    const int64_t p = cpair[c];
+    if (depth) depth[p] = dep[c];
+    for (int k = 0; k < 3; ++k) {
+        if (normal) normal[3 * p + k] = nrm[3 * c + k];
+        if (cpA) cpA[3 * p + k] = ca[3 * c + k];
+        if (cpB) cpB[3 * p + k] = cb[3 * c + k];
+    }
+    if (epa_status) epa_status[p] = est[c];
+}
+
+unsigned grid_for(int64_t n, int threads, int64_t max_blocks) {
+    int64_t b = (n + threads - 1) / threads;
+    if (b < 1) b = 1;
+    if (b > max_blocks) b = max_blocks;
+    return (unsigned)b;
+}
+
+}  // namespace
+
+int mwd_launch_gjk(const BodyView& A, const BodyView& B, int64_t n_cap, const int64_t* n_dev, int32_t* collision,
+                   Simplex* simplex, hipStream_t s) {
+    if (n_cap <= 0) return MWD_OK;
+    hipLaunchKernelGGL(k_gjk, dim3(grid_for(n_cap, 256, 8192)), dim3(256), 0, s, A, B, n_cap, n_dev, collision,
+                       simplex);
+    MWD_LAUNCH_CHECK();
+    return MWD_OK;
+}
+
+int mwd_launch_compact_contacts(const int32_t* collision, int64_t n_cap, const int64_t* n_dev, int32_t* flag,
+                                int64_t* off, int32_t* contact_pair, int64_t contact_cap, int64_t* n_contacts,
+                                uint32_t* flags, int64_t* tmp, int64_t tmp_len, hipStream_t s) {
+    if (n_cap <= 0) {
+        MWD_HIP_CHECK(hipMemsetAsync(n_contacts, 0, sizeof(int64_t), s));
+        return MWD_OK;
+    }
+    hipLaunchKernelGGL(k_contact_flags, dim3(grid_for(n_cap, 256, 8192)), dim3(256), 0, s, collision, n_cap, n_dev,
+                       flag);
+    MWD_LAUNCH_CHECK();
+    int rc = mwd_scan_i32(flag, off, n_contacts, n_cap, n_dev, tmp, tmp_len, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_contact_scatter, dim3(grid_for(n_cap, 256, 8192)), dim3(256), 0, s, collision, off, n_cap,
+                       n_dev, contact_pair, contact_cap, n_contacts, flags);
+    MWD_LAUNCH_CHECK();
+    return MWD_OK;
+}
+
+int mwd_launch_epa_force(const BodyView& A, const BodyView& B, const mwd_law* laws2, const int32_t* contact_pair,
+                         const Simplex* simplex, int64_t contact_cap, const int64_t* n_contacts, const double* dispt,
+                         const int32_t* idA, const int32_t* idB, double* depth, double* normal, double* cpA,
+                         double* cpB, double* Fn, double* tA, double* tB, double* disp_out, int32_t* epa_status,
+                         uint32_t* rec_key, int32_t* rec_val, int64_t* n_records, uint32_t* flags, hipStream_t s) {
+    if (contact_cap <= 0) return MWD_OK;
+    hipLaunchKernelGGL(k_epa_force, dim3(grid_for(contact_cap, 256, 4096)), dim3(256), 0, s, A, B, laws2,
+                       contact_pair, simplex, contact_cap, n_contacts, dispt, idA, idB, depth, normal, cpA, cpB, Fn,
+                       tA, tB, disp_out, epa_status, rec_key, rec_val, n_records, flags);
+    MWD_LAUNCH_CHECK();
+    return MWD_OK;
+}
+
+int mwd_launch_segments(const uint32_t* rec_key, int64_t rec_cap, const int64_t* n_rec, int32_t* seg_begin,
+                        int32_t* seg_end, hipStream_t s) {
+    if (rec_cap <= 0) return MWD_OK;
+    hipLaunchKernelGGL(k_segments, dim3(grid_for(rec_cap, 256, 8192)), dim3(256), 0, s, rec_key, rec_cap, n_rec,
+                       seg_begin, seg_end);
+    MWD_LAUNCH_CHECK();
+    return MWD_OK;
+}
+
+int mwd_launch_add_sums(const int32_t* seg_begin, const int32_t* seg_end, const int32_t* rec_val, const double* Fn,
+                        const double* tA, const double* tB, double* force, double* torque, int64_t n_rows,
+                        hipStream_t s) {
+    if (n_rows <= 0) return MWD_OK;
+    hipLaunchKernelGGL(k_add_sums, dim3((unsigned)((n_rows + 255) / 256)), dim3(256), 0, s, seg_begin, seg_end,
+                       rec_val, Fn, tA, tB, force, torque, n_rows);
+    MWD_LAUNCH_CHECK();
+    return MWD_OK;
+}
+
+int mwd_launch_zero_pairs(int64_t P, double* depth, double* normal, double* cpA, double* cpB, int32_t* epa_status,
+                          double* disp, hipStream_t s) {
+    if (P <= 0) return MWD_OK;
+    hipLaunchKernelGGL(k_zero_pairs, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, P, depth, normal, cpA, cpB,
+                       epa_status, disp);
+    MWD_LAUNCH_CHECK();
+    return MWD_OK;
+}
+
+int mwd_launch_scatter_contacts(const int32_t* cpair, int64_t cap, const int64_t* n_contacts, const double* dep,
+                                const double* nrm, const double* ca, const double* cb, const int32_t* est, int64_t P,
+                                double* depth, double* normal, double* cpA, double* cpB, int32_t* epa_status,
+                                double* disp, hipStream_t s) {
+    (void)disp;
+    (void)P;
+    if (cap <= 0 || !(depth || normal || cpA || cpB || epa_status)) return MWD_OK;
+    hipLaunchKernelGGL(k_scatter_contacts, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, s, cpair, cap,
+                       n_contacts, dep, nrm, ca, cb, est, depth, normal, cpA, cpB, epa_status);
+    MWD_LAUNCH_CHECK();
+    return MWD_OK;
+}

@@ -1,0 +1,347 @@
+// Coarse contact detection: cell keys, wall slab test and the linked-cell
+// candidate list, rebuilt as sort + scan + emit on the GPU.
+//
+// Reference semantics reproduced exactly (so the candidate list is identical,
+// order included):
+//   * keys: trunc(com/cellsize) per axis, key = xc + yc*ncx + zc*ncx*ncy
+//     (MWDiem.py:813-816);
+//   * wall pairs first: walls k outer, particles j inner, pair [j,k] when
+//     |n_k.(x_j-c_k)| < 6*max_particlesize (MWDiem.py:807-811);
+//   * then Linked_cell_hash (LIBRARY_Coarse_contact_detection.py:84-155):
+//     cells in order of their lowest particle id (dict insertion order), the
+//     members of a cell in CPython set iteration order, same-cell pairs j<k,
+//     then the 7 forward offsets +1, +ncx, +ncx*ncy, +ncx*ncy+1,
+//     +ncx*ncy+1+ncx, +ncx*ncy+ncx, +ncx+1 with A in the lower cell.
+//
+// Pipeline: keys -> stable radix sort of (key, row) -> cell heads (scan) ->
+// cell rank by lowest row (scan over rows) -> per-cell set order, neighbour
+// lookup (binary search of the sorted cell keys) and pair count -> scan of
+// counts in rank order -> emission. Wall tests are a flag/scan/scatter over
+// the (scene, wall, particle) test space.
+#include "mwd_internal.h"
+
+namespace {
+
+__device__ __forceinline__ int64_t key_of(d3 c, const double* cell, int64_t ncx, int64_t ncy) {
+    const int64_t xc = (int64_t)(c.x / cell[0]);
+    const int64_t yc = (int64_t)(c.y / cell[1]);
+    const int64_t zc = (int64_t)(c.z / cell[2]);
+    return xc + yc * ncx + zc * ncx * ncy;
+}
+
+__global__ void k_cell_keys(const double* __restrict__ com, int64_t n, double cx, double cy, double cz, int64_t ncx,
+                            int64_t ncy, int64_t* __restrict__ keys) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double cell[3] = {cx, cy, cz};
+    keys[i] = key_of(ld3(com + 3 * i), cell, ncx, ncy);
+}
+
+// Global sort keys of the moving particles.
+__global__ void k_sort_keys(const double* __restrict__ com, const int64_t* __restrict__ keys_in,
+                            const int32_t* __restrict__ plist, int64_t M, const int32_t* __restrict__ scene_of,
+                            const SceneDev* __restrict__ scenes, uint32_t* __restrict__ keys,
+                            int32_t* __restrict__ vals, uint32_t* flags) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    const int32_t row = plist[i];
+    const SceneDev& sc = scenes[scene_of[row]];
+    const int64_t key = keys_in ? keys_in[row] : key_of(ld3(com + 3 * row), sc.cell, sc.ncx, sc.ncy);
+    int64_t loc = key - sc.kmin;
+    if (loc < 0 || loc >= sc.kspan) {
+        atomicOr(flags, MWD_F_KEYRANGE);
+        loc = loc < 0 ? 0 : sc.kspan - 1;
+    }
+    keys[i] = (uint32_t)(sc.kbase + loc);
+    vals[i] = row;
+}
+
+__global__ void k_heads(const uint32_t* __restrict__ keys, int64_t M, int32_t* __restrict__ head) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M) return;
+    head[s] = (s == 0 || keys[s] != keys[s - 1]) ? 1 : 0;
+}
+
+__global__ void k_cells(const uint32_t* __restrict__ keys, const int32_t* __restrict__ vals, int64_t M,
+                        const int32_t* __restrict__ head, const int64_t* __restrict__ head_off,
+                        const int64_t* __restrict__ n_cells, int32_t* __restrict__ cell_start,
+                        uint32_t* __restrict__ cell_key, int32_t* __restrict__ row_first) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M) return;
+    if (head[s]) {
+        const int64_t c = head_off[s];
+        cell_start[c] = (int32_t)s;
+        cell_key[c] = keys[s];
+        row_first[vals[s]] = 1;
+    }
+    if (s == M - 1) cell_start[*n_cells] = (int32_t)M;
+}
+
+// CPython set iteration order of non-negative ints inserted in the given
+// order (Objects/setobject.c: 9 linear probes, then perturbation; resize to
+// the smallest power of two above 4*used once fill*5 >= mask*3).
+__device__ void cpy_probe(int32_t* tab, uint32_t mask, int32_t key) {
+    uint64_t perturb = (uint64_t)key;
+    uint64_t i = (uint64_t)key & mask;
+    for (;;) {
+        if (tab[i] < 0) {
+            tab[i] = key;
+            return;
+        }
+        if (i + 9 <= mask) {
+            for (uint64_t j = 1; j <= 9; ++j)
+                if (tab[i + j] < 0) {
+                    tab[i + j] = key;
+                    return;
+                }
+        }
+        perturb >>= 5;
+        i = (i * 5 + 1 + perturb) & mask;
+    }
+}
+
+constexpr int SET_TAB = 128;  // holds up to MWD_MAX_CELL members
+
+__device__ void set_order(const int32_t* rows, int n, int64_t row0, int32_t* out) {
+    if (n <= 1) {
+        if (n == 1) out[0] = rows[0];
+        return;
+    }
+    if (n <= 4) {  // 8-slot table, no resize
+        int32_t tab[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) tab[k] = -1;
+        for (int k = 0; k < n; ++k) cpy_probe(tab, 7u, (int32_t)(rows[k] - row0));
+        int o = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (tab[k] >= 0) out[o++] = (int32_t)(tab[k] + row0);
+        return;
+    }
+    int32_t tab[SET_TAB], tmp[SET_TAB];
+    uint32_t size = 8;
+    for (int k = 0; k < SET_TAB; ++k) tab[k] = -1;
+    uint32_t fill = 0;
+    for (int k = 0; k < n; ++k) {
+        cpy_probe(tab, size - 1, (int32_t)(rows[k] - row0));
+        ++fill;
+        if (fill * 5 >= (size - 1) * 3) {
+            const uint32_t minused = fill * 4;
+            uint32_t ns = 8;
+            while (ns <= minused) ns <<= 1;
+            for (uint32_t q = 0; q < ns; ++q) tmp[q] = -1;
+            for (uint32_t q = 0; q < size; ++q)
+                if (tab[q] >= 0) cpy_probe(tmp, ns - 1, tab[q]);
+            for (uint32_t q = 0; q < ns; ++q) tab[q] = tmp[q];
+            size = ns;
+        }
+    }
+    int o = 0;
+    for (uint32_t q = 0; q < size; ++q)
+        if (tab[q] >= 0) out[o++] = (int32_t)(tab[q] + row0);
+}
+
+__device__ __forceinline__ int64_t find_cell(const uint32_t* cell_key, int64_t C, uint32_t key) {
+    int64_t lo = 0, hi = C;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (cell_key[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < C && cell_key[lo] == key) ? lo : -1;
+}
+
+__global__ void k_cell_info(const int32_t* __restrict__ vals, const int32_t* __restrict__ cell_start,
+                            const uint32_t* __restrict__ cell_key, const int64_t* __restrict__ n_cells,
+                            int64_t M, const int32_t* __restrict__ scene_of, const SceneDev* __restrict__ scenes,
+                            const int64_t* __restrict__ row_rank, int32_t* __restrict__ perm,
+                            int32_t* __restrict__ cell_nb, int32_t* __restrict__ cell_by_rank,
+                            int32_t* __restrict__ cell_cnt, uint32_t* flags) {
+    const int64_t C = *n_cells;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    const int32_t start = cell_start[c], end = cell_start[c + 1];
+    const int n = end - start;
+    const int32_t row_min = vals[start];
+    const SceneDev& sc = scenes[scene_of[row_min]];
+    if (n > MWD_MAX_CELL) {
+        atomicOr(flags, MWD_F_CELL);
+        for (int k = 0; k < n; ++k) perm[start + k] = vals[start + k];
+    } else {
+        set_order(vals + start, n, sc.row0, perm + start);
+    }
+    const int64_t lk = (int64_t)cell_key[c] - sc.kbase + sc.kmin;
+    const int64_t offs[7] = {1, sc.ncx, sc.ncx * sc.ncy, sc.ncx * sc.ncy + 1, sc.ncx * sc.ncy + 1 + sc.ncx,
+                             sc.ncx * sc.ncy + sc.ncx, sc.ncx + 1};
+    int64_t cnt = (int64_t)n * (n - 1) / 2;
+    for (int o = 0; o < 7; ++o) {
+        const int64_t t = lk + offs[o] - sc.kmin;
+        int64_t nb = -1;
+        if (t >= 0 && t < sc.kspan) nb = find_cell(cell_key, C, (uint32_t)(sc.kbase + t));
+        cell_nb[c * 7 + o] = (int32_t)nb;
+        if (nb >= 0) cnt += (int64_t)n * (cell_start[nb + 1] - cell_start[nb]);
+    }
+    const int64_t r = row_rank[row_min];
+    cell_by_rank[r] = (int32_t)c;
+    cell_cnt[r] = (int32_t)cnt;
+}
+
+__global__ void k_cell_fill(const int32_t* __restrict__ perm, const int32_t* __restrict__ cell_start,
+                            const int32_t* __restrict__ cell_nb, const int32_t* __restrict__ cell_by_rank,
+                            const int64_t* __restrict__ cell_off, const int64_t* __restrict__ counters,
+                            int32_t* __restrict__ pairs, int64_t cap) {
+    const int64_t C = counters[0];
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= C) return;
+    const int32_t c = cell_by_rank[r];
+    int64_t pos = counters[1] + cell_off[r];
+    const int32_t s0 = cell_start[c], s1 = cell_start[c + 1];
+    for (int32_t j = s0; j < s1; ++j) {
+        const int32_t a = perm[j];
+        for (int32_t k = j + 1; k < s1; ++k, ++pos)
+            if (pos < cap) {
+                pairs[2 * pos] = a;
+                pairs[2 * pos + 1] = perm[k];
+            }
+    }
+    for (int o = 0; o < 7; ++o) {
+        const int32_t nb = cell_nb[c * 7 + o];
+        if (nb < 0) continue;
+        const int32_t t0 = cell_start[nb], t1 = cell_start[nb + 1];
+        for (int32_t j = s0; j < s1; ++j) {
+            const int32_t a = perm[j];
+            for (int32_t k = t0; k < t1; ++k, ++pos)
+                if (pos < cap) {
+                    pairs[2 * pos] = a;
+                    pairs[2 * pos + 1] = perm[k];
+                }
+        }
+    }
+}
+
+__device__ __forceinline__ int scene_of_test(const int64_t* wt_off, int S, int64_t t) {
+    int lo = 0, hi = S;  // largest s with wt_off[s] <= t
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (wt_off[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_wall_flags(const double* __restrict__ com, const double* __restrict__ wall_n,
+                             const double* __restrict__ wall_c, const SceneDev* __restrict__ scenes, int S,
+                             const int64_t* __restrict__ wt_off, int64_t T, int32_t* __restrict__ flag) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    const int s = scene_of_test(wt_off, S, t);
+    const SceneDev& sc = scenes[s];
+    const int64_t np = sc.n_active - sc.n_walls;
+    const int64_t tl = t - wt_off[s];
+    const int64_t k = tl / np, j = tl - k * np;
+    const int64_t rk = sc.row0 + k, rj = sc.row0 + sc.n_walls + j;
+    const d3 d = ld3(com + 3 * rj) - ld3(wall_c + 3 * rk);
+    const double dd = dot3(ld3(wall_n + 3 * rk), d);
+    flag[t] = fabs(dd) < sc.wall_thr ? 1 : 0;
+}
+
+__global__ void k_wall_fill(const int32_t* __restrict__ flag, const int64_t* __restrict__ off,
+                            const SceneDev* __restrict__ scenes, int S, const int64_t* __restrict__ wt_off,
+                            int64_t T, int32_t* __restrict__ pairs, int64_t cap) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T || !flag[t]) return;
+    const int64_t pos = off[t];
+    if (pos >= cap) return;
+    const int s = scene_of_test(wt_off, S, t);
+    const SceneDev& sc = scenes[s];
+    const int64_t np = sc.n_active - sc.n_walls;
+    const int64_t tl = t - wt_off[s];
+    const int64_t k = tl / np, j = tl - k * np;
+    pairs[2 * pos] = (int32_t)(sc.row0 + sc.n_walls + j);
+    pairs[2 * pos + 1] = (int32_t)(sc.row0 + k);
+}
+
+__global__ void k_finish_pairs(int64_t* counters, int64_t cap, uint32_t* flags) {
+    const int64_t P = counters[1] + counters[2];
+    counters[3] = P < cap ? P : cap;
+    counters[7] = P;
+    if (P > cap) atomicOr(flags, MWD_F_PAIR_OVERFLOW);
+}
+
+__global__ void k_zero_i64(int64_t* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256 > 0 ? (n + 255) / 256 : 1); }
+
+}  // namespace
+
+int mwd_launch_cell_keys(const double* com, int64_t n, const double* cell, int64_t ncx, int64_t ncy, int64_t* keys,
+                         hipStream_t s) {
+    if (n <= 0) return MWD_OK;
+    hipLaunchKernelGGL(k_cell_keys, dim3(blocks_for(n)), dim3(256), 0, s, com, n, cell[0], cell[1], cell[2], ncx, ncy,
+                       keys);
+    MWD_LAUNCH_CHECK();
+    return MWD_OK;
+}
+
+int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t* keys_in) {
+    hipStream_t s = ws->stream;
+    const int64_t M = ws->n_moving;
+    int64_t* ctr = ws->counters;
+    int rc;
+    // counters 0..3 = n_cells, n_wall_pairs, n_cell_pairs, n_pairs
+    mwd_mark(ws, 0);
+    hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(64), 0, s, ctr, 4);
+    MWD_LAUNCH_CHECK();
+    // -- wall slab pairs
+    if (ws->n_wall_tests > 0 && st->wall_n && st->wall_c) {
+        const int64_t T = ws->n_wall_tests;
+        hipLaunchKernelGGL(k_wall_flags, dim3(blocks_for(T)), dim3(256), 0, s, st->com, st->wall_n, st->wall_c,
+                           ws->scenes_dev, ws->n_scenes, ws->scene_wt_off, T, ws->wall_flag);
+        MWD_LAUNCH_CHECK();
+        rc = mwd_scan_i32(ws->wall_flag, ws->wall_off, ctr + 1, T, nullptr, ws->scan_tmp, ws->scan_tmp_len, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_wall_fill, dim3(blocks_for(T)), dim3(256), 0, s, ws->wall_flag, ws->wall_off,
+                           ws->scenes_dev, ws->n_scenes, ws->scene_wt_off, T, ws->pairs, ws->pair_cap);
+        MWD_LAUNCH_CHECK();
+    }
+    // -- linked-cell pairs
+    if (M > 0) {
+        hipLaunchKernelGGL(k_sort_keys, dim3(blocks_for(M)), dim3(256), 0, s, st->com, keys_in, ws->plist, M,
+                           ws->scene_of, ws->scenes_dev, ws->keys, ws->vals, ws->flags);
+        MWD_LAUNCH_CHECK();
+        uint32_t* ks;
+        int32_t* vs;
+        rc = mwd_radix_sort(ws->keys, ws->vals, ws->keys_alt, ws->vals_alt, M, nullptr, ws->key_passes, ws->hist,
+                            ws->hist_off, ws->scan_tmp, ws->scan_tmp_len, s, &ks, &vs);
+        if (rc) return rc;
+        ws->keys_sorted = ks;
+        ws->vals_sorted = vs;
+        mwd_mark(ws, 1);
+        hipLaunchKernelGGL(k_heads, dim3(blocks_for(M)), dim3(256), 0, s, ks, M, ws->head);
+        MWD_LAUNCH_CHECK();
+        rc = mwd_scan_i32(ws->head, ws->head_off, ctr + 0, M, nullptr, ws->scan_tmp, ws->scan_tmp_len, s);
+        if (rc) return rc;
+        rc = mwd_fill_i32(ws->row_first, 0, ws->n_rows, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_cells, dim3(blocks_for(M)), dim3(256), 0, s, ks, vs, M, ws->head,
+                           ws->head_off, ctr + 0, ws->cell_start, ws->cell_key, ws->row_first);
+        MWD_LAUNCH_CHECK();
+        rc = mwd_scan_i32(ws->row_first, ws->row_rank, nullptr, ws->n_rows, nullptr, ws->scan_tmp,
+                          ws->scan_tmp_len, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_cell_info, dim3(blocks_for(M)), dim3(256), 0, s, vs, ws->cell_start,
+                           ws->cell_key, ctr + 0, M, ws->scene_of, ws->scenes_dev, ws->row_rank, ws->perm,
+                           ws->cell_nb, ws->cell_by_rank, ws->cell_cnt, ws->flags);
+        MWD_LAUNCH_CHECK();
+        rc = mwd_scan_i32(ws->cell_cnt, ws->cell_off, ctr + 2, M, ctr + 0, ws->scan_tmp, ws->scan_tmp_len, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_cell_fill, dim3(blocks_for(M)), dim3(256), 0, s, ws->perm, ws->cell_start, ws->cell_nb,
+                           ws->cell_by_rank, ws->cell_off, ctr, ws->pairs, ws->pair_cap);
+        MWD_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_finish_pairs, dim3(1), dim3(1), 0, s, ctr, ws->pair_cap, ws->flags);
+    MWD_LAUNCH_CHECK();
+    return MWD_OK;
+}

@@ -1,0 +1,197 @@
+// Internal declarations shared by the translation units of libmwdiem_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/mwdiem_hip.h"
+#include "mwd_math.h"
+
+#define MWD_WAVE 64
+#define MWD_N_STAGES 7  // wall+keys+sort, cells+emit, gjk, compact, epa+law, record sort, reduce+verlet
+#define MWD_TIMING_RING 512
+
+// ------------------------------------------------------------------ errors
+void mwd_set_error(const std::string& msg);
+#define MWD_HIP_CHECK(expr)                                                                 \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            mwd_set_error(std::string(#expr) + ": " + hipGetErrorString(e_));               \
+            return MWD_E_HIP;                                                               \
+        }                                                                                   \
+    } while (0)
+#define MWD_LAUNCH_CHECK()                                                                  \
+    do {                                                                                    \
+        hipError_t e_ = hipGetLastError();                                                  \
+        if (e_ != hipSuccess) {                                                             \
+            mwd_set_error(std::string("kernel launch: ") + hipGetErrorString(e_));          \
+            return MWD_E_HIP;                                                               \
+        }                                                                                   \
+    } while (0)
+
+// ------------------------------------------------------------ device scene
+// Device copy of one scene plus the derived offsets the kernels need.
+struct SceneDev {
+    int64_t row0, n_walls, n_active;
+    double box[6];
+    double cell[3];
+    int64_t ncx, ncy, kmin, kspan;
+    int64_t kbase;      // first global (sorted) key of the scene
+    int64_t wt_off;     // first wall-slab test index of the scene
+    int64_t plist_off;  // first entry of the scene in the moving-particle list
+    double wall_thr;
+    mwd_law law_p, law_w;
+};
+
+// Final GJK simplex of a colliding pair (input of EPA).
+struct Simplex {
+    double s[4][3];
+    int32_t ia[4], ib[4];
+};
+
+// A body view: where the kernels find a pair's A (or B) side. Row r of the
+// view is rows[p] (or p when rows is NULL) for pair p.
+struct BodyView {
+    const double* com;
+    const double* verts;
+    const int32_t* nverts;
+    const double* vel;
+    const double* angvel;
+    const double* quat;
+    const double* isize;
+    const uint8_t* is_wall;     // per row of the view
+    const int32_t* scene;       // per row of the view (NULL: scene 0)
+    const int32_t* rows;        // NULL: identity
+    int32_t row_stride;         // pairs[p*row_stride] (2 for the interleaved pair list)
+    int32_t vmax;
+};
+
+// --------------------------------------------------------------- workspace
+struct mwd_workspace {
+    int device;
+    hipStream_t stream;
+    int32_t n_scenes;
+    int64_t n_rows, pair_cap, contact_cap;
+    int32_t vmax;
+    double dt;
+    mwd_scene* scenes_host;
+    SceneDev* scenes_dev;
+    int64_t n_moving;          // M: moving particles over all scenes
+    int64_t n_wall_tests;      // sum over scenes of W_s * (n_active_s - W_s)
+    int64_t key_span_total;    // sum of kspan
+    int key_passes, row_passes;
+    // per-row tables
+    int32_t* plist;            // [M] moving rows, row order
+    int32_t* scene_of;         // [n_rows]
+    uint8_t* is_wall;          // [n_rows]
+    // coarse stage
+    uint32_t *keys, *keys_alt; // [M]
+    int32_t *vals, *vals_alt;  // [M]
+    int32_t* head;             // [M] head flags, later set-ordered members
+    int64_t* head_off;         // [M]
+    int32_t* cell_start;       // [M+1]
+    uint32_t* cell_key;        // [M]
+    int32_t* cell_nb;          // [M*7]
+    int32_t* row_first;        // [n_rows]
+    int64_t* row_rank;         // [n_rows]
+    int32_t* cell_by_rank;     // [M]
+    int32_t* cell_cnt;         // [M] pair count by rank
+    int64_t* cell_off;         // [M]
+    int32_t* perm;             // [M] members of each cell in set order
+    int32_t* wall_flag;        // [n_wall_tests]
+    int64_t* wall_off;         // [n_wall_tests]
+    // candidate pairs
+    int32_t* pairs;            // [pair_cap*2]
+    int32_t* collision;        // [pair_cap]
+    Simplex* simplex;          // [pair_cap]
+    int32_t* pair_flag;        // [pair_cap]
+    int64_t* contact_off;      // [pair_cap]
+    // contacts
+    int32_t* contact_pair;     // [contact_cap]
+    double *depth, *normal, *cpA, *cpB, *Fn, *tA, *tB;
+    int32_t* epa_status;       // [contact_cap]
+    uint32_t *rec_key, *rec_key_alt;  // [2*contact_cap]
+    int32_t *rec_val, *rec_val_alt;
+    uint32_t* rec_key_sorted;         // whichever of rec_key / rec_key_alt holds the sorted records
+    int32_t* rec_val_sorted;
+    uint32_t* keys_sorted;            // whichever of keys / keys_alt holds the sorted cell keys
+    int32_t* vals_sorted;
+    int32_t *seg_begin, *seg_end;     // [n_rows]
+    // sort / scan scratch
+    int32_t* hist;             // [256 * max_tiles]
+    int64_t* hist_off;         // [256 * max_tiles]
+    int64_t* scan_tmp;         // block sums
+    int64_t scan_tmp_len;
+    // counters (device): 0 n_cells, 1 n_wall_pairs, 2 n_cell_pairs, 3 n_pairs,
+    // 4 n_contacts, 5 n_records, 6 n_moving, 7 scratch
+    int64_t* counters;
+    uint32_t* flags;
+    mwd_law* laws;             // [2*n_scenes] (particle law, wall law)
+    int64_t* scene_wt_off;     // [n_scenes+1] wall-test prefix (device)
+    // optional per-stage timing (HIP events on the workspace stream)
+    int timing;
+    hipEvent_t* ev;            // [MWD_TIMING_RING * (MWD_N_STAGES + 1)]
+    int ring_slot;             // steps recorded and not yet accumulated
+    double stage_ms[MWD_N_STAGES];
+    int64_t stage_calls;
+};
+
+// stage marks (events recorded before each stage when timing is on)
+void mwd_mark(mwd_workspace* ws, int stage);
+void mwd_mark_end(mwd_workspace* ws);
+
+// ----------------------------------------------------------- primitives
+// Exclusive scan of int32 values into int64 offsets. n is n_dev[0] when
+// n_dev is given (n_cap bounds it), else n_cap. total (device) may be NULL.
+int mwd_scan_i32(const int32_t* in, int64_t* out, int64_t* total, int64_t n_cap, const int64_t* n_dev,
+                 int64_t* tmp, int64_t tmp_len, hipStream_t s);
+int64_t mwd_scan_tmp_len(int64_t n_cap);
+// Stable LSD radix sort of (key, val) pairs on the low `passes`*8 key bits,
+// ping-ponging between (keys, vals) and the alt buffers; *keys_out/*vals_out
+// name the buffers holding the result (the alt ones when passes is odd).
+int mwd_radix_sort(uint32_t* keys, int32_t* vals, uint32_t* keys_alt, int32_t* vals_alt, int64_t n_cap,
+                   const int64_t* n_dev, int passes, int32_t* hist, int64_t* hist_off, int64_t* tmp,
+                   int64_t tmp_len, hipStream_t s, uint32_t** keys_out, int32_t** vals_out);
+int64_t mwd_sort_tiles(int64_t n_cap);
+int mwd_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
+int mwd_set_i64(int64_t* p, int64_t v, hipStream_t s);
+
+// ------------------------------------------------------------ stages
+int mwd_launch_verlet(double* v, double* v12, double* w12bf, double* q12, double* q, double* qall, double* wbf,
+                      double* w, const double* F, const double* tau, double* verts, double* com,
+                      const double* mass, const double* Iinv, const double* I, double* dxall, int32_t vmax,
+                      const int32_t* nrot, double dt, const double* box, int64_t pmin, int64_t pmax,
+                      hipStream_t s);
+int mwd_launch_reduce_verlet(mwd_workspace* ws, mwd_state* st);
+int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t* keys);
+int mwd_launch_cell_keys(const double* com, int64_t n, const double* cell, int64_t ncx, int64_t ncy,
+                         int64_t* keys, hipStream_t s);
+// GJK over pairs [0, n) (n from n_dev), writing collision and simplex.
+int mwd_launch_gjk(const BodyView& A, const BodyView& B, int64_t n_cap, const int64_t* n_dev,
+                   int32_t* collision, Simplex* simplex, hipStream_t s);
+// Contact compaction: contact_pair[c] = pair index of the c-th collision==1.
+int mwd_launch_compact_contacts(const int32_t* collision, int64_t n_cap, const int64_t* n_dev, int32_t* flag,
+                                int64_t* off, int32_t* contact_pair, int64_t contact_cap, int64_t* n_contacts,
+                                uint32_t* flags, int64_t* tmp, int64_t tmp_len, hipStream_t s);
+// EPA + force law over contacts [0, K). Reduction records are keyed by
+// idA[p]/idB[p] when given, else by the view rows.
+int mwd_launch_epa_force(const BodyView& A, const BodyView& B, const mwd_law* laws2, const int32_t* contact_pair,
+                         const Simplex* simplex, int64_t contact_cap, const int64_t* n_contacts, const double* dispt,
+                         const int32_t* idA, const int32_t* idB, double* depth, double* normal, double* cpA,
+                         double* cpB, double* Fn, double* tA, double* tB, double* disp_out, int32_t* epa_status,
+                         uint32_t* rec_key, int32_t* rec_val, int64_t* n_records, uint32_t* flags, hipStream_t s);
+// Zero the per-pair outputs, then copy per-contact results to their pairs.
+int mwd_launch_zero_pairs(int64_t P, double* depth, double* normal, double* cpA, double* cpB, int32_t* epa_status,
+                          double* disp, hipStream_t s);
+int mwd_launch_scatter_contacts(const int32_t* cpair, int64_t cap, const int64_t* n_contacts, const double* dep,
+                                const double* nrm, const double* ca, const double* cb, const int32_t* est, int64_t P,
+                                double* depth, double* normal, double* cpA, double* cpB, int32_t* epa_status,
+                                double* disp, hipStream_t s);
+// Segment bounds of sorted records per row.
+int mwd_launch_segments(const uint32_t* rec_key, int64_t rec_cap, const int64_t* n_rec, int32_t* seg_begin,
+                        int32_t* seg_end, hipStream_t s);
+// Add per-row contact sums onto force/torque rows (drop-in contact path).
+int mwd_launch_add_sums(const int32_t* seg_begin, const int32_t* seg_end, const int32_t* rec_val,
+                        const double* Fn, const double* tA, const double* tB, double* force, double* torque,
+                        int64_t n_rows, hipStream_t s);

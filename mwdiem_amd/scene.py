@@ -1,0 +1,444 @@
+"""Scene generator: convex polyhedra in a walled box, in the reference's data layout.
+
+A scene is what the reference driver packs into its SoA arrays before the time
+loop (MWDiem.py:603-647): rows [0, n_walls) are wall boxes, the remaining rows
+are particles; world vertices [N, vmax, 3] zero-padded; centre of mass; mass;
+body-frame inertia (diagonal) and its inverse; the inertia "size"
+(trace/3, Particle_class.py:151); external force m*g.
+
+Particle geometry follows Particle_class.Spherical_convex_polygon_maker_3D
+(CLASSES/Particle_class.py:239-303): the convex hull of random surface points;
+the centre of mass is the mean of the hull vertices (not the volume centroid);
+the inertia tensor about it is computed with Dobrovolskis' polyhedron
+formulae (:125-156); I_bf = diag(eigenvalues) in LAPACK order while the
+vertices stay in the hull frame (:288-291). Points are drawn area-uniformly on
+an ellipsoid, each particle gets a uniformly random orientation, and particles
+are seeded without overlap on a jittered body-centred-cubic lattice.
+
+Units: mm, kg, s.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+from scipy.spatial import ConvexHull
+
+G_MM = 9810.0          # mm/s^2
+RHO_KG_MM3 = 2.5e-6    # 2500 kg/m^3
+
+
+@dataclass
+class Law4:
+    Kn: float = 1e4
+    en: float = 0.5
+    et: float = 0.5
+    mu: float = 0.3
+
+
+@dataclass
+class SceneData:
+    n_walls: int
+    vmax: int
+    com: np.ndarray
+    verts: np.ndarray
+    nverts: np.ndarray
+    mass: np.ndarray
+    I: np.ndarray
+    Iinv: np.ndarray
+    isize: np.ndarray
+    fext: np.ndarray
+    text: np.ndarray
+    wall_n: np.ndarray
+    wall_c: np.ndarray
+    box: np.ndarray            # x_max, x_min, y_max, y_min, z_max, z_min (MWDiem.py:553-572)
+    cell: np.ndarray           # cellsize x, y, z (MWDiem.py:578-580)
+    ncx: int
+    ncy: int
+    ncz: int
+    kmin: int
+    kspan: int
+    max_particlesize: float
+    law_p: Law4 = field(default_factory=Law4)
+    law_w: Law4 = field(default_factory=Law4)
+    dt: float = 1e-6
+    name: str = ""
+    inner: tuple = (0.0, 0.0, 0.0)   # inner box extent (Lx, Ly, Lz)
+
+    @property
+    def n_rows(self) -> int:
+        return self.com.shape[0]
+
+    @property
+    def n_particles(self) -> int:
+        return self.n_rows - self.n_walls
+
+    @property
+    def wall_thr(self) -> float:
+        return 6 * self.max_particlesize
+
+    def copy(self) -> "SceneData":
+        import copy
+        return copy.deepcopy(self)
+
+
+# ----------------------------------------------------------------- geometry
+def ellipsoid_points(rng: np.random.Generator, n: int, axes) -> np.ndarray:
+    """n points distributed uniformly by area on the ellipsoid with semi-axes `axes`."""
+    a, b, c = axes
+    gmax = max(b * c, a * c, a * b)
+    out = np.empty((0, 3))
+    while out.shape[0] < n:
+        m = 4 * (n - out.shape[0]) + 8
+        u = rng.normal(size=(m, 3))
+        u /= np.linalg.norm(u, axis=1, keepdims=True)
+        g = np.sqrt((b * c * u[:, 0]) ** 2 + (a * c * u[:, 1]) ** 2 + (a * b * u[:, 2]) ** 2)
+        keep = rng.random(m) * gmax < g
+        out = np.vstack([out, u[keep] * np.array([a, b, c])])
+    return out[:n]
+
+
+def random_rotation(rng: np.random.Generator) -> np.ndarray:
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([
+        [1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+        [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+        [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)],
+    ])
+
+
+def hull_body(points: np.ndarray):
+    """Hull vertices (in scipy's vertex order), outward-oriented triangles as
+    vertex indices, and the hull volume."""
+    h = ConvexHull(points)
+    vidx = h.vertices
+    remap = -np.ones(points.shape[0], dtype=np.int64)
+    remap[vidx] = np.arange(vidx.size)
+    tri = remap[h.simplices]
+    V = points[vidx]
+    # orient each face outward w.r.t. the vertex mean (Particle_class.py:109-113)
+    cm = V.mean(axis=0)
+    d0, d1, d2 = V[tri[:, 0]], V[tri[:, 1]], V[tri[:, 2]]
+    nrm = np.cross(d1 - d0, d2 - d1)
+    flip = np.einsum("ij,ij->i", nrm, d0 - cm) < 0
+    tri[flip] = tri[flip][:, [0, 2, 1]]
+    return V, tri, float(h.volume)
+
+
+def polyhedron_inertia(V: np.ndarray, tri: np.ndarray, density: float, reference_volume_element: bool = False):
+    """Inertia tensor about the origin of a closed triangulated polyhedron
+    (Dobrovolskis 1996, the formulae of Particle_class.Calc_inertia :125-156).
+
+    With reference_volume_element=True the per-face volume element is the
+    reference's D.n_unit/6 (n normalised, Particle_class.py:110,137) instead of
+    the tetrahedron volume D.(E x F)/6.
+    """
+    D, E, F = V[tri[:, 0]], V[tri[:, 1]], V[tri[:, 2]]
+    if reference_volume_element:
+        n = np.cross(E - D, F - E)
+        n /= np.linalg.norm(n, axis=1, keepdims=True)
+        dV = np.einsum("ij,ij->i", D, n) / 6
+    else:
+        dV = np.einsum("ij,ij->i", D, np.cross(E, F)) / 6
+
+    def P(i, j):
+        return np.sum(density * dV / 20 * (2 * D[:, i] * D[:, j] + 2 * E[:, i] * E[:, j] + 2 * F[:, i] * F[:, j]
+                                           + D[:, i] * E[:, j] + D[:, j] * E[:, i] + D[:, i] * F[:, j]
+                                           + D[:, j] * F[:, i] + E[:, i] * F[:, j] + E[:, j] * F[:, i]))
+    Pxx, Pyy, Pzz = P(0, 0), P(1, 1), P(2, 2)
+    Pxy, Pxz, Pyz = P(0, 1), P(0, 2), P(1, 2)
+    I = np.array([[Pyy + Pzz, -Pxy, -Pxz], [-Pxy, Pxx + Pzz, -Pyz], [-Pxz, -Pyz, Pxx + Pyy]])
+    return I, (I[0, 0] + I[1, 1] + I[2, 2]) / 3
+
+
+def body_frame(I: np.ndarray):
+    """I_bf = diag(eig(I)) in LAPACK order and its inverse (Particle_class.py:289-291)."""
+    ev = np.linalg.eig(I)[0].real
+    Ibf = np.diag(ev)
+    return Ibf, np.linalg.inv(Ibf)
+
+
+def make_particle(rng, npts: int, axes, size: float, density: float, ref_inertia: bool):
+    pts = ellipsoid_points(rng, npts, np.asarray(axes) * size)
+    V, tri, vol = hull_body(pts)
+    cm = V.mean(axis=0)
+    V = V - cm
+    I, isize = polyhedron_inertia(V, tri, density, ref_inertia)
+    Ibf, Iinv = body_frame(I)
+    R = random_rotation(rng)
+    Vw = V @ R.T
+    return Vw, density * vol, Ibf, Iinv, isize, face_normals(Vw, tri)
+
+
+_CHUNK = 2048
+
+
+def _particle_chunk(args):
+    seed, chunk, n, npts, axes, size, density, ref_inertia = args
+    rng = np.random.Generator(np.random.PCG64([seed, chunk]))
+    out = []
+    for _ in range(n):
+        k = int(rng.integers(npts[0], npts[1] + 1))
+        s = float(rng.uniform(size[0], size[1]))
+        out.append(make_particle(rng, k, axes, s, density, ref_inertia))
+    return out
+
+
+def make_particles(n: int, seed: int, npts, axes, size, density, ref_inertia, workers: int | None = None):
+    """n particles in chunks of 2048, chunk c drawn from PCG64([seed, c]): the
+    result does not depend on the number of worker processes."""
+    jobs = [(seed, c, min(_CHUNK, n - c * _CHUNK), tuple(npts), tuple(axes), tuple(size), density, ref_inertia)
+            for c in range((n + _CHUNK - 1) // _CHUNK)]
+    if len(jobs) <= 1:
+        return [p for j in jobs for p in _particle_chunk(j)]
+    import multiprocessing as mp
+    import os
+    nw = workers or min(len(jobs), int(os.environ.get("MWD_GEN_WORKERS", "0")) or min(16, os.cpu_count() or 1))
+    if nw <= 1:
+        return [p for j in jobs for p in _particle_chunk(j)]
+    with mp.get_context("fork").Pool(nw) as pool:
+        res = pool.map(_particle_chunk, jobs)
+    return [p for r in res for p in r]
+
+
+def box_corners(lo, hi):
+    lo, hi = np.asarray(lo, float), np.asarray(hi, float)
+    return np.array([[x, y, z] for x in (hi[0], lo[0]) for y in (hi[1], lo[1]) for z in (hi[2], lo[2])])
+
+
+def walls_for_box(L, T: float, density: float, ref_inertia: bool):
+    """Six wall boxes of thickness T around [0,Lx]x[0,Ly]x[0,Lz]:
+    floor, ceiling, x-, x+, y-, y+. Returns per-wall (vertices, normal,
+    centre, mass, I, Iinv, isize)."""
+    Lx, Ly, Lz = L
+    specs = [
+        ((-T, -T, -T), (Lx + T, Ly + T, 0.0), (0.0, 0.0, 1.0)),
+        ((-T, -T, Lz), (Lx + T, Ly + T, Lz + T), (0.0, 0.0, 1.0)),
+        ((-T, -T, 0.0), (0.0, Ly + T, Lz), (1.0, 0.0, 0.0)),
+        ((Lx, -T, 0.0), (Lx + T, Ly + T, Lz), (1.0, 0.0, 0.0)),
+        ((0.0, -T, 0.0), (Lx, 0.0, Lz), (0.0, 1.0, 0.0)),
+        ((0.0, Ly, 0.0), (Lx, Ly + T, Lz), (0.0, 1.0, 0.0)),
+    ]
+    out = []
+    for lo, hi, n in specs:
+        P = box_corners(lo, hi)
+        V, tri, vol = hull_body(P)
+        c = V.mean(axis=0)
+        I, isize = polyhedron_inertia(V - c, tri, density, ref_inertia)
+        Ibf, Iinv = body_frame(I)
+        out.append((V, np.array(n), c, density * vol, Ibf, Iinv, isize))
+    return out
+
+
+def bcc_sites(L, spacing: float, r: float, n: int) -> np.ndarray:
+    """First n sites of a body-centred-cubic packing filling [r, L-r] from the
+    floor up (nearest neighbours `spacing` apart)."""
+    Lx, Ly, Lz = L
+    a = spacing
+    dz = a / math.sqrt(2.0)
+    sites = []
+    k = 0
+    while len(sites) < n:
+        z = r + k * dz
+        if z > Lz - r + 1e-12:
+            raise ValueError(f"box too small for {n} particles at spacing {a}")
+        off = 0.0 if k % 2 == 0 else a / 2
+        xs = np.arange(r + off, Lx - r + 1e-12, a)
+        ys = np.arange(r + off, Ly - r + 1e-12, a)
+        X, Y = np.meshgrid(xs, ys, indexing="ij")
+        layer = np.stack([X.ravel(), Y.ravel(), np.full(X.size, z)], axis=1)
+        sites.extend(layer.tolist())
+        k += 1
+    return np.asarray(sites[:n])
+
+
+def face_normals(V: np.ndarray, tri: np.ndarray) -> np.ndarray:
+    n = np.cross(V[tri[:, 1]] - V[tri[:, 0]], V[tri[:, 2]] - V[tri[:, 1]])
+    return n / np.linalg.norm(n, axis=1, keepdims=True)
+
+
+def _separated(VA, NA, VB, NB) -> bool:
+    """True when a face normal of A or B separates the two hulls (a
+    sufficient test: hulls it cannot separate are treated as overlapping)."""
+    N = np.vstack([NA, NB])
+    pa = VA @ N.T
+    pb = VB @ N.T
+    return bool(np.any(pa.max(axis=0) < pb.min(axis=0)) or np.any(pb.max(axis=0) < pa.min(axis=0)))
+
+
+def deposit_sites(parts_V, parts_N, L, rng, gap: float = 0.02, window: int = 4000) -> np.ndarray:
+    """Bottom-up deposition: particle i goes to the lowest free candidate site
+    (fine grid, random order inside each layer) where it lies inside the box
+    and is separated from every particle already placed."""
+    Lx, Ly, Lz = L
+    n = len(parts_V)
+    rcirc = np.array([np.max(np.linalg.norm(V, axis=1)) for V in parts_V])
+    rin = np.array([np.min(np.einsum("ij,ij->i", N, V[np.argmax(V @ N.T, axis=0)]))
+                    for V, N in zip(parts_V, parts_N)])
+    rin_min = float(np.min(rin))
+    h = 0.4 * float(np.mean(rcirc))
+    xs = np.arange(h / 2, Lx, h)
+    ys = np.arange(h / 2, Ly, h)
+    zs = np.arange(h / 2, Lz, h)
+    layers = []
+    for z in zs:
+        X, Y = np.meshgrid(xs, ys, indexing="ij")
+        lay = np.stack([X.ravel(), Y.ravel(), np.full(X.size, z)], axis=1)
+        layers.append(lay[rng.permutation(lay.shape[0])])
+    sites = np.concatenate(layers)
+    free = np.ones(sites.shape[0], dtype=bool)
+    centers = np.zeros((n, 3))
+    start = 0
+    for i in range(n):
+        V, N = parts_V[i], parts_N[i]
+        lo, hi = V.min(axis=0), V.max(axis=0)
+        placed = False
+        w0 = start
+        while not placed and w0 < sites.shape[0]:
+            idx = np.arange(w0, min(w0 + window, sites.shape[0]))
+            idx = idx[free[idx]]
+            c = sites[idx]
+            ok = np.all(c + lo >= gap, axis=1) & (c[:, 0] + hi[0] <= Lx - gap) & (c[:, 1] + hi[1] <= Ly - gap) \
+                & (c[:, 2] + hi[2] <= Lz - gap)
+            if i:
+                d = np.linalg.norm(c[:, None, :] - centers[None, :i, :], axis=2)
+                ok &= np.all(d >= rin[None, :i] + rin[i], axis=1)
+            cand = idx[ok]
+            if cand.size:
+                C = sites[cand]
+                good = np.ones(cand.size, dtype=bool)
+                if i:
+                    dmin = np.linalg.norm(C[:, None, :] - centers[None, :i, :], axis=2)
+                    near = np.nonzero(np.any(dmin < rcirc[:i] + rcirc[i] + gap, axis=0))[0]
+                    pA = V @ N.T
+                    for j in near:
+                        Nij = np.vstack([N, parts_N[j]])
+                        pa = V @ Nij.T                       # (nv, M)
+                        pb = (parts_V[j] + centers[j]) @ Nij.T
+                        shift = C @ Nij.T                   # (K, M)
+                        sep = np.any((pa.max(axis=0) + shift < pb.min(axis=0)) |
+                                     (pb.max(axis=0) < pa.min(axis=0) + shift), axis=1)
+                        far = dmin[:, j] >= rcirc[j] + rcirc[i] + gap
+                        good &= sep | far
+                    del pA
+                hit = np.nonzero(good)[0]
+                if hit.size:
+                    k = cand[hit[0]]
+                    ck = sites[k]
+                    centers[i] = ck
+                    # sites inside this particle's insphere can host no centre
+                    dd = np.linalg.norm(sites - ck, axis=1)
+                    free[dd < rin[i] + rin_min] = False
+                    free[k] = False
+                    placed = True
+            w0 += window
+        if not placed:
+            raise ValueError(f"deposition could not place particle {i}")
+        while start < sites.shape[0] and not free[start]:
+            start += 1
+    return centers
+
+
+def grid_for(box: np.ndarray, max_particlesize: float):
+    """Cell grid of MWDiem.py:575-580 and the key range of coms inside the box."""
+    xmax, xmin, ymax, ymin, zmax, zmin = box
+    ncx = int((xmax - xmin) / max_particlesize)
+    ncy = int((ymax - ymin) / max_particlesize)
+    ncz = int((zmax - zmin) / max_particlesize)
+    cell = np.array([(xmax - xmin) / ncx, (ymax - ymin) / ncy, (zmax - zmin) / ncz])
+    lo = [int(xmin / cell[0]), int(ymin / cell[1]), int(zmin / cell[2])]
+    hi = [int(xmax / cell[0]), int(ymax / cell[1]), int(zmax / cell[2])]
+    kmin = lo[0] + lo[1] * ncx + lo[2] * ncx * ncy
+    kmax = hi[0] + hi[1] * ncx + hi[2] * ncx * ncy
+    return cell, ncx, ncy, ncz, kmin, kmax - kmin + 1
+
+
+def make_scene(n_particles: int, box_mm, seed: int, npts=(12, 12), axes=(1.0, 0.8, 0.6), size=(0.8, 1.2),
+               density: float = RHO_KG_MM3, law: Law4 | None = None, wall_law: Law4 | None = None,
+               dt: float = 1e-6, g=(0.0, 0.0, -G_MM), wall_thickness_factor: float = 10.0,
+               spacing_factor: float = 1.05, ref_inertia: bool = False, name: str = "") -> SceneData:
+    """A bed of `n_particles` hulls of npts[0]..npts[1] points on an ellipsoid
+    with semi-axes axes*U(size) in a walled box (the configurations' generator).
+    The cell size (max_particlesize) is 2x the largest circumradius."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    law = law or Law4()
+    wall_law = wall_law or law
+    parts = make_particles(n_particles, seed, npts, axes, size, density, ref_inertia)
+    rmax = max(float(np.max(np.linalg.norm(p[0], axis=1))) for p in parts) if parts else 1.0
+    ms = 2.0 * rmax
+    spacing = spacing_factor * ms
+    try:
+        sites = bcc_sites(box_mm, spacing, rmax, n_particles)
+        jit = (spacing - ms) / 2.0 / math.sqrt(3.0)
+        sites = sites + rng.uniform(-jit, jit, size=sites.shape)
+    except ValueError:
+        # too tight for bounding spheres: deposit with an exact separation test
+        sites = deposit_sites([p[0] for p in parts], [p[5] for p in parts], box_mm, rng)
+    T = wall_thickness_factor * ms
+    walls = walls_for_box(box_mm, T, density, ref_inertia)
+    W = len(walls)
+    N = W + n_particles
+    vmax = max([w[0].shape[0] for w in walls] + [p[0].shape[0] for p in parts])
+    com = np.zeros((N, 3))
+    verts = np.zeros((N, vmax, 3))
+    nverts = np.zeros(N, dtype=np.int32)
+    mass = np.zeros(N)
+    I = np.zeros((N, 3, 3))
+    Iinv = np.zeros((N, 3, 3))
+    isize = np.zeros(N)
+    wall_n = np.zeros((N, 3))
+    wall_c = np.zeros((N, 3))
+    for k, (V, n, c, m, Ibf, Iiv, isz) in enumerate(walls):
+        com[k] = c
+        verts[k, :V.shape[0]] = V
+        nverts[k] = V.shape[0]
+        mass[k], I[k], Iinv[k], isize[k] = m, Ibf, Iiv, isz
+        wall_n[k], wall_c[k] = n, c
+    for i, (V, m, Ibf, Iiv, isz, _) in enumerate(parts):
+        r = W + i
+        com[r] = sites[i]
+        verts[r, :V.shape[0]] = V + sites[i]
+        nverts[r] = V.shape[0]
+        mass[r], I[r], Iinv[r], isize[r] = m, Ibf, Iiv, isz
+    allv = np.concatenate([w[0] for w in walls])
+    box = np.array([allv[:, 0].max(), allv[:, 0].min(), allv[:, 1].max(), allv[:, 1].min(),
+                    allv[:, 2].max(), allv[:, 2].min()])
+    cell, ncx, ncy, ncz, kmin, kspan = grid_for(box, ms)
+    fext = mass[:, None] * np.asarray(g)[None, :]
+    fext[:W] = 0.0
+    return SceneData(n_walls=W, vmax=vmax, com=com, verts=verts, nverts=nverts, mass=mass, I=I, Iinv=Iinv,
+                     isize=isize, fext=fext, text=np.zeros((N, 3)), wall_n=wall_n, wall_c=wall_c, box=box,
+                     cell=cell, ncx=ncx, ncy=ncy, ncz=ncz, kmin=kmin, kspan=kspan, max_particlesize=ms,
+                     law_p=law, law_w=wall_law, dt=dt, name=name, inner=tuple(float(x) for x in box_mm))
+
+
+# ------------------------------------------------------- the configurations
+def config_small(seed: int = 1) -> SceneData:
+    """200 hulls of 12 points, 10x10x30 mm box (plain method, small scene)."""
+    return make_scene(200, (10.0, 10.0, 30.0), seed, npts=(12, 12), name="C1-small-200")
+
+
+def config_large(seed: int = 2, n: int = 100_000) -> SceneData:
+    """100,000 hulls of 8-20 points, 200x200x400 mm box (plain method, large settle)."""
+    return make_scene(n, (200.0, 200.0, 400.0), seed, npts=(8, 20), name=f"C2-large-{n}")
+
+
+def compact_dense(sc: SceneData, gap: float = -0.02, seed: int = 0) -> SceneData:
+    """Copy of a scene with the particles moved onto a BCC lattice of spacing
+    (1+gap)*2*r_max from the floor up, so neighbouring hulls overlap (a
+    contact-heavy state for stage tests). Only centres and vertices change."""
+    out = sc.copy()
+    W = sc.n_walls
+    rmax = sc.max_particlesize / 2
+    sites = bcc_sites(sc.inner, (1.0 + gap) * 2 * rmax, rmax, sc.n_particles)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sites = sites + rng.uniform(-0.01 * rmax, 0.01 * rmax, size=sites.shape)
+    for i in range(sc.n_particles):
+        r = W + i
+        d = sites[i] - out.com[r]
+        out.com[r] = sites[i]
+        out.verts[r, :out.nverts[r]] += d
+    return out

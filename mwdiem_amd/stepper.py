@@ -1,0 +1,237 @@
+"""Device-resident time stepping: the reference's time-loop body
+(MWDiem.py:805-1001) as one stream-ordered sequence of HIP kernels.
+
+One Stepper holds one or many independent scenes (the batched form): rows of
+all scenes are concatenated, each scene keeps its own walls, grid, law and
+box, and every stage runs as one launch over all scenes. State lives in
+PyTorch-ROCm tensors (FP64, the reference's array layouts); the kernels are
+reached through the C ABI (mwdiem_amd/_lib.py).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from .scene import SceneData
+
+STATE_KEYS = ("com", "v", "v12", "w", "wbf", "w12bf", "q", "q12", "qall", "dxall", "verts")
+CONST_KEYS = ("nverts", "mass", "I", "Iinv", "isize", "fext", "text", "wall_n", "wall_c")
+
+
+def _require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError("mwdiem_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    _lib.load()
+
+
+class Stepper:
+    """Device-resident DEM step over one or more scenes.
+
+    pair_cap / contact_cap bound the candidate and contact buffers; a step
+    that exceeds them raises (after the step) from check().
+    """
+
+    def __init__(self, scenes: SceneData | Sequence[SceneData], device: int = 0, pair_cap: int | None = None,
+                 contact_cap: int | None = None, stream: torch.cuda.Stream | None = None):
+        _require_gpu()
+        if isinstance(scenes, SceneData):
+            scenes = [scenes]
+        self.scenes = list(scenes)
+        if len({sc.dt for sc in self.scenes}) != 1:
+            raise ValueError("all scenes of a Stepper share one dt")
+        self.dt = self.scenes[0].dt
+        self.device = torch.device("cuda", device)
+        self.stream = stream or torch.cuda.current_stream(self.device)
+        self.vmax = max(sc.vmax for sc in self.scenes)
+        rows = [sc.n_rows for sc in self.scenes]
+        self.row0 = np.concatenate([[0], np.cumsum(rows)[:-1]]).astype(np.int64)
+        self.n_rows = int(sum(rows))
+        n_mov = sum(sc.n_particles for sc in self.scenes)
+        n_wall_tests = sum(sc.n_walls * sc.n_particles for sc in self.scenes)
+        self.pair_cap = int(pair_cap or max(4096, 32 * n_mov + n_wall_tests))
+        self.contact_cap = int(contact_cap or max(1024, 8 * n_mov))
+        dev = self.device
+
+        def cat(key, dtype):
+            parts = []
+            for sc in self.scenes:
+                a = getattr(sc, key)
+                if key == "verts" and a.shape[1] < self.vmax:
+                    a = np.concatenate([a, np.zeros((a.shape[0], self.vmax - a.shape[1], 3))], axis=1)
+                parts.append(a)
+            return torch.from_numpy(np.ascontiguousarray(np.concatenate(parts), dtype=dtype)).to(dev)
+
+        self.t = {}
+        self.t["com"] = cat("com", np.float64)
+        self.t["verts"] = cat("verts", np.float64)
+        for k in ("v", "v12", "w", "wbf", "w12bf", "dxall"):
+            self.t[k] = torch.zeros((self.n_rows, 3), dtype=torch.float64, device=dev)
+        for k in ("q", "q12", "qall"):
+            q = torch.zeros((self.n_rows, 4), dtype=torch.float64, device=dev)
+            q[:, 0] = 1.0
+            self.t[k] = q
+        self.t["nverts"] = cat("nverts", np.int32)
+        for k in ("mass", "I", "Iinv", "isize", "fext", "text", "wall_n", "wall_c"):
+            self.t[k] = cat(k, np.float64)
+
+        sc_arr = (_lib.Scene * len(self.scenes))()
+        for s, sc in enumerate(self.scenes):
+            d = sc_arr[s]
+            d.row0 = int(self.row0[s])
+            d.n_walls = sc.n_walls
+            d.n_active = sc.n_rows
+            for k in range(6):
+                d.box[k] = float(sc.box[k])
+            for k in range(3):
+                d.cell[k] = float(sc.cell[k])
+            d.ncx, d.ncy, d.kmin, d.kspan = sc.ncx, sc.ncy, sc.kmin, sc.kspan
+            d.wall_thr = sc.wall_thr
+            d.law_p = _lib.law(sc.law_p.Kn, sc.law_p.en, sc.law_p.et, sc.law_p.mu, self.dt)
+            d.law_w = _lib.law(sc.law_w.Kn, sc.law_w.en, sc.law_w.et, sc.law_w.mu, self.dt)
+        self._scene_arr = sc_arr
+        lib = _lib.load()
+        ws = C.c_void_p()
+        with torch.cuda.device(self.device):
+            torch.cuda.synchronize(self.device)
+            _lib.check(lib.mwd_workspace_create(C.byref(ws), self.device.index, sc_arr, len(self.scenes),
+                                                self.n_rows, self.vmax, self.pair_cap, self.contact_cap, self.dt,
+                                                self.stream.cuda_stream or None), "mwd_workspace_create")
+        self.ws = ws
+        self.state = _lib.State()
+        self._fill_state()
+        self.steps_done = 0
+
+    # ------------------------------------------------------------ plumbing
+    def _fill_state(self):
+        s = self.state
+        s.n_rows = self.n_rows
+        s.vmax = self.vmax
+        for k in STATE_KEYS + CONST_KEYS:
+            setattr(s, k, self.t[k].data_ptr())
+
+    def close(self):
+        if getattr(self, "ws", None):
+            _lib.load().mwd_workspace_destroy(self.ws)
+            self.ws = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --------------------------------------------------------------- stages
+    def candidates(self):
+        _lib.check(_lib.load().mwd_candidates(self.ws, C.byref(self.state), None), "mwd_candidates")
+
+    def fine_contact(self):
+        _lib.check(_lib.load().mwd_fine_contact(self.ws, C.byref(self.state)), "mwd_fine_contact")
+
+    def reduce_verlet(self):
+        _lib.check(_lib.load().mwd_reduce_verlet(self.ws, C.byref(self.state)), "mwd_reduce_verlet")
+
+    def step(self, n: int = 1):
+        lib = _lib.load()
+        for _ in range(n):
+            _lib.check(lib.mwd_step(self.ws, C.byref(self.state)), "mwd_step")
+        self.steps_done += n
+
+    def set_timing(self, on: bool = True):
+        _lib.check(_lib.load().mwd_set_timing(self.ws, 1 if on else 0), "mwd_set_timing")
+
+    def stage_times(self) -> dict:
+        """Accumulated per-stage milliseconds since the last call (timing on)."""
+        ms = (C.c_double * _lib.N_STAGES)()
+        n = C.c_int64()
+        _lib.check(_lib.load().mwd_stage_times(self.ws, ms, C.byref(n)), "mwd_stage_times")
+        return dict(steps=n.value, ms=dict(zip(_lib.STAGE_NAMES, list(ms))))
+
+    # --------------------------------------------------------------- status
+    def status(self) -> dict:
+        st = _lib.Status()
+        _lib.check(_lib.load().mwd_step_status(self.ws, C.byref(st)), "mwd_step_status")
+        return dict(n_pairs=st.n_pairs, n_wall_pairs=st.n_wall_pairs, n_contacts=st.n_contacts,
+                    n_cells=st.n_cells, flags=st.flags)
+
+    def check(self) -> dict:
+        s = self.status()
+        f = s["flags"]
+        if f & _lib.F_PAIR_OVERFLOW:
+            raise _lib.MwdError(_lib.MWD_E_CAPACITY, "step", f"{s['n_pairs']} candidate pairs > pair_cap {self.pair_cap}")
+        if f & _lib.F_CONTACT_OVERFLOW:
+            raise _lib.MwdError(_lib.MWD_E_CAPACITY, "step", f"{s['n_contacts']} contacts > contact_cap {self.contact_cap}")
+        if f & _lib.F_KEYRANGE:
+            raise _lib.MwdError(_lib.MWD_E_KEYRANGE, "step", "a cell key left its scene's key range")
+        if f & _lib.F_CELL:
+            raise _lib.MwdError(_lib.MWD_E_CELL, "step", f"a cell holds more than {_lib.MAX_CELL} particles")
+        return s
+
+    # ---------------------------------------------------------------- views
+    def pairs(self) -> np.ndarray:
+        p = C.c_void_p()
+        npairs = self.status()["n_pairs"]
+        _lib.check(_lib.load().mwd_pairs_view(self.ws, C.byref(p), None), "mwd_pairs_view")
+        n = min(npairs, self.pair_cap)
+        return _copy_from_device(p.value, n * 2, np.int32).reshape(n, 2)
+
+    def contacts(self) -> dict:
+        ptrs = [C.c_void_p() for _ in range(11)]
+        _lib.check(_lib.load().mwd_contacts_view(self.ws, *[C.byref(x) for x in ptrs]), "mwd_contacts_view")
+        s = self.status()
+        P = min(s["n_pairs"], self.pair_cap)
+        K = min(s["n_contacts"], self.contact_cap)
+        coll = _copy_from_device(ptrs[0].value, P, np.int32)
+        out = dict(collision=coll, pair=_copy_from_device(ptrs[1].value, K, np.int32))
+        for i, name in ((2, "depth"),):
+            out[name] = _copy_from_device(ptrs[i].value, K, np.float64)
+        for i, name in ((3, "normal"), (4, "cpA"), (5, "cpB"), (6, "Fn"), (7, "tauA"), (8, "tauB")):
+            out[name] = _copy_from_device(ptrs[i].value, 3 * K, np.float64).reshape(K, 3)
+        out["epa_status"] = _copy_from_device(ptrs[9].value, K, np.int32)
+        return out
+
+    def numpy_state(self) -> dict:
+        torch.cuda.synchronize(self.device)
+        return {k: self.t[k].cpu().numpy().copy() for k in STATE_KEYS}
+
+    def scene_rows(self, s: int) -> slice:
+        return slice(int(self.row0[s]), int(self.row0[s]) + self.scenes[s].n_rows)
+
+    def set_state(self, st: dict):
+        """Overwrite state tensors from numpy arrays (rows of all scenes)."""
+        for k, a in st.items():
+            self.t[k].copy_(torch.from_numpy(np.ascontiguousarray(a)).to(self.device))
+
+
+# ------------------------------------------------------------ raw copies
+_HIP = None
+
+
+def _hip():
+    """The HIP runtime already loaded by torch (matched by its soname, so no
+    second runtime is mapped into the process)."""
+    global _HIP
+    if _HIP is None:
+        import ctypes
+        _HIP = ctypes.CDLL("libamdhip64.so.7")
+        _HIP.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        _HIP.hipMemcpy.restype = ctypes.c_int
+    return _HIP
+
+
+def _check_hip(rc):
+    if rc != 0:
+        raise RuntimeError(f"hipMemcpy failed ({rc})")
+
+
+def _copy_from_device(addr: int, n: int, dtype) -> np.ndarray:
+    out = np.empty(n, dtype=dtype)
+    if n:
+        torch.cuda.synchronize()
+        import ctypes
+        _check_hip(_hip().hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(addr),
+                                    ctypes.c_size_t(out.nbytes), 2))  # hipMemcpyDeviceToHost
+    return out

@@ -1,0 +1,869 @@
+// TEST INFRASTRUCTURE ONLY -- the oracle.
+//
+// A CPU, FP64 restatement of the reference's per-timestep DEM path for convex
+// polyhedra. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+// leg may load this library, and only as the checker or the CPU baseline; the
+// product (mwdiem_amd/) never calls it.
+//
+// What it restates (file:line of the reference, kgillemot/MWDiEM):
+//   cell keys ........... MWDiem.py:813-816
+//   wall slab test ...... MWDiem.py:807-811
+//   Linked_cell_hash .... LIBRARIES/LIBRARY_Coarse_contact_detection.py:84-155
+//                         (CPython dict insertion order and CPython set
+//                          iteration order for int members are emulated)
+//   Support_funtion ..... LIBRARIES/gpu_fine_contact_detection.cu:55-101
+//   Check_linesegment ... gpu_fine_contact_detection.cu:103-124
+//   Check_triangle ...... gpu_fine_contact_detection.cu:127-186
+//   Check_Tetrahedron ... gpu_fine_contact_detection.cu:188-254
+//   CorrectFlat ......... gpu_fine_contact_detection.cu:256-268
+//   AddTriangle ......... gpu_fine_contact_detection.cu:270-325
+//   EPA ................. gpu_fine_contact_detection.cu:328-603
+//   Force ............... gpu_fine_contact_detection.cu:647-871
+//   contact_detection ... gpu_fine_contact_detection.cu:874-1322
+//   Verlet .............. LIBRARIES/gpu_Verlet.cu:85-404
+//   force assembly ...... MWDiem.py:676-677,977-981
+//
+// Deliberate, documented differences from the reference text:
+//   * FP64 everywhere (the reference contact path is FP32, MWDiem.py:43).
+//   * EPA face storage is EPA_MAX_FACES (the reference's T/n/dist/valid[50]
+//     overflow is undefined behaviour, .cu:359-373); on overflow the expansion
+//     stops and status bit 2 is raised.
+//   * The per-particle force sum runs in ascending pair order (the reference
+//     sums with FP32 atomics in an unspecified order, .cu:1304-1319).
+//   * displacement_tang is indexed [4P+k] (the reference's [P+k], .cu:974-976,
+//     races between neighbouring pairs; the driver always passes zeros).
+//
+// Compiled with -ffp-contract=off so every expression rounds as written.
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <unordered_map>
+
+#define ORC_EXPORT extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+const int EPA_MAX_FACES = 128;
+const int EPA_MAX_VERTS = 32;
+
+// ---------------------------------------------------------------- flop counter
+// CD is a double that counts every floating-point add/sub/mul/div/sqrt it
+// executes (FMA is never formed: the library is built with -ffp-contract=off).
+static uint64_t g_flops = 0;
+struct CD {
+    double v;
+    CD() : v(0) {}
+    CD(double x) : v(x) {}
+};
+inline CD operator+(CD a, CD b) { ++g_flops; return CD(a.v + b.v); }
+inline CD operator-(CD a, CD b) { ++g_flops; return CD(a.v - b.v); }
+inline CD operator*(CD a, CD b) { ++g_flops; return CD(a.v * b.v); }
+inline CD operator/(CD a, CD b) { ++g_flops; return CD(a.v / b.v); }
+inline CD operator-(CD a) { return CD(-a.v); }
+inline bool operator<(CD a, CD b) { return a.v < b.v; }
+inline bool operator>(CD a, CD b) { return a.v > b.v; }
+inline bool operator<=(CD a, CD b) { return a.v <= b.v; }
+inline bool operator>=(CD a, CD b) { return a.v >= b.v; }
+inline bool operator==(CD a, CD b) { return a.v == b.v; }
+inline bool operator!=(CD a, CD b) { return a.v != b.v; }
+inline CD sqrt_(CD a) { ++g_flops; return CD(std::sqrt(a.v)); }
+inline double sqrt_(double a) { return std::sqrt(a); }
+inline CD fabs_(CD a) { return CD(std::fabs(a.v)); }
+inline double fabs_(double a) { return std::fabs(a); }
+inline double val(double a) { return a; }
+inline double val(CD a) { return a.v; }
+
+template <typename T> struct V3 { T x, y, z; };
+template <typename T> inline V3<T> mk(T x, T y, T z) { V3<T> r; r.x = x; r.y = y; r.z = z; return r; }
+template <typename T> inline V3<T> operator+(V3<T> a, V3<T> b) { return mk<T>(a.x + b.x, a.y + b.y, a.z + b.z); }
+template <typename T> inline V3<T> operator-(V3<T> a, V3<T> b) { return mk<T>(a.x - b.x, a.y - b.y, a.z - b.z); }
+template <typename T> inline V3<T> operator*(V3<T> a, T b) { return mk<T>(a.x * b, a.y * b, a.z * b); }
+template <typename T> inline V3<T> operator/(V3<T> a, T b) { return mk<T>(a.x / b, a.y / b, a.z / b); }
+template <typename T> inline V3<T> neg(V3<T> a) { return mk<T>(-a.x, -a.y, -a.z); }
+template <typename T> inline T dot(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <typename T> inline V3<T> cross(V3<T> a, V3<T> b) {
+    return mk<T>(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+template <typename T> inline T norm2(V3<T> a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+template <typename T> inline V3<T> load3(const double* p) { return mk<T>(T(p[0]), T(p[1]), T(p[2])); }
+template <typename T> inline bool is_zero(V3<T> a) { return a.x == T(0.0) && a.y == T(0.0) && a.z == T(0.0); }
+
+// ------------------------------------------------------------------ contact law
+// Pair-independent coefficients of Force (.cu:685-701), evaluated in the
+// reference's operator order.
+struct Law {
+    double Kn, mu, ln_en, ln_et, A_en, tc, Gn, Kt_num, inv_mu, dt;
+};
+
+Law make_law(double Kn, double en, double et, double mu, double dt) {
+    Law L;
+    const double pi = 3.141592653589793;
+    const double pi2 = pi * pi;
+    L.Kn = Kn;
+    L.mu = mu;
+    L.ln_en = std::log(en);
+    L.ln_et = std::log(et);
+    L.A_en = pi2 + L.ln_en * L.ln_en;
+    L.tc = std::sqrt((mu * L.A_en) / Kn);
+    L.Gn = ((-2.0 * mu) / L.tc) * L.ln_en;
+    L.Kt_num = ((Kn * (pi2 + L.ln_et * L.ln_et)) / L.A_en) * 1.0;
+    L.inv_mu = 1.0 / mu;
+    L.dt = dt;
+    return L;
+}
+
+// ------------------------------------------------------------------ quaternions
+template <typename T> struct Q4 { T w, x, y, z; };
+template <typename T> inline Q4<T> qmk(T w, V3<T> v) { Q4<T> q; q.w = w; q.x = v.x; q.y = v.y; q.z = v.z; return q; }
+template <typename T> inline V3<T> qvec(Q4<T> q) { return mk<T>(q.x, q.y, q.z); }
+template <typename T> inline Q4<T> qinv(Q4<T> q) { Q4<T> r; r.w = q.w; r.x = -q.x; r.y = -q.y; r.z = -q.z; return r; }
+// Quat_mult, gpu_Verlet.cu:43-54 / .cu:613-624
+template <typename T> inline Q4<T> qmul(Q4<T> a, Q4<T> b) {
+    V3<T> v1 = qvec(a), v2 = qvec(b);
+    V3<T> s = (v2 * a.w + v1 * b.w) + cross(v1, v2);
+    T f = a.w * b.w - dot(v1, v2);
+    return qmk(f, s);
+}
+// Quat_tripl, gpu_Verlet.cu:57-74: vector part of q1 (0,v) q2
+template <typename T> inline V3<T> qtripl(Q4<T> q1, V3<T> v, Q4<T> q2) {
+    Q4<T> vq = qmk(T(0.0), v);
+    return qvec(qmul(q1, qmul(vq, q2)));
+}
+template <typename T> inline Q4<T> qload(const double* p) { Q4<T> q; q.w = T(p[0]); q.x = T(p[1]); q.y = T(p[2]); q.z = T(p[3]); return q; }
+
+// ------------------------------------------------------------------ support map
+template <typename T> struct Body {
+    const double* v;  // [n][3]
+    int n;
+};
+
+// Support_funtion, .cu:55-101: first index of the strict maximum.
+template <typename T>
+inline V3<T> support(const Body<T>& A, const Body<T>& B, V3<T> d, int* ia, int* ib) {
+    T best = T(-1e30);
+    int i1 = 0;
+    for (int i = 0; i < A.n; ++i) {
+        T t = dot(d, load3<T>(A.v + 3 * i));
+        if (t > best) { best = t; i1 = i; }
+    }
+    V3<T> nd = neg(d);
+    T best2 = T(-1e30);
+    int i2 = 0;
+    for (int i = 0; i < B.n; ++i) {
+        T t = dot(nd, load3<T>(B.v + 3 * i));
+        if (t > best2) { best2 = t; i2 = i; }
+    }
+    *ia = i1;
+    *ib = i2;
+    return load3<T>(A.v + 3 * i1) - load3<T>(B.v + 3 * i2);
+}
+
+// Check_linesegment, .cu:103-124
+template <typename T>
+inline void check_segment(V3<T> P1, V3<T> P2, int* coll, V3<T>* d) {
+    *coll = -1;
+    *d = mk<T>(T(0.0), T(0.0), T(0.0));
+    V3<T> V21 = P2 - P1;
+    if (dot(V21, P2) < T(0.0)) { *coll = 0; return; }
+    V3<T> t = cross(V21, P2);
+    if (is_zero(t)) { *coll = 1; return; }
+    V3<T> Vt = cross(V21, neg(P2));
+    *d = cross(Vt, V21);
+}
+
+// Check_triangle, .cu:127-186
+template <typename T>
+inline void check_triangle(V3<T> P1, V3<T> P2, V3<T> P3, int* coll, V3<T>* d) {
+    *coll = -1;
+    *d = mk<T>(T(0.0), T(0.0), T(0.0));
+    int c2, c3;
+    check_segment(P2, P3, &c2, d);
+    if (c2 == 1) { *coll = 1; return; }
+    check_segment(P1, P3, &c3, d);
+    if (c3 == 1) { *coll = 1; return; }
+    if (c2 + c3 > -1) { *coll = 0; return; }
+    V3<T> V32 = P3 - P2;
+    V3<T> V13 = P1 - P3;
+    V3<T> n = cross(V32, V13);
+    V3<T> n2 = cross(V32, n);
+    V3<T> n3 = cross(V13, n);
+    if (dot(n2, P2) < T(0.0)) { *d = n2; return; }
+    if (dot(n3, P3) < T(0.0)) { *d = n3; return; }
+    if (dot(P3, n) == T(0.0)) { *coll = 1; return; }
+    if (dot(n, P3) > T(0.0)) *d = neg(n); else *d = n;
+}
+
+// Check_Tetrahedron, .cu:188-254
+template <typename T>
+inline void check_tetra(V3<T> P1, V3<T> P2, V3<T> P3, V3<T> P4, int* coll, V3<T>* d, int* todel) {
+    *coll = -1;
+    *d = mk<T>(T(0.0), T(0.0), T(0.0));
+    *todel = 0;
+    int c1, c2, c3;
+    check_triangle(P1, P2, P4, &c1, d);
+    if (c1 == 1) { *coll = 1; return; }
+    check_triangle(P2, P3, P4, &c2, d);
+    if (c2 == 1) { *coll = 1; return; }
+    check_triangle(P3, P1, P4, &c3, d);
+    if (c3 == 1) { *coll = 1; return; }
+    if (c1 + c2 + c3 > -2) { *coll = 0; return; }
+    V3<T> V13 = P1 - P3, V21 = P2 - P1, V32 = P3 - P2, V41 = P4 - P1;
+    V3<T> n2 = cross(V21, V41);
+    V3<T> V42 = P4 - P2;
+    V3<T> n3 = cross(V32, V42);
+    V3<T> V43 = P4 - P3;
+    V3<T> n4 = cross(V13, V43);
+    if (dot(n2, P4) < T(0.0)) { *d = n2; *todel = 3; }
+    else if (dot(n3, P4) < T(0.0)) { *d = n3; *todel = 1; }
+    else if (dot(n4, P4) < T(0.0)) { *d = n4; *todel = 2; }
+    else *coll = 1;
+}
+
+// CorrectFlat, .cu:256-268
+template <typename T>
+inline int correct_flat(V3<T> S1, V3<T> S2, V3<T> S3, V3<T> S4) {
+    T det1 = S1.x * (S2.y * S3.z - S3.y * S2.z) - S2.x * (S1.y * S3.z - S3.y * S1.z) + S3.x * (S1.y * S2.z - S2.y * S1.z);
+    if (det1 < T(0.000001) && det1 > T(-0.000001)) {
+        T det2 = S1.x * (S2.y * S4.z - S4.y * S2.z) - S2.x * (S1.y * S4.z - S4.y * S1.z) + S4.x * (S1.y * S2.z - S2.y * S1.z);
+        if (det2 < T(0.000001) && det2 > T(-0.000001)) return 1;
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------------ EPA
+template <typename T> struct Poly {
+    V3<T> V[EPA_MAX_VERTS];
+    int opA[EPA_MAX_VERTS], opB[EPA_MAX_VERTS];
+    int Tr[EPA_MAX_FACES][3];
+    V3<T> n[EPA_MAX_FACES];
+    T dist[EPA_MAX_FACES];
+    int valid[EPA_MAX_FACES];
+};
+
+// AddTriangle, .cu:270-325
+template <typename T>
+inline void add_triangle(Poly<T>& P, int e0, int e1, int e2, int id) {
+    P.Tr[id][0] = e0; P.Tr[id][1] = e1; P.Tr[id][2] = e2;
+    V3<T> nn = cross(P.V[e1] - P.V[e0], P.V[e2] - P.V[e1]);
+    T len = sqrt_(norm2(nn));
+    nn = nn / len;
+    T dd = dot(nn, P.V[e0]);
+    P.valid[id] = 1;
+    if (dd < T(0.0)) {
+        int o = P.Tr[id][2]; P.Tr[id][2] = P.Tr[id][1]; P.Tr[id][1] = o;
+        dd = dd * T(-1.0);
+        nn = nn * T(-1.0);
+    }
+    const V3<T>& a = P.V[P.Tr[id][0]];
+    const V3<T>& b = P.V[P.Tr[id][1]];
+    const V3<T>& c = P.V[P.Tr[id][2]];
+    bool inside = dot(cross(b - a, nn), neg(a)) < T(0.0) &&
+                  dot(cross(c - b, nn), neg(b)) < T(0.0) &&
+                  dot(cross(a - c, nn), neg(c)) < T(0.0);
+    if (!inside) { P.valid[id] = -2; dd = T(1e30); }
+    P.n[id] = nn;
+    P.dist[id] = dd;
+}
+
+struct EpaOut { double depth; double n[3]; double cpA[3]; double cpB[3]; int status; int faces; int iters; };
+
+// EPA, .cu:328-603. status bit 1: expansion cap reached (.cu:443-447);
+// bit 2: face storage exhausted (reference: undefined behaviour).
+template <typename T>
+void epa(const V3<T> S[4], const int ia[4], const int ib[4], const Body<T>& A, const Body<T>& B,
+         T* depth, V3<T>* cpA, V3<T>* cpB, V3<T>* normal, int* status, int* faces_used, int* iters) {
+    Poly<T> P;
+    T xmin_old = T(1e29);
+    T proj = T(2.0) * xmin_old + T(1.0);
+    int counter = 0, nv = 4, nt = 0, indmin = 0;
+    T xmin_new = T(0.0);
+    int st = 0;
+    for (int k = 0; k < 4; ++k) { P.V[k] = S[k]; P.opA[k] = ia[k]; P.opB[k] = ib[k]; }
+    add_triangle(P, 0, 1, 2, 0);
+    add_triangle(P, 1, 2, 3, 1);
+    add_triangle(P, 2, 0, 3, 2);
+    add_triangle(P, 0, 1, 3, 3);
+    nt = 4;
+    int it = 0;
+    while (xmin_old < proj) {
+        ++it;
+        indmin = 0;
+        T xm = P.dist[0];
+        for (int k = 0; k < nt; ++k) if (P.dist[k] < xm) { xm = P.dist[k]; indmin = k; }
+        xmin_new = P.dist[indmin];
+        xmin_old = xmin_new;
+        int a, b;
+        V3<T> Suj = support(A, B, P.n[indmin], &a, &b);
+        proj = dot(P.n[indmin], Suj);
+        P.opA[nv] = a;
+        P.opB[nv] = b;
+        if (xmin_old > proj || (xmin_old * T(1.00001) > proj && xmin_old * T(0.99999) < proj)) break;
+        if (counter > 49) { st |= 1; break; }
+        counter = counter + 1;
+        P.V[nv] = Suj;
+        nv = nv + 1;
+        for (int i = 0; i < nt; ++i)
+            if ((P.valid[i] == 1 || P.valid[i] == -2) && dot(P.V[nv - 1] - P.V[P.Tr[i][0]], P.n[i]) >= T(0.0))
+                P.valid[i] = -1;
+        // delpair (.cu:470-493): an edge of a visible face is on the horizon iff
+        // no other visible face contains the same undirected edge.
+        bool horizon[EPA_MAX_FACES][3];
+        for (int i = 0; i < nt; ++i) {
+            if (P.valid[i] != -1) continue;
+            for (int e = 0; e < 3; ++e) {
+                int u = P.Tr[i][e], v = P.Tr[i][(e + 1) % 3];
+                int cnt = 0;
+                for (int j = 0; j < nt; ++j) {
+                    if (P.valid[j] != -1) continue;
+                    for (int f = 0; f < 3; ++f) {
+                        int x = P.Tr[j][f], y = P.Tr[j][(f + 1) % 3];
+                        if ((x == u && y == v) || (x == v && y == u)) ++cnt;
+                    }
+                }
+                horizon[i][e] = cnt < 2;
+            }
+        }
+        bool overflow = false;
+        int nt0 = nt;
+        for (int i = 0; i < nt0 && !overflow; ++i) {
+            if (P.valid[i] != -1) continue;
+            for (int e = 0; e < 3; ++e) {
+                if (!horizon[i][e]) continue;
+                if (nt == EPA_MAX_FACES) { overflow = true; break; }
+                add_triangle(P, P.Tr[i][e], P.Tr[i][(e + 1) % 3], nv - 1, nt);
+                nt = nt + 1;
+            }
+            if (overflow) break;
+            P.valid[i] = 0;
+            P.dist[i] = T(3e20);
+        }
+        if (overflow) { st |= 2; break; }
+        counter = counter + 1;
+    }
+    *depth = xmin_new;
+    int a = P.Tr[indmin][0], b = P.Tr[indmin][1], c = P.Tr[indmin][2];
+    V3<T> nrm = P.n[indmin];
+    V3<T> Proj = nrm * (*depth);
+    V3<T> Aa = P.V[a], Bb = P.V[b], Cc = P.V[c];
+    V3<T> v0 = Bb - Aa, v1 = Cc - Aa, v2 = Proj - Aa;
+    if (dot(cross(v0, v1), nrm) < T(0.0)) { v0 = Cc - Aa; v1 = Bb - Aa; }
+    T v00 = dot(v0, v0), v01 = dot(v0, v1), v11 = dot(v1, v1), v20 = dot(v2, v0), v21 = dot(v2, v1);
+    T denom = v00 * v11 - v01 * v01;
+    T vv = (v11 * v20 - v01 * v21) / denom;
+    T ww = (v00 * v21 - v01 * v20) / denom;
+    T uu = T(1.0) - ww - vv;
+    *cpA = (load3<T>(A.v + 3 * P.opA[a]) * uu + load3<T>(A.v + 3 * P.opA[b]) * vv) + load3<T>(A.v + 3 * P.opA[c]) * ww;
+    *cpB = (load3<T>(B.v + 3 * P.opB[a]) * uu + load3<T>(B.v + 3 * P.opB[b]) * vv) + load3<T>(B.v + 3 * P.opB[c]) * ww;
+    *normal = nrm;
+    *status = st;
+    *faces_used = nt;
+    *iters = it;
+}
+
+// ----------------------------------------------------------------------- Force
+template <typename T> struct PairIn {
+    V3<T> comA, comB, vA, vB, wA, wB, dispt;
+    Q4<T> qA;
+    T isA, isB;
+    int wall;
+};
+template <typename T> struct ForceOut { V3<T> Fn, tA, tB, disp; };
+
+// Force, .cu:647-871 (Q1, Q3, Q4, Q5 reproduced).
+template <typename T>
+ForceOut<T> force_law(const PairIn<T>& in, T depth, V3<T> n, V3<T> cpA, V3<T> cpB, const Law& L) {
+    ForceOut<T> o;
+    V3<T> cp = (cpA - cpB) / T(2.0);
+    V3<T> dA = cp - in.comA, dB = cp - in.comB;
+    T dAn = norm2(dA), dBn = norm2(dB);
+    T C = (T(L.inv_mu) + dAn / in.isA) + dBn / in.isB;
+    T Kt = T(L.Kt_num) / C;
+    T Gt = (T(-2.0) / C) * T(L.ln_et);
+    T Gn = T(L.Gn);
+    if (dot(in.comB - in.comA, n) < T(0.0)) n = n * T(-1.0);
+    T nn = sqrt_(norm2(n));
+    n = n / nn;
+    V3<T> disp_n = n * depth;
+    V3<T> u = ((in.vA + cross(in.wA, cpA - in.comA)) - in.vB) - cross(in.wB, cpB - in.comB);
+    V3<T> un = n * dot(u, n);
+    V3<T> ut = u - un;
+    T utn = sqrt_(norm2(ut));
+    T dtn = sqrt_(norm2(in.dispt));
+    V3<T> t = mk<T>(T(0.0), T(0.0), T(0.0));
+    if (utn != T(0.0)) t = ut / utn;
+    if (utn == T(0.0) && dtn != T(0.0)) t = in.dispt / dtn;
+    V3<T> Fn = (disp_n * T(-1.0)) * T(L.Kn) - un * Gn;
+    T Fnn = sqrt_(norm2(Fn));
+    V3<T> Ft = (in.dispt * T(-1.0)) * Kt - ut * Gt;
+    T Ftn = sqrt_(norm2(Ft));
+    T muF = T(L.mu) * Fnn;
+    V3<T> dnew = mk<T>(T(0.0), T(0.0), T(0.0));
+    if (Ftn > muF) Ft = ((t * Fnn) * T(-1.0)) * T(L.mu);
+    if (Ftn <= muF) {
+        V3<T> r = qtripl(in.qA, in.dispt, qinv(in.qA));
+        dnew = r + ut * T(L.dt);
+    } else {
+        dnew = ((((t * Fnn) * T(L.mu)) * T(-1.0)) - ut * Gt) / Kt;
+    }
+    V3<T> Ff = Ft + Fn;
+    o.Fn = Fn;
+    o.tA = cross(cp - in.comA, Ff);
+    o.tB = cross(cp - in.comB, Ff * T(-1.0));
+    o.disp = dnew;
+    return o;
+}
+
+// ------------------------------------------------------- contact_detection
+struct PairResult {
+    int collision;
+    int epa_status;
+    int faces;
+    int iters;
+    double depth, n[3], cpA[3], cpB[3], Fn[3], tA[3], tB[3], disp[4];
+};
+
+// contact_detection, .cu:874-1322, for one pair.
+template <typename T>
+PairResult contact_one(const Body<T>& A, const Body<T>& B, const PairIn<T>& in, const Law& L) {
+    PairResult r;
+    std::memset(&r, 0, sizeof(r));
+    r.collision = -1;
+    V3<T> S[4];
+    int ia[4] = {0, 0, 0, 0}, ib[4] = {0, 0, 0, 0};
+    V3<T> d1 = in.comB - in.comA;
+    S[0] = support(A, B, d1, &ia[0], &ib[0]);
+    if (dot(d1, S[0]) < T(0.0)) { r.collision = 0; return r; }
+    V3<T> d2 = neg(d1);
+    S[1] = support(A, B, d2, &ia[1], &ib[1]);
+    if (dot(d2, S[1]) < T(0.0)) { r.collision = 0; return r; }
+    int coll;
+    V3<T> d3;
+    check_segment(S[0], S[1], &coll, &d3);
+    if (coll == 0) { r.collision = 0; return r; }
+    S[2] = support(A, B, d3, &ia[2], &ib[2]);
+    if (dot(d3, S[2]) < T(0.0)) { r.collision = 0; return r; }
+    V3<T> d4;
+    check_triangle(S[0], S[1], S[2], &coll, &d4);
+    if (coll == 0) { r.collision = 0; return r; }
+    for (int zz = 0; zz < 50; ++zz) {
+        S[3] = support(A, B, d4, &ia[3], &ib[3]);
+        if (dot(d4, S[3]) < T(0.0)) { r.collision = 0; return r; }
+        bool hit = false;
+        if (is_zero(S[3])) {
+            hit = true;
+        } else {
+            int todel;
+            check_tetra(S[0], S[1], S[2], S[3], &coll, &d4, &todel);
+            if (coll == 0) { r.collision = 0; return r; }
+            if (coll == 1) hit = true;
+            else {
+                V3<T> n1, n2, n3;
+                int a1, b1, a2, b2, a3, b3;
+                if (todel == 1) { n1 = S[1]; a1 = ia[1]; b1 = ib[1]; n2 = S[2]; a2 = ia[2]; b2 = ib[2]; }
+                else if (todel == 2) { n1 = S[2]; a1 = ia[2]; b1 = ib[2]; n2 = S[0]; a2 = ia[0]; b2 = ib[0]; }
+                else { n1 = S[0]; a1 = ia[0]; b1 = ib[0]; n2 = S[1]; a2 = ia[1]; b2 = ib[1]; }
+                n3 = S[3]; a3 = ia[3]; b3 = ib[3];
+                S[0] = n1; ia[0] = a1; ib[0] = b1;
+                S[1] = n2; ia[1] = a2; ib[1] = b2;
+                S[2] = n3; ia[2] = a3; ib[2] = b3;
+            }
+        }
+        if (hit) {
+            if (correct_flat(S[0], S[1], S[2], S[3])) {
+                d4 = cross(S[0], S[1]);
+                S[3] = support(A, B, d4, &ia[3], &ib[3]);
+            }
+            r.collision = 1;
+            T depth;
+            V3<T> cpA, cpB, nrm;
+            epa(S, ia, ib, A, B, &depth, &cpA, &cpB, &nrm, &r.epa_status, &r.faces, &r.iters);
+            ForceOut<T> f = force_law(in, depth, nrm, cpA, cpB, L);
+            r.depth = val(depth);
+            r.n[0] = val(nrm.x); r.n[1] = val(nrm.y); r.n[2] = val(nrm.z);
+            r.cpA[0] = val(cpA.x); r.cpA[1] = val(cpA.y); r.cpA[2] = val(cpA.z);
+            r.cpB[0] = val(cpB.x); r.cpB[1] = val(cpB.y); r.cpB[2] = val(cpB.z);
+            r.Fn[0] = val(f.Fn.x); r.Fn[1] = val(f.Fn.y); r.Fn[2] = val(f.Fn.z);
+            r.tA[0] = val(f.tA.x); r.tA[1] = val(f.tA.y); r.tA[2] = val(f.tA.z);
+            r.tB[0] = val(f.tB.x); r.tB[1] = val(f.tB.y); r.tB[2] = val(f.tB.z);
+            r.disp[0] = val(f.disp.x); r.disp[1] = val(f.disp.y); r.disp[2] = val(f.disp.z); r.disp[3] = 1.0;
+            return r;
+        }
+    }
+    return r;  // tetrahedron cap exhausted: collision stays -1, no force (.cu:1104,1287)
+}
+
+// ----------------------------------------------------------------------- Verlet
+struct StateView {
+    double *v, *v12, *w12bf, *q12, *q, *qall, *wbf, *w;
+    const double *F, *tau;
+    double *verts;
+    double *com;
+    const double *mass, *Iinv, *I;
+    double *dxall;
+};
+
+// Verlet, gpu_Verlet.cu:85-404, for particle P.
+void verlet_one(const StateView& s, int64_t P, int vmax, int nrot, double dt, const double* box) {
+    typedef double T;
+    V3<T> velocity = load3<T>(s.v + 3 * P);
+    V3<T> v12 = load3<T>(s.v12 + 3 * P);
+    V3<T> w12bf = load3<T>(s.w12bf + 3 * P);
+    V3<T> wbf = load3<T>(s.wbf + 3 * P);
+    V3<T> w = load3<T>(s.w + 3 * P);
+    V3<T> com = load3<T>(s.com + 3 * P);
+    V3<T> F = load3<T>(s.F + 3 * P);
+    V3<T> tau = load3<T>(s.tau + 3 * P);
+    double m = s.mass[P];
+    const double* Ii = s.Iinv + 9 * P;
+    const double* I = s.I + 9 * P;
+    Q4<T> q = qload<T>(s.q + 4 * P), q12 = qload<T>(s.q12 + 4 * P);
+    V3<T> dxall = load3<T>(s.dxall + 3 * P);
+
+    V3<T> v32 = v12 + (F / m) * dt;
+    Q4<T> qi = qinv(q);
+    V3<T> tbf = qtripl(qi, tau, q);
+    V3<T> p1 = mk(I[0] * wbf.x + I[1] * wbf.y + I[2] * wbf.z,
+                  I[3] * wbf.x + I[4] * wbf.y + I[5] * wbf.z,
+                  I[6] * wbf.x + I[7] * wbf.y + I[8] * wbf.z);
+    V3<T> p2 = tbf - cross(wbf, p1);
+    V3<T> abf = mk(Ii[0] * p2.x + Ii[1] * p2.y + Ii[2] * p2.z,
+                   Ii[3] * p2.x + Ii[4] * p2.y + Ii[5] * p2.z,
+                   Ii[6] * p2.x + Ii[7] * p2.y + Ii[8] * p2.z);
+    V3<T> w32bf = w12bf + abf * dt;
+    T wn = std::sqrt(norm2(w));
+    V3<T> t11 = mk(0.0, 0.0, 0.0);
+    if (wn != 0) t11 = (w / wn) * std::sin((wn * dt) / 2);
+    T t112 = std::cos((wn * dt) / 2);
+    Q4<T> q32 = qmul(qmk(t112, t11), q12);
+    v12 = v32;
+    q12 = q32;
+    w12bf = w32bf;
+    V3<T> dx = v12 * dt;
+    dxall = dxall + dx;
+    V3<T> w34bf = w12bf + (abf * dt) / 4.0;
+    V3<T> w34 = qtripl(q12, w34bf, qinv(q12));
+    T n34 = std::sqrt(norm2(w34));
+    V3<T> t1 = mk(0.0, 0.0, 0.0);
+    if (n34 != 0) t1 = (w34 / n34) * std::sin((n34 * dt) / 4);
+    T t12 = std::cos((n34 * dt) / 4);
+    Q4<T> qold = q;
+    q = qmul(qmk(t12, t1), q12);
+    Q4<T> qall = qmul(q, qold);
+    wbf = w12bf + (abf * dt) / 2.0;
+    w = qtripl(q, wbf, qinv(q));
+    velocity = v12 + (F * dt) / m;
+    Q4<T> qinv2 = qinv(q);
+    double* vp = s.verts + (int64_t)vmax * 3 * P;
+    for (int i = 0; i < nrot; ++i) {
+        V3<T> r = qtripl(q, load3<T>(vp + 3 * i) - com, qinv2);
+        V3<T> nvx = r + com;
+        vp[3 * i] = nvx.x; vp[3 * i + 1] = nvx.y; vp[3 * i + 2] = nvx.z;
+    }
+    com = com + dx;
+    for (int i = 0; i < nrot; ++i) {
+        vp[3 * i] = vp[3 * i] + dx.x; vp[3 * i + 1] = vp[3 * i + 1] + dx.y; vp[3 * i + 2] = vp[3 * i + 2] + dx.z;
+    }
+    // box order: x_max, x_min, y_max, y_min, z_max, z_min (gpu_Verlet.cu:338)
+    if (com.x > box[0] || com.x < box[1] || com.y > box[2] || com.y < box[3] || com.z > box[4] || com.z < box[5]) {
+        velocity = mk(0.0, 0.0, 0.0);
+        v12 = mk(0.0, 0.0, 0.0);
+        w12bf = mk(0.0, 0.0, 0.0);
+        wbf = mk(0.0, 0.0, 0.0);
+        w = mk(0.0, 0.0, 0.0);
+        V3<T> du = dx * -1.0;
+        com = com + du;
+        for (int i = 0; i < nrot; ++i) {
+            vp[3 * i] = vp[3 * i] + du.x; vp[3 * i + 1] = vp[3 * i + 1] + du.y; vp[3 * i + 2] = vp[3 * i + 2] + du.z;
+        }
+    }
+    auto st3 = [&](double* p, V3<T> a) { p[3 * P] = a.x; p[3 * P + 1] = a.y; p[3 * P + 2] = a.z; };
+    auto st4 = [&](double* p, Q4<T> a) { p[4 * P] = a.w; p[4 * P + 1] = a.x; p[4 * P + 2] = a.y; p[4 * P + 3] = a.z; };
+    st3(s.v, velocity); st3(s.v12, v12); st3(s.w12bf, w12bf); st3(s.com, com); st3(s.dxall, dxall);
+    st3(s.wbf, wbf); st3(s.w, w);
+    st4(s.q12, q12); st4(s.q, q); st4(s.qall, qall);
+}
+
+// ------------------------------------------------------------ CPython set order
+// Iteration order of a CPython (3.7+) set of non-negative ints filled in the
+// given order: open addressing with 9 linear probes then perturbation,
+// growth to 4*used when fill*5 >= mask*3 (Objects/setobject.c).
+void probe_insert(std::vector<int64_t>& tab, size_t mask, int64_t key) {
+    size_t h = (size_t)key;
+    size_t perturb = h;
+    size_t i = h & mask;
+    for (;;) {
+        if (tab[i] < 0) { tab[i] = key; return; }
+        if (i + 9 <= mask) {
+            for (size_t j = 1; j <= 9; ++j)
+                if (tab[i + j] < 0) { tab[i + j] = key; return; }
+        }
+        perturb >>= 5;
+        i = (i * 5 + 1 + perturb) & mask;
+    }
+}
+
+std::vector<int64_t> set_order(const std::vector<int64_t>& keys) {
+    size_t size = 8;
+    std::vector<int64_t> tab(size, -1);
+    size_t fill = 0;
+    for (int64_t k : keys) {
+        probe_insert(tab, size - 1, k);
+        ++fill;
+        if (fill * 5 >= (size - 1) * 3) {
+            size_t minused = fill > 50000 ? fill * 2 : fill * 4;
+            size_t ns = 8;
+            while (ns <= minused) ns <<= 1;
+            std::vector<int64_t> nt(ns, -1);
+            for (int64_t e : tab) if (e >= 0) probe_insert(nt, ns - 1, e);
+            tab.swap(nt);
+            size = ns;
+        }
+    }
+    std::vector<int64_t> out;
+    for (int64_t e : tab) if (e >= 0) out.push_back(e);
+    return out;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+ORC_EXPORT int orc_epa_max_faces() { return EPA_MAX_FACES; }
+
+ORC_EXPORT void orc_set_order(const int64_t* ids, int64_t n, int64_t* out) {
+    std::vector<int64_t> k(ids, ids + n);
+    std::vector<int64_t> o = set_order(k);
+    for (size_t i = 0; i < o.size(); ++i) out[i] = o[i];
+}
+
+// Cell keys, MWDiem.py:813-816: truncation toward zero, no origin offset.
+ORC_EXPORT void orc_cell_keys(const double* com, int64_t n, double csx, double csy, double csz,
+                              int64_t ncx, int64_t ncy, int64_t* keys) {
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t xc = (int64_t)(com[3 * i] / csx);
+        int64_t yc = (int64_t)(com[3 * i + 1] / csy);
+        int64_t zc = (int64_t)(com[3 * i + 2] / csz);
+        keys[i] = xc + yc * ncx + zc * ncx * ncy;
+    }
+}
+
+// Wall slab test, MWDiem.py:807-811: walls k in [0,W) outer, particles j in
+// [W,n) inner, pair [j,k] when |n_k . (x_j - c_k)| < thr. Returns the count;
+// writes at most cap pairs.
+ORC_EXPORT int64_t orc_wall_pairs(const double* com, int64_t W, int64_t n, const double* wall_n,
+                                  const double* wall_c, double thr, int32_t* pairs, int64_t cap) {
+    int64_t cnt = 0;
+    for (int64_t k = 0; k < W; ++k) {
+        for (int64_t j = W; j < n; ++j) {
+            double d0 = com[3 * j] - wall_c[3 * k], d1 = com[3 * j + 1] - wall_c[3 * k + 1], d2 = com[3 * j + 2] - wall_c[3 * k + 2];
+            double dd = wall_n[3 * k] * d0 + wall_n[3 * k + 1] * d1 + wall_n[3 * k + 2] * d2;
+            if (std::fabs(dd) < thr) {
+                if (cnt < cap) { pairs[2 * cnt] = (int32_t)j; pairs[2 * cnt + 1] = (int32_t)k; }
+                ++cnt;
+            }
+        }
+    }
+    return cnt;
+}
+
+// Linked_cell_hash, LIBRARY_Coarse_contact_detection.py:84-155, over ids
+// [pmin,pmax) with keys[i]. Returns the count; writes at most cap pairs.
+ORC_EXPORT int64_t orc_linked_cell_hash(const int64_t* keys, int64_t pmin, int64_t pmax, int64_t ncx,
+                                        int64_t ncy, int32_t* pairs, int64_t cap) {
+    std::unordered_map<int64_t, size_t> where;
+    std::vector<int64_t> order;
+    std::vector<std::vector<int64_t>> members;
+    for (int64_t i = pmin; i < pmax; ++i) {
+        auto it = where.find(keys[i]);
+        if (it == where.end()) {
+            where[keys[i]] = members.size();
+            order.push_back(keys[i]);
+            members.push_back(std::vector<int64_t>(1, i));
+        } else {
+            members[it->second].push_back(i);
+        }
+    }
+    for (auto& m : members) m = set_order(m);
+    const int64_t offs[7] = {1, ncx, ncx * ncy, ncx * ncy + 1, ncx * ncy + 1 + ncx, ncx * ncy + ncx, ncx + 1};
+    int64_t cnt = 0;
+    auto emit = [&](int64_t a, int64_t b) {
+        if (cnt < cap) { pairs[2 * cnt] = (int32_t)a; pairs[2 * cnt + 1] = (int32_t)b; }
+        ++cnt;
+    };
+    for (size_t c = 0; c < order.size(); ++c) {
+        const std::vector<int64_t>& L = members[c];
+        for (size_t j = 0; j < L.size(); ++j)
+            for (size_t k = j + 1; k < L.size(); ++k) emit(L[j], L[k]);
+        for (int o = 0; o < 7; ++o) {
+            auto it = where.find(order[c] + offs[o]);
+            if (it == where.end()) continue;
+            const std::vector<int64_t>& M = members[it->second];
+            for (size_t j = 0; j < L.size(); ++j)
+                for (size_t k = 0; k < M.size(); ++k) emit(L[j], M[k]);
+        }
+    }
+    return cnt;
+}
+
+// Law coefficients as the product computes them (for cross-checks).
+ORC_EXPORT void orc_law(double Kn, double en, double et, double mu, double dt, double* out10) {
+    Law L = make_law(Kn, en, et, mu, dt);
+    double v[10] = {L.Kn, L.mu, L.ln_en, L.ln_et, L.A_en, L.tc, L.Gn, L.Kt_num, L.inv_mu, L.dt};
+    for (int i = 0; i < 10; ++i) out10[i] = v[i];
+}
+
+// Per-pair fine contact in the reference's own input layout
+// (gpu_fine_contact_detection.py:76-119 with FP64 arrays):
+//   comA/comB/vA/vB/wA/wB [P,3], vertsA/vertsB [P,vmax,3], nA/nB/idA/idB [P],
+//   dispt [P,4], qA/qB [P,4], isA/isB [P].
+// Outputs per pair: collision, epa status, depth, normal, cpA, cpB, Fn, tauA,
+// tauB, disp [P,4]. Per-particle force/torque [N,3] are accumulated in
+// ascending pair order onto their current contents.
+ORC_EXPORT void orc_contact_pairs(int64_t P, int vmax,
+                                  const double* comA, const double* comB,
+                                  const double* vertsA, const double* vertsB,
+                                  const double* vA, const double* vB, const double* wA, const double* wB,
+                                  const int32_t* nA, const int32_t* nB, const int32_t* idA, const int32_t* idB,
+                                  const double* dispt, const double* qA, const double* qB,
+                                  const double* isA, const double* isB,
+                                  const double* law_p, const double* law_w, int64_t W, double dt,
+                                  int32_t* collision, int32_t* epa_status, double* depth, double* normal,
+                                  double* cpA, double* cpB, double* Fn, double* tauA, double* tauB,
+                                  double* disp_out, double* force, double* torque, int32_t* faces_used) {
+    Law Lp = make_law(law_p[0], law_p[1], law_p[2], law_p[3], dt);
+    Law Lw = make_law(law_w[0], law_w[1], law_w[2], law_w[3], dt);
+    (void)qB;
+    std::unordered_map<int64_t, std::vector<double>> sums;
+    for (int64_t p = 0; p < P; ++p) {
+        Body<double> A{vertsA + (int64_t)vmax * 3 * p, nA[p]};
+        Body<double> B{vertsB + (int64_t)vmax * 3 * p, nB[p]};
+        PairIn<double> in;
+        in.comA = load3<double>(comA + 3 * p); in.comB = load3<double>(comB + 3 * p);
+        in.vA = load3<double>(vA + 3 * p); in.vB = load3<double>(vB + 3 * p);
+        in.wA = load3<double>(wA + 3 * p); in.wB = load3<double>(wB + 3 * p);
+        in.dispt = load3<double>(dispt + 4 * p);
+        in.qA = qload<double>(qA + 4 * p);
+        in.isA = isA[p]; in.isB = isB[p];
+        in.wall = (idA[p] < W || idB[p] < W) ? 1 : 0;
+        PairResult r = contact_one(A, B, in, in.wall ? Lw : Lp);
+        collision[p] = r.collision;
+        if (epa_status) epa_status[p] = r.epa_status;
+        if (faces_used) faces_used[p] = r.faces;
+        if (depth) depth[p] = r.depth;
+        for (int k = 0; k < 3; ++k) {
+            if (normal) normal[3 * p + k] = r.n[k];
+            if (cpA) cpA[3 * p + k] = r.cpA[k];
+            if (cpB) cpB[3 * p + k] = r.cpB[k];
+            if (Fn) Fn[3 * p + k] = r.Fn[k];
+            if (tauA) tauA[3 * p + k] = r.tA[k];
+            if (tauB) tauB[3 * p + k] = r.tB[k];
+        }
+        if (disp_out) for (int k = 0; k < 4; ++k) disp_out[4 * p + k] = r.disp[k];
+        if (r.collision == 1 && force) {
+            int a = idA[p], b = idB[p];
+            std::vector<double>& sa = sums[a];
+            std::vector<double>& sb = sums[b];
+            if (sa.empty()) sa.assign(6, 0.0);
+            if (sb.empty()) sb.assign(6, 0.0);
+            for (int k = 0; k < 3; ++k) {
+                sa[k] = sa[k] + r.Fn[k];
+                sb[k] = sb[k] + (-r.Fn[k]);
+                sa[3 + k] = sa[3 + k] + r.tA[k];
+                sb[3 + k] = sb[3 + k] + r.tB[k];
+            }
+        }
+    }
+    // contact sums (from zero, ascending pair order) are added once to the
+    // caller's force/torque rows.
+    if (force)
+        for (auto& kv : sums)
+            for (int k = 0; k < 3; ++k) {
+                force[3 * kv.first + k] = force[3 * kv.first + k] + kv.second[k];
+                torque[3 * kv.first + k] = torque[3 * kv.first + k] + kv.second[3 + k];
+            }
+}
+
+// FP64 flop count of the fine-contact stage (GJK + EPA + force law) over a
+// pair list indexed into per-particle state (the device-resident layout).
+ORC_EXPORT uint64_t orc_count_flops(int64_t P, const int32_t* pa, const int32_t* pb, int vmax,
+                                    const double* com, const double* verts, const int32_t* nverts,
+                                    const double* vel, const double* angvel, const double* q,
+                                    const double* isize, const double* law_p, const double* law_w,
+                                    int64_t W, double dt) {
+    Law Lp = make_law(law_p[0], law_p[1], law_p[2], law_p[3], dt);
+    Law Lw = make_law(law_w[0], law_w[1], law_w[2], law_w[3], dt);
+    g_flops = 0;
+    for (int64_t p = 0; p < P; ++p) {
+        int a = pa[p], b = pb[p];
+        Body<CD> A{verts + (int64_t)vmax * 3 * a, nverts[a]};
+        Body<CD> B{verts + (int64_t)vmax * 3 * b, nverts[b]};
+        PairIn<CD> in;
+        in.comA = load3<CD>(com + 3 * a); in.comB = load3<CD>(com + 3 * b);
+        in.vA = load3<CD>(vel + 3 * a); in.vB = load3<CD>(vel + 3 * b);
+        in.wA = load3<CD>(angvel + 3 * a); in.wB = load3<CD>(angvel + 3 * b);
+        in.dispt = mk<CD>(CD(0.0), CD(0.0), CD(0.0));
+        in.qA = qload<CD>(q + 4 * a);
+        in.isA = CD(isize[a]); in.isB = CD(isize[b]);
+        in.wall = (a < W || b < W) ? 1 : 0;
+        contact_one(A, B, in, in.wall ? Lw : Lp);
+    }
+    return g_flops;
+}
+
+// Verlet over rows [pmin,pmax) of the reference's arrays (gpu_Verlet.py:53-92).
+// nrot: vertex slots rotated per particle (vmax for the reference behaviour).
+ORC_EXPORT void orc_verlet(double* v, double* v12, double* w12bf, double* q12, double* q, double* qall,
+                           double* wbf, double* w, const double* F, const double* tau, double* verts,
+                           double* com, const double* mass, const double* Iinv, const double* I,
+                           double* dxall, int vmax, const int32_t* nrot, double dt, const double* box,
+                           int64_t pmin, int64_t pmax) {
+    StateView s{v, v12, w12bf, q12, q, qall, wbf, w, F, tau, verts, com, mass, Iinv, I, dxall};
+    for (int64_t P = pmin; P < pmax; ++P) verlet_one(s, P, vmax, nrot ? nrot[P] : vmax, dt, box);
+}
+
+// Per-particle contact sums (ascending pair order) for a pair list indexed into
+// per-particle state; writes collision codes and returns the contact count.
+ORC_EXPORT int64_t orc_contact_state(int64_t P, const int32_t* pa, const int32_t* pb, int vmax,
+                                     const double* com, const double* verts, const int32_t* nverts,
+                                     const double* vel, const double* angvel, const double* q,
+                                     const double* isize, const double* law_p, const double* law_w,
+                                     int64_t W, double dt, int32_t* collision, double* depth,
+                                     double* normal, double* cpA, double* cpB, double* force,
+                                     double* torque, int32_t* epa_status) {
+    Law Lp = make_law(law_p[0], law_p[1], law_p[2], law_p[3], dt);
+    Law Lw = make_law(law_w[0], law_w[1], law_w[2], law_w[3], dt);
+    int64_t K = 0;
+    for (int64_t p = 0; p < P; ++p) {
+        int a = pa[p], b = pb[p];
+        Body<double> A{verts + (int64_t)vmax * 3 * a, nverts[a]};
+        Body<double> B{verts + (int64_t)vmax * 3 * b, nverts[b]};
+        PairIn<double> in;
+        in.comA = load3<double>(com + 3 * a); in.comB = load3<double>(com + 3 * b);
+        in.vA = load3<double>(vel + 3 * a); in.vB = load3<double>(vel + 3 * b);
+        in.wA = load3<double>(angvel + 3 * a); in.wB = load3<double>(angvel + 3 * b);
+        in.dispt = mk<double>(0.0, 0.0, 0.0);
+        in.qA = qload<double>(q + 4 * a);
+        in.isA = isize[a]; in.isB = isize[b];
+        in.wall = (a < W || b < W) ? 1 : 0;
+        PairResult r = contact_one(A, B, in, in.wall ? Lw : Lp);
+        collision[p] = r.collision;
+        if (epa_status) epa_status[p] = r.epa_status;
+        if (depth) depth[p] = r.depth;
+        for (int k = 0; k < 3; ++k) {
+            if (normal) normal[3 * p + k] = r.n[k];
+            if (cpA) cpA[3 * p + k] = r.cpA[k];
+            if (cpB) cpB[3 * p + k] = r.cpB[k];
+        }
+        if (r.collision == 1) {
+            ++K;
+            for (int k = 0; k < 3; ++k) {
+                force[3 * a + k] = force[3 * a + k] + r.Fn[k];
+                force[3 * b + k] = force[3 * b + k] + (-r.Fn[k]);
+                torque[3 * a + k] = torque[3 * a + k] + r.tA[k];
+                torque[3 * b + k] = torque[3 * b + k] + r.tB[k];
+            }
+        }
+    }
+    return K;
+}

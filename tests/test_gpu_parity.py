@@ -1,0 +1,229 @@
+"""Parity of the HIP path (through the C ABI) with the oracle, on the GPU.
+
+Bars (from the project's parity contract):
+  * candidate pair lists and collision codes: identical (order included);
+  * penetration depth, normal, contact points: within 1e-9 relative;
+  * per-step integrator outputs: within 1e-12 relative (the only differences
+    allowed are the last-ulp differences of sin/cos between the device math
+    library and glibc);
+  * positions and quaternions after 100 full steps: within 1e-8 relative.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import max_rel, random_motion
+
+pytestmark = pytest.mark.gpu
+
+
+def _stepper(sc, **kw):
+    from mwdiem_amd.stepper import Stepper
+    return Stepper(sc, **kw)
+
+
+def test_native_library_is_loaded_once(gpu):
+    with open("/proc/self/maps") as f:
+        maps = f.read()
+    assert "libmwdiem_hip.so" in maps
+    hip = {ln.split()[-1] for ln in maps.splitlines() if "libamdhip64" in ln}
+    assert len(hip) == 1, f"more than one HIP runtime mapped: {hip}"
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_verlet_dropin_one_step(gpu, small_scene, seed):
+    from mwdiem_amd.dropin import gpu_Verlet
+    sc = small_scene
+    W, N = sc.n_walls, sc.n_rows
+    st = random_motion(O.initial_state(sc), sc, seed)
+    rng = np.random.Generator(np.random.PCG64(100 + seed))
+    F = rng.normal(size=(N, 3)) * 1e-3
+    T = rng.normal(size=(N, 3)) * 1e-6
+    ref = {k: v.copy() for k, v in st.items()}
+    O.verlet(ref, F, T, sc, W, N)
+    got = {k: v.copy() for k, v in st.items()}
+    out = gpu_Verlet.Verlet_GPU(got["v"], got["v12"], got["w12bf"], got["q12"], got["q"], got["qall"], got["wbf"],
+                                got["w"], F, T, got["verts"], got["com"], sc.dt, sc.mass.reshape(-1, 1), N, sc.Iinv,
+                                sc.I, got["dxall"], sc.vmax, *sc.box, W, N)
+    # reference return order, gpu_Verlet.py:89-92, and in-place update
+    order = ("v12", "q", "wbf", "w12bf", "w", "q12", "v", "com", "verts", "qall", "dxall")
+    for name, arr in zip(order, out):
+        assert arr is got[name]
+    for k in order:
+        err = max_rel(got[k][W:], ref[k][W:])
+        assert err <= 1e-12, (k, err)
+    # walls untouched
+    for k in order:
+        assert np.array_equal(got[k][:W], st[k][:W]), k
+
+
+def test_verlet_100_steps(gpu, small_scene):
+    from mwdiem_amd.dropin import gpu_Verlet
+    sc = small_scene
+    W, N = sc.n_walls, sc.n_rows
+    st = random_motion(O.initial_state(sc), sc, 7)
+    rng = np.random.Generator(np.random.PCG64(7))
+    F = rng.normal(size=(N, 3)) * 1e-3
+    T = rng.normal(size=(N, 3)) * 1e-6
+    ref = {k: v.copy() for k, v in st.items()}
+    got = {k: v.copy() for k, v in st.items()}
+    for _ in range(100):
+        O.verlet(ref, F, T, sc, W, N)
+        gpu_Verlet.Verlet_GPU(got["v"], got["v12"], got["w12bf"], got["q12"], got["q"], got["qall"], got["wbf"],
+                              got["w"], F, T, got["verts"], got["com"], sc.dt, sc.mass.reshape(-1, 1), N, sc.Iinv,
+                              sc.I, got["dxall"], sc.vmax, *sc.box, W, N)
+    for k in ("com", "q", "qall", "v", "w"):
+        err = max_rel(got[k][W:], ref[k][W:])
+        assert err <= 1e-8, (k, err)
+
+
+@pytest.mark.parametrize("which", ["small", "dense"])
+def test_candidates_identical(gpu, small_scene, dense_scene, which):
+    sc = small_scene if which == "small" else dense_scene
+    S = _stepper(sc)
+    S.candidates()
+    s = S.check()
+    got = S.pairs()
+    ref = O.candidates(O.initial_state(sc), sc)
+    assert got.shape == ref.shape
+    assert np.array_equal(got, ref)
+    nw = O.wall_pairs(sc.com, sc.n_walls, sc.n_rows, sc.wall_n, sc.wall_c, sc.wall_thr).shape[0]
+    assert s["n_wall_pairs"] == nw
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_fine_contact_matches_oracle(gpu, dense_scene, seed):
+    sc = dense_scene
+    S = _stepper(sc)
+    st = random_motion(O.initial_state(sc), sc, seed)
+    S.set_state(st)
+    S.candidates()
+    S.fine_contact()
+    S.check()
+    pairs = S.pairs()
+    c = S.contacts()
+    coll, depth, nrm, ca, cb, F, T, est = O.contact_state(pairs, st, sc)
+    assert np.array_equal(c["collision"], coll)
+    idx = np.nonzero(coll == 1)[0]
+    assert idx.size > 20
+    assert np.array_equal(c["pair"], idx)
+    assert max_rel(c["depth"], depth[idx]) <= 1e-9
+    for name, ref in (("normal", nrm), ("cpA", ca), ("cpB", cb)):
+        assert max_rel(c[name], ref[idx]) <= 1e-9, name
+    assert np.array_equal(c["epa_status"], est[idx])
+
+
+def test_step_reduction_matches_oracle(gpu, dense_scene):
+    """One full step: the fused reduction + integrator sees the same forces."""
+    sc = dense_scene
+    S = _stepper(sc)
+    st = random_motion(O.initial_state(sc), sc, 5)
+    S.set_state(st)
+    ref = {k: v.copy() for k, v in st.items()}
+    O.step(ref, sc)
+    S.step(1)
+    S.check()
+    got = S.numpy_state()
+    W = sc.n_walls
+    for k in ("com", "v", "v12", "w", "wbf", "w12bf", "q", "q12", "qall", "dxall"):
+        assert max_rel(got[k][W:], ref[k][W:]) <= 1e-12, k
+
+
+def test_100_steps_match_oracle(gpu, dense_scene):
+    sc = dense_scene
+    S = _stepper(sc)
+    st = O.initial_state(sc)
+    W = sc.n_walls
+    for i in range(100):
+        pairs_ref, _ = O.step(st, sc)
+        S.step(1)
+        if i < 5:
+            assert np.array_equal(S.pairs(), pairs_ref), i
+    S.check()
+    got = S.numpy_state()
+    for k in ("com", "q", "qall"):
+        assert max_rel(got[k][W:], st[k][W:]) <= 1e-8, k
+    nv = sc.nverts
+    for r in range(W, sc.n_rows):
+        assert max_rel(got["verts"][r, :nv[r]], st["verts"][r, :nv[r]], scale=1.0) <= 1e-8
+
+
+def test_contact_dropin_matches_oracle(gpu, dense_scene):
+    """Contact_detection_GPU with the driver's FP32 per-pair arrays."""
+    from mwdiem_amd.dropin import gpu_fine_contact_detection as FC
+    sc = dense_scene
+    st = random_motion(O.initial_state(sc), sc, 9)
+    pairs = O.candidates(st, sc)
+    P, vmax, N, W = pairs.shape[0], sc.vmax, sc.n_rows, sc.n_walls
+    a, b = pairs[:, 0], pairs[:, 1]
+    f32 = np.float32
+    vertsA = st["verts"][a].astype(f32)
+    vertsB = st["verts"][b].astype(f32)
+    for k in range(P):  # zero padding as the driver gathers (MWDiem.py:850-859)
+        vertsA[k, sc.nverts[a[k]]:] = 0
+        vertsB[k, sc.nverts[b[k]]:] = 0
+    args = dict(comA=st["com"][a].astype(f32), comB=st["com"][b].astype(f32), vA=st["v"][a].astype(f32),
+                vB=st["v"][b].astype(f32), wA=st["w"][a].astype(f32), wB=st["w"][b].astype(f32),
+                nA=sc.nverts[a].astype(np.int32), nB=sc.nverts[b].astype(np.int32), idA=a.astype(np.int32),
+                idB=b.astype(np.int32), disp=np.zeros((P, 4), f32), qA=st["q"][a].astype(f32),
+                qB=st["q"][b].astype(f32), isA=sc.isize[a].astype(f32), isB=sc.isize[b].astype(f32))
+    force = np.zeros((N, 3), f32)
+    torque = np.zeros((N, 3), f32)
+    L, Lw = sc.law_p, sc.law_w
+    out = FC.Contact_detection_GPU(args["comA"], args["comB"], vertsA, vertsB, args["vA"], args["vB"], args["wA"],
+                                   args["wB"], args["nA"], args["nB"], args["idA"], args["idB"], args["disp"], L.Kn,
+                                   L.en, L.et, L.mu, Lw.Kn, Lw.en, Lw.et, Lw.mu, W, force, torque, vmax, args["qA"],
+                                   args["qB"], sc.dt, args["isA"], args["isB"])
+    fo, to, disp, coll = out
+    assert fo is force and to is torque and disp is args["disp"]
+    Fr = np.zeros((N, 3))
+    Tr = np.zeros((N, 3))
+    ref = O.contact_pairs(args["comA"], args["comB"], vertsA, vertsB, args["vA"], args["vB"], args["wA"], args["wB"],
+                          args["nA"], args["nB"], args["idA"], args["idB"], np.zeros((P, 4)), args["qA"], args["qB"],
+                          args["isA"], args["isB"], [L.Kn, L.en, L.et, L.mu], [Lw.Kn, Lw.en, Lw.et, Lw.mu], W,
+                          sc.dt, Fr, Tr)
+    assert np.array_equal(coll, ref["collision"])
+    assert (coll == 1).sum() > 20
+    assert np.array_equal(force, Fr.astype(f32))
+    assert np.array_equal(torque, Tr.astype(f32))
+    assert np.array_equal(disp, ref["disp"].astype(f32))
+
+
+def test_hash_dropin_matches_reference_order(gpu, dense_scene):
+    from mwdiem_amd.dropin import LIBRARY_Coarse_contact_detection as CC
+    sc = dense_scene
+    keys = O.cell_keys(sc.com, sc.cell, sc.ncx, sc.ncy)
+    start = [[1, 2]]
+    out = CC.Linked_cell_hash(None, sc.ncx, sc.ncy, sc.n_walls, sc.n_rows, start, keys.reshape(-1, 1))
+    assert out is start
+    ref = O.linked_cell_hash_py(keys, sc.n_walls, sc.n_rows, sc.ncx, sc.ncy)
+    assert out[1:] == ref
+
+
+def test_batched_equals_separate(gpu, small_scene, dense_scene):
+    """Several scenes in one Stepper: each scene's pairs (in order) and state
+    are bit-identical to a Stepper of that scene alone."""
+    from mwdiem_amd import scene
+    from mwdiem_amd.stepper import Stepper
+    sc3 = scene.compact_dense(scene.make_scene(40, (8.0, 8.0, 20.0), 12, law=scene.Law4(Kn=3e4, en=0.3, et=0.4,
+                                                                                        mu=0.5)), gap=-0.5)
+    scs = [dense_scene, small_scene, sc3]
+    B = Stepper(scs)
+    singles = [Stepper(s) for s in scs]
+    for _ in range(3):
+        B.step(1)
+        for s in singles:
+            s.step(1)
+    B.check()
+    pb = B.pairs()
+    for i, s in enumerate(singles):
+        s.check()
+        rows = B.scene_rows(i)
+        sel = (pb[:, 0] >= rows.start) & (pb[:, 0] < rows.stop)
+        assert np.array_equal(pb[sel] - rows.start, s.pairs()), i
+        gb = B.numpy_state()
+        gs = s.numpy_state()
+        for k in ("com", "q", "v", "w", "verts"):
+            assert np.array_equal(gb[k][rows], gs[k]), (i, k)
