@@ -110,7 +110,8 @@ int mwd_law_init(mwd_law* out, double Kn, double en, double et, double mu, doubl
 
 /* Workspace owning every scratch buffer of the step: sort buffers, cell
  * tables, the candidate pair list (pair_cap pairs), contacts (contact_cap).
- * Scenes are copied in. Calls on distinct workspaces are independent. */
+ * Scenes are copied in. Calls on distinct workspaces are independent.
+ * Limits: vmax <= 255, n_rows < 2^31. */
 int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scene* scenes, int32_t n_scenes,
                          int64_t n_rows, int32_t vmax, int64_t pair_cap, int64_t contact_cap, double dt,
                          void* stream);
@@ -130,6 +131,7 @@ int mwd_verlet(double* v, double* v12, double* w12bf, double* q12, double* q, do
  *   dispt [P,4], qA [P,4], isA/isB [P].
  * Outputs (each may be NULL): collision [P] (-1 undecided, 0 apart, 1
  * contact), epa_status [P], depth [P], normal/cpA/cpB [P,3], disp [P,4].
+ * Every idA/idB must lie in [0, n_particles); vmax <= 255.
  * force/torque [n_particles,3] receive, per particle, the contact sum taken in
  * ascending pair order (Fn on A, -Fn on B, tauA on A, tauB on B), added once
  * to their current contents. Synchronises `stream` before returning. */

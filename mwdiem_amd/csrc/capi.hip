@@ -179,7 +179,8 @@ MWD_API int mwd_law_init(mwd_law* out, double Kn, double en, double et, double m
 MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scene* scenes, int32_t n_scenes,
                                  int64_t n_rows, int32_t vmax, int64_t pair_cap, int64_t contact_cap, double dt,
                                  void* stream) {
-    if (!out || !scenes || n_scenes <= 0 || n_rows <= 0 || vmax <= 0 || pair_cap <= 0 || contact_cap <= 0) {
+    if (!out || !scenes || n_scenes <= 0 || n_rows <= 0 || vmax <= 0 || vmax > 255 || pair_cap <= 0 ||
+        contact_cap <= 0 || n_rows > INT32_MAX || pair_cap > INT32_MAX / 2 || contact_cap > INT32_MAX / 4) {
         mwd_set_error("mwd_workspace_create: invalid argument");
         return MWD_E_INVALID;
     }
@@ -493,7 +494,7 @@ MWD_API int mwd_contact_pairs(int64_t P, int32_t vmax, const double* comA, const
                               int32_t* collision, int32_t* epa_status, double* depth, double* normal, double* cpA,
                               double* cpB, double* disp, double* force, double* torque, int64_t n_particles,
                               void* stream) {
-    if (P < 0 || vmax <= 0 || !law_p || !law_w || n_particles < 0) {
+    if (P < 0 || P > INT32_MAX / 4 || vmax <= 0 || vmax > 255 || !law_p || !law_w || n_particles < 0) {
         mwd_set_error("mwd_contact_pairs: invalid argument");
         return MWD_E_INVALID;
     }

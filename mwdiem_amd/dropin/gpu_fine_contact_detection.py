@@ -66,6 +66,8 @@ def Contact_detection_GPU(particle_A_center_of_mass, particle_B_center_of_mass,
     F = to_dev(force, _F64, dev)
     T = to_dev(torque, _F64, dev)
     n_particles = F.shape[0]
+    if P and (int(torch.minimum(idA.min(), idB.min())) < 0 or int(torch.maximum(idA.max(), idB.max())) >= n_particles):
+        raise ValueError("particle ids must index the force/torque rows")
     collision = torch.full((P,), -1, dtype=_I32, device=dev)
     disp_out = torch.zeros((P, 4), dtype=_F64, device=dev)
     lp = _lib.law(Kn, en, et, mu, timestep_size)
