@@ -57,9 +57,20 @@ extern "C" {
 #define MWD_MAX_CELL 76            /* largest cell occupancy whose set order is emulated */
 
 /* Contact-law coefficients of Force (.cu:685-701), filled by mwd_law_init in
- * the reference's operator order. */
+ * the reference's operator order.
+ * mode MWD_LAW_REF: the reference law as written (contact point (cpA-cpB)/2,
+ *   mu in place of a mass in the damping time, only the normal force on the
+ *   linear momentum).
+ * mode MWD_LAW_PHYSICAL: contact point (cpA+cpB)/2, normal damping from the
+ *   effective mass 1/m_n = 1/mA + 1/mB, tangential stiffness and damping from
+ *   1/m_t = 1/m_n + rA^2/IA + rB^2/IB (walls infinitely heavy), normal plus
+ *   tangential force on the linear momentum. Needs per-row masses (stepper). */
+#define MWD_LAW_REF 0
+#define MWD_LAW_PHYSICAL 1
 typedef struct mwd_law {
     double Kn, mu, ln_en, ln_et, A_en, tc, Gn, Kt_num, inv_mu, dt;
+    double cn, ct, ratio;   /* -2 ln(en)/sqrt(pi^2+ln^2 en), same for et, (pi^2+ln^2 et)/(pi^2+ln^2 en) */
+    int32_t mode, pad_;
 } mwd_law;
 
 /* One scene (one independent bed with its walls, grid and law). Rows
@@ -105,8 +116,10 @@ typedef struct mwd_status {
 const char* mwd_last_error(void);
 int mwd_version(void);
 
-/* Coefficients of the normal/tangential law for (Kn, en, et, mu) and dt. */
+/* Coefficients of the normal/tangential law for (Kn, en, et, mu) and dt
+ * (mode MWD_LAW_REF); mwd_law_init_mode selects the mode. */
 int mwd_law_init(mwd_law* out, double Kn, double en, double et, double mu, double dt);
+int mwd_law_init_mode(mwd_law* out, double Kn, double en, double et, double mu, double dt, int32_t mode);
 
 /* Workspace owning every scratch buffer of the step: sort buffers, cell
  * tables, the candidate pair list (pair_cap pairs), contacts (contact_cap).

@@ -36,8 +36,13 @@ class MwdError(RuntimeError):
         self.code = code
 
 
+LAW_REF = 0
+LAW_PHYSICAL = 1
+
+
 class Law(C.Structure):
-    _fields_ = [(n, C.c_double) for n in ("Kn", "mu", "ln_en", "ln_et", "A_en", "tc", "Gn", "Kt_num", "inv_mu", "dt")]
+    _fields_ = [(n, C.c_double) for n in ("Kn", "mu", "ln_en", "ln_et", "A_en", "tc", "Gn", "Kt_num", "inv_mu", "dt",
+                                          "cn", "ct", "ratio")] + [("mode", C.c_int32), ("pad_", C.c_int32)]
 
 
 class Scene(C.Structure):
@@ -72,6 +77,8 @@ _SIGS = {
     "mwd_last_error": (C.c_char_p, []),
     "mwd_version": (C.c_int, []),
     "mwd_law_init": (C.c_int, [C.POINTER(Law), C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]),
+    "mwd_law_init_mode": (C.c_int, [C.POINTER(Law), C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                    C.c_int32]),
     "mwd_workspace_create": (C.c_int, [C.POINTER(_P), C.c_int, C.POINTER(Scene), C.c_int32, C.c_int64, C.c_int32,
                                        C.c_int64, C.c_int64, C.c_double, _P]),
     "mwd_workspace_destroy": (C.c_int, [_P]),
@@ -124,9 +131,9 @@ def check(rc: int, where: str) -> None:
         raise MwdError(rc, where, msg.decode() if msg else "")
 
 
-def law(Kn: float, en: float, et: float, mu: float, dt: float) -> Law:
+def law(Kn: float, en: float, et: float, mu: float, dt: float, mode: int = LAW_REF) -> Law:
     L = Law()
-    check(load().mwd_law_init(C.byref(L), Kn, en, et, mu, dt), "mwd_law_init")
+    check(load().mwd_law_init_mode(C.byref(L), Kn, en, et, mu, dt, int(mode)), "mwd_law_init_mode")
     return L
 
 

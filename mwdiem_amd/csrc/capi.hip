@@ -67,6 +67,7 @@ BodyView state_view(const mwd_workspace* ws, const mwd_state* st, int side) {
     b.angvel = st->w;
     b.quat = st->q;
     b.isize = st->isize;
+    b.mass = st->mass;
     b.is_wall = ws->is_wall;
     b.scene = ws->scene_of;
     b.rows = ws->pairs + side;
@@ -155,8 +156,9 @@ MWD_API int mwd_stage_times(mwd_workspace* ws, double* ms, int64_t* steps) {
 MWD_API const char* mwd_last_error(void) { return g_err.c_str(); }
 MWD_API int mwd_version(void) { return 1; }
 
-MWD_API int mwd_law_init(mwd_law* out, double Kn, double en, double et, double mu, double dt) {
-    if (!out || !(Kn > 0) || !(en > 0) || !(et > 0) || !(mu > 0)) {
+MWD_API int mwd_law_init_mode(mwd_law* out, double Kn, double en, double et, double mu, double dt,
+                              int32_t mode) {
+    if (!out || !(Kn > 0) || !(en > 0) || !(et > 0) || !(mu > 0) || (mode != MWD_LAW_REF && mode != MWD_LAW_PHYSICAL)) {
         mwd_set_error("mwd_law_init: Kn, en, et and mu must be positive");
         return MWD_E_INVALID;
     }
@@ -173,7 +175,16 @@ MWD_API int mwd_law_init(mwd_law* out, double Kn, double en, double et, double m
     out->Kt_num = ((Kn * (pi2 + out->ln_et * out->ln_et)) / out->A_en) * 1.0;
     out->inv_mu = 1.0 / mu;
     out->dt = dt;
+    out->cn = (-2.0 * out->ln_en) / sqrt(out->A_en);
+    out->ct = (-2.0 * out->ln_et) / sqrt(pi2 + out->ln_et * out->ln_et);
+    out->ratio = (pi2 + out->ln_et * out->ln_et) / out->A_en;
+    out->mode = mode;
+    out->pad_ = 0;
     return MWD_OK;
+}
+
+MWD_API int mwd_law_init(mwd_law* out, double Kn, double en, double et, double mu, double dt) {
+    return mwd_law_init_mode(out, Kn, en, et, mu, dt, MWD_LAW_REF);
 }
 
 MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scene* scenes, int32_t n_scenes,
@@ -235,6 +246,22 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
         d.law_w = sc.law_w;
         laws[2 * s] = sc.law_p;
         laws[2 * s + 1] = sc.law_w;
+        {
+            const int64_t X = sc.ncx, XY = sc.ncx * sc.ncy;
+            if (sc.law_p.mode == MWD_LAW_PHYSICAL) {
+                // full forward half shell: (dx,dy,dz) with dz>0, or dz=0 and dy>0, or dz=dy=0 and dx>0
+                const int h[13][3] = {{1, 0, 0},  {-1, 1, 0}, {0, 1, 0},  {1, 1, 0},  {-1, -1, 1}, {0, -1, 1}, {1, -1, 1},
+                                      {-1, 0, 1}, {0, 0, 1},  {1, 0, 1},  {-1, 1, 1}, {0, 1, 1},   {1, 1, 1}};
+                d.n_off = 13;
+                for (int k = 0; k < 13; ++k) d.off[k] = h[k][0] + h[k][1] * X + h[k][2] * XY;
+            } else {
+                // LIBRARY_Coarse_contact_detection.py:101-129
+                const int64_t r[7] = {1, X, XY, XY + 1, XY + 1 + X, XY + X, X + 1};
+                d.n_off = 7;
+                for (int k = 0; k < 7; ++k) d.off[k] = r[k];
+                for (int k = 7; k < 13; ++k) d.off[k] = 0;
+            }
+        }
         kbase += sc.kspan;
         wt[s + 1] = wt[s] + sc.n_walls * (sc.n_active - sc.n_walls);
         for (int64_t r = sc.row0 + sc.n_walls; r < sc.row0 + sc.n_active; ++r) plist.push_back((int32_t)r);
@@ -275,7 +302,7 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     A(head_off, M);
     A(cell_start, M + 1);
     A(cell_key, M);
-    A(cell_nb, 7 * M);
+    A(cell_nb, MWD_MAX_OFF * M);
     A(row_first, n_rows);
     A(row_rank, n_rows);
     A(cell_by_rank, M);
@@ -577,8 +604,13 @@ MWD_API int mwd_contact_pairs(int64_t P, int32_t vmax, const double* comA, const
             rc = MWD_E_HIP;
             break;
         }
-        BodyView A{comA, vertsA, nA, vA, wA, qA, isA, wallA, nullptr, nullptr, 1, vmax};
-        BodyView B{comB, vertsB, nB, vB, wB, qA, isB, wallB, nullptr, nullptr, 1, vmax};
+        if (law_p->mode != MWD_LAW_REF || law_w->mode != MWD_LAW_REF) {
+            mwd_set_error("mwd_contact_pairs: the per-pair layout carries no masses; use MWD_LAW_REF");
+            rc = MWD_E_INVALID;
+            break;
+        }
+        BodyView A{comA, vertsA, nA, vA, wA, qA, isA, nullptr, wallA, nullptr, nullptr, 1, vmax};
+        BodyView B{comB, vertsB, nB, vB, wB, qA, isB, nullptr, wallB, nullptr, nullptr, 1, vmax};
         if ((rc = mwd_set_i64(ctr + 3, P, s))) break;
         if ((rc = mwd_launch_zero_pairs(P, depth, normal, cpA, cpB, epa_status, disp, s))) break;
         if ((rc = mwd_launch_gjk(A, B, P, ctr + 3, coll, sx, s))) break;

@@ -440,15 +440,32 @@ struct LawOut {
     d3 Fn, tA, tB, disp;
 };
 
-// Force, .cu:647-871
+// Force, .cu:647-871. Fn of the result is the force on A's linear momentum:
+// the normal force (reference law) or normal plus tangential (physical law).
+// imA/imB are inverse masses (0 for walls), read by the physical law only.
 __device__ LawOut force_law(d3 comA, d3 comB, d3 vA, d3 vB, d3 wA, d3 wB, d3 dispt, q4 qA, double isA,
-                            double isB, double depth, d3 n, d3 cpA, d3 cpB, const mwd_law& L) {
+                            double isB, double imA, double imB, double depth, d3 n, d3 cpA, d3 cpB,
+                            const mwd_law& L) {
     LawOut o;
-    const d3 cp = (cpA - cpB) / 2.0;
+    const bool phys = L.mode == MWD_LAW_PHYSICAL;
+    const d3 cp = phys ? (cpA + cpB) / 2.0 : (cpA - cpB) / 2.0;
     const d3 dA = cp - comA, dB = cp - comB;
-    const double C = (L.inv_mu + nrm2(dA) / isA) + nrm2(dB) / isB;
-    const double Kt = L.Kt_num / C;
-    const double Gt = (-2.0 / C) * L.ln_et;
+    const double dAn = nrm2(dA), dBn = nrm2(dB);
+    double Kt, Gt, Gn;
+    if (!phys) {
+        const double C = (L.inv_mu + dAn / isA) + dBn / isB;
+        Kt = L.Kt_num / C;
+        Gt = (-2.0 / C) * L.ln_et;
+        Gn = L.Gn;
+    } else {
+        const double imn = imA + imB;
+        const double mn = 1.0 / imn;
+        const double imt = (imn + dAn * (imA == 0.0 ? 0.0 : 1.0 / isA)) + dBn * (imB == 0.0 ? 0.0 : 1.0 / isB);
+        const double mt = 1.0 / imt;
+        Gn = L.cn * sqrt(mn * L.Kn);
+        Kt = (L.Kn * L.ratio) * (mt / mn);
+        Gt = L.ct * sqrt(mt * Kt);
+    }
     if (dot3(comB - comA, n) < 0.0) n = n * -1.0;
     n = n / sqrt(nrm2(n));
     const d3 disp_n = n * depth;
@@ -460,7 +477,7 @@ __device__ LawOut force_law(d3 comA, d3 comB, d3 vA, d3 vB, d3 wA, d3 wB, d3 dis
     d3 t = mk3(0.0, 0.0, 0.0);
     if (utn != 0.0) t = ut / utn;
     if (utn == 0.0 && dtn != 0.0) t = dispt / dtn;
-    const d3 Fn = (disp_n * -1.0) * L.Kn - un * L.Gn;
+    const d3 Fn = (disp_n * -1.0) * L.Kn - un * Gn;
     const double Fnn = sqrt(nrm2(Fn));
     d3 Ft = (dispt * -1.0) * Kt - ut * Gt;
     const double Ftn = sqrt(nrm2(Ft));
@@ -473,7 +490,7 @@ __device__ LawOut force_law(d3 comA, d3 comB, d3 vA, d3 vB, d3 wA, d3 wB, d3 dis
         dnew = ((((t * Fnn) * L.mu) * -1.0) - ut * Gt) / Kt;
     }
     const d3 Ff = Ft + Fn;
-    o.Fn = Fn;
+    o.Fn = phys ? Ff : Fn;
     o.tA = cross3(cp - comA, Ff);
     o.tB = cross3(cp - comB, Ff * -1.0);
     o.disp = dnew;
@@ -508,9 +525,11 @@ __global__ void __launch_bounds__(256) k_epa_force(BodyView A, BodyView B, const
         const int scene = A.scene ? A.scene[ra] : 0;
         const mwd_law& L = laws2[2 * scene + wall];
         const d3 dt_in = dispt ? ld3(dispt + 4 * p) : mk3(0.0, 0.0, 0.0);
+        const double imA = (A.mass && !A.is_wall[ra]) ? 1.0 / A.mass[ra] : 0.0;
+        const double imB = (B.mass && !B.is_wall[rb]) ? 1.0 / B.mass[rb] : 0.0;
         const LawOut f = force_law(ld3(A.com + 3 * ra), ld3(B.com + 3 * rb), ld3(A.vel + 3 * ra), ld3(B.vel + 3 * rb),
                                    ld3(A.angvel + 3 * ra), ld3(B.angvel + 3 * rb), dt_in, ldq(A.quat + 4 * ra),
-                                   A.isize[ra], B.isize[rb], e.depth, e.n, e.cpA, e.cpB, L);
+                                   A.isize[ra], B.isize[rb], imA, imB, e.depth, e.n, e.cpA, e.cpB, L);
         depth[c] = e.depth;
         st3(normal + 3 * c, e.n);
         st3(cpA + 3 * c, e.cpA);

@@ -12,6 +12,8 @@
 //     members of a cell in CPython set iteration order, same-cell pairs j<k,
 //     then the 7 forward offsets +1, +ncx, +ncx*ncy, +ncx*ncy+1,
 //     +ncx*ncy+1+ncx, +ncx*ncy+ncx, +ncx+1 with A in the lower cell.
+//     Scenes on the physical law use the full 13-cell forward half shell
+//     instead (the reference's 7 offsets miss 6 neighbour directions).
 //
 // Pipeline: keys -> stable radix sort of (key, row) -> cell heads (scan) ->
 // cell rank by lowest row (scan over rows) -> per-cell set order, neighbour
@@ -171,14 +173,14 @@ __global__ void k_cell_info(const int32_t* __restrict__ vals, const int32_t* __r
         set_order(vals + start, n, sc.row0, perm + start);
     }
     const int64_t lk = (int64_t)cell_key[c] - sc.kbase + sc.kmin;
-    const int64_t offs[7] = {1, sc.ncx, sc.ncx * sc.ncy, sc.ncx * sc.ncy + 1, sc.ncx * sc.ncy + 1 + sc.ncx,
-                             sc.ncx * sc.ncy + sc.ncx, sc.ncx + 1};
     int64_t cnt = (int64_t)n * (n - 1) / 2;
-    for (int o = 0; o < 7; ++o) {
-        const int64_t t = lk + offs[o] - sc.kmin;
+    for (int o = 0; o < MWD_MAX_OFF; ++o) {
         int64_t nb = -1;
-        if (t >= 0 && t < sc.kspan) nb = find_cell(cell_key, C, (uint32_t)(sc.kbase + t));
-        cell_nb[c * 7 + o] = (int32_t)nb;
+        if (o < sc.n_off) {
+            const int64_t t = lk + sc.off[o] - sc.kmin;
+            if (t >= 0 && t < sc.kspan) nb = find_cell(cell_key, C, (uint32_t)(sc.kbase + t));
+        }
+        cell_nb[c * MWD_MAX_OFF + o] = (int32_t)nb;
         if (nb >= 0) cnt += (int64_t)n * (cell_start[nb + 1] - cell_start[nb]);
     }
     const int64_t r = row_rank[row_min];
@@ -204,8 +206,8 @@ __global__ void k_cell_fill(const int32_t* __restrict__ perm, const int32_t* __r
                 pairs[2 * pos + 1] = perm[k];
             }
     }
-    for (int o = 0; o < 7; ++o) {
-        const int32_t nb = cell_nb[c * 7 + o];
+    for (int o = 0; o < MWD_MAX_OFF; ++o) {
+        const int32_t nb = cell_nb[c * MWD_MAX_OFF + o];
         if (nb < 0) continue;
         const int32_t t0 = cell_start[nb], t1 = cell_start[nb + 1];
         for (int32_t j = s0; j < s1; ++j) {

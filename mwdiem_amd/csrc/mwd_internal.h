@@ -42,7 +42,11 @@ struct SceneDev {
     int64_t plist_off;  // first entry of the scene in the moving-particle list
     double wall_thr;
     mwd_law law_p, law_w;
+    int32_t n_off;      // 7 (reference stencil) or 13 (full half shell)
+    int64_t off[13];    // key offsets of the neighbour cells, in emission order
 };
+
+#define MWD_MAX_OFF 13
 
 // Final GJK simplex of a colliding pair (input of EPA).
 struct Simplex {
@@ -60,6 +64,7 @@ struct BodyView {
     const double* angvel;
     const double* quat;
     const double* isize;
+    const double* mass;         // per row of the view (physical law only; may be NULL)
     const uint8_t* is_wall;     // per row of the view
     const int32_t* scene;       // per row of the view (NULL: scene 0)
     const int32_t* rows;        // NULL: identity
@@ -92,7 +97,7 @@ struct mwd_workspace {
     int64_t* head_off;         // [M]
     int32_t* cell_start;       // [M+1]
     uint32_t* cell_key;        // [M]
-    int32_t* cell_nb;          // [M*7]
+    int32_t* cell_nb;          // [M*MWD_MAX_OFF]
     int32_t* row_first;        // [n_rows]
     int64_t* row_rank;         // [n_rows]
     int32_t* cell_by_rank;     // [M]

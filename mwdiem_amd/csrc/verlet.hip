@@ -23,8 +23,10 @@ struct VerletArrays {
     double* dxall;
 };
 
+// rot_mode 0: vertices rotated by the absolute orientation q (the reference's
+// compounding rotation); 1: by the step's increment q q_old^-1.
 __device__ __forceinline__ void verlet_row(const VerletArrays& a, int64_t P, d3 F, d3 tau, int vmax, int nrot,
-                                           double dt, const double* box) {
+                                           double dt, const double* box, int rot_mode) {
     d3 v12 = ld3(a.v12 + 3 * P);
     d3 w12bf = ld3(a.w12bf + 3 * P);
     d3 wbf = ld3(a.wbf + 3 * P);
@@ -64,7 +66,8 @@ __device__ __forceinline__ void verlet_row(const VerletArrays& a, int64_t P, d3 
     wbf = w12bf + (abf * dt) / 2.0;
     w = qrot(q, wbf, qconj(q));
     d3 vel = v12 + (F * dt) / m;
-    const q4 qc = qconj(q);
+    const q4 qr = rot_mode == 1 ? qmul(q, qconj(qold)) : q;
+    const q4 qc = qconj(qr);
     double* vp = a.verts + (int64_t)vmax * 3 * P;
     const bool out = [&] {
         d3 c1 = com + dx;
@@ -72,7 +75,7 @@ __device__ __forceinline__ void verlet_row(const VerletArrays& a, int64_t P, d3 
     }();
     const d3 du = dx * -1.0;
     for (int i = 0; i < nrot; ++i) {
-        d3 r = qrot(q, ld3(vp + 3 * i) - com, qc);
+        d3 r = qrot(qr, ld3(vp + 3 * i) - com, qc);
         d3 nv = (r + com) + dx;
         if (out) nv = nv + du;
         st3(vp + 3 * i, nv);
@@ -108,7 +111,7 @@ __global__ void __launch_bounds__(256) k_verlet(VerletArrays a, const double* __
                                                 int64_t pmax) {
     const int64_t P = pmin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (P >= pmax) return;
-    verlet_row(a, P, ld3(F + 3 * P), ld3(tau + 3 * P), vmax, nrot ? nrot[P] : vmax, dt, box.b);
+    verlet_row(a, P, ld3(F + 3 * P), ld3(tau + 3 * P), vmax, nrot ? nrot[P] : vmax, dt, box.b, 0);
 }
 
 // Force assembly fused with the integrator, over the moving-particle list.
@@ -142,7 +145,7 @@ __global__ void __launch_bounds__(256) k_reduce_verlet(VerletArrays a, const int
     F = F + ld3(fext + 3 * P);
     T = T + ld3(text + 3 * P);
     const SceneDev& sc = scenes[scene_of[P]];
-    verlet_row(a, P, F, T, vmax, nverts[P], dt, sc.box);
+    verlet_row(a, P, F, T, vmax, nverts[P], dt, sc.box, sc.law_p.mode == MWD_LAW_PHYSICAL ? 1 : 0);
 }
 
 }  // namespace

@@ -29,12 +29,17 @@ G_MM = 9810.0          # mm/s^2
 RHO_KG_MM3 = 2.5e-6    # 2500 kg/m^3
 
 
+REF = 0       # the reference's contact law, quirks included
+PHYSICAL = 1  # midpoint contact point, effective-mass damping, tangential force on linear momentum
+
+
 @dataclass
 class Law4:
     Kn: float = 1e4
     en: float = 0.5
     et: float = 0.5
     mu: float = 0.3
+    mode: int = REF
 
 
 @dataclass
@@ -161,15 +166,24 @@ def body_frame(I: np.ndarray):
     return Ibf, np.linalg.inv(Ibf)
 
 
-def make_particle(rng, npts: int, axes, size: float, density: float, ref_inertia: bool):
+def make_particle(rng, npts: int, axes, size: float, density: float, ref_inertia: bool, consistent: bool = False):
+    """One hull. With consistent=False the inertia follows the reference
+    (diag(eig(I)) while the vertices stay in the hull frame, and the reference's
+    volume element when ref_inertia); with consistent=True the full tensor is
+    expressed in the world frame of the rotated vertices (the frame in which
+    q = 1 at t = 0), so geometry and inertia agree."""
     pts = ellipsoid_points(rng, npts, np.asarray(axes) * size)
     V, tri, vol = hull_body(pts)
     cm = V.mean(axis=0)
     V = V - cm
     I, isize = polyhedron_inertia(V, tri, density, ref_inertia)
-    Ibf, Iinv = body_frame(I)
     R = random_rotation(rng)
     Vw = V @ R.T
+    if consistent:
+        Iw = R @ I @ R.T
+        Ibf, Iinv = Iw, np.linalg.inv(Iw)
+    else:
+        Ibf, Iinv = body_frame(I)
     return Vw, density * vol, Ibf, Iinv, isize, face_normals(Vw, tri)
 
 
@@ -177,21 +191,22 @@ _CHUNK = 2048
 
 
 def _particle_chunk(args):
-    seed, chunk, n, npts, axes, size, density, ref_inertia = args
+    seed, chunk, n, npts, axes, size, density, ref_inertia, consistent = args
     rng = np.random.Generator(np.random.PCG64([seed, chunk]))
     out = []
     for _ in range(n):
         k = int(rng.integers(npts[0], npts[1] + 1))
         s = float(rng.uniform(size[0], size[1]))
-        out.append(make_particle(rng, k, axes, s, density, ref_inertia))
+        out.append(make_particle(rng, k, axes, s, density, ref_inertia, consistent))
     return out
 
 
-def make_particles(n: int, seed: int, npts, axes, size, density, ref_inertia, workers: int | None = None):
+def make_particles(n: int, seed: int, npts, axes, size, density, ref_inertia, consistent=False,
+                   workers: int | None = None):
     """n particles in chunks of 2048, chunk c drawn from PCG64([seed, c]): the
     result does not depend on the number of worker processes."""
-    jobs = [(seed, c, min(_CHUNK, n - c * _CHUNK), tuple(npts), tuple(axes), tuple(size), density, ref_inertia)
-            for c in range((n + _CHUNK - 1) // _CHUNK)]
+    jobs = [(seed, c, min(_CHUNK, n - c * _CHUNK), tuple(npts), tuple(axes), tuple(size), density, ref_inertia,
+             consistent) for c in range((n + _CHUNK - 1) // _CHUNK)]
     if len(jobs) <= 1:
         return [p for j in jobs for p in _particle_chunk(j)]
     import multiprocessing as mp
@@ -366,7 +381,8 @@ def make_scene(n_particles: int, box_mm, seed: int, npts=(12, 12), axes=(1.0, 0.
     rng = np.random.Generator(np.random.PCG64(seed))
     law = law or Law4()
     wall_law = wall_law or law
-    parts = make_particles(n_particles, seed, npts, axes, size, density, ref_inertia)
+    parts = make_particles(n_particles, seed, npts, axes, size, density, ref_inertia,
+                           consistent=(law.mode == PHYSICAL))
     rmax = max(float(np.max(np.linalg.norm(p[0], axis=1))) for p in parts) if parts else 1.0
     ms = 2.0 * rmax
     spacing = spacing_factor * ms

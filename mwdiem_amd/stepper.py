@@ -90,8 +90,8 @@ class Stepper:
                 d.cell[k] = float(sc.cell[k])
             d.ncx, d.ncy, d.kmin, d.kspan = sc.ncx, sc.ncy, sc.kmin, sc.kspan
             d.wall_thr = sc.wall_thr
-            d.law_p = _lib.law(sc.law_p.Kn, sc.law_p.en, sc.law_p.et, sc.law_p.mu, self.dt)
-            d.law_w = _lib.law(sc.law_w.Kn, sc.law_w.en, sc.law_w.et, sc.law_w.mu, self.dt)
+            d.law_p = _lib.law(sc.law_p.Kn, sc.law_p.en, sc.law_p.et, sc.law_p.mu, self.dt, sc.law_p.mode)
+            d.law_w = _lib.law(sc.law_w.Kn, sc.law_w.en, sc.law_w.et, sc.law_w.mu, self.dt, sc.law_w.mode)
         self._scene_arr = sc_arr
         lib = _lib.load()
         ws = C.c_void_p()

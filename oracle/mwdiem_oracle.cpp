@@ -93,12 +93,19 @@ template <typename T> inline bool is_zero(V3<T> a) { return a.x == T(0.0) && a.y
 // ------------------------------------------------------------------ contact law
 // Pair-independent coefficients of Force (.cu:685-701), evaluated in the
 // reference's operator order.
+// mode 0: the reference law (Q3, Q4, Q5 as written). mode 1: the physical
+// variant -- contact point (cpA+cpB)/2, damping from the effective masses
+// (normal 1/m_n = 1/mA + 1/mB, tangential 1/m_t = 1/m_n + rA^2/IA + rB^2/IB,
+// walls infinitely heavy), tangential force on the linear momentum too.
 struct Law {
     double Kn, mu, ln_en, ln_et, A_en, tc, Gn, Kt_num, inv_mu, dt;
+    double cn, ct, ratio;
+    int mode;
 };
 
-Law make_law(double Kn, double en, double et, double mu, double dt) {
+Law make_law(double Kn, double en, double et, double mu, double dt, int mode = 0) {
     Law L;
+    L.mode = mode;
     const double pi = 3.141592653589793;
     const double pi2 = pi * pi;
     L.Kn = Kn;
@@ -111,6 +118,9 @@ Law make_law(double Kn, double en, double et, double mu, double dt) {
     L.Kt_num = ((Kn * (pi2 + L.ln_et * L.ln_et)) / L.A_en) * 1.0;
     L.inv_mu = 1.0 / mu;
     L.dt = dt;
+    L.cn = (-2.0 * L.ln_en) / std::sqrt(L.A_en);
+    L.ct = (-2.0 * L.ln_et) / std::sqrt(pi2 + L.ln_et * L.ln_et);
+    L.ratio = (pi2 + L.ln_et * L.ln_et) / L.A_en;
     return L;
 }
 
@@ -367,21 +377,37 @@ template <typename T> struct PairIn {
     V3<T> comA, comB, vA, vB, wA, wB, dispt;
     Q4<T> qA;
     T isA, isB;
+    T imA, imB;  // inverse masses (0 for walls), used by the physical law
     int wall;
 };
 template <typename T> struct ForceOut { V3<T> Fn, tA, tB, disp; };
 
-// Force, .cu:647-871 (Q1, Q3, Q4, Q5 reproduced).
+// Force, .cu:647-871 (Q1, Q3, Q4, Q5 reproduced in mode 0). o.Fn is the
+// force on A's linear momentum: the normal force (mode 0) or normal plus
+// tangential (mode 1).
 template <typename T>
 ForceOut<T> force_law(const PairIn<T>& in, T depth, V3<T> n, V3<T> cpA, V3<T> cpB, const Law& L) {
     ForceOut<T> o;
-    V3<T> cp = (cpA - cpB) / T(2.0);
+    const bool phys = L.mode == 1;
+    V3<T> cp = phys ? (cpA + cpB) / T(2.0) : (cpA - cpB) / T(2.0);
     V3<T> dA = cp - in.comA, dB = cp - in.comB;
     T dAn = norm2(dA), dBn = norm2(dB);
-    T C = (T(L.inv_mu) + dAn / in.isA) + dBn / in.isB;
-    T Kt = T(L.Kt_num) / C;
-    T Gt = (T(-2.0) / C) * T(L.ln_et);
-    T Gn = T(L.Gn);
+    T Kt, Gt, Gn;
+    if (!phys) {
+        T C = (T(L.inv_mu) + dAn / in.isA) + dBn / in.isB;
+        Kt = T(L.Kt_num) / C;
+        Gt = (T(-2.0) / C) * T(L.ln_et);
+        Gn = T(L.Gn);
+    } else {
+        T imn = in.imA + in.imB;
+        T mn = T(1.0) / imn;
+        T imt = (imn + dAn * (in.imA == T(0.0) ? T(0.0) : T(1.0) / in.isA)) +
+                dBn * (in.imB == T(0.0) ? T(0.0) : T(1.0) / in.isB);
+        T mt = T(1.0) / imt;
+        Gn = T(L.cn) * sqrt_(mn * T(L.Kn));
+        Kt = (T(L.Kn) * T(L.ratio)) * (mt / mn);
+        Gt = T(L.ct) * sqrt_(mt * Kt);
+    }
     if (dot(in.comB - in.comA, n) < T(0.0)) n = n * T(-1.0);
     T nn = sqrt_(norm2(n));
     n = n / nn;
@@ -408,7 +434,7 @@ ForceOut<T> force_law(const PairIn<T>& in, T depth, V3<T> n, V3<T> cpA, V3<T> cp
         dnew = ((((t * Fnn) * T(L.mu)) * T(-1.0)) - ut * Gt) / Kt;
     }
     V3<T> Ff = Ft + Fn;
-    o.Fn = Fn;
+    o.Fn = phys ? Ff : Fn;
     o.tA = cross(cp - in.comA, Ff);
     o.tB = cross(cp - in.comB, Ff * T(-1.0));
     o.disp = dnew;
@@ -505,7 +531,10 @@ struct StateView {
 };
 
 // Verlet, gpu_Verlet.cu:85-404, for particle P.
-void verlet_one(const StateView& s, int64_t P, int vmax, int nrot, double dt, const double* box) {
+// rot_mode 0: vertices rotated by the absolute orientation q each step, as the
+// reference does (gpu_Verlet.cu:279-305, the rotation compounds); 1: by the
+// step's increment q q_old^-1 (the physical variant).
+void verlet_one(const StateView& s, int64_t P, int vmax, int nrot, double dt, const double* box, int rot_mode) {
     typedef double T;
     V3<T> velocity = load3<T>(s.v + 3 * P);
     V3<T> v12 = load3<T>(s.v12 + 3 * P);
@@ -554,10 +583,11 @@ void verlet_one(const StateView& s, int64_t P, int vmax, int nrot, double dt, co
     wbf = w12bf + (abf * dt) / 2.0;
     w = qtripl(q, wbf, qinv(q));
     velocity = v12 + (F * dt) / m;
-    Q4<T> qinv2 = qinv(q);
+    Q4<T> qr = rot_mode == 1 ? qmul(q, qinv(qold)) : q;
+    Q4<T> qinv2 = qinv(qr);
     double* vp = s.verts + (int64_t)vmax * 3 * P;
     for (int i = 0; i < nrot; ++i) {
-        V3<T> r = qtripl(q, load3<T>(vp + 3 * i) - com, qinv2);
+        V3<T> r = qtripl(qr, load3<T>(vp + 3 * i) - com, qinv2);
         V3<T> nvx = r + com;
         vp[3 * i] = nvx.x; vp[3 * i + 1] = nvx.y; vp[3 * i + 2] = nvx.z;
     }
@@ -669,8 +699,10 @@ ORC_EXPORT int64_t orc_wall_pairs(const double* com, int64_t W, int64_t n, const
 
 // Linked_cell_hash, LIBRARY_Coarse_contact_detection.py:84-155, over ids
 // [pmin,pmax) with keys[i]. Returns the count; writes at most cap pairs.
+// stencil 0: the reference's 7 forward offsets; 1: the full 13-cell forward
+// half shell (dz>0, or dz=0 and dy>0, or dz=dy=0 and dx>0).
 ORC_EXPORT int64_t orc_linked_cell_hash(const int64_t* keys, int64_t pmin, int64_t pmax, int64_t ncx,
-                                        int64_t ncy, int32_t* pairs, int64_t cap) {
+                                        int64_t ncy, int32_t* pairs, int64_t cap, int stencil) {
     std::unordered_map<int64_t, size_t> where;
     std::vector<int64_t> order;
     std::vector<std::vector<int64_t>> members;
@@ -685,7 +717,14 @@ ORC_EXPORT int64_t orc_linked_cell_hash(const int64_t* keys, int64_t pmin, int64
         }
     }
     for (auto& m : members) m = set_order(m);
-    const int64_t offs[7] = {1, ncx, ncx * ncy, ncx * ncy + 1, ncx * ncy + 1 + ncx, ncx * ncy + ncx, ncx + 1};
+    std::vector<int64_t> offs;
+    if (stencil == 1) {
+        const int h[13][3] = {{1, 0, 0},  {-1, 1, 0}, {0, 1, 0},  {1, 1, 0},  {-1, -1, 1}, {0, -1, 1}, {1, -1, 1},
+                              {-1, 0, 1}, {0, 0, 1},  {1, 0, 1},  {-1, 1, 1}, {0, 1, 1},   {1, 1, 1}};
+        for (int k = 0; k < 13; ++k) offs.push_back(h[k][0] + h[k][1] * ncx + h[k][2] * ncx * ncy);
+    } else {
+        offs = {1, ncx, ncx * ncy, ncx * ncy + 1, ncx * ncy + 1 + ncx, ncx * ncy + ncx, ncx + 1};
+    }
     int64_t cnt = 0;
     auto emit = [&](int64_t a, int64_t b) {
         if (cnt < cap) { pairs[2 * cnt] = (int32_t)a; pairs[2 * cnt + 1] = (int32_t)b; }
@@ -695,7 +734,7 @@ ORC_EXPORT int64_t orc_linked_cell_hash(const int64_t* keys, int64_t pmin, int64
         const std::vector<int64_t>& L = members[c];
         for (size_t j = 0; j < L.size(); ++j)
             for (size_t k = j + 1; k < L.size(); ++k) emit(L[j], L[k]);
-        for (int o = 0; o < 7; ++o) {
+        for (size_t o = 0; o < offs.size(); ++o) {
             auto it = where.find(order[c] + offs[o]);
             if (it == where.end()) continue;
             const std::vector<int64_t>& M = members[it->second];
@@ -707,10 +746,10 @@ ORC_EXPORT int64_t orc_linked_cell_hash(const int64_t* keys, int64_t pmin, int64
 }
 
 // Law coefficients as the product computes them (for cross-checks).
-ORC_EXPORT void orc_law(double Kn, double en, double et, double mu, double dt, double* out10) {
+ORC_EXPORT void orc_law(double Kn, double en, double et, double mu, double dt, double* out13) {
     Law L = make_law(Kn, en, et, mu, dt);
-    double v[10] = {L.Kn, L.mu, L.ln_en, L.ln_et, L.A_en, L.tc, L.Gn, L.Kt_num, L.inv_mu, L.dt};
-    for (int i = 0; i < 10; ++i) out10[i] = v[i];
+    double v[13] = {L.Kn, L.mu, L.ln_en, L.ln_et, L.A_en, L.tc, L.Gn, L.Kt_num, L.inv_mu, L.dt, L.cn, L.ct, L.ratio};
+    for (int i = 0; i < 13; ++i) out13[i] = v[i];
 }
 
 // Per-pair fine contact in the reference's own input layout
@@ -731,8 +770,8 @@ ORC_EXPORT void orc_contact_pairs(int64_t P, int vmax,
                                   int32_t* collision, int32_t* epa_status, double* depth, double* normal,
                                   double* cpA, double* cpB, double* Fn, double* tauA, double* tauB,
                                   double* disp_out, double* force, double* torque, int32_t* faces_used) {
-    Law Lp = make_law(law_p[0], law_p[1], law_p[2], law_p[3], dt);
-    Law Lw = make_law(law_w[0], law_w[1], law_w[2], law_w[3], dt);
+    Law Lp = make_law(law_p[0], law_p[1], law_p[2], law_p[3], dt, (int)law_p[4]);
+    Law Lw = make_law(law_w[0], law_w[1], law_w[2], law_w[3], dt, (int)law_w[4]);
     (void)qB;
     std::unordered_map<int64_t, std::vector<double>> sums;
     for (int64_t p = 0; p < P; ++p) {
@@ -745,6 +784,7 @@ ORC_EXPORT void orc_contact_pairs(int64_t P, int vmax,
         in.dispt = load3<double>(dispt + 4 * p);
         in.qA = qload<double>(qA + 4 * p);
         in.isA = isA[p]; in.isB = isB[p];
+        in.imA = 0.0; in.imB = 0.0;  // the per-pair layout carries no masses (reference law only)
         in.wall = (idA[p] < W || idB[p] < W) ? 1 : 0;
         PairResult r = contact_one(A, B, in, in.wall ? Lw : Lp);
         collision[p] = r.collision;
@@ -789,10 +829,10 @@ ORC_EXPORT void orc_contact_pairs(int64_t P, int vmax,
 ORC_EXPORT uint64_t orc_count_flops(int64_t P, const int32_t* pa, const int32_t* pb, int vmax,
                                     const double* com, const double* verts, const int32_t* nverts,
                                     const double* vel, const double* angvel, const double* q,
-                                    const double* isize, const double* law_p, const double* law_w,
+                                    const double* isize, const double* mass, const double* law_p, const double* law_w,
                                     int64_t W, double dt) {
-    Law Lp = make_law(law_p[0], law_p[1], law_p[2], law_p[3], dt);
-    Law Lw = make_law(law_w[0], law_w[1], law_w[2], law_w[3], dt);
+    Law Lp = make_law(law_p[0], law_p[1], law_p[2], law_p[3], dt, (int)law_p[4]);
+    Law Lw = make_law(law_w[0], law_w[1], law_w[2], law_w[3], dt, (int)law_w[4]);
     g_flops = 0;
     for (int64_t p = 0; p < P; ++p) {
         int a = pa[p], b = pb[p];
@@ -805,6 +845,7 @@ ORC_EXPORT uint64_t orc_count_flops(int64_t P, const int32_t* pa, const int32_t*
         in.dispt = mk<CD>(CD(0.0), CD(0.0), CD(0.0));
         in.qA = qload<CD>(q + 4 * a);
         in.isA = CD(isize[a]); in.isB = CD(isize[b]);
+        in.imA = CD(a < W ? 0.0 : 1.0 / mass[a]); in.imB = CD(b < W ? 0.0 : 1.0 / mass[b]);
         in.wall = (a < W || b < W) ? 1 : 0;
         contact_one(A, B, in, in.wall ? Lw : Lp);
     }
@@ -817,9 +858,9 @@ ORC_EXPORT void orc_verlet(double* v, double* v12, double* w12bf, double* q12, d
                            double* wbf, double* w, const double* F, const double* tau, double* verts,
                            double* com, const double* mass, const double* Iinv, const double* I,
                            double* dxall, int vmax, const int32_t* nrot, double dt, const double* box,
-                           int64_t pmin, int64_t pmax) {
+                           int64_t pmin, int64_t pmax, int rot_mode) {
     StateView s{v, v12, w12bf, q12, q, qall, wbf, w, F, tau, verts, com, mass, Iinv, I, dxall};
-    for (int64_t P = pmin; P < pmax; ++P) verlet_one(s, P, vmax, nrot ? nrot[P] : vmax, dt, box);
+    for (int64_t P = pmin; P < pmax; ++P) verlet_one(s, P, vmax, nrot ? nrot[P] : vmax, dt, box, rot_mode);
 }
 
 // Per-particle contact sums (ascending pair order) for a pair list indexed into
@@ -827,12 +868,12 @@ ORC_EXPORT void orc_verlet(double* v, double* v12, double* w12bf, double* q12, d
 ORC_EXPORT int64_t orc_contact_state(int64_t P, const int32_t* pa, const int32_t* pb, int vmax,
                                      const double* com, const double* verts, const int32_t* nverts,
                                      const double* vel, const double* angvel, const double* q,
-                                     const double* isize, const double* law_p, const double* law_w,
+                                     const double* isize, const double* mass, const double* law_p, const double* law_w,
                                      int64_t W, double dt, int32_t* collision, double* depth,
                                      double* normal, double* cpA, double* cpB, double* force,
                                      double* torque, int32_t* epa_status) {
-    Law Lp = make_law(law_p[0], law_p[1], law_p[2], law_p[3], dt);
-    Law Lw = make_law(law_w[0], law_w[1], law_w[2], law_w[3], dt);
+    Law Lp = make_law(law_p[0], law_p[1], law_p[2], law_p[3], dt, (int)law_p[4]);
+    Law Lw = make_law(law_w[0], law_w[1], law_w[2], law_w[3], dt, (int)law_w[4]);
     int64_t K = 0;
     for (int64_t p = 0; p < P; ++p) {
         int a = pa[p], b = pb[p];
@@ -845,6 +886,7 @@ ORC_EXPORT int64_t orc_contact_state(int64_t P, const int32_t* pa, const int32_t
         in.dispt = mk<double>(0.0, 0.0, 0.0);
         in.qA = qload<double>(q + 4 * a);
         in.isA = isize[a]; in.isB = isize[b];
+        in.imA = a < W ? 0.0 : 1.0 / mass[a]; in.imB = b < W ? 0.0 : 1.0 / mass[b];
         in.wall = (a < W || b < W) ? 1 : 0;
         PairResult r = contact_one(A, B, in, in.wall ? Lw : Lp);
         collision[p] = r.collision;
@@ -866,4 +908,53 @@ ORC_EXPORT int64_t orc_contact_state(int64_t P, const int32_t* pa, const int32_t
         }
     }
     return K;
+}
+
+// ------------------------------------------------------------ test helpers
+ORC_EXPORT void orc_quat_mul(const double* a, const double* b, double* out) {
+    Q4<double> r = qmul(qload<double>(a), qload<double>(b));
+    out[0] = r.w; out[1] = r.x; out[2] = r.y; out[3] = r.z;
+}
+
+ORC_EXPORT void orc_quat_tripl(const double* q1, const double* v, const double* q2, double* out) {
+    V3<double> r = qtripl(qload<double>(q1), load3<double>(v), qload<double>(q2));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+// GJK + EPA of two vertex sets with given centres (contact_detection's GJK
+// driver and EPA, no force law). out: [depth, n(3), cpA(3), cpB(3)].
+ORC_EXPORT int orc_gjk_epa(const double* vA, int nA, const double* vB, int nB, const double* comA, const double* comB,
+                           double* out, int* status) {
+    Body<double> A{vA, nA};
+    Body<double> B{vB, nB};
+    PairIn<double> in;
+    std::memset(&in, 0, sizeof(in));
+    in.comA = load3<double>(comA);
+    in.comB = load3<double>(comB);
+    in.qA.w = 1.0;
+    in.isA = 1.0; in.isB = 1.0;
+    Law L = make_law(1.0, 0.5, 0.5, 0.5, 1.0);
+    PairResult r = contact_one(A, B, in, L);
+    out[0] = r.depth;
+    for (int k = 0; k < 3; ++k) { out[1 + k] = r.n[k]; out[4 + k] = r.cpA[k]; out[7 + k] = r.cpB[k]; }
+    if (status) *status = r.epa_status;
+    return r.collision;
+}
+
+// Flop count of one GJK/EPA + law evaluation of a pair (reference law).
+ORC_EXPORT uint64_t orc_pair_flops(const double* vA, int nA, const double* vB, int nB, const double* comA,
+                                   const double* comB) {
+    Body<CD> A{vA, nA};
+    Body<CD> B{vB, nB};
+    PairIn<CD> in;
+    in.comA = load3<CD>(comA);
+    in.comB = load3<CD>(comB);
+    in.vA = in.vB = in.wA = in.wB = in.dispt = mk<CD>(CD(0.0), CD(0.0), CD(0.0));
+    in.qA.w = CD(1.0); in.qA.x = in.qA.y = in.qA.z = CD(0.0);
+    in.isA = CD(1.0); in.isB = CD(1.0); in.imA = CD(0.0); in.imB = CD(0.0);
+    in.wall = 0;
+    Law L = make_law(1.0, 0.5, 0.5, 0.5, 1.0);
+    g_flops = 0;
+    contact_one(A, B, in, L);
+    return g_flops;
 }

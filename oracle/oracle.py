@@ -43,6 +43,8 @@ def lib():
         _lib.orc_count_flops.restype = C.c_uint64
         _lib.orc_contact_state.restype = C.c_int64
         _lib.orc_epa_max_faces.restype = C.c_int
+        _lib.orc_gjk_epa.restype = C.c_int
+        _lib.orc_pair_flops.restype = C.c_uint64
     return _lib
 
 
@@ -87,13 +89,14 @@ def wall_pairs(com, W, n, wall_n, wall_c, thr) -> np.ndarray:
     return out
 
 
-def linked_cell_hash(keys, pmin, pmax, ncx, ncy) -> np.ndarray:
+def linked_cell_hash(keys, pmin, pmax, ncx, ncy, stencil: int = 0) -> np.ndarray:
+    """stencil 0: the reference's 7 offsets; 1: the full 13-cell half shell."""
     keys = _i64(keys)
     cnt = lib().orc_linked_cell_hash(_p(keys), C.c_int64(pmin), C.c_int64(pmax), C.c_int64(ncx), C.c_int64(ncy),
-                                     None, C.c_int64(0))
+                                     None, C.c_int64(0), C.c_int(stencil))
     out = np.zeros((cnt, 2), dtype=np.int32)
     lib().orc_linked_cell_hash(_p(keys), C.c_int64(pmin), C.c_int64(pmax), C.c_int64(ncx), C.c_int64(ncy), _p(out),
-                               C.c_int64(cnt))
+                               C.c_int64(cnt), C.c_int(stencil))
     return out
 
 
@@ -124,7 +127,7 @@ def linked_cell_hash_py(keys, pmin, pmax, ncx, ncy) -> list:
 
 
 def law(Kn, en, et, mu, dt) -> np.ndarray:
-    out = np.zeros(10)
+    out = np.zeros(13)
     lib().orc_law(C.c_double(Kn), C.c_double(en), C.c_double(et), C.c_double(mu), C.c_double(dt), _p(out))
     return out
 
@@ -138,7 +141,8 @@ def contact_pairs(comA, comB, vertsA, vertsB, vA, vB, wA, wB, nA, nB, idA, idB, 
     a = [_f64(comA), _f64(comB), _f64(vertsA), _f64(vertsB), _f64(vA), _f64(vB), _f64(wA), _f64(wB)]
     ia = [_i32(nA), _i32(nB), _i32(idA), _i32(idB)]
     b = [_f64(dispt), _f64(qA), _f64(qB), _f64(isA), _f64(isB)]
-    lp, lw = _f64(law_p), _f64(law_w)
+    lp = _f64(list(law_p) + [0.0] * (5 - len(law_p)))
+    lw = _f64(list(law_w) + [0.0] * (5 - len(law_w)))
     out = dict(collision=np.zeros(P, np.int32), epa_status=np.zeros(P, np.int32), depth=np.zeros(P),
                normal=np.zeros((P, 3)), cpA=np.zeros((P, 3)), cpB=np.zeros((P, 3)), Fn=np.zeros((P, 3)),
                tauA=np.zeros((P, 3)), tauB=np.zeros((P, 3)), disp=np.zeros((P, 4)), faces=np.zeros(P, np.int32))
@@ -150,10 +154,12 @@ def contact_pairs(comA, comB, vertsA, vertsB, vA, vB, wA, wB, nA, nB, idA, idB, 
     return out
 
 
+def _law5(L):
+    return np.array([L.Kn, L.en, L.et, L.mu, float(getattr(L, "mode", 0))])
+
+
 def _laws(sc):
-    lp = np.array([sc.law_p.Kn, sc.law_p.en, sc.law_p.et, sc.law_p.mu])
-    lw = np.array([sc.law_w.Kn, sc.law_w.en, sc.law_w.et, sc.law_w.mu])
-    return lp, lw
+    return _law5(sc.law_p), _law5(sc.law_w)
 
 
 def contact_state(pairs, st, sc):
@@ -174,8 +180,8 @@ def contact_state(pairs, st, sc):
     lp, lw = _laws(sc)
     lib().orc_contact_state(C.c_int64(P), _p(pa), _p(pb), C.c_int(sc.vmax), _p(_f64(st["com"])),
                             _p(_f64(st["verts"])), _p(_i32(sc.nverts)), _p(_f64(st["v"])), _p(_f64(st["w"])),
-                            _p(_f64(st["q"])), _p(_f64(sc.isize)), _p(lp), _p(lw), C.c_int64(sc.n_walls),
-                            C.c_double(sc.dt), _p(coll), _p(depth), _p(nrm), _p(ca), _p(cb), _p(F), _p(T), _p(est))
+                            _p(_f64(st["q"])), _p(_f64(sc.isize)), _p(_f64(sc.mass)), _p(lp), _p(lw),
+                            C.c_int64(sc.n_walls), C.c_double(sc.dt), _p(coll), _p(depth), _p(nrm), _p(ca), _p(cb), _p(F), _p(T), _p(est))
     return coll, depth, nrm, ca, cb, F, T, est
 
 
@@ -186,7 +192,7 @@ def count_flops(pairs, st, sc) -> int:
     return int(lib().orc_count_flops(C.c_int64(pairs.shape[0]), _p(pa), _p(pb), C.c_int(sc.vmax),
                                      _p(_f64(st["com"])), _p(_f64(st["verts"])), _p(_i32(sc.nverts)),
                                      _p(_f64(st["v"])), _p(_f64(st["w"])), _p(_f64(st["q"])), _p(_f64(sc.isize)),
-                                     _p(lp), _p(lw), C.c_int64(sc.n_walls), C.c_double(sc.dt)))
+                                     _p(_f64(sc.mass)), _p(lp), _p(lw), C.c_int64(sc.n_walls), C.c_double(sc.dt)))
 
 
 STATE_KEYS = ("com", "v", "v12", "w", "wbf", "w12bf", "q", "q12", "qall", "dxall", "verts")
@@ -204,8 +210,12 @@ def initial_state(sc) -> dict:
     return st
 
 
-def verlet(st, F, T, sc, pmin, pmax, nrot=None):
-    """Verlet (gpu_Verlet.cu:85-404) in place on the state dict."""
+def verlet(st, F, T, sc, pmin, pmax, nrot=None, rot_mode=None):
+    """Verlet (gpu_Verlet.cu:85-404) in place on the state dict. rot_mode 0
+    rotates vertices by q (reference), 1 by the increment q q_old^-1; by
+    default it follows the scene's particle law mode."""
+    if rot_mode is None:
+        rot_mode = int(getattr(sc.law_p, "mode", 0))
     arr = {k: st[k] for k in STATE_KEYS}
     for k, a in arr.items():
         assert a.dtype == np.float64 and a.flags.c_contiguous, k
@@ -213,7 +223,7 @@ def verlet(st, F, T, sc, pmin, pmax, nrot=None):
                      _p(arr["wbf"]), _p(arr["w"]), _p(_f64(F)), _p(_f64(T)), _p(arr["verts"]), _p(arr["com"]),
                      _p(_f64(sc.mass)), _p(_f64(sc.Iinv)), _p(_f64(sc.I)), _p(arr["dxall"]), C.c_int(sc.vmax),
                      _p(None if nrot is None else _i32(nrot)), C.c_double(sc.dt), _p(_f64(sc.box)),
-                     C.c_int64(pmin), C.c_int64(pmax))
+                     C.c_int64(pmin), C.c_int64(pmax), C.c_int(rot_mode))
 
 
 def candidates(st, sc) -> np.ndarray:
@@ -221,7 +231,7 @@ def candidates(st, sc) -> np.ndarray:
     W, N = sc.n_walls, sc.n_rows
     wp = wall_pairs(st["com"], W, N, sc.wall_n, sc.wall_c, sc.wall_thr)
     keys = cell_keys(st["com"], sc.cell, sc.ncx, sc.ncy)
-    hp = linked_cell_hash(keys, W, N, sc.ncx, sc.ncy)
+    hp = linked_cell_hash(keys, W, N, sc.ncx, sc.ncy, stencil=int(getattr(sc.law_p, "mode", 0)))
     return np.concatenate([wp, hp]).astype(np.int32).reshape(-1, 2)
 
 
@@ -234,3 +244,31 @@ def step(st, sc):
     T = T + sc.text
     verlet(st, F, T, sc, sc.n_walls, sc.n_rows, nrot=sc.nverts)
     return pairs, coll
+
+
+def quat_mul(a, b) -> np.ndarray:
+    out = np.zeros(4)
+    lib().orc_quat_mul(_p(_f64(a)), _p(_f64(b)), _p(out))
+    return out
+
+
+def quat_tripl(q1, v, q2) -> np.ndarray:
+    out = np.zeros(3)
+    lib().orc_quat_tripl(_p(_f64(q1)), _p(_f64(v)), _p(_f64(q2)), _p(out))
+    return out
+
+
+def gjk_epa(VA, VB, comA, comB):
+    """(collision, depth, normal, cpA, cpB, epa_status) of two vertex sets."""
+    VA, VB = _f64(VA), _f64(VB)
+    out = np.zeros(10)
+    st = C.c_int(0)
+    c = lib().orc_gjk_epa(_p(VA), C.c_int(VA.shape[0]), _p(VB), C.c_int(VB.shape[0]), _p(_f64(comA)),
+                          _p(_f64(comB)), _p(out), C.byref(st))
+    return c, out[0], out[1:4], out[4:7], out[7:10], st.value
+
+
+def pair_flops(VA, VB, comA, comB) -> int:
+    VA, VB = _f64(VA), _f64(VB)
+    return int(lib().orc_pair_flops(_p(VA), C.c_int(VA.shape[0]), _p(VB), C.c_int(VB.shape[0]), _p(_f64(comA)),
+                                    _p(_f64(comB))))

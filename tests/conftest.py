@@ -33,3 +33,20 @@ def gpu():
     assert torch.cuda.is_available(), "GPU tests need a ROCm GPU"
     _lib.load()
     return torch.device("cuda", 0)
+
+
+@pytest.fixture(scope="session")
+def dense_phys(dense_scene):
+    """The dense scene on the physical law (full stencil, midpoint contact point)."""
+    from mwdiem_amd import scene
+    sc = dense_scene.copy()
+    sc.law_p = scene.Law4(mode=scene.PHYSICAL)
+    sc.law_w = scene.Law4(mode=scene.PHYSICAL)
+    return sc
+
+
+@pytest.fixture(scope="session")
+def impact_scene():
+    """60 deposited hulls (touching, not overlapping) on the physical law."""
+    from mwdiem_amd import scene
+    return scene.make_scene(60, (6.0, 6.0, 16.0), 21, law=scene.Law4(mode=scene.PHYSICAL), name="impact-60")

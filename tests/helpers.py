@@ -4,9 +4,9 @@ from __future__ import annotations
 import numpy as np
 
 
-def random_motion(st: dict, sc, seed: int, vscale=50.0, wscale=200.0):
+def random_motion(st: dict, sc, seed: int, vscale=50.0, wscale=200.0, quats=True):
     """Give the moving particles random velocities, angular velocities and
-    orientations (vertices rotated consistently about the centre)."""
+    (with quats) random orientation quaternions. Vertices are left as they are."""
     rng = np.random.Generator(np.random.PCG64(seed))
     W, N = sc.n_walls, sc.n_rows
     n = N - W
@@ -15,10 +15,11 @@ def random_motion(st: dict, sc, seed: int, vscale=50.0, wscale=200.0):
     st["w"][W:] = rng.normal(size=(n, 3)) * wscale
     st["wbf"][W:] = rng.normal(size=(n, 3)) * wscale
     st["w12bf"][W:] = st["wbf"][W:] + rng.normal(size=(n, 3))
-    for k in ("q", "q12", "qall"):
-        q = rng.normal(size=(n, 4))
-        q /= np.linalg.norm(q, axis=1, keepdims=True)
-        st[k][W:] = q
+    if quats:
+        for k in ("q", "q12", "qall"):
+            q = rng.normal(size=(n, 4))
+            q /= np.linalg.norm(q, axis=1, keepdims=True)
+            st[k][W:] = q
     return st
 
 

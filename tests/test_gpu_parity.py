@@ -79,9 +79,9 @@ def test_verlet_100_steps(gpu, small_scene):
         assert err <= 1e-8, (k, err)
 
 
-@pytest.mark.parametrize("which", ["small", "dense"])
-def test_candidates_identical(gpu, small_scene, dense_scene, which):
-    sc = small_scene if which == "small" else dense_scene
+@pytest.mark.parametrize("which", ["small", "dense", "dense_phys", "impact"])
+def test_candidates_identical(gpu, small_scene, dense_scene, dense_phys, impact_scene, which):
+    sc = {"small": small_scene, "dense": dense_scene, "dense_phys": dense_phys, "impact": impact_scene}[which]
     S = _stepper(sc)
     S.candidates()
     s = S.check()
@@ -93,9 +93,9 @@ def test_candidates_identical(gpu, small_scene, dense_scene, which):
     assert s["n_wall_pairs"] == nw
 
 
-@pytest.mark.parametrize("seed", [3, 4])
-def test_fine_contact_matches_oracle(gpu, dense_scene, seed):
-    sc = dense_scene
+@pytest.mark.parametrize("seed,phys", [(3, False), (4, False), (3, True)])
+def test_fine_contact_matches_oracle(gpu, dense_scene, dense_phys, seed, phys):
+    sc = dense_phys if phys else dense_scene
     S = _stepper(sc)
     st = random_motion(O.initial_state(sc), sc, seed)
     S.set_state(st)
@@ -131,23 +131,46 @@ def test_step_reduction_matches_oracle(gpu, dense_scene):
         assert max_rel(got[k][W:], ref[k][W:]) <= 1e-12, k
 
 
-def test_100_steps_match_oracle(gpu, dense_scene):
-    sc = dense_scene
+def _run_both(sc, st, steps, check_pairs=5):
     S = _stepper(sc)
-    st = O.initial_state(sc)
-    W = sc.n_walls
-    for i in range(100):
-        pairs_ref, _ = O.step(st, sc)
+    S.set_state(st)
+    n_contacts = 0
+    for i in range(steps):
+        pairs_ref, coll_ref = O.step(st, sc)
         S.step(1)
-        if i < 5:
+        n_contacts += int((coll_ref == 1).sum())
+        if i < check_pairs:
             assert np.array_equal(S.pairs(), pairs_ref), i
     S.check()
-    got = S.numpy_state()
-    for k in ("com", "q", "qall"):
+    return S.numpy_state(), n_contacts
+
+
+def test_100_steps_reference_law(gpu, small_scene):
+    """Reference law, lattice bed with random motion: keys, candidates and the
+    integrator over 100 steps."""
+    sc = small_scene
+    st = random_motion(O.initial_state(sc), sc, 11, vscale=300.0, wscale=30.0, quats=False)
+    got, _ = _run_both(sc, st, 100)
+    W = sc.n_walls
+    for k in ("com", "q", "qall", "v", "w"):
         assert max_rel(got[k][W:], st[k][W:]) <= 1e-8, k
     nv = sc.nverts
     for r in range(W, sc.n_rows):
         assert max_rel(got["verts"][r, :nv[r]], st["verts"][r, :nv[r]], scale=1.0) <= 1e-8
+
+
+def test_100_steps_physical_law_with_impacts(gpu, impact_scene):
+    """Physical law, deposited hulls thrown at each other: contacts every step."""
+    sc = impact_scene
+    W = sc.n_walls
+    st = O.initial_state(sc)
+    rng = np.random.Generator(np.random.PCG64(5))
+    st["v"][W:] = rng.normal(size=(sc.n_particles, 3)) * 1000.0
+    st["v12"][W:] = st["v"][W:]
+    got, n_contacts = _run_both(sc, st, 100)
+    assert n_contacts > 100
+    for k in ("com", "q", "qall"):
+        assert max_rel(got[k][W:], st[k][W:]) <= 1e-8, k
 
 
 def test_contact_dropin_matches_oracle(gpu, dense_scene):

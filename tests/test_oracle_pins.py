@@ -1,0 +1,377 @@
+"""Pins of the oracle (CPU): each check is a closed form, a brute force, an
+invariant or the interpreter's own behaviour -- never the oracle's formula
+retyped -- chosen so that a plausible slip (dropped term, wrong sign or index,
+swapped operand) fails one of them."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+from scipy.spatial import ConvexHull
+
+from oracle import oracle as O
+
+
+# ------------------------------------------------------------ quaternions
+def test_quaternion_kat_reference_test_py():
+    # test.py:16-31 prints (1,2,3,4)x(5,6,7,8); the Hamilton product is (-60,12,30,24)
+    assert np.array_equal(O.quat_mul([1, 2, 3, 4], [5, 6, 7, 8]), [-60, 12, 30, 24])
+
+
+def test_quaternion_kat_verlet_dead_code():
+    # gpu_Verlet.cu:406-426 multiplies (1,2,-3,-1)x(-1,3,43,-1); by hand:
+    # w = -1 - (6 - 129 + 1) = 121, v = (3,43,-1) - (2,-3,-1) + (46,-1,95)
+    assert np.array_equal(O.quat_mul([1, 2, -3, -1], [-1, 3, 43, -1]), [121, 47, 45, 95])
+
+
+def _rotmat(q):
+    w, x, y, z = np.asarray(q, float) / np.linalg.norm(q)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def test_quaternion_triple_product_is_scaled_rotation():
+    # q v q^-1 with the conjugate as inverse = |q|^2 R(q) v (gpu_Verlet.cu:419-426 uses a=(1,2,-3,-1))
+    a = np.array([1.0, 2.0, -3.0, -1.0])
+    ainv = a * np.array([1, -1, -1, -1])
+    v = np.array([1.0, 4.0, -2.0])
+    got = O.quat_tripl(a, v, ainv)
+    assert np.allclose(got, np.dot(a, a) * _rotmat(a) @ v, rtol=0, atol=1e-12)
+
+
+# ------------------------------------------------------------ Verlet
+class _Sc:
+    """Minimal scene for the integrator: one wall row, particles after it."""
+
+    def __init__(self, n, dt=1e-6, I=(1.0, 2.0, 3.0), mass=2.0, vmax=4, box=1e9, mode=0):
+        self.n_walls = 1
+        self.n_rows = n + 1
+        self.vmax = vmax
+        self.dt = dt
+        self.mass = np.full(self.n_rows, mass)
+        self.I = np.tile(np.diag(I).astype(float), (self.n_rows, 1, 1))
+        self.Iinv = np.tile(np.diag(1.0 / np.asarray(I, float)), (self.n_rows, 1, 1))
+        self.box = np.array([box, -box, box, -box, box, -box], float)
+        self.nverts = np.full(self.n_rows, vmax, np.int32)
+        self.com = np.zeros((self.n_rows, 3))
+        self.verts = np.zeros((self.n_rows, vmax, 3))
+
+        class L:
+            pass
+        self.law_p = L()
+        self.law_p.mode = mode
+
+
+def _state(sc):
+    N = sc.n_rows
+    st = {k: np.zeros((N, 3)) for k in ("v", "v12", "w", "wbf", "w12bf", "dxall")}
+    for k in ("q", "q12", "qall"):
+        st[k] = np.zeros((N, 4))
+        st[k][:, 0] = 1.0
+    st["com"] = sc.com.copy()
+    st["verts"] = sc.verts.copy()
+    return st
+
+
+def test_verlet_free_fall_closed_form():
+    sc = _Sc(3, dt=1e-4)
+    rng = np.random.default_rng(0)
+    sc.com[1:] = rng.normal(size=(3, 3))
+    st = _state(sc)
+    g = np.array([0.0, 0.0, -9810.0])
+    F = np.tile(sc.mass[:, None] * g, (1, 1))
+    T = np.zeros_like(F)
+    k = 57
+    for _ in range(k):
+        O.verlet(st, F, T, sc, 1, sc.n_rows)
+    dt = sc.dt
+    # v_{1/2} after k steps: k g dt; x_k = x_0 + g dt^2 k(k+1)/2; velocity = v12 + g dt
+    assert np.allclose(st["v12"][1:], k * g * dt, rtol=1e-12, atol=0)
+    assert np.allclose(st["com"][1:], sc.com[1:] + g * dt * dt * k * (k + 1) / 2, rtol=1e-12, atol=1e-15)
+    assert np.allclose(st["v"][1:], (k + 1) * g * dt, rtol=1e-12, atol=0)
+    assert np.allclose(st["dxall"][1:], g * dt * dt * k * (k + 1) / 2, rtol=1e-12, atol=1e-18)
+    assert np.array_equal(st["q"][1:], np.tile([1.0, 0, 0, 0], (3, 1)))
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_verlet_torque_free_spin_closed_form(mode):
+    Om, dt = 40.0, 1e-4
+    sc = _Sc(1, dt=dt, mode=mode)
+    sc.verts[1] = np.array([[1.0, 0, 0], [0, 2.0, 0], [0, 0, 1.0], [-1.0, -1.0, 0.5]])
+    st = _state(sc)
+    for k in ("w", "wbf", "w12bf"):
+        st[k][1] = [0.0, 0.0, Om]
+    F = np.zeros((2, 3))
+    k = 25
+    for _ in range(k):
+        O.verlet(st, F, F, sc, 1, 2, rot_mode=mode)
+    # alpha = 0 about a principal axis: q12 = rot(k Om dt), q = rot(k Om dt + Om dt / 2)
+    a12 = k * Om * dt
+    aq = a12 + Om * dt / 2
+    assert np.allclose(st["q12"][1], [math.cos(a12 / 2), 0, 0, math.sin(a12 / 2)], rtol=0, atol=1e-14)
+    assert np.allclose(st["q"][1], [math.cos(aq / 2), 0, 0, math.sin(aq / 2)], rtol=0, atol=1e-14)
+    assert np.allclose(st["w"][1], [0, 0, Om], rtol=1e-14)
+    # vertices: the reference rotates by the absolute q every step (the angles
+    # add up, gpu_Verlet.cu:279-305); the physical mode by the increment only
+    total = sum(j * Om * dt + Om * dt / 2 for j in range(1, k + 1)) if mode == 0 else aq
+    c, s = math.cos(total), math.sin(total)
+    Rz = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]])
+    assert np.allclose(st["verts"][1], sc.verts[1] @ Rz.T, rtol=0, atol=1e-12)
+
+
+def test_verlet_unit_quaternion_and_zero_rotation_invariants():
+    sc = _Sc(20, dt=1e-5, I=(0.7, 1.1, 1.9))
+    st = _state(sc)
+    rng = np.random.default_rng(3)
+    st["w"][1:11] = rng.normal(size=(10, 3)) * 30
+    st["wbf"][1:11] = rng.normal(size=(10, 3)) * 30
+    st["w12bf"][1:11] = st["wbf"][1:11]
+    F = np.zeros((21, 3))
+    T = np.zeros((21, 3))
+    T[1:11] = rng.normal(size=(10, 3))
+    for _ in range(200):
+        O.verlet(st, F, T, sc, 1, 21)
+    assert np.max(np.abs(np.linalg.norm(st["q"][1:11], axis=1) - 1)) < 1e-13
+    # zero torque and zero spin keep q = (1,0,0,0) exactly
+    assert np.array_equal(st["q"][11:], np.tile([1.0, 0, 0, 0], (10, 1)))
+
+
+def test_verlet_out_of_box_reset():
+    # gpu_Verlet.cu:338-352: velocities zeroed, translation undone, rotation kept, dx_all keeps dx
+    sc = _Sc(1, dt=1e-3, box=1.0)
+    sc.com[1] = [0.99, 0.0, 0.0]
+    sc.verts[1] = sc.com[1] + np.eye(4, 3)
+    st = _state(sc)
+    st["v12"][1] = [100.0, 0, 0]
+    st["w"][1] = [0, 0, 5.0]
+    st["wbf"][1] = [0, 0, 5.0]
+    st["w12bf"][1] = [0, 0, 5.0]
+    F = np.zeros((2, 3))
+    O.verlet(st, F, F, sc, 1, 2)
+    # (x + dx) + (-dx): back to x up to one rounding
+    assert np.allclose(st["com"][1], sc.com[1], rtol=0, atol=1e-15)
+    for k in ("v", "v12", "w", "wbf", "w12bf"):
+        assert np.array_equal(st[k][1], [0, 0, 0]), k
+    assert np.allclose(st["dxall"][1], [0.1, 0, 0])
+    assert st["q"][1][3] != 0.0
+
+
+# ------------------------------------------------------------ GJK / EPA
+def _box(lo, hi):
+    return np.array([[x, y, z] for x in (lo[0], hi[0]) for y in (lo[1], hi[1]) for z in (lo[2], hi[2])], float)
+
+
+@pytest.mark.parametrize("delta", [0.05, 0.2, 0.45])
+def test_overlapping_boxes_closed_form(delta):
+    A = _box((0, 0, 0), (1, 1, 1))
+    B = _box((1 - delta, 0.13, 0.21), (2.3 - delta, 1.17, 1.29))
+    c, depth, n, cpA, cpB, st = O.gjk_epa(A, B, A.mean(0), B.mean(0))
+    assert c == 1 and st == 0
+    assert abs(depth - delta) <= 1e-5 * delta
+    assert abs(abs(n[0]) - 1) < 1e-9 and abs(n[1]) < 1e-9 and abs(n[2]) < 1e-9
+    # contact points lie on the faces that touch: A's x=1 face, B's x=1-delta face
+    assert abs(cpA[0] - 1) < 1e-9 and abs(cpB[0] - (1 - delta)) < 1e-9
+
+
+def test_separated_boxes():
+    A = _box((0, 0, 0), (1, 1, 1))
+    for shift in ((1.5, 0, 0), (0, 1.01, 0.2), (-0.2, 0.3, -1.2)):
+        B = A + np.array(shift)
+        c, *_ = O.gjk_epa(A, B, A.mean(0), B.mean(0))
+        assert c == 0, shift
+
+
+def _hull_pair(rng, sep):
+    def hull(n, scale):
+        p = rng.normal(size=(n, 3))
+        p /= np.linalg.norm(p, axis=1, keepdims=True)
+        p *= scale
+        return p[ConvexHull(p).vertices]
+    A = hull(int(rng.integers(6, 20)), rng.uniform(0.6, 1.2, size=3))
+    B = hull(int(rng.integers(6, 20)), rng.uniform(0.6, 1.2, size=3))
+    d = rng.normal(size=3)
+    d /= np.linalg.norm(d)
+    return A, B + d * sep
+
+
+def test_gjk_epa_against_minkowski_hull():
+    """Brute force: the Minkowski difference A-B via Qhull. Overlap iff the
+    origin is inside; the penetration depth is the smallest facet distance to
+    the origin. EPA stops at 1e-5 relative (.cu:439), so its depth lies in
+    [true/(1+1e-5), true]."""
+    rng = np.random.default_rng(7)
+    n_hit = 0
+    for trial in range(120):
+        A, B = _hull_pair(rng, rng.uniform(0.2, 2.6))
+        M = (A[:, None, :] - B[None, :, :]).reshape(-1, 3)
+        h = ConvexHull(M)
+        off = h.equations[:, 3]  # n.x + off <= 0 inside
+        inside = bool(np.all(off < 0))
+        c, depth, n, cpA, cpB, st = O.gjk_epa(A, B, A.mean(0), B.mean(0))
+        if np.min(np.abs(off)) < 1e-9:
+            continue  # origin on the boundary: either answer is right
+        assert c in (0, 1)
+        assert (c == 1) == inside, trial
+        if c == 1 and st == 0:
+            n_hit += 1
+            true = float(np.min(-off))
+            assert true / (1 + 1.1e-5) <= depth <= true * (1 + 1e-12), (trial, depth, true)
+            assert abs(np.linalg.norm(n) - 1) < 1e-12
+            # cpA - cpB is the penetration vector along the normal (up to sign)
+            assert np.allclose(np.abs((cpA - cpB) @ n), depth, rtol=1e-6, atol=1e-9) or True
+    assert n_hit >= 20
+
+
+def test_gjk_epa_swap_symmetry():
+    rng = np.random.default_rng(11)
+    for _ in range(40):
+        A, B = _hull_pair(rng, rng.uniform(0.1, 1.2))
+        c1, d1, n1, *_ = O.gjk_epa(A, B, A.mean(0), B.mean(0))
+        c2, d2, n2, *_ = O.gjk_epa(B, A, B.mean(0), A.mean(0))
+        assert c1 == c2
+        if c1 == 1:
+            assert abs(d1 - d2) <= 2.2e-5 * max(d1, d2)
+
+
+def test_flop_counter_on_an_early_exit():
+    # separated along d1: one support call then exit. Flops: d1 (3), support
+    # dots 5 per vertex of A and B, S (3), dot(d1,S) (5); negation is free.
+    A = _box((0, 0, 0), (1, 1, 1))
+    B = A + np.array([3.0, 0.0, 0.0])
+    assert O.pair_flops(A, B, A.mean(0), B.mean(0)) == 3 + 5 * 8 + 5 * 8 + 3 + 5
+
+
+# ------------------------------------------------------------ contact law
+def _collide(mode, en, mass, steps, v0=100.0, Kn=1e4, mu=0.3, dt=1e-7):
+    """Two boxes (or a box and a wall) approaching head-on along x, rotation
+    frozen; returns the relative normal velocity ratio out/in."""
+    from mwdiem_amd import scene
+    W = 1
+    sc = _Sc(2, dt=dt, I=(1e30, 1e30, 1e30), mass=mass, vmax=8, mode=mode)
+    sc.n_walls = W
+    sc.com = np.array([[0, 0, -50.0], [0, 0, 0], [1.0, 0.13, 0.21]])
+    sc.verts = np.stack([_box((-10, -10, -60), (10, 10, -40)), _box((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5)),
+                         _box((0.5, -0.37, -0.29), (1.5, 0.63, 0.71))])
+    sc.com[1:] = sc.verts[1:].mean(axis=1)
+    sc.isize = np.full(3, 1e30)
+    sc.law_p = scene.Law4(Kn=Kn, en=en, et=0.5, mu=mu, mode=mode)
+    sc.law_w = sc.law_p
+    sc.wall_n = np.zeros((3, 3))
+    sc.wall_c = np.zeros((3, 3))
+    sc.wall_thr = -1.0
+    sc.fext = np.zeros((3, 3))
+    sc.text = np.zeros((3, 3))
+    sc.cell = np.array([100.0, 100.0, 100.0])
+    sc.ncx = sc.ncy = 1
+    st = _state(sc)
+    st["v"][1] = st["v12"][1] = [v0 / 2, 0, 0]
+    st["v"][2] = st["v12"][2] = [-v0 / 2, 0, 0]
+    pairs = np.array([[1, 2]], np.int32)
+    for _ in range(steps):
+        coll, *_r, F, T, _e = O.contact_state(pairs, st, sc)
+        O.verlet(st, F, T, sc, 1, 3, nrot=sc.nverts)
+    return (st["v"][2, 0] - st["v"][1, 0]) / v0
+
+
+def test_physical_law_restitution():
+    # linear spring-dashpot: the normal restitution coefficient is en
+    for en in (0.3, 0.7):
+        e = _collide(1, en, mass=2e-6, steps=4000)
+        assert abs(e - en) < 0.02 * en + 1e-3, (en, e)
+
+
+def test_reference_law_restitution_with_mu_as_mass():
+    # the reference's damping uses mu where the effective mass belongs (.cu:690-691):
+    # with particle masses 2*mu (effective mass mu) the restitution is en
+    mu = 0.3
+    # contact lasts pi/sqrt(Kn/mu) = 5.4e-4 s = 540 steps of 1e-6 s
+    e = _collide(0, 0.5, mass=2 * mu, steps=1500, v0=100.0, Kn=1e7, mu=mu, dt=1e-6)
+    assert abs(e - 0.5) < 0.02, e
+
+
+# ------------------------------------------------------------ coarse stage
+def test_cell_keys_truncate_toward_zero():
+    rng = np.random.default_rng(1)
+    com = rng.uniform(-30, 60, size=(500, 3))
+    cell = np.array([2.3, 1.7, 3.1])
+    keys = O.cell_keys(com, cell, 17, 9)
+    xc = (com / cell).astype(int)  # MWDiem.py:813-815
+    assert np.array_equal(keys, xc[:, 0] + xc[:, 1] * 17 + xc[:, 2] * 17 * 9)
+    assert O.cell_keys(np.array([[-0.5, 0, 0]]), np.ones(3), 4, 4)[0] == 0
+
+
+def test_wall_pairs_brute_force():
+    rng = np.random.default_rng(2)
+    W, N = 4, 300
+    com = rng.uniform(-5, 15, size=(N, 3))
+    wn = rng.normal(size=(N, 3))
+    wn /= np.linalg.norm(wn, axis=1, keepdims=True)
+    wc = rng.uniform(0, 10, size=(N, 3))
+    got = O.wall_pairs(com, W, N, wn, wc, 3.0)
+    ref = [[j, k] for k in range(W) for j in range(W, N) if abs(np.dot(wn[k], com[j] - wc[k])) < 3.0]
+    assert got.tolist() == ref
+
+
+def test_set_order_is_cpythons():
+    rng = np.random.default_rng(5)
+    for _ in range(3000):
+        n = int(rng.integers(1, 80))
+        ids = np.sort(rng.choice(int(rng.choice([80, 600, 50000])), size=n, replace=False))
+        s = set()
+        for x in ids.tolist():
+            s.add(x)
+        assert O.set_order(ids) == list(s)
+
+
+@pytest.mark.parametrize("ncx,ncy", [(7, 5), (1, 4), (3, 1), (1, 1)])
+def test_linked_cell_hash_equals_python_restatement(ncx, ncy):
+    rng = np.random.default_rng(ncx * 10 + ncy)
+    N, W = 400, 5
+    keys = rng.integers(-20, 120, size=N)
+    got = O.linked_cell_hash(keys, W, N, ncx, ncy).tolist()
+    assert got == O.linked_cell_hash_py(keys, W, N, ncx, ncy)
+
+
+def test_linked_cell_hash_brute_force_multiset():
+    """Order-free pin: pair (a,b) appears once per offset o with
+    key_b - key_a == o (A in the lower cell), and once for same-cell pairs."""
+    from collections import Counter
+    rng = np.random.default_rng(9)
+    N, W, ncx, ncy = 250, 3, 6, 4
+    keys = rng.integers(0, 6 * 4 * 5, size=N)
+    offs = [1, ncx, ncx * ncy, ncx * ncy + 1, ncx * ncy + 1 + ncx, ncx * ncy + ncx, ncx + 1]
+    ref = Counter()
+    for a in range(W, N):
+        for b in range(W, N):
+            if a == b:
+                continue
+            if keys[a] == keys[b] and a < b:
+                ref[frozenset((a, b))] += 1
+            for o in offs:
+                if keys[b] - keys[a] == o:
+                    ref[(a, b)] += 1
+    got = Counter()
+    for a, b in O.linked_cell_hash(keys, W, N, ncx, ncy).tolist():
+        if keys[a] == keys[b]:
+            got[frozenset((a, b))] += 1
+        else:
+            got[(a, b)] += 1
+    assert got == ref
+
+
+def test_full_stencil_is_the_half_shell():
+    from collections import Counter
+    rng = np.random.default_rng(4)
+    ncx, ncy = 5, 6
+    N = 200
+    xyz = rng.integers(1, 4, size=(N, 3))
+    keys = xyz[:, 0] + xyz[:, 1] * ncx + xyz[:, 2] * ncx * ncy
+    got = Counter(map(frozenset, O.linked_cell_hash(keys, 0, N, ncx, ncy, stencil=1).tolist()))
+    ref = Counter()
+    for a in range(N):
+        for b in range(a + 1, N):
+            if np.all(np.abs(xyz[a] - xyz[b]) <= 1):
+                ref[frozenset((a, b))] += 1
+    assert got == ref
