@@ -81,6 +81,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-graph", action="store_true", help="launch every step eagerly instead of graph replays")
+    ap.add_argument("--graph-steps", type=int, default=20, help="steps captured per HIP graph")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
@@ -103,15 +105,22 @@ def main():
         S.step(1)
     S.check()
     torch.cuda.synchronize()
+    if not args.no_graph:
+        S.capture(args.graph_steps)   # records graph_steps steps; capture runs nothing
 
-    # timed region: K full steps, stage events on the same stream
-    S.set_timing(True)
+
+    # timed region: K full steps
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    S.step(args.steps)
+    S.run(args.steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     stats = S.check()
+    # per-stage durations: HIP events between the stages on the stepper's
+    # stream, over further eager steps (events cannot sit inside the graph)
+    n_stage = min(args.steps, 500)
+    S.set_timing(True)
+    S.step(n_stage)
     st = S.stage_times()
     S.set_timing(False)
     ms_step = 1e3 * wall / args.steps

@@ -99,7 +99,13 @@ typedef struct mwd_state {
     const double *mass, *I, *Iinv, *isize;     /* [n], [n,3,3], [n,3,3], [n] */
     const double *fext, *text;                 /* [n,3] external force/torque */
     const double *wall_n, *wall_c;             /* [n,3] wall normal and centre (rows of walls) */
+    double *radius;                            /* [n] bounding radius about com (optional, see below) */
 } mwd_state;
+/* radius: when given, radius[r] must bound max |vertex - com| of row r with a
+ * margin (mwd_radius_init writes such values; mwd_step keeps them current for
+ * moving rows). GJK then decides pairs whose bounding spheres are apart without
+ * reading vertices: for them the reference's first support test
+ * dot(d1, S1) < 0 (.cu:1013) holds, so the result (collision 0) is the same. */
 
 typedef struct mwd_workspace mwd_workspace;
 
@@ -199,6 +205,11 @@ int mwd_step(mwd_workspace* ws, mwd_state* state);
 #define MWD_N_STAGES 7
 int mwd_set_timing(mwd_workspace* ws, int enable);
 int mwd_stage_times(mwd_workspace* ws, double* ms, int64_t* steps);
+
+/* radius[r] = max_k |verts[r,k] - com[r]| * (1 + 1e-9) + 1e-9 * (1 + max|com[r]|)
+ * for rows [0, n). */
+int mwd_radius_init(const double* com, const double* verts, const int32_t* nverts, int64_t n, int32_t vmax,
+                    double* radius, void* stream);
 
 /* Counts and flags of the last step (synchronises the workspace stream). */
 int mwd_step_status(mwd_workspace* ws, mwd_status* out);

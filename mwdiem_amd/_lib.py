@@ -63,7 +63,7 @@ class State(C.Structure):
         ("com", _P), ("v", _P), ("v12", _P), ("w", _P), ("wbf", _P), ("w12bf", _P),
         ("q", _P), ("q12", _P), ("qall", _P), ("dxall", _P), ("verts", _P), ("nverts", _P),
         ("mass", _P), ("I", _P), ("Iinv", _P), ("isize", _P), ("fext", _P), ("text", _P),
-        ("wall_n", _P), ("wall_c", _P),
+        ("wall_n", _P), ("wall_c", _P), ("radius", _P),
     ]
 
 
@@ -95,6 +95,7 @@ _SIGS = {
     "mwd_step_status": (C.c_int, [_P, C.POINTER(Status)]),
     "mwd_clear_flags": (C.c_int, [_P]),
     "mwd_set_timing": (C.c_int, [_P, C.c_int]),
+    "mwd_radius_init": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int32, _P, _P]),
     "mwd_stage_times": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
 }
 

@@ -25,10 +25,7 @@ int bits_for(int64_t span) {
     }
     return b;
 }
-int passes_for(int64_t span) {
-    const int b = bits_for(span);
-    return b == 0 ? 1 : (b + 7) / 8;
-}
+
 
 template <typename T>
 int dalloc(T** p, int64_t count) {
@@ -44,7 +41,7 @@ int dalloc(T** p, int64_t count) {
 
 void free_ws(mwd_workspace* ws) {
     if (!ws) return;
-    void* ptrs[] = {ws->scenes_dev, ws->plist, ws->scene_of, ws->is_wall, ws->keys, ws->keys_alt, ws->vals,
+    void* ptrs[] = {ws->scenes_dev, ws->cell_of, ws->plist, ws->scene_of, ws->is_wall, ws->keys, ws->keys_alt, ws->vals,
                     ws->vals_alt, ws->head, ws->head_off, ws->cell_start, ws->cell_key, ws->cell_nb, ws->row_first,
                     ws->row_rank, ws->cell_by_rank, ws->cell_cnt, ws->cell_off, ws->perm, ws->wall_flag, ws->wall_off,
                     ws->pairs, ws->collision, ws->simplex, ws->pair_flag, ws->contact_off, ws->contact_pair,
@@ -68,6 +65,7 @@ BodyView state_view(const mwd_workspace* ws, const mwd_state* st, int side) {
     b.quat = st->q;
     b.isize = st->isize;
     b.mass = st->mass;
+    b.radius = st->radius;
     b.is_wall = ws->is_wall;
     b.scene = ws->scene_of;
     b.rows = ws->pairs + side;
@@ -276,12 +274,13 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     ws->n_moving = M;
     ws->n_wall_tests = wt[n_scenes];
     ws->key_span_total = kbase;
-    ws->key_passes = passes_for(kbase);
-    ws->row_passes = passes_for(n_rows);
+    ws->key_bits = bits_for(kbase);
+    ws->row_bits = bits_for(n_rows);
     const int64_t rec_cap = 2 * contact_cap;
     const int64_t max_sort = std::max(M, rec_cap);
     const int64_t tiles = mwd_sort_tiles(max_sort);
-    const int64_t max_scan = std::max({M, n_rows, pair_cap, ws->n_wall_tests, 256 * tiles});
+    const int64_t hist_len = (int64_t(1) << MWD_SORT_MAX_BITS) * tiles;
+    const int64_t max_scan = std::max({M, n_rows, pair_cap, ws->n_wall_tests, hist_len});
     ws->scan_tmp_len = mwd_scan_tmp_len(max_scan);
 
     int rc = 0;
@@ -309,6 +308,9 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     A(cell_cnt, M);
     A(cell_off, M);
     A(perm, M);
+    if (ws->key_span_total <= (int64_t(1) << 28)) {
+        A(cell_of, ws->key_span_total);
+    }
     A(wall_flag, ws->n_wall_tests);
     A(wall_off, ws->n_wall_tests);
     A(pairs, 2 * pair_cap);
@@ -331,8 +333,8 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     A(rec_val_alt, rec_cap);
     A(seg_begin, n_rows);
     A(seg_end, n_rows);
-    A(hist, 256 * tiles);
-    A(hist_off, 256 * tiles);
+    A(hist, hist_len);
+    A(hist_off, hist_len);
     A(scan_tmp, ws->scan_tmp_len);
     A(counters, 8);
     A(flags, 1);
@@ -354,7 +356,10 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     H2D(laws, laws.data(), 2 * n_scenes);
     H2D(scene_wt_off, wt.data(), n_scenes + 1);
 #undef H2D
-    if (hipMemsetAsync(ws->counters, 0, sizeof(int64_t) * 8, s) != hipSuccess ||
+    if ((ws->cell_of &&
+         hipMemsetAsync(ws->cell_of, 0xFF, sizeof(int32_t) * ws->key_span_total, s) != hipSuccess) ||
+        hipMemsetAsync(ws->counters, 0, sizeof(int64_t) * 8, s) != hipSuccess ||
+        hipMemsetAsync(ws->scan_tmp, 0, sizeof(int64_t) * ws->scan_tmp_len, s) != hipSuccess ||
         hipMemsetAsync(ws->flags, 0, sizeof(uint32_t), s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
         mwd_set_error("mwd_workspace_create: initialisation failed");
@@ -438,7 +443,7 @@ MWD_API int mwd_fine_contact(mwd_workspace* ws, const mwd_state* st) {
     uint32_t* ks;
     int32_t* vs;
     rc = mwd_radix_sort(ws->rec_key, ws->rec_val, ws->rec_key_alt, ws->rec_val_alt, 2 * ws->contact_cap, ctr + 5,
-                        ws->row_passes, ws->hist, ws->hist_off, ws->scan_tmp, ws->scan_tmp_len, s, &ks, &vs);
+                        ws->row_bits, ws->hist, ws->hist_off, ws->scan_tmp, ws->scan_tmp_len, s, &ks, &vs);
     if (rc) return rc;
     ws->rec_key_sorted = ks;
     ws->rec_val_sorted = vs;
@@ -492,6 +497,15 @@ MWD_API int mwd_step(mwd_workspace* ws, mwd_state* st) {
     return MWD_OK;
 }
 
+MWD_API int mwd_radius_init(const double* com, const double* verts, const int32_t* nverts, int64_t n, int32_t vmax,
+                            double* radius, void* stream) {
+    if (!com || !verts || !nverts || !radius || n < 0 || vmax <= 0) {
+        mwd_set_error("mwd_radius_init: invalid argument");
+        return MWD_E_INVALID;
+    }
+    return mwd_launch_radius(com, verts, nverts, n, vmax, radius, (hipStream_t)stream);
+}
+
 MWD_API int mwd_step_status(mwd_workspace* ws, mwd_status* out) {
     if (!ws || !out) return MWD_E_INVALID;
     int64_t c[8];
@@ -543,7 +557,8 @@ MWD_API int mwd_contact_pairs(int64_t P, int32_t vmax, const double* comA, const
     mwd_law* laws = nullptr;
     const int64_t rec = 2 * P;
     const int64_t tiles = mwd_sort_tiles(rec);
-    const int64_t tmp_len = mwd_scan_tmp_len(std::max({P, rec, 256 * tiles, n_particles}));
+    const int64_t hist_len = (int64_t(1) << MWD_SORT_MAX_BITS) * tiles;
+    const int64_t tmp_len = mwd_scan_tmp_len(std::max({P, rec, hist_len, n_particles}));
     int rc = MWD_OK;
     std::vector<void*> owned;
     auto al = [&](auto** p, int64_t n) {
@@ -564,8 +579,8 @@ MWD_API int mwd_contact_pairs(int64_t P, int32_t vmax, const double* comA, const
     al(&sx, P);
     al(&off, P);
     al(&ctr, 8);
-    al(&hist, 256 * tiles);
-    al(&hist_off, 256 * tiles);
+    al(&hist, hist_len);
+    al(&hist_off, hist_len);
     al(&tmp, tmp_len);
     al(&dep, P);
     al(&nrm, 3 * P);
@@ -592,6 +607,7 @@ MWD_API int mwd_contact_pairs(int64_t P, int32_t vmax, const double* comA, const
     do {
         if (hipMemcpyAsync(laws, lw, sizeof(lw), hipMemcpyHostToDevice, s) != hipSuccess ||
             hipMemsetAsync(ctr, 0, 8 * sizeof(int64_t), s) != hipSuccess ||
+            hipMemsetAsync(tmp, 0, sizeof(int64_t) * tmp_len, s) != hipSuccess ||
             hipMemsetAsync(flags, 0, sizeof(uint32_t), s) != hipSuccess) {
             mwd_set_error("mwd_contact_pairs: upload failed");
             rc = MWD_E_HIP;
@@ -609,8 +625,8 @@ MWD_API int mwd_contact_pairs(int64_t P, int32_t vmax, const double* comA, const
             rc = MWD_E_INVALID;
             break;
         }
-        BodyView A{comA, vertsA, nA, vA, wA, qA, isA, nullptr, wallA, nullptr, nullptr, 1, vmax};
-        BodyView B{comB, vertsB, nB, vB, wB, qA, isB, nullptr, wallB, nullptr, nullptr, 1, vmax};
+        BodyView A{comA, vertsA, nA, vA, wA, qA, isA, nullptr, nullptr, wallA, nullptr, nullptr, 1, vmax};
+        BodyView B{comB, vertsB, nB, vB, wB, qA, isB, nullptr, nullptr, wallB, nullptr, nullptr, 1, vmax};
         if ((rc = mwd_set_i64(ctr + 3, P, s))) break;
         if ((rc = mwd_launch_zero_pairs(P, depth, normal, cpA, cpB, epa_status, disp, s))) break;
         if ((rc = mwd_launch_gjk(A, B, P, ctr + 3, coll, sx, s))) break;
@@ -633,7 +649,7 @@ MWD_API int mwd_contact_pairs(int64_t P, int32_t vmax, const double* comA, const
         if (force) {
             uint32_t* ks;
             int32_t* vs;
-            if ((rc = mwd_radix_sort(rkey, rval, rkey2, rval2, rec, ctr + 5, passes_for(n_particles), hist, hist_off,
+            if ((rc = mwd_radix_sort(rkey, rval, rkey2, rval2, rec, ctr + 5, bits_for(n_particles), hist, hist_off,
                                      tmp, tmp_len, s, &ks, &vs)))
                 break;
             if ((rc = mwd_fill_i32(sb, 0, n_particles, s))) break;

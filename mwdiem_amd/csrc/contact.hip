@@ -235,11 +235,20 @@ __global__ void __launch_bounds__(256) k_gjk(BodyView A, BodyView B, int64_t n_c
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
         const int64_t ra = view_row(A, p), rb = view_row(B, p);
+        const d3 ca = ld3(A.com + 3 * ra), cb = ld3(B.com + 3 * rb);
+        if (A.radius) {
+            // bounding spheres apart: the first support test (.cu:1013) rejects the pair
+            const double rr = A.radius[ra] + B.radius[rb];
+            if (nrm2(cb - ca) > rr * rr) {
+                collision[p] = 0;
+                continue;
+            }
+        }
         const Side sa{A.verts + (int64_t)A.vmax * 3 * ra, A.nverts[ra]};
         const Side sb{B.verts + (int64_t)B.vmax * 3 * rb, B.nverts[rb]};
         d3 S[4];
         int ia[4] = {0, 0, 0, 0}, ib[4] = {0, 0, 0, 0};
-        const int c = gjk_pair(sa, sb, ld3(A.com + 3 * ra), ld3(B.com + 3 * rb), S, ia, ib);
+        const int c = gjk_pair(sa, sb, ca, cb, S, ia, ib);
         collision[p] = c;
         if (c == 1) {
             Simplex& x = simplex[p];

@@ -64,10 +64,18 @@ __global__ void k_heads(const uint32_t* __restrict__ keys, int64_t M, int32_t* _
     head[s] = (s == 0 || keys[s] != keys[s - 1]) ? 1 : 0;
 }
 
+// Forget the previous step's cells in the key -> cell table.
+__global__ void k_clear_cells(const uint32_t* __restrict__ cell_key, const int64_t* __restrict__ n_prev,
+                              int32_t* __restrict__ cell_of) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < *n_prev) cell_of[cell_key[c]] = -1;
+}
+
 __global__ void k_cells(const uint32_t* __restrict__ keys, const int32_t* __restrict__ vals, int64_t M,
                         const int32_t* __restrict__ head, const int64_t* __restrict__ head_off,
                         const int64_t* __restrict__ n_cells, int32_t* __restrict__ cell_start,
-                        uint32_t* __restrict__ cell_key, int32_t* __restrict__ row_first) {
+                        uint32_t* __restrict__ cell_key, int32_t* __restrict__ row_first,
+                        int32_t* __restrict__ cell_of) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= M) return;
     if (head[s]) {
@@ -75,6 +83,7 @@ __global__ void k_cells(const uint32_t* __restrict__ keys, const int32_t* __rest
         cell_start[c] = (int32_t)s;
         cell_key[c] = keys[s];
         row_first[vals[s]] = 1;
+        if (cell_of) cell_of[keys[s]] = (int32_t)c;
     }
     if (s == M - 1) cell_start[*n_cells] = (int32_t)M;
 }
@@ -109,15 +118,31 @@ __device__ void set_order(const int32_t* rows, int n, int64_t row0, int32_t* out
         if (n == 1) out[0] = rows[0];
         return;
     }
-    if (n <= 4) {  // 8-slot table, no resize
-        int32_t tab[8];
+    if (n <= 4) {
+        // 8-slot table, never resized at this size; with mask 7 no linear
+        // probe fits, so each collision perturbs. Slots live in registers.
+        uint32_t occ = 0;
+        int slot[4] = {8, 8, 8, 8};
 #pragma unroll
-        for (int k = 0; k < 8; ++k) tab[k] = -1;
-        for (int k = 0; k < n; ++k) cpy_probe(tab, 7u, (int32_t)(rows[k] - row0));
+        for (int k = 0; k < 4; ++k) {
+            if (k < n) {
+                const uint64_t key = (uint64_t)(rows[k] - row0);
+                uint64_t perturb = key;
+                uint32_t i = (uint32_t)(key & 7u);
+                while ((occ >> i) & 1u) {
+                    perturb >>= 5;
+                    i = (uint32_t)((i * 5u + 1u + perturb) & 7u);
+                }
+                occ |= 1u << i;
+                slot[k] = (int)i;
+            }
+        }
         int o = 0;
+        for (int sl = 0; sl < 8; ++sl) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
-            if (tab[k] >= 0) out[o++] = (int32_t)(tab[k] + row0);
+            for (int k = 0; k < 4; ++k)
+                if (slot[k] == sl) out[o++] = rows[k];
+        }
         return;
     }
     int32_t tab[SET_TAB], tmp[SET_TAB];
@@ -158,7 +183,7 @@ __global__ void k_cell_info(const int32_t* __restrict__ vals, const int32_t* __r
                             int64_t M, const int32_t* __restrict__ scene_of, const SceneDev* __restrict__ scenes,
                             const int64_t* __restrict__ row_rank, int32_t* __restrict__ perm,
                             int32_t* __restrict__ cell_nb, int32_t* __restrict__ cell_by_rank,
-                            int32_t* __restrict__ cell_cnt, uint32_t* flags) {
+                            int32_t* __restrict__ cell_cnt, uint32_t* flags, const int32_t* __restrict__ cell_of) {
     const int64_t C = *n_cells;
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C) return;
@@ -178,7 +203,8 @@ __global__ void k_cell_info(const int32_t* __restrict__ vals, const int32_t* __r
         int64_t nb = -1;
         if (o < sc.n_off) {
             const int64_t t = lk + sc.off[o] - sc.kmin;
-            if (t >= 0 && t < sc.kspan) nb = find_cell(cell_key, C, (uint32_t)(sc.kbase + t));
+            if (t >= 0 && t < sc.kspan)
+                nb = cell_of ? cell_of[sc.kbase + t] : find_cell(cell_key, C, (uint32_t)(sc.kbase + t));
         }
         cell_nb[c * MWD_MAX_OFF + o] = (int32_t)nb;
         if (nb >= 0) cnt += (int64_t)n * (cell_start[nb + 1] - cell_start[nb]);
@@ -264,6 +290,7 @@ __global__ void k_wall_fill(const int32_t* __restrict__ flag, const int64_t* __r
 }
 
 __global__ void k_finish_pairs(int64_t* counters, int64_t cap, uint32_t* flags) {
+    counters[6] = counters[0];  // cells of this step, cleared from the key table next step
     const int64_t P = counters[1] + counters[2];
     counters[3] = P < cap ? P : cap;
     counters[7] = P;
@@ -315,7 +342,7 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
         MWD_LAUNCH_CHECK();
         uint32_t* ks;
         int32_t* vs;
-        rc = mwd_radix_sort(ws->keys, ws->vals, ws->keys_alt, ws->vals_alt, M, nullptr, ws->key_passes, ws->hist,
+        rc = mwd_radix_sort(ws->keys, ws->vals, ws->keys_alt, ws->vals_alt, M, nullptr, ws->key_bits, ws->hist,
                             ws->hist_off, ws->scan_tmp, ws->scan_tmp_len, s, &ks, &vs);
         if (rc) return rc;
         ws->keys_sorted = ks;
@@ -327,15 +354,20 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
         if (rc) return rc;
         rc = mwd_fill_i32(ws->row_first, 0, ws->n_rows, s);
         if (rc) return rc;
+        if (ws->cell_of) {
+            hipLaunchKernelGGL(k_clear_cells, dim3(blocks_for(M)), dim3(256), 0, s, ws->cell_key, ctr + 6,
+                               ws->cell_of);
+            MWD_LAUNCH_CHECK();
+        }
         hipLaunchKernelGGL(k_cells, dim3(blocks_for(M)), dim3(256), 0, s, ks, vs, M, ws->head,
-                           ws->head_off, ctr + 0, ws->cell_start, ws->cell_key, ws->row_first);
+                           ws->head_off, ctr + 0, ws->cell_start, ws->cell_key, ws->row_first, ws->cell_of);
         MWD_LAUNCH_CHECK();
         rc = mwd_scan_i32(ws->row_first, ws->row_rank, nullptr, ws->n_rows, nullptr, ws->scan_tmp,
                           ws->scan_tmp_len, s);
         if (rc) return rc;
         hipLaunchKernelGGL(k_cell_info, dim3(blocks_for(M)), dim3(256), 0, s, vs, ws->cell_start,
                            ws->cell_key, ctr + 0, M, ws->scene_of, ws->scenes_dev, ws->row_rank, ws->perm,
-                           ws->cell_nb, ws->cell_by_rank, ws->cell_cnt, ws->flags);
+                           ws->cell_nb, ws->cell_by_rank, ws->cell_cnt, ws->flags, ws->cell_of);
         MWD_LAUNCH_CHECK();
         rc = mwd_scan_i32(ws->cell_cnt, ws->cell_off, ctr + 2, M, ctr + 0, ws->scan_tmp, ws->scan_tmp_len, s);
         if (rc) return rc;

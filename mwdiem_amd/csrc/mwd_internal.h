@@ -65,6 +65,7 @@ struct BodyView {
     const double* quat;
     const double* isize;
     const double* mass;         // per row of the view (physical law only; may be NULL)
+    const double* radius;       // per row bounding radius (NULL: no sphere test)
     const uint8_t* is_wall;     // per row of the view
     const int32_t* scene;       // per row of the view (NULL: scene 0)
     const int32_t* rows;        // NULL: identity
@@ -85,7 +86,7 @@ struct mwd_workspace {
     int64_t n_moving;          // M: moving particles over all scenes
     int64_t n_wall_tests;      // sum over scenes of W_s * (n_active_s - W_s)
     int64_t key_span_total;    // sum of kspan
-    int key_passes, row_passes;
+    int key_bits, row_bits;    // significant bits of the sort keys
     // per-row tables
     int32_t* plist;            // [M] moving rows, row order
     int32_t* scene_of;         // [n_rows]
@@ -104,6 +105,7 @@ struct mwd_workspace {
     int32_t* cell_cnt;         // [M] pair count by rank
     int64_t* cell_off;         // [M]
     int32_t* perm;             // [M] members of each cell in set order
+    int32_t* cell_of;          // [key_span_total] cell index of a sorted key, -1 if empty (NULL: binary search)
     int32_t* wall_flag;        // [n_wall_tests]
     int64_t* wall_off;         // [n_wall_tests]
     // candidate pairs
@@ -128,8 +130,9 @@ struct mwd_workspace {
     int64_t* hist_off;         // [256 * max_tiles]
     int64_t* scan_tmp;         // block sums
     int64_t scan_tmp_len;
-    // counters (device): 0 n_cells, 1 n_wall_pairs, 2 n_cell_pairs, 3 n_pairs,
-    // 4 n_contacts, 5 n_records, 6 n_moving, 7 scratch
+    // counters (device): 0 n_cells, 1 n_wall_pairs, 2 n_cell_pairs, 3 n_pairs
+    // (capped), 4 n_contacts, 5 n_records, 6 n_cells of the previous step,
+    // 7 n_pairs before capping
     int64_t* counters;
     uint32_t* flags;
     mwd_law* laws;             // [2*n_scenes] (particle law, wall law)
@@ -152,11 +155,14 @@ void mwd_mark_end(mwd_workspace* ws);
 int mwd_scan_i32(const int32_t* in, int64_t* out, int64_t* total, int64_t n_cap, const int64_t* n_dev,
                  int64_t* tmp, int64_t tmp_len, hipStream_t s);
 int64_t mwd_scan_tmp_len(int64_t n_cap);
-// Stable LSD radix sort of (key, val) pairs on the low `passes`*8 key bits,
-// ping-ponging between (keys, vals) and the alt buffers; *keys_out/*vals_out
-// name the buffers holding the result (the alt ones when passes is odd).
+// Stable LSD radix sort of (key, val) pairs on the low key_bits key bits, in
+// ceil(key_bits / MWD_SORT_MAX_BITS) passes, ping-ponging between (keys,
+// vals) and the alt buffers; *keys_out/*vals_out name the buffers holding the
+// result (the alt ones when the pass count is odd). hist/hist_off hold
+// 2^MWD_SORT_MAX_BITS * mwd_sort_tiles(n_cap) entries.
+#define MWD_SORT_MAX_BITS 11
 int mwd_radix_sort(uint32_t* keys, int32_t* vals, uint32_t* keys_alt, int32_t* vals_alt, int64_t n_cap,
-                   const int64_t* n_dev, int passes, int32_t* hist, int64_t* hist_off, int64_t* tmp,
+                   const int64_t* n_dev, int key_bits, int32_t* hist, int64_t* hist_off, int64_t* tmp,
                    int64_t tmp_len, hipStream_t s, uint32_t** keys_out, int32_t** vals_out);
 int64_t mwd_sort_tiles(int64_t n_cap);
 int mwd_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
@@ -169,6 +175,8 @@ int mwd_launch_verlet(double* v, double* v12, double* w12bf, double* q12, double
                       const int32_t* nrot, double dt, const double* box, int64_t pmin, int64_t pmax,
                       hipStream_t s);
 int mwd_launch_reduce_verlet(mwd_workspace* ws, mwd_state* st);
+int mwd_launch_radius(const double* com, const double* verts, const int32_t* nverts, int64_t n, int32_t vmax,
+                      double* radius, hipStream_t s);
 int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t* keys);
 int mwd_launch_cell_keys(const double* com, int64_t n, const double* cell, int64_t ncx, int64_t ncy,
                          int64_t* keys, hipStream_t s);

@@ -86,3 +86,10 @@ MWD_HD d3 mat3v(const double* m, d3 v) {
     return mk3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
                m[6] * v.x + m[7] * v.y + m[8] * v.z);
 }
+
+// Bounding radius with a rounding margin: every quantity the support-map
+// test computes from these coordinates rounds well inside it.
+MWD_HD double inflate_radius(double r2max, d3 com) {
+    const double cm = fmax(fabs(com.x), fmax(fabs(com.y), fabs(com.z)));
+    return sqrt(r2max) * (1.0 + 1e-9) + 1e-9 * (1.0 + cm);
+}

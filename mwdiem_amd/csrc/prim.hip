@@ -47,113 +47,129 @@ __device__ __forceinline__ int64_t block_inclusive_scan(int64_t v, int64_t* lds_
     return x + off;
 }
 
-__global__ void k_scan_reduce(const int32_t* __restrict__ in, int64_t n_cap, const int64_t* n_dev,
-                              int64_t* __restrict__ bsum) {
-    __shared__ int64_t lw[SORT_WAVES];
-    const int64_t n = live_n(n_cap, n_dev);
-    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
-    int64_t acc = 0;
-    if (base < n) {
-#pragma unroll
-        for (int k = 0; k < SCAN_ITEMS; ++k) {
-            int64_t i = base + (int64_t)k * SCAN_THREADS + threadIdx.x;
-            if (i < n) acc += in[i];
-        }
-    }
-    int64_t tot;
-    block_inclusive_scan(acc, lw, &tot);
-    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
-}
+// ------------------------------------------------- single-pass scan
+// Decoupled look-back (Merrill & Garland, "Single-pass Parallel Prefix Scan
+// with Decoupled Look-back", 2016). Tiles are claimed in order through an
+// atomic counter; each publishes its aggregate, then its inclusive prefix
+// once known; successors add up predecessors' published values. Status words
+// pack a 2-bit flag above a 62-bit value. The last block to finish resets the
+// status words and counters, so the scratch is clean for the next call
+// without a separate clear.
+constexpr uint64_t ST_AGG = 1ull << 62;
+constexpr uint64_t ST_INC = 2ull << 62;
+constexpr uint64_t ST_VAL = (1ull << 62) - 1;
 
-// Exclusive scan of nb block sums in place by one 1024-thread block.
-__global__ void k_scan_blocks(int64_t* bsum, int64_t nb, int64_t* total) {
-    __shared__ int64_t part[1024];
-    const int t = threadIdx.x;
-    const int64_t per = (nb + 1023) / 1024;
-    const int64_t b0 = t * per;
-    int64_t s = 0;
-    for (int64_t i = 0; i < per; ++i) {
-        int64_t j = b0 + i;
-        if (j < nb) s += bsum[j];
-    }
-    part[t] = s;
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_dlb(const int32_t* __restrict__ in, int64_t* __restrict__ out,
+                                                           int64_t* total, int64_t n_cap, const int64_t* n_dev,
+                                                           uint64_t* status, uint32_t* ctr, int64_t ntiles) {
+    __shared__ int64_t lw[SORT_WAVES];
+    __shared__ int64_t s_prefix;
+    __shared__ uint32_t s_tile;
+    const int64_t n = live_n(n_cap, n_dev);
+    if (threadIdx.x == 0) s_tile = atomicAdd(&ctr[0], 1u);
     __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {
-        int64_t y = (t >= d) ? part[t - d] : 0;
-        __syncthreads();
-        part[t] += y;
-        __syncthreads();
-    }
-    int64_t run = (t == 0) ? 0 : part[t - 1];
-    for (int64_t i = 0; i < per; ++i) {
-        int64_t j = b0 + i;
-        if (j < nb) {
-            int64_t v = bsum[j];
-            bsum[j] = run;
-            run += v;
-        }
-    }
-    if (t == 1023 && total) *total = part[1023];
-}
-
-__global__ void k_scan_apply(const int32_t* __restrict__ in, int64_t* __restrict__ out, int64_t n_cap,
-                             const int64_t* n_dev, const int64_t* __restrict__ bsum) {
-    __shared__ int64_t lw[SORT_WAVES];
-    const int64_t n = live_n(n_cap, n_dev);
-    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
-    if (base >= n) return;
-    // blocked arrangement: thread t owns items [base + t*ITEMS, +ITEMS)
+    const int64_t tile = s_tile;
+    const int64_t base = tile * SCAN_TILE;
     int32_t v[SCAN_ITEMS];
     int64_t acc = 0;
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) {
-        int64_t i = base + (int64_t)threadIdx.x * SCAN_ITEMS + k;
+        const int64_t i = base + (int64_t)threadIdx.x * SCAN_ITEMS + k;
         v[k] = (i < n) ? in[i] : 0;
         acc += v[k];
     }
-    int64_t incl = block_inclusive_scan(acc, lw, nullptr);
-    int64_t run = incl - acc + bsum[blockIdx.x];
+    int64_t agg;
+    const int64_t incl = block_inclusive_scan(acc, lw, &agg);
+    if (threadIdx.x == 0) {
+        int64_t prefix = 0;
+        if (tile == 0) {
+            __hip_atomic_store(&status[0], ST_INC | (uint64_t)agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(&status[tile], ST_AGG | (uint64_t)agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            int64_t j = tile - 1;
+            for (;;) {
+                const uint64_t s = __hip_atomic_load(&status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                const uint64_t f = s & ~ST_VAL;
+                if (f == 0) continue;  // predecessor not published yet (it has started: tiles are claimed in order)
+                prefix += (int64_t)(s & ST_VAL);
+                if (f == ST_INC) break;
+                --j;
+            }
+            __hip_atomic_store(&status[tile], ST_INC | (uint64_t)(prefix + agg), __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_prefix = prefix;
+        if (tile == ntiles - 1 && total) *total = prefix + agg;
+    }
+    __syncthreads();
+    int64_t run = incl - acc + s_prefix;
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) {
-        int64_t i = base + (int64_t)threadIdx.x * SCAN_ITEMS + k;
+        const int64_t i = base + (int64_t)threadIdx.x * SCAN_ITEMS + k;
         if (i < n) out[i] = run;
         run += v[k];
+    }
+    // the last block out resets the status words and the counters
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const uint32_t done = atomicAdd(&ctr[1], 1u);
+        s_tile = (done == (uint32_t)(ntiles - 1)) ? 1u : 0u;
+    }
+    __syncthreads();
+    if (s_tile) {
+        for (int64_t j = threadIdx.x; j < ntiles; j += SCAN_THREADS) status[j] = 0;
+        if (threadIdx.x == 0) {
+            ctr[0] = 0;
+            ctr[1] = 0;
+        }
     }
 }
 
 // --------------------------------------------------------------- radix sort
-__global__ void k_radix_hist(const uint32_t* __restrict__ keys, int64_t n_cap, const int64_t* n_dev, int shift,
-                             int32_t* __restrict__ hist, int64_t ntiles) {
-    __shared__ int32_t h[RADIX];
-    h[threadIdx.x] = 0;
+// LSD radix sort with digits of up to BITS bits (2^BITS bins); the digit of a
+// pass is (key >> shift) & (2^width - 1).
+template <int BITS>
+__global__ void __launch_bounds__(SORT_THREADS) k_radix_hist(const uint32_t* __restrict__ keys, int64_t n_cap,
+                                                             const int64_t* n_dev, int shift, uint32_t mask,
+                                                             int32_t* __restrict__ hist, int64_t ntiles) {
+    constexpr int NB = 1 << BITS;
+    __shared__ int32_t h[NB];
+    for (int i = threadIdx.x; i < NB; i += SORT_THREADS) h[i] = 0;
     __syncthreads();
     const int64_t n = live_n(n_cap, n_dev);
     const int64_t base = (int64_t)blockIdx.x * SORT_TILE;
     if (base < n) {
         for (int k = 0; k < SORT_ROUNDS; ++k) {
-            int64_t i = base + (int64_t)k * SORT_THREADS + threadIdx.x;
-            if (i < n) atomicAdd(&h[(keys[i] >> shift) & 0xff], 1);
+            const int64_t i = base + (int64_t)k * SORT_THREADS + threadIdx.x;
+            if (i < n) atomicAdd(&h[(keys[i] >> shift) & mask], 1);
         }
     }
     __syncthreads();
-    hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+    for (int d = threadIdx.x; d < NB; d += SORT_THREADS) hist[(int64_t)d * ntiles + blockIdx.x] = h[d];
 }
 
 // Stable scatter. Item order inside a tile is wave-major, then round, then
 // lane; ranks among equal digits follow that order, so the sort is stable.
-__global__ void k_radix_scatter(const uint32_t* __restrict__ keys, const int32_t* __restrict__ vals,
-                                uint32_t* __restrict__ keys_out, int32_t* __restrict__ vals_out, int64_t n_cap,
-                                const int64_t* n_dev, int shift, const int64_t* __restrict__ hist_off,
-                                int64_t ntiles) {
-    __shared__ int32_t cnt[SORT_WAVES][RADIX];
+template <int BITS>
+__global__ void __launch_bounds__(SORT_THREADS) k_radix_scatter(const uint32_t* __restrict__ keys,
+                                                                const int32_t* __restrict__ vals,
+                                                                uint32_t* __restrict__ keys_out,
+                                                                int32_t* __restrict__ vals_out, int64_t n_cap,
+                                                                const int64_t* n_dev, int shift, int width,
+                                                                const int64_t* __restrict__ hist_off,
+                                                                int64_t ntiles) {
+    constexpr int NB = 1 << BITS;
+    __shared__ int32_t cnt[SORT_WAVES][NB];
     const int lane = threadIdx.x & (MWD_WAVE - 1);
     const int wave = threadIdx.x / MWD_WAVE;
-    for (int i = threadIdx.x; i < SORT_WAVES * RADIX; i += SORT_THREADS) (&cnt[0][0])[i] = 0;
+    for (int i = threadIdx.x; i < SORT_WAVES * NB; i += SORT_THREADS) (&cnt[0][0])[i] = 0;
     __syncthreads();
     const int64_t n = live_n(n_cap, n_dev);
-    const int64_t base = (int64_t)blockIdx.x * SORT_TILE + (int64_t)wave * (MWD_WAVE * SORT_ROUNDS);
     if ((int64_t)blockIdx.x * SORT_TILE >= n) return;  // uniform per block
+    const int64_t base = (int64_t)blockIdx.x * SORT_TILE + (int64_t)wave * (MWD_WAVE * SORT_ROUNDS);
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (MWD_WAVE - lane));
+    const uint32_t mask = (1u << width) - 1u;
     uint32_t k[SORT_ROUNDS];
     int32_t v[SORT_ROUNDS];
     int32_t r[SORT_ROUNDS];
@@ -163,10 +179,9 @@ __global__ void k_radix_scatter(const uint32_t* __restrict__ keys, const int32_t
         const bool act = i < n;
         k[rd] = act ? keys[i] : 0u;
         v[rd] = act ? vals[i] : 0;
-        const uint32_t d = (k[rd] >> shift) & 0xffu;
+        const uint32_t d = (k[rd] >> shift) & mask;
         uint64_t match = __ballot(act);
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < width; ++b) {
             const uint64_t m = __ballot((d >> b) & 1u);
             match &= ((d >> b) & 1u) ? m : ~m;
         }
@@ -179,13 +194,11 @@ __global__ void k_radix_scatter(const uint32_t* __restrict__ keys, const int32_t
         r[rd] = act ? pre + below : -1;
     }
     __syncthreads();
-    // wave offsets per digit
-    {
-        const int d = threadIdx.x;  // SORT_THREADS == RADIX
+    for (int d = threadIdx.x; d < NB; d += SORT_THREADS) {  // wave offsets per digit
         int32_t run = 0;
 #pragma unroll
         for (int w = 0; w < SORT_WAVES; ++w) {
-            int32_t t = cnt[w][d];
+            const int32_t t = cnt[w][d];
             cnt[w][d] = run;
             run += t;
         }
@@ -194,7 +207,7 @@ __global__ void k_radix_scatter(const uint32_t* __restrict__ keys, const int32_t
 #pragma unroll
     for (int rd = 0; rd < SORT_ROUNDS; ++rd) {
         if (r[rd] < 0) continue;
-        const uint32_t d = (k[rd] >> shift) & 0xffu;
+        const uint32_t d = (k[rd] >> shift) & mask;
         const int64_t pos = hist_off[(int64_t)d * ntiles + blockIdx.x] + cnt[wave][d] + r[rd];
         keys_out[pos] = k[rd];
         vals_out[pos] = v[rd];
@@ -211,7 +224,7 @@ __global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
 
 }  // namespace
 
-int64_t mwd_scan_tmp_len(int64_t n_cap) { return (n_cap + SCAN_TILE - 1) / SCAN_TILE + 1; }
+int64_t mwd_scan_tmp_len(int64_t n_cap) { return (n_cap + SCAN_TILE - 1) / SCAN_TILE + 2; }
 int64_t mwd_sort_tiles(int64_t n_cap) { return (n_cap + SORT_TILE - 1) / SORT_TILE; }
 
 int mwd_scan_i32(const int32_t* in, int64_t* out, int64_t* total, int64_t n_cap, const int64_t* n_dev,
@@ -224,37 +237,50 @@ int mwd_scan_i32(const int32_t* in, int64_t* out, int64_t* total, int64_t n_cap,
         return MWD_OK;
     }
     const int64_t nb = (n_cap + SCAN_TILE - 1) / SCAN_TILE;
-    if (nb > tmp_len) {
+    // scratch layout: [0] two uint32 counters, [1 .. nb] tile status words
+    if (nb + 1 > tmp_len) {
         mwd_set_error("scan scratch too small");
         return MWD_E_CAPACITY;
     }
-    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, s, in, n_cap, n_dev, tmp);
-    MWD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, tmp, nb, total);
-    MWD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, s, in, out, n_cap, n_dev, tmp);
+    hipLaunchKernelGGL(k_scan_dlb, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, s, in, out, total, n_cap, n_dev,
+                       reinterpret_cast<uint64_t*>(tmp + 1), reinterpret_cast<uint32_t*>(tmp), nb);
     MWD_LAUNCH_CHECK();
     return MWD_OK;
 }
 
 int mwd_radix_sort(uint32_t* keys, int32_t* vals, uint32_t* keys_alt, int32_t* vals_alt, int64_t n_cap,
-                   const int64_t* n_dev, int passes, int32_t* hist, int64_t* hist_off, int64_t* tmp,
+                   const int64_t* n_dev, int key_bits, int32_t* hist, int64_t* hist_off, int64_t* tmp,
                    int64_t tmp_len, hipStream_t s, uint32_t** keys_out, int32_t** vals_out) {
     uint32_t* ka = keys;
     int32_t* va = vals;
     uint32_t* kb = keys_alt;
     int32_t* vb = vals_alt;
+    if (key_bits < 1) key_bits = 1;
+    const int passes = (key_bits + MWD_SORT_MAX_BITS - 1) / MWD_SORT_MAX_BITS;
+    const int width = (key_bits + passes - 1) / passes;
     if (n_cap > 0) {
         const int64_t nt = mwd_sort_tiles(n_cap);
+        const bool wide = width > 8;
+        const int64_t bins = wide ? (1 << MWD_SORT_MAX_BITS) : 256;
         for (int p = 0; p < passes; ++p) {
-            const int shift = 8 * p;
-            hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)nt), dim3(RADIX), 0, s, ka, n_cap, n_dev, shift, hist,
-                               nt);
+            const int shift = width * p;
+            const int w = (shift + width <= key_bits) ? width : key_bits - shift;
+            const uint32_t mask = (1u << w) - 1u;
+            if (wide)
+                hipLaunchKernelGGL(k_radix_hist<MWD_SORT_MAX_BITS>, dim3((unsigned)nt), dim3(SORT_THREADS), 0, s, ka,
+                                   n_cap, n_dev, shift, mask, hist, nt);
+            else
+                hipLaunchKernelGGL(k_radix_hist<8>, dim3((unsigned)nt), dim3(SORT_THREADS), 0, s, ka, n_cap, n_dev,
+                                   shift, mask, hist, nt);
             MWD_LAUNCH_CHECK();
-            int rc = mwd_scan_i32(hist, hist_off, nullptr, RADIX * nt, nullptr, tmp, tmp_len, s);
+            int rc = mwd_scan_i32(hist, hist_off, nullptr, bins * nt, nullptr, tmp, tmp_len, s);
             if (rc) return rc;
-            hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)nt), dim3(SORT_THREADS), 0, s, ka, va, kb, vb, n_cap,
-                               n_dev, shift, hist_off, nt);
+            if (wide)
+                hipLaunchKernelGGL(k_radix_scatter<MWD_SORT_MAX_BITS>, dim3((unsigned)nt), dim3(SORT_THREADS), 0, s,
+                                   ka, va, kb, vb, n_cap, n_dev, shift, w, hist_off, nt);
+            else
+                hipLaunchKernelGGL(k_radix_scatter<8>, dim3((unsigned)nt), dim3(SORT_THREADS), 0, s, ka, va, kb, vb,
+                                   n_cap, n_dev, shift, w, hist_off, nt);
             MWD_LAUNCH_CHECK();
             uint32_t* tk = ka; ka = kb; kb = tk;
             int32_t* tv = va; va = vb; vb = tv;

@@ -21,6 +21,7 @@ struct VerletArrays {
     double* com;
     const double *mass, *Iinv, *I;
     double* dxall;
+    double* radius;  // optional: refreshed bounding radius of the moved vertices
 };
 
 // rot_mode 0: vertices rotated by the absolute orientation q (the reference's
@@ -74,12 +75,17 @@ __device__ __forceinline__ void verlet_row(const VerletArrays& a, int64_t P, d3 
         return c1.x > box[0] || c1.x < box[1] || c1.y > box[2] || c1.y < box[3] || c1.z > box[4] || c1.z < box[5];
     }();
     const d3 du = dx * -1.0;
+    d3 cnew = com + dx;
+    if (out) cnew = cnew + du;
+    double r2 = 0.0;
     for (int i = 0; i < nrot; ++i) {
         d3 r = qrot(qr, ld3(vp + 3 * i) - com, qc);
         d3 nv = (r + com) + dx;
         if (out) nv = nv + du;
         st3(vp + 3 * i, nv);
+        r2 = fmax(r2, nrm2(nv - cnew));
     }
+    if (a.radius) a.radius[P] = inflate_radius(r2, cnew);
     com = com + dx;
     if (out) {
         vel = mk3(0.0, 0.0, 0.0);
@@ -156,7 +162,7 @@ int mwd_launch_verlet(double* v, double* v12, double* w12bf, double* q12, double
                       const int32_t* nrot, double dt, const double* box, int64_t pmin, int64_t pmax,
                       hipStream_t s) {
     if (pmax <= pmin) return MWD_OK;
-    VerletArrays a{v, v12, w12bf, q12, q, qall, wbf, w, verts, com, mass, Iinv, I, dxall};
+    VerletArrays a{v, v12, w12bf, q12, q, qall, wbf, w, verts, com, mass, Iinv, I, dxall, nullptr};
     Box6 b;
     for (int k = 0; k < 6; ++k) b.b[k] = box[k];
     const int64_t n = pmax - pmin;
@@ -169,11 +175,32 @@ int mwd_launch_verlet(double* v, double* v12, double* w12bf, double* q12, double
 int mwd_launch_reduce_verlet(mwd_workspace* ws, mwd_state* st) {
     if (ws->n_moving <= 0) return MWD_OK;
     VerletArrays a{st->v, st->v12, st->w12bf, st->q12, st->q, st->qall, st->wbf, st->w, st->verts, st->com,
-                   st->mass, st->Iinv, st->I, st->dxall};
+                   st->mass, st->Iinv, st->I, st->dxall, st->radius};
     const int64_t M = ws->n_moving;
     hipLaunchKernelGGL(k_reduce_verlet, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, ws->stream, a, ws->plist, M,
                        ws->scene_of, ws->scenes_dev, ws->seg_begin, ws->seg_end, ws->rec_val, ws->Fn, ws->tA,
                        ws->tB, st->fext, st->text, st->nverts, (int)st->vmax, ws->dt);
+    MWD_LAUNCH_CHECK();
+    return MWD_OK;
+}
+
+namespace {
+__global__ void k_radius(const double* __restrict__ com, const double* __restrict__ verts,
+                         const int32_t* __restrict__ nverts, int64_t n, int vmax, double* __restrict__ radius) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const d3 c = ld3(com + 3 * r);
+    double r2 = 0.0;
+    for (int i = 0; i < nverts[r]; ++i) r2 = fmax(r2, nrm2(ld3(verts + ((int64_t)vmax * r + i) * 3) - c));
+    radius[r] = inflate_radius(r2, c);
+}
+}  // namespace
+
+int mwd_launch_radius(const double* com, const double* verts, const int32_t* nverts, int64_t n, int32_t vmax,
+                      double* radius, hipStream_t s) {
+    if (n <= 0) return MWD_OK;
+    hipLaunchKernelGGL(k_radius, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, com, verts, nverts, n, (int)vmax,
+                       radius);
     MWD_LAUNCH_CHECK();
     return MWD_OK;
 }

@@ -36,7 +36,7 @@ class Stepper:
     """
 
     def __init__(self, scenes: SceneData | Sequence[SceneData], device: int = 0, pair_cap: int | None = None,
-                 contact_cap: int | None = None, stream: torch.cuda.Stream | None = None):
+                 contact_cap: int | None = None, stream: torch.cuda.Stream | None = None, sphere_test: bool = True):
         _require_gpu()
         if isinstance(scenes, SceneData):
             scenes = [scenes]
@@ -45,7 +45,8 @@ class Stepper:
             raise ValueError("all scenes of a Stepper share one dt")
         self.dt = self.scenes[0].dt
         self.device = torch.device("cuda", device)
-        self.stream = stream or torch.cuda.current_stream(self.device)
+        # a stream of its own (default priority): steps can be captured in a graph
+        self.stream = stream or torch.cuda.Stream(device=self.device)
         self.vmax = max(sc.vmax for sc in self.scenes)
         rows = [sc.n_rows for sc in self.scenes]
         self.row0 = np.concatenate([[0], np.cumsum(rows)[:-1]]).astype(np.int64)
@@ -101,8 +102,11 @@ class Stepper:
                                                 self.n_rows, self.vmax, self.pair_cap, self.contact_cap, self.dt,
                                                 self.stream.cuda_stream or None), "mwd_workspace_create")
         self.ws = ws
+        self.sphere_test = sphere_test
+        self.t["radius"] = torch.zeros(self.n_rows, dtype=torch.float64, device=dev)
         self.state = _lib.State()
         self._fill_state()
+        self.refresh_radius()
         self.steps_done = 0
 
     # ------------------------------------------------------------ plumbing
@@ -112,6 +116,15 @@ class Stepper:
         s.vmax = self.vmax
         for k in STATE_KEYS + CONST_KEYS:
             setattr(s, k, self.t[k].data_ptr())
+        s.radius = self.t["radius"].data_ptr() if self.sphere_test else None
+
+    def refresh_radius(self):
+        """Recompute the bounding radii from the current vertices (after set_state)."""
+        self._enter()
+        _lib.check(_lib.load().mwd_radius_init(self.t["com"].data_ptr(), self.t["verts"].data_ptr(),
+                                               self.t["nverts"].data_ptr(), self.n_rows, self.vmax,
+                                               self.t["radius"].data_ptr(), self.stream.cuda_stream or None),
+                   "mwd_radius_init")
 
     def close(self):
         if getattr(self, "ws", None):
@@ -134,11 +147,44 @@ class Stepper:
     def reduce_verlet(self):
         _lib.check(_lib.load().mwd_reduce_verlet(self.ws, C.byref(self.state)), "mwd_reduce_verlet")
 
+    def _enter(self):
+        self.stream.wait_stream(torch.cuda.current_stream(self.device))
+
     def step(self, n: int = 1):
+        """n steps, launched eagerly on the stepper's stream."""
         lib = _lib.load()
+        self._enter()
         for _ in range(n):
             _lib.check(lib.mwd_step(self.ws, C.byref(self.state)), "mwd_step")
         self.steps_done += n
+
+    def capture(self, steps_per_graph: int = 10):
+        """Capture steps_per_graph consecutive steps in one HIP graph."""
+        self._enter()
+        torch.cuda.synchronize(self.device)
+        g = torch.cuda.CUDAGraph()
+        lib = _lib.load()
+        with torch.cuda.graph(g, stream=self.stream):
+            for _ in range(steps_per_graph):
+                _lib.check(lib.mwd_step(self.ws, C.byref(self.state)), "mwd_step")
+        torch.cuda.synchronize(self.device)
+        self._graph = g
+        self._graph_steps = steps_per_graph
+        return g
+
+    def run(self, n: int):
+        """n steps: graph replays for the bulk (when captured), eager steps for the rest."""
+        g = getattr(self, "_graph", None)
+        done = 0
+        if g is not None:
+            self._enter()
+            with torch.cuda.stream(self.stream):
+                while n - done >= self._graph_steps:
+                    g.replay()
+                    done += self._graph_steps
+            self.steps_done += done
+        if n > done:
+            self.step(n - done)
 
     def set_timing(self, on: bool = True):
         _lib.check(_lib.load().mwd_set_timing(self.ws, 1 if on else 0), "mwd_set_timing")
@@ -202,8 +248,11 @@ class Stepper:
 
     def set_state(self, st: dict):
         """Overwrite state tensors from numpy arrays (rows of all scenes)."""
+        torch.cuda.synchronize(self.device)
         for k, a in st.items():
             self.t[k].copy_(torch.from_numpy(np.ascontiguousarray(a)).to(self.device))
+        torch.cuda.synchronize(self.device)
+        self.refresh_radius()
 
 
 # ------------------------------------------------------------ raw copies
