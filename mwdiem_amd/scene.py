@@ -87,6 +87,27 @@ class SceneData:
         import copy
         return copy.deepcopy(self)
 
+    _ARRAYS = ("com", "verts", "nverts", "mass", "I", "Iinv", "isize", "fext", "text", "wall_n", "wall_c", "box",
+               "cell")
+
+    def to_arrays(self) -> dict:
+        """Plain arrays (npz-friendly) that from_arrays() turns back into the scene."""
+        d = {k: getattr(self, k) for k in self._ARRAYS}
+        d["ints"] = np.array([self.n_walls, self.vmax, self.ncx, self.ncy, self.ncz, self.kmin, self.kspan], np.int64)
+        d["floats"] = np.array([self.max_particlesize, self.dt] + list(self.inner))
+        d["laws"] = np.array([[L.Kn, L.en, L.et, L.mu, L.mode] for L in (self.law_p, self.law_w)], float)
+        return d
+
+    @classmethod
+    def from_arrays(cls, d) -> "SceneData":
+        i = [int(x) for x in d["ints"]]
+        f = [float(x) for x in d["floats"]]
+        laws = [Law4(Kn=float(r[0]), en=float(r[1]), et=float(r[2]), mu=float(r[3]), mode=int(r[4]))
+                for r in np.asarray(d["laws"])]
+        kw = {k: np.array(d[k]) for k in cls._ARRAYS}
+        return cls(n_walls=i[0], vmax=i[1], ncx=i[2], ncy=i[3], ncz=i[4], kmin=i[5], kspan=i[6],
+                   max_particlesize=f[0], dt=f[1], inner=tuple(f[2:5]), law_p=laws[0], law_w=laws[1], **kw)
+
 
 # ----------------------------------------------------------------- geometry
 def ellipsoid_points(rng: np.random.Generator, n: int, axes) -> np.ndarray:

@@ -115,6 +115,23 @@ def test_fine_contact_matches_oracle(gpu, dense_scene, dense_phys, seed, phys):
     assert np.array_equal(c["epa_status"], est[idx])
 
 
+def test_fine_contact_is_bitwise(gpu, dense_scene):
+    """Same operator order, no contraction, correctly rounded division and
+    square root on both sides: the EPA results are bit-identical."""
+    sc = dense_scene
+    S = _stepper(sc)
+    st = random_motion(O.initial_state(sc), sc, 13)
+    S.set_state(st)
+    S.candidates()
+    S.fine_contact()
+    S.check()
+    c = S.contacts()
+    coll, depth, nrm, ca, cb, F, T, est = O.contact_state(S.pairs(), st, sc)
+    idx = np.nonzero(coll == 1)[0]
+    for name, ref in (("depth", depth), ("normal", nrm), ("cpA", ca), ("cpB", cb)):
+        assert np.array_equal(c[name], ref[idx]), name
+
+
 def test_step_reduction_matches_oracle(gpu, dense_scene):
     """One full step: the fused reduction + integrator sees the same forces."""
     sc = dense_scene
