@@ -72,6 +72,56 @@ def cpu_baseline(sc, budget_s: float = 12.0):
                       f"(oracle/mwdiem_oracle.cpp, g++ -O2, one thread), {dt:.1f} s"}
 
 
+def bench_batched(n_scenes: int, steps: int, plain_steps: int, graph_steps: int):
+    """The batched form: n_scenes hopper-discharge scenes (particle counts
+    log-uniform 500-20,000, per-scene friction, stiffness and restitution,
+    physical law) in one Stepper -- one launch per stage over all scenes,
+    graph-captured -- against the same scenes run one after another, each in
+    its own graph-captured Stepper."""
+    import torch
+
+    from mwdiem_amd import scene
+    from mwdiem_amd.stepper import Stepper
+
+    t0 = time.perf_counter()
+    scs = scene.config_batched(n_scenes=n_scenes)
+    n_tot = sum(sc.n_particles for sc in scs)
+    print(f"# batched: {n_scenes} scenes, {n_tot} particles, generated in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    B = Stepper(scs)
+    B.step(5)
+    B.check()
+    B.capture(graph_steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    B.run(steps)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    sb = B.check()
+    batched = n_tot * steps / tb
+    B.close()
+    del B
+    torch.cuda.empty_cache()
+    # plain: every scene alone, one after another
+    t_plain = 0.0
+    for sc in scs:
+        S = Stepper(sc)
+        S.step(2)
+        S.capture(graph_steps)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        S.run(plain_steps)
+        torch.cuda.synchronize()
+        t_plain += time.perf_counter() - t0
+        S.check()
+        S.close()
+    plain = n_tot * plain_steps / t_plain
+    print(f"# batched {batched:.4g} vs plain {plain:.4g} particle-steps/s", file=sys.stderr, flush=True)
+    return {"scenes": n_scenes, "particles": n_tot, "steps": steps, "ms_per_step": 1e3 * tb / steps,
+            "particle_steps_per_s": batched, "pairs_per_step": sb["n_pairs"], "contacts_per_step": sb["n_contacts"],
+            "plain_steps_per_scene": plain_steps, "plain_particle_steps_per_s": plain, "batched_vs_plain": batched / plain}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +133,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-graph", action="store_true", help="launch every step eagerly instead of graph replays")
     ap.add_argument("--graph-steps", type=int, default=20, help="steps captured per HIP graph")
+    ap.add_argument("--batched-scenes", type=int, default=256)
+    ap.add_argument("--batched-steps", type=int, default=1000)
+    ap.add_argument("--plain-steps", type=int, default=100)
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
@@ -137,6 +191,13 @@ def main():
     print(f"# pairs {stats['n_pairs']} contacts {stats['n_contacts']} cells {stats['n_cells']}", file=sys.stderr,
           flush=True)
 
+    S.close()
+    del S
+    torch.cuda.empty_cache()
+    batched = None
+    if not args.no_batched:
+        batched = bench_batched(args.batched_scenes, args.batched_steps, args.plain_steps, args.graph_steps)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sc, args.cpu_budget)
@@ -152,6 +213,8 @@ def main():
                    "stage_ms_per_step": {k: round(v, 5) for k, v in stage_ms.items()}},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "batched_vs_plain": None if batched is None else batched["batched_vs_plain"],
     }
     print(json.dumps(out))
 

@@ -215,6 +215,47 @@ int mwd_radius_init(const double* com, const double* verts, const int32_t* nvert
 int mwd_step_status(mwd_workspace* ws, mwd_status* out);
 int mwd_clear_flags(mwd_workspace* ws);
 
+/* ---- Tangent (forward-mode) step: derivatives of the state with respect to
+ * three law parameters theta_k (normally Kn, en, mu). The reference has no
+ * differentiable form; these entry points are additions.
+ *
+ * d(coefficient)/d(theta_k) of a law, k = 0..MWD_N_TAN-1. */
+#define MWD_N_TAN 3
+typedef struct mwd_law_tan {
+    double Kn[MWD_N_TAN], mu[MWD_N_TAN], Gn[MWD_N_TAN], Kt_num[MWD_N_TAN], inv_mu[MWD_N_TAN];
+    double ln_et[MWD_N_TAN], cn[MWD_N_TAN], ct[MWD_N_TAN], ratio[MWD_N_TAN];
+} mwd_law_tan;
+
+/* Law coefficient derivatives for (Kn, en, et, mu), given seed[4][MWD_N_TAN]
+ * = d(Kn, en, et, mu)/d(theta_k) (row-major). */
+int mwd_law_tangent(mwd_law_tan* out, double Kn, double en, double et, double mu, const double* seed);
+
+/* Install the law derivatives of every scene, laws2[2*s] (particle law) and
+ * laws2[2*s+1] (wall law), and allocate the per-contact tangent buffers. */
+int mwd_set_law_tangents(mwd_workspace* ws, const mwd_law_tan* laws2);
+
+/* The state arrays that carry tangents, in the order of `tan` below. */
+#define MWD_TAN_COM 0
+#define MWD_TAN_V 1
+#define MWD_TAN_V12 2
+#define MWD_TAN_W 3
+#define MWD_TAN_WBF 4
+#define MWD_TAN_W12BF 5
+#define MWD_TAN_Q 6
+#define MWD_TAN_Q12 7
+#define MWD_TAN_QALL 8
+#define MWD_TAN_DXALL 9
+#define MWD_TAN_VERTS 10
+#define MWD_N_TAN_ARRAYS 11
+
+/* One step of mwd_step that also advances tangents: tan[MWD_N_TAN * a + k]
+ * is the device array d(state array a)/d(theta_k), laid out like the state
+ * array. The values it produces are bit-identical to mwd_step's. Contact
+ * topology (which pairs touch, and EPA's final face) is held fixed within the
+ * step; depth, normal and contact points are differentiated through that
+ * face's support vertices. Requires mwd_set_law_tangents. */
+int mwd_step_tangent(mwd_workspace* ws, mwd_state* state, double* const* tan);
+
 #ifdef __cplusplus
 }
 #endif

@@ -45,6 +45,15 @@ class Law(C.Structure):
                                           "cn", "ct", "ratio")] + [("mode", C.c_int32), ("pad_", C.c_int32)]
 
 
+N_TAN = 3
+TAN_ARRAYS = ("com", "v", "v12", "w", "wbf", "w12bf", "q", "q12", "qall", "dxall", "verts")
+
+
+class LawTan(C.Structure):
+    _fields_ = [(n, C.c_double * N_TAN) for n in ("Kn", "mu", "Gn", "Kt_num", "inv_mu", "ln_et", "cn", "ct",
+                                                   "ratio")]
+
+
 class Scene(C.Structure):
     _fields_ = [
         ("row0", C.c_int64), ("n_walls", C.c_int64), ("n_active", C.c_int64),
@@ -97,6 +106,10 @@ _SIGS = {
     "mwd_set_timing": (C.c_int, [_P, C.c_int]),
     "mwd_radius_init": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int32, _P, _P]),
     "mwd_stage_times": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "mwd_law_tangent": (C.c_int, [C.POINTER(LawTan), C.c_double, C.c_double, C.c_double, C.c_double,
+                                  C.POINTER(C.c_double)]),
+    "mwd_set_law_tangents": (C.c_int, [_P, C.POINTER(LawTan)]),
+    "mwd_step_tangent": (C.c_int, [_P, C.POINTER(State), C.POINTER(_P)]),
 }
 
 N_STAGES = 7
@@ -136,6 +149,16 @@ def law(Kn: float, en: float, et: float, mu: float, dt: float, mode: int = LAW_R
     L = Law()
     check(load().mwd_law_init_mode(C.byref(L), Kn, en, et, mu, dt, int(mode)), "mwd_law_init_mode")
     return L
+
+
+def law_tangent(Kn: float, en: float, et: float, mu: float, seed) -> LawTan:
+    """Derivatives of a law's coefficients; seed[i][k] = d(Kn, en, et, mu)[i] / d theta_k."""
+    import numpy as np
+    sd = np.ascontiguousarray(np.asarray(seed, dtype=np.float64).reshape(4, N_TAN))
+    T = LawTan()
+    check(load().mwd_law_tangent(C.byref(T), Kn, en, et, mu, sd.ctypes.data_as(C.POINTER(C.c_double))),
+          "mwd_law_tangent")
+    return T
 
 
 def ptr(t) -> int | None:

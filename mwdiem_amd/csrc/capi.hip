@@ -44,10 +44,11 @@ void free_ws(mwd_workspace* ws) {
     void* ptrs[] = {ws->scenes_dev, ws->cell_of, ws->plist, ws->scene_of, ws->is_wall, ws->keys, ws->keys_alt, ws->vals,
                     ws->vals_alt, ws->head, ws->head_off, ws->cell_start, ws->cell_key, ws->cell_nb, ws->row_first,
                     ws->row_rank, ws->cell_by_rank, ws->cell_cnt, ws->cell_off, ws->perm, ws->wall_flag, ws->wall_off,
-                    ws->pairs, ws->collision, ws->simplex, ws->pair_flag, ws->contact_off, ws->contact_pair,
+                    ws->pairs, ws->collision, ws->simplex, ws->sidx, ws->pair_flag, ws->contact_off, ws->contact_pair,
                     ws->depth, ws->normal, ws->cpA, ws->cpB, ws->Fn, ws->tA, ws->tB, ws->epa_status, ws->rec_key,
                     ws->rec_key_alt, ws->rec_val, ws->rec_val_alt, ws->seg_begin, ws->seg_end, ws->hist,
-                    ws->hist_off, ws->scan_tmp, ws->counters, ws->flags, ws->laws, ws->scene_wt_off};
+                    ws->hist_off, ws->scan_tmp, ws->counters, ws->flags, ws->laws, ws->scene_wt_off, ws->ctan,
+                    ws->law_tans};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     destroy_events(ws);
@@ -315,7 +316,8 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     A(wall_off, ws->n_wall_tests);
     A(pairs, 2 * pair_cap);
     A(collision, pair_cap);
-    A(simplex, pair_cap);
+    A(simplex, contact_cap);
+    A(sidx, pair_cap);
     A(pair_flag, pair_cap);
     A(contact_off, pair_cap);
     A(contact_pair, contact_cap);
@@ -336,7 +338,7 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     A(hist, hist_len);
     A(hist_off, hist_len);
     A(scan_tmp, ws->scan_tmp_len);
-    A(counters, 8);
+    A(counters, 16);
     A(flags, 1);
     A(laws, 2 * n_scenes);
     A(scene_wt_off, n_scenes + 1);
@@ -358,7 +360,7 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
 #undef H2D
     if ((ws->cell_of &&
          hipMemsetAsync(ws->cell_of, 0xFF, sizeof(int32_t) * ws->key_span_total, s) != hipSuccess) ||
-        hipMemsetAsync(ws->counters, 0, sizeof(int64_t) * 8, s) != hipSuccess ||
+        hipMemsetAsync(ws->counters, 0, sizeof(int64_t) * 16, s) != hipSuccess ||
         hipMemsetAsync(ws->scan_tmp, 0, sizeof(int64_t) * ws->scan_tmp_len, s) != hipSuccess ||
         hipMemsetAsync(ws->flags, 0, sizeof(uint32_t), s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
@@ -418,7 +420,30 @@ MWD_API int mwd_pairs_view(mwd_workspace* ws, int32_t** pairs, int64_t** n_pairs
     return MWD_OK;
 }
 
-MWD_API int mwd_fine_contact(mwd_workspace* ws, const mwd_state* st) {
+static TArr<MWD_NT> tarr_of(double* v, double* tan_base, int64_t stride, int a) {
+    TArr<MWD_NT> t;
+    t.v = v;
+    for (int k = 0; k < MWD_NT; ++k) t.t[k] = tan_base ? tan_base + (int64_t)(MWD_NT * a + k) * stride : nullptr;
+    return t;
+}
+
+// Per-contact outputs: values in the workspace arrays, tangents (when
+// allocated) in ws->ctan as [array][k][contact_cap * 3].
+static ContactOut contact_out(mwd_workspace* ws, bool tangents) {
+    ContactOut co;
+    double* tb = tangents ? ws->ctan : nullptr;
+    const int64_t stride = ws->contact_cap * 3;
+    co.Fn = tarr_of(ws->Fn, tb, stride, 0);
+    co.tA = tarr_of(ws->tA, tb, stride, 1);
+    co.tB = tarr_of(ws->tB, tb, stride, 2);
+    return co;
+}
+
+static int fine_contact(mwd_workspace* ws, const mwd_state* st, double* const* tan);
+
+MWD_API int mwd_fine_contact(mwd_workspace* ws, const mwd_state* st) { return fine_contact(ws, st, nullptr); }
+
+static int fine_contact(mwd_workspace* ws, const mwd_state* st, double* const* tan) {
     if (!ws || !st || st->vmax != ws->vmax) {
         mwd_set_error("mwd_fine_contact: invalid argument");
         return MWD_E_INVALID;
@@ -427,7 +452,10 @@ MWD_API int mwd_fine_contact(mwd_workspace* ws, const mwd_state* st) {
     const BodyView A = state_view(ws, st, 0), B = state_view(ws, st, 1);
     int64_t* ctr = ws->counters;
     mwd_mark(ws, 2);
-    int rc = mwd_launch_gjk(A, B, ws->pair_cap, ctr + 3, ws->collision, ws->simplex, s);
+    int rc = mwd_set_i64(ctr + 8, 0, s);
+    if (rc) return rc;
+    rc = mwd_launch_gjk(A, B, ws->pair_cap, ctr + 3, ws->collision, ws->simplex, ws->sidx, ctr + 8, ws->contact_cap,
+                        ws->flags, s);
     if (rc) return rc;
     mwd_mark(ws, 3);
     rc = mwd_launch_compact_contacts(ws->collision, ws->pair_cap, ctr + 3, ws->pair_flag, ws->contact_off,
@@ -435,15 +463,29 @@ MWD_API int mwd_fine_contact(mwd_workspace* ws, const mwd_state* st) {
                                      ws->scan_tmp_len, s);
     if (rc) return rc;
     mwd_mark(ws, 4);
-    rc = mwd_launch_epa_force(A, B, ws->laws, ws->contact_pair, ws->simplex, ws->contact_cap, ctr + 4, nullptr,
-                              nullptr, nullptr, ws->depth, ws->normal, ws->cpA, ws->cpB, ws->Fn, ws->tA, ws->tB,
-                              nullptr, ws->epa_status, ws->rec_key, ws->rec_val, ctr + 5, ws->flags, s);
+    if (tan) {
+        const VState vs = mwd_vstate(st, tan);
+        BodyTan bt;
+        bt.com = vs.com;
+        bt.verts = vs.verts;
+        bt.vel = vs.v;
+        bt.angvel = vs.w;
+        bt.quat = vs.q;
+        rc = mwd_launch_epa_force_tan(A, B, bt, ws->laws, ws->law_tans, ws->contact_pair, ws->simplex, ws->sidx,
+                                      ws->contact_cap, ctr + 4, ws->depth, ws->normal, ws->cpA, ws->cpB,
+                                      contact_out(ws, true), ws->epa_status, ws->rec_key, ws->rec_val, ctr + 5,
+                                      ws->flags, s);
+    } else {
+        rc = mwd_launch_epa_force(A, B, ws->laws, ws->contact_pair, ws->simplex, ws->sidx, ws->contact_cap, ctr + 4,
+                                  nullptr, nullptr, nullptr, ws->depth, ws->normal, ws->cpA, ws->cpB, ws->Fn, ws->tA,
+                                  ws->tB, nullptr, ws->epa_status, ws->rec_key, ws->rec_val, ctr + 5, ws->flags, s);
+    }
     if (rc) return rc;
     mwd_mark(ws, 5);
     uint32_t* ks;
     int32_t* vs;
     rc = mwd_radix_sort(ws->rec_key, ws->rec_val, ws->rec_key_alt, ws->rec_val_alt, 2 * ws->contact_cap, ctr + 5,
-                        ws->row_bits, ws->hist, ws->hist_off, ws->scan_tmp, ws->scan_tmp_len, s, &ks, &vs);
+                        ws->row_bits, ws->hist, ws->hist_off, ctr + 9, ws->scan_tmp, ws->scan_tmp_len, s, &ks, &vs);
     if (rc) return rc;
     ws->rec_key_sorted = ks;
     ws->rec_val_sorted = vs;
@@ -477,13 +519,8 @@ MWD_API int mwd_reduce_verlet(mwd_workspace* ws, mwd_state* st) {
         mwd_set_error("mwd_reduce_verlet: invalid argument");
         return MWD_E_INVALID;
     }
-    // the fused kernel reads the sorted record values
-    int32_t* saved = ws->rec_val;
-    ws->rec_val = ws->rec_val_sorted;
     mwd_mark(ws, 6);
-    int rc = mwd_launch_reduce_verlet(ws, st);
-    ws->rec_val = saved;
-    return rc;
+    return mwd_launch_reduce_verlet(ws, st, nullptr, contact_out(ws, false));
 }
 
 MWD_API int mwd_step(mwd_workspace* ws, mwd_state* st) {
@@ -492,6 +529,85 @@ MWD_API int mwd_step(mwd_workspace* ws, mwd_state* st) {
     rc = mwd_fine_contact(ws, st);
     if (rc) return rc;
     rc = mwd_reduce_verlet(ws, st);
+    if (rc) return rc;
+    mwd_mark_end(ws);
+    return MWD_OK;
+}
+
+// Law coefficients as functions of (Kn, en, et, mu), in the operator order of
+// mwd_law_init_mode; T = Dual gives their derivatives.
+template <class T>
+static void law_coeffs(T Kn, T en, T et, T mu, T* out /* Kn mu Gn Kt_num inv_mu ln_et cn ct ratio */) {
+    const double pi = 3.141592653589793;
+    const double pi2 = pi * pi;
+    const T ln_en = dlog(en), ln_et = dlog(et);
+    const T A_en = pi2 + ln_en * ln_en;
+    const T tc = dsqrt((mu * A_en) / Kn);
+    out[0] = Kn;
+    out[1] = mu;
+    out[2] = ((-2.0 * mu) / tc) * ln_en;
+    out[3] = ((Kn * (pi2 + ln_et * ln_et)) / A_en) * 1.0;
+    out[4] = 1.0 / mu;
+    out[5] = ln_et;
+    out[6] = (-2.0 * ln_en) / dsqrt(A_en);
+    out[7] = (-2.0 * ln_et) / dsqrt(pi2 + ln_et * ln_et);
+    out[8] = (pi2 + ln_et * ln_et) / A_en;
+}
+
+MWD_API int mwd_law_tangent(mwd_law_tan* out, double Kn, double en, double et, double mu, const double* seed) {
+    if (!out || !seed || !(Kn > 0) || !(en > 0) || !(et > 0) || !(mu > 0)) {
+        mwd_set_error("mwd_law_tangent: invalid argument");
+        return MWD_E_INVALID;
+    }
+    typedef Dual<MWD_NT> D;
+    D in[4];
+    const double v[4] = {Kn, en, et, mu};
+    for (int i = 0; i < 4; ++i) {
+        in[i].v = v[i];
+        for (int k = 0; k < MWD_NT; ++k) in[i].d[k] = seed[MWD_NT * i + k];
+    }
+    D c[9];
+    law_coeffs<D>(in[0], in[1], in[2], in[3], c);
+    double* dst[9] = {out->Kn, out->mu, out->Gn, out->Kt_num, out->inv_mu, out->ln_et, out->cn, out->ct, out->ratio};
+    for (int j = 0; j < 9; ++j)
+        for (int k = 0; k < MWD_NT; ++k) dst[j][k] = c[j].d[k];
+    return MWD_OK;
+}
+
+MWD_API int mwd_set_law_tangents(mwd_workspace* ws, const mwd_law_tan* laws2) {
+    if (!ws || !laws2) {
+        mwd_set_error("mwd_set_law_tangents: invalid argument");
+        return MWD_E_INVALID;
+    }
+    MWD_HIP_CHECK(hipSetDevice(ws->device));
+    if (!ws->law_tans) MWD_HIP_CHECK(hipMalloc((void**)&ws->law_tans, sizeof(mwd_law_tan) * 2 * ws->n_scenes));
+    if (!ws->ctan) MWD_HIP_CHECK(hipMalloc((void**)&ws->ctan, sizeof(double) * 3 * MWD_NT * ws->contact_cap * 3));
+    MWD_HIP_CHECK(hipMemcpyAsync(ws->law_tans, laws2, sizeof(mwd_law_tan) * 2 * ws->n_scenes, hipMemcpyHostToDevice,
+                                 ws->stream));
+    MWD_HIP_CHECK(hipStreamSynchronize(ws->stream));
+    return MWD_OK;
+}
+
+MWD_API int mwd_step_tangent(mwd_workspace* ws, mwd_state* st, double* const* tan) {
+    if (!ws || !st || !tan) {
+        mwd_set_error("mwd_step_tangent: invalid argument");
+        return MWD_E_INVALID;
+    }
+    if (!ws->law_tans || !ws->ctan) {
+        mwd_set_error("mwd_step_tangent: call mwd_set_law_tangents first");
+        return MWD_E_INVALID;
+    }
+    for (int i = 0; i < MWD_N_TAN_ARRAYS * MWD_NT; ++i)
+        if (!tan[i]) {
+            mwd_set_error("mwd_step_tangent: tangent array " + std::to_string(i) + " is NULL");
+            return MWD_E_INVALID;
+        }
+    int rc = mwd_candidates(ws, st, nullptr);
+    if (rc) return rc;
+    rc = fine_contact(ws, st, tan);
+    if (rc) return rc;
+    mwd_mark(ws, 6);
+    rc = mwd_launch_reduce_verlet(ws, st, tan, contact_out(ws, true));
     if (rc) return rc;
     mwd_mark_end(ws);
     return MWD_OK;
@@ -548,7 +664,7 @@ MWD_API int mwd_contact_pairs(int64_t P, int32_t vmax, const double* comA, const
     hipStream_t s = (hipStream_t)stream;
     // temporaries of this call
     int32_t *coll = nullptr, *flag = nullptr, *cpair = nullptr, *est = nullptr, *rval = nullptr, *rval2 = nullptr;
-    int32_t *sb = nullptr, *se = nullptr, *hist = nullptr;
+    int32_t *sb = nullptr, *se = nullptr, *hist = nullptr, *sidx = nullptr;
     uint8_t *wallA = nullptr, *wallB = nullptr;
     Simplex* sx = nullptr;
     int64_t *off = nullptr, *ctr = nullptr, *hist_off = nullptr, *tmp = nullptr;
@@ -577,8 +693,9 @@ MWD_API int mwd_contact_pairs(int64_t P, int32_t vmax, const double* comA, const
     al(&wallA, P);
     al(&wallB, P);
     al(&sx, P);
+    al(&sidx, P);
     al(&off, P);
-    al(&ctr, 8);
+    al(&ctr, 16);
     al(&hist, hist_len);
     al(&hist_off, hist_len);
     al(&tmp, tmp_len);
@@ -606,7 +723,7 @@ MWD_API int mwd_contact_pairs(int64_t P, int32_t vmax, const double* comA, const
     mwd_law lw[2] = {*law_p, *law_w};
     do {
         if (hipMemcpyAsync(laws, lw, sizeof(lw), hipMemcpyHostToDevice, s) != hipSuccess ||
-            hipMemsetAsync(ctr, 0, 8 * sizeof(int64_t), s) != hipSuccess ||
+            hipMemsetAsync(ctr, 0, 16 * sizeof(int64_t), s) != hipSuccess ||
             hipMemsetAsync(tmp, 0, sizeof(int64_t) * tmp_len, s) != hipSuccess ||
             hipMemsetAsync(flags, 0, sizeof(uint32_t), s) != hipSuccess) {
             mwd_set_error("mwd_contact_pairs: upload failed");
@@ -629,11 +746,11 @@ MWD_API int mwd_contact_pairs(int64_t P, int32_t vmax, const double* comA, const
         BodyView B{comB, vertsB, nB, vB, wB, qA, isB, nullptr, nullptr, wallB, nullptr, nullptr, 1, vmax};
         if ((rc = mwd_set_i64(ctr + 3, P, s))) break;
         if ((rc = mwd_launch_zero_pairs(P, depth, normal, cpA, cpB, epa_status, disp, s))) break;
-        if ((rc = mwd_launch_gjk(A, B, P, ctr + 3, coll, sx, s))) break;
+        if ((rc = mwd_launch_gjk(A, B, P, ctr + 3, coll, sx, sidx, ctr + 8, P, flags, s))) break;
         if ((rc = mwd_launch_compact_contacts(coll, P, ctr + 3, flag, off, cpair, P, ctr + 4, flags, tmp, tmp_len,
                                               s)))
             break;
-        if ((rc = mwd_launch_epa_force(A, B, laws, cpair, sx, P, ctr + 4, dispt, idA, idB, dep, nrm, ca, cb, fn, ta,
+        if ((rc = mwd_launch_epa_force(A, B, laws, cpair, sx, sidx, P, ctr + 4, dispt, idA, idB, dep, nrm, ca, cb, fn, ta,
                                        tb, disp, est, rkey, rval, ctr + 5, flags, s)))
             break;
         // per-pair outputs scattered back from contact order
@@ -650,7 +767,7 @@ MWD_API int mwd_contact_pairs(int64_t P, int32_t vmax, const double* comA, const
             uint32_t* ks;
             int32_t* vs;
             if ((rc = mwd_radix_sort(rkey, rval, rkey2, rval2, rec, ctr + 5, bits_for(n_particles), hist, hist_off,
-                                     tmp, tmp_len, s, &ks, &vs)))
+                                     ctr + 9, tmp, tmp_len, s, &ks, &vs)))
                 break;
             if ((rc = mwd_fill_i32(sb, 0, n_particles, s))) break;
             if ((rc = mwd_fill_i32(se, 0, n_particles, s))) break;

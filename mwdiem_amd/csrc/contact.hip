@@ -15,6 +15,10 @@
 // stores the final simplex of colliding ones; after compaction k_epa_force
 // runs EPA and the law over contacts only, writing per-contact force/torque
 // and two reduction records (row of A, row of B) per contact.
+#include <string.h>
+
+#include <type_traits>
+
 #include "mwd_internal.h"
 
 namespace {
@@ -229,7 +233,9 @@ __device__ int gjk_pair(const Side& A, const Side& B, d3 comA, d3 comB, d3* S, i
 }
 
 __global__ void __launch_bounds__(256) k_gjk(BodyView A, BodyView B, int64_t n_cap, const int64_t* n_dev,
-                                             int32_t* __restrict__ collision, Simplex* __restrict__ simplex) {
+                                             int32_t* __restrict__ collision, Simplex* __restrict__ simplex,
+                                             int32_t* __restrict__ sidx, int64_t* slots, int64_t slot_cap,
+                                             uint32_t* flags) {
     int64_t n = n_dev ? *n_dev : n_cap;
     if (n > n_cap) n = n_cap;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -251,7 +257,13 @@ __global__ void __launch_bounds__(256) k_gjk(BodyView A, BodyView B, int64_t n_c
         const int c = gjk_pair(sa, sb, ca, cb, S, ia, ib);
         collision[p] = c;
         if (c == 1) {
-            Simplex& x = simplex[p];
+            const int64_t slot = atomicAdd((unsigned long long*)slots, 1ull);
+            sidx[p] = (int32_t)slot;
+            if (slot >= slot_cap) {
+                atomicOr(flags, MWD_F_CONTACT_OVERFLOW);
+                continue;
+            }
+            Simplex& x = simplex[slot];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 x.s[k][0] = S[k].x;
@@ -298,6 +310,7 @@ struct Poly {
     double dist[EPA_FACES];
     uint8_t tr[EPA_FACES][3];
     int8_t valid[EPA_FACES];
+    uint8_t flip[EPA_FACES];
     uint8_t opA[EPA_VERTS], opB[EPA_VERTS];
     uint8_t vis[EPA_FACES];
 };
@@ -308,12 +321,14 @@ __device__ __forceinline__ void add_triangle(Poly& P, int e0, int e1, int e2, in
     const double len = sqrt(nrm2(nn));
     nn = nn / len;
     double dd = dot3(nn, P.V[e0]);
+    P.flip[id] = 0;
     if (dd < 0.0) {
         const int o = e2;
         e2 = e1;
         e1 = o;
         dd = dd * -1.0;
         nn = nn * -1.0;
+        P.flip[id] = 1;
     }
     P.tr[id][0] = (uint8_t)e0;
     P.tr[id][1] = (uint8_t)e1;
@@ -331,6 +346,9 @@ struct EpaResult {
     double depth;
     d3 n, cpA, cpB;
     int status;
+    // the final face: support indices of its vertices in stored order, and
+    // whether AddTriangle swapped its last two vertices
+    int fa[3], fb[3], flip;
 };
 
 // EPA, .cu:328-603. status 1: expansion cap; 2: face storage exhausted.
@@ -442,63 +460,158 @@ __device__ EpaResult epa(const Simplex& sx, const Side& A, const Side& B) {
     r.cpB = (ld3(B.v + 3 * P.opB[a]) * uu + ld3(B.v + 3 * P.opB[b]) * vv) + ld3(B.v + 3 * P.opB[c]) * ww;
     r.n = nrm;
     r.status = st;
+    r.fa[0] = P.opA[a];
+    r.fa[1] = P.opA[b];
+    r.fa[2] = P.opA[c];
+    r.fb[0] = P.opB[a];
+    r.fb[1] = P.opB[b];
+    r.fb[2] = P.opB[c];
+    r.flip = P.flip[indmin];
     return r;
 }
 
-struct LawOut {
-    d3 Fn, tA, tB, disp;
+// Depth, normal and contact points of EPA's final face, recomputed from the
+// face's support pairs with exactly EPA's operations (AddTriangle, .cu:278-291,
+// and the contact construction, .cu:548-601): the values are EPA's, and with
+// T = Dual they carry the derivatives with respect to both bodies' vertices.
+template <class T>
+struct FaceGeo {
+    T depth;
+    V3<T> n, cpA, cpB;
 };
+template <class T>
+__device__ FaceGeo<T> face_geometry(const TArr<MWD_NT>& VA, int64_t ba, const TArr<MWD_NT>& VB, int64_t bb,
+                                    const int* fa, const int* fb, int flip) {
+    V3<T> a[3], b[3], S[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        a[k] = ldv<T>(VA, ba + fa[k]);
+        b[k] = ldv<T>(VB, bb + fb[k]);
+        S[k] = a[k] - b[k];
+    }
+    // creation order: a swap in AddTriangle exchanged the stored vertices 1 and 2
+    const int e1 = flip ? 2 : 1, e2 = flip ? 1 : 2;
+    V3<T> nn = cross3(S[e1] - S[0], S[e2] - S[e1]);
+    const T len = dsqrt(nrm2(nn));
+    nn = nn / len;
+    T dd = dot3(nn, S[0]);
+    if (flip) {
+        dd = dd * -1.0;
+        nn = nn * -1.0;
+    }
+    FaceGeo<T> g;
+    g.depth = dd;
+    g.n = nn;
+    const V3<T> Proj = nn * dd;
+    V3<T> v0 = S[1] - S[0], v1 = S[2] - S[0];
+    const V3<T> v2 = Proj - S[0];
+    if (dot3(cross3(v0, v1), nn) < 0.0) {
+        v0 = S[2] - S[0];
+        v1 = S[1] - S[0];
+    }
+    const T v00 = dot3(v0, v0), v01 = dot3(v0, v1), v11 = dot3(v1, v1), v20 = dot3(v2, v0), v21 = dot3(v2, v1);
+    const T denom = v00 * v11 - v01 * v01;
+    const T vv = (v11 * v20 - v01 * v21) / denom;
+    const T ww = (v00 * v21 - v01 * v20) / denom;
+    const T uu = (1.0 - ww) - vv;
+    g.cpA = (a[0] * uu + a[1] * vv) + a[2] * ww;
+    g.cpB = (b[0] * uu + b[1] * vv) + b[2] * ww;
+    return g;
+}
+
+template <class T>
+struct LawOut {
+    V3<T> Fn, tA, tB, disp;
+};
+
+// The law coefficients a contact needs, as values (T = double) or with their
+// derivatives with respect to (Kn, en, mu) (T = Dual).
+template <class T>
+struct LawT {
+    T Kn, mu, Gn, Kt_num, inv_mu, ln_et, cn, ct, ratio;
+    double dt;
+    int mode;
+};
+__device__ __forceinline__ LawT<double> law_of(const mwd_law& L, const mwd_law_tan*, double*) {
+    LawT<double> o;
+    o.Kn = L.Kn; o.mu = L.mu; o.Gn = L.Gn; o.Kt_num = L.Kt_num; o.inv_mu = L.inv_mu; o.ln_et = L.ln_et;
+    o.cn = L.cn; o.ct = L.ct; o.ratio = L.ratio; o.dt = L.dt; o.mode = L.mode;
+    return o;
+}
+__device__ __forceinline__ Dual<MWD_NT> dual_of(double v, const double* d) {
+    Dual<MWD_NT> r;
+    r.v = v;
+#pragma unroll
+    for (int k = 0; k < MWD_NT; ++k) r.d[k] = d ? d[k] : 0.0;
+    return r;
+}
+__device__ __forceinline__ LawT<Dual<MWD_NT>> law_of(const mwd_law& L, const mwd_law_tan* t, Dual<MWD_NT>*) {
+    LawT<Dual<MWD_NT>> o;
+    o.Kn = dual_of(L.Kn, t ? t->Kn : nullptr);
+    o.mu = dual_of(L.mu, t ? t->mu : nullptr);
+    o.Gn = dual_of(L.Gn, t ? t->Gn : nullptr);
+    o.Kt_num = dual_of(L.Kt_num, t ? t->Kt_num : nullptr);
+    o.inv_mu = dual_of(L.inv_mu, t ? t->inv_mu : nullptr);
+    o.ln_et = dual_of(L.ln_et, t ? t->ln_et : nullptr);
+    o.cn = dual_of(L.cn, t ? t->cn : nullptr);
+    o.ct = dual_of(L.ct, t ? t->ct : nullptr);
+    o.ratio = dual_of(L.ratio, t ? t->ratio : nullptr);
+    o.dt = L.dt;
+    o.mode = L.mode;
+    return o;
+}
 
 // Force, .cu:647-871. Fn of the result is the force on A's linear momentum:
 // the normal force (reference law) or normal plus tangential (physical law).
 // imA/imB are inverse masses (0 for walls), read by the physical law only.
-__device__ LawOut force_law(d3 comA, d3 comB, d3 vA, d3 vB, d3 wA, d3 wB, d3 dispt, q4 qA, double isA,
-                            double isB, double imA, double imB, double depth, d3 n, d3 cpA, d3 cpB,
-                            const mwd_law& L) {
-    LawOut o;
+template <class T>
+__device__ LawOut<T> force_law(V3<T> comA, V3<T> comB, V3<T> vA, V3<T> vB, V3<T> wA, V3<T> wB, V3<T> dispt, Q4<T> qA,
+                               double isA, double isB, double imA, double imB, T depth, V3<T> n, V3<T> cpA, V3<T> cpB,
+                               const LawT<T>& L) {
+    LawOut<T> o;
     const bool phys = L.mode == MWD_LAW_PHYSICAL;
-    const d3 cp = phys ? (cpA + cpB) / 2.0 : (cpA - cpB) / 2.0;
-    const d3 dA = cp - comA, dB = cp - comB;
-    const double dAn = nrm2(dA), dBn = nrm2(dB);
-    double Kt, Gt, Gn;
+    const V3<T> cp = phys ? (cpA + cpB) / 2.0 : (cpA - cpB) / 2.0;
+    const V3<T> dA = cp - comA, dB = cp - comB;
+    const T dAn = nrm2(dA), dBn = nrm2(dB);
+    T Kt, Gt, Gn;
     if (!phys) {
-        const double C = (L.inv_mu + dAn / isA) + dBn / isB;
+        const T C = (L.inv_mu + dAn / isA) + dBn / isB;
         Kt = L.Kt_num / C;
         Gt = (-2.0 / C) * L.ln_et;
         Gn = L.Gn;
     } else {
         const double imn = imA + imB;
         const double mn = 1.0 / imn;
-        const double imt = (imn + dAn * (imA == 0.0 ? 0.0 : 1.0 / isA)) + dBn * (imB == 0.0 ? 0.0 : 1.0 / isB);
-        const double mt = 1.0 / imt;
-        Gn = L.cn * sqrt(mn * L.Kn);
+        const T imt = (T(imn) + dAn * (imA == 0.0 ? 0.0 : 1.0 / isA)) + dBn * (imB == 0.0 ? 0.0 : 1.0 / isB);
+        const T mt = 1.0 / imt;
+        Gn = L.cn * dsqrt(mn * L.Kn);
         Kt = (L.Kn * L.ratio) * (mt / mn);
-        Gt = L.ct * sqrt(mt * Kt);
+        Gt = L.ct * dsqrt(mt * Kt);
     }
     if (dot3(comB - comA, n) < 0.0) n = n * -1.0;
-    n = n / sqrt(nrm2(n));
-    const d3 disp_n = n * depth;
-    const d3 u = ((vA + cross3(wA, cpA - comA)) - vB) - cross3(wB, cpB - comB);
-    const d3 un = n * dot3(u, n);
-    const d3 ut = u - un;
-    const double utn = sqrt(nrm2(ut));
-    const double dtn = sqrt(nrm2(dispt));
-    d3 t = mk3(0.0, 0.0, 0.0);
+    n = n / dsqrt(nrm2(n));
+    const V3<T> disp_n = n * depth;
+    const V3<T> u = ((vA + cross3(wA, cpA - comA)) - vB) - cross3(wB, cpB - comB);
+    const V3<T> un = n * dot3(u, n);
+    const V3<T> ut = u - un;
+    const T utn = dsqrt(nrm2(ut));
+    const T dtn = dsqrt(nrm2(dispt));
+    V3<T> t = zero3<T>();
     if (utn != 0.0) t = ut / utn;
     if (utn == 0.0 && dtn != 0.0) t = dispt / dtn;
-    const d3 Fn = (disp_n * -1.0) * L.Kn - un * Gn;
-    const double Fnn = sqrt(nrm2(Fn));
-    d3 Ft = (dispt * -1.0) * Kt - ut * Gt;
-    const double Ftn = sqrt(nrm2(Ft));
-    const double muF = L.mu * Fnn;
-    d3 dnew;
+    const V3<T> Fn = (disp_n * -1.0) * L.Kn - un * Gn;
+    const T Fnn = dsqrt(nrm2(Fn));
+    V3<T> Ft = (dispt * -1.0) * Kt - ut * Gt;
+    const T Ftn = dsqrt(nrm2(Ft));
+    const T muF = L.mu * Fnn;
+    V3<T> dnew;
     if (Ftn > muF) Ft = ((t * Fnn) * -1.0) * L.mu;
     if (Ftn <= muF) {
         dnew = qrot(qA, dispt, qconj(qA)) + ut * L.dt;
     } else {
         dnew = ((((t * Fnn) * L.mu) * -1.0) - ut * Gt) / Kt;
     }
-    const d3 Ff = Ft + Fn;
+    const V3<T> Ff = Ft + Fn;
     o.Fn = phys ? Ff : Fn;
     o.tA = cross3(cp - comA, Ff);
     o.tB = cross3(cp - comB, Ff * -1.0);
@@ -506,18 +619,38 @@ __device__ LawOut force_law(d3 comA, d3 comB, d3 vA, d3 vB, d3 wA, d3 wB, d3 dis
     return o;
 }
 
-__global__ void __launch_bounds__(256) k_epa_force(BodyView A, BodyView B, const mwd_law* __restrict__ laws2,
+template <class T>
+__device__ __forceinline__ V3<T> body3(const double* p, const TArr<MWD_NT>& t, int64_t r) {
+    if constexpr (std::is_same<T, double>::value)
+        return ld3(p + 3 * r);
+    else
+        return ldv<T>(t, r);
+}
+template <class T>
+__device__ __forceinline__ Q4<T> bodyq(const double* p, const TArr<MWD_NT>& t, int64_t r) {
+    if constexpr (std::is_same<T, double>::value)
+        return ldq(p + 4 * r);
+    else
+        return ldqT<T>(t, r);
+}
+
+// EPA + law per contact. T = double: the forward path (any pair layout).
+// T = Dual: the tangent path, where A and B index one state whose tangents bt
+// carries; EPA runs on values and the final face is re-evaluated in T.
+template <class T>
+__global__ void __launch_bounds__(256) k_epa_force(BodyView A, BodyView B, BodyTan bt,
+                                                   const mwd_law* __restrict__ laws2,
+                                                   const mwd_law_tan* __restrict__ lawt2,
                                                    const int32_t* __restrict__ contact_pair,
-                                                   const Simplex* __restrict__ simplex, int64_t contact_cap,
+                                                   const Simplex* __restrict__ simplex,
+                                                   const int32_t* __restrict__ sidx, int64_t contact_cap,
                                                    const int64_t* n_contacts, const double* __restrict__ dispt,
                                                    const int32_t* __restrict__ idA, const int32_t* __restrict__ idB,
                                                    double* __restrict__ depth, double* __restrict__ normal,
-                                                   double* __restrict__ cpA, double* __restrict__ cpB,
-                                                   double* __restrict__ Fn, double* __restrict__ tA,
-                                                   double* __restrict__ tB, double* __restrict__ disp_out,
-                                                   int32_t* __restrict__ epa_status, uint32_t* __restrict__ rec_key,
-                                                   int32_t* __restrict__ rec_val, int64_t* n_records,
-                                                   uint32_t* flags) {
+                                                   double* __restrict__ cpA, double* __restrict__ cpB, ContactOut co,
+                                                   double* __restrict__ disp_out, int32_t* __restrict__ epa_status,
+                                                   uint32_t* __restrict__ rec_key, int32_t* __restrict__ rec_val,
+                                                   int64_t* n_records, uint32_t* flags) {
     int64_t K = *n_contacts;
     if (K > contact_cap) K = contact_cap;
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -529,31 +662,50 @@ __global__ void __launch_bounds__(256) k_epa_force(BodyView A, BodyView B, const
         const int64_t ra = view_row(A, p), rb = view_row(B, p);
         const Side sa{A.verts + (int64_t)A.vmax * 3 * ra, A.nverts[ra]};
         const Side sb{B.verts + (int64_t)B.vmax * 3 * rb, B.nverts[rb]};
-        const EpaResult e = epa(simplex[p], sa, sb);
+        const int32_t slot = sidx[p];
+        if (slot < 0 || slot >= contact_cap) continue;  // simplex storage overflowed (flagged by k_gjk)
+        const EpaResult e = epa(simplex[slot], sa, sb);
         const int wall = (A.is_wall[ra] || B.is_wall[rb]) ? 1 : 0;
         const int scene = A.scene ? A.scene[ra] : 0;
-        const mwd_law& L = laws2[2 * scene + wall];
-        const d3 dt_in = dispt ? ld3(dispt + 4 * p) : mk3(0.0, 0.0, 0.0);
+        const LawT<T> L = law_of(laws2[2 * scene + wall], lawt2 ? lawt2 + 2 * scene + wall : nullptr, (T*)nullptr);
+        const V3<T> dt_in = dispt ? mk3<T>(T(dispt[4 * p]), T(dispt[4 * p + 1]), T(dispt[4 * p + 2])) : zero3<T>();
         const double imA = (A.mass && !A.is_wall[ra]) ? 1.0 / A.mass[ra] : 0.0;
         const double imB = (B.mass && !B.is_wall[rb]) ? 1.0 / B.mass[rb] : 0.0;
-        const LawOut f = force_law(ld3(A.com + 3 * ra), ld3(B.com + 3 * rb), ld3(A.vel + 3 * ra), ld3(B.vel + 3 * rb),
-                                   ld3(A.angvel + 3 * ra), ld3(B.angvel + 3 * rb), dt_in, ldq(A.quat + 4 * ra),
-                                   A.isize[ra], B.isize[rb], imA, imB, e.depth, e.n, e.cpA, e.cpB, L);
+        T dep;
+        V3<T> nrm, ca, cb;
+        if constexpr (std::is_same<T, double>::value) {
+            dep = e.depth;
+            nrm = e.n;
+            ca = e.cpA;
+            cb = e.cpB;
+        } else {
+            const FaceGeo<T> g = face_geometry<T>(bt.verts, (int64_t)A.vmax * ra, bt.verts, (int64_t)B.vmax * rb,
+                                                  e.fa, e.fb, e.flip);
+            dep = g.depth;
+            nrm = g.n;
+            ca = g.cpA;
+            cb = g.cpB;
+        }
+        const LawOut<T> f = force_law<T>(body3<T>(A.com, bt.com, ra), body3<T>(B.com, bt.com, rb),
+                                         body3<T>(A.vel, bt.vel, ra), body3<T>(B.vel, bt.vel, rb),
+                                         body3<T>(A.angvel, bt.angvel, ra), body3<T>(B.angvel, bt.angvel, rb), dt_in,
+                                         bodyq<T>(A.quat, bt.quat, ra), A.isize[ra], B.isize[rb], imA, imB, dep, nrm,
+                                         ca, cb, L);
         depth[c] = e.depth;
         st3(normal + 3 * c, e.n);
         st3(cpA + 3 * c, e.cpA);
         st3(cpB + 3 * c, e.cpB);
-        st3(Fn + 3 * c, f.Fn);
-        st3(tA + 3 * c, f.tA);
-        st3(tB + 3 * c, f.tB);
+        stv(co.Fn, c, f.Fn);
+        stv(co.tA, c, f.tA);
+        stv(co.tB, c, f.tB);
         if (epa_status) epa_status[c] = e.status;
         if (e.status & 1) fl |= MWD_F_EPA_CAP;
         if (e.status & 2) fl |= MWD_F_EPA_FACES;
         if (disp_out) {
             double* dp = disp_out + 4 * p;
-            dp[0] = f.disp.x;
-            dp[1] = f.disp.y;
-            dp[2] = f.disp.z;
+            dp[0] = val(f.disp.x);
+            dp[1] = val(f.disp.y);
+            dp[2] = val(f.disp.z);
             dp[3] = 1.0;
         }
         rec_key[2 * c] = (uint32_t)(idA ? idA[p] : ra);
@@ -645,10 +797,10 @@ unsigned grid_for(int64_t n, int threads, int64_t max_blocks) {
 }  // namespace
 
 int mwd_launch_gjk(const BodyView& A, const BodyView& B, int64_t n_cap, const int64_t* n_dev, int32_t* collision,
-                   Simplex* simplex, hipStream_t s) {
+                   Simplex* simplex, int32_t* sidx, int64_t* slots, int64_t slot_cap, uint32_t* flags, hipStream_t s) {
     if (n_cap <= 0) return MWD_OK;
     hipLaunchKernelGGL(k_gjk, dim3(grid_for(n_cap, 256, 8192)), dim3(256), 0, s, A, B, n_cap, n_dev, collision,
-                       simplex);
+                       simplex, sidx, slots, slot_cap, flags);
     MWD_LAUNCH_CHECK();
     return MWD_OK;
 }
@@ -671,15 +823,44 @@ int mwd_launch_compact_contacts(const int32_t* collision, int64_t n_cap, const i
     return MWD_OK;
 }
 
+static TArr<MWD_NT> plain(double* v) {
+    TArr<MWD_NT> a;
+    a.v = v;
+    for (int k = 0; k < MWD_NT; ++k) a.t[k] = nullptr;
+    return a;
+}
+
 int mwd_launch_epa_force(const BodyView& A, const BodyView& B, const mwd_law* laws2, const int32_t* contact_pair,
-                         const Simplex* simplex, int64_t contact_cap, const int64_t* n_contacts, const double* dispt,
+                         const Simplex* simplex, const int32_t* sidx, int64_t contact_cap, const int64_t* n_contacts,
+                         const double* dispt,
                          const int32_t* idA, const int32_t* idB, double* depth, double* normal, double* cpA,
                          double* cpB, double* Fn, double* tA, double* tB, double* disp_out, int32_t* epa_status,
                          uint32_t* rec_key, int32_t* rec_val, int64_t* n_records, uint32_t* flags, hipStream_t s) {
     if (contact_cap <= 0) return MWD_OK;
-    hipLaunchKernelGGL(k_epa_force, dim3(grid_for(contact_cap, 256, 4096)), dim3(256), 0, s, A, B, laws2,
-                       contact_pair, simplex, contact_cap, n_contacts, dispt, idA, idB, depth, normal, cpA, cpB, Fn,
-                       tA, tB, disp_out, epa_status, rec_key, rec_val, n_records, flags);
+    BodyTan bt;
+    memset(&bt, 0, sizeof(bt));
+    ContactOut co;
+    co.Fn = plain(Fn);
+    co.tA = plain(tA);
+    co.tB = plain(tB);
+    hipLaunchKernelGGL(k_epa_force<double>, dim3(grid_for(contact_cap, 256, 4096)), dim3(256), 0, s, A, B, bt, laws2,
+                       (const mwd_law_tan*)nullptr, contact_pair, simplex, sidx, contact_cap, n_contacts, dispt, idA,
+                       idB, depth, normal, cpA, cpB, co, disp_out, epa_status, rec_key, rec_val, n_records, flags);
+    MWD_LAUNCH_CHECK();
+    return MWD_OK;
+}
+
+int mwd_launch_epa_force_tan(const BodyView& A, const BodyView& B, const BodyTan& bt, const mwd_law* laws2,
+                             const mwd_law_tan* lawt2, const int32_t* contact_pair, const Simplex* simplex,
+                             const int32_t* sidx, int64_t contact_cap, const int64_t* n_contacts, double* depth,
+                             double* normal, double* cpA, double* cpB, const ContactOut& co, int32_t* epa_status,
+                             uint32_t* rec_key, int32_t* rec_val, int64_t* n_records, uint32_t* flags,
+                             hipStream_t s) {
+    if (contact_cap <= 0) return MWD_OK;
+    hipLaunchKernelGGL(k_epa_force<Dual<MWD_NT>>, dim3(grid_for(contact_cap, 256, 4096)), dim3(256), 0, s, A, B, bt,
+                       laws2, lawt2, contact_pair, simplex, sidx, contact_cap, n_contacts, (const double*)nullptr,
+                       (const int32_t*)nullptr, (const int32_t*)nullptr, depth, normal, cpA, cpB, co,
+                       (double*)nullptr, epa_status, rec_key, rec_val, n_records, flags);
     MWD_LAUNCH_CHECK();
     return MWD_OK;
 }

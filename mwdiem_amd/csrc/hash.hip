@@ -343,7 +343,7 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
         uint32_t* ks;
         int32_t* vs;
         rc = mwd_radix_sort(ws->keys, ws->vals, ws->keys_alt, ws->vals_alt, M, nullptr, ws->key_bits, ws->hist,
-                            ws->hist_off, ws->scan_tmp, ws->scan_tmp_len, s, &ks, &vs);
+                            ws->hist_off, ctr + 9, ws->scan_tmp, ws->scan_tmp_len, s, &ks, &vs);
         if (rc) return rc;
         ws->keys_sorted = ks;
         ws->vals_sorted = vs;

@@ -73,6 +73,23 @@ struct BodyView {
     int32_t vmax;
 };
 
+// A state array and its MWD_N_TAN tangents (tangent pointers NULL in the
+// forward path).
+#define MWD_NT MWD_N_TAN
+struct VState {
+    TArr<MWD_NT> com, v, v12, w, wbf, w12bf, q, q12, qall, dxall, verts;
+    const double *mass, *Iinv, *I;
+    double* radius;
+};
+// Per-contact force on A, torques on A and B (with tangents in the tangent path).
+struct ContactOut {
+    TArr<MWD_NT> Fn, tA, tB;
+};
+// Tangents of the body arrays the contact stage reads.
+struct BodyTan {
+    TArr<MWD_NT> com, verts, vel, angvel, quat;
+};
+
 // --------------------------------------------------------------- workspace
 struct mwd_workspace {
     int device;
@@ -111,13 +128,16 @@ struct mwd_workspace {
     // candidate pairs
     int32_t* pairs;            // [pair_cap*2]
     int32_t* collision;        // [pair_cap]
-    Simplex* simplex;          // [pair_cap]
+    Simplex* simplex;          // [contact_cap], slot claimed by GJK per colliding pair
+    int32_t* sidx;             // [pair_cap] simplex slot of a colliding pair
     int32_t* pair_flag;        // [pair_cap]
     int64_t* contact_off;      // [pair_cap]
     // contacts
     int32_t* contact_pair;     // [contact_cap]
     double *depth, *normal, *cpA, *cpB, *Fn, *tA, *tB;
     int32_t* epa_status;       // [contact_cap]
+    double* ctan;              // [3 arrays * MWD_NT * contact_cap * 3] tangents of Fn, tA, tB (tangent step)
+    mwd_law_tan* law_tans;     // [2*n_scenes] (tangent step)
     uint32_t *rec_key, *rec_key_alt;  // [2*contact_cap]
     int32_t *rec_val, *rec_val_alt;
     uint32_t* rec_key_sorted;         // whichever of rec_key / rec_key_alt holds the sorted records
@@ -132,7 +152,8 @@ struct mwd_workspace {
     int64_t scan_tmp_len;
     // counters (device): 0 n_cells, 1 n_wall_pairs, 2 n_cell_pairs, 3 n_pairs
     // (capped), 4 n_contacts, 5 n_records, 6 n_cells of the previous step,
-    // 7 n_pairs before capping
+    // 7 n_pairs before capping, 8 simplex slots claimed this step, 9 radix
+    // histogram entries (scratch)
     int64_t* counters;
     uint32_t* flags;
     mwd_law* laws;             // [2*n_scenes] (particle law, wall law)
@@ -159,11 +180,12 @@ int64_t mwd_scan_tmp_len(int64_t n_cap);
 // ceil(key_bits / MWD_SORT_MAX_BITS) passes, ping-ponging between (keys,
 // vals) and the alt buffers; *keys_out/*vals_out name the buffers holding the
 // result (the alt ones when the pass count is odd). hist/hist_off hold
-// 2^MWD_SORT_MAX_BITS * mwd_sort_tiles(n_cap) entries.
+// 2^MWD_SORT_MAX_BITS * mwd_sort_tiles(n_cap) entries; hcount is one device
+// int64 of scratch.
 #define MWD_SORT_MAX_BITS 11
 int mwd_radix_sort(uint32_t* keys, int32_t* vals, uint32_t* keys_alt, int32_t* vals_alt, int64_t n_cap,
-                   const int64_t* n_dev, int key_bits, int32_t* hist, int64_t* hist_off, int64_t* tmp,
-                   int64_t tmp_len, hipStream_t s, uint32_t** keys_out, int32_t** vals_out);
+                   const int64_t* n_dev, int key_bits, int32_t* hist, int64_t* hist_off, int64_t* hcount,
+                   int64_t* tmp, int64_t tmp_len, hipStream_t s, uint32_t** keys_out, int32_t** vals_out);
 int64_t mwd_sort_tiles(int64_t n_cap);
 int mwd_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
 int mwd_set_i64(int64_t* p, int64_t v, hipStream_t s);
@@ -174,15 +196,19 @@ int mwd_launch_verlet(double* v, double* v12, double* w12bf, double* q12, double
                       const double* mass, const double* Iinv, const double* I, double* dxall, int32_t vmax,
                       const int32_t* nrot, double dt, const double* box, int64_t pmin, int64_t pmax,
                       hipStream_t s);
-int mwd_launch_reduce_verlet(mwd_workspace* ws, mwd_state* st);
+VState mwd_vstate(const mwd_state* st, double* const* tan);
+// Force assembly + Verlet over the moving rows; tan != NULL runs the tangent path.
+int mwd_launch_reduce_verlet(mwd_workspace* ws, mwd_state* st, double* const* tan, const ContactOut& co);
 int mwd_launch_radius(const double* com, const double* verts, const int32_t* nverts, int64_t n, int32_t vmax,
                       double* radius, hipStream_t s);
 int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t* keys);
 int mwd_launch_cell_keys(const double* com, int64_t n, const double* cell, int64_t ncx, int64_t ncy,
                          int64_t* keys, hipStream_t s);
-// GJK over pairs [0, n) (n from n_dev), writing collision and simplex.
+// GJK over pairs [0, n) (n from n_dev), writing collision; a colliding pair
+// claims a simplex slot (counter *slots, zero on entry) and stores it in sidx.
 int mwd_launch_gjk(const BodyView& A, const BodyView& B, int64_t n_cap, const int64_t* n_dev,
-                   int32_t* collision, Simplex* simplex, hipStream_t s);
+                   int32_t* collision, Simplex* simplex, int32_t* sidx, int64_t* slots, int64_t slot_cap,
+                   uint32_t* flags, hipStream_t s);
 // Contact compaction: contact_pair[c] = pair index of the c-th collision==1.
 int mwd_launch_compact_contacts(const int32_t* collision, int64_t n_cap, const int64_t* n_dev, int32_t* flag,
                                 int64_t* off, int32_t* contact_pair, int64_t contact_cap, int64_t* n_contacts,
@@ -190,10 +216,18 @@ int mwd_launch_compact_contacts(const int32_t* collision, int64_t n_cap, const i
 // EPA + force law over contacts [0, K). Reduction records are keyed by
 // idA[p]/idB[p] when given, else by the view rows.
 int mwd_launch_epa_force(const BodyView& A, const BodyView& B, const mwd_law* laws2, const int32_t* contact_pair,
-                         const Simplex* simplex, int64_t contact_cap, const int64_t* n_contacts, const double* dispt,
+                         const Simplex* simplex, const int32_t* sidx, int64_t contact_cap, const int64_t* n_contacts, const double* dispt,
                          const int32_t* idA, const int32_t* idB, double* depth, double* normal, double* cpA,
                          double* cpB, double* Fn, double* tA, double* tB, double* disp_out, int32_t* epa_status,
                          uint32_t* rec_key, int32_t* rec_val, int64_t* n_records, uint32_t* flags, hipStream_t s);
+// The same over the stepper's state with tangents: A and B index one state
+// (bt), the law derivatives are per scene like laws2, and co receives values
+// and tangents of Fn, tA, tB.
+int mwd_launch_epa_force_tan(const BodyView& A, const BodyView& B, const BodyTan& bt, const mwd_law* laws2,
+                             const mwd_law_tan* lawt2, const int32_t* contact_pair, const Simplex* simplex,
+                             const int32_t* sidx, int64_t contact_cap, const int64_t* n_contacts, double* depth,
+                             double* normal, double* cpA, double* cpB, const ContactOut& co, int32_t* epa_status,
+                             uint32_t* rec_key, int32_t* rec_val, int64_t* n_records, uint32_t* flags, hipStream_t s);
 // Zero the per-pair outputs, then copy per-contact results to their pairs.
 int mwd_launch_zero_pairs(int64_t P, double* depth, double* normal, double* cpA, double* cpB, int32_t* epa_status,
                           double* disp, hipStream_t s);

@@ -51,77 +51,122 @@ __device__ __forceinline__ int64_t block_inclusive_scan(int64_t v, int64_t* lds_
 // Decoupled look-back (Merrill & Garland, "Single-pass Parallel Prefix Scan
 // with Decoupled Look-back", 2016). Tiles are claimed in order through an
 // atomic counter; each publishes its aggregate, then its inclusive prefix
-// once known; successors add up predecessors' published values. Status words
-// pack a 2-bit flag above a 62-bit value. The last block to finish resets the
-// status words and counters, so the scratch is clean for the next call
-// without a separate clear.
-constexpr uint64_t ST_AGG = 1ull << 62;
-constexpr uint64_t ST_INC = 2ull << 62;
-constexpr uint64_t ST_VAL = (1ull << 62) - 1;
+// once known; a successor's first wave reads 64 predecessors' status words at
+// a time and adds them up to the nearest inclusive one.
+//
+// A status word packs [epoch:22 | flag:2 | value:40]. Each call has its own
+// epoch (1 .. 2^22-1, advanced by the call's last block), and a word of
+// another epoch reads as "not published", so the words never need clearing
+// between calls. A reader needs nothing but that one word, so loads and
+// stores are relaxed device-scope atomics with no fences: a device-scope
+// fence would write back or invalidate the XCD's L2 on every publish and
+// poll. Only when the epoch wraps does every block fence before it finishes
+// and the last one clear the words.
+constexpr int ST_VBITS = 40;
+constexpr uint64_t ST_VAL = (1ull << ST_VBITS) - 1;
+constexpr uint64_t ST_AGG = 1ull << ST_VBITS;
+constexpr uint64_t ST_INC = 2ull << ST_VBITS;
+constexpr int ST_EPOCH_SHIFT = ST_VBITS + 2;
+constexpr uint64_t ST_EPOCH_MAX = (1ull << (64 - ST_EPOCH_SHIFT)) - 1;
 
+__device__ __forceinline__ uint64_t status_load(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void status_store(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// scratch: ctr[0] tile claims, ctr[1] blocks done, *epoch the last call's
+// epoch, status[ntiles]
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_dlb(const int32_t* __restrict__ in, int64_t* __restrict__ out,
                                                            int64_t* total, int64_t n_cap, const int64_t* n_dev,
-                                                           uint64_t* status, uint32_t* ctr, int64_t ntiles) {
+                                                           uint64_t* status, uint32_t* ctr, uint64_t* epoch,
+                                                           int64_t ntiles) {
     __shared__ int64_t lw[SORT_WAVES];
     __shared__ int64_t s_prefix;
     __shared__ uint32_t s_tile;
+    __shared__ uint64_t s_ep;
     const int64_t n = live_n(n_cap, n_dev);
-    if (threadIdx.x == 0) s_tile = atomicAdd(&ctr[0], 1u);
+    if (threadIdx.x == 0) {
+        s_tile = atomicAdd(&ctr[0], 1u);
+        s_ep = status_load(epoch) % ST_EPOCH_MAX + 1;
+    }
     __syncthreads();
     const int64_t tile = s_tile;
+    const uint64_t ep = s_ep;
+    const uint64_t tag = ep << ST_EPOCH_SHIFT;
     const int64_t base = tile * SCAN_TILE;
-    int32_t v[SCAN_ITEMS];
-    int64_t acc = 0;
+    if (base < n) {
+        int32_t v[SCAN_ITEMS];
+        int64_t acc = 0;
 #pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; ++k) {
-        const int64_t i = base + (int64_t)threadIdx.x * SCAN_ITEMS + k;
-        v[k] = (i < n) ? in[i] : 0;
-        acc += v[k];
-    }
-    int64_t agg;
-    const int64_t incl = block_inclusive_scan(acc, lw, &agg);
-    if (threadIdx.x == 0) {
-        int64_t prefix = 0;
-        if (tile == 0) {
-            __hip_atomic_store(&status[0], ST_INC | (uint64_t)agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            __hip_atomic_store(&status[tile], ST_AGG | (uint64_t)agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            int64_t j = tile - 1;
-            for (;;) {
-                const uint64_t s = __hip_atomic_load(&status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                const uint64_t f = s & ~ST_VAL;
-                if (f == 0) continue;  // predecessor not published yet (it has started: tiles are claimed in order)
-                prefix += (int64_t)(s & ST_VAL);
-                if (f == ST_INC) break;
-                --j;
-            }
-            __hip_atomic_store(&status[tile], ST_INC | (uint64_t)(prefix + agg), __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
+        for (int k = 0; k < SCAN_ITEMS; ++k) {
+            const int64_t i = base + (int64_t)threadIdx.x * SCAN_ITEMS + k;
+            v[k] = (i < n) ? in[i] : 0;
+            acc += v[k];
         }
-        s_prefix = prefix;
-        if (tile == ntiles - 1 && total) *total = prefix + agg;
-    }
-    __syncthreads();
-    int64_t run = incl - acc + s_prefix;
+        int64_t agg;
+        const int64_t incl = block_inclusive_scan(acc, lw, &agg);
+        if (threadIdx.x < MWD_WAVE) {
+            const int lane = threadIdx.x;
+            int64_t prefix = 0;
+            if (tile == 0) {
+                if (lane == 0) status_store(&status[0], tag | ST_INC | ((uint64_t)agg & ST_VAL));
+            } else {
+                if (lane == 0) status_store(&status[tile], tag | ST_AGG | ((uint64_t)agg & ST_VAL));
+                int64_t j = tile - 1;  // window: predecessors j, j-1, ..., j-63
+                for (;;) {
+                    const int64_t t = j - lane;
+                    uint64_t sw = t >= 0 ? status_load(&status[t]) : (tag | ST_INC);
+                    if ((sw >> ST_EPOCH_SHIFT) != ep) sw = 0;  // another call's word: not published
+                    const uint64_t f = sw & (ST_AGG | ST_INC);
+                    const uint64_t inc = __ballot(f == ST_INC);
+                    const uint64_t none = __ballot(f == 0);
+                    // lanes up to and including the nearest inclusive predecessor
+                    const uint64_t upto = inc ? (inc & (~inc + 1)) * 2 - 1 : ~0ull;
+                    if (none & upto) continue;  // a predecessor in range has not published yet
+                    int64_t x = ((upto >> lane) & 1) ? (int64_t)(sw & ST_VAL) : 0;
 #pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; ++k) {
-        const int64_t i = base + (int64_t)threadIdx.x * SCAN_ITEMS + k;
-        if (i < n) out[i] = run;
-        run += v[k];
+                    for (int d = MWD_WAVE / 2; d >= 1; d >>= 1) x += __shfl_xor(x, d, MWD_WAVE);
+                    prefix += x;
+                    if (inc) break;
+                    j -= MWD_WAVE;
+                }
+                if (lane == 0) status_store(&status[tile], tag | ST_INC | ((uint64_t)(prefix + agg) & ST_VAL));
+            }
+            if (lane == 0) {
+                s_prefix = prefix;
+                if (total && base + SCAN_TILE >= n) *total = prefix + agg;  // the last live tile
+            }
+        }
+        __syncthreads();
+        int64_t run = incl - acc + s_prefix;
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; ++k) {
+            const int64_t i = base + (int64_t)threadIdx.x * SCAN_ITEMS + k;
+            if (i < n) out[i] = run;
+            run += v[k];
+        }
+    } else if (tile == 0 && total && threadIdx.x == 0) {
+        *total = 0;  // nothing live
     }
-    // the last block out resets the status words and the counters
+    // the last block out resets the counters and advances the epoch
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();
+        if (ep == ST_EPOCH_MAX) __threadfence();  // status stores land before the clear below
         const uint32_t done = atomicAdd(&ctr[1], 1u);
         s_tile = (done == (uint32_t)(ntiles - 1)) ? 1u : 0u;
     }
     __syncthreads();
     if (s_tile) {
-        for (int64_t j = threadIdx.x; j < ntiles; j += SCAN_THREADS) status[j] = 0;
+        if (ep == ST_EPOCH_MAX) {
+            __threadfence();
+            for (int64_t j = threadIdx.x; j < ntiles; j += SCAN_THREADS) status_store(&status[j], 0);
+        }
         if (threadIdx.x == 0) {
-            ctr[0] = 0;
-            ctr[1] = 0;
+            atomicExch(&ctr[0], 0u);
+            atomicExch(&ctr[1], 0u);
+            status_store(epoch, ep == ST_EPOCH_MAX ? 0 : ep);
         }
     }
 }
@@ -129,24 +174,31 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_dlb(const int32_t* __rest
 // --------------------------------------------------------------- radix sort
 // LSD radix sort with digits of up to BITS bits (2^BITS bins); the digit of a
 // pass is (key >> shift) & (2^width - 1).
+// The histogram covers the live tiles only: digit d of tile t sits at
+// hist[d * nlive + t] for digits d <= mask, and block 0 stores the entry count
+// (mask + 1) * nlive in *hcount for the scan, so the scan's cost follows the
+// live count rather than the capacity.
+__device__ __forceinline__ int64_t live_tiles(int64_t n) { return (n + SORT_TILE - 1) / SORT_TILE; }
+
 template <int BITS>
 __global__ void __launch_bounds__(SORT_THREADS) k_radix_hist(const uint32_t* __restrict__ keys, int64_t n_cap,
                                                              const int64_t* n_dev, int shift, uint32_t mask,
-                                                             int32_t* __restrict__ hist, int64_t ntiles) {
+                                                             int32_t* __restrict__ hist, int64_t* hcount) {
     constexpr int NB = 1 << BITS;
     __shared__ int32_t h[NB];
+    const int64_t n = live_n(n_cap, n_dev);
+    const int64_t nlive = live_tiles(n);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *hcount = (int64_t)(mask + 1) * nlive;
+    if ((int64_t)blockIdx.x >= nlive) return;  // uniform per block
     for (int i = threadIdx.x; i < NB; i += SORT_THREADS) h[i] = 0;
     __syncthreads();
-    const int64_t n = live_n(n_cap, n_dev);
     const int64_t base = (int64_t)blockIdx.x * SORT_TILE;
-    if (base < n) {
-        for (int k = 0; k < SORT_ROUNDS; ++k) {
-            const int64_t i = base + (int64_t)k * SORT_THREADS + threadIdx.x;
-            if (i < n) atomicAdd(&h[(keys[i] >> shift) & mask], 1);
-        }
+    for (int k = 0; k < SORT_ROUNDS; ++k) {
+        const int64_t i = base + (int64_t)k * SORT_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & mask], 1);
     }
     __syncthreads();
-    for (int d = threadIdx.x; d < NB; d += SORT_THREADS) hist[(int64_t)d * ntiles + blockIdx.x] = h[d];
+    for (int d = threadIdx.x; d <= (int)mask; d += SORT_THREADS) hist[(int64_t)d * nlive + blockIdx.x] = h[d];
 }
 
 // Stable scatter. Item order inside a tile is wave-major, then round, then
@@ -157,8 +209,7 @@ __global__ void __launch_bounds__(SORT_THREADS) k_radix_scatter(const uint32_t* 
                                                                 uint32_t* __restrict__ keys_out,
                                                                 int32_t* __restrict__ vals_out, int64_t n_cap,
                                                                 const int64_t* n_dev, int shift, int width,
-                                                                const int64_t* __restrict__ hist_off,
-                                                                int64_t ntiles) {
+                                                                const int64_t* __restrict__ hist_off) {
     constexpr int NB = 1 << BITS;
     __shared__ int32_t cnt[SORT_WAVES][NB];
     const int lane = threadIdx.x & (MWD_WAVE - 1);
@@ -166,7 +217,8 @@ __global__ void __launch_bounds__(SORT_THREADS) k_radix_scatter(const uint32_t* 
     for (int i = threadIdx.x; i < SORT_WAVES * NB; i += SORT_THREADS) (&cnt[0][0])[i] = 0;
     __syncthreads();
     const int64_t n = live_n(n_cap, n_dev);
-    if ((int64_t)blockIdx.x * SORT_TILE >= n) return;  // uniform per block
+    const int64_t nlive = live_tiles(n);
+    if ((int64_t)blockIdx.x >= nlive) return;  // uniform per block
     const int64_t base = (int64_t)blockIdx.x * SORT_TILE + (int64_t)wave * (MWD_WAVE * SORT_ROUNDS);
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (MWD_WAVE - lane));
     const uint32_t mask = (1u << width) - 1u;
@@ -208,7 +260,7 @@ __global__ void __launch_bounds__(SORT_THREADS) k_radix_scatter(const uint32_t* 
     for (int rd = 0; rd < SORT_ROUNDS; ++rd) {
         if (r[rd] < 0) continue;
         const uint32_t d = (k[rd] >> shift) & mask;
-        const int64_t pos = hist_off[(int64_t)d * ntiles + blockIdx.x] + cnt[wave][d] + r[rd];
+        const int64_t pos = hist_off[(int64_t)d * nlive + blockIdx.x] + cnt[wave][d] + r[rd];
         keys_out[pos] = k[rd];
         vals_out[pos] = v[rd];
     }
@@ -237,20 +289,21 @@ int mwd_scan_i32(const int32_t* in, int64_t* out, int64_t* total, int64_t n_cap,
         return MWD_OK;
     }
     const int64_t nb = (n_cap + SCAN_TILE - 1) / SCAN_TILE;
-    // scratch layout: [0] two uint32 counters, [1 .. nb] tile status words
-    if (nb + 1 > tmp_len) {
+    // scratch layout: [0] two uint32 counters, [1] epoch, [2 .. nb+1] tile status words
+    if (nb + 2 > tmp_len) {
         mwd_set_error("scan scratch too small");
         return MWD_E_CAPACITY;
     }
     hipLaunchKernelGGL(k_scan_dlb, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, s, in, out, total, n_cap, n_dev,
-                       reinterpret_cast<uint64_t*>(tmp + 1), reinterpret_cast<uint32_t*>(tmp), nb);
+                       reinterpret_cast<uint64_t*>(tmp + 2), reinterpret_cast<uint32_t*>(tmp),
+                       reinterpret_cast<uint64_t*>(tmp + 1), nb);
     MWD_LAUNCH_CHECK();
     return MWD_OK;
 }
 
 int mwd_radix_sort(uint32_t* keys, int32_t* vals, uint32_t* keys_alt, int32_t* vals_alt, int64_t n_cap,
-                   const int64_t* n_dev, int key_bits, int32_t* hist, int64_t* hist_off, int64_t* tmp,
-                   int64_t tmp_len, hipStream_t s, uint32_t** keys_out, int32_t** vals_out) {
+                   const int64_t* n_dev, int key_bits, int32_t* hist, int64_t* hist_off, int64_t* hcount,
+                   int64_t* tmp, int64_t tmp_len, hipStream_t s, uint32_t** keys_out, int32_t** vals_out) {
     uint32_t* ka = keys;
     int32_t* va = vals;
     uint32_t* kb = keys_alt;
@@ -268,19 +321,19 @@ int mwd_radix_sort(uint32_t* keys, int32_t* vals, uint32_t* keys_alt, int32_t* v
             const uint32_t mask = (1u << w) - 1u;
             if (wide)
                 hipLaunchKernelGGL(k_radix_hist<MWD_SORT_MAX_BITS>, dim3((unsigned)nt), dim3(SORT_THREADS), 0, s, ka,
-                                   n_cap, n_dev, shift, mask, hist, nt);
+                                   n_cap, n_dev, shift, mask, hist, hcount);
             else
                 hipLaunchKernelGGL(k_radix_hist<8>, dim3((unsigned)nt), dim3(SORT_THREADS), 0, s, ka, n_cap, n_dev,
-                                   shift, mask, hist, nt);
+                                   shift, mask, hist, hcount);
             MWD_LAUNCH_CHECK();
-            int rc = mwd_scan_i32(hist, hist_off, nullptr, bins * nt, nullptr, tmp, tmp_len, s);
+            int rc = mwd_scan_i32(hist, hist_off, nullptr, bins * nt, hcount, tmp, tmp_len, s);
             if (rc) return rc;
             if (wide)
                 hipLaunchKernelGGL(k_radix_scatter<MWD_SORT_MAX_BITS>, dim3((unsigned)nt), dim3(SORT_THREADS), 0, s,
-                                   ka, va, kb, vb, n_cap, n_dev, shift, w, hist_off, nt);
+                                   ka, va, kb, vb, n_cap, n_dev, shift, w, hist_off);
             else
                 hipLaunchKernelGGL(k_radix_scatter<8>, dim3((unsigned)nt), dim3(SORT_THREADS), 0, s, ka, va, kb, vb,
-                                   n_cap, n_dev, shift, w, hist_off, nt);
+                                   n_cap, n_dev, shift, w, hist_off);
             MWD_LAUNCH_CHECK();
             uint32_t* tk = ka; ka = kb; kb = tk;
             int32_t* tv = va; va = vb; vb = tv;

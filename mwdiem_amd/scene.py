@@ -463,6 +463,23 @@ def config_large(seed: int = 2, n: int = 100_000) -> SceneData:
     return make_scene(n, (200.0, 200.0, 400.0), seed, npts=(8, 20), name=f"C2-large-{n}")
 
 
+def config_pile(seed: int = 5, n: int = 4096, v_pour: float = 100.0, law: Law4 | None = None):
+    """The differentiable form's pile: n hulls of 8-20 points on ellipsoids of
+    semi-axes (1.0, 0.8, 0.6) x U(0.6, 0.9) mm, packed from the floor of a
+    40x40x60 mm box (bounding spheres 2% apart; 4096 of the larger C2-sized
+    hulls would not fit as a lattice) and poured, i.e. given random velocities
+    of rms v_pour mm/s, so contacts open and close through the run. Physical
+    law Kn=1e4, en=et=0.5, mu=0.3. Returns the scene and the initial
+    velocities [n_rows, 3] (zero for walls)."""
+    law = law or Law4(Kn=1e4, en=0.5, et=0.5, mu=0.3, mode=PHYSICAL)
+    sc = make_scene(n, (40.0, 40.0, 60.0), seed, npts=(8, 20), size=(0.6, 0.9), law=law, spacing_factor=1.02,
+                    name=f"C5-pile-{n}")
+    rng = np.random.Generator(np.random.PCG64([seed, 11]))
+    v = np.zeros((sc.n_rows, 3))
+    v[sc.n_walls:] = rng.normal(0.0, v_pour / math.sqrt(3.0), size=(n, 3))
+    return sc, v
+
+
 def compact_dense(sc: SceneData, gap: float = -0.02, seed: int = 0) -> SceneData:
     """Copy of a scene with the particles moved onto a BCC lattice of spacing
     (1+gap)*2*r_max from the floor up, so neighbouring hulls overlap (a
@@ -479,3 +496,144 @@ def compact_dense(sc: SceneData, gap: float = -0.02, seed: int = 0) -> SceneData
         out.com[r] = sites[i]
         out.verts[r, :out.nverts[r]] += d
     return out
+
+
+# ------------------------------------------------------------ batched form
+def shape_pool(n_shapes: int, seed: int, npts=(8, 20), axes=(1.0, 0.8, 0.6), size=(0.8, 1.2),
+               density: float = RHO_KG_MM3):
+    """n_shapes distinct hulls (body frame, centred on the vertex mean) with
+    mass, inertia size and inertia tensor; scenes draw from it and give each
+    particle its own random orientation."""
+    rng = np.random.Generator(np.random.PCG64([seed, 7]))
+    out = []
+    for _ in range(n_shapes):
+        k = int(rng.integers(npts[0], npts[1] + 1))
+        sz = float(rng.uniform(size[0], size[1]))
+        pts = ellipsoid_points(rng, k, np.asarray(axes) * sz)
+        V, tri, vol = hull_body(pts)
+        V = V - V.mean(axis=0)
+        I, isize = polyhedron_inertia(V, tri, density)
+        out.append((V, density * vol, isize, I))
+    return out
+
+
+def random_rotations(rng: np.random.Generator, n: int) -> np.ndarray:
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    R = np.empty((n, 3, 3))
+    R[:, 0, 0] = 1 - 2 * (y * y + z * z)
+    R[:, 0, 1] = 2 * (x * y - w * z)
+    R[:, 0, 2] = 2 * (x * z + w * y)
+    R[:, 1, 0] = 2 * (x * y + w * z)
+    R[:, 1, 1] = 1 - 2 * (x * x + z * z)
+    R[:, 1, 2] = 2 * (y * z - w * x)
+    R[:, 2, 0] = 2 * (x * z - w * y)
+    R[:, 2, 1] = 2 * (y * z + w * x)
+    R[:, 2, 2] = 1 - 2 * (x * x + y * y)
+    return R
+
+
+def hopper_scene(n_particles: int, seed: int, pool, law: Law4, orifice: float = 12.0, bin_height: float = 40.0,
+                 density: float = RHO_KG_MM3, dt: float = 1e-6, name: str = "") -> SceneData:
+    """A flat-bottomed silo discharging through a slot orifice: particles seeded
+    on a jittered lattice in the upper chamber, whose floor is two plates with
+    a gap of `orifice` mm; a collection bin of `bin_height` mm below. Particles
+    come from a shape pool with a random orientation each. Walls: floor,
+    ceiling, x-, x+, y-, y+ and the two orifice plates (8 walls)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    idx = rng.integers(0, len(pool), size=n_particles)
+    prad = np.array([float(np.max(np.linalg.norm(p[0], axis=1))) for p in pool])
+    rmax = float(prad[idx].max())
+    ms = 2.0 * rmax
+    spacing = 1.05 * ms
+    dens = math.sqrt(2.0) / spacing ** 3          # BCC sites per mm^3
+    w = max(4 * ms + orifice, (n_particles / dens / 1.5) ** (1.0 / 3.0) + 2 * ms)
+    h_fill = 1.5 * w
+    for _ in range(20):
+        try:
+            sites = bcc_sites((w, w, h_fill), spacing, rmax, n_particles)
+            break
+        except ValueError:
+            h_fill *= 1.15
+    z0 = bin_height
+    Lz = z0 + h_fill + ms
+    sites = sites + np.array([0.0, 0.0, z0])
+    jit = (spacing - ms) / 2.0 / math.sqrt(3.0)
+    sites = sites + rng.uniform(-jit, jit, size=sites.shape)
+    T = 10.0 * ms
+    walls = walls_for_box((w, w, Lz), T, density, False)
+    tp = 2.0 * ms  # orifice plate thickness
+    for lo, hi in (((0.0, 0.0, z0 - tp), (w / 2 - orifice / 2, w, z0)),
+                   ((w / 2 + orifice / 2, 0.0, z0 - tp), (w, w, z0))):
+        P = box_corners(lo, hi)
+        V, tri, vol = hull_body(P)
+        c = V.mean(axis=0)
+        I, isize = polyhedron_inertia(V - c, tri, density)
+        walls.append((V, np.array([0.0, 0.0, 1.0]), c, density * vol, I, np.linalg.inv(I), isize))
+    W = len(walls)
+    N = W + n_particles
+    vmax = max([wl[0].shape[0] for wl in walls] + [p[0].shape[0] for p in pool])
+    com = np.zeros((N, 3))
+    verts = np.zeros((N, vmax, 3))
+    nverts = np.zeros(N, dtype=np.int32)
+    mass = np.zeros(N)
+    I = np.zeros((N, 3, 3))
+    Iinv = np.zeros((N, 3, 3))
+    isize = np.zeros(N)
+    wall_n = np.zeros((N, 3))
+    wall_c = np.zeros((N, 3))
+    for k, (V, nrm, c, m, Ik, Iik, isz) in enumerate(walls):
+        com[k] = c
+        verts[k, :V.shape[0]] = V
+        nverts[k] = V.shape[0]
+        mass[k], I[k], Iinv[k], isize[k] = m, Ik, Iik, isz
+        wall_n[k], wall_c[k] = nrm, c
+    # vectorised: pooled body vertices and tensors, one random rotation each
+    pv = max(p[0].shape[0] for p in pool)
+    Vp = np.zeros((len(pool), pv, 3))
+    npool = np.array([p[0].shape[0] for p in pool], dtype=np.int32)
+    for j, p in enumerate(pool):
+        Vp[j, :p[0].shape[0]] = p[0]
+    Ip = np.stack([p[3] for p in pool])
+    R = random_rotations(rng, n_particles)
+    Vw = np.einsum("nij,nkj->nki", R, Vp[idx])
+    Iw = np.einsum("nij,njk,nlk->nil", R, Ip[idx], R)
+    pad = np.arange(pv)[None, :] >= npool[idx][:, None]
+    Vw = Vw + sites[:, None, :]
+    Vw[pad] = 0.0
+    com[W:] = sites
+    verts[W:, :pv] = Vw
+    nverts[W:] = npool[idx]
+    mass[W:] = np.array([p[1] for p in pool])[idx]
+    I[W:] = Iw
+    Iinv[W:] = np.linalg.inv(Iw)
+    isize[W:] = np.array([p[2] for p in pool])[idx]
+    allv = np.concatenate([wl[0] for wl in walls])
+    box = np.array([allv[:, 0].max(), allv[:, 0].min(), allv[:, 1].max(), allv[:, 1].min(),
+                    allv[:, 2].max(), allv[:, 2].min()])
+    cell, ncx, ncy, ncz, kmin, kspan = grid_for(box, ms)
+    fext = mass[:, None] * np.array([0.0, 0.0, -G_MM])[None, :]
+    fext[:W] = 0.0
+    return SceneData(n_walls=W, vmax=vmax, com=com, verts=verts, nverts=nverts, mass=mass, I=I, Iinv=Iinv,
+                     isize=isize, fext=fext, text=np.zeros((N, 3)), wall_n=wall_n, wall_c=wall_c, box=box,
+                     cell=cell, ncx=ncx, ncy=ncy, ncz=ncz, kmin=kmin, kspan=kspan, max_particlesize=ms,
+                     law_p=law, law_w=law, dt=dt, name=name, inner=(w, w, Lz))
+
+
+def config_batched(n_scenes: int = 256, seed: int = 4, n_min: int = 500, n_max: int = 20_000,
+                   n_shapes: int = 4096, mode: int = PHYSICAL):
+    """The batched form: n_scenes hopper-discharge scenes with particle counts
+    log-uniform in [n_min, n_max], friction U(0.1,0.6), Kn log-uniform
+    5e3-5e4, restitution U(0.2,0.8); scene s is drawn from seed 4*10^6+s."""
+    pool = shape_pool(n_shapes, seed)
+    scenes = []
+    for s in range(n_scenes):
+        rng = np.random.Generator(np.random.PCG64(4_000_000 * (seed // 4 if seed >= 4 else 1) + s))
+        n = int(round(math.exp(rng.uniform(math.log(n_min), math.log(n_max)))))
+        mu = float(rng.uniform(0.1, 0.6))
+        Kn = float(math.exp(rng.uniform(math.log(5e3), math.log(5e4))))
+        e = float(rng.uniform(0.2, 0.8))
+        law = Law4(Kn=Kn, en=e, et=e, mu=mu, mode=mode)
+        scenes.append(hopper_scene(n, int(rng.integers(1 << 31)), pool, law, name=f"hopper-{s}-{n}"))
+    return scenes

@@ -158,22 +158,29 @@ class Stepper:
             _lib.check(lib.mwd_step(self.ws, C.byref(self.state)), "mwd_step")
         self.steps_done += n
 
-    def capture(self, steps_per_graph: int = 10):
-        """Capture steps_per_graph consecutive steps in one HIP graph."""
+    def capture(self, steps_per_graph: int = 10, tangent: bool = False):
+        """Capture steps_per_graph consecutive steps (tangent steps if asked)
+        in one HIP graph."""
         self._enter()
         torch.cuda.synchronize(self.device)
         g = torch.cuda.CUDAGraph()
         lib = _lib.load()
         with torch.cuda.graph(g, stream=self.stream):
             for _ in range(steps_per_graph):
-                _lib.check(lib.mwd_step(self.ws, C.byref(self.state)), "mwd_step")
+                if tangent:
+                    _lib.check(lib.mwd_step_tangent(self.ws, C.byref(self.state), self._tan_ptrs),
+                               "mwd_step_tangent")
+                else:
+                    _lib.check(lib.mwd_step(self.ws, C.byref(self.state)), "mwd_step")
         torch.cuda.synchronize(self.device)
         self._graph = g
         self._graph_steps = steps_per_graph
+        self._graph_tangent = tangent
         return g
 
     def run(self, n: int):
-        """n steps: graph replays for the bulk (when captured), eager steps for the rest."""
+        """n steps: graph replays for the bulk (when captured), eager steps for
+        the rest (tangent steps when the graph holds tangent steps)."""
         g = getattr(self, "_graph", None)
         done = 0
         if g is not None:
@@ -184,7 +191,10 @@ class Stepper:
                     done += self._graph_steps
             self.steps_done += done
         if n > done:
-            self.step(n - done)
+            if getattr(self, "_graph_tangent", False):
+                self.step_tangent(n - done)
+            else:
+                self.step(n - done)
 
     def set_timing(self, on: bool = True):
         _lib.check(_lib.load().mwd_set_timing(self.ws, 1 if on else 0), "mwd_set_timing")
@@ -195,6 +205,45 @@ class Stepper:
         n = C.c_int64()
         _lib.check(_lib.load().mwd_stage_times(self.ws, ms, C.byref(n)), "mwd_stage_times")
         return dict(steps=n.value, ms=dict(zip(_lib.STAGE_NAMES, list(ms))))
+
+    # ------------------------------------------------------------ tangents
+    def enable_tangents(self, seed=None):
+        """Carry d(state)/d(theta) for three law parameters theta_k.
+
+        seed[i][k] = d(input_i)/d(theta_k) for the law inputs (Kn, en, et, mu),
+        applied to both laws (particle and wall) of every scene. The default
+        theta is (Kn, en, mu), with et held fixed. Tangents start at zero: the
+        initial state does not depend on theta."""
+        lib = _lib.load()
+        if seed is None:
+            seed = [[1, 0, 0], [0, 1, 0], [0, 0, 0], [0, 0, 1]]
+        tans = (_lib.LawTan * (2 * len(self.scenes)))()
+        for s, sc in enumerate(self.scenes):
+            for j, L in enumerate((sc.law_p, sc.law_w)):
+                tans[2 * s + j] = _lib.law_tangent(L.Kn, L.en, L.et, L.mu, seed)
+        self._enter()
+        torch.cuda.synchronize(self.device)
+        _lib.check(lib.mwd_set_law_tangents(self.ws, tans), "mwd_set_law_tangents")
+        self.tan = {k: torch.zeros((_lib.N_TAN,) + tuple(self.t[k].shape), dtype=torch.float64, device=self.device)
+                    for k in _lib.TAN_ARRAYS}
+        arr = (C.c_void_p * (len(_lib.TAN_ARRAYS) * _lib.N_TAN))()
+        for a, k in enumerate(_lib.TAN_ARRAYS):
+            base, n = self.tan[k].data_ptr(), self.tan[k][0].numel()
+            for j in range(_lib.N_TAN):
+                arr[_lib.N_TAN * a + j] = base + 8 * n * j
+        self._tan_ptrs = arr
+
+    def reset_tangents(self):
+        for v in self.tan.values():
+            v.zero_()
+
+    def step_tangent(self, n: int = 1):
+        """n steps that also advance the tangents (values identical to step())."""
+        lib = _lib.load()
+        self._enter()
+        for _ in range(n):
+            _lib.check(lib.mwd_step_tangent(self.ws, C.byref(self.state), self._tan_ptrs), "mwd_step_tangent")
+        self.steps_done += n
 
     # --------------------------------------------------------------- status
     def status(self) -> dict:

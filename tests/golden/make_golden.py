@@ -22,10 +22,9 @@ from mwdiem_amd import scene  # noqa: E402
 from oracle import oracle as O  # noqa: E402
 
 OUT = Path(__file__).resolve().parent
-STEPS = 10
 
 
-def fixture(sc, name):
+def fixture(sc, name, steps):
     st = O.initial_state(sc)
     W = sc.n_walls
     rng = np.random.Generator(np.random.PCG64(77))
@@ -34,12 +33,12 @@ def fixture(sc, name):
     st0 = {k: v.copy() for k, v in st.items()}
     pairs = O.candidates(st, sc)
     coll, depth, nrm, ca, cb, F, T, est = O.contact_state(pairs, st, sc)
-    for _ in range(STEPS):
+    for _ in range(steps):
         O.step(st, sc)
     d = sc.to_arrays()
     d.update({f"in_{k}": v for k, v in st0.items()})
     d.update(pairs=pairs, collision=coll, depth=depth, normal=nrm, cpA=ca, cpB=cb, force=F, torque=T,
-             steps=np.array(STEPS))
+             steps=np.array(steps))
     d.update({f"out_{k}": st[k] for k in ("com", "v", "w", "q", "q12", "qall", "verts")})
     np.savez_compressed(OUT / f"{name}.npz", **d)
     print(name, "pairs", len(pairs), "contacts", int((coll == 1).sum()))
@@ -47,9 +46,12 @@ def fixture(sc, name):
 
 def main():
     base = scene.make_scene(24, (6.0, 6.0, 12.0), 41, name="golden")
-    fixture(scene.compact_dense(base, gap=-0.55), "dense_ref")
+    # the reference law amplifies a one-ulp difference about tenfold per step
+    # on this overlapped bed (its damping uses mu as a mass), so its fixture
+    # stops after 2 steps; the physical law is stable over 10
+    fixture(scene.compact_dense(base, gap=-0.55), "dense_ref", 2)
     phys = scene.make_scene(24, (6.0, 6.0, 12.0), 43, law=scene.Law4(mode=scene.PHYSICAL), name="golden-phys")
-    fixture(scene.compact_dense(phys, gap=-0.55), "dense_phys")
+    fixture(scene.compact_dense(phys, gap=-0.55), "dense_phys", 10)
 
 
 if __name__ == "__main__":

@@ -119,3 +119,23 @@ def test_inertia_of_a_cube():
     assert abs(vol - 8.0) < 1e-12
     assert np.allclose(I, np.eye(3) * m * s * s / 6, rtol=1e-12, atol=1e-12)
     assert abs(isize - m * s * s / 6) < 1e-12
+
+
+def test_law_tangent_matches_central_differences():
+    """mwd_law_tangent's derivatives of every coefficient against central
+    differences of mwd_law_init_mode, for a seed that mixes the inputs."""
+    from mwdiem_amd import _lib
+    base = np.array([1e4, 0.5, 0.4, 0.3])
+    seed = np.array([[1.0, 0.0, 2.0], [0.0, 1.0, 0.5], [0.0, 1.0, 0.0], [0.3, 0.0, 1.0]])  # d input / d theta
+    T = _lib.law_tangent(*base, seed)
+    names = ("Kn", "mu", "Gn", "Kt_num", "inv_mu", "ln_et", "cn", "ct", "ratio")
+    for k in range(3):
+        h = 1e-6
+        xp = base + h * seed[:, k]
+        xm = base - h * seed[:, k]
+        Lp = _lib.law(*xp, 1e-6, _lib.LAW_PHYSICAL)
+        Lm = _lib.law(*xm, 1e-6, _lib.LAW_PHYSICAL)
+        for n in names:
+            fd = (getattr(Lp, n) - getattr(Lm, n)) / (2 * h)
+            got = getattr(T, n)[k]
+            assert abs(got - fd) <= 1e-6 * max(1.0, abs(fd)), (n, k, got, fd)
