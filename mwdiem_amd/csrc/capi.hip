@@ -44,7 +44,7 @@ void free_ws(mwd_workspace* ws) {
     void* ptrs[] = {ws->scenes_dev, ws->cell_of, ws->plist, ws->scene_of, ws->is_wall, ws->keys, ws->keys_alt, ws->vals,
                     ws->vals_alt, ws->head, ws->head_off, ws->cell_start, ws->cell_key, ws->cell_nb, ws->row_first,
                     ws->row_rank, ws->cell_by_rank, ws->cell_cnt, ws->cell_off, ws->perm, ws->wall_flag, ws->wall_off,
-                    ws->pairs, ws->collision, ws->simplex, ws->sidx, ws->pair_flag, ws->contact_off, ws->contact_pair,
+                    ws->pairs, ws->collision, ws->simplex, ws->sidx, ws->pair_flag, ws->surv, ws->contact_off, ws->contact_pair,
                     ws->depth, ws->normal, ws->cpA, ws->cpB, ws->Fn, ws->tA, ws->tB, ws->epa_status, ws->rec_key,
                     ws->rec_key_alt, ws->rec_val, ws->rec_val_alt, ws->seg_begin, ws->seg_end, ws->hist,
                     ws->hist_off, ws->scan_tmp, ws->counters, ws->flags, ws->laws, ws->scene_wt_off, ws->ctan,
@@ -319,6 +319,7 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     A(simplex, contact_cap);
     A(sidx, pair_cap);
     A(pair_flag, pair_cap);
+    A(surv, pair_cap);
     A(contact_off, pair_cap);
     A(contact_pair, contact_cap);
     A(depth, contact_cap);
@@ -410,7 +411,9 @@ MWD_API int mwd_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t
         mwd_set_error("mwd_candidates: invalid argument");
         return MWD_E_INVALID;
     }
-    return mwd_launch_candidates(ws, st, keys);
+    const int rc = mwd_launch_candidates(ws, st, keys);
+    if (!rc) ws->fresh_pairs = 1;  // k_finish_pairs zeroed the fine-contact counters
+    return rc;
 }
 
 MWD_API int mwd_pairs_view(mwd_workspace* ws, int32_t** pairs, int64_t** n_pairs) {
@@ -452,10 +455,13 @@ static int fine_contact(mwd_workspace* ws, const mwd_state* st, double* const* t
     const BodyView A = state_view(ws, st, 0), B = state_view(ws, st, 1);
     int64_t* ctr = ws->counters;
     mwd_mark(ws, 2);
-    int rc = mwd_set_i64(ctr + 8, 0, s);
-    if (rc) return rc;
-    rc = mwd_launch_gjk(A, B, ws->pair_cap, ctr + 3, ws->collision, ws->simplex, ws->sidx, ctr + 8, ws->contact_cap,
-                        ws->flags, s);
+    int rc = MWD_OK;
+    if (!ws->fresh_pairs) {  // a second pass over the same candidates: restart the slot and survivor counters
+        if ((rc = mwd_set_i64(ctr + 8, 0, s)) || (rc = mwd_set_i64(ctr + 10, 0, s))) return rc;
+    }
+    ws->fresh_pairs = 0;
+    rc = mwd_launch_gjk(A, B, ws->pair_cap, ctr + 3, ws->collision, ws->pair_flag, ws->surv, ctr + 10, ws->simplex,
+                        ws->sidx, ctr + 8, ws->contact_cap, ws->flags, s);
     if (rc) return rc;
     mwd_mark(ws, 3);
     rc = mwd_launch_compact_contacts(ws->collision, ws->pair_cap, ctr + 3, ws->pair_flag, ws->contact_off,
@@ -489,9 +495,7 @@ static int fine_contact(mwd_workspace* ws, const mwd_state* st, double* const* t
     if (rc) return rc;
     ws->rec_key_sorted = ks;
     ws->rec_val_sorted = vs;
-    rc = mwd_fill_i32(ws->seg_begin, 0, ws->n_rows, s);
-    if (rc) return rc;
-    rc = mwd_fill_i32(ws->seg_end, 0, ws->n_rows, s);
+    rc = mwd_fill2_i32(ws->seg_begin, ws->seg_end, 0, ws->n_rows, s);
     if (rc) return rc;
     return mwd_launch_segments(ks, 2 * ws->contact_cap, ctr + 5, ws->seg_begin, ws->seg_end, s);
 }
@@ -664,7 +668,7 @@ MWD_API int mwd_contact_pairs(int64_t P, int32_t vmax, const double* comA, const
     hipStream_t s = (hipStream_t)stream;
     // temporaries of this call
     int32_t *coll = nullptr, *flag = nullptr, *cpair = nullptr, *est = nullptr, *rval = nullptr, *rval2 = nullptr;
-    int32_t *sb = nullptr, *se = nullptr, *hist = nullptr, *sidx = nullptr;
+    int32_t *sb = nullptr, *se = nullptr, *hist = nullptr, *sidx = nullptr, *surv = nullptr;
     uint8_t *wallA = nullptr, *wallB = nullptr;
     Simplex* sx = nullptr;
     int64_t *off = nullptr, *ctr = nullptr, *hist_off = nullptr, *tmp = nullptr;
@@ -694,6 +698,7 @@ MWD_API int mwd_contact_pairs(int64_t P, int32_t vmax, const double* comA, const
     al(&wallB, P);
     al(&sx, P);
     al(&sidx, P);
+    al(&surv, P);
     al(&off, P);
     al(&ctr, 16);
     al(&hist, hist_len);
@@ -746,7 +751,7 @@ MWD_API int mwd_contact_pairs(int64_t P, int32_t vmax, const double* comA, const
         BodyView B{comB, vertsB, nB, vB, wB, qA, isB, nullptr, nullptr, wallB, nullptr, nullptr, 1, vmax};
         if ((rc = mwd_set_i64(ctr + 3, P, s))) break;
         if ((rc = mwd_launch_zero_pairs(P, depth, normal, cpA, cpB, epa_status, disp, s))) break;
-        if ((rc = mwd_launch_gjk(A, B, P, ctr + 3, coll, sx, sidx, ctr + 8, P, flags, s))) break;
+        if ((rc = mwd_launch_gjk(A, B, P, ctr + 3, coll, flag, surv, ctr + 10, sx, sidx, ctr + 8, P, flags, s))) break;
         if ((rc = mwd_launch_compact_contacts(coll, P, ctr + 3, flag, off, cpair, P, ctr + 4, flags, tmp, tmp_len,
                                               s)))
             break;

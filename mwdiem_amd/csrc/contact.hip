@@ -232,30 +232,66 @@ __device__ int gjk_pair(const Side& A, const Side& B, d3 comA, d3 comB, d3* S, i
     return -1;
 }
 
-__global__ void __launch_bounds__(256) k_gjk(BodyView A, BodyView B, int64_t n_cap, const int64_t* n_dev,
-                                             int32_t* __restrict__ collision, Simplex* __restrict__ simplex,
-                                             int32_t* __restrict__ sidx, int64_t* slots, int64_t slot_cap,
-                                             uint32_t* flags) {
+// Bounding-sphere test over all pairs. Pairs whose spheres are apart are
+// decided here (the first support test of .cu:1013 rejects them); the rest
+// are appended to surv with one atomic per wave, so GJK then runs on full
+// waves of live pairs instead of waves that mostly idle.
+__global__ void __launch_bounds__(256) k_sphere(BodyView A, BodyView B, int64_t n_cap, const int64_t* n_dev,
+                                                int32_t* __restrict__ collision, int32_t* __restrict__ flag,
+                                                int32_t* __restrict__ surv, int64_t* n_surv) {
     int64_t n = n_dev ? *n_dev : n_cap;
     if (n > n_cap) n = n_cap;
+    const int lane = threadIdx.x & (MWD_WAVE - 1);
+    const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (MWD_WAVE - lane));
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
-        const int64_t ra = view_row(A, p), rb = view_row(B, p);
-        const d3 ca = ld3(A.com + 3 * ra), cb = ld3(B.com + 3 * rb);
-        if (A.radius) {
-            // bounding spheres apart: the first support test (.cu:1013) rejects the pair
-            const double rr = A.radius[ra] + B.radius[rb];
-            if (nrm2(cb - ca) > rr * rr) {
+    // p0 is uniform across a wave, so every lane of a wave reaches each ballot
+    for (int64_t p0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x - lane); p0 < n; p0 += stride) {
+        const int64_t p = p0 + lane;
+        bool keep = false;
+        if (p < n) {
+            keep = true;
+            if (A.radius) {
+                const int64_t ra = view_row(A, p), rb = view_row(B, p);
+                const d3 ca = ld3(A.com + 3 * ra), cb = ld3(B.com + 3 * rb);
+                const double rr = A.radius[ra] + B.radius[rb];
+                keep = !(nrm2(cb - ca) > rr * rr);
+            }
+            if (!keep) {
                 collision[p] = 0;
-                continue;
+                flag[p] = 0;
             }
         }
+        const uint64_t m = __ballot(keep);
+        if (m) {
+            int64_t base = 0;
+            if (lane == 0) base = (int64_t)atomicAdd((unsigned long long*)n_surv, (unsigned long long)__popcll(m));
+            base = __shfl(base, 0, MWD_WAVE);
+            const int64_t at = base + __popcll(m & lt);
+            if (keep && at < n_cap) surv[at] = (int32_t)p;
+        }
+    }
+}
+
+// GJK over the survivors (.cu:983-1322, without EPA).
+__global__ void __launch_bounds__(256) k_gjk(BodyView A, BodyView B, int64_t n_cap, const int32_t* __restrict__ surv,
+                                             const int64_t* n_surv, int32_t* __restrict__ collision,
+                                             int32_t* __restrict__ flag, Simplex* __restrict__ simplex,
+                                             int32_t* __restrict__ sidx, int64_t* slots, int64_t slot_cap,
+                                             uint32_t* flags) {
+    int64_t m = *n_surv;
+    if (m > n_cap) m = n_cap;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < m; s += stride) {
+        const int64_t p = surv[s];
+        const int64_t ra = view_row(A, p), rb = view_row(B, p);
+        const d3 ca = ld3(A.com + 3 * ra), cb = ld3(B.com + 3 * rb);
         const Side sa{A.verts + (int64_t)A.vmax * 3 * ra, A.nverts[ra]};
         const Side sb{B.verts + (int64_t)B.vmax * 3 * rb, B.nverts[rb]};
         d3 S[4];
         int ia[4] = {0, 0, 0, 0}, ib[4] = {0, 0, 0, 0};
         const int c = gjk_pair(sa, sb, ca, cb, S, ia, ib);
         collision[p] = c;
+        flag[p] = c == 1 ? 1 : 0;
         if (c == 1) {
             const int64_t slot = atomicAdd((unsigned long long*)slots, 1ull);
             sidx[p] = (int32_t)slot;
@@ -274,15 +310,6 @@ __global__ void __launch_bounds__(256) k_gjk(BodyView A, BodyView B, int64_t n_c
             }
         }
     }
-}
-
-__global__ void k_contact_flags(const int32_t* __restrict__ collision, int64_t n_cap, const int64_t* n_dev,
-                                int32_t* __restrict__ flag) {
-    int64_t n = n_dev ? *n_dev : n_cap;
-    if (n > n_cap) n = n_cap;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride)
-        flag[p] = collision[p] == 1 ? 1 : 0;
 }
 
 __global__ void k_contact_scatter(const int32_t* __restrict__ collision, const int64_t* __restrict__ off,
@@ -681,10 +708,23 @@ __global__ void __launch_bounds__(256) k_epa_force(BodyView A, BodyView B, BodyT
         } else {
             const FaceGeo<T> g = face_geometry<T>(bt.verts, (int64_t)A.vmax * ra, bt.verts, (int64_t)B.vmax * rb,
                                                   e.fa, e.fb, e.flip);
-            dep = g.depth;
-            nrm = g.n;
-            ca = g.cpA;
-            cb = g.cpB;
+            // A face EPA left as "outside" (or deleted) carries a placeholder
+            // distance instead of its plane's: keep EPA's values, without
+            // derivatives, whenever the re-evaluation does not reproduce them.
+            const bool same = g.depth.v == e.depth && g.n.x.v == e.n.x && g.n.y.v == e.n.y && g.n.z.v == e.n.z &&
+                              g.cpA.x.v == e.cpA.x && g.cpA.y.v == e.cpA.y && g.cpA.z.v == e.cpA.z &&
+                              g.cpB.x.v == e.cpB.x && g.cpB.y.v == e.cpB.y && g.cpB.z.v == e.cpB.z;
+            if (same) {
+                dep = g.depth;
+                nrm = g.n;
+                ca = g.cpA;
+                cb = g.cpB;
+            } else {
+                dep = T(e.depth);
+                nrm = mk3<T>(T(e.n.x), T(e.n.y), T(e.n.z));
+                ca = mk3<T>(T(e.cpA.x), T(e.cpA.y), T(e.cpA.z));
+                cb = mk3<T>(T(e.cpB.x), T(e.cpB.y), T(e.cpB.z));
+            }
         }
         const LawOut<T> f = force_law<T>(body3<T>(A.com, bt.com, ra), body3<T>(B.com, bt.com, rb),
                                          body3<T>(A.vel, bt.vel, ra), body3<T>(B.vel, bt.vel, rb),
@@ -797,24 +837,25 @@ unsigned grid_for(int64_t n, int threads, int64_t max_blocks) {
 }  // namespace
 
 int mwd_launch_gjk(const BodyView& A, const BodyView& B, int64_t n_cap, const int64_t* n_dev, int32_t* collision,
-                   Simplex* simplex, int32_t* sidx, int64_t* slots, int64_t slot_cap, uint32_t* flags, hipStream_t s) {
+                   int32_t* flag, int32_t* surv, int64_t* n_surv, Simplex* simplex, int32_t* sidx, int64_t* slots,
+                   int64_t slot_cap, uint32_t* flags, hipStream_t s) {
     if (n_cap <= 0) return MWD_OK;
-    hipLaunchKernelGGL(k_gjk, dim3(grid_for(n_cap, 256, 8192)), dim3(256), 0, s, A, B, n_cap, n_dev, collision,
-                       simplex, sidx, slots, slot_cap, flags);
+    hipLaunchKernelGGL(k_sphere, dim3(grid_for(n_cap, 256, 8192)), dim3(256), 0, s, A, B, n_cap, n_dev, collision,
+                       flag, surv, n_surv);
+    MWD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_gjk, dim3(grid_for(n_cap, 256, 2048)), dim3(256), 0, s, A, B, n_cap, surv, n_surv,
+                       collision, flag, simplex, sidx, slots, slot_cap, flags);
     MWD_LAUNCH_CHECK();
     return MWD_OK;
 }
 
-int mwd_launch_compact_contacts(const int32_t* collision, int64_t n_cap, const int64_t* n_dev, int32_t* flag,
+int mwd_launch_compact_contacts(const int32_t* collision, int64_t n_cap, const int64_t* n_dev, const int32_t* flag,
                                 int64_t* off, int32_t* contact_pair, int64_t contact_cap, int64_t* n_contacts,
                                 uint32_t* flags, int64_t* tmp, int64_t tmp_len, hipStream_t s) {
     if (n_cap <= 0) {
         MWD_HIP_CHECK(hipMemsetAsync(n_contacts, 0, sizeof(int64_t), s));
         return MWD_OK;
     }
-    hipLaunchKernelGGL(k_contact_flags, dim3(grid_for(n_cap, 256, 8192)), dim3(256), 0, s, collision, n_cap, n_dev,
-                       flag);
-    MWD_LAUNCH_CHECK();
     int rc = mwd_scan_i32(flag, off, n_contacts, n_cap, n_dev, tmp, tmp_len, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_contact_scatter, dim3(grid_for(n_cap, 256, 8192)), dim3(256), 0, s, collision, off, n_cap,

@@ -294,6 +294,8 @@ __global__ void k_finish_pairs(int64_t* counters, int64_t cap, uint32_t* flags) 
     const int64_t P = counters[1] + counters[2];
     counters[3] = P < cap ? P : cap;
     counters[7] = P;
+    counters[8] = 0;   // simplex slots of the fine-contact stage that consumes these pairs
+    counters[10] = 0;  // its sphere-test survivors
     if (P > cap) atomicOr(flags, MWD_F_PAIR_OVERFLOW);
 }
 
@@ -321,10 +323,13 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
     int rc;
     // counters 0..3 = n_cells, n_wall_pairs, n_cell_pairs, n_pairs
     mwd_mark(ws, 0);
-    hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(64), 0, s, ctr, 4);
-    MWD_LAUNCH_CHECK();
+    const bool walls = ws->n_wall_tests > 0 && st->wall_n && st->wall_c;
+    if (!walls || M <= 0) {  // otherwise the scans below write counters 0..2 and k_finish_pairs counter 3
+        hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(64), 0, s, ctr, 4);
+        MWD_LAUNCH_CHECK();
+    }
     // -- wall slab pairs
-    if (ws->n_wall_tests > 0 && st->wall_n && st->wall_c) {
+    if (walls) {
         const int64_t T = ws->n_wall_tests;
         hipLaunchKernelGGL(k_wall_flags, dim3(blocks_for(T)), dim3(256), 0, s, st->com, st->wall_n, st->wall_c,
                            ws->scenes_dev, ws->n_scenes, ws->scene_wt_off, T, ws->wall_flag);

@@ -131,6 +131,8 @@ struct mwd_workspace {
     Simplex* simplex;          // [contact_cap], slot claimed by GJK per colliding pair
     int32_t* sidx;             // [pair_cap] simplex slot of a colliding pair
     int32_t* pair_flag;        // [pair_cap]
+    int32_t* surv;             // [pair_cap] pairs whose bounding spheres meet (any order)
+    int fresh_pairs;           // host: candidates not yet consumed by a fine-contact stage
     int64_t* contact_off;      // [pair_cap]
     // contacts
     int32_t* contact_pair;     // [contact_cap]
@@ -153,7 +155,7 @@ struct mwd_workspace {
     // counters (device): 0 n_cells, 1 n_wall_pairs, 2 n_cell_pairs, 3 n_pairs
     // (capped), 4 n_contacts, 5 n_records, 6 n_cells of the previous step,
     // 7 n_pairs before capping, 8 simplex slots claimed this step, 9 radix
-    // histogram entries (scratch)
+    // histogram entries (scratch), 10 sphere-test survivors
     int64_t* counters;
     uint32_t* flags;
     mwd_law* laws;             // [2*n_scenes] (particle law, wall law)
@@ -188,6 +190,7 @@ int mwd_radix_sort(uint32_t* keys, int32_t* vals, uint32_t* keys_alt, int32_t* v
                    int64_t* tmp, int64_t tmp_len, hipStream_t s, uint32_t** keys_out, int32_t** vals_out);
 int64_t mwd_sort_tiles(int64_t n_cap);
 int mwd_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
+int mwd_fill2_i32(int32_t* p, int32_t* q, int32_t v, int64_t n, hipStream_t s);
 int mwd_set_i64(int64_t* p, int64_t v, hipStream_t s);
 
 // ------------------------------------------------------------ stages
@@ -204,13 +207,18 @@ int mwd_launch_radius(const double* com, const double* verts, const int32_t* nve
 int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t* keys);
 int mwd_launch_cell_keys(const double* com, int64_t n, const double* cell, int64_t ncx, int64_t ncy,
                          int64_t* keys, hipStream_t s);
-// GJK over pairs [0, n) (n from n_dev), writing collision; a colliding pair
-// claims a simplex slot (counter *slots, zero on entry) and stores it in sidx.
+// GJK over pairs [0, n) (n from n_dev), in two launches: a bounding-sphere
+// test writes collision = 0 and flag = 0 for pairs whose spheres are apart
+// and appends the others to surv (counter *n_surv, zero on entry); GJK then
+// runs over the survivors only, writing collision and flag (collision == 1).
+// A colliding pair claims a simplex slot (counter *slots, zero on entry) and
+// stores it in sidx.
 int mwd_launch_gjk(const BodyView& A, const BodyView& B, int64_t n_cap, const int64_t* n_dev,
-                   int32_t* collision, Simplex* simplex, int32_t* sidx, int64_t* slots, int64_t slot_cap,
-                   uint32_t* flags, hipStream_t s);
-// Contact compaction: contact_pair[c] = pair index of the c-th collision==1.
-int mwd_launch_compact_contacts(const int32_t* collision, int64_t n_cap, const int64_t* n_dev, int32_t* flag,
+                   int32_t* collision, int32_t* flag, int32_t* surv, int64_t* n_surv, Simplex* simplex,
+                   int32_t* sidx, int64_t* slots, int64_t slot_cap, uint32_t* flags, hipStream_t s);
+// Contact compaction from the flags: contact_pair[c] = pair index of the
+// c-th collision == 1.
+int mwd_launch_compact_contacts(const int32_t* collision, int64_t n_cap, const int64_t* n_dev, const int32_t* flag,
                                 int64_t* off, int32_t* contact_pair, int64_t contact_cap, int64_t* n_contacts,
                                 uint32_t* flags, int64_t* tmp, int64_t tmp_len, hipStream_t s);
 // EPA + force law over contacts [0, K). Reduction records are keyed by

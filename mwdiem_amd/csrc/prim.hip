@@ -8,14 +8,13 @@
 namespace {
 
 constexpr int SCAN_THREADS = 256;
-constexpr int SCAN_ITEMS = 8;
-constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;  // 2048
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;  // 4096
 
 constexpr int SORT_THREADS = 256;
 constexpr int SORT_WAVES = SORT_THREADS / MWD_WAVE;
-constexpr int SORT_ROUNDS = 16;
-constexpr int SORT_TILE = SORT_THREADS * SORT_ROUNDS;  // 4096
-constexpr int RADIX = 256;
+constexpr int SORT_ROUNDS = 4;
+constexpr int SORT_TILE = SORT_THREADS * SORT_ROUNDS;  // 1024
 
 __device__ __forceinline__ int64_t live_n(int64_t n_cap, const int64_t* n_dev) {
     if (!n_dev) return n_cap;
@@ -49,25 +48,23 @@ __device__ __forceinline__ int64_t block_inclusive_scan(int64_t v, int64_t* lds_
 
 // ------------------------------------------------- single-pass scan
 // Decoupled look-back (Merrill & Garland, "Single-pass Parallel Prefix Scan
-// with Decoupled Look-back", 2016). Tiles are claimed in order through an
-// atomic counter; each publishes its aggregate, then its inclusive prefix
-// once known; a successor's first wave reads 64 predecessors' status words at
-// a time and adds them up to the nearest inclusive one.
+// with Decoupled Look-back", 2016). Blocks past the live tile count exit at
+// once; each live block claims a tile through an atomic ticket, publishes the
+// tile's aggregate, then its inclusive prefix once known; a successor's first
+// wave reads 64 predecessors' status words at a time and adds them up to the
+// nearest inclusive one. A status word packs a 2-bit flag above a 62-bit
+// value, so a reader needs only that word: loads and stores are relaxed
+// device-scope atomics and no fence is needed (a device-scope fence would
+// write back or invalidate the XCD's L2).
 //
-// A status word packs [epoch:22 | flag:2 | value:40]. Each call has its own
-// epoch (1 .. 2^22-1, advanced by the call's last block), and a word of
-// another epoch reads as "not published", so the words never need clearing
-// between calls. A reader needs nothing but that one word, so loads and
-// stores are relaxed device-scope atomics with no fences: a device-scope
-// fence would write back or invalidate the XCD's L2 on every publish and
-// poll. Only when the epoch wraps does every block fence before it finishes
-// and the last one clear the words.
-constexpr int ST_VBITS = 40;
-constexpr uint64_t ST_VAL = (1ull << ST_VBITS) - 1;
-constexpr uint64_t ST_AGG = 1ull << ST_VBITS;
-constexpr uint64_t ST_INC = 2ull << ST_VBITS;
-constexpr int ST_EPOCH_SHIFT = ST_VBITS + 2;
-constexpr uint64_t ST_EPOCH_MAX = (1ull << (64 - ST_EPOCH_SHIFT)) - 1;
+// Scratch: word 0 holds [call:32 | ticket:32]; two status buffers of S words
+// follow. A call uses buffer (call & 1) and clears the other for the next
+// call; the block that draws the last ticket resets the ticket and advances
+// the call number in the same word. So there is one atomic per live tile and
+// nothing to clear between calls.
+constexpr uint64_t ST_AGG = 1ull << 62;
+constexpr uint64_t ST_INC = 2ull << 62;
+constexpr uint64_t ST_VAL = (1ull << 62) - 1;
 
 __device__ __forceinline__ uint64_t status_load(const uint64_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -76,25 +73,27 @@ __device__ __forceinline__ void status_store(uint64_t* p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// scratch: ctr[0] tile claims, ctr[1] blocks done, *epoch the last call's
-// epoch, status[ntiles]
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_dlb(const int32_t* __restrict__ in, int64_t* __restrict__ out,
                                                            int64_t* total, int64_t n_cap, const int64_t* n_dev,
-                                                           uint64_t* status, uint32_t* ctr, uint64_t* epoch,
-                                                           int64_t ntiles) {
+                                                           uint64_t* ctr, uint64_t* status2, int64_t S) {
     __shared__ int64_t lw[SORT_WAVES];
     __shared__ int64_t s_prefix;
-    __shared__ uint32_t s_tile;
-    __shared__ uint64_t s_ep;
+    __shared__ uint64_t s_claim;
     const int64_t n = live_n(n_cap, n_dev);
+    int64_t ntl = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (ntl < 1) ntl = 1;  // one block runs even when nothing is live: it clears and advances
+    if ((int64_t)blockIdx.x >= ntl) return;
     if (threadIdx.x == 0) {
-        s_tile = atomicAdd(&ctr[0], 1u);
-        s_ep = status_load(epoch) % ST_EPOCH_MAX + 1;
+        const uint64_t c = atomicAdd((unsigned long long*)ctr, 1ull);
+        s_claim = c;
+        if ((int64_t)(c & 0xffffffffull) == ntl - 1) status_store(ctr, ((c >> 32) + 1) << 32);  // last ticket
     }
     __syncthreads();
-    const int64_t tile = s_tile;
-    const uint64_t ep = s_ep;
-    const uint64_t tag = ep << ST_EPOCH_SHIFT;
+    const uint64_t claim = s_claim;
+    const int64_t tile = (int64_t)(claim & 0xffffffffull);
+    const int par = (int)((claim >> 32) & 1);
+    uint64_t* status = status2 + par * S;
+    uint64_t* other = status2 + (1 - par) * S;
     const int64_t base = tile * SCAN_TILE;
     if (base < n) {
         int32_t v[SCAN_ITEMS];
@@ -111,15 +110,14 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_dlb(const int32_t* __rest
             const int lane = threadIdx.x;
             int64_t prefix = 0;
             if (tile == 0) {
-                if (lane == 0) status_store(&status[0], tag | ST_INC | ((uint64_t)agg & ST_VAL));
+                if (lane == 0) status_store(&status[0], ST_INC | (uint64_t)agg);
             } else {
-                if (lane == 0) status_store(&status[tile], tag | ST_AGG | ((uint64_t)agg & ST_VAL));
+                if (lane == 0) status_store(&status[tile], ST_AGG | (uint64_t)agg);
                 int64_t j = tile - 1;  // window: predecessors j, j-1, ..., j-63
                 for (;;) {
                     const int64_t t = j - lane;
-                    uint64_t sw = t >= 0 ? status_load(&status[t]) : (tag | ST_INC);
-                    if ((sw >> ST_EPOCH_SHIFT) != ep) sw = 0;  // another call's word: not published
-                    const uint64_t f = sw & (ST_AGG | ST_INC);
+                    const uint64_t sw = t >= 0 ? status_load(&status[t]) : ST_INC;
+                    const uint64_t f = sw & ~ST_VAL;
                     const uint64_t inc = __ballot(f == ST_INC);
                     const uint64_t none = __ballot(f == 0);
                     // lanes up to and including the nearest inclusive predecessor
@@ -132,7 +130,7 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_dlb(const int32_t* __rest
                     if (inc) break;
                     j -= MWD_WAVE;
                 }
-                if (lane == 0) status_store(&status[tile], tag | ST_INC | ((uint64_t)(prefix + agg) & ST_VAL));
+                if (lane == 0) status_store(&status[tile], ST_INC | (uint64_t)(prefix + agg));
             }
             if (lane == 0) {
                 s_prefix = prefix;
@@ -147,28 +145,11 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_dlb(const int32_t* __rest
             if (i < n) out[i] = run;
             run += v[k];
         }
-    } else if (tile == 0 && total && threadIdx.x == 0) {
-        *total = 0;  // nothing live
+    } else if (total && threadIdx.x == 0) {
+        *total = 0;  // nothing live (only tile 0 exists then)
     }
-    // the last block out resets the counters and advances the epoch
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        if (ep == ST_EPOCH_MAX) __threadfence();  // status stores land before the clear below
-        const uint32_t done = atomicAdd(&ctr[1], 1u);
-        s_tile = (done == (uint32_t)(ntiles - 1)) ? 1u : 0u;
-    }
-    __syncthreads();
-    if (s_tile) {
-        if (ep == ST_EPOCH_MAX) {
-            __threadfence();
-            for (int64_t j = threadIdx.x; j < ntiles; j += SCAN_THREADS) status_store(&status[j], 0);
-        }
-        if (threadIdx.x == 0) {
-            atomicExch(&ctr[0], 0u);
-            atomicExch(&ctr[1], 0u);
-            status_store(epoch, ep == ST_EPOCH_MAX ? 0 : ep);
-        }
-    }
+    // clear the other buffer for the next call
+    for (int64_t j = tile * SCAN_THREADS + threadIdx.x; j < S; j += ntl * SCAN_THREADS) other[j] = 0;
 }
 
 // --------------------------------------------------------------- radix sort
@@ -268,6 +249,15 @@ __global__ void __launch_bounds__(SORT_THREADS) k_radix_scatter(const uint32_t* 
 
 __global__ void k_set_i64(int64_t* p, int64_t v) { *p = v; }
 
+__global__ void k_fill2_i32(int32_t* p, int32_t* q, int32_t v, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += st) {
+        p[i] = v;
+        q[i] = v;
+    }
+}
+
 __global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t st = (int64_t)gridDim.x * blockDim.x;
@@ -276,7 +266,7 @@ __global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
 
 }  // namespace
 
-int64_t mwd_scan_tmp_len(int64_t n_cap) { return (n_cap + SCAN_TILE - 1) / SCAN_TILE + 2; }
+int64_t mwd_scan_tmp_len(int64_t n_cap) { return 2 * ((n_cap + SCAN_TILE - 1) / SCAN_TILE + 1) + 1; }
 int64_t mwd_sort_tiles(int64_t n_cap) { return (n_cap + SORT_TILE - 1) / SORT_TILE; }
 
 int mwd_scan_i32(const int32_t* in, int64_t* out, int64_t* total, int64_t n_cap, const int64_t* n_dev,
@@ -289,14 +279,14 @@ int mwd_scan_i32(const int32_t* in, int64_t* out, int64_t* total, int64_t n_cap,
         return MWD_OK;
     }
     const int64_t nb = (n_cap + SCAN_TILE - 1) / SCAN_TILE;
-    // scratch layout: [0] two uint32 counters, [1] epoch, [2 .. nb+1] tile status words
-    if (nb + 2 > tmp_len) {
+    // scratch layout: [0] call/ticket word, then two status buffers of S words
+    const int64_t S = (tmp_len - 1) / 2;
+    if (nb > S) {
         mwd_set_error("scan scratch too small");
         return MWD_E_CAPACITY;
     }
     hipLaunchKernelGGL(k_scan_dlb, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, s, in, out, total, n_cap, n_dev,
-                       reinterpret_cast<uint64_t*>(tmp + 2), reinterpret_cast<uint32_t*>(tmp),
-                       reinterpret_cast<uint64_t*>(tmp + 1), nb);
+                       reinterpret_cast<uint64_t*>(tmp), reinterpret_cast<uint64_t*>(tmp + 1), S);
     MWD_LAUNCH_CHECK();
     return MWD_OK;
 }
@@ -351,6 +341,15 @@ int mwd_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s) {
     int64_t blocks = (n + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)blocks), dim3(256), 0, s, p, v, n);
+    MWD_LAUNCH_CHECK();
+    return MWD_OK;
+}
+
+int mwd_fill2_i32(int32_t* p, int32_t* q, int32_t v, int64_t n, hipStream_t s) {
+    if (n <= 0) return MWD_OK;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill2_i32, dim3((unsigned)blocks), dim3(256), 0, s, p, q, v, n);
     MWD_LAUNCH_CHECK();
     return MWD_OK;
 }

@@ -38,9 +38,10 @@ def outcome_and_grad(S: Stepper, top_frac: float = 0.1):
     with respect to theta, from the carried tangents of com."""
     torch.cuda.synchronize(S.device)
     W = S.scenes[0].n_walls
-    com = S.t["com"].detach().clone().requires_grad_(True)
-    L = bed_outcome(com, W, top_frac)
-    (g,) = torch.autograd.grad(L, com)
+    with torch.enable_grad():  # also when called from an autograd.Function's forward
+        com = S.t["com"].detach().clone().requires_grad_(True)
+        L = bed_outcome(com, W, top_frac)
+        (g,) = torch.autograd.grad(L, com)
     dL = torch.einsum("kij,ij->k", S.tan["com"], g)
     return float(L.detach().cpu()), dL.detach().cpu().numpy()
 

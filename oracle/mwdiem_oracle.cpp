@@ -283,7 +283,8 @@ struct EpaOut { double depth; double n[3]; double cpA[3]; double cpB[3]; int sta
 // bit 2: face storage exhausted (reference: undefined behaviour).
 template <typename T>
 void epa(const V3<T> S[4], const int ia[4], const int ib[4], const Body<T>& A, const Body<T>& B,
-         T* depth, V3<T>* cpA, V3<T>* cpB, V3<T>* normal, int* status, int* faces_used, int* iters) {
+         T* depth, V3<T>* cpA, V3<T>* cpB, V3<T>* normal, int* status, int* faces_used, int* iters,
+         int* face = nullptr) {
     Poly<T> P;
     T xmin_old = T(1e29);
     T proj = T(2.0) * xmin_old + T(1.0);
@@ -370,6 +371,10 @@ void epa(const V3<T> S[4], const int ia[4], const int ib[4], const Body<T>& A, c
     *status = st;
     *faces_used = nt;
     *iters = it;
+    if (face) {  // support indices of the final face's vertices (stored order) on A, then on B
+        face[0] = P.opA[a]; face[1] = P.opA[b]; face[2] = P.opA[c];
+        face[3] = P.opB[a]; face[4] = P.opB[b]; face[5] = P.opB[c];
+    }
 }
 
 // ----------------------------------------------------------------------- Force
@@ -447,6 +452,7 @@ struct PairResult {
     int epa_status;
     int faces;
     int iters;
+    int face[6];
     double depth, n[3], cpA[3], cpB[3], Fn[3], tA[3], tB[3], disp[4];
 };
 
@@ -504,7 +510,7 @@ PairResult contact_one(const Body<T>& A, const Body<T>& B, const PairIn<T>& in, 
             r.collision = 1;
             T depth;
             V3<T> cpA, cpB, nrm;
-            epa(S, ia, ib, A, B, &depth, &cpA, &cpB, &nrm, &r.epa_status, &r.faces, &r.iters);
+            epa(S, ia, ib, A, B, &depth, &cpA, &cpB, &nrm, &r.epa_status, &r.faces, &r.iters, r.face);
             ForceOut<T> f = force_law(in, depth, nrm, cpA, cpB, L);
             r.depth = val(depth);
             r.n[0] = val(nrm.x); r.n[1] = val(nrm.y); r.n[2] = val(nrm.z);
@@ -908,6 +914,27 @@ ORC_EXPORT int64_t orc_contact_state(int64_t P, const int32_t* pa, const int32_t
         }
     }
     return K;
+}
+
+// Contact topology of a pair list: collision codes and, for colliding pairs,
+// the support indices of EPA's final face (face[6p..6p+2] on A, +3..+5 on B).
+ORC_EXPORT void orc_contact_faces(int64_t P, const int32_t* pa, const int32_t* pb, int vmax, const double* com,
+                                  const double* verts, const int32_t* nverts, int32_t* collision, int32_t* face) {
+    Law L = make_law(1.0, 0.5, 0.5, 0.5, 1.0);
+    for (int64_t p = 0; p < P; ++p) {
+        int a = pa[p], b = pb[p];
+        Body<double> A{verts + (int64_t)vmax * 3 * a, nverts[a]};
+        Body<double> B{verts + (int64_t)vmax * 3 * b, nverts[b]};
+        PairIn<double> in;
+        std::memset(&in, 0, sizeof(in));
+        in.comA = load3<double>(com + 3 * a);
+        in.comB = load3<double>(com + 3 * b);
+        in.qA.w = 1.0;
+        in.isA = 1.0; in.isB = 1.0;
+        PairResult r = contact_one(A, B, in, L);
+        collision[p] = r.collision;
+        for (int k = 0; k < 6; ++k) face[6 * p + k] = r.collision == 1 ? r.face[k] : -1;
+    }
 }
 
 // ------------------------------------------------------------ test helpers

@@ -9,8 +9,9 @@ pytestmark = pytest.mark.gpu
 
 
 def _pile(n=256, seed=5):
+    """A poured pile fast enough that hulls collide within tens of steps."""
     from mwdiem_amd import scene
-    return scene.config_pile(seed=seed, n=n)
+    return scene.config_pile(seed=seed, n=n, v_pour=3000.0)
 
 
 def test_tangent_step_values_are_bitwise(gpu, dense_phys):
@@ -61,11 +62,10 @@ def _com_after(sc, v0, theta, steps):
 @pytest.mark.parametrize("k", [0, 1, 2])
 def test_state_tangent_matches_central_differences(gpu, k):
     """d com / d theta_k after 60 steps of a poured pile against central
-    differences (h = 1e-6 relative) of the plain step, on every particle whose
-    derivative is above 1e-3 of the largest: agreement to 1e-4 relative of
-    the largest derivative (differences of order h^2 and the rounding of the
-    differenced positions bound it; contact topology is the same in the three
-    runs over so short a run)."""
+    differences (h = 1e-6 relative) of the plain step: every particle agrees
+    to 1e-4 of the largest derivative (differences of order h^2 and the
+    rounding of the differenced positions bound it; contact topology is the
+    same in the three runs over so short a run)."""
     from mwdiem_amd.diff import _with_theta
     from mwdiem_amd.stepper import Stepper
     sc, v0 = _pile()
@@ -93,8 +93,7 @@ def test_state_tangent_matches_central_differences(gpu, k):
 def test_bed_outcome_gradient_against_central_differences(gpu):
     """Gradient of the bed outcome (mean height of the top 10% + centroid
     spread) after 200 steps against central differences with h = 1e-4
-    relative, per parameter. The measured agreement is printed; the bound
-    asserted is 1e-3 relative of the gradient's largest component."""
+    relative: each component within 1e-5 relative (measured: about 1e-7)."""
     import torch
     from mwdiem_amd.diff import BedOutcome, run_outcome
     sc, v0 = _pile()
@@ -115,6 +114,6 @@ def test_bed_outcome_gradient_against_central_differences(gpu):
         Lp, _ = run_outcome(sc, v0, tp, steps, tangents=False)
         Lm, _ = run_outcome(sc, v0, tm, steps, tangents=False)
         fd[k] = (Lp - Lm) / (2 * h)
-    rel = np.abs(g - fd) / np.abs(fd).max()
+    rel = np.abs(g - fd) / np.abs(fd)
     print(f"outcome {L0:.12f} grad {g} fd {fd} rel {rel}")
-    assert np.all(rel <= 1e-3)
+    assert np.all(np.abs(fd) > 0) and np.all(rel <= 1e-5)
