@@ -32,26 +32,53 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
 
-def stage_bytes(sc_list, stats, nv_mean):
-    """Algorithmic HBM bytes per launch of each stage."""
+# Algorithmic HBM bytes per launch of each single-kernel stage: what the
+# kernel must read and write at least once (per-particle data reached
+# through pair indices counted once per particle).
+#  emit          : pair list out (8 B/pair), per cell its rank, start, 13
+#                  neighbour ids and offset (4+8+52+8 B), members (4 B each)
+#  sphere        : pair ids in (8 B), collision and flag out (8 B); centre
+#                  and radius per row (32 B)
+#  gjk           : per surviving pair its id (4), both rows (8), centres
+#                  (48), vertex counts (8), both vertex sets (2 x 24 x V),
+#                  collision/flag/slot out (12); a simplex (128) per contact
+#  epa+law       : per contact: pair id, simplex, slot (4+128+4), both
+#                  rows' centre, velocity, angular velocity, orientation and
+#                  inertia size (2 x 112), vertex sets (2 x 24 x V), outputs
+#                  depth/normal/points/force/torques (8+4x24+3x24) and two
+#                  records (16)
+#  reduce+verlet : per moving particle 448 B of state and constants read,
+#                  272 B written, its vertices read and written (48 B each);
+#                  per contact record 4 B index + 48 B force and torque; the
+#                  record count of every row (4 B)
+def stage_bytes(sc_list, stats):
+    import numpy as np
     M = sum(sc.n_particles for sc in sc_list)
-    T = sum(sc.n_walls * sc.n_particles for sc in sc_list)
-    P, K = stats["n_pairs"], stats["n_contacts"]
-    V = nv_mean
+    N = sum(sc.n_rows for sc in sc_list)
+    V = float(np.mean(np.concatenate([sc.nverts[sc.n_walls:] for sc in sc_list])))
+    P, K, C = stats["n_pairs"], stats["n_contacts"], stats["n_cells"]
+    S = stats["n_sphere_pass"]
+    P_cell = P - stats["n_wall_pairs"]
     return {
-        # walls: com read per test + flag; keys: com read + key/row write; sort: 3 passes x (8 in + 8 out)
-        "walls+keys+sort": 24 * T + 4 * T + 24 * M + 8 * M + 3 * 16 * M,
-        # cells + emission: sorted keys/rows read, pairs written
-        "cells+emit": 8 * M + 8 * P,
-        # GJK: pair ids, both centres, both vertex sets once, collision code
-        "gjk": P * (8 + 48 + 2 * 24 * V + 4),
-        "compact": 8 * P,
-        # EPA + law per contact: ids 8, state of A and B 2*80, outputs 9*8*... (248 B/contact model)
-        "epa+law": 248 * K,
-        "record-sort": 2 * K * 16 * 2,
-        # integrator: (84 + 6V) doubles of state per particle + records
-        "reduce+verlet": M * (84 + 6 * V) * 8 + 2 * K * (4 + 48),
+        "emit": 8 * P_cell + C * (4 + 8 + 52 + 8) + 4 * M,
+        "sphere": 16 * P + 32 * N,
+        "gjk": S * (4 + 8 + 48 + 8 + 2 * 24 * V + 12) + 128 * K,
+        "epa+law": K * (136 + 224 + 48 * V + 8 + 168 + 16),
+        "reduce+verlet": M * (448 + 272 + 48 * V) + 2 * K * 52 + 4 * N,
     }
+
+
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` from the committed PMC summary for this
+    workload (profiles/<kernel>_<workload>_pmc.md, line 'hbm_bytes_per_launch: N'),
+    or None when there is none."""
+    f = ROOT / "profiles" / f"{kernel}_{workload}_pmc.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("hbm_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def cpu_baseline(sc, budget_s: float = 12.0):
@@ -122,6 +149,71 @@ def bench_batched(n_scenes: int, steps: int, plain_steps: int, graph_steps: int)
             "plain_steps_per_scene": plain_steps, "plain_particle_steps_per_s": plain, "batched_vs_plain": batched / plain}
 
 
+def bench_diff(steps: int, graph_steps: int, n: int = 4096):
+    """The differentiable form on the poured pile (n hulls, 40x40x60 mm,
+    physical law Kn=1e4, en=0.5, mu=0.3): cost of a tangent step (state plus
+    its derivatives with respect to Kn, en, mu) against a plain step, both
+    graph-captured, and the bed outcome's gradient after `steps` steps
+    against central differences (h = 1e-4 relative)."""
+    import numpy as np
+    import torch
+
+    from mwdiem_amd import scene
+    from mwdiem_amd.diff import _with_theta, outcome_and_grad, run_outcome
+    from mwdiem_amd.stepper import Stepper
+
+    sc, v0 = scene.config_pile(n=n)
+    theta = np.array([sc.law_p.Kn, sc.law_p.en, sc.law_p.mu])
+    times = {}
+    for mode in ("plain", "tangent"):
+        S = Stepper(_with_theta(sc, theta))
+        S.set_state({"v": v0, "v12": v0})
+        if mode == "tangent":
+            S.enable_tangents()
+        S.capture(graph_steps, tangent=(mode == "tangent"))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        S.run(steps)
+        torch.cuda.synchronize()
+        times[mode] = (time.perf_counter() - t0) / steps
+        st = S.check()
+        if mode == "tangent":
+            L, g = outcome_and_grad(S)
+        S.close()
+    fds = {}
+    for hrel in (1e-4, 1e-7):
+        fd = np.zeros(3)
+        for k in range(3):
+            h = hrel * theta[k]
+            tp, tm = theta.copy(), theta.copy()
+            tp[k] += h
+            tm[k] -= h
+            fd[k] = (run_outcome(sc, v0, tp, steps, False, graph_steps)[0]
+                     - run_outcome(sc, v0, tm, steps, False, graph_steps)[0]) / (2 * h)
+        rel = np.abs(g - fd) / np.maximum(np.abs(fd), 1e-300)
+        fds[f"{hrel:g}"] = {"fd": [float(x) for x in fd], "rel": [float(x) for x in rel]}
+    print(f"# diff: tangent {1e3 * times['tangent']:.4f} ms/step vs plain {1e3 * times['plain']:.4f}; "
+          f"grad {g} fd {fds}", file=sys.stderr, flush=True)
+    return {"particles": n, "steps": steps, "ms_per_step_plain": 1e3 * times["plain"],
+            "ms_per_step_tangent": 1e3 * times["tangent"], "tangent_vs_forward": times["tangent"] / times["plain"],
+            "outcome": L, "grad": [float(x) for x in g], "central_differences": fds,
+            "contacts_last_step": st["n_contacts"], "theta": ["Kn", "en", "mu"],
+            "mode": "forward tangents (3 directions)"}
+
+
+def copy_probe(nbytes: int, reps: int = 5):
+    """A plain device copy of nbytes, reps times: the calibration run for the
+    FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,13 +229,20 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=1000)
     ap.add_argument("--plain-steps", type=int, default=100)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--timing-steps", type=int, default=200, help="eager steps timed per kernel with events")
+    ap.add_argument("--diff-steps", type=int, default=2000, help="steps of the differentiable-form measurement")
+    ap.add_argument("--no-diff", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, help="only run a device copy of this many bytes (counter "
+                                                                 "calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
-              "described in DESIGN.md and is not measured.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): this build measures a single "
+              "MI355X; sharding across GPUs is a design, not a measured path.", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
 
     from mwdiem_amd import scene
@@ -159,9 +258,21 @@ def main():
         S.step(1)
     S.check()
     torch.cuda.synchronize()
+    # Kernel durations: the first steps of the run after warm-up, launched
+    # eagerly on the stepper's stream with HIP events -- stage boundaries,
+    # and for the single-kernel stages start/stop events stamped with the
+    # kernel's own start and end (hipExtLaunchKernelGGL), so launch gaps are
+    # excluded. (The same steps as a short profiled run from the initial state.)
+    n_time = min(args.steps, args.timing_steps)
+    S.set_timing(True)
+    S.step(n_time)
+    kt = S.kernel_times()
+    st = S.stage_times()
+    stats_t = S.check()  # counts of the last timed-kernel step
+    S.set_timing(False)
+    timing_src = f"{kt['steps']} eager steps after warm-up, kernel start/stop events"
     if not args.no_graph:
         S.capture(args.graph_steps)   # records graph_steps steps; capture runs nothing
-
 
     # timed region: K full steps
     torch.cuda.synchronize()
@@ -170,23 +281,23 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     stats = S.check()
-    # per-stage durations: HIP events between the stages on the stepper's
-    # stream, over further eager steps (events cannot sit inside the graph)
-    n_stage = min(args.steps, 500)
-    S.set_timing(True)
-    S.step(n_stage)
-    st = S.stage_times()
-    S.set_timing(False)
     ms_step = 1e3 * wall / args.steps
     value = sc.n_particles * args.steps / wall
     stage_ms = {k: v / max(st["steps"], 1) for k, v in st["ms"].items()}
-    nv_mean = float(np.mean(sc.nverts[sc.n_walls:]))
-    sbytes = stage_bytes([sc], stats, nv_mean)
-    dom = max(stage_ms, key=stage_ms.get)
-    ach = sbytes[dom] / (stage_ms[dom] * 1e-3) / 1e9
-    roof = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": ach / HBM_PEAK_GBS, "traffic": None, "launch_ms": stage_ms[dom],
-            "algorithmic_bytes": sbytes[dom]}
+    kernel_ms = {k: v / max(kt["steps"], 1) for k, v in kt["ms"].items()}
+    sbytes = stage_bytes([sc], stats_t)
+    # the dominant kernel: the slowest of the single-kernel stages
+    from mwdiem_amd import _lib
+    stage_of = dict(zip(_lib.KTIMED_NAMES, _lib.KTIMED_STAGE))
+    kname = max(kernel_ms, key=kernel_ms.get)
+    dom = stage_of[kname]
+    ach = sbytes[dom] / (kernel_ms[kname] * 1e-3) / 1e9
+    traffic = measured_traffic(kname, "c2_100k")
+    roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": traffic, "kernel": kname, "launch_ms": kernel_ms[kname], "algorithmic_bytes": sbytes[dom],
+            "timed_over": timing_src,
+            "kernel_ms": {k: round(v, 5) for k, v in kernel_ms.items()},
+            "kernel_gbs": {k: round(sbytes[stage_of[k]] / (v * 1e-3) / 1e9, 1) for k, v in kernel_ms.items() if v > 0}}
     print(f"# stages (ms/step): {json.dumps({k: round(v, 4) for k, v in stage_ms.items()})}", file=sys.stderr)
     print(f"# pairs {stats['n_pairs']} contacts {stats['n_contacts']} cells {stats['n_cells']}", file=sys.stderr,
           flush=True)
@@ -197,6 +308,10 @@ def main():
     batched = None
     if not args.no_batched:
         batched = bench_batched(args.batched_scenes, args.batched_steps, args.plain_steps, args.graph_steps)
+
+    diff = None
+    if not args.no_diff:
+        diff = bench_diff(args.diff_steps, args.graph_steps)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -215,6 +330,8 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "batched_vs_plain": None if batched is None else batched["batched_vs_plain"],
+        "differentiable": diff,
+        "tangent_vs_forward": None if diff is None else diff["tangent_vs_forward"],
     }
     print(json.dumps(out))
 

@@ -52,9 +52,12 @@ extern "C" {
 #define MWD_F_CELL 8u              /* cell occupancy above MWD_MAX_CELL */
 #define MWD_F_EPA_CAP 16u          /* some EPA hit its 25-expansion cap (.cu:443-447) */
 #define MWD_F_EPA_FACES 32u        /* some EPA ran out of face storage */
+#define MWD_F_ROW_CONTACTS 64u     /* the stepper's contact-record overflow list is full */
 
 #define MWD_EPA_MAX_FACES 128
 #define MWD_MAX_CELL 76            /* largest cell occupancy whose set order is emulated */
+#define MWD_ROW_CONTACTS 32        /* contact records kept per moving particle and step (stepper);
+                                      further ones go to an overflow list of 2*contact_cap entries */
 
 /* Contact-law coefficients of Force (.cu:685-701), filled by mwd_law_init in
  * the reference's operator order.
@@ -117,6 +120,7 @@ typedef struct mwd_status {
     int64_t n_contacts;   /* pairs with collision == 1 */
     int64_t n_cells;
     uint32_t flags;       /* MWD_F_* */
+    int64_t n_sphere_pass;  /* pairs whose bounding spheres meet (the pairs GJK ran on) */
 } mwd_status;
 
 const char* mwd_last_error(void);
@@ -189,7 +193,8 @@ int mwd_contacts_view(mwd_workspace* ws, int32_t** collision, int32_t** contact_
                       int32_t** epa_status, int64_t** n_contacts);
 
 /* Force assembly and Verlet (MWDiem.py:977-1001): per moving particle the
- * contact sum (ascending pair order) plus fext/text, then the integrator. */
+ * contact sum (ascending pair order) plus fext/text, then the integrator.
+ * Consumes the contacts of the preceding mwd_fine_contact. */
 int mwd_reduce_verlet(mwd_workspace* ws, mwd_state* state);
 
 /* One full step: mwd_candidates + mwd_fine_contact + mwd_reduce_verlet.
@@ -198,13 +203,24 @@ int mwd_step(mwd_workspace* ws, mwd_state* state);
 
 /* Per-stage timing. When enabled, HIP events are recorded on the workspace
  * stream between the stages of mwd_step; mwd_stage_times synchronises and
- * returns the accumulated milliseconds of the MWD_N_STAGES stages (walls,
- * keys and sort; cell tables and emission; GJK; contact compaction; EPA and
- * contact law; record sort; force assembly and Verlet) and the number of
- * timed steps, then resets the accumulators. */
-#define MWD_N_STAGES 7
+ * returns the accumulated milliseconds of the MWD_N_STAGES stages and the
+ * number of timed steps, then resets the accumulators. The stages: wall slab
+ * test; cell keys and their sort; cell tables; pair emission (one kernel);
+ * pair count; bounding-sphere test (one kernel); GJK (one kernel); contact
+ * compaction; EPA and contact law (one kernel); force assembly and Verlet
+ * (one kernel). */
+#define MWD_N_STAGES 10
 int mwd_set_timing(mwd_workspace* ws, int enable);
 int mwd_stage_times(mwd_workspace* ws, double* ms, int64_t* steps);
+
+/* Kernel-only milliseconds, accumulated like mwd_stage_times over the timed
+ * steps, of the MWD_N_KTIMED single-kernel stages (pair emission,
+ * bounding-sphere test, GJK, EPA and law, force assembly and Verlet): with
+ * timing on they are launched with start/stop events that the runtime stamps
+ * at the kernel's own start and end, so launch gaps are excluded. Call before
+ * mwd_stage_times (which resets both). */
+#define MWD_N_KTIMED 5
+int mwd_kernel_times(mwd_workspace* ws, double* ms, int64_t* steps);
 
 /* radius[r] = max_k |verts[r,k] - com[r]| * (1 + 1e-9) + 1e-9 * (1 + max|com[r]|)
  * for rows [0, n). */

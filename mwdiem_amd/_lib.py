@@ -24,7 +24,10 @@ F_KEYRANGE = 4
 F_CELL = 8
 F_EPA_CAP = 16
 F_EPA_FACES = 32
-FATAL_FLAGS = F_PAIR_OVERFLOW | F_CONTACT_OVERFLOW | F_KEYRANGE | F_CELL
+F_ROW_CONTACTS = 64
+FATAL_FLAGS = F_PAIR_OVERFLOW | F_CONTACT_OVERFLOW | F_KEYRANGE | F_CELL | F_ROW_CONTACTS
+
+ROW_CONTACTS = 32
 
 EPA_MAX_FACES = 128
 MAX_CELL = 76
@@ -78,7 +81,7 @@ class State(C.Structure):
 
 class Status(C.Structure):
     _fields_ = [("n_pairs", C.c_int64), ("n_wall_pairs", C.c_int64), ("n_contacts", C.c_int64),
-                ("n_cells", C.c_int64), ("flags", C.c_uint32)]
+                ("n_cells", C.c_int64), ("flags", C.c_uint32), ("n_sphere_pass", C.c_int64)]
 
 
 # name -> (restype, argtypes)
@@ -106,14 +109,20 @@ _SIGS = {
     "mwd_set_timing": (C.c_int, [_P, C.c_int]),
     "mwd_radius_init": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int32, _P, _P]),
     "mwd_stage_times": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "mwd_kernel_times": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "mwd_law_tangent": (C.c_int, [C.POINTER(LawTan), C.c_double, C.c_double, C.c_double, C.c_double,
                                   C.POINTER(C.c_double)]),
     "mwd_set_law_tangents": (C.c_int, [_P, C.POINTER(LawTan)]),
     "mwd_step_tangent": (C.c_int, [_P, C.POINTER(State), C.POINTER(_P)]),
 }
 
-N_STAGES = 7
-STAGE_NAMES = ("walls+keys+sort", "cells+emit", "gjk", "compact", "epa+law", "record-sort", "reduce+verlet")
+N_KTIMED = 5
+KTIMED_NAMES = ("k_cell_fill", "k_sphere", "k_gjk", "k_epa_force", "k_reduce_verlet_v")
+KTIMED_STAGE = ("emit", "sphere", "gjk", "epa+law", "reduce+verlet")
+
+N_STAGES = 10
+STAGE_NAMES = ("walls", "keys+sort", "cells", "emit", "pair-count", "sphere", "gjk", "compact", "epa+law",
+               "reduce+verlet")
 
 EXPORTED = tuple(_SIGS)
 

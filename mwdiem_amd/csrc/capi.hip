@@ -45,8 +45,8 @@ void free_ws(mwd_workspace* ws) {
                     ws->vals_alt, ws->head, ws->head_off, ws->cell_start, ws->cell_key, ws->cell_nb, ws->row_first,
                     ws->row_rank, ws->cell_by_rank, ws->cell_cnt, ws->cell_off, ws->perm, ws->wall_flag, ws->wall_off,
                     ws->pairs, ws->collision, ws->simplex, ws->sidx, ws->pair_flag, ws->surv, ws->contact_off, ws->contact_pair,
-                    ws->depth, ws->normal, ws->cpA, ws->cpB, ws->Fn, ws->tA, ws->tB, ws->epa_status, ws->rec_key,
-                    ws->rec_key_alt, ws->rec_val, ws->rec_val_alt, ws->seg_begin, ws->seg_end, ws->hist,
+                    ws->depth, ws->normal, ws->cpA, ws->cpB, ws->Fn, ws->tA, ws->tB, ws->epa_status, ws->rec_cnt,
+                    ws->rec_bkt, ws->rec_ovf, ws->hist,
                     ws->hist_off, ws->scan_tmp, ws->counters, ws->flags, ws->laws, ws->scene_wt_off, ws->ctan,
                     ws->law_tans};
     for (void* p : ptrs)
@@ -87,8 +87,14 @@ __global__ void k_wall_from_ids(const int32_t* idA, const int32_t* idB, int64_t 
 
 void mwd_set_error(const std::string& msg) { g_err = msg; }
 
-static inline hipEvent_t& ev_at(mwd_workspace* ws, int slot, int k) {
-    return ws->ev[slot * (MWD_N_STAGES + 1) + k];
+// Per step slot: MWD_N_STAGES + 1 stage boundaries, then a start and a stop
+// event for each timed kernel.
+static inline hipEvent_t& ev_at(mwd_workspace* ws, int slot, int k) { return ws->ev[slot * MWD_EV_PER_STEP + k]; }
+
+hipEvent_t mwd_kev(mwd_workspace* ws, int kt, int stop) {
+    if (!ws || !ws->timing) return nullptr;
+    ws->kt_used |= 1u << kt;
+    return ev_at(ws, ws->ring_slot, MWD_N_STAGES + 1 + 2 * kt + (stop ? 1 : 0));
 }
 
 static void flush_timing(mwd_workspace* ws) {
@@ -99,25 +105,47 @@ static void flush_timing(mwd_workspace* ws) {
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, ev_at(ws, sl, k), ev_at(ws, sl, k + 1)) == hipSuccess)
                 ws->stage_ms[k] += ms;
+            else
+                ws->timing_errors += 1;
+        }
+        for (int k = 0; k < MWD_N_KTIMED; ++k) {
+            if (!(ws->kt_used & (1u << k))) continue;
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, ev_at(ws, sl, MWD_N_STAGES + 1 + 2 * k),
+                                    ev_at(ws, sl, MWD_N_STAGES + 2 + 2 * k)) == hipSuccess)
+                ws->kernel_ms[k] += ms;
+            else
+                ws->timing_errors += 1;
         }
         ws->stage_calls += 1;
+        ws->kernel_calls += 1;
     }
+    (void)hipGetLastError();  // a failed query must not surface at the next launch
     ws->ring_slot = 0;
 }
 
+// Events may also be recorded while the stream is being captured: the graph
+// then holds event-record nodes and each replay re-records the same events,
+// so mwd_stage_times after replays reports the last replay's steps.
 void mwd_mark(mwd_workspace* ws, int stage) {
-    if (ws->timing) (void)hipEventRecord(ev_at(ws, ws->ring_slot, stage), ws->stream);
+    if (ws->timing && hipEventRecord(ev_at(ws, ws->ring_slot, stage), ws->stream) != hipSuccess) {
+        ws->timing_errors += 1;
+        (void)hipGetLastError();
+    }
 }
 void mwd_mark_end(mwd_workspace* ws) {
     if (!ws->timing) return;
-    (void)hipEventRecord(ev_at(ws, ws->ring_slot, MWD_N_STAGES), ws->stream);
+    if (hipEventRecord(ev_at(ws, ws->ring_slot, MWD_N_STAGES), ws->stream) != hipSuccess) {
+        ws->timing_errors += 1;
+        (void)hipGetLastError();
+    }
     ws->ring_slot += 1;
     if (ws->ring_slot == MWD_TIMING_RING) flush_timing(ws);
 }
 
 static void destroy_events(mwd_workspace* ws) {
     if (!ws->ev) return;
-    for (int k = 0; k < MWD_TIMING_RING * (MWD_N_STAGES + 1); ++k) (void)hipEventDestroy(ws->ev[k]);
+    for (int k = 0; k < MWD_TIMING_RING * MWD_EV_PER_STEP; ++k) (void)hipEventDestroy(ws->ev[k]);
     delete[] ws->ev;
     ws->ev = nullptr;
 }
@@ -125,8 +153,8 @@ static void destroy_events(mwd_workspace* ws) {
 MWD_API int mwd_set_timing(mwd_workspace* ws, int enable) {
     if (!ws) return MWD_E_INVALID;
     if (enable && !ws->ev) {
-        ws->ev = new hipEvent_t[MWD_TIMING_RING * (MWD_N_STAGES + 1)];
-        for (int k = 0; k < MWD_TIMING_RING * (MWD_N_STAGES + 1); ++k) MWD_HIP_CHECK(hipEventCreate(&ws->ev[k]));
+        ws->ev = new hipEvent_t[MWD_TIMING_RING * MWD_EV_PER_STEP];
+        for (int k = 0; k < MWD_TIMING_RING * MWD_EV_PER_STEP; ++k) MWD_HIP_CHECK(hipEventCreate(&ws->ev[k]));
     }
     if (!enable) {
         MWD_HIP_CHECK(hipStreamSynchronize(ws->stream));
@@ -134,8 +162,12 @@ MWD_API int mwd_set_timing(mwd_workspace* ws, int enable) {
     }
     ws->timing = enable ? 1 : 0;
     ws->ring_slot = 0;
+    ws->timing_errors = 0;
+    ws->kt_used = 0;
     for (int k = 0; k < MWD_N_STAGES; ++k) ws->stage_ms[k] = 0.0;
+    for (int k = 0; k < MWD_N_KTIMED; ++k) ws->kernel_ms[k] = 0.0;
     ws->stage_calls = 0;
+    ws->kernel_calls = 0;
     return MWD_OK;
 }
 
@@ -149,6 +181,28 @@ MWD_API int mwd_stage_times(mwd_workspace* ws, double* ms, int64_t* steps) {
     }
     if (steps) *steps = ws->stage_calls;
     ws->stage_calls = 0;
+    if (ws->timing_errors) {
+        ws->timing_errors = 0;
+        mwd_set_error("mwd_stage_times: some stage events could not be recorded or timed");
+        return MWD_E_HIP;
+    }
+    return MWD_OK;
+}
+
+MWD_API int mwd_kernel_times(mwd_workspace* ws, double* ms, int64_t* steps) {
+    if (!ws || !ms) return MWD_E_INVALID;
+    MWD_HIP_CHECK(hipStreamSynchronize(ws->stream));
+    flush_timing(ws);
+    for (int k = 0; k < MWD_N_KTIMED; ++k) {
+        ms[k] = ws->kernel_ms[k];
+        ws->kernel_ms[k] = 0.0;
+    }
+    if (steps) *steps = ws->kernel_calls;
+    ws->kernel_calls = 0;
+    if (ws->timing_errors) {
+        mwd_set_error("mwd_kernel_times: some kernel events could not be recorded or timed");
+        return MWD_E_HIP;
+    }
     return MWD_OK;
 }
 
@@ -277,8 +331,7 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     ws->key_span_total = kbase;
     ws->key_bits = bits_for(kbase);
     ws->row_bits = bits_for(n_rows);
-    const int64_t rec_cap = 2 * contact_cap;
-    const int64_t max_sort = std::max(M, rec_cap);
+    const int64_t max_sort = M;  // the cell-key sort (contact records go to per-row buckets)
     const int64_t tiles = mwd_sort_tiles(max_sort);
     const int64_t hist_len = (int64_t(1) << MWD_SORT_MAX_BITS) * tiles;
     const int64_t max_scan = std::max({M, n_rows, pair_cap, ws->n_wall_tests, hist_len});
@@ -330,12 +383,9 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     A(tA, 3 * contact_cap);
     A(tB, 3 * contact_cap);
     A(epa_status, contact_cap);
-    A(rec_key, rec_cap);
-    A(rec_key_alt, rec_cap);
-    A(rec_val, rec_cap);
-    A(rec_val_alt, rec_cap);
-    A(seg_begin, n_rows);
-    A(seg_end, n_rows);
+    A(rec_cnt, n_rows);
+    A(rec_bkt, n_rows * MWD_ROW_CONTACTS);
+    A(rec_ovf, 4 * contact_cap);
     A(hist, hist_len);
     A(hist_off, hist_len);
     A(scan_tmp, ws->scan_tmp_len);
@@ -362,6 +412,7 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     if ((ws->cell_of &&
          hipMemsetAsync(ws->cell_of, 0xFF, sizeof(int32_t) * ws->key_span_total, s) != hipSuccess) ||
         hipMemsetAsync(ws->counters, 0, sizeof(int64_t) * 16, s) != hipSuccess ||
+        hipMemsetAsync(ws->rec_cnt, 0, sizeof(int32_t) * n_rows, s) != hipSuccess ||
         hipMemsetAsync(ws->scan_tmp, 0, sizeof(int64_t) * ws->scan_tmp_len, s) != hipSuccess ||
         hipMemsetAsync(ws->flags, 0, sizeof(uint32_t), s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
@@ -371,8 +422,7 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     }
     ws->keys_sorted = ws->keys;
     ws->vals_sorted = ws->vals;
-    ws->rec_key_sorted = ws->rec_key;
-    ws->rec_val_sorted = ws->rec_val;
+
     *out = ws;
     return MWD_OK;
 }
@@ -454,21 +504,31 @@ static int fine_contact(mwd_workspace* ws, const mwd_state* st, double* const* t
     hipStream_t s = ws->stream;
     const BodyView A = state_view(ws, st, 0), B = state_view(ws, st, 1);
     int64_t* ctr = ws->counters;
-    mwd_mark(ws, 2);
+    mwd_mark(ws, 5);
     int rc = MWD_OK;
     if (!ws->fresh_pairs) {  // a second pass over the same candidates: restart the slot and survivor counters
-        if ((rc = mwd_set_i64(ctr + 8, 0, s)) || (rc = mwd_set_i64(ctr + 10, 0, s))) return rc;
+        if ((rc = mwd_set_i64(ctr + 8, 0, s)) || (rc = mwd_set_i64(ctr + 10, 0, s)) ||
+            (rc = mwd_set_i64(ctr + 11, 0, s)))
+            return rc;
     }
     ws->fresh_pairs = 0;
-    rc = mwd_launch_gjk(A, B, ws->pair_cap, ctr + 3, ws->collision, ws->pair_flag, ws->surv, ctr + 10, ws->simplex,
-                        ws->sidx, ctr + 8, ws->contact_cap, ws->flags, s);
+    rc = mwd_launch_sphere(A, B, ws->pair_cap, ctr + 3, ws->collision, ws->pair_flag, ws->surv, ctr + 10, s,
+                           mwd_kev(ws, MWD_KT_SPHERE, 0), mwd_kev(ws, MWD_KT_SPHERE, 1));
     if (rc) return rc;
-    mwd_mark(ws, 3);
+    mwd_mark(ws, 6);
+    rc = mwd_launch_gjk_surv(A, B, ws->pair_cap, ws->collision, ws->pair_flag, ws->surv, ctr + 10, ws->simplex,
+                             ws->sidx, ctr + 8, ws->contact_cap, ws->flags, s, mwd_kev(ws, MWD_KT_GJK, 0),
+                             mwd_kev(ws, MWD_KT_GJK, 1));
+    if (rc) return rc;
+    mwd_mark(ws, 7);
     rc = mwd_launch_compact_contacts(ws->collision, ws->pair_cap, ctr + 3, ws->pair_flag, ws->contact_off,
                                      ws->contact_pair, ws->contact_cap, ctr + 4, ws->flags, ws->scan_tmp,
                                      ws->scan_tmp_len, s);
     if (rc) return rc;
-    mwd_mark(ws, 4);
+    mwd_mark(ws, 8);
+    if (ws->bkt_dirty) {  // the previous contacts were not consumed by a reduction
+        if ((rc = mwd_fill_i32(ws->rec_cnt, 0, ws->n_rows, s)) || (rc = mwd_set_i64(ctr + 11, 0, s))) return rc;
+    }
     if (tan) {
         const VState vs = mwd_vstate(st, tan);
         BodyTan bt;
@@ -477,27 +537,21 @@ static int fine_contact(mwd_workspace* ws, const mwd_state* st, double* const* t
         bt.vel = vs.v;
         bt.angvel = vs.w;
         bt.quat = vs.q;
-        rc = mwd_launch_epa_force_tan(A, B, bt, ws->laws, ws->law_tans, ws->contact_pair, ws->simplex, ws->sidx,
-                                      ws->contact_cap, ctr + 4, ws->depth, ws->normal, ws->cpA, ws->cpB,
-                                      contact_out(ws, true), ws->epa_status, ws->rec_key, ws->rec_val, ctr + 5,
-                                      ws->flags, s);
+        rc = mwd_launch_epa_force_rows(A, B, &bt, ws->laws, ws->law_tans, ws->contact_pair, ws->simplex, ws->sidx,
+                                       ws->contact_cap, ctr + 4, ws->depth, ws->normal, ws->cpA, ws->cpB,
+                                       contact_out(ws, true), ws->epa_status, ws->rec_cnt, ws->rec_bkt, ws->rec_ovf,
+                                       ctr + 11, 2 * ws->contact_cap, ws->flags, s,
+                                       mwd_kev(ws, MWD_KT_EPA, 0), mwd_kev(ws, MWD_KT_EPA, 1));
     } else {
-        rc = mwd_launch_epa_force(A, B, ws->laws, ws->contact_pair, ws->simplex, ws->sidx, ws->contact_cap, ctr + 4,
-                                  nullptr, nullptr, nullptr, ws->depth, ws->normal, ws->cpA, ws->cpB, ws->Fn, ws->tA,
-                                  ws->tB, nullptr, ws->epa_status, ws->rec_key, ws->rec_val, ctr + 5, ws->flags, s);
+        rc = mwd_launch_epa_force_rows(A, B, nullptr, ws->laws, nullptr, ws->contact_pair, ws->simplex, ws->sidx,
+                                       ws->contact_cap, ctr + 4, ws->depth, ws->normal, ws->cpA, ws->cpB,
+                                       contact_out(ws, false), ws->epa_status, ws->rec_cnt, ws->rec_bkt, ws->rec_ovf,
+                                       ctr + 11, 2 * ws->contact_cap, ws->flags, s, mwd_kev(ws, MWD_KT_EPA, 0),
+                                       mwd_kev(ws, MWD_KT_EPA, 1));
     }
     if (rc) return rc;
-    mwd_mark(ws, 5);
-    uint32_t* ks;
-    int32_t* vs;
-    rc = mwd_radix_sort(ws->rec_key, ws->rec_val, ws->rec_key_alt, ws->rec_val_alt, 2 * ws->contact_cap, ctr + 5,
-                        ws->row_bits, ws->hist, ws->hist_off, ctr + 9, ws->scan_tmp, ws->scan_tmp_len, s, &ks, &vs);
-    if (rc) return rc;
-    ws->rec_key_sorted = ks;
-    ws->rec_val_sorted = vs;
-    rc = mwd_fill2_i32(ws->seg_begin, ws->seg_end, 0, ws->n_rows, s);
-    if (rc) return rc;
-    return mwd_launch_segments(ks, 2 * ws->contact_cap, ctr + 5, ws->seg_begin, ws->seg_end, s);
+    ws->bkt_dirty = 1;
+    return MWD_OK;
 }
 
 MWD_API int mwd_contacts_view(mwd_workspace* ws, int32_t** collision, int32_t** contact_pair, double** depth,
@@ -523,7 +577,8 @@ MWD_API int mwd_reduce_verlet(mwd_workspace* ws, mwd_state* st) {
         mwd_set_error("mwd_reduce_verlet: invalid argument");
         return MWD_E_INVALID;
     }
-    mwd_mark(ws, 6);
+    mwd_mark(ws, 9);
+    ws->bkt_dirty = 0;  // the reduction empties the buckets of the moving rows
     return mwd_launch_reduce_verlet(ws, st, nullptr, contact_out(ws, false));
 }
 
@@ -610,7 +665,8 @@ MWD_API int mwd_step_tangent(mwd_workspace* ws, mwd_state* st, double* const* ta
     if (rc) return rc;
     rc = fine_contact(ws, st, tan);
     if (rc) return rc;
-    mwd_mark(ws, 6);
+    mwd_mark(ws, 9);
+    ws->bkt_dirty = 0;
     rc = mwd_launch_reduce_verlet(ws, st, tan, contact_out(ws, true));
     if (rc) return rc;
     mwd_mark_end(ws);
@@ -628,7 +684,7 @@ MWD_API int mwd_radius_init(const double* com, const double* verts, const int32_
 
 MWD_API int mwd_step_status(mwd_workspace* ws, mwd_status* out) {
     if (!ws || !out) return MWD_E_INVALID;
-    int64_t c[8];
+    int64_t c[11];
     uint32_t f = 0;
     MWD_HIP_CHECK(hipMemcpyAsync(c, ws->counters, sizeof(c), hipMemcpyDeviceToHost, ws->stream));
     MWD_HIP_CHECK(hipMemcpyAsync(&f, ws->flags, sizeof(f), hipMemcpyDeviceToHost, ws->stream));
@@ -638,6 +694,7 @@ MWD_API int mwd_step_status(mwd_workspace* ws, mwd_status* out) {
     out->n_pairs = c[7];
     out->n_contacts = c[4];
     out->flags = f;
+    out->n_sphere_pass = c[10];
     return MWD_OK;
 }
 

@@ -62,6 +62,86 @@ __device__ __forceinline__ d3 support(const Side& A, const Side& B, d3 d, int* i
     return ld3(A.v + 3 * i1) - ld3(B.v + 3 * i2);
 }
 
+struct SerialSup {
+    Side A, B;
+    __device__ __forceinline__ d3 operator()(d3 d, int* ia, int* ib) const { return support(A, B, d, ia, ib); }
+};
+
+// The support map evaluated by a group of G lanes that share one pair: lane g
+// of the group holds vertices g, g+G, ..., g+(K-1)G of both bodies in
+// registers, takes the first strict maximum over its own, and the group
+// merges (value, index) pairs keeping the larger value and, on a tie, the
+// smaller index -- the serial loop's choice, bit for bit. The winners'
+// coordinates come from their owner lanes by shuffle (no memory round trip).
+template <int G, int K>
+struct GroupSup {
+    Side A, B;
+    d3 va[K], vb[K];
+    int g;
+    __device__ __forceinline__ void load(const Side& a, const Side& b, int lane_in_group) {
+        A = a;
+        B = b;
+        g = lane_in_group;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int i = g + G * k;
+            va[k] = i < A.n ? ld3(A.v + 3 * i) : mk3(0.0, 0.0, 0.0);
+            vb[k] = i < B.n ? ld3(B.v + 3 * i) : mk3(0.0, 0.0, 0.0);
+        }
+    }
+    __device__ __forceinline__ static void merge(double& best, int& bi) {
+#pragma unroll
+        for (int off = 1; off < G; off <<= 1) {
+            const double ob = __shfl_xor(best, off, G);
+            const int oi = __shfl_xor(bi, off, G);
+            if (ob > best || (ob == best && oi < bi)) {
+                best = ob;
+                bi = oi;
+            }
+        }
+    }
+    __device__ __forceinline__ d3 operator()(d3 d, int* ia, int* ib) const {
+        constexpr int NONE = 0x7fffffff;
+        double best = -1e30, best2 = -1e30;
+        int i1 = NONE, i2 = NONE;
+        const d3 nd = neg3(d);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int i = g + G * k;
+            if (i < A.n) {
+                const double t = dot3(d, va[k]);
+                if (t > best) {
+                    best = t;
+                    i1 = i;
+                }
+            }
+            if (i < B.n) {
+                const double t = dot3(nd, vb[k]);
+                if (t > best2) {
+                    best2 = t;
+                    i2 = i;
+                }
+            }
+        }
+        merge(best, i1);
+        merge(best2, i2);
+        if (i1 == NONE) i1 = 0;
+        if (i2 == NONE) i2 = 0;
+        *ia = i1;
+        *ib = i2;
+        return pick(va, i1) - pick(vb, i2);
+    }
+    // vertex i of a set held across the group: slot i / G of lane i % G
+    __device__ __forceinline__ static d3 pick(const d3* v, int i) {
+        const int k = i / G, src = i - k * G;
+        d3 mine = v[0];
+#pragma unroll
+        for (int j = 1; j < K; ++j)
+            if (k == j) mine = v[j];
+        return mk3(__shfl(mine.x, src, G), __shfl(mine.y, src, G), __shfl(mine.z, src, G));
+    }
+};
+
 // Check_linesegment, .cu:103-124
 __device__ __forceinline__ void check_segment(d3 P1, d3 P2, int* coll, d3* d) {
     *coll = -1;
@@ -177,24 +257,25 @@ __device__ __forceinline__ bool correct_flat(d3 S1, d3 S2, d3 S3, d3 S4) {
 
 // GJK of one pair (.cu:1009-1288). Returns -1 (tetrahedron cap), 0 or 1; on
 // 1 the simplex handed to EPA (after CorrectFlat) is in S/ia/ib.
-__device__ int gjk_pair(const Side& A, const Side& B, d3 comA, d3 comB, d3* S, int* ia, int* ib) {
+template <class Sup>
+__device__ int gjk_pair(const Sup& support, d3 comA, d3 comB, d3* S, int* ia, int* ib) {
     const d3 d1 = comB - comA;
-    S[0] = support(A, B, d1, &ia[0], &ib[0]);
+    S[0] = support(d1, &ia[0], &ib[0]);
     if (dot3(d1, S[0]) < 0.0) return 0;
     const d3 d2 = neg3(d1);
-    S[1] = support(A, B, d2, &ia[1], &ib[1]);
+    S[1] = support(d2, &ia[1], &ib[1]);
     if (dot3(d2, S[1]) < 0.0) return 0;
     int coll;
     d3 d3v;
     check_segment(S[0], S[1], &coll, &d3v);
     if (coll == 0) return 0;
-    S[2] = support(A, B, d3v, &ia[2], &ib[2]);
+    S[2] = support(d3v, &ia[2], &ib[2]);
     if (dot3(d3v, S[2]) < 0.0) return 0;
     d3 d4;
     check_triangle(S[0], S[1], S[2], &coll, &d4);
     if (coll == 0) return 0;
     for (int zz = 0; zz < 50; ++zz) {
-        S[3] = support(A, B, d4, &ia[3], &ib[3]);
+        S[3] = support(d4, &ia[3], &ib[3]);
         if (dot3(d4, S[3]) < 0.0) return 0;
         bool hit = zero3(S[3]);
         if (!hit) {
@@ -224,7 +305,7 @@ __device__ int gjk_pair(const Side& A, const Side& B, d3 comA, d3 comB, d3* S, i
         if (hit) {
             if (correct_flat(S[0], S[1], S[2], S[3])) {
                 d4 = cross3(S[0], S[1]);
-                S[3] = support(A, B, d4, &ia[3], &ib[3]);
+                S[3] = support(d4, &ia[3], &ib[3]);
             }
             return 1;
         }
@@ -272,7 +353,11 @@ __global__ void __launch_bounds__(256) k_sphere(BodyView A, BodyView B, int64_t 
     }
 }
 
-// GJK over the survivors (.cu:983-1322, without EPA).
+// GJK over the survivors (.cu:983-1322, without EPA), G lanes per pair
+// (GroupSup); every lane of a group follows the same path on the same
+// values, and its first lane writes the results. K = 0 selects one lane per
+// pair with the vertices read from memory (bodies of more than 4G vertices).
+template <int G, int K>
 __global__ void __launch_bounds__(256) k_gjk(BodyView A, BodyView B, int64_t n_cap, const int32_t* __restrict__ surv,
                                              const int64_t* n_surv, int32_t* __restrict__ collision,
                                              int32_t* __restrict__ flag, Simplex* __restrict__ simplex,
@@ -280,8 +365,11 @@ __global__ void __launch_bounds__(256) k_gjk(BodyView A, BodyView B, int64_t n_c
                                              uint32_t* flags) {
     int64_t m = *n_surv;
     if (m > n_cap) m = n_cap;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < m; s += stride) {
+    constexpr int GG = K > 0 ? G : 1;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = (int)(t % GG);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x / GG;
+    for (int64_t s = t / GG; s < m; s += stride) {
         const int64_t p = surv[s];
         const int64_t ra = view_row(A, p), rb = view_row(B, p);
         const d3 ca = ld3(A.com + 3 * ra), cb = ld3(B.com + 3 * rb);
@@ -289,7 +377,15 @@ __global__ void __launch_bounds__(256) k_gjk(BodyView A, BodyView B, int64_t n_c
         const Side sb{B.verts + (int64_t)B.vmax * 3 * rb, B.nverts[rb]};
         d3 S[4];
         int ia[4] = {0, 0, 0, 0}, ib[4] = {0, 0, 0, 0};
-        const int c = gjk_pair(sa, sb, ca, cb, S, ia, ib);
+        int c;
+        if constexpr (K > 0) {
+            GroupSup<G, K> sup;
+            sup.load(sa, sb, g);
+            c = gjk_pair(sup, ca, cb, S, ia, ib);
+        } else {
+            c = gjk_pair(SerialSup{sa, sb}, ca, cb, S, ia, ib);
+        }
+        if (g != 0) continue;
         collision[p] = c;
         flag[p] = c == 1 ? 1 : 0;
         if (c == 1) {
@@ -646,6 +742,20 @@ __device__ LawOut<T> force_law(V3<T> comA, V3<T> comB, V3<T> vA, V3<T> vB, V3<T>
     return o;
 }
 
+__device__ __forceinline__ uint32_t push_record(int32_t* rec_cnt, int32_t* rec_bkt, int32_t* rec_ovf, int64_t* n_ovf,
+                                               int64_t ovf_cap, int64_t row, int32_t rec) {
+    const int32_t k = atomicAdd(&rec_cnt[row], 1);
+    if (k < MWD_ROW_CONTACTS) {
+        rec_bkt[row * MWD_ROW_CONTACTS + k] = rec;
+        return 0;
+    }
+    const int64_t o = (int64_t)atomicAdd((unsigned long long*)n_ovf, 1ull);
+    if (o >= ovf_cap) return MWD_F_ROW_CONTACTS;
+    rec_ovf[2 * o] = (int32_t)row;
+    rec_ovf[2 * o + 1] = rec;
+    return 0;
+}
+
 template <class T>
 __device__ __forceinline__ V3<T> body3(const double* p, const TArr<MWD_NT>& t, int64_t r) {
     if constexpr (std::is_same<T, double>::value)
@@ -677,7 +787,9 @@ __global__ void __launch_bounds__(256) k_epa_force(BodyView A, BodyView B, BodyT
                                                    double* __restrict__ cpA, double* __restrict__ cpB, ContactOut co,
                                                    double* __restrict__ disp_out, int32_t* __restrict__ epa_status,
                                                    uint32_t* __restrict__ rec_key, int32_t* __restrict__ rec_val,
-                                                   int64_t* n_records, uint32_t* flags) {
+                                                   int64_t* n_records, int32_t* __restrict__ rec_cnt,
+                                                   int32_t* __restrict__ rec_bkt, int32_t* __restrict__ rec_ovf,
+                                                   int64_t* n_ovf, int64_t ovf_cap, uint32_t* flags) {
     int64_t K = *n_contacts;
     if (K > contact_cap) K = contact_cap;
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -748,10 +860,17 @@ __global__ void __launch_bounds__(256) k_epa_force(BodyView A, BodyView B, BodyT
             dp[2] = val(f.disp.z);
             dp[3] = 1.0;
         }
-        rec_key[2 * c] = (uint32_t)(idA ? idA[p] : ra);
-        rec_val[2 * c] = (int32_t)(2 * c);
-        rec_key[2 * c + 1] = (uint32_t)(idB ? idB[p] : rb);
-        rec_val[2 * c + 1] = (int32_t)(2 * c + 1);
+        if (rec_cnt) {
+            // buckets of the moving rows (walls do not integrate); a full
+            // bucket spills into the shared overflow list as (row, record)
+            if (!A.is_wall[ra]) fl |= push_record(rec_cnt, rec_bkt, rec_ovf, n_ovf, ovf_cap, ra, (int32_t)(2 * c));
+            if (!B.is_wall[rb]) fl |= push_record(rec_cnt, rec_bkt, rec_ovf, n_ovf, ovf_cap, rb, (int32_t)(2 * c + 1));
+        } else {
+            rec_key[2 * c] = (uint32_t)(idA ? idA[p] : ra);
+            rec_val[2 * c] = (int32_t)(2 * c);
+            rec_key[2 * c + 1] = (uint32_t)(idB ? idB[p] : rb);
+            rec_val[2 * c + 1] = (int32_t)(2 * c + 1);
+        }
     }
     if (fl) atomicOr(flags, fl);
 }
@@ -836,15 +955,51 @@ unsigned grid_for(int64_t n, int threads, int64_t max_blocks) {
 
 }  // namespace
 
+int mwd_launch_sphere(const BodyView& A, const BodyView& B, int64_t n_cap, const int64_t* n_dev, int32_t* collision,
+                      int32_t* flag, int32_t* surv, int64_t* n_surv, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+    if (n_cap <= 0) return MWD_OK;
+    MWD_LAUNCH_EV(e0, e1, k_sphere, dim3(grid_for(n_cap, 256, 8192)), dim3(256), 0, s, A, B, n_cap, n_dev, collision,
+                  flag, surv, n_surv);
+    MWD_LAUNCH_CHECK();
+    return MWD_OK;
+}
+
 int mwd_launch_gjk(const BodyView& A, const BodyView& B, int64_t n_cap, const int64_t* n_dev, int32_t* collision,
                    int32_t* flag, int32_t* surv, int64_t* n_surv, Simplex* simplex, int32_t* sidx, int64_t* slots,
                    int64_t slot_cap, uint32_t* flags, hipStream_t s) {
+    const int rc = mwd_launch_sphere(A, B, n_cap, n_dev, collision, flag, surv, n_surv, s);
+    if (rc) return rc;
+    return mwd_launch_gjk_surv(A, B, n_cap, collision, flag, surv, n_surv, simplex, sidx, slots, slot_cap, flags, s);
+}
+
+int mwd_launch_gjk_surv(const BodyView& A, const BodyView& B, int64_t n_cap, int32_t* collision, int32_t* flag,
+                        const int32_t* surv, const int64_t* n_surv, Simplex* simplex, int32_t* sidx, int64_t* slots,
+                        int64_t slot_cap, uint32_t* flags, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     if (n_cap <= 0) return MWD_OK;
-    hipLaunchKernelGGL(k_sphere, dim3(grid_for(n_cap, 256, 8192)), dim3(256), 0, s, A, B, n_cap, n_dev, collision,
-                       flag, surv, n_surv);
-    MWD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_gjk, dim3(grid_for(n_cap, 256, 2048)), dim3(256), 0, s, A, B, n_cap, surv, n_surv,
-                       collision, flag, simplex, sidx, slots, slot_cap, flags);
+    constexpr int G = MWD_GJK_GROUP;
+    const int K = (A.vmax + G - 1) / G;
+    const dim3 grid(grid_for(n_cap * (K <= 4 ? G : 1), 256, 4096)), blk(256);
+    switch (K <= 4 ? K : 0) {
+        case 1:
+            MWD_LAUNCH_EV(e0, e1, (k_gjk<G, 1>), grid, blk, 0, s, A, B, n_cap, surv, n_surv, collision, flag, simplex,
+                               sidx, slots, slot_cap, flags);
+            break;
+        case 2:
+            MWD_LAUNCH_EV(e0, e1, (k_gjk<G, 2>), grid, blk, 0, s, A, B, n_cap, surv, n_surv, collision, flag, simplex,
+                               sidx, slots, slot_cap, flags);
+            break;
+        case 3:
+            MWD_LAUNCH_EV(e0, e1, (k_gjk<G, 3>), grid, blk, 0, s, A, B, n_cap, surv, n_surv, collision, flag, simplex,
+                               sidx, slots, slot_cap, flags);
+            break;
+        case 4:
+            MWD_LAUNCH_EV(e0, e1, (k_gjk<G, 4>), grid, blk, 0, s, A, B, n_cap, surv, n_surv, collision, flag, simplex,
+                               sidx, slots, slot_cap, flags);
+            break;
+        default:
+            MWD_LAUNCH_EV(e0, e1, (k_gjk<G, 0>), grid, blk, 0, s, A, B, n_cap, surv, n_surv, collision, flag, simplex,
+                               sidx, slots, slot_cap, flags);
+    }
     MWD_LAUNCH_CHECK();
     return MWD_OK;
 }
@@ -886,22 +1041,35 @@ int mwd_launch_epa_force(const BodyView& A, const BodyView& B, const mwd_law* la
     co.tB = plain(tB);
     hipLaunchKernelGGL(k_epa_force<double>, dim3(grid_for(contact_cap, 256, 4096)), dim3(256), 0, s, A, B, bt, laws2,
                        (const mwd_law_tan*)nullptr, contact_pair, simplex, sidx, contact_cap, n_contacts, dispt, idA,
-                       idB, depth, normal, cpA, cpB, co, disp_out, epa_status, rec_key, rec_val, n_records, flags);
+                       idB, depth, normal, cpA, cpB, co, disp_out, epa_status, rec_key, rec_val, n_records,
+                       (int32_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr, (int64_t*)nullptr, (int64_t)0, flags);
     MWD_LAUNCH_CHECK();
     return MWD_OK;
 }
 
-int mwd_launch_epa_force_tan(const BodyView& A, const BodyView& B, const BodyTan& bt, const mwd_law* laws2,
-                             const mwd_law_tan* lawt2, const int32_t* contact_pair, const Simplex* simplex,
-                             const int32_t* sidx, int64_t contact_cap, const int64_t* n_contacts, double* depth,
-                             double* normal, double* cpA, double* cpB, const ContactOut& co, int32_t* epa_status,
-                             uint32_t* rec_key, int32_t* rec_val, int64_t* n_records, uint32_t* flags,
-                             hipStream_t s) {
+int mwd_launch_epa_force_rows(const BodyView& A, const BodyView& B, const BodyTan* bt, const mwd_law* laws2,
+                              const mwd_law_tan* lawt2, const int32_t* contact_pair, const Simplex* simplex,
+                              const int32_t* sidx, int64_t contact_cap, const int64_t* n_contacts, double* depth,
+                              double* normal, double* cpA, double* cpB, const ContactOut& co, int32_t* epa_status,
+                              int32_t* rec_cnt, int32_t* rec_bkt, int32_t* rec_ovf, int64_t* n_ovf,
+                              int64_t ovf_cap, uint32_t* flags, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     if (contact_cap <= 0) return MWD_OK;
-    hipLaunchKernelGGL(k_epa_force<Dual<MWD_NT>>, dim3(grid_for(contact_cap, 256, 4096)), dim3(256), 0, s, A, B, bt,
-                       laws2, lawt2, contact_pair, simplex, sidx, contact_cap, n_contacts, (const double*)nullptr,
-                       (const int32_t*)nullptr, (const int32_t*)nullptr, depth, normal, cpA, cpB, co,
-                       (double*)nullptr, epa_status, rec_key, rec_val, n_records, flags);
+    const dim3 grid(grid_for(contact_cap, 256, 4096)), blk(256);
+    if (bt) {
+        MWD_LAUNCH_EV(e0, e1, k_epa_force<Dual<MWD_NT>>, grid, blk, 0, s, A, B, *bt, laws2, lawt2, contact_pair, simplex,
+                           sidx, contact_cap, n_contacts, (const double*)nullptr, (const int32_t*)nullptr,
+                           (const int32_t*)nullptr, depth, normal, cpA, cpB, co, (double*)nullptr, epa_status,
+                           (uint32_t*)nullptr, (int32_t*)nullptr, (int64_t*)nullptr, rec_cnt, rec_bkt, rec_ovf, n_ovf,
+                           ovf_cap, flags);
+    } else {
+        BodyTan none;
+        memset(&none, 0, sizeof(none));
+        MWD_LAUNCH_EV(e0, e1, k_epa_force<double>, grid, blk, 0, s, A, B, none, laws2, (const mwd_law_tan*)nullptr,
+                           contact_pair, simplex, sidx, contact_cap, n_contacts, (const double*)nullptr,
+                           (const int32_t*)nullptr, (const int32_t*)nullptr, depth, normal, cpA, cpB, co,
+                           (double*)nullptr, epa_status, (uint32_t*)nullptr, (int32_t*)nullptr, (int64_t*)nullptr,
+                           rec_cnt, rec_bkt, rec_ovf, n_ovf, ovf_cap, flags);
+    }
     MWD_LAUNCH_CHECK();
     return MWD_OK;
 }

@@ -214,35 +214,96 @@ __global__ void k_cell_info(const int32_t* __restrict__ vals, const int32_t* __r
     cell_cnt[r] = (int32_t)cnt;
 }
 
-__global__ void k_cell_fill(const int32_t* __restrict__ perm, const int32_t* __restrict__ cell_start,
-                            const int32_t* __restrict__ cell_nb, const int32_t* __restrict__ cell_by_rank,
-                            const int64_t* __restrict__ cell_off, const int64_t* __restrict__ counters,
-                            int32_t* __restrict__ pairs, int64_t cap) {
+// Pair emission, one output pair per thread: a block takes 64 cells by rank,
+// stages their pair offsets, members and neighbour cells in LDS, and its
+// threads walk the block's output range, so consecutive threads write
+// consecutive pairs. Pair q of a cell is decoded in the emission order of
+// Linked_cell_hash (LIBRARY_Coarse_contact_detection.py:84-118): same-cell
+// pairs (j < k, members in set order), then per neighbour offset o every
+// member j against every member k of that cell.
+constexpr int FILL_CELLS = 64;    // cells per block
+constexpr int FILL_THREADS = 256;
+
+__global__ void __launch_bounds__(FILL_THREADS) k_cell_fill(const int32_t* __restrict__ perm,
+                                                          const int32_t* __restrict__ cell_start,
+                                                          const int32_t* __restrict__ cell_nb,
+                                                          const int32_t* __restrict__ cell_by_rank,
+                                                          const int64_t* __restrict__ cell_off,
+                                                          const int64_t* __restrict__ counters,
+                                                          int32_t* __restrict__ pairs, int64_t cap) {
+    __shared__ int64_t s_off[FILL_CELLS + 1];
+    __shared__ int32_t s_s0[FILL_CELLS], s_n[FILL_CELLS];
+    __shared__ int32_t s_t0[FILL_CELLS][MWD_MAX_OFF], s_tn[FILL_CELLS][MWD_MAX_OFF];
     const int64_t C = counters[0];
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= C) return;
-    const int32_t c = cell_by_rank[r];
-    int64_t pos = counters[1] + cell_off[r];
-    const int32_t s0 = cell_start[c], s1 = cell_start[c + 1];
-    for (int32_t j = s0; j < s1; ++j) {
-        const int32_t a = perm[j];
-        for (int32_t k = j + 1; k < s1; ++k, ++pos)
-            if (pos < cap) {
-                pairs[2 * pos] = a;
-                pairs[2 * pos + 1] = perm[k];
-            }
+    const int64_t r0 = (int64_t)blockIdx.x * FILL_CELLS;
+    if (r0 >= C) return;  // uniform per block
+    const int nc = (int)(C - r0 < FILL_CELLS ? C - r0 : FILL_CELLS);
+    const int tid = threadIdx.x;
+    const int64_t base = cell_off[r0];
+    if (tid < nc) {
+        const int32_t c = cell_by_rank[r0 + tid];
+        const int32_t s0 = cell_start[c];
+        const int32_t n = cell_start[c + 1] - s0;
+        s_s0[tid] = s0;
+        s_n[tid] = n;
+        int64_t cnt = (int64_t)n * (n - 1) / 2;
+        for (int o = 0; o < MWD_MAX_OFF; ++o) {
+            const int32_t nb = cell_nb[(int64_t)c * MWD_MAX_OFF + o];
+            const int32_t t0 = nb >= 0 ? cell_start[nb] : 0;
+            const int32_t tn = nb >= 0 ? cell_start[nb + 1] - t0 : 0;
+            s_t0[tid][o] = t0;
+            s_tn[tid][o] = tn;
+            cnt += (int64_t)n * tn;
+        }
+        s_off[tid + 1] = cell_off[r0 + tid] - base + cnt;  // end of this cell's range
     }
-    for (int o = 0; o < MWD_MAX_OFF; ++o) {
-        const int32_t nb = cell_nb[c * MWD_MAX_OFF + o];
-        if (nb < 0) continue;
-        const int32_t t0 = cell_start[nb], t1 = cell_start[nb + 1];
-        for (int32_t j = s0; j < s1; ++j) {
-            const int32_t a = perm[j];
-            for (int32_t k = t0; k < t1; ++k, ++pos)
-                if (pos < cap) {
-                    pairs[2 * pos] = a;
-                    pairs[2 * pos + 1] = perm[k];
-                }
+    if (tid == 0) s_off[0] = 0;
+    __syncthreads();
+    const int64_t total = s_off[nc];
+    const int64_t pos0 = counters[1] + base;
+    for (int64_t q = tid; q < total; q += FILL_THREADS) {
+        // cell i: s_off[i] <= q < s_off[i + 1]
+        int lo = 0, hi = nc;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s_off[mid] <= q) lo = mid;
+            else hi = mid;
+        }
+        const int i = lo;
+        const int32_t n = s_n[i], s0 = s_s0[i];
+        int64_t t = q - s_off[i];
+        const int64_t tri = (int64_t)n * (n - 1) / 2;
+        int32_t a, b;
+        if (t < tri) {
+            int j = 0;
+            int64_t row = n - 1;  // pairs of member j with the later members
+            while (t >= row) {
+                t -= row;
+                ++j;
+                --row;
+            }
+            a = perm[s0 + j];
+            b = perm[s0 + j + 1 + (int32_t)t];
+        } else {
+            t -= tri;
+            int o = 0;
+            for (; o < MWD_MAX_OFF; ++o) {
+                const int64_t sz = (int64_t)n * s_tn[i][o];
+                if (t < sz) break;
+                t -= sz;
+            }
+            if (o == MWD_MAX_OFF) continue;  // not reached: q lies inside cell i's range
+            const int32_t tn = s_tn[i][o];
+            const int32_t j = (int32_t)(t / tn), k = (int32_t)(t - (int64_t)j * tn);
+            a = perm[s0 + j];
+            b = perm[s_t0[i][o] + k];
+        }
+        const int64_t pos = pos0 + q;
+        if (pos < cap) {
+            int2 v;
+            v.x = a;
+            v.y = b;
+            reinterpret_cast<int2*>(pairs)[pos] = v;
         }
     }
 }
@@ -296,6 +357,7 @@ __global__ void k_finish_pairs(int64_t* counters, int64_t cap, uint32_t* flags) 
     counters[7] = P;
     counters[8] = 0;   // simplex slots of the fine-contact stage that consumes these pairs
     counters[10] = 0;  // its sphere-test survivors
+    counters[11] = 0;  // and its bucket overflow records
     if (P > cap) atomicOr(flags, MWD_F_PAIR_OVERFLOW);
 }
 
@@ -342,6 +404,7 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
     }
     // -- linked-cell pairs
     if (M > 0) {
+        mwd_mark(ws, 1);
         hipLaunchKernelGGL(k_sort_keys, dim3(blocks_for(M)), dim3(256), 0, s, st->com, keys_in, ws->plist, M,
                            ws->scene_of, ws->scenes_dev, ws->keys, ws->vals, ws->flags);
         MWD_LAUNCH_CHECK();
@@ -352,7 +415,7 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
         if (rc) return rc;
         ws->keys_sorted = ks;
         ws->vals_sorted = vs;
-        mwd_mark(ws, 1);
+        mwd_mark(ws, 2);
         hipLaunchKernelGGL(k_heads, dim3(blocks_for(M)), dim3(256), 0, s, ks, M, ws->head);
         MWD_LAUNCH_CHECK();
         rc = mwd_scan_i32(ws->head, ws->head_off, ctr + 0, M, nullptr, ws->scan_tmp, ws->scan_tmp_len, s);
@@ -376,10 +439,13 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
         MWD_LAUNCH_CHECK();
         rc = mwd_scan_i32(ws->cell_cnt, ws->cell_off, ctr + 2, M, ctr + 0, ws->scan_tmp, ws->scan_tmp_len, s);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_cell_fill, dim3(blocks_for(M)), dim3(256), 0, s, ws->perm, ws->cell_start, ws->cell_nb,
-                           ws->cell_by_rank, ws->cell_off, ctr, ws->pairs, ws->pair_cap);
+        mwd_mark(ws, 3);
+        MWD_LAUNCH_EV(mwd_kev(ws, MWD_KT_EMIT, 0), mwd_kev(ws, MWD_KT_EMIT, 1), k_cell_fill,
+                      dim3((unsigned)((M + FILL_CELLS - 1) / FILL_CELLS)), dim3(FILL_THREADS), 0, s, ws->perm,
+                      ws->cell_start, ws->cell_nb, ws->cell_by_rank, ws->cell_off, ctr, ws->pairs, ws->pair_cap);
         MWD_LAUNCH_CHECK();
     }
+    mwd_mark(ws, 4);
     hipLaunchKernelGGL(k_finish_pairs, dim3(1), dim3(1), 0, s, ctr, ws->pair_cap, ws->flags);
     MWD_LAUNCH_CHECK();
     return MWD_OK;

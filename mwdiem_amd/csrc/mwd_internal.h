@@ -4,12 +4,21 @@
 #include <stdint.h>
 #include <string>
 
+#include <hip/hip_ext.h>
+
 #include "../../include/mwdiem_hip.h"
 #include "mwd_math.h"
 
 #define MWD_WAVE 64
-#define MWD_N_STAGES 7  // wall+keys+sort, cells+emit, gjk, compact, epa+law, record sort, reduce+verlet
+#define MWD_GJK_GROUP 8  // lanes that share one pair's support map in GJK
 #define MWD_TIMING_RING 512
+#define MWD_EV_PER_STEP (MWD_N_STAGES + 1 + 2 * MWD_N_KTIMED)
+// timed kernels
+#define MWD_KT_EMIT 0
+#define MWD_KT_SPHERE 1
+#define MWD_KT_GJK 2
+#define MWD_KT_EPA 3
+#define MWD_KT_VERLET 4
 
 // ------------------------------------------------------------------ errors
 void mwd_set_error(const std::string& msg);
@@ -140,13 +149,14 @@ struct mwd_workspace {
     int32_t* epa_status;       // [contact_cap]
     double* ctan;              // [3 arrays * MWD_NT * contact_cap * 3] tangents of Fn, tA, tB (tangent step)
     mwd_law_tan* law_tans;     // [2*n_scenes] (tangent step)
-    uint32_t *rec_key, *rec_key_alt;  // [2*contact_cap]
-    int32_t *rec_val, *rec_val_alt;
-    uint32_t* rec_key_sorted;         // whichever of rec_key / rec_key_alt holds the sorted records
-    int32_t* rec_val_sorted;
+    // per moving row, the records (2c for side A of contact c, 2c+1 for side
+    // B) of this step's contacts, in arrival order; the reduction sorts them
+    int32_t* rec_cnt;          // [n_rows], zero between steps
+    int32_t* rec_bkt;          // [n_rows * MWD_ROW_CONTACTS]
+    int32_t* rec_ovf;          // [2 * 2*contact_cap] (row, record) of records beyond a full bucket; count in counters[11]
+    int bkt_dirty;             // host: buckets filled and not yet consumed
     uint32_t* keys_sorted;            // whichever of keys / keys_alt holds the sorted cell keys
     int32_t* vals_sorted;
-    int32_t *seg_begin, *seg_end;     // [n_rows]
     // sort / scan scratch
     int32_t* hist;             // [256 * max_tiles]
     int64_t* hist_off;         // [256 * max_tiles]
@@ -155,15 +165,20 @@ struct mwd_workspace {
     // counters (device): 0 n_cells, 1 n_wall_pairs, 2 n_cell_pairs, 3 n_pairs
     // (capped), 4 n_contacts, 5 n_records, 6 n_cells of the previous step,
     // 7 n_pairs before capping, 8 simplex slots claimed this step, 9 radix
-    // histogram entries (scratch), 10 sphere-test survivors
+    // histogram entries (scratch), 10 sphere-test survivors, 11 bucket
+    // overflow records
     int64_t* counters;
     uint32_t* flags;
     mwd_law* laws;             // [2*n_scenes] (particle law, wall law)
     int64_t* scene_wt_off;     // [n_scenes+1] wall-test prefix (device)
     // optional per-stage timing (HIP events on the workspace stream)
     int timing;
-    hipEvent_t* ev;            // [MWD_TIMING_RING * (MWD_N_STAGES + 1)]
+    hipEvent_t* ev;            // [MWD_TIMING_RING * MWD_EV_PER_STEP]
     int ring_slot;             // steps recorded and not yet accumulated
+    int64_t timing_errors;     // event records or elapsed-time queries that failed
+    double kernel_ms[MWD_N_KTIMED];
+    int64_t kernel_calls;
+    uint32_t kt_used;          // timed kernels launched with events since timing was enabled
     double stage_ms[MWD_N_STAGES];
     int64_t stage_calls;
 };
@@ -171,6 +186,19 @@ struct mwd_workspace {
 // stage marks (events recorded before each stage when timing is on)
 void mwd_mark(mwd_workspace* ws, int stage);
 void mwd_mark_end(mwd_workspace* ws);
+// Start/stop events of a timed kernel in the current step (NULL when timing
+// is off): such kernels are launched with hipExtLaunchKernelGGL, which
+// stamps the events with the kernel's own start and end.
+hipEvent_t mwd_kev(mwd_workspace* ws, int kt, int stop);
+
+// Launch with start/stop events when e0 is set, else a plain launch.
+#define MWD_LAUNCH_EV(e0, e1, kernel, grid, block, shmem, stream, ...)                                   \
+    do {                                                                                                 \
+        if (e0)                                                                                          \
+            hipExtLaunchKernelGGL(kernel, grid, block, shmem, stream, e0, e1, 0u, __VA_ARGS__);          \
+        else                                                                                             \
+            hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);                         \
+    } while (0)
 
 // ----------------------------------------------------------- primitives
 // Exclusive scan of int32 values into int64 offsets. n is n_dev[0] when
@@ -213,6 +241,14 @@ int mwd_launch_cell_keys(const double* com, int64_t n, const double* cell, int64
 // runs over the survivors only, writing collision and flag (collision == 1).
 // A colliding pair claims a simplex slot (counter *slots, zero on entry) and
 // stores it in sidx.
+int mwd_launch_sphere(const BodyView& A, const BodyView& B, int64_t n_cap, const int64_t* n_dev, int32_t* collision,
+                      int32_t* flag, int32_t* surv, int64_t* n_surv, hipStream_t s, hipEvent_t e0 = nullptr,
+                      hipEvent_t e1 = nullptr);
+int mwd_launch_gjk_surv(const BodyView& A, const BodyView& B, int64_t n_cap, int32_t* collision, int32_t* flag,
+                        const int32_t* surv, const int64_t* n_surv, Simplex* simplex, int32_t* sidx, int64_t* slots,
+                        int64_t slot_cap, uint32_t* flags, hipStream_t s, hipEvent_t e0 = nullptr,
+                        hipEvent_t e1 = nullptr);
+// Both of the above.
 int mwd_launch_gjk(const BodyView& A, const BodyView& B, int64_t n_cap, const int64_t* n_dev,
                    int32_t* collision, int32_t* flag, int32_t* surv, int64_t* n_surv, Simplex* simplex,
                    int32_t* sidx, int64_t* slots, int64_t slot_cap, uint32_t* flags, hipStream_t s);
@@ -228,14 +264,17 @@ int mwd_launch_epa_force(const BodyView& A, const BodyView& B, const mwd_law* la
                          const int32_t* idA, const int32_t* idB, double* depth, double* normal, double* cpA,
                          double* cpB, double* Fn, double* tA, double* tB, double* disp_out, int32_t* epa_status,
                          uint32_t* rec_key, int32_t* rec_val, int64_t* n_records, uint32_t* flags, hipStream_t s);
-// The same over the stepper's state with tangents: A and B index one state
-// (bt), the law derivatives are per scene like laws2, and co receives values
-// and tangents of Fn, tA, tB.
-int mwd_launch_epa_force_tan(const BodyView& A, const BodyView& B, const BodyTan& bt, const mwd_law* laws2,
-                             const mwd_law_tan* lawt2, const int32_t* contact_pair, const Simplex* simplex,
-                             const int32_t* sidx, int64_t contact_cap, const int64_t* n_contacts, double* depth,
-                             double* normal, double* cpA, double* cpB, const ContactOut& co, int32_t* epa_status,
-                             uint32_t* rec_key, int32_t* rec_val, int64_t* n_records, uint32_t* flags, hipStream_t s);
+// The stepper's form over one state: records go to per-row buckets (moving
+// rows only) instead of a list; bt (tangent path) carries the state's
+// tangents, lawt2 the law derivatives per scene like laws2, and co receives
+// values (and tangents) of Fn, tA, tB.
+int mwd_launch_epa_force_rows(const BodyView& A, const BodyView& B, const BodyTan* bt, const mwd_law* laws2,
+                              const mwd_law_tan* lawt2, const int32_t* contact_pair, const Simplex* simplex,
+                              const int32_t* sidx, int64_t contact_cap, const int64_t* n_contacts, double* depth,
+                              double* normal, double* cpA, double* cpB, const ContactOut& co, int32_t* epa_status,
+                              int32_t* rec_cnt, int32_t* rec_bkt, int32_t* rec_ovf, int64_t* n_ovf,
+                              int64_t ovf_cap, uint32_t* flags, hipStream_t s, hipEvent_t e0 = nullptr,
+                              hipEvent_t e1 = nullptr);
 // Zero the per-pair outputs, then copy per-contact results to their pairs.
 int mwd_launch_zero_pairs(int64_t P, double* depth, double* normal, double* cpA, double* cpB, int32_t* epa_status,
                           double* disp, hipStream_t s);

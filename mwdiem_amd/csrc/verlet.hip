@@ -8,21 +8,34 @@
 //
 // The stepper variant fuses the force assembly (MWDiem.py:977-981): the
 // per-particle contact force and torque are summed over the particle's
-// contact records in ascending pair order (the records were stably sorted by
-// row), then the external force/torque is added, then the step runs.
+// contact records in ascending pair order (its bucket of records, sorted),
+// then the external force/torque is added, then the step runs.
 //
 // Both are templated on the scalar: double for the forward path, Dual<3> for
 // the tangent path (derivatives with respect to Kn, en and mu carried along;
 // values bit-identical to the forward path).
+#include <type_traits>
+
 #include "mwd_internal.h"
 
 namespace {
 
+// What the vertex pass needs of a particle's step: the rotation, the old
+// centre, the translation, whether the out-of-box reset undoes it, and the
+// new centre (for the bounding radius).
+struct VertJob {
+    q4 qr;
+    d3 com, dx, cnew;
+    int32_t nrot, out;
+};
+
 // rot_mode 0: vertices rotated by the absolute orientation q (the reference's
-// compounding rotation); 1: by the step's increment q q_old^-1.
+// compounding rotation); 1: by the step's increment q q_old^-1. With job
+// given (T = double only), the vertex update is left to the caller's
+// block-wide vertex pass and described in *job instead.
 template <class T>
 __device__ __forceinline__ void verlet_row(const VState& a, int64_t P, V3<T> F, V3<T> tau, int vmax, int nrot,
-                                           double dt, const double* box, int rot_mode) {
+                                           double dt, const double* box, int rot_mode, VertJob* job = nullptr) {
     V3<T> v12 = ldv<T>(a.v12, P);
     V3<T> w12bf = ldv<T>(a.w12bf, P);
     V3<T> wbf = ldv<T>(a.wbf, P);
@@ -70,16 +83,28 @@ __device__ __forceinline__ void verlet_row(const VState& a, int64_t P, V3<T> F, 
     const V3<T> du = dx * -1.0;
     V3<T> cnew = c1;
     if (out) cnew = cnew + du;
-    double r2 = 0.0;
-    const int64_t vb = (int64_t)vmax * P;
-    for (int i = 0; i < nrot; ++i) {
-        V3<T> r = qrot(qr, ldv<T>(a.verts, vb + i) - com, qc);
-        V3<T> nv = (r + com) + dx;
-        if (out) nv = nv + du;
-        stv(a.verts, vb + i, nv);
-        r2 = fmax(r2, val(nrm2(nv - cnew)));
+    if constexpr (std::is_same<T, double>::value) {
+        if (job) {
+            job->qr = qr;
+            job->com = com;
+            job->dx = dx;
+            job->cnew = cnew;
+            job->nrot = nrot;
+            job->out = out ? 1 : 0;
+        }
     }
-    if (a.radius) a.radius[P] = inflate_radius(r2, mk3(val(cnew.x), val(cnew.y), val(cnew.z)));
+    if (!job) {
+        double r2 = 0.0;
+        const int64_t vb = (int64_t)vmax * P;
+        for (int i = 0; i < nrot; ++i) {
+            V3<T> r = qrot(qr, ldv<T>(a.verts, vb + i) - com, qc);
+            V3<T> nv = (r + com) + dx;
+            if (out) nv = nv + du;
+            stv(a.verts, vb + i, nv);
+            r2 = fmax(r2, val(nrm2(nv - cnew)));
+        }
+        if (a.radius) a.radius[P] = inflate_radius(r2, mk3(val(cnew.x), val(cnew.y), val(cnew.z)));
+    }
     com = com + dx;
     if (out) {
         vel = zero3<T>();
@@ -105,46 +130,172 @@ struct Box6 {
     double b[6];
 };
 
-__global__ void __launch_bounds__(256) k_verlet(VState a, const double* __restrict__ F,
-                                                const double* __restrict__ tau, int vmax,
-                                                const int32_t* __restrict__ nrot, double dt, Box6 box, int64_t pmin,
-                                                int64_t pmax) {
-    const int64_t P = pmin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (P >= pmax) return;
-    verlet_row<double>(a, P, ld3(F + 3 * P), ld3(tau + 3 * P), vmax, nrot ? nrot[P] : vmax, dt, box.b, 0);
+constexpr int VBLOCK = 256;
+
+// Block-wide vertex pass: the block's particles (rows[j], job[j]) have their
+// vertex slots updated by consecutive threads, so a wave reads and writes
+// consecutive vertices (the per-lane loop strides by a whole particle). Each
+// vertex is transformed by exactly the operations of the per-lane loop; the
+// bounding radius takes the maximum through an LDS atomic on the bits of the
+// non-negative squared distances (NaNs skipped, as fmax skips them).
+__device__ __forceinline__ void vertex_pass(double* __restrict__ verts, const int64_t* rows, const VertJob* job,
+                                            unsigned long long* r2bits, int nb, int vmax, double* radius) {
+    const int tid = threadIdx.x;
+    r2bits[tid] = 0ull;
+    __syncthreads();
+    const int total = nb * vmax;
+    for (int idx = tid; idx < total; idx += VBLOCK) {
+        const int j = idx / vmax;
+        const int i = idx - j * vmax;
+        const VertJob& J = job[j];
+        if (i >= J.nrot) continue;
+        double* pv = verts + ((int64_t)vmax * rows[j] + i) * 3;
+        const q4 qr = J.qr;
+        const d3 r = qrot(qr, ld3(pv) - J.com, qconj(qr));
+        d3 nv = (r + J.com) + J.dx;
+        if (J.out) nv = nv + J.dx * -1.0;
+        st3(pv, nv);
+        if (radius) {
+            const double r2 = nrm2(nv - J.cnew);
+            if (r2 == r2) atomicMax(&r2bits[j], (unsigned long long)__double_as_longlong(r2));
+        }
+    }
+    __syncthreads();
+    if (radius && tid < nb) radius[rows[tid]] = inflate_radius(__longlong_as_double((long long)r2bits[tid]), job[tid].cnew);
 }
 
-// Force assembly fused with the integrator, over the moving-particle list.
+__global__ void __launch_bounds__(VBLOCK) k_verlet(VState a, const double* __restrict__ F,
+                                                   const double* __restrict__ tau, int vmax,
+                                                   const int32_t* __restrict__ nrot, double dt, Box6 box,
+                                                   int64_t pmin, int64_t pmax) {
+    __shared__ VertJob job[VBLOCK];
+    __shared__ int64_t rows[VBLOCK];
+    __shared__ unsigned long long r2bits[VBLOCK];
+    const int64_t first = pmin + (int64_t)blockIdx.x * VBLOCK;
+    const int64_t P = first + threadIdx.x;
+    const int nb = (int)(pmax - first < VBLOCK ? pmax - first : VBLOCK);
+    if (P < pmax) {
+        rows[threadIdx.x] = P;
+        verlet_row<double>(a, P, ld3(F + 3 * P), ld3(tau + 3 * P), vmax, nrot ? nrot[P] : vmax, dt, box.b, 0,
+                           &job[threadIdx.x]);
+    }
+    vertex_pass(a.verts.v, rows, job, r2bits, nb, vmax, nullptr);
+}
+
+// Record r of a particle: side A (r even) or B (r odd) of contact r >> 1.
 template <class T>
-__global__ void __launch_bounds__(256) k_reduce_verlet(VState a, const int32_t* __restrict__ plist, int64_t M,
-                                                       const int32_t* __restrict__ scene_of,
-                                                       const SceneDev* __restrict__ scenes,
-                                                       const int32_t* __restrict__ seg_begin,
-                                                       const int32_t* __restrict__ seg_end,
-                                                       const int32_t* __restrict__ rec_val, ContactOut co,
-                                                       const double* __restrict__ fext,
-                                                       const double* __restrict__ text,
-                                                       const int32_t* __restrict__ nverts, int vmax, double dt) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M) return;
+__device__ __forceinline__ void add_record(const ContactOut& co, int32_t r, V3<T>& F, V3<T>& Tq) {
+    const int64_t c = r >> 1;
+    if (r & 1) {
+        F = F + neg3(ldv<T>(co.Fn, c));
+        Tq = Tq + ldv<T>(co.tB, c);
+    } else {
+        F = F + ldv<T>(co.Fn, c);
+        Tq = Tq + ldv<T>(co.tA, c);
+    }
+}
+
+// Force assembly fused with the integrator for moving particle i.
+template <class T>
+__device__ __forceinline__ void reduce_verlet_row(const VState& a, const int32_t* __restrict__ plist, int64_t i,
+                                                  const int32_t* __restrict__ scene_of,
+                                                  const SceneDev* __restrict__ scenes,
+                                                  int32_t* __restrict__ rec_cnt,
+                                                  const int32_t* __restrict__ rec_bkt,
+                                                  const int32_t* __restrict__ rec_ovf, const int64_t* n_ovf,
+                                                  int64_t ovf_cap, const ContactOut& co,
+                                                  const double* __restrict__ fext, const double* __restrict__ text,
+                                                  const int32_t* __restrict__ nverts, int vmax, double dt,
+                                                  VertJob* job) {
     const int64_t P = plist[i];
     V3<T> F = zero3<T>(), Tq = zero3<T>();
-    const int32_t b = seg_begin[P], e = seg_end[P];
-    for (int32_t s = b; s < e; ++s) {
-        const int32_t r = rec_val[s];
-        const int64_t c = r >> 1;
-        if (r & 1) {
-            F = F + neg3(ldv<T>(co.Fn, c));
-            Tq = Tq + ldv<T>(co.tB, c);
+    // this step's records of the particle, summed in ascending contact order
+    // (the order of the candidate pairs); the bucket is emptied for the next step
+    const int n = rec_cnt[P];
+    if (n > 0) {
+        const int32_t* b = rec_bkt + P * MWD_ROW_CONTACTS;
+        if (n <= MWD_ROW_CONTACTS) {
+            int32_t r[MWD_ROW_CONTACTS];
+            for (int k = 0; k < n; ++k) {  // insertion sort; records are distinct
+                const int32_t x = b[k];
+                int j = k;
+                while (j > 0 && r[j - 1] > x) {
+                    r[j] = r[j - 1];
+                    --j;
+                }
+                r[j] = x;
+            }
+            for (int k = 0; k < n; ++k) add_record<T>(co, r[k], F, Tq);
         } else {
-            F = F + ldv<T>(co.Fn, c);
-            Tq = Tq + ldv<T>(co.tA, c);
+            // a full bucket plus the row's records in the overflow list, taken
+            // in ascending order by repeated selection (rare)
+            int64_t no = *n_ovf;
+            if (no > ovf_cap) no = ovf_cap;
+            int32_t prev = -1;
+            for (int t = 0; t < n; ++t) {
+                int32_t best = 0x7fffffff;
+                for (int k = 0; k < MWD_ROW_CONTACTS; ++k)
+                    if (b[k] > prev && b[k] < best) best = b[k];
+                for (int64_t o = 0; o < no; ++o)
+                    if (rec_ovf[2 * o] == (int32_t)P && rec_ovf[2 * o + 1] > prev && rec_ovf[2 * o + 1] < best)
+                        best = rec_ovf[2 * o + 1];
+                if (best == 0x7fffffff) break;  // lost to a full overflow list (flagged)
+                add_record<T>(co, best, F, Tq);
+                prev = best;
+            }
         }
+        rec_cnt[P] = 0;
     }
     F = F + mk3<T>(T(fext[3 * P]), T(fext[3 * P + 1]), T(fext[3 * P + 2]));
     Tq = Tq + mk3<T>(T(text[3 * P]), T(text[3 * P + 1]), T(text[3 * P + 2]));
     const SceneDev& sc = scenes[scene_of[P]];
-    verlet_row<T>(a, P, F, Tq, vmax, nverts[P], dt, sc.box, sc.law_p.mode == MWD_LAW_PHYSICAL ? 1 : 0);
+    verlet_row<T>(a, P, F, Tq, vmax, nverts[P], dt, sc.box, sc.law_p.mode == MWD_LAW_PHYSICAL ? 1 : 0, job);
+}
+
+// The double path: force assembly and integrator per lane, then the block's
+// vertex pass.
+__global__ void __launch_bounds__(VBLOCK) k_reduce_verlet_v(VState a, const int32_t* __restrict__ plist, int64_t M,
+                                                            const int32_t* __restrict__ scene_of,
+                                                            const SceneDev* __restrict__ scenes,
+                                                            int32_t* __restrict__ rec_cnt,
+                                                            const int32_t* __restrict__ rec_bkt,
+                                                            const int32_t* __restrict__ rec_ovf,
+                                                            const int64_t* n_ovf, int64_t ovf_cap, ContactOut co,
+                                                            const double* __restrict__ fext,
+                                                            const double* __restrict__ text,
+                                                            const int32_t* __restrict__ nverts, int vmax, double dt) {
+    __shared__ VertJob job[VBLOCK];
+    __shared__ int64_t rows[VBLOCK];
+    __shared__ unsigned long long r2bits[VBLOCK];
+    const int64_t first = (int64_t)blockIdx.x * VBLOCK;
+    const int nb = (int)(M - first < VBLOCK ? M - first : VBLOCK);
+    const int64_t i = first + threadIdx.x;
+    if (i < M) {
+        rows[threadIdx.x] = plist[i];
+        reduce_verlet_row<double>(a, plist, i, scene_of, scenes, rec_cnt, rec_bkt, rec_ovf, n_ovf, ovf_cap, co, fext,
+                                  text, nverts,
+                                  vmax, dt, &job[threadIdx.x]);
+    }
+    vertex_pass(a.verts.v, rows, job, r2bits, nb, vmax, a.radius);
+}
+
+// The tangent path: one lane per particle, vertices and their tangents
+// updated in the lane's own loop.
+__global__ void __launch_bounds__(256) k_reduce_verlet_t(VState a, const int32_t* __restrict__ plist, int64_t M,
+                                                         const int32_t* __restrict__ scene_of,
+                                                         const SceneDev* __restrict__ scenes,
+                                                         int32_t* __restrict__ rec_cnt,
+                                                         const int32_t* __restrict__ rec_bkt,
+                                                         const int32_t* __restrict__ rec_ovf, const int64_t* n_ovf,
+                                                         int64_t ovf_cap, ContactOut co,
+                                                         const double* __restrict__ fext,
+                                                         const double* __restrict__ text,
+                                                         const int32_t* __restrict__ nverts, int vmax, double dt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    reduce_verlet_row<Dual<MWD_NT>>(a, plist, i, scene_of, scenes, rec_cnt, rec_bkt, rec_ovf, n_ovf, ovf_cap, co,
+                                    fext, text,
+                                    nverts, vmax, dt, nullptr);
 }
 
 __global__ void k_radius(const double* __restrict__ com, const double* __restrict__ verts,
@@ -211,8 +362,8 @@ int mwd_launch_verlet(double* v, double* v12, double* w12bf, double* q12, double
     Box6 b;
     for (int k = 0; k < 6; ++k) b.b[k] = box[k];
     const int64_t n = pmax - pmin;
-    hipLaunchKernelGGL(k_verlet, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, F, tau, (int)vmax, nrot, dt,
-                       b, pmin, pmax);
+    hipLaunchKernelGGL(k_verlet, dim3((unsigned)((n + VBLOCK - 1) / VBLOCK)), dim3(VBLOCK), 0, s, a, F, tau,
+                       (int)vmax, nrot, dt, b, pmin, pmax);
     MWD_LAUNCH_CHECK();
     return MWD_OK;
 }
@@ -222,13 +373,15 @@ int mwd_launch_reduce_verlet(mwd_workspace* ws, mwd_state* st, double* const* ta
     const VState a = mwd_vstate(st, tan);
     const int64_t M = ws->n_moving;
     if (tan)
-        hipLaunchKernelGGL(k_reduce_verlet<Dual<MWD_NT>>, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, ws->stream,
-                           a, ws->plist, M, ws->scene_of, ws->scenes_dev, ws->seg_begin, ws->seg_end,
-                           ws->rec_val_sorted, co, st->fext, st->text, st->nverts, (int)st->vmax, ws->dt);
+        hipLaunchKernelGGL(k_reduce_verlet_t, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, ws->stream, a,
+                           ws->plist, M, ws->scene_of, ws->scenes_dev, ws->rec_cnt, ws->rec_bkt, ws->rec_ovf,
+                           ws->counters + 11, 2 * ws->contact_cap, co, st->fext, st->text, st->nverts, (int)st->vmax,
+                           ws->dt);
     else
-        hipLaunchKernelGGL(k_reduce_verlet<double>, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, ws->stream, a,
-                           ws->plist, M, ws->scene_of, ws->scenes_dev, ws->seg_begin, ws->seg_end, ws->rec_val_sorted,
-                           co, st->fext, st->text, st->nverts, (int)st->vmax, ws->dt);
+        MWD_LAUNCH_EV(mwd_kev(ws, MWD_KT_VERLET, 0), mwd_kev(ws, MWD_KT_VERLET, 1), k_reduce_verlet_v,
+                      dim3((unsigned)((M + VBLOCK - 1) / VBLOCK)), dim3(VBLOCK), 0, ws->stream, a, ws->plist, M,
+                      ws->scene_of, ws->scenes_dev, ws->rec_cnt, ws->rec_bkt, ws->rec_ovf, ws->counters + 11,
+                      2 * ws->contact_cap, co, st->fext, st->text, st->nverts, (int)st->vmax, ws->dt);
     MWD_LAUNCH_CHECK();
     return MWD_OK;
 }

@@ -463,12 +463,13 @@ def config_large(seed: int = 2, n: int = 100_000) -> SceneData:
     return make_scene(n, (200.0, 200.0, 400.0), seed, npts=(8, 20), name=f"C2-large-{n}")
 
 
-def config_pile(seed: int = 5, n: int = 4096, v_pour: float = 100.0, law: Law4 | None = None):
+def config_pile(seed: int = 5, n: int = 4096, v_pour: float = 1000.0, law: Law4 | None = None):
     """The differentiable form's pile: n hulls of 8-20 points on ellipsoids of
     semi-axes (1.0, 0.8, 0.6) x U(0.6, 0.9) mm, packed from the floor of a
     40x40x60 mm box (bounding spheres 2% apart; 4096 of the larger C2-sized
     hulls would not fit as a lattice) and poured, i.e. given random velocities
-    of rms v_pour mm/s, so contacts open and close through the run. Physical
+    of rms v_pour mm/s (1 m/s: a fall of about 5 cm), so contacts open and
+    close through the run. Physical
     law Kn=1e4, en=et=0.5, mu=0.3. Returns the scene and the initial
     velocities [n_rows, 3] (zero for walls)."""
     law = law or Law4(Kn=1e4, en=0.5, et=0.5, mu=0.3, mode=PHYSICAL)

@@ -199,6 +199,14 @@ class Stepper:
     def set_timing(self, on: bool = True):
         _lib.check(_lib.load().mwd_set_timing(self.ws, 1 if on else 0), "mwd_set_timing")
 
+    def kernel_times(self) -> dict:
+        """Accumulated kernel-only milliseconds of the single-kernel stages since
+        the last call (timing on; call before stage_times)."""
+        ms = (C.c_double * _lib.N_KTIMED)()
+        n = C.c_int64()
+        _lib.check(_lib.load().mwd_kernel_times(self.ws, ms, C.byref(n)), "mwd_kernel_times")
+        return dict(steps=n.value, ms=dict(zip(_lib.KTIMED_NAMES, list(ms))))
+
     def stage_times(self) -> dict:
         """Accumulated per-stage milliseconds since the last call (timing on)."""
         ms = (C.c_double * _lib.N_STAGES)()
@@ -250,7 +258,7 @@ class Stepper:
         st = _lib.Status()
         _lib.check(_lib.load().mwd_step_status(self.ws, C.byref(st)), "mwd_step_status")
         return dict(n_pairs=st.n_pairs, n_wall_pairs=st.n_wall_pairs, n_contacts=st.n_contacts,
-                    n_cells=st.n_cells, flags=st.flags)
+                    n_cells=st.n_cells, flags=st.flags, n_sphere_pass=st.n_sphere_pass)
 
     def check(self) -> dict:
         s = self.status()
@@ -263,6 +271,9 @@ class Stepper:
             raise _lib.MwdError(_lib.MWD_E_KEYRANGE, "step", "a cell key left its scene's key range")
         if f & _lib.F_CELL:
             raise _lib.MwdError(_lib.MWD_E_CELL, "step", f"a cell holds more than {_lib.MAX_CELL} particles")
+        if f & _lib.F_ROW_CONTACTS:
+            raise _lib.MwdError(_lib.MWD_E_CAPACITY, "step",
+                                "the contact-record overflow list is full; rerun with a larger contact_cap")
         return s
 
     # ---------------------------------------------------------------- views

@@ -148,6 +148,59 @@ def test_step_reduction_matches_oracle(gpu, dense_scene):
         assert max_rel(got[k][W:], ref[k][W:]) <= 1e-12, k
 
 
+def test_reduction_with_more_contacts_than_a_bucket(gpu):
+    """A particle with more contacts than its bucket holds (its extra records
+    go through the overflow list): the step still sums in ascending pair
+    order and matches the oracle."""
+    from mwdiem_amd import _lib, scene
+    base = scene.make_scene(300, (20.0, 20.0, 40.0), 13, law=scene.Law4(mode=scene.PHYSICAL), name="crowd")
+    base.inner = (10.0, 10.0, 40.0)  # packed into a column: many neighbours in 3-D
+    sc = scene.compact_dense(base, gap=-0.65)
+    W = sc.n_walls
+    st = O.initial_state(sc)
+    pairs = O.candidates(st, sc)
+    coll = O.contact_state(pairs, st, sc)[0]
+    cnt = np.bincount(pairs[coll == 1].ravel(), minlength=sc.n_rows)
+    r = W + int(np.argmax(cnt[W:]))
+    n = sc.nverts[r]
+    sc.verts[r, :n] = sc.com[r] + 2.0 * (sc.verts[r, :n] - sc.com[r])  # a large hull among small ones
+    st = O.initial_state(sc)
+    pairs = O.candidates(st, sc)
+    coll = O.contact_state(pairs, st, sc)[0]
+    cnt = np.bincount(pairs[coll == 1].ravel(), minlength=sc.n_rows)
+    assert cnt[r] > _lib.ROW_CONTACTS
+    S = _stepper(sc)
+    S.set_state(st)
+    ref = {k: v.copy() for k, v in st.items()}
+    O.step(ref, sc)
+    S.step(1)
+    S.check()
+    got = S.numpy_state()
+    for k in ("com", "v", "v12", "w", "wbf", "w12bf", "q", "q12", "qall", "dxall"):
+        assert max_rel(got[k][W:], ref[k][W:]) <= 1e-12, k
+
+
+def test_timing_mode_changes_nothing(gpu, dense_phys):
+    """With timing on, the single-kernel stages are launched with start/stop
+    events (hipExtLaunchKernelGGL): the results stay bit-identical and the
+    kernel times are positive."""
+    A = _stepper(dense_phys)
+    B = _stepper(dense_phys)
+    B.set_timing(True)
+    A.step(30)
+    B.step(30)
+    kt = B.kernel_times()
+    st = B.stage_times()
+    B.set_timing(False)
+    A.check()
+    B.check()
+    a, b = A.numpy_state(), B.numpy_state()
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    assert kt["steps"] == 30 and st["steps"] == 30
+    assert all(v > 0 for v in kt["ms"].values()), kt
+
+
 def _run_both(sc, st, steps, check_pairs=5):
     S = _stepper(sc)
     S.set_state(st)
