@@ -149,12 +149,15 @@ def bench_batched(n_scenes: int, steps: int, plain_steps: int, graph_steps: int)
             "plain_steps_per_scene": plain_steps, "plain_particle_steps_per_s": plain, "batched_vs_plain": batched / plain}
 
 
-def bench_diff(steps: int, graph_steps: int, n: int = 4096):
+def bench_diff(steps: int, graph_steps: int, n: int = 4096, short: int = 200):
     """The differentiable form on the poured pile (n hulls, 40x40x60 mm,
     physical law Kn=1e4, en=0.5, mu=0.3): cost of a tangent step (state plus
     its derivatives with respect to Kn, en, mu) against a plain step, both
-    graph-captured, and the bed outcome's gradient after `steps` steps
-    against central differences (h = 1e-4 relative)."""
+    graph-captured, and the bed outcome's gradient against central
+    differences after `short` and after `steps` steps. Over the long horizon
+    contacts open and close many times: the outcome is then only piecewise
+    smooth in the parameters, and differences across such events are not
+    derivatives (both step sizes are reported)."""
     import numpy as np
     import torch
 
@@ -177,27 +180,28 @@ def bench_diff(steps: int, graph_steps: int, n: int = 4096):
         torch.cuda.synchronize()
         times[mode] = (time.perf_counter() - t0) / steps
         st = S.check()
-        if mode == "tangent":
-            L, g = outcome_and_grad(S)
         S.close()
-    fds = {}
-    for hrel in (1e-4, 1e-7):
-        fd = np.zeros(3)
-        for k in range(3):
-            h = hrel * theta[k]
-            tp, tm = theta.copy(), theta.copy()
-            tp[k] += h
-            tm[k] -= h
-            fd[k] = (run_outcome(sc, v0, tp, steps, False, graph_steps)[0]
-                     - run_outcome(sc, v0, tm, steps, False, graph_steps)[0]) / (2 * h)
-        rel = np.abs(g - fd) / np.maximum(np.abs(fd), 1e-300)
-        fds[f"{hrel:g}"] = {"fd": [float(x) for x in fd], "rel": [float(x) for x in rel]}
-    print(f"# diff: tangent {1e3 * times['tangent']:.4f} ms/step vs plain {1e3 * times['plain']:.4f}; "
-          f"grad {g} fd {fds}", file=sys.stderr, flush=True)
+    checks = {}
+    for horizon, hs in ((short, (1e-4,)), (steps, (1e-4, 1e-7))):
+        L, g = run_outcome(sc, v0, theta, horizon, True, graph_steps)
+        out = {"outcome": L, "grad": [float(x) for x in g]}
+        for hrel in hs:
+            fd = np.zeros(3)
+            for k in range(3):
+                h = hrel * theta[k]
+                tp, tm = theta.copy(), theta.copy()
+                tp[k] += h
+                tm[k] -= h
+                fd[k] = (run_outcome(sc, v0, tp, horizon, False, graph_steps)[0]
+                         - run_outcome(sc, v0, tm, horizon, False, graph_steps)[0]) / (2 * h)
+            rel = np.abs(g - fd) / np.maximum(np.abs(fd), 1e-300)
+            out[f"fd_h{hrel:g}"] = {"fd": [float(x) for x in fd], "rel": [float(x) for x in rel]}
+        checks[f"{horizon}_steps"] = out
+    print(f"# diff: tangent {1e3 * times['tangent']:.4f} ms/step vs plain {1e3 * times['plain']:.4f}; {checks}",
+          file=sys.stderr, flush=True)
     return {"particles": n, "steps": steps, "ms_per_step_plain": 1e3 * times["plain"],
             "ms_per_step_tangent": 1e3 * times["tangent"], "tangent_vs_forward": times["tangent"] / times["plain"],
-            "outcome": L, "grad": [float(x) for x in g], "central_differences": fds,
-            "contacts_last_step": st["n_contacts"], "theta": ["Kn", "en", "mu"],
+            "gradient_checks": checks, "contacts_last_step": st["n_contacts"], "theta": ["Kn", "en", "mu"],
             "mode": "forward tangents (3 directions)"}
 
 

@@ -42,9 +42,9 @@ int dalloc(T** p, int64_t count) {
 void free_ws(mwd_workspace* ws) {
     if (!ws) return;
     void* ptrs[] = {ws->scenes_dev, ws->cell_of, ws->plist, ws->scene_of, ws->is_wall, ws->keys, ws->keys_alt, ws->vals,
-                    ws->vals_alt, ws->head, ws->head_off, ws->cell_start, ws->cell_key, ws->cell_nb, ws->row_first,
-                    ws->row_rank, ws->cell_by_rank, ws->cell_cnt, ws->cell_off, ws->perm, ws->wall_flag, ws->wall_off,
-                    ws->pairs, ws->collision, ws->simplex, ws->sidx, ws->pair_flag, ws->surv, ws->contact_off, ws->contact_pair,
+                    ws->vals_alt, ws->cell_start, ws->cell_key, ws->cell_nb, ws->row_first,
+                    ws->row_rank, ws->cell_by_rank, ws->cell_cnt, ws->cell_off, ws->perm,
+                    ws->pairs, ws->collision, ws->simplex, ws->sidx, ws->surv, ws->contact_pair,
                     ws->depth, ws->normal, ws->cpA, ws->cpB, ws->Fn, ws->tA, ws->tB, ws->epa_status, ws->rec_cnt,
                     ws->rec_bkt, ws->rec_ovf, ws->hist,
                     ws->hist_off, ws->scan_tmp, ws->counters, ws->flags, ws->laws, ws->scene_wt_off, ws->ctan,
@@ -351,8 +351,6 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     A(keys_alt, M);
     A(vals, M);
     A(vals_alt, M);
-    A(head, M);
-    A(head_off, M);
     A(cell_start, M + 1);
     A(cell_key, M);
     A(cell_nb, MWD_MAX_OFF * M);
@@ -365,15 +363,11 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     if (ws->key_span_total <= (int64_t(1) << 28)) {
         A(cell_of, ws->key_span_total);
     }
-    A(wall_flag, ws->n_wall_tests);
-    A(wall_off, ws->n_wall_tests);
     A(pairs, 2 * pair_cap);
     A(collision, pair_cap);
     A(simplex, contact_cap);
     A(sidx, pair_cap);
-    A(pair_flag, pair_cap);
     A(surv, pair_cap);
-    A(contact_off, pair_cap);
     A(contact_pair, contact_cap);
     A(depth, contact_cap);
     A(normal, 3 * contact_cap);
@@ -512,18 +506,17 @@ static int fine_contact(mwd_workspace* ws, const mwd_state* st, double* const* t
             return rc;
     }
     ws->fresh_pairs = 0;
-    rc = mwd_launch_sphere(A, B, ws->pair_cap, ctr + 3, ws->collision, ws->pair_flag, ws->surv, ctr + 10, s,
+    rc = mwd_launch_sphere(A, B, ws->pair_cap, ctr + 3, ws->collision, ws->surv, ctr + 10, s,
                            mwd_kev(ws, MWD_KT_SPHERE, 0), mwd_kev(ws, MWD_KT_SPHERE, 1));
     if (rc) return rc;
     mwd_mark(ws, 6);
-    rc = mwd_launch_gjk_surv(A, B, ws->pair_cap, ws->collision, ws->pair_flag, ws->surv, ctr + 10, ws->simplex,
+    rc = mwd_launch_gjk_surv(A, B, ws->pair_cap, ws->collision, ws->surv, ctr + 10, ws->simplex,
                              ws->sidx, ctr + 8, ws->contact_cap, ws->flags, s, mwd_kev(ws, MWD_KT_GJK, 0),
                              mwd_kev(ws, MWD_KT_GJK, 1));
     if (rc) return rc;
     mwd_mark(ws, 7);
-    rc = mwd_launch_compact_contacts(ws->collision, ws->pair_cap, ctr + 3, ws->pair_flag, ws->contact_off,
-                                     ws->contact_pair, ws->contact_cap, ctr + 4, ws->flags, ws->scan_tmp,
-                                     ws->scan_tmp_len, s);
+    rc = mwd_launch_compact_contacts(ws->collision, ws->pair_cap, ctr + 3, ws->contact_pair, ws->contact_cap, ctr + 4,
+                                     ws->flags, ws->scan_tmp, ws->scan_tmp_len, s);
     if (rc) return rc;
     mwd_mark(ws, 8);
     if (ws->bkt_dirty) {  // the previous contacts were not consumed by a reduction
@@ -724,11 +717,11 @@ MWD_API int mwd_contact_pairs(int64_t P, int32_t vmax, const double* comA, const
     }
     hipStream_t s = (hipStream_t)stream;
     // temporaries of this call
-    int32_t *coll = nullptr, *flag = nullptr, *cpair = nullptr, *est = nullptr, *rval = nullptr, *rval2 = nullptr;
+    int32_t *coll = nullptr, *cpair = nullptr, *est = nullptr, *rval = nullptr, *rval2 = nullptr;
     int32_t *sb = nullptr, *se = nullptr, *hist = nullptr, *sidx = nullptr, *surv = nullptr;
     uint8_t *wallA = nullptr, *wallB = nullptr;
     Simplex* sx = nullptr;
-    int64_t *off = nullptr, *ctr = nullptr, *hist_off = nullptr, *tmp = nullptr;
+    int64_t *ctr = nullptr, *hist_off = nullptr, *tmp = nullptr;
     double *dep = nullptr, *nrm = nullptr, *ca = nullptr, *cb = nullptr, *fn = nullptr, *ta = nullptr, *tb = nullptr;
     uint32_t *rkey = nullptr, *rkey2 = nullptr, *flags = nullptr;
     mwd_law* laws = nullptr;
@@ -744,7 +737,6 @@ MWD_API int mwd_contact_pairs(int64_t P, int32_t vmax, const double* comA, const
         if (!rc) owned.push_back((void*)*p);
     };
     al(&coll, P);
-    al(&flag, P);
     al(&cpair, P);
     al(&est, P);
     al(&rval, rec);
@@ -756,7 +748,6 @@ MWD_API int mwd_contact_pairs(int64_t P, int32_t vmax, const double* comA, const
     al(&sx, P);
     al(&sidx, P);
     al(&surv, P);
-    al(&off, P);
     al(&ctr, 16);
     al(&hist, hist_len);
     al(&hist_off, hist_len);
@@ -808,10 +799,8 @@ MWD_API int mwd_contact_pairs(int64_t P, int32_t vmax, const double* comA, const
         BodyView B{comB, vertsB, nB, vB, wB, qA, isB, nullptr, nullptr, wallB, nullptr, nullptr, 1, vmax};
         if ((rc = mwd_set_i64(ctr + 3, P, s))) break;
         if ((rc = mwd_launch_zero_pairs(P, depth, normal, cpA, cpB, epa_status, disp, s))) break;
-        if ((rc = mwd_launch_gjk(A, B, P, ctr + 3, coll, flag, surv, ctr + 10, sx, sidx, ctr + 8, P, flags, s))) break;
-        if ((rc = mwd_launch_compact_contacts(coll, P, ctr + 3, flag, off, cpair, P, ctr + 4, flags, tmp, tmp_len,
-                                              s)))
-            break;
+        if ((rc = mwd_launch_gjk(A, B, P, ctr + 3, coll, surv, ctr + 10, sx, sidx, ctr + 8, P, flags, s))) break;
+        if ((rc = mwd_launch_compact_contacts(coll, P, ctr + 3, cpair, P, ctr + 4, flags, tmp, tmp_len, s))) break;
         if ((rc = mwd_launch_epa_force(A, B, laws, cpair, sx, sidx, P, ctr + 4, dispt, idA, idB, dep, nrm, ca, cb, fn, ta,
                                        tb, disp, est, rkey, rval, ctr + 5, flags, s)))
             break;

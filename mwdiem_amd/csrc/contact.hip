@@ -20,6 +20,7 @@
 #include <type_traits>
 
 #include "mwd_internal.h"
+#include "mwd_scan.h"
 
 namespace {
 
@@ -258,7 +259,7 @@ __device__ __forceinline__ bool correct_flat(d3 S1, d3 S2, d3 S3, d3 S4) {
 // GJK of one pair (.cu:1009-1288). Returns -1 (tetrahedron cap), 0 or 1; on
 // 1 the simplex handed to EPA (after CorrectFlat) is in S/ia/ib.
 template <class Sup>
-__device__ int gjk_pair(const Sup& support, d3 comA, d3 comB, d3* S, int* ia, int* ib) {
+__device__ __forceinline__ int gjk_pair(const Sup& support, d3 comA, d3 comB, d3* S, int* ia, int* ib) {
     const d3 d1 = comB - comA;
     S[0] = support(d1, &ia[0], &ib[0]);
     if (dot3(d1, S[0]) < 0.0) return 0;
@@ -318,8 +319,8 @@ __device__ int gjk_pair(const Sup& support, d3 comA, d3 comB, d3* S, int* ia, in
 // are appended to surv with one atomic per wave, so GJK then runs on full
 // waves of live pairs instead of waves that mostly idle.
 __global__ void __launch_bounds__(256) k_sphere(BodyView A, BodyView B, int64_t n_cap, const int64_t* n_dev,
-                                                int32_t* __restrict__ collision, int32_t* __restrict__ flag,
-                                                int32_t* __restrict__ surv, int64_t* n_surv) {
+                                                int32_t* __restrict__ collision, int32_t* __restrict__ surv,
+                                                int64_t* n_surv) {
     int64_t n = n_dev ? *n_dev : n_cap;
     if (n > n_cap) n = n_cap;
     const int lane = threadIdx.x & (MWD_WAVE - 1);
@@ -337,10 +338,7 @@ __global__ void __launch_bounds__(256) k_sphere(BodyView A, BodyView B, int64_t 
                 const double rr = A.radius[ra] + B.radius[rb];
                 keep = !(nrm2(cb - ca) > rr * rr);
             }
-            if (!keep) {
-                collision[p] = 0;
-                flag[p] = 0;
-            }
+            if (!keep) collision[p] = 0;
         }
         const uint64_t m = __ballot(keep);
         if (m) {
@@ -360,7 +358,7 @@ __global__ void __launch_bounds__(256) k_sphere(BodyView A, BodyView B, int64_t 
 template <int G, int K>
 __global__ void __launch_bounds__(256) k_gjk(BodyView A, BodyView B, int64_t n_cap, const int32_t* __restrict__ surv,
                                              const int64_t* n_surv, int32_t* __restrict__ collision,
-                                             int32_t* __restrict__ flag, Simplex* __restrict__ simplex,
+                                             Simplex* __restrict__ simplex,
                                              int32_t* __restrict__ sidx, int64_t* slots, int64_t slot_cap,
                                              uint32_t* flags) {
     int64_t m = *n_surv;
@@ -387,7 +385,6 @@ __global__ void __launch_bounds__(256) k_gjk(BodyView A, BodyView B, int64_t n_c
         }
         if (g != 0) continue;
         collision[p] = c;
-        flag[p] = c == 1 ? 1 : 0;
         if (c == 1) {
             const int64_t slot = atomicAdd((unsigned long long*)slots, 1ull);
             sidx[p] = (int32_t)slot;
@@ -408,23 +405,21 @@ __global__ void __launch_bounds__(256) k_gjk(BodyView A, BodyView B, int64_t n_c
     }
 }
 
-__global__ void k_contact_scatter(const int32_t* __restrict__ collision, const int64_t* __restrict__ off,
-                                  int64_t n_cap, const int64_t* n_dev, int32_t* __restrict__ contact_pair,
-                                  int64_t contact_cap, int64_t* n_contacts, uint32_t* flags) {
-    int64_t n = n_dev ? *n_dev : n_cap;
-    if (n > n_cap) n = n_cap;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid == 0 && *n_contacts > contact_cap) {
-        atomicOr(flags, MWD_F_CONTACT_OVERFLOW);
+// Contact compaction: contact c is the c-th pair with collision == 1, in pair
+// order; a count beyond contact_cap is flagged.
+struct ContactIO {
+    const int32_t* collision;
+    int32_t* contact_pair;
+    int64_t cap;
+    uint32_t* flags;
+    __device__ __forceinline__ int32_t load(int64_t p) const { return collision[p] == 1 ? 1 : 0; }
+    __device__ __forceinline__ void store(int64_t p, int64_t c, int32_t v) const {
+        if (v && c < cap) contact_pair[c] = (int32_t)p;
     }
-    for (int64_t p = gid; p < n; p += stride) {
-        if (collision[p] == 1) {
-            const int64_t c = off[p];
-            if (c < contact_cap) contact_pair[c] = (int32_t)p;
-        }
+    __device__ __forceinline__ void finish(int64_t total) const {
+        if (total > cap) atomicOr(flags, MWD_F_CONTACT_OVERFLOW);
     }
-}
+};
 
 // ------------------------------------------------------------------- EPA
 struct Poly {
@@ -956,23 +951,23 @@ unsigned grid_for(int64_t n, int threads, int64_t max_blocks) {
 }  // namespace
 
 int mwd_launch_sphere(const BodyView& A, const BodyView& B, int64_t n_cap, const int64_t* n_dev, int32_t* collision,
-                      int32_t* flag, int32_t* surv, int64_t* n_surv, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+                      int32_t* surv, int64_t* n_surv, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     if (n_cap <= 0) return MWD_OK;
     MWD_LAUNCH_EV(e0, e1, k_sphere, dim3(grid_for(n_cap, 256, 8192)), dim3(256), 0, s, A, B, n_cap, n_dev, collision,
-                  flag, surv, n_surv);
+                  surv, n_surv);
     MWD_LAUNCH_CHECK();
     return MWD_OK;
 }
 
 int mwd_launch_gjk(const BodyView& A, const BodyView& B, int64_t n_cap, const int64_t* n_dev, int32_t* collision,
-                   int32_t* flag, int32_t* surv, int64_t* n_surv, Simplex* simplex, int32_t* sidx, int64_t* slots,
-                   int64_t slot_cap, uint32_t* flags, hipStream_t s) {
-    const int rc = mwd_launch_sphere(A, B, n_cap, n_dev, collision, flag, surv, n_surv, s);
+                   int32_t* surv, int64_t* n_surv, Simplex* simplex, int32_t* sidx, int64_t* slots, int64_t slot_cap,
+                   uint32_t* flags, hipStream_t s) {
+    const int rc = mwd_launch_sphere(A, B, n_cap, n_dev, collision, surv, n_surv, s);
     if (rc) return rc;
-    return mwd_launch_gjk_surv(A, B, n_cap, collision, flag, surv, n_surv, simplex, sidx, slots, slot_cap, flags, s);
+    return mwd_launch_gjk_surv(A, B, n_cap, collision, surv, n_surv, simplex, sidx, slots, slot_cap, flags, s);
 }
 
-int mwd_launch_gjk_surv(const BodyView& A, const BodyView& B, int64_t n_cap, int32_t* collision, int32_t* flag,
+int mwd_launch_gjk_surv(const BodyView& A, const BodyView& B, int64_t n_cap, int32_t* collision,
                         const int32_t* surv, const int64_t* n_surv, Simplex* simplex, int32_t* sidx, int64_t* slots,
                         int64_t slot_cap, uint32_t* flags, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     if (n_cap <= 0) return MWD_OK;
@@ -981,42 +976,34 @@ int mwd_launch_gjk_surv(const BodyView& A, const BodyView& B, int64_t n_cap, int
     const dim3 grid(grid_for(n_cap * (K <= 4 ? G : 1), 256, 4096)), blk(256);
     switch (K <= 4 ? K : 0) {
         case 1:
-            MWD_LAUNCH_EV(e0, e1, (k_gjk<G, 1>), grid, blk, 0, s, A, B, n_cap, surv, n_surv, collision, flag, simplex,
+            MWD_LAUNCH_EV(e0, e1, (k_gjk<G, 1>), grid, blk, 0, s, A, B, n_cap, surv, n_surv, collision, simplex,
                                sidx, slots, slot_cap, flags);
             break;
         case 2:
-            MWD_LAUNCH_EV(e0, e1, (k_gjk<G, 2>), grid, blk, 0, s, A, B, n_cap, surv, n_surv, collision, flag, simplex,
+            MWD_LAUNCH_EV(e0, e1, (k_gjk<G, 2>), grid, blk, 0, s, A, B, n_cap, surv, n_surv, collision, simplex,
                                sidx, slots, slot_cap, flags);
             break;
         case 3:
-            MWD_LAUNCH_EV(e0, e1, (k_gjk<G, 3>), grid, blk, 0, s, A, B, n_cap, surv, n_surv, collision, flag, simplex,
+            MWD_LAUNCH_EV(e0, e1, (k_gjk<G, 3>), grid, blk, 0, s, A, B, n_cap, surv, n_surv, collision, simplex,
                                sidx, slots, slot_cap, flags);
             break;
         case 4:
-            MWD_LAUNCH_EV(e0, e1, (k_gjk<G, 4>), grid, blk, 0, s, A, B, n_cap, surv, n_surv, collision, flag, simplex,
+            MWD_LAUNCH_EV(e0, e1, (k_gjk<G, 4>), grid, blk, 0, s, A, B, n_cap, surv, n_surv, collision, simplex,
                                sidx, slots, slot_cap, flags);
             break;
         default:
-            MWD_LAUNCH_EV(e0, e1, (k_gjk<G, 0>), grid, blk, 0, s, A, B, n_cap, surv, n_surv, collision, flag, simplex,
+            MWD_LAUNCH_EV(e0, e1, (k_gjk<G, 0>), grid, blk, 0, s, A, B, n_cap, surv, n_surv, collision, simplex,
                                sidx, slots, slot_cap, flags);
     }
     MWD_LAUNCH_CHECK();
     return MWD_OK;
 }
 
-int mwd_launch_compact_contacts(const int32_t* collision, int64_t n_cap, const int64_t* n_dev, const int32_t* flag,
-                                int64_t* off, int32_t* contact_pair, int64_t contact_cap, int64_t* n_contacts,
-                                uint32_t* flags, int64_t* tmp, int64_t tmp_len, hipStream_t s) {
-    if (n_cap <= 0) {
-        MWD_HIP_CHECK(hipMemsetAsync(n_contacts, 0, sizeof(int64_t), s));
-        return MWD_OK;
-    }
-    int rc = mwd_scan_i32(flag, off, n_contacts, n_cap, n_dev, tmp, tmp_len, s);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_contact_scatter, dim3(grid_for(n_cap, 256, 8192)), dim3(256), 0, s, collision, off, n_cap,
-                       n_dev, contact_pair, contact_cap, n_contacts, flags);
-    MWD_LAUNCH_CHECK();
-    return MWD_OK;
+int mwd_launch_compact_contacts(const int32_t* collision, int64_t n_cap, const int64_t* n_dev, int32_t* contact_pair,
+                                int64_t contact_cap, int64_t* n_contacts, uint32_t* flags, int64_t* tmp,
+                                int64_t tmp_len, hipStream_t s) {
+    return mwd_scan_io(ContactIO{collision, contact_pair, contact_cap, flags}, n_cap, n_dev, n_contacts, tmp, tmp_len,
+                       s);
 }
 
 static TArr<MWD_NT> plain(double* v) {

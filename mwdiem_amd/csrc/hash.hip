@@ -21,6 +21,7 @@
 // counts in rank order -> emission. Wall tests are a flag/scan/scatter over
 // the (scene, wall, particle) test space.
 #include "mwd_internal.h"
+#include "mwd_scan.h"
 
 namespace {
 
@@ -56,36 +57,6 @@ __global__ void k_sort_keys(const double* __restrict__ com, const int64_t* __res
     }
     keys[i] = (uint32_t)(sc.kbase + loc);
     vals[i] = row;
-}
-
-__global__ void k_heads(const uint32_t* __restrict__ keys, int64_t M, int32_t* __restrict__ head) {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= M) return;
-    head[s] = (s == 0 || keys[s] != keys[s - 1]) ? 1 : 0;
-}
-
-// Forget the previous step's cells in the key -> cell table.
-__global__ void k_clear_cells(const uint32_t* __restrict__ cell_key, const int64_t* __restrict__ n_prev,
-                              int32_t* __restrict__ cell_of) {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < *n_prev) cell_of[cell_key[c]] = -1;
-}
-
-__global__ void k_cells(const uint32_t* __restrict__ keys, const int32_t* __restrict__ vals, int64_t M,
-                        const int32_t* __restrict__ head, const int64_t* __restrict__ head_off,
-                        const int64_t* __restrict__ n_cells, int32_t* __restrict__ cell_start,
-                        uint32_t* __restrict__ cell_key, int32_t* __restrict__ row_first,
-                        int32_t* __restrict__ cell_of) {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= M) return;
-    if (head[s]) {
-        const int64_t c = head_off[s];
-        cell_start[c] = (int32_t)s;
-        cell_key[c] = keys[s];
-        row_first[vals[s]] = 1;
-        if (cell_of) cell_of[keys[s]] = (int32_t)c;
-    }
-    if (s == M - 1) cell_start[*n_cells] = (int32_t)M;
 }
 
 // CPython set iteration order of non-negative ints inserted in the given
@@ -318,36 +289,102 @@ __device__ __forceinline__ int scene_of_test(const int64_t* wt_off, int S, int64
     return lo;
 }
 
-__global__ void k_wall_flags(const double* __restrict__ com, const double* __restrict__ wall_n,
-                             const double* __restrict__ wall_c, const SceneDev* __restrict__ scenes, int S,
-                             const int64_t* __restrict__ wt_off, int64_t T, int32_t* __restrict__ flag) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= T) return;
-    const int s = scene_of_test(wt_off, S, t);
-    const SceneDev& sc = scenes[s];
-    const int64_t np = sc.n_active - sc.n_walls;
-    const int64_t tl = t - wt_off[s];
-    const int64_t k = tl / np, j = tl - k * np;
-    const int64_t rk = sc.row0 + k, rj = sc.row0 + sc.n_walls + j;
-    const d3 d = ld3(com + 3 * rj) - ld3(wall_c + 3 * rk);
-    const double dd = dot3(ld3(wall_n + 3 * rk), d);
-    flag[t] = fabs(dd) < sc.wall_thr ? 1 : 0;
-}
+// Wall slab test (MWDiem.py:807-811) fused with its compaction: test t of a
+// scene pairs wall k with particle j (walls outer, particles inner); a pair
+// whose centre lies within wall_thr of the wall's plane is written, as
+// [j, k], at its offset among the scene-ordered hits.
+struct WallIO {
+    const double* com;
+    const double* wall_n;
+    const double* wall_c;
+    const SceneDev* scenes;
+    int S;
+    const int64_t* wt_off;
+    int32_t* pairs;
+    int64_t cap;
+    __device__ __forceinline__ void decode(int64_t t, const SceneDev** sc, int64_t* rj, int64_t* rk) const {
+        const int s = scene_of_test(wt_off, S, t);
+        *sc = &scenes[s];
+        const int64_t np = (*sc)->n_active - (*sc)->n_walls;
+        const int64_t tl = t - wt_off[s];
+        const int64_t k = tl / np, j = tl - k * np;
+        *rk = (*sc)->row0 + k;
+        *rj = (*sc)->row0 + (*sc)->n_walls + j;
+    }
+    __device__ __forceinline__ int32_t load(int64_t t) const {
+        const SceneDev* sc;
+        int64_t rj, rk;
+        decode(t, &sc, &rj, &rk);
+        const d3 d = ld3(com + 3 * rj) - ld3(wall_c + 3 * rk);
+        const double dd = dot3(ld3(wall_n + 3 * rk), d);
+        return fabs(dd) < sc->wall_thr ? 1 : 0;
+    }
+    __device__ __forceinline__ void store(int64_t t, int64_t pos, int32_t v) const {
+        if (!v || pos >= cap) return;
+        const SceneDev* sc;
+        int64_t rj, rk;
+        decode(t, &sc, &rj, &rk);
+        pairs[2 * pos] = (int32_t)rj;
+        pairs[2 * pos + 1] = (int32_t)rk;
+    }
+    __device__ __forceinline__ void finish(int64_t) const {}
+};
 
-__global__ void k_wall_fill(const int32_t* __restrict__ flag, const int64_t* __restrict__ off,
-                            const SceneDev* __restrict__ scenes, int S, const int64_t* __restrict__ wt_off,
-                            int64_t T, int32_t* __restrict__ pairs, int64_t cap) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= T || !flag[t]) return;
-    const int64_t pos = off[t];
-    if (pos >= cap) return;
-    const int s = scene_of_test(wt_off, S, t);
-    const SceneDev& sc = scenes[s];
-    const int64_t np = sc.n_active - sc.n_walls;
-    const int64_t tl = t - wt_off[s];
-    const int64_t k = tl / np, j = tl - k * np;
-    pairs[2 * pos] = (int32_t)(sc.row0 + sc.n_walls + j);
-    pairs[2 * pos + 1] = (int32_t)(sc.row0 + k);
+// Cell heads over the sorted keys fused with their numbering: item s starts
+// a cell when its key differs from the previous one; the cell's index is
+// its offset. Writes the cell's start, key, the first row's flag and the
+// key -> cell table; the last entry of cell_start closes the list.
+struct HeadIO {
+    const uint32_t* keys;
+    const int32_t* vals;
+    int64_t M;
+    int32_t* cell_start;
+    uint32_t* cell_key;
+    int32_t* row_first;
+    int32_t* cell_of;
+    __device__ __forceinline__ int32_t load(int64_t s) const {
+        return (s == 0 || keys[s] != keys[s - 1]) ? 1 : 0;
+    }
+    __device__ __forceinline__ void store(int64_t s, int64_t c, int32_t v) const {
+        if (!v) return;
+        cell_start[c] = (int32_t)s;
+        cell_key[c] = keys[s];
+        row_first[vals[s]] = 1;
+        if (cell_of) cell_of[keys[s]] = (int32_t)c;
+    }
+    __device__ __forceinline__ void finish(int64_t n_cells) const { cell_start[n_cells] = (int32_t)M; }
+};
+
+// Scan of the per-cell pair counts; its total closes the candidate count
+// (counters 1 wall pairs + 2 cell pairs -> 3 capped, 7 uncapped) and
+// restarts the counters of the fine-contact stage that consumes the pairs.
+struct CellCountIO {
+    const int32_t* cnt;
+    int64_t* off;
+    int64_t* counters;
+    int64_t cap;
+    uint32_t* flags;
+    __device__ __forceinline__ int32_t load(int64_t i) const { return cnt[i]; }
+    __device__ __forceinline__ void store(int64_t i, int64_t o, int32_t) const { off[i] = o; }
+    __device__ __forceinline__ void finish(int64_t total) const {
+        counters[6] = counters[0];  // cells of this step, cleared from the key table next step
+        const int64_t P = counters[1] + total;
+        counters[3] = P < cap ? P : cap;
+        counters[7] = P;
+        counters[8] = 0;   // simplex slots of the fine-contact stage
+        counters[10] = 0;  // its sphere-test survivors
+        counters[11] = 0;  // and its bucket overflow records
+        if (P > cap) atomicOr(flags, MWD_F_PAIR_OVERFLOW);
+    }
+};
+
+// Before the cells of this step: clear the row flags and forget the previous
+// step's cells in the key -> cell table.
+__global__ void k_cells_prep(int32_t* __restrict__ row_first, int64_t n_rows, const uint32_t* __restrict__ cell_key,
+                             const int64_t* __restrict__ n_prev, int32_t* __restrict__ cell_of) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_rows) row_first[i] = 0;
+    if (cell_of && i < *n_prev) cell_of[cell_key[i]] = -1;
 }
 
 __global__ void k_finish_pairs(int64_t* counters, int64_t cap, uint32_t* flags) {
@@ -386,21 +423,16 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
     // counters 0..3 = n_cells, n_wall_pairs, n_cell_pairs, n_pairs
     mwd_mark(ws, 0);
     const bool walls = ws->n_wall_tests > 0 && st->wall_n && st->wall_c;
-    if (!walls || M <= 0) {  // otherwise the scans below write counters 0..2 and k_finish_pairs counter 3
+    if (!walls || M <= 0) {  // otherwise the scans below write counters 0..3
         hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(64), 0, s, ctr, 4);
         MWD_LAUNCH_CHECK();
     }
-    // -- wall slab pairs
+    // -- wall slab pairs: test, count and write in one pass
     if (walls) {
-        const int64_t T = ws->n_wall_tests;
-        hipLaunchKernelGGL(k_wall_flags, dim3(blocks_for(T)), dim3(256), 0, s, st->com, st->wall_n, st->wall_c,
-                           ws->scenes_dev, ws->n_scenes, ws->scene_wt_off, T, ws->wall_flag);
-        MWD_LAUNCH_CHECK();
-        rc = mwd_scan_i32(ws->wall_flag, ws->wall_off, ctr + 1, T, nullptr, ws->scan_tmp, ws->scan_tmp_len, s);
+        const WallIO io{st->com, st->wall_n, st->wall_c, ws->scenes_dev, ws->n_scenes, ws->scene_wt_off,
+                        ws->pairs, ws->pair_cap};
+        rc = mwd_scan_io(io, ws->n_wall_tests, nullptr, ctr + 1, ws->scan_tmp, ws->scan_tmp_len, s);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_wall_fill, dim3(blocks_for(T)), dim3(256), 0, s, ws->wall_flag, ws->wall_off,
-                           ws->scenes_dev, ws->n_scenes, ws->scene_wt_off, T, ws->pairs, ws->pair_cap);
-        MWD_LAUNCH_CHECK();
     }
     // -- linked-cell pairs
     if (M > 0) {
@@ -416,20 +448,13 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
         ws->keys_sorted = ks;
         ws->vals_sorted = vs;
         mwd_mark(ws, 2);
-        hipLaunchKernelGGL(k_heads, dim3(blocks_for(M)), dim3(256), 0, s, ks, M, ws->head);
+        const int64_t prep = ws->n_rows > M ? ws->n_rows : M;
+        hipLaunchKernelGGL(k_cells_prep, dim3(blocks_for(prep)), dim3(256), 0, s, ws->row_first, ws->n_rows,
+                           ws->cell_key, ctr + 6, ws->cell_of);
         MWD_LAUNCH_CHECK();
-        rc = mwd_scan_i32(ws->head, ws->head_off, ctr + 0, M, nullptr, ws->scan_tmp, ws->scan_tmp_len, s);
+        rc = mwd_scan_io(HeadIO{ks, vs, M, ws->cell_start, ws->cell_key, ws->row_first, ws->cell_of}, M, nullptr,
+                         ctr + 0, ws->scan_tmp, ws->scan_tmp_len, s);
         if (rc) return rc;
-        rc = mwd_fill_i32(ws->row_first, 0, ws->n_rows, s);
-        if (rc) return rc;
-        if (ws->cell_of) {
-            hipLaunchKernelGGL(k_clear_cells, dim3(blocks_for(M)), dim3(256), 0, s, ws->cell_key, ctr + 6,
-                               ws->cell_of);
-            MWD_LAUNCH_CHECK();
-        }
-        hipLaunchKernelGGL(k_cells, dim3(blocks_for(M)), dim3(256), 0, s, ks, vs, M, ws->head,
-                           ws->head_off, ctr + 0, ws->cell_start, ws->cell_key, ws->row_first, ws->cell_of);
-        MWD_LAUNCH_CHECK();
         rc = mwd_scan_i32(ws->row_first, ws->row_rank, nullptr, ws->n_rows, nullptr, ws->scan_tmp,
                           ws->scan_tmp_len, s);
         if (rc) return rc;
@@ -437,7 +462,8 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
                            ws->cell_key, ctr + 0, M, ws->scene_of, ws->scenes_dev, ws->row_rank, ws->perm,
                            ws->cell_nb, ws->cell_by_rank, ws->cell_cnt, ws->flags, ws->cell_of);
         MWD_LAUNCH_CHECK();
-        rc = mwd_scan_i32(ws->cell_cnt, ws->cell_off, ctr + 2, M, ctr + 0, ws->scan_tmp, ws->scan_tmp_len, s);
+        rc = mwd_scan_io(CellCountIO{ws->cell_cnt, ws->cell_off, ctr, ws->pair_cap, ws->flags}, M, ctr + 0,
+                         ctr + 2, ws->scan_tmp, ws->scan_tmp_len, s);
         if (rc) return rc;
         mwd_mark(ws, 3);
         MWD_LAUNCH_EV(mwd_kev(ws, MWD_KT_EMIT, 0), mwd_kev(ws, MWD_KT_EMIT, 1), k_cell_fill,
@@ -446,7 +472,9 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
         MWD_LAUNCH_CHECK();
     }
     mwd_mark(ws, 4);
-    hipLaunchKernelGGL(k_finish_pairs, dim3(1), dim3(1), 0, s, ctr, ws->pair_cap, ws->flags);
-    MWD_LAUNCH_CHECK();
+    if (M <= 0) {  // no cell-count scan closed the candidate count
+        hipLaunchKernelGGL(k_finish_pairs, dim3(1), dim3(1), 0, s, ctr, ws->pair_cap, ws->flags);
+        MWD_LAUNCH_CHECK();
+    }
     return MWD_OK;
 }

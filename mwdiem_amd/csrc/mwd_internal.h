@@ -120,8 +120,6 @@ struct mwd_workspace {
     // coarse stage
     uint32_t *keys, *keys_alt; // [M]
     int32_t *vals, *vals_alt;  // [M]
-    int32_t* head;             // [M] head flags, later set-ordered members
-    int64_t* head_off;         // [M]
     int32_t* cell_start;       // [M+1]
     uint32_t* cell_key;        // [M]
     int32_t* cell_nb;          // [M*MWD_MAX_OFF]
@@ -132,17 +130,13 @@ struct mwd_workspace {
     int64_t* cell_off;         // [M]
     int32_t* perm;             // [M] members of each cell in set order
     int32_t* cell_of;          // [key_span_total] cell index of a sorted key, -1 if empty (NULL: binary search)
-    int32_t* wall_flag;        // [n_wall_tests]
-    int64_t* wall_off;         // [n_wall_tests]
     // candidate pairs
     int32_t* pairs;            // [pair_cap*2]
     int32_t* collision;        // [pair_cap]
     Simplex* simplex;          // [contact_cap], slot claimed by GJK per colliding pair
     int32_t* sidx;             // [pair_cap] simplex slot of a colliding pair
-    int32_t* pair_flag;        // [pair_cap]
     int32_t* surv;             // [pair_cap] pairs whose bounding spheres meet (any order)
     int fresh_pairs;           // host: candidates not yet consumed by a fine-contact stage
-    int64_t* contact_off;      // [pair_cap]
     // contacts
     int32_t* contact_pair;     // [contact_cap]
     double *depth, *normal, *cpA, *cpB, *Fn, *tA, *tB;
@@ -236,27 +230,26 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
 int mwd_launch_cell_keys(const double* com, int64_t n, const double* cell, int64_t ncx, int64_t ncy,
                          int64_t* keys, hipStream_t s);
 // GJK over pairs [0, n) (n from n_dev), in two launches: a bounding-sphere
-// test writes collision = 0 and flag = 0 for pairs whose spheres are apart
-// and appends the others to surv (counter *n_surv, zero on entry); GJK then
-// runs over the survivors only, writing collision and flag (collision == 1).
-// A colliding pair claims a simplex slot (counter *slots, zero on entry) and
-// stores it in sidx.
+// test writes collision = 0 for pairs whose spheres are apart and appends
+// the others to surv (counter *n_surv, zero on entry); GJK then runs over the
+// survivors only, writing their collision codes. A colliding pair claims a
+// simplex slot (counter *slots, zero on entry) and stores it in sidx.
 int mwd_launch_sphere(const BodyView& A, const BodyView& B, int64_t n_cap, const int64_t* n_dev, int32_t* collision,
-                      int32_t* flag, int32_t* surv, int64_t* n_surv, hipStream_t s, hipEvent_t e0 = nullptr,
+                      int32_t* surv, int64_t* n_surv, hipStream_t s, hipEvent_t e0 = nullptr,
                       hipEvent_t e1 = nullptr);
-int mwd_launch_gjk_surv(const BodyView& A, const BodyView& B, int64_t n_cap, int32_t* collision, int32_t* flag,
+int mwd_launch_gjk_surv(const BodyView& A, const BodyView& B, int64_t n_cap, int32_t* collision,
                         const int32_t* surv, const int64_t* n_surv, Simplex* simplex, int32_t* sidx, int64_t* slots,
                         int64_t slot_cap, uint32_t* flags, hipStream_t s, hipEvent_t e0 = nullptr,
                         hipEvent_t e1 = nullptr);
 // Both of the above.
 int mwd_launch_gjk(const BodyView& A, const BodyView& B, int64_t n_cap, const int64_t* n_dev,
-                   int32_t* collision, int32_t* flag, int32_t* surv, int64_t* n_surv, Simplex* simplex,
+                   int32_t* collision, int32_t* surv, int64_t* n_surv, Simplex* simplex,
                    int32_t* sidx, int64_t* slots, int64_t slot_cap, uint32_t* flags, hipStream_t s);
-// Contact compaction from the flags: contact_pair[c] = pair index of the
+// Contact compaction (one fused scan): contact_pair[c] = pair index of the
 // c-th collision == 1.
-int mwd_launch_compact_contacts(const int32_t* collision, int64_t n_cap, const int64_t* n_dev, const int32_t* flag,
-                                int64_t* off, int32_t* contact_pair, int64_t contact_cap, int64_t* n_contacts,
-                                uint32_t* flags, int64_t* tmp, int64_t tmp_len, hipStream_t s);
+int mwd_launch_compact_contacts(const int32_t* collision, int64_t n_cap, const int64_t* n_dev, int32_t* contact_pair,
+                                int64_t contact_cap, int64_t* n_contacts, uint32_t* flags, int64_t* tmp,
+                                int64_t tmp_len, hipStream_t s);
 // EPA + force law over contacts [0, K). Reduction records are keyed by
 // idA[p]/idB[p] when given, else by the view rows.
 int mwd_launch_epa_force(const BodyView& A, const BodyView& B, const mwd_law* laws2, const int32_t* contact_pair,

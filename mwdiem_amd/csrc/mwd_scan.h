@@ -1,0 +1,185 @@
+// Single-pass exclusive scan over int32 counts, fused with what produces the
+// counts and what consumes the offsets.
+//
+// Decoupled look-back (Merrill & Garland, "Single-pass Parallel Prefix Scan
+// with Decoupled Look-back", 2016). Blocks past the live tile count exit at
+// once; each live block claims a tile through an atomic ticket, publishes the
+// tile's aggregate, then its inclusive prefix once known; a successor's first
+// wave reads 64 predecessors' status words at a time and adds them up to the
+// nearest inclusive one. A status word packs a 2-bit flag above a 62-bit
+// value, so a reader needs only that word: loads and stores are relaxed
+// device-scope atomics and no fence is needed (a device-scope fence would
+// write back or invalidate the XCD's L2).
+//
+// Scratch: word 0 holds [call:32 | ticket:32]; two status buffers of S words
+// follow. A call uses buffer (call & 1) and clears the other for the next
+// call; the block that draws the last ticket resets the ticket and advances
+// the call number in the same word. So there is one atomic per live tile and
+// nothing to clear between calls.
+//
+// IO is a small struct with
+//   __device__ int32_t load(int64_t i) const;             count of item i
+//   __device__ void store(int64_t i, int64_t off, int32_t v) const;
+//                                                         its exclusive offset
+//   __device__ void finish(int64_t total) const;          once, with the total
+// load/store are called for the live items i < n only.
+#pragma once
+#include "mwd_internal.h"
+
+namespace {
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_WAVES = SCAN_THREADS / MWD_WAVE;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;  // 4096
+
+__device__ __forceinline__ int64_t live_n(int64_t n_cap, const int64_t* n_dev) {
+    if (!n_dev) return n_cap;
+    int64_t n = *n_dev;
+    return n < n_cap ? n : n_cap;
+}
+
+// Inclusive scan across a 256-thread block; returns the inclusive value.
+__device__ __forceinline__ int64_t block_inclusive_scan(int64_t v, int64_t* lds_wave /*[4]*/, int64_t* total) {
+    const int lane = threadIdx.x & (MWD_WAVE - 1);
+    const int wave = threadIdx.x / MWD_WAVE;
+    int64_t x = v;
+#pragma unroll
+    for (int d = 1; d < MWD_WAVE; d <<= 1) {
+        int64_t y = __shfl_up(x, d, MWD_WAVE);
+        if (lane >= d) x += y;
+    }
+    if (lane == MWD_WAVE - 1) lds_wave[wave] = x;
+    __syncthreads();
+    int64_t off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < SCAN_WAVES; ++w) {
+        int64_t t = lds_wave[w];
+        if (w < wave) off += t;
+        tot += t;
+    }
+    __syncthreads();
+    if (total) *total = tot;
+    return x + off;
+}
+
+constexpr uint64_t ST_AGG = 1ull << 62;
+constexpr uint64_t ST_INC = 2ull << 62;
+constexpr uint64_t ST_VAL = (1ull << 62) - 1;
+
+__device__ __forceinline__ uint64_t status_load(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void status_store(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class IO>
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_io(IO io, int64_t n_cap, const int64_t* n_dev, int64_t* total,
+                                                          uint64_t* ctr, uint64_t* status2, int64_t S) {
+    __shared__ int64_t lw[SCAN_WAVES];
+    __shared__ int64_t s_prefix;
+    __shared__ uint64_t s_claim;
+    const int64_t n = live_n(n_cap, n_dev);
+    int64_t ntl = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (ntl < 1) ntl = 1;  // one block runs even when nothing is live: it clears and advances
+    if ((int64_t)blockIdx.x >= ntl) return;
+    if (threadIdx.x == 0) {
+        const uint64_t c = atomicAdd((unsigned long long*)ctr, 1ull);
+        s_claim = c;
+        if ((int64_t)(c & 0xffffffffull) == ntl - 1) status_store(ctr, ((c >> 32) + 1) << 32);  // last ticket
+    }
+    __syncthreads();
+    const uint64_t claim = s_claim;
+    const int64_t tile = (int64_t)(claim & 0xffffffffull);
+    const int par = (int)((claim >> 32) & 1);
+    uint64_t* status = status2 + par * S;
+    uint64_t* other = status2 + (1 - par) * S;
+    const int64_t base = tile * SCAN_TILE;
+    if (base < n) {
+        int32_t v[SCAN_ITEMS];
+        int64_t acc = 0;
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; ++k) {
+            const int64_t i = base + (int64_t)threadIdx.x * SCAN_ITEMS + k;
+            v[k] = (i < n) ? io.load(i) : 0;
+            acc += v[k];
+        }
+        int64_t agg;
+        const int64_t incl = block_inclusive_scan(acc, lw, &agg);
+        if (threadIdx.x < MWD_WAVE) {
+            const int lane = threadIdx.x;
+            int64_t prefix = 0;
+            if (tile == 0) {
+                if (lane == 0) status_store(&status[0], ST_INC | (uint64_t)agg);
+            } else {
+                if (lane == 0) status_store(&status[tile], ST_AGG | (uint64_t)agg);
+                int64_t j = tile - 1;  // window: predecessors j, j-1, ..., j-63
+                for (;;) {
+                    const int64_t t = j - lane;
+                    const uint64_t sw = t >= 0 ? status_load(&status[t]) : ST_INC;
+                    const uint64_t f = sw & ~ST_VAL;
+                    const uint64_t inc = __ballot(f == ST_INC);
+                    const uint64_t none = __ballot(f == 0);
+                    // lanes up to and including the nearest inclusive predecessor
+                    const uint64_t upto = inc ? (inc & (~inc + 1)) * 2 - 1 : ~0ull;
+                    if (none & upto) continue;  // a predecessor in range has not published yet
+                    int64_t x = ((upto >> lane) & 1) ? (int64_t)(sw & ST_VAL) : 0;
+#pragma unroll
+                    for (int d = MWD_WAVE / 2; d >= 1; d >>= 1) x += __shfl_xor(x, d, MWD_WAVE);
+                    prefix += x;
+                    if (inc) break;
+                    j -= MWD_WAVE;
+                }
+                if (lane == 0) status_store(&status[tile], ST_INC | (uint64_t)(prefix + agg));
+            }
+            if (lane == 0) {
+                s_prefix = prefix;
+                if (base + SCAN_TILE >= n) {  // the last live tile
+                    if (total) *total = prefix + agg;
+                    io.finish(prefix + agg);
+                }
+            }
+        }
+        __syncthreads();
+        int64_t run = incl - acc + s_prefix;
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; ++k) {
+            const int64_t i = base + (int64_t)threadIdx.x * SCAN_ITEMS + k;
+            if (i < n) io.store(i, run, v[k]);
+            run += v[k];
+        }
+    } else if (threadIdx.x == 0) {  // nothing live (only tile 0 exists then)
+        if (total) *total = 0;
+        io.finish(0);
+    }
+    // clear the other buffer for the next call
+    for (int64_t j = tile * SCAN_THREADS + threadIdx.x; j < S; j += ntl * SCAN_THREADS) other[j] = 0;
+}
+
+// out[i] = exclusive prefix of in[i]
+struct ScanIO {
+    const int32_t* in;
+    int64_t* out;
+    __device__ __forceinline__ int32_t load(int64_t i) const { return in[i]; }
+    __device__ __forceinline__ void store(int64_t i, int64_t off, int32_t) const { out[i] = off; }
+    __device__ __forceinline__ void finish(int64_t) const {}
+};
+
+template <class IO>
+int mwd_scan_io(const IO& io, int64_t n_cap, const int64_t* n_dev, int64_t* total, int64_t* tmp, int64_t tmp_len,
+                hipStream_t s) {
+    const int64_t nb = n_cap > 0 ? (n_cap + SCAN_TILE - 1) / SCAN_TILE : 1;
+    // scratch layout: [0] call/ticket word, then two status buffers of S words
+    const int64_t S = (tmp_len - 1) / 2;
+    if (nb > S) {
+        mwd_set_error("scan scratch too small");
+        return MWD_E_CAPACITY;
+    }
+    hipLaunchKernelGGL(k_scan_io<IO>, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, s, io, n_cap > 0 ? n_cap : 0, n_dev,
+                       total, reinterpret_cast<uint64_t*>(tmp), reinterpret_cast<uint64_t*>(tmp + 1), S);
+    MWD_LAUNCH_CHECK();
+    return MWD_OK;
+}
+
+}  // namespace

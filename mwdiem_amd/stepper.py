@@ -128,6 +128,9 @@ class Stepper:
 
     def close(self):
         if getattr(self, "ws", None):
+            # work still queued on the stepper's stream must finish before the
+            # workspace and (once this object goes) the state tensors are freed
+            self.stream.synchronize()
             _lib.load().mwd_workspace_destroy(self.ws)
             self.ws = None
 
@@ -240,6 +243,12 @@ class Stepper:
             for j in range(_lib.N_TAN):
                 arr[_lib.N_TAN * a + j] = base + 8 * n * j
         self._tan_ptrs = arr
+
+    def numpy_tangents(self) -> dict:
+        """The tangent arrays [N_TAN, ...] as numpy (after the stepper's stream
+        has finished)."""
+        torch.cuda.synchronize(self.device)
+        return {k: v.cpu().numpy().copy() for k, v in self.tan.items()}
 
     def reset_tangents(self):
         for v in self.tan.values():

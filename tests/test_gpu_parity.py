@@ -169,7 +169,7 @@ def test_reduction_with_more_contacts_than_a_bucket(gpu):
     coll = O.contact_state(pairs, st, sc)[0]
     cnt = np.bincount(pairs[coll == 1].ravel(), minlength=sc.n_rows)
     assert cnt[r] > _lib.ROW_CONTACTS
-    S = _stepper(sc)
+    S = _stepper(sc, pair_cap=8 * len(pairs))
     S.set_state(st)
     ref = {k: v.copy() for k, v in st.items()}
     O.step(ref, sc)

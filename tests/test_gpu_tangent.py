@@ -28,7 +28,7 @@ def test_tangent_step_values_are_bitwise(gpu, dense_phys):
     for k in a:
         assert np.array_equal(a[k], b[k]), k
     # something moved and some tangent is nonzero
-    assert float(B.tan["com"].abs().max()) > 0.0
+    assert np.abs(B.numpy_tangents()["com"]).max() > 0.0
 
 
 def test_tangent_graph_replay_equals_eager(gpu, dense_phys):
@@ -40,8 +40,9 @@ def test_tangent_graph_replay_equals_eager(gpu, dense_phys):
     B.enable_tangents()
     B.capture(10, tangent=True)
     B.run(30)
-    for k in A.tan:
-        assert np.array_equal(A.tan[k].cpu().numpy(), B.tan[k].cpu().numpy()), k
+    ta, tb = A.numpy_tangents(), B.numpy_tangents()  # both streams finished
+    for k in ta:
+        assert np.array_equal(ta[k], tb[k]), k
     a, b = A.numpy_state(), B.numpy_state()
     for k in a:
         assert np.array_equal(a[k], b[k]), k
@@ -77,7 +78,7 @@ def test_state_tangent_matches_central_differences(gpu, k):
     S.enable_tangents()
     S.step_tangent(steps)
     S.check()
-    tan = S.tan["com"][k].cpu().numpy()
+    tan = S.numpy_tangents()["com"][k]
     h = 1e-6 * theta[k]
     tp, tm = theta.copy(), theta.copy()
     tp[k] += h
