@@ -53,6 +53,8 @@ extern "C" {
 #define MWD_F_EPA_CAP 16u          /* some EPA hit its 25-expansion cap (.cu:443-447) */
 #define MWD_F_EPA_FACES 32u        /* some EPA ran out of face storage */
 #define MWD_F_ROW_CONTACTS 64u     /* the stepper's contact-record overflow list is full */
+#define MWD_F_EPA_UNRESOLVED 128u  /* some EPA ended on a placeholder face (epa_status bit 4);
+                                      the physical law applies no force for such a contact */
 
 #define MWD_EPA_MAX_FACES 128
 #define MWD_MAX_CELL 76            /* largest cell occupancy whose set order is emulated */

@@ -25,6 +25,7 @@ F_CELL = 8
 F_EPA_CAP = 16
 F_EPA_FACES = 32
 F_ROW_CONTACTS = 64
+F_EPA_UNRESOLVED = 128
 FATAL_FLAGS = F_PAIR_OVERFLOW | F_CONTACT_OVERFLOW | F_KEYRANGE | F_CELL | F_ROW_CONTACTS
 
 ROW_CONTACTS = 32

@@ -25,6 +25,10 @@
 namespace {
 
 constexpr int EPA_FACES = MWD_EPA_MAX_FACES;
+// Face distances at or above this are EPA's placeholders (1e30 for a face
+// whose plane misses the origin's projection, 3e20 for a deleted face), not
+// penetration depths.
+constexpr double EPA_PLACEHOLDER = 1e20;
 constexpr int EPA_VERTS = 32;
 
 struct Side {
@@ -688,6 +692,13 @@ __device__ LawOut<T> force_law(V3<T> comA, V3<T> comB, V3<T> vA, V3<T> vB, V3<T>
                                const LawT<T>& L) {
     LawOut<T> o;
     const bool phys = L.mode == MWD_LAW_PHYSICAL;
+    if (phys && !(depth < EPA_PLACEHOLDER)) {
+        // EPA ended on a face whose distance is its "outside" or "deleted"
+        // placeholder (1e30 / 3e20): no depth was resolved; the physical law
+        // applies no force for the step (the reference law takes the value)
+        o.Fn = o.tA = o.tB = o.disp = zero3<T>();
+        return o;
+    }
     const V3<T> cp = phys ? (cpA + cpB) / 2.0 : (cpA - cpB) / 2.0;
     const V3<T> dA = cp - comA, dB = cp - comB;
     const T dAn = nrm2(dA), dBn = nrm2(dB);
@@ -845,9 +856,11 @@ __global__ void __launch_bounds__(256) k_epa_force(BodyView A, BodyView B, BodyT
         stv(co.Fn, c, f.Fn);
         stv(co.tA, c, f.tA);
         stv(co.tB, c, f.tB);
-        if (epa_status) epa_status[c] = e.status;
-        if (e.status & 1) fl |= MWD_F_EPA_CAP;
-        if (e.status & 2) fl |= MWD_F_EPA_FACES;
+        const int est = e.status | (e.depth < EPA_PLACEHOLDER ? 0 : 4);
+        if (epa_status) epa_status[c] = est;
+        if (est & 1) fl |= MWD_F_EPA_CAP;
+        if (est & 2) fl |= MWD_F_EPA_FACES;
+        if (est & 4) fl |= MWD_F_EPA_UNRESOLVED;
         if (disp_out) {
             double* dp = disp_out + 4 * p;
             dp[0] = val(f.disp.x);

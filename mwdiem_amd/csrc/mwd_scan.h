@@ -74,12 +74,22 @@ __device__ __forceinline__ void status_store(uint64_t* p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Tile items are loaded and stored in striped order (item k*256 + t by
+// thread t: consecutive lanes touch consecutive items) and scanned in blocked
+// order (items 16t .. 16t+15 by thread t), transposed through LDS. The LDS
+// index is padded by one word per 32 to keep the blocked accesses off a
+// single bank.
+__device__ __forceinline__ int scan_pad(int i) { return i + (i >> 5); }
+constexpr int SCAN_LDS = SCAN_TILE + SCAN_TILE / 32;
+
 template <class IO>
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_io(IO io, int64_t n_cap, const int64_t* n_dev, int64_t* total,
                                                           uint64_t* ctr, uint64_t* status2, int64_t S) {
     __shared__ int64_t lw[SCAN_WAVES];
     __shared__ int64_t s_prefix;
     __shared__ uint64_t s_claim;
+    __shared__ int32_t s_v[SCAN_LDS];
+    __shared__ int64_t s_off[SCAN_LDS];
     const int64_t n = live_n(n_cap, n_dev);
     int64_t ntl = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (ntl < 1) ntl = 1;  // one block runs even when nothing is live: it clears and advances
@@ -97,12 +107,18 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_io(IO io, int64_t n_cap, 
     uint64_t* other = status2 + (1 - par) * S;
     const int64_t base = tile * SCAN_TILE;
     if (base < n) {
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; ++k) {  // striped loads
+            const int il = k * SCAN_THREADS + threadIdx.x;
+            const int64_t i = base + il;
+            s_v[scan_pad(il)] = (i < n) ? io.load(i) : 0;
+        }
+        __syncthreads();
         int32_t v[SCAN_ITEMS];
         int64_t acc = 0;
 #pragma unroll
-        for (int k = 0; k < SCAN_ITEMS; ++k) {
-            const int64_t i = base + (int64_t)threadIdx.x * SCAN_ITEMS + k;
-            v[k] = (i < n) ? io.load(i) : 0;
+        for (int k = 0; k < SCAN_ITEMS; ++k) {  // blocked values
+            v[k] = s_v[scan_pad(threadIdx.x * SCAN_ITEMS + k)];
             acc += v[k];
         }
         int64_t agg;
@@ -144,10 +160,16 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_io(IO io, int64_t n_cap, 
         __syncthreads();
         int64_t run = incl - acc + s_prefix;
 #pragma unroll
-        for (int k = 0; k < SCAN_ITEMS; ++k) {
-            const int64_t i = base + (int64_t)threadIdx.x * SCAN_ITEMS + k;
-            if (i < n) io.store(i, run, v[k]);
+        for (int k = 0; k < SCAN_ITEMS; ++k) {  // blocked offsets
+            s_off[scan_pad(threadIdx.x * SCAN_ITEMS + k)] = run;
             run += v[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; ++k) {  // striped stores
+            const int il = k * SCAN_THREADS + threadIdx.x;
+            const int64_t i = base + il;
+            if (i < n) io.store(i, s_off[scan_pad(il)], s_v[scan_pad(il)]);
         }
     } else if (threadIdx.x == 0) {  // nothing live (only tile 0 exists then)
         if (total) *total = 0;

@@ -54,14 +54,35 @@ def _with_theta(sc: SceneData, theta) -> SceneData:
     return out
 
 
-def run_outcome(sc: SceneData, v0: np.ndarray | None, theta, steps: int, tangents: bool = True,
-                steps_per_graph: int = 0, top_frac: float = 0.1):
-    """Run `steps` steps of the scene with law parameters theta = (Kn, en, mu)
-    (both laws) from its initial state; return (outcome, d outcome/d theta)
-    (gradient None without tangents)."""
-    S = Stepper(_with_theta(sc, theta))
+def form_pile(sc: SceneData, v0: np.ndarray | None = None, steps: int = 20_000, dt: float = 2e-6,
+              steps_per_graph: int = 50) -> dict:
+    """Let the scene's particles fall and settle into a pile (its own law, a
+    coarser time step dt) and return the state (numpy arrays of the state
+    keys) to start runs from."""
+    f = sc.copy()
+    f.dt = dt
+    S = Stepper(f)
     if v0 is not None:
         S.set_state({"v": v0, "v12": v0})
+    S.capture(steps_per_graph)
+    S.run(steps)
+    S.check()
+    st = S.numpy_state()
+    S.close()
+    return st
+
+
+def run_outcome(sc: SceneData, init, theta, steps: int, tangents: bool = True,
+                steps_per_graph: int = 0, top_frac: float = 0.1):
+    """Run `steps` steps of the scene with law parameters theta = (Kn, en, mu)
+    (both laws) from `init` -- None (the scene's initial state), an array of
+    initial velocities, or a state dict; return (outcome, d outcome/d theta)
+    (gradient None without tangents)."""
+    S = Stepper(_with_theta(sc, theta))
+    if isinstance(init, dict):
+        S.set_state(init)
+    elif init is not None:
+        S.set_state({"v": init, "v12": init})
     if tangents:
         S.enable_tangents()
     if steps_per_graph:
@@ -82,12 +103,13 @@ def run_outcome(sc: SceneData, v0: np.ndarray | None, theta, steps: int, tangent
 
 
 class BedOutcome(torch.autograd.Function):
-    """outcome = BedOutcome.apply(theta, scene, v0, steps): theta a float64
-    tensor (Kn, en, mu); backward returns grad_output * d outcome/d theta."""
+    """outcome = BedOutcome.apply(theta, scene, init, steps): theta a float64
+    tensor (Kn, en, mu), init as for run_outcome; backward returns
+    grad_output * d outcome/d theta."""
 
     @staticmethod
-    def forward(ctx, theta, sc, v0, steps, steps_per_graph=0):
-        L, g = run_outcome(sc, v0, theta.detach().cpu().numpy(), int(steps), True, int(steps_per_graph))
+    def forward(ctx, theta, sc, init, steps, steps_per_graph=0):
+        L, g = run_outcome(sc, init, theta.detach().cpu().numpy(), int(steps), True, int(steps_per_graph))
         ctx.save_for_backward(torch.from_numpy(g).to(theta.dtype))
         return theta.new_tensor(L)
 

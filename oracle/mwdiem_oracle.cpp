@@ -243,6 +243,10 @@ inline int correct_flat(V3<T> S1, V3<T> S2, V3<T> S3, V3<T> S4) {
 }
 
 // ------------------------------------------------------------------------ EPA
+// Face distances at or above this are EPA's placeholders (1e30 for a face
+// whose plane misses the origin's projection, 3e20 for a deleted face), not
+// penetration depths.
+constexpr double EPA_PLACEHOLDER = 1e20;
 template <typename T> struct Poly {
     V3<T> V[EPA_MAX_VERTS];
     int opA[EPA_MAX_VERTS], opB[EPA_MAX_VERTS];
@@ -394,6 +398,13 @@ template <typename T>
 ForceOut<T> force_law(const PairIn<T>& in, T depth, V3<T> n, V3<T> cpA, V3<T> cpB, const Law& L) {
     ForceOut<T> o;
     const bool phys = L.mode == 1;
+    if (phys && !(depth < T(EPA_PLACEHOLDER))) {
+        // EPA ended on a face whose distance is its "outside" or "deleted"
+        // placeholder (1e30 / 3e20): no depth was resolved; the physical law
+        // applies no force for the step (the reference law takes the value)
+        o.Fn = o.tA = o.tB = o.disp = mk<T>(T(0.0), T(0.0), T(0.0));
+        return o;
+    }
     V3<T> cp = phys ? (cpA + cpB) / T(2.0) : (cpA - cpB) / T(2.0);
     V3<T> dA = cp - in.comA, dB = cp - in.comB;
     T dAn = norm2(dA), dBn = norm2(dB);
@@ -511,6 +522,7 @@ PairResult contact_one(const Body<T>& A, const Body<T>& B, const PairIn<T>& in, 
             T depth;
             V3<T> cpA, cpB, nrm;
             epa(S, ia, ib, A, B, &depth, &cpA, &cpB, &nrm, &r.epa_status, &r.faces, &r.iters, r.face);
+            if (!(val(depth) < EPA_PLACEHOLDER)) r.epa_status |= 4;
             ForceOut<T> f = force_law(in, depth, nrm, cpA, cpB, L);
             r.depth = val(depth);
             r.n[0] = val(nrm.x); r.n[1] = val(nrm.y); r.n[2] = val(nrm.z);

@@ -62,3 +62,48 @@ def test_gpu_reproduces_golden(gpu, name):
     W = sc.n_walls
     for k in ("com", "q", "qall"):
         assert max_rel(got[k][W:], d[f"out_{k}"][W:]) <= 1e-10, k
+
+
+def test_oracle_flags_unresolved_epa():
+    """A pair met while settling the pile (tests/golden/make_epa_unresolved.py)
+    whose EPA ends on a deleted face: the oracle returns the placeholder
+    distance 3e20 and status bit 4."""
+    d = np.load(GOLD / "epa_unresolved.npz")
+    c, depth, n, ca, cb, st = O.gjk_epa(d["VA"], d["VB"], d["comA"], d["comB"])
+    assert c == 1 and depth == 3e20 and st & 4
+    assert c == int(d["collision"]) and depth == float(d["depth"]) and st == int(d["status"])
+
+
+@pytest.mark.gpu
+def test_gpu_physical_law_skips_unresolved_epa(gpu):
+    """The same pair in a two-hull scene on the physical law: one GPU step
+    equals the oracle's (no force from the unresolved contact) and the
+    workspace reports MWD_F_EPA_UNRESOLVED."""
+    from mwdiem_amd import _lib, scene
+    from mwdiem_amd.stepper import Stepper
+    from tests.helpers import max_rel
+    d = np.load(GOLD / "epa_unresolved.npz")
+    sc = scene.make_scene(2, (40.0, 40.0, 60.0), 5, npts=(8, 20), size=(0.6, 0.9),
+                          law=scene.Law4(mode=scene.PHYSICAL), name="epa-unresolved")
+    W = sc.n_walls
+    vmax = max(sc.vmax, d["VA"].shape[0], d["VB"].shape[0])
+    if vmax > sc.vmax:
+        sc.verts = np.concatenate([sc.verts, np.zeros((sc.n_rows, vmax - sc.vmax, 3))], axis=1)
+        sc.vmax = vmax
+    for r, V, c in ((W, d["VA"], d["comA"]), (W + 1, d["VB"], d["comB"])):
+        sc.verts[r] = 0.0
+        sc.verts[r, :V.shape[0]] = V
+        sc.nverts[r] = V.shape[0]
+        sc.com[r] = c
+    st = O.initial_state(sc)
+    S = Stepper(sc)
+    S.set_state(st)
+    ref = {k: v.copy() for k, v in st.items()}
+    pairs, coll = O.step(ref, sc)
+    S.step(1)
+    s = S.check()
+    assert s["flags"] & _lib.F_EPA_UNRESOLVED
+    got = S.numpy_state()
+    for k in ("com", "v", "w", "q"):
+        assert max_rel(got[k][W:], ref[k][W:]) <= 1e-12, k
+    assert np.abs(got["v"][W:]).max() < 1e3  # no blow-up
