@@ -91,8 +91,20 @@ void mwd_set_error(const std::string& msg) { g_err = msg; }
 // event for each timed kernel.
 static inline hipEvent_t& ev_at(mwd_workspace* ws, int slot, int k) { return ws->ev[slot * MWD_EV_PER_STEP + k]; }
 
+// Timing is for eager steps: nothing is recorded while the stream is being
+// captured into a graph.
+static bool timing_active(mwd_workspace* ws) {
+    if (!ws || !ws->timing) return false;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(ws->stream, &cs) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return cs == hipStreamCaptureStatusNone;
+}
+
 hipEvent_t mwd_kev(mwd_workspace* ws, int kt, int stop) {
-    if (!ws || !ws->timing) return nullptr;
+    if (!timing_active(ws)) return nullptr;
     ws->kt_used |= 1u << kt;
     return ev_at(ws, ws->ring_slot, MWD_N_STAGES + 1 + 2 * kt + (stop ? 1 : 0));
 }
@@ -124,17 +136,14 @@ static void flush_timing(mwd_workspace* ws) {
     ws->ring_slot = 0;
 }
 
-// Events may also be recorded while the stream is being captured: the graph
-// then holds event-record nodes and each replay re-records the same events,
-// so mwd_stage_times after replays reports the last replay's steps.
 void mwd_mark(mwd_workspace* ws, int stage) {
-    if (ws->timing && hipEventRecord(ev_at(ws, ws->ring_slot, stage), ws->stream) != hipSuccess) {
+    if (timing_active(ws) && hipEventRecord(ev_at(ws, ws->ring_slot, stage), ws->stream) != hipSuccess) {
         ws->timing_errors += 1;
         (void)hipGetLastError();
     }
 }
 void mwd_mark_end(mwd_workspace* ws) {
-    if (!ws->timing) return;
+    if (!timing_active(ws)) return;
     if (hipEventRecord(ev_at(ws, ws->ring_slot, MWD_N_STAGES), ws->stream) != hipSuccess) {
         ws->timing_errors += 1;
         (void)hipGetLastError();
