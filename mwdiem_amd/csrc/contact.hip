@@ -264,54 +264,62 @@ __device__ __forceinline__ bool correct_flat(d3 S1, d3 S2, d3 S3, d3 S4) {
 // 1 the simplex handed to EPA (after CorrectFlat) is in S/ia/ib.
 template <class Sup>
 __device__ __forceinline__ int gjk_pair(const Sup& support, d3 comA, d3 comB, d3* S, int* ia, int* ib) {
+    // the simplex lives in named scalars (registers); the arrays are written
+    // once, on a collision
+    d3 S0, S1, S2, S3;
+    int a0, a1, a2, a3, b0, b1, b2, b3;
     const d3 d1 = comB - comA;
-    S[0] = support(d1, &ia[0], &ib[0]);
-    if (dot3(d1, S[0]) < 0.0) return 0;
+    S0 = support(d1, &a0, &b0);
+    if (dot3(d1, S0) < 0.0) return 0;
     const d3 d2 = neg3(d1);
-    S[1] = support(d2, &ia[1], &ib[1]);
-    if (dot3(d2, S[1]) < 0.0) return 0;
+    S1 = support(d2, &a1, &b1);
+    if (dot3(d2, S1) < 0.0) return 0;
     int coll;
     d3 d3v;
-    check_segment(S[0], S[1], &coll, &d3v);
+    check_segment(S0, S1, &coll, &d3v);
     if (coll == 0) return 0;
-    S[2] = support(d3v, &ia[2], &ib[2]);
-    if (dot3(d3v, S[2]) < 0.0) return 0;
+    S2 = support(d3v, &a2, &b2);
+    if (dot3(d3v, S2) < 0.0) return 0;
     d3 d4;
-    check_triangle(S[0], S[1], S[2], &coll, &d4);
+    check_triangle(S0, S1, S2, &coll, &d4);
     if (coll == 0) return 0;
     for (int zz = 0; zz < 50; ++zz) {
-        S[3] = support(d4, &ia[3], &ib[3]);
-        if (dot3(d4, S[3]) < 0.0) return 0;
-        bool hit = zero3(S[3]);
+        S3 = support(d4, &a3, &b3);
+        if (dot3(d4, S3) < 0.0) return 0;
+        bool hit = zero3(S3);
         if (!hit) {
             int todel;
-            check_tetra(S[0], S[1], S[2], S[3], &coll, &d4, &todel);
+            check_tetra(S0, S1, S2, S3, &coll, &d4, &todel);
             if (coll == 0) return 0;
             if (coll == 1) {
                 hit = true;
             } else {
+                // keep the two vertices the reference keeps, then the new one
                 d3 n1, n2;
-                int a1, b1, a2, b2;
+                int x1, y1, x2, y2;
                 if (todel == 1) {
-                    n1 = S[1]; a1 = ia[1]; b1 = ib[1];
-                    n2 = S[2]; a2 = ia[2]; b2 = ib[2];
+                    n1 = S1; x1 = a1; y1 = b1;
+                    n2 = S2; x2 = a2; y2 = b2;
                 } else if (todel == 2) {
-                    n1 = S[2]; a1 = ia[2]; b1 = ib[2];
-                    n2 = S[0]; a2 = ia[0]; b2 = ib[0];
+                    n1 = S2; x1 = a2; y1 = b2;
+                    n2 = S0; x2 = a0; y2 = b0;
                 } else {
-                    n1 = S[0]; a1 = ia[0]; b1 = ib[0];
-                    n2 = S[1]; a2 = ia[1]; b2 = ib[1];
+                    n1 = S0; x1 = a0; y1 = b0;
+                    n2 = S1; x2 = a1; y2 = b1;
                 }
-                S[0] = n1; ia[0] = a1; ib[0] = b1;
-                S[1] = n2; ia[1] = a2; ib[1] = b2;
-                S[2] = S[3]; ia[2] = ia[3]; ib[2] = ib[3];
+                S0 = n1; a0 = x1; b0 = y1;
+                S1 = n2; a1 = x2; b1 = y2;
+                S2 = S3; a2 = a3; b2 = b3;
             }
         }
         if (hit) {
-            if (correct_flat(S[0], S[1], S[2], S[3])) {
-                d4 = cross3(S[0], S[1]);
-                S[3] = support(d4, &ia[3], &ib[3]);
+            if (correct_flat(S0, S1, S2, S3)) {
+                d4 = cross3(S0, S1);
+                S3 = support(d4, &a3, &b3);
             }
+            S[0] = S0; S[1] = S1; S[2] = S2; S[3] = S3;
+            ia[0] = a0; ia[1] = a1; ia[2] = a2; ia[3] = a3;
+            ib[0] = b0; ib[1] = b1; ib[2] = b2; ib[3] = b3;
             return 1;
         }
     }
@@ -986,7 +994,9 @@ int mwd_launch_gjk_surv(const BodyView& A, const BodyView& B, int64_t n_cap, int
     if (n_cap <= 0) return MWD_OK;
     constexpr int G = MWD_GJK_GROUP;
     const int K = (A.vmax + G - 1) / G;
-    const dim3 grid(grid_for(n_cap * (K <= 4 ? G : 1), 256, 4096)), blk(256);
+    // the survivors are few next to the candidates: a grid-strided launch of
+    // at most 1,024 blocks (8 waves per CU at this register use)
+    const dim3 grid(grid_for(n_cap * (K <= 4 ? G : 1), 256, 1024)), blk(256);
     switch (K <= 4 ? K : 0) {
         case 1:
             MWD_LAUNCH_EV(e0, e1, (k_gjk<G, 1>), grid, blk, 0, s, A, B, n_cap, surv, n_surv, collision, simplex,

@@ -30,8 +30,11 @@ namespace {
 
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_WAVES = SCAN_THREADS / MWD_WAVE;
-constexpr int SCAN_ITEMS = 16;
-constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;  // 4096
+// Items per thread: 16 (4,096-item tiles) for long scans, 4 (1,024-item
+// tiles) below a million items, where more tiles give more blocks in flight.
+constexpr int SCAN_ITEMS_LONG = 16;
+constexpr int SCAN_ITEMS_SHORT = 4;
+constexpr int64_t SCAN_SHORT_MAX = 1 << 20;
 
 __device__ __forceinline__ int64_t live_n(int64_t n_cap, const int64_t* n_dev) {
     if (!n_dev) return n_cap;
@@ -80,11 +83,12 @@ __device__ __forceinline__ void status_store(uint64_t* p, uint64_t v) {
 // index is padded by one word per 32 to keep the blocked accesses off a
 // single bank.
 __device__ __forceinline__ int scan_pad(int i) { return i + (i >> 5); }
-constexpr int SCAN_LDS = SCAN_TILE + SCAN_TILE / 32;
 
-template <class IO>
+template <class IO, int SCAN_ITEMS>
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_io(IO io, int64_t n_cap, const int64_t* n_dev, int64_t* total,
                                                           uint64_t* ctr, uint64_t* status2, int64_t S) {
+    constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+    constexpr int SCAN_LDS = SCAN_TILE + SCAN_TILE / 32;
     __shared__ int64_t lw[SCAN_WAVES];
     __shared__ int64_t s_prefix;
     __shared__ uint64_t s_claim;
@@ -191,15 +195,23 @@ struct ScanIO {
 template <class IO>
 int mwd_scan_io(const IO& io, int64_t n_cap, const int64_t* n_dev, int64_t* total, int64_t* tmp, int64_t tmp_len,
                 hipStream_t s) {
-    const int64_t nb = n_cap > 0 ? (n_cap + SCAN_TILE - 1) / SCAN_TILE : 1;
+    const bool shrt = n_cap <= SCAN_SHORT_MAX;
+    const int64_t tile = SCAN_THREADS * (shrt ? SCAN_ITEMS_SHORT : SCAN_ITEMS_LONG);
+    const int64_t nb = n_cap > 0 ? (n_cap + tile - 1) / tile : 1;
     // scratch layout: [0] call/ticket word, then two status buffers of S words
     const int64_t S = (tmp_len - 1) / 2;
     if (nb > S) {
         mwd_set_error("scan scratch too small");
         return MWD_E_CAPACITY;
     }
-    hipLaunchKernelGGL(k_scan_io<IO>, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, s, io, n_cap > 0 ? n_cap : 0, n_dev,
-                       total, reinterpret_cast<uint64_t*>(tmp), reinterpret_cast<uint64_t*>(tmp + 1), S);
+    uint64_t* ctr = reinterpret_cast<uint64_t*>(tmp);
+    uint64_t* st2 = reinterpret_cast<uint64_t*>(tmp + 1);
+    if (shrt)
+        hipLaunchKernelGGL((k_scan_io<IO, SCAN_ITEMS_SHORT>), dim3((unsigned)nb), dim3(SCAN_THREADS), 0, s, io,
+                           n_cap > 0 ? n_cap : 0, n_dev, total, ctr, st2, S);
+    else
+        hipLaunchKernelGGL((k_scan_io<IO, SCAN_ITEMS_LONG>), dim3((unsigned)nb), dim3(SCAN_THREADS), 0, s, io,
+                           n_cap > 0 ? n_cap : 0, n_dev, total, ctr, st2, S);
     MWD_LAUNCH_CHECK();
     return MWD_OK;
 }

@@ -127,7 +127,10 @@ __global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
 
 }  // namespace
 
-int64_t mwd_scan_tmp_len(int64_t n_cap) { return 2 * ((n_cap + SCAN_TILE - 1) / SCAN_TILE + 1) + 1; }
+int64_t mwd_scan_tmp_len(int64_t n_cap) {
+    const int64_t smallest_tile = SCAN_THREADS * SCAN_ITEMS_SHORT;
+    return 2 * ((n_cap + smallest_tile - 1) / smallest_tile + 1) + 1;
+}
 int64_t mwd_sort_tiles(int64_t n_cap) { return (n_cap + SORT_TILE - 1) / SORT_TILE; }
 
 int mwd_scan_i32(const int32_t* in, int64_t* out, int64_t* total, int64_t n_cap, const int64_t* n_dev,

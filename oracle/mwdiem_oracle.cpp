@@ -464,6 +464,7 @@ struct PairResult {
     int faces;
     int iters;
     int face[6];
+    int gjk_iters;  // tetrahedron iterations of GJK (.cu:1104-1288)
     double depth, n[3], cpA[3], cpB[3], Fn[3], tA[3], tB[3], disp[4];
 };
 
@@ -491,6 +492,7 @@ PairResult contact_one(const Body<T>& A, const Body<T>& B, const PairIn<T>& in, 
     check_triangle(S[0], S[1], S[2], &coll, &d4);
     if (coll == 0) { r.collision = 0; return r; }
     for (int zz = 0; zz < 50; ++zz) {
+        r.gjk_iters = zz + 1;
         S[3] = support(A, B, d4, &ia[3], &ib[3]);
         if (dot(d4, S[3]) < T(0.0)) { r.collision = 0; return r; }
         bool hit = false;
@@ -946,6 +948,27 @@ ORC_EXPORT void orc_contact_faces(int64_t P, const int32_t* pa, const int32_t* p
         PairResult r = contact_one(A, B, in, L);
         collision[p] = r.collision;
         for (int k = 0; k < 6; ++k) face[6 * p + k] = r.collision == 1 ? r.face[k] : -1;
+    }
+}
+
+// GJK's tetrahedron iteration count and collision code per pair (test and
+// measurement helper).
+ORC_EXPORT void orc_gjk_iters(int64_t P, const int32_t* pa, const int32_t* pb, int vmax, const double* com,
+                              const double* verts, const int32_t* nverts, int32_t* collision, int32_t* iters) {
+    Law L = make_law(1.0, 0.5, 0.5, 0.5, 1.0);
+    for (int64_t p = 0; p < P; ++p) {
+        int a = pa[p], b = pb[p];
+        Body<double> A{verts + (int64_t)vmax * 3 * a, nverts[a]};
+        Body<double> B{verts + (int64_t)vmax * 3 * b, nverts[b]};
+        PairIn<double> in;
+        std::memset(&in, 0, sizeof(in));
+        in.comA = load3<double>(com + 3 * a);
+        in.comB = load3<double>(com + 3 * b);
+        in.qA.w = 1.0;
+        in.isA = 1.0; in.isB = 1.0;
+        PairResult r = contact_one(A, B, in, L);
+        collision[p] = r.collision;
+        iters[p] = r.gjk_iters;
     }
 }
 
