@@ -142,6 +142,32 @@ int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scene* scene
                          void* stream);
 int mwd_workspace_destroy(mwd_workspace* ws);
 
+/* Launch choices of a workspace. No counterpart in the reference (its
+ * kernels have fixed launches, gpu_fine_contact_detection.py:64-73); every
+ * choice gives the same results, bit for bit, and the autotuner
+ * (mwdiem_amd/tune.py) picks among them by timing.
+ *   cell_pass    MWD_CELLS_TABLE: cells from key-indexed count/lowest-row
+ *                tables, no sort (available while the summed key span is at
+ *                most 2^28; the default then); MWD_CELLS_SORTED: radix sort
+ *                of (key, row).
+ *   gjk_group    lanes sharing one pair's support map: 4, 8 (default) or 16,
+ *                i.e. 16, 8 or 4 pairs per wave.
+ *   gjk_blocks   grid cap of the grid-strided GJK launch, 64..65536 (1024).
+ *   sphere_block threads per block of the bounding-sphere test, a multiple
+ *                of 64 in 64..1024 (256).
+ * mwd_set_tuning applies to later steps (graphs captured before keep their
+ * launches) and returns MWD_E_INVALID for a choice out of range. */
+#define MWD_CELLS_TABLE 0
+#define MWD_CELLS_SORTED 1
+typedef struct mwd_tuning {
+    int32_t cell_pass;
+    int32_t gjk_group;
+    int32_t gjk_blocks;
+    int32_t sphere_block;
+} mwd_tuning;
+int mwd_get_tuning(const mwd_workspace* ws, mwd_tuning* out);
+int mwd_set_tuning(mwd_workspace* ws, const mwd_tuning* t);
+
 /* Verlet over rows [pmin, pmax) of the reference's arrays
  * (gpu_Verlet.cu:85-431). nrot: per-row count of vertex slots to rotate, or
  * NULL to rotate all vmax slots as the reference does. box as in mwd_scene. */

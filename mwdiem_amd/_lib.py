@@ -85,6 +85,15 @@ class Status(C.Structure):
                 ("n_cells", C.c_int64), ("flags", C.c_uint32), ("n_sphere_pass", C.c_int64)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("cell_pass", C.c_int32), ("gjk_group", C.c_int32), ("gjk_blocks", C.c_int32),
+                ("sphere_block", C.c_int32)]
+
+
+CELLS_TABLE = 0
+CELLS_SORTED = 1
+
+
 # name -> (restype, argtypes)
 _SIGS = {
     "mwd_last_error": (C.c_char_p, []),
@@ -95,6 +104,8 @@ _SIGS = {
     "mwd_workspace_create": (C.c_int, [C.POINTER(_P), C.c_int, C.POINTER(Scene), C.c_int32, C.c_int64, C.c_int32,
                                        C.c_int64, C.c_int64, C.c_double, _P]),
     "mwd_workspace_destroy": (C.c_int, [_P]),
+    "mwd_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+    "mwd_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "mwd_verlet": (C.c_int, [_P] * 16 + [C.c_int32, _P, C.c_double, _P, C.c_int64, C.c_int64, _P]),
     "mwd_contact_pairs": (C.c_int, [C.c_int64, C.c_int32] + [_P] * 16 + [_P, _P, C.c_int64] + [_P] * 9
                           + [C.c_int64, _P]),

@@ -42,8 +42,9 @@ int dalloc(T** p, int64_t count) {
 void free_ws(mwd_workspace* ws) {
     if (!ws) return;
     void* ptrs[] = {ws->scenes_dev, ws->cell_of, ws->plist, ws->scene_of, ws->is_wall, ws->keys, ws->keys_alt, ws->vals,
-                    ws->vals_alt, ws->cell_start, ws->cell_key, ws->cell_nb, ws->row_first,
-                    ws->row_rank, ws->cell_by_rank, ws->cell_cnt, ws->cell_off, ws->perm,
+                    ws->vals_alt, ws->cell_start, ws->cell_key, ws->cell_self, ws->cell_seg,
+                    ws->row_first, ws->row_rank, ws->cell_cnt, ws->cell_off, ws->perm, ws->key_cnt,
+                    ws->key_min, ws->row_key,
                     ws->pairs, ws->collision, ws->simplex, ws->sidx, ws->surv, ws->contact_pair,
                     ws->depth, ws->normal, ws->cpA, ws->cpB, ws->Fn, ws->tA, ws->tB, ws->epa_status, ws->rec_cnt,
                     ws->rec_bkt, ws->rec_ovf, ws->hist,
@@ -259,8 +260,7 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     }
     *out = nullptr;
     MWD_HIP_CHECK(hipSetDevice(device));
-    mwd_workspace* ws = new mwd_workspace();
-    memset((void*)ws, 0, sizeof(*ws));
+    mwd_workspace* ws = new mwd_workspace();  // value-initialised: zeros, then the members' own defaults
     ws->device = device;
     ws->stream = (hipStream_t)stream;
     ws->n_scenes = n_scenes;
@@ -362,16 +362,23 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     A(vals_alt, M);
     A(cell_start, M + 1);
     A(cell_key, M);
-    A(cell_nb, MWD_MAX_OFF * M);
+    A(cell_self, M);
+    A(cell_seg, MWD_MAX_OFF * M);
     A(row_first, n_rows);
     A(row_rank, n_rows);
-    A(cell_by_rank, M);
     A(cell_cnt, M);
     A(cell_off, M);
     A(perm, M);
-    if (ws->key_span_total <= (int64_t(1) << 28)) {
+    // key-indexed tables (12 bytes per key) for the cell pass without a sort
+    // (the empty lowest-row mark 0x7F7F7F7F must lie above every row)
+    ws->cell_table_ok = ws->key_span_total <= (int64_t(1) << 28) && n_rows < 0x7F7F7F7F;
+    ws->cell_table = ws->cell_table_ok;
+    if (ws->cell_table_ok) {
         A(cell_of, ws->key_span_total);
+        A(key_cnt, ws->key_span_total);
+        A(key_min, ws->key_span_total);
     }
+    A(row_key, n_rows);
     A(pairs, 2 * pair_cap);
     A(collision, pair_cap);
     A(simplex, contact_cap);
@@ -412,8 +419,11 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     H2D(laws, laws.data(), 2 * n_scenes);
     H2D(scene_wt_off, wt.data(), n_scenes + 1);
 #undef H2D
-    if ((ws->cell_of &&
-         hipMemsetAsync(ws->cell_of, 0xFF, sizeof(int32_t) * ws->key_span_total, s) != hipSuccess) ||
+    const size_t kbytes = sizeof(int32_t) * (size_t)ws->key_span_total;
+    if ((ws->cell_table_ok && (hipMemsetAsync(ws->cell_of, 0xFF, kbytes, s) != hipSuccess ||
+                            hipMemsetAsync(ws->key_cnt, 0, kbytes, s) != hipSuccess ||
+                            hipMemsetAsync(ws->key_min, 0x7F, kbytes, s) != hipSuccess)) ||
+        hipMemsetAsync(ws->row_key, 0xFF, sizeof(int32_t) * n_rows, s) != hipSuccess ||
         hipMemsetAsync(ws->counters, 0, sizeof(int64_t) * 16, s) != hipSuccess ||
         hipMemsetAsync(ws->rec_cnt, 0, sizeof(int32_t) * n_rows, s) != hipSuccess ||
         hipMemsetAsync(ws->scan_tmp, 0, sizeof(int64_t) * ws->scan_tmp_len, s) != hipSuccess ||
@@ -423,10 +433,41 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
         free_ws(ws);
         return MWD_E_HIP;
     }
-    ws->keys_sorted = ws->keys;
-    ws->vals_sorted = ws->vals;
 
     *out = ws;
+    return MWD_OK;
+}
+
+MWD_API int mwd_get_tuning(const mwd_workspace* ws, mwd_tuning* out) {
+    if (!ws || !out) {
+        mwd_set_error("mwd_get_tuning: null argument");
+        return MWD_E_INVALID;
+    }
+    out->cell_pass = ws->cell_table ? MWD_CELLS_TABLE : MWD_CELLS_SORTED;
+    out->gjk_group = ws->tune.gjk_group;
+    out->gjk_blocks = ws->tune.gjk_blocks;
+    out->sphere_block = ws->tune.sphere_block;
+    return MWD_OK;
+}
+
+MWD_API int mwd_set_tuning(mwd_workspace* ws, const mwd_tuning* t) {
+    if (!ws || !t) {
+        mwd_set_error("mwd_set_tuning: null argument");
+        return MWD_E_INVALID;
+    }
+    const bool pass_ok = t->cell_pass == MWD_CELLS_SORTED || (t->cell_pass == MWD_CELLS_TABLE && ws->cell_table_ok);
+    const bool group_ok = t->gjk_group == 4 || t->gjk_group == 8 || t->gjk_group == 16;
+    const bool blocks_ok = t->gjk_blocks >= 64 && t->gjk_blocks <= 65536;
+    const bool sphere_ok = t->sphere_block >= 64 && t->sphere_block <= 1024 && t->sphere_block % 64 == 0;
+    if (!pass_ok || !group_ok || !blocks_ok || !sphere_ok) {
+        mwd_set_error(!pass_ok ? "mwd_set_tuning: cell_pass unknown, or MWD_CELLS_TABLE without key tables"
+                               : "mwd_set_tuning: gjk_group, gjk_blocks or sphere_block out of range");
+        return MWD_E_INVALID;
+    }
+    ws->cell_table = t->cell_pass == MWD_CELLS_TABLE;
+    ws->tune.gjk_group = t->gjk_group;
+    ws->tune.gjk_blocks = t->gjk_blocks;
+    ws->tune.sphere_block = t->sphere_block;
     return MWD_OK;
 }
 
@@ -516,12 +557,12 @@ static int fine_contact(mwd_workspace* ws, const mwd_state* st, double* const* t
     }
     ws->fresh_pairs = 0;
     rc = mwd_launch_sphere(A, B, ws->pair_cap, ctr + 3, ws->collision, ws->surv, ctr + 10, s,
-                           mwd_kev(ws, MWD_KT_SPHERE, 0), mwd_kev(ws, MWD_KT_SPHERE, 1));
+                           mwd_kev(ws, MWD_KT_SPHERE, 0), mwd_kev(ws, MWD_KT_SPHERE, 1), ws->tune);
     if (rc) return rc;
     mwd_mark(ws, 6);
     rc = mwd_launch_gjk_surv(A, B, ws->pair_cap, ws->collision, ws->surv, ctr + 10, ws->simplex,
                              ws->sidx, ctr + 8, ws->contact_cap, ws->flags, s, mwd_kev(ws, MWD_KT_GJK, 0),
-                             mwd_kev(ws, MWD_KT_GJK, 1));
+                             mwd_kev(ws, MWD_KT_GJK, 1), ws->tune);
     if (rc) return rc;
     mwd_mark(ws, 7);
     rc = mwd_launch_compact_contacts(ws->collision, ws->pair_cap, ctr + 3, ws->contact_pair, ws->contact_cap, ctr + 4,

@@ -330,7 +330,7 @@ __device__ __forceinline__ int gjk_pair(const Sup& support, d3 comA, d3 comB, d3
 // decided here (the first support test of .cu:1013 rejects them); the rest
 // are appended to surv with one atomic per wave, so GJK then runs on full
 // waves of live pairs instead of waves that mostly idle.
-__global__ void __launch_bounds__(256) k_sphere(BodyView A, BodyView B, int64_t n_cap, const int64_t* n_dev,
+__global__ void __launch_bounds__(1024) k_sphere(BodyView A, BodyView B, int64_t n_cap, const int64_t* n_dev,
                                                 int32_t* __restrict__ collision, int32_t* __restrict__ surv,
                                                 int64_t* n_surv) {
     int64_t n = n_dev ? *n_dev : n_cap;
@@ -366,7 +366,7 @@ __global__ void __launch_bounds__(256) k_sphere(BodyView A, BodyView B, int64_t 
 // GJK over the survivors (.cu:983-1322, without EPA), G lanes per pair
 // (GroupSup); every lane of a group follows the same path on the same
 // values, and its first lane writes the results. K = 0 selects one lane per
-// pair with the vertices read from memory (bodies of more than 4G vertices).
+// pair with the vertices read from memory (bodies of more than 6G vertices).
 template <int G, int K>
 __global__ void __launch_bounds__(256) k_gjk(BodyView A, BodyView B, int64_t n_cap, const int32_t* __restrict__ surv,
                                              const int64_t* n_surv, int32_t* __restrict__ collision,
@@ -972,10 +972,12 @@ unsigned grid_for(int64_t n, int threads, int64_t max_blocks) {
 }  // namespace
 
 int mwd_launch_sphere(const BodyView& A, const BodyView& B, int64_t n_cap, const int64_t* n_dev, int32_t* collision,
-                      int32_t* surv, int64_t* n_surv, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+                      int32_t* surv, int64_t* n_surv, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                      const PairTune& tu) {
     if (n_cap <= 0) return MWD_OK;
-    MWD_LAUNCH_EV(e0, e1, k_sphere, dim3(grid_for(n_cap, 256, 8192)), dim3(256), 0, s, A, B, n_cap, n_dev, collision,
-                  surv, n_surv);
+    const int tb = tu.sphere_block;
+    MWD_LAUNCH_EV(e0, e1, k_sphere, dim3(grid_for(n_cap, tb, (8192 * 256) / tb)), dim3(tb), 0, s, A, B, n_cap, n_dev,
+                  collision, surv, n_surv);
     MWD_LAUNCH_CHECK();
     return MWD_OK;
 }
@@ -988,35 +990,61 @@ int mwd_launch_gjk(const BodyView& A, const BodyView& B, int64_t n_cap, const in
     return mwd_launch_gjk_surv(A, B, n_cap, collision, surv, n_surv, simplex, sidx, slots, slot_cap, flags, s);
 }
 
+namespace {
+
+// k_gjk<G, K> for K = ceil(vmax / G) up to GJK_MAX_K vertices per lane; the
+// serial form (K = 0) beyond.
+constexpr int GJK_MAX_K = 6;
+
+template <int G, int K>
+void launch_gjk_k(int k, dim3 grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const BodyView& A,
+                  const BodyView& B, int64_t n_cap, const int32_t* surv, const int64_t* n_surv, int32_t* collision,
+                  Simplex* simplex, int32_t* sidx, int64_t* slots, int64_t slot_cap, uint32_t* flags) {
+    if constexpr (K > GJK_MAX_K) {
+        MWD_LAUNCH_EV(e0, e1, (k_gjk<G, 0>), grid, dim3(256), 0, s, A, B, n_cap, surv, n_surv, collision, simplex,
+                      sidx, slots, slot_cap, flags);
+    } else {
+        if (k == K)
+            MWD_LAUNCH_EV(e0, e1, (k_gjk<G, K>), grid, dim3(256), 0, s, A, B, n_cap, surv, n_surv, collision,
+                          simplex, sidx, slots, slot_cap, flags);
+        else
+            launch_gjk_k<G, K + 1>(k, grid, s, e0, e1, A, B, n_cap, surv, n_surv, collision, simplex, sidx, slots,
+                                   slot_cap, flags);
+    }
+}
+
+template <int G>
+void launch_gjk_g(const BodyView& A, const BodyView& B, int64_t n_cap, int32_t* collision, const int32_t* surv,
+                  const int64_t* n_surv, Simplex* simplex, int32_t* sidx, int64_t* slots, int64_t slot_cap,
+                  uint32_t* flags, hipStream_t s, hipEvent_t e0, hipEvent_t e1, int max_blocks) {
+    const int K = (A.vmax + G - 1) / G;
+    const bool grouped = K <= GJK_MAX_K;
+    // the survivors are few next to the candidates: a grid-strided launch
+    // with a capped grid
+    const dim3 grid(grid_for(n_cap * (grouped ? G : 1), 256, max_blocks));
+    launch_gjk_k<G, 1>(grouped ? K : GJK_MAX_K + 1, grid, s, e0, e1, A, B, n_cap, surv, n_surv, collision, simplex,
+                       sidx, slots, slot_cap, flags);
+}
+
+}  // namespace
+
 int mwd_launch_gjk_surv(const BodyView& A, const BodyView& B, int64_t n_cap, int32_t* collision,
                         const int32_t* surv, const int64_t* n_surv, Simplex* simplex, int32_t* sidx, int64_t* slots,
-                        int64_t slot_cap, uint32_t* flags, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+                        int64_t slot_cap, uint32_t* flags, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                        const PairTune& tu) {
     if (n_cap <= 0) return MWD_OK;
-    constexpr int G = MWD_GJK_GROUP;
-    const int K = (A.vmax + G - 1) / G;
-    // the survivors are few next to the candidates: a grid-strided launch of
-    // at most 1,024 blocks (8 waves per CU at this register use)
-    const dim3 grid(grid_for(n_cap * (K <= 4 ? G : 1), 256, 1024)), blk(256);
-    switch (K <= 4 ? K : 0) {
-        case 1:
-            MWD_LAUNCH_EV(e0, e1, (k_gjk<G, 1>), grid, blk, 0, s, A, B, n_cap, surv, n_surv, collision, simplex,
-                               sidx, slots, slot_cap, flags);
-            break;
-        case 2:
-            MWD_LAUNCH_EV(e0, e1, (k_gjk<G, 2>), grid, blk, 0, s, A, B, n_cap, surv, n_surv, collision, simplex,
-                               sidx, slots, slot_cap, flags);
-            break;
-        case 3:
-            MWD_LAUNCH_EV(e0, e1, (k_gjk<G, 3>), grid, blk, 0, s, A, B, n_cap, surv, n_surv, collision, simplex,
-                               sidx, slots, slot_cap, flags);
-            break;
+    switch (tu.gjk_group) {
         case 4:
-            MWD_LAUNCH_EV(e0, e1, (k_gjk<G, 4>), grid, blk, 0, s, A, B, n_cap, surv, n_surv, collision, simplex,
-                               sidx, slots, slot_cap, flags);
+            launch_gjk_g<4>(A, B, n_cap, collision, surv, n_surv, simplex, sidx, slots, slot_cap, flags, s, e0, e1,
+                            tu.gjk_blocks);
+            break;
+        case 16:
+            launch_gjk_g<16>(A, B, n_cap, collision, surv, n_surv, simplex, sidx, slots, slot_cap, flags, s, e0, e1,
+                             tu.gjk_blocks);
             break;
         default:
-            MWD_LAUNCH_EV(e0, e1, (k_gjk<G, 0>), grid, blk, 0, s, A, B, n_cap, surv, n_surv, collision, simplex,
-                               sidx, slots, slot_cap, flags);
+            launch_gjk_g<8>(A, B, n_cap, collision, surv, n_surv, simplex, sidx, slots, slot_cap, flags, s, e0, e1,
+                            tu.gjk_blocks);
     }
     MWD_LAUNCH_CHECK();
     return MWD_OK;

@@ -15,10 +15,22 @@
 //     Scenes on the physical law use the full 13-cell forward half shell
 //     instead (the reference's 7 offsets miss 6 neighbour directions).
 //
-// Pipeline: keys -> stable radix sort of (key, row) -> cell heads (scan) ->
-// cell rank by lowest row (scan over rows) -> per-cell set order, neighbour
-// lookup (binary search of the sorted cell keys) and pair count -> scan of
-// counts in rank order -> emission. Wall tests are a flag/scan/scatter over
+// Two ways to the cells, both ending in per-cell tables by rank (cells in
+// order of their lowest row) that the emission reads:
+//   key table (default while the summed key span is at most 2^28): a
+//     counting sort whose single digit is the whole key. Per particle, count
+//     the key's members and keep its lowest row (atomics on a key-indexed
+//     table); one scan over rows numbers the cells by rank and places their
+//     member ranges (a packed two-count scan); per particle, claim a slot in
+//     its cell; per cell (fused into the scan of pair counts), sort the few
+//     members, put them in set order, look up the neighbour cells in the
+//     table and count pairs. The tables return to their empty state as they
+//     are used, so nothing is cleared between steps.
+//   sorted keys: stable radix sort of (key, row) -> cell heads (scan) ->
+//     cell rank by lowest row (scan over rows) -> per-cell set order,
+//     neighbour lookup (binary search of the sorted cell keys) and pair
+//     count -> scan of counts in rank order.
+// Then emission, one pair per thread. Wall tests are a flag/scan/scatter over
 // the (scene, wall, particle) test space.
 #include "mwd_internal.h"
 #include "mwd_scan.h"
@@ -40,7 +52,20 @@ __global__ void k_cell_keys(const double* __restrict__ com, int64_t n, double cx
     keys[i] = key_of(ld3(com + 3 * i), cell, ncx, ncy);
 }
 
-// Global sort keys of the moving particles.
+// Global key of a moving row: its scene's key offset by the scene's base in
+// the concatenated key space (clamped, with a flag, outside the scene's span).
+__device__ __forceinline__ uint32_t global_key(const double* __restrict__ com, const int64_t* __restrict__ keys_in,
+                                               int32_t row, const SceneDev& sc, uint32_t* flags) {
+    const int64_t key = keys_in ? keys_in[row] : key_of(ld3(com + 3 * row), sc.cell, sc.ncx, sc.ncy);
+    int64_t loc = key - sc.kmin;
+    if (loc < 0 || loc >= sc.kspan) {
+        atomicOr(flags, MWD_F_KEYRANGE);
+        loc = loc < 0 ? 0 : sc.kspan - 1;
+    }
+    return (uint32_t)(sc.kbase + loc);
+}
+
+// Sorted path: global sort keys of the moving particles.
 __global__ void k_sort_keys(const double* __restrict__ com, const int64_t* __restrict__ keys_in,
                             const int32_t* __restrict__ plist, int64_t M, const int32_t* __restrict__ scene_of,
                             const SceneDev* __restrict__ scenes, uint32_t* __restrict__ keys,
@@ -48,15 +73,39 @@ __global__ void k_sort_keys(const double* __restrict__ com, const int64_t* __res
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
     const int32_t row = plist[i];
-    const SceneDev& sc = scenes[scene_of[row]];
-    const int64_t key = keys_in ? keys_in[row] : key_of(ld3(com + 3 * row), sc.cell, sc.ncx, sc.ncy);
-    int64_t loc = key - sc.kmin;
-    if (loc < 0 || loc >= sc.kspan) {
-        atomicOr(flags, MWD_F_KEYRANGE);
-        loc = loc < 0 ? 0 : sc.kspan - 1;
-    }
-    keys[i] = (uint32_t)(sc.kbase + loc);
+    keys[i] = global_key(com, keys_in, row, scenes[scene_of[row]], flags);
     vals[i] = row;
+}
+
+// Key-table path, one thread per moving particle: its key, the member count
+// of that key and the key's lowest row.
+constexpr int32_t KEY_MIN_EMPTY = 0x7F7F7F7F;  // above every row; the byte-fill value of an empty table
+
+__global__ void k_key_count(const double* __restrict__ com, const int64_t* __restrict__ keys_in,
+                            const int32_t* __restrict__ plist, int64_t M, const int32_t* __restrict__ scene_of,
+                            const SceneDev* __restrict__ scenes, int32_t* __restrict__ row_key,
+                            int32_t* __restrict__ key_cnt, int32_t* __restrict__ key_min, uint32_t* flags) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    const int32_t row = plist[i];
+    const uint32_t k = global_key(com, keys_in, row, scenes[scene_of[row]], flags);
+    row_key[row] = (int32_t)k;
+    atomicAdd(&key_cnt[k], 1);
+    atomicMin(&key_min[k], row);
+}
+
+// Key-table path: each particle takes a slot in its cell's member range. The
+// decrements bring the key counts back to zero; slots come in no set order,
+// so the cell pass sorts its members.
+__global__ void k_key_scatter(const int32_t* __restrict__ plist, int64_t M, const int32_t* __restrict__ row_key,
+                              int32_t* __restrict__ key_cnt, const int32_t* __restrict__ cell_of,
+                              const int32_t* __restrict__ cell_start, int32_t* __restrict__ members) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    const int32_t row = plist[i];
+    const int32_t k = row_key[row];
+    const int32_t slot = atomicSub(&key_cnt[k], 1) - 1;
+    members[cell_start[cell_of[k]] + slot] = row;
 }
 
 // CPython set iteration order of non-negative ints inserted in the given
@@ -149,86 +198,135 @@ __device__ __forceinline__ int64_t find_cell(const uint32_t* cell_key, int64_t C
     return (lo < C && cell_key[lo] == key) ? lo : -1;
 }
 
-__global__ void k_cell_info(const int32_t* __restrict__ vals, const int32_t* __restrict__ cell_start,
+// The per-cell work of both paths: members in set order into perm, the
+// cell's and its neighbours' (first slot, members) into the rank-ordered
+// tables, and the cell's pair count. Cell c holds member slots
+// [cell_start[c], cell_start[c+1]); `sorted` says whether they are already
+// in ascending row order. nb_of(t) gives the cell of global key t or -1.
+template <class NbOf>
+__device__ __forceinline__ int32_t cell_work(int64_t c, int64_t r, bool sorted, int32_t* __restrict__ members,
+                                             const int32_t* __restrict__ cell_start,
+                                             const uint32_t* __restrict__ cell_key,
+                                             const int32_t* __restrict__ scene_of,
+                                             const SceneDev* __restrict__ scenes, int32_t* __restrict__ perm,
+                                             int2* __restrict__ cell_self, int2* __restrict__ cell_seg,
+                                             uint32_t* flags, const NbOf& nb_of) {
+    const int32_t start = cell_start[c], end = cell_start[c + 1];
+    const int n = end - start;
+    int32_t* m = members + start;
+    // every member of a cell lies in one scene (keys are offset per scene)
+    const SceneDev& sc = scenes[scene_of[m[0]]];
+    const int64_t lk = (int64_t)cell_key[c] - sc.kbase + sc.kmin;
+    // neighbour lookups, then their member ranges, then the stores: phases
+    // of independent loads, so their latencies overlap
+    int64_t nb[MWD_MAX_OFF];
+#pragma unroll
+    for (int o = 0; o < MWD_MAX_OFF; ++o) {
+        nb[o] = -1;
+        if (o < sc.n_off) {
+            const int64_t t = lk + sc.off[o] - sc.kmin;
+            if (t >= 0 && t < sc.kspan) nb[o] = nb_of((uint32_t)(sc.kbase + t));
+        }
+    }
+    int2 seg[MWD_MAX_OFF];
+#pragma unroll
+    for (int o = 0; o < MWD_MAX_OFF; ++o) {
+        seg[o] = make_int2(0, 0);
+        if (nb[o] >= 0) {
+            seg[o].x = cell_start[nb[o]];
+            seg[o].y = cell_start[nb[o] + 1] - seg[o].x;
+        }
+    }
+    if (!sorted) {  // insertion sort; cells hold a few members
+        for (int k = 1; k < n; ++k) {
+            const int32_t x = m[k];
+            int j = k - 1;
+            while (j >= 0 && m[j] > x) {
+                m[j + 1] = m[j];
+                --j;
+            }
+            m[j + 1] = x;
+        }
+    }
+    if (n > MWD_MAX_CELL) {
+        atomicOr(flags, MWD_F_CELL);
+        for (int k = 0; k < n; ++k) perm[start + k] = m[k];
+    } else {
+        set_order(m, n, sc.row0, perm + start);
+    }
+    cell_self[r] = make_int2(start, n);
+    int64_t cnt = (int64_t)n * (n - 1) / 2;
+#pragma unroll
+    for (int o = 0; o < MWD_MAX_OFF; ++o) {
+        cell_seg[r * MWD_MAX_OFF + o] = seg[o];
+        cnt += (int64_t)n * seg[o].y;
+    }
+    return cnt < INT32_MAX ? (int32_t)cnt : INT32_MAX;  // beyond only with MWD_F_CELL raised
+}
+
+struct NbSorted {  // binary search of the sorted cell keys
+    const uint32_t* cell_key;
+    int64_t C;
+    __device__ __forceinline__ int64_t operator()(uint32_t key) const { return find_cell(cell_key, C, key); }
+};
+
+struct NbTable {  // key -> cell table, whose stale entries fail the key check
+    const int32_t* cell_of;
+    const uint32_t* cell_key;
+    int64_t C;
+    __device__ __forceinline__ int64_t operator()(uint32_t key) const {
+        const int32_t c = cell_of[key];
+        return (c >= 0 && c < C && cell_key[c] == key) ? c : -1;
+    }
+};
+
+// Sorted path: one thread per cell (cells in key order, members sorted).
+__global__ void k_cell_info(int32_t* __restrict__ vals, const int32_t* __restrict__ cell_start,
                             const uint32_t* __restrict__ cell_key, const int64_t* __restrict__ n_cells,
-                            int64_t M, const int32_t* __restrict__ scene_of, const SceneDev* __restrict__ scenes,
+                            const int32_t* __restrict__ scene_of, const SceneDev* __restrict__ scenes,
                             const int64_t* __restrict__ row_rank, int32_t* __restrict__ perm,
-                            int32_t* __restrict__ cell_nb, int32_t* __restrict__ cell_by_rank,
-                            int32_t* __restrict__ cell_cnt, uint32_t* flags, const int32_t* __restrict__ cell_of) {
+                            int2* __restrict__ cell_self, int2* __restrict__ cell_seg,
+                            int32_t* __restrict__ cell_cnt, uint32_t* flags) {
     const int64_t C = *n_cells;
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C) return;
-    const int32_t start = cell_start[c], end = cell_start[c + 1];
-    const int n = end - start;
-    const int32_t row_min = vals[start];
-    const SceneDev& sc = scenes[scene_of[row_min]];
-    if (n > MWD_MAX_CELL) {
-        atomicOr(flags, MWD_F_CELL);
-        for (int k = 0; k < n; ++k) perm[start + k] = vals[start + k];
-    } else {
-        set_order(vals + start, n, sc.row0, perm + start);
-    }
-    const int64_t lk = (int64_t)cell_key[c] - sc.kbase + sc.kmin;
-    int64_t cnt = (int64_t)n * (n - 1) / 2;
-    for (int o = 0; o < MWD_MAX_OFF; ++o) {
-        int64_t nb = -1;
-        if (o < sc.n_off) {
-            const int64_t t = lk + sc.off[o] - sc.kmin;
-            if (t >= 0 && t < sc.kspan)
-                nb = cell_of ? cell_of[sc.kbase + t] : find_cell(cell_key, C, (uint32_t)(sc.kbase + t));
-        }
-        cell_nb[c * MWD_MAX_OFF + o] = (int32_t)nb;
-        if (nb >= 0) cnt += (int64_t)n * (cell_start[nb + 1] - cell_start[nb]);
-    }
-    const int64_t r = row_rank[row_min];
-    cell_by_rank[r] = (int32_t)c;
-    cell_cnt[r] = (int32_t)cnt;
+    const int64_t r = row_rank[vals[cell_start[c]]];
+    cell_cnt[r] = cell_work(c, r, true, vals, cell_start, cell_key, scene_of, scenes, perm, cell_self, cell_seg,
+                            flags, NbSorted{cell_key, C});
 }
 
 // Pair emission, one output pair per thread: a block takes 64 cells by rank,
-// stages their pair offsets, members and neighbour cells in LDS, and its
-// threads walk the block's output range, so consecutive threads write
-// consecutive pairs. Pair q of a cell is decoded in the emission order of
-// Linked_cell_hash (LIBRARY_Coarse_contact_detection.py:84-118): same-cell
-// pairs (j < k, members in set order), then per neighbour offset o every
-// member j against every member k of that cell.
+// stages their pair offsets, member ranges and neighbour ranges in LDS
+// (coalesced loads of the rank-ordered tables), and its threads walk the
+// block's output range, so consecutive threads write consecutive pairs. Pair
+// q of a cell is decoded in the emission order of Linked_cell_hash
+// (LIBRARY_Coarse_contact_detection.py:84-118): same-cell pairs (j < k,
+// members in set order), then per neighbour offset o every member j against
+// every member k of that cell.
 constexpr int FILL_CELLS = 64;    // cells per block
 constexpr int FILL_THREADS = 256;
 
 __global__ void __launch_bounds__(FILL_THREADS) k_cell_fill(const int32_t* __restrict__ perm,
-                                                          const int32_t* __restrict__ cell_start,
-                                                          const int32_t* __restrict__ cell_nb,
-                                                          const int32_t* __restrict__ cell_by_rank,
+                                                          const int2* __restrict__ cell_self,
+                                                          const int2* __restrict__ cell_seg,
                                                           const int64_t* __restrict__ cell_off,
                                                           const int64_t* __restrict__ counters,
                                                           int32_t* __restrict__ pairs, int64_t cap) {
     __shared__ int64_t s_off[FILL_CELLS + 1];
-    __shared__ int32_t s_s0[FILL_CELLS], s_n[FILL_CELLS];
-    __shared__ int32_t s_t0[FILL_CELLS][MWD_MAX_OFF], s_tn[FILL_CELLS][MWD_MAX_OFF];
+    __shared__ int2 s_self[FILL_CELLS];
+    __shared__ int2 s_seg[FILL_CELLS * MWD_MAX_OFF];
     const int64_t C = counters[0];
     const int64_t r0 = (int64_t)blockIdx.x * FILL_CELLS;
     if (r0 >= C) return;  // uniform per block
     const int nc = (int)(C - r0 < FILL_CELLS ? C - r0 : FILL_CELLS);
     const int tid = threadIdx.x;
     const int64_t base = cell_off[r0];
+    for (int k = tid; k < nc * MWD_MAX_OFF; k += FILL_THREADS) s_seg[k] = cell_seg[r0 * MWD_MAX_OFF + k];
     if (tid < nc) {
-        const int32_t c = cell_by_rank[r0 + tid];
-        const int32_t s0 = cell_start[c];
-        const int32_t n = cell_start[c + 1] - s0;
-        s_s0[tid] = s0;
-        s_n[tid] = n;
-        int64_t cnt = (int64_t)n * (n - 1) / 2;
-        for (int o = 0; o < MWD_MAX_OFF; ++o) {
-            const int32_t nb = cell_nb[(int64_t)c * MWD_MAX_OFF + o];
-            const int32_t t0 = nb >= 0 ? cell_start[nb] : 0;
-            const int32_t tn = nb >= 0 ? cell_start[nb + 1] - t0 : 0;
-            s_t0[tid][o] = t0;
-            s_tn[tid][o] = tn;
-            cnt += (int64_t)n * tn;
-        }
-        s_off[tid + 1] = cell_off[r0 + tid] - base + cnt;  // end of this cell's range
+        s_self[tid] = cell_self[r0 + tid];
+        s_off[tid] = cell_off[r0 + tid] - base;
     }
-    if (tid == 0) s_off[0] = 0;
+    if (tid == 0) s_off[nc] = (r0 + nc < C ? cell_off[r0 + nc] : counters[2]) - base;  // counters[2]: all cell pairs
     __syncthreads();
     const int64_t total = s_off[nc];
     const int64_t pos0 = counters[1] + base;
@@ -241,7 +339,7 @@ __global__ void __launch_bounds__(FILL_THREADS) k_cell_fill(const int32_t* __res
             else hi = mid;
         }
         const int i = lo;
-        const int32_t n = s_n[i], s0 = s_s0[i];
+        const int32_t s0 = s_self[i].x, n = s_self[i].y;
         int64_t t = q - s_off[i];
         const int64_t tri = (int64_t)n * (n - 1) / 2;
         int32_t a, b;
@@ -259,15 +357,15 @@ __global__ void __launch_bounds__(FILL_THREADS) k_cell_fill(const int32_t* __res
             t -= tri;
             int o = 0;
             for (; o < MWD_MAX_OFF; ++o) {
-                const int64_t sz = (int64_t)n * s_tn[i][o];
+                const int64_t sz = (int64_t)n * s_seg[i * MWD_MAX_OFF + o].y;
                 if (t < sz) break;
                 t -= sz;
             }
             if (o == MWD_MAX_OFF) continue;  // not reached: q lies inside cell i's range
-            const int32_t tn = s_tn[i][o];
-            const int32_t j = (int32_t)(t / tn), k = (int32_t)(t - (int64_t)j * tn);
+            const int2 sg = s_seg[i * MWD_MAX_OFF + o];
+            const int32_t j = (int32_t)(t / sg.y), k = (int32_t)(t - (int64_t)j * sg.y);
             a = perm[s0 + j];
-            b = perm[s_t0[i][o] + k];
+            b = perm[sg.x + k];
         }
         const int64_t pos = pos0 + q;
         if (pos < cap) {
@@ -307,7 +405,10 @@ struct WallIO {
         *sc = &scenes[s];
         const int64_t np = (*sc)->n_active - (*sc)->n_walls;
         const int64_t tl = t - wt_off[s];
-        const int64_t k = tl / np, j = tl - k * np;
+        int64_t k;
+        if ((tl >> 32) == 0) k = (uint32_t)tl / (uint32_t)np;  // the common case: a 32-bit division
+        else k = tl / np;
+        const int64_t j = tl - k * np;
         *rk = (*sc)->row0 + k;
         *rj = (*sc)->row0 + (*sc)->n_walls + j;
     }
@@ -330,10 +431,10 @@ struct WallIO {
     __device__ __forceinline__ void finish(int64_t) const {}
 };
 
-// Cell heads over the sorted keys fused with their numbering: item s starts
-// a cell when its key differs from the previous one; the cell's index is
-// its offset. Writes the cell's start, key, the first row's flag and the
-// key -> cell table; the last entry of cell_start closes the list.
+// Sorted path: cell heads over the sorted keys fused with their numbering:
+// item s starts a cell when its key differs from the previous one; the
+// cell's index is its offset. Writes the cell's start, key and the first
+// row's flag; the last entry of cell_start closes the list.
 struct HeadIO {
     const uint32_t* keys;
     const int32_t* vals;
@@ -341,7 +442,6 @@ struct HeadIO {
     int32_t* cell_start;
     uint32_t* cell_key;
     int32_t* row_first;
-    int32_t* cell_of;
     __device__ __forceinline__ int32_t load(int64_t s) const {
         return (s == 0 || keys[s] != keys[s - 1]) ? 1 : 0;
     }
@@ -350,15 +450,59 @@ struct HeadIO {
         cell_start[c] = (int32_t)s;
         cell_key[c] = keys[s];
         row_first[vals[s]] = 1;
-        if (cell_of) cell_of[keys[s]] = (int32_t)c;
     }
     __device__ __forceinline__ void finish(int64_t n_cells) const { cell_start[n_cells] = (int32_t)M; }
 };
 
-// Scan of the per-cell pair counts; its total closes the candidate count
-// (counters 1 wall pairs + 2 cell pairs -> 3 capped, 7 uncapped) and
-// restarts the counters of the fine-contact stage that consumes the pairs.
-struct CellCountIO {
+// Key-table path: one scan over rows numbers the cells by the rank of their
+// lowest row and places their member ranges. Row r counts (1 cell, n
+// members of its key) when it is its key's lowest row, else nothing; the two
+// counts travel packed as cells << 32 | members, so the exclusive offset of
+// such a row gives both its cell's rank and its first member slot. The store
+// also resets the key's lowest row for the next step.
+struct CellTableIO {
+    const int32_t* row_key;
+    const int32_t* key_cnt;
+    int32_t* key_min;
+    int32_t* cell_start;
+    uint32_t* cell_key;
+    int32_t* cell_of;
+    int64_t* n_cells;
+    __device__ __forceinline__ int64_t load(int64_t r) const {
+        const int32_t k = row_key[r];
+        if (k < 0 || key_min[k] != (int32_t)r) return 0;
+        return (int64_t(1) << 32) | (int64_t)key_cnt[k];
+    }
+    __device__ __forceinline__ void store(int64_t r, int64_t off, int64_t v) const {
+        if (!v) return;
+        const int32_t k = row_key[r];
+        const int64_t c = off >> 32;
+        cell_start[c] = (int32_t)(off & 0xffffffffll);
+        cell_key[c] = (uint32_t)k;
+        cell_of[k] = (int32_t)c;
+        key_min[k] = KEY_MIN_EMPTY;
+    }
+    __device__ __forceinline__ void finish(int64_t total) const {
+        *n_cells = total >> 32;
+        cell_start[total >> 32] = (int32_t)(total & 0xffffffffll);
+    }
+};
+
+// Scan of the per-cell pair counts (cells by rank); its total closes the
+// candidate count (counters 1 wall pairs + 2 cell pairs -> 3 capped, 7
+// uncapped) and restarts the counters of the fine-contact stage that
+// consumes the pairs.
+__device__ __forceinline__ void close_pairs(int64_t* counters, int64_t cell_pairs, int64_t cap, uint32_t* flags) {
+    const int64_t P = counters[1] + cell_pairs;
+    counters[3] = P < cap ? P : cap;
+    counters[7] = P;
+    counters[8] = 0;   // simplex slots of the fine-contact stage
+    counters[10] = 0;  // its sphere-test survivors
+    counters[11] = 0;  // and its bucket overflow records
+    if (P > cap) atomicOr(flags, MWD_F_PAIR_OVERFLOW);
+}
+
+struct CellCountIO {  // sorted path: counts written by k_cell_info
     const int32_t* cnt;
     int64_t* off;
     int64_t* counters;
@@ -366,36 +510,39 @@ struct CellCountIO {
     uint32_t* flags;
     __device__ __forceinline__ int32_t load(int64_t i) const { return cnt[i]; }
     __device__ __forceinline__ void store(int64_t i, int64_t o, int32_t) const { off[i] = o; }
-    __device__ __forceinline__ void finish(int64_t total) const {
-        counters[6] = counters[0];  // cells of this step, cleared from the key table next step
-        const int64_t P = counters[1] + total;
-        counters[3] = P < cap ? P : cap;
-        counters[7] = P;
-        counters[8] = 0;   // simplex slots of the fine-contact stage
-        counters[10] = 0;  // its sphere-test survivors
-        counters[11] = 0;  // and its bucket overflow records
-        if (P > cap) atomicOr(flags, MWD_F_PAIR_OVERFLOW);
-    }
+    __device__ __forceinline__ void finish(int64_t total) const { close_pairs(counters, total, cap, flags); }
 };
 
-// Before the cells of this step: clear the row flags and forget the previous
-// step's cells in the key -> cell table.
-__global__ void k_cells_prep(int32_t* __restrict__ row_first, int64_t n_rows, const uint32_t* __restrict__ cell_key,
-                             const int64_t* __restrict__ n_prev, int32_t* __restrict__ cell_of) {
+struct CellInfoIO {  // key-table path: the per-cell work runs as the scan's load
+    int32_t* members;
+    const int32_t* cell_start;
+    const uint32_t* cell_key;
+    const int32_t* cell_of;
+    const int32_t* scene_of;
+    const SceneDev* scenes;
+    int32_t* perm;
+    int2* cell_self;
+    int2* cell_seg;
+    int64_t* off;
+    int64_t* counters;
+    int64_t cap;
+    uint32_t* flags;
+    __device__ __forceinline__ int32_t load(int64_t c) const {
+        return cell_work(c, c, false, members, cell_start, cell_key, scene_of, scenes, perm, cell_self, cell_seg,
+                         flags, NbTable{cell_of, cell_key, counters[0]});
+    }
+    __device__ __forceinline__ void store(int64_t i, int64_t o, int32_t) const { off[i] = o; }
+    __device__ __forceinline__ void finish(int64_t total) const { close_pairs(counters, total, cap, flags); }
+};
+
+// Sorted path, before the cells of this step: clear the row flags.
+__global__ void k_cells_prep(int32_t* __restrict__ row_first, int64_t n_rows) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n_rows) row_first[i] = 0;
-    if (cell_of && i < *n_prev) cell_of[cell_key[i]] = -1;
 }
 
 __global__ void k_finish_pairs(int64_t* counters, int64_t cap, uint32_t* flags) {
-    counters[6] = counters[0];  // cells of this step, cleared from the key table next step
-    const int64_t P = counters[1] + counters[2];
-    counters[3] = P < cap ? P : cap;
-    counters[7] = P;
-    counters[8] = 0;   // simplex slots of the fine-contact stage that consumes these pairs
-    counters[10] = 0;  // its sphere-test survivors
-    counters[11] = 0;  // and its bucket overflow records
-    if (P > cap) atomicOr(flags, MWD_F_PAIR_OVERFLOW);
+    close_pairs(counters, counters[2], cap, flags);
 }
 
 __global__ void k_zero_i64(int64_t* p, int n) {
@@ -437,38 +584,54 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
     // -- linked-cell pairs
     if (M > 0) {
         mwd_mark(ws, 1);
-        hipLaunchKernelGGL(k_sort_keys, dim3(blocks_for(M)), dim3(256), 0, s, st->com, keys_in, ws->plist, M,
-                           ws->scene_of, ws->scenes_dev, ws->keys, ws->vals, ws->flags);
-        MWD_LAUNCH_CHECK();
-        uint32_t* ks;
-        int32_t* vs;
-        rc = mwd_radix_sort(ws->keys, ws->vals, ws->keys_alt, ws->vals_alt, M, nullptr, ws->key_bits, ws->hist,
-                            ws->hist_off, ctr + 9, ws->scan_tmp, ws->scan_tmp_len, s, &ks, &vs);
-        if (rc) return rc;
-        ws->keys_sorted = ks;
-        ws->vals_sorted = vs;
-        mwd_mark(ws, 2);
-        const int64_t prep = ws->n_rows > M ? ws->n_rows : M;
-        hipLaunchKernelGGL(k_cells_prep, dim3(blocks_for(prep)), dim3(256), 0, s, ws->row_first, ws->n_rows,
-                           ws->cell_key, ctr + 6, ws->cell_of);
-        MWD_LAUNCH_CHECK();
-        rc = mwd_scan_io(HeadIO{ks, vs, M, ws->cell_start, ws->cell_key, ws->row_first, ws->cell_of}, M, nullptr,
-                         ctr + 0, ws->scan_tmp, ws->scan_tmp_len, s);
-        if (rc) return rc;
-        rc = mwd_scan_i32(ws->row_first, ws->row_rank, nullptr, ws->n_rows, nullptr, ws->scan_tmp,
-                          ws->scan_tmp_len, s);
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_cell_info, dim3(blocks_for(M)), dim3(256), 0, s, vs, ws->cell_start,
-                           ws->cell_key, ctr + 0, M, ws->scene_of, ws->scenes_dev, ws->row_rank, ws->perm,
-                           ws->cell_nb, ws->cell_by_rank, ws->cell_cnt, ws->flags, ws->cell_of);
-        MWD_LAUNCH_CHECK();
-        rc = mwd_scan_io(CellCountIO{ws->cell_cnt, ws->cell_off, ctr, ws->pair_cap, ws->flags}, M, ctr + 0,
-                         ctr + 2, ws->scan_tmp, ws->scan_tmp_len, s);
-        if (rc) return rc;
+        if (ws->cell_table) {
+            hipLaunchKernelGGL(k_key_count, dim3(blocks_for(M)), dim3(256), 0, s, st->com, keys_in, ws->plist, M,
+                               ws->scene_of, ws->scenes_dev, ws->row_key, ws->key_cnt, ws->key_min, ws->flags);
+            MWD_LAUNCH_CHECK();
+            mwd_mark(ws, 2);
+            rc = mwd_scan_io(CellTableIO{ws->row_key, ws->key_cnt, ws->key_min, ws->cell_start, ws->cell_key,
+                                         ws->cell_of, ctr + 0},
+                             ws->n_rows, nullptr, nullptr, ws->scan_tmp, ws->scan_tmp_len, s);
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_key_scatter, dim3(blocks_for(M)), dim3(256), 0, s, ws->plist, M, ws->row_key,
+                               ws->key_cnt, ws->cell_of, ws->cell_start, ws->vals);
+            MWD_LAUNCH_CHECK();
+            rc = mwd_scan_io<CellInfoIO, 1>(CellInfoIO{ws->vals, ws->cell_start, ws->cell_key, ws->cell_of, ws->scene_of,
+                                        ws->scenes_dev, ws->perm, ws->cell_self, ws->cell_seg, ws->cell_off, ctr,
+                                        ws->pair_cap, ws->flags},
+                             M, ctr + 0, ctr + 2, ws->scan_tmp, ws->scan_tmp_len, s);
+            if (rc) return rc;
+        } else {
+            hipLaunchKernelGGL(k_sort_keys, dim3(blocks_for(M)), dim3(256), 0, s, st->com, keys_in, ws->plist, M,
+                               ws->scene_of, ws->scenes_dev, ws->keys, ws->vals, ws->flags);
+            MWD_LAUNCH_CHECK();
+            uint32_t* ks;
+            int32_t* vs;
+            rc = mwd_radix_sort(ws->keys, ws->vals, ws->keys_alt, ws->vals_alt, M, nullptr, ws->key_bits, ws->hist,
+                                ws->hist_off, ctr + 9, ws->scan_tmp, ws->scan_tmp_len, s, &ks, &vs);
+            if (rc) return rc;
+            mwd_mark(ws, 2);
+            hipLaunchKernelGGL(k_cells_prep, dim3(blocks_for(ws->n_rows)), dim3(256), 0, s, ws->row_first,
+                               ws->n_rows);
+            MWD_LAUNCH_CHECK();
+            rc = mwd_scan_io(HeadIO{ks, vs, M, ws->cell_start, ws->cell_key, ws->row_first}, M, nullptr, ctr + 0,
+                             ws->scan_tmp, ws->scan_tmp_len, s);
+            if (rc) return rc;
+            rc = mwd_scan_i32(ws->row_first, ws->row_rank, nullptr, ws->n_rows, nullptr, ws->scan_tmp,
+                              ws->scan_tmp_len, s);
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_cell_info, dim3(blocks_for(M)), dim3(256), 0, s, vs, ws->cell_start,
+                               ws->cell_key, ctr + 0, ws->scene_of, ws->scenes_dev, ws->row_rank, ws->perm,
+                               ws->cell_self, ws->cell_seg, ws->cell_cnt, ws->flags);
+            MWD_LAUNCH_CHECK();
+            rc = mwd_scan_io(CellCountIO{ws->cell_cnt, ws->cell_off, ctr, ws->pair_cap, ws->flags}, M, ctr + 0,
+                             ctr + 2, ws->scan_tmp, ws->scan_tmp_len, s);
+            if (rc) return rc;
+        }
         mwd_mark(ws, 3);
         MWD_LAUNCH_EV(mwd_kev(ws, MWD_KT_EMIT, 0), mwd_kev(ws, MWD_KT_EMIT, 1), k_cell_fill,
                       dim3((unsigned)((M + FILL_CELLS - 1) / FILL_CELLS)), dim3(FILL_THREADS), 0, s, ws->perm,
-                      ws->cell_start, ws->cell_nb, ws->cell_by_rank, ws->cell_off, ctr, ws->pairs, ws->pair_cap);
+                      ws->cell_self, ws->cell_seg, ws->cell_off, ctr, ws->pairs, ws->pair_cap);
         MWD_LAUNCH_CHECK();
     }
     mwd_mark(ws, 4);

@@ -10,7 +10,6 @@
 #include "mwd_math.h"
 
 #define MWD_WAVE 64
-#define MWD_GJK_GROUP 8  // lanes that share one pair's support map in GJK
 #define MWD_TIMING_RING 512
 #define MWD_EV_PER_STEP (MWD_N_STAGES + 1 + 2 * MWD_N_KTIMED)
 // timed kernels
@@ -99,6 +98,13 @@ struct BodyTan {
     TArr<MWD_NT> com, verts, vel, angvel, quat;
 };
 
+// Launch shapes of the pair tests (results do not depend on them).
+struct PairTune {
+    int gjk_group = 8;       // lanes sharing one pair's support map: 4, 8 or 16
+    int gjk_blocks = 1024;   // grid cap of the grid-strided GJK launch
+    int sphere_block = 256;  // threads per block of the sphere test
+};
+
 // --------------------------------------------------------------- workspace
 struct mwd_workspace {
     int device;
@@ -120,16 +126,23 @@ struct mwd_workspace {
     // coarse stage
     uint32_t *keys, *keys_alt; // [M]
     int32_t *vals, *vals_alt;  // [M]
-    int32_t* cell_start;       // [M+1]
-    uint32_t* cell_key;        // [M]
-    int32_t* cell_nb;          // [M*MWD_MAX_OFF]
-    int32_t* row_first;        // [n_rows]
-    int64_t* row_rank;         // [n_rows]
-    int32_t* cell_by_rank;     // [M]
+    int32_t* cell_start;       // [M+1] first member slot of each cell (cell index order)
+    uint32_t* cell_key;        // [M] key of each cell
+    int2* cell_self;           // [M] by rank: (first member slot, members) of the cell
+    int2* cell_seg;            // [M*MWD_MAX_OFF] by rank: (first slot, members) of each neighbour cell, (0,0) if none
+    int32_t* row_first;        // [n_rows] (sorted path) 1 where a row is the lowest of its cell
+    int64_t* row_rank;         // [n_rows] (sorted path) exclusive scan of row_first
     int32_t* cell_cnt;         // [M] pair count by rank
     int64_t* cell_off;         // [M]
     int32_t* perm;             // [M] members of each cell in set order
-    int32_t* cell_of;          // [key_span_total] cell index of a sorted key, -1 if empty (NULL: binary search)
+    // key-table path (cell_table = 1): cells numbered by rank directly, no sort
+    int cell_table;            // key-table path in use (MWD_CELLS_TABLE)
+    int cell_table_ok;         // its tables exist
+    PairTune tune;
+    int32_t* cell_of;          // [key_span_total] cell of a key; valid only where cell_key[cell_of[k]] == k
+    int32_t* key_cnt;          // [key_span_total] members per key, zero between steps
+    int32_t* key_min;          // [key_span_total] lowest row per key, INT32_MAX between steps
+    int32_t* row_key;          // [n_rows] key of each moving row this step, -1 for walls
     // candidate pairs
     int32_t* pairs;            // [pair_cap*2]
     int32_t* collision;        // [pair_cap]
@@ -149,15 +162,13 @@ struct mwd_workspace {
     int32_t* rec_bkt;          // [n_rows * MWD_ROW_CONTACTS]
     int32_t* rec_ovf;          // [2 * 2*contact_cap] (row, record) of records beyond a full bucket; count in counters[11]
     int bkt_dirty;             // host: buckets filled and not yet consumed
-    uint32_t* keys_sorted;            // whichever of keys / keys_alt holds the sorted cell keys
-    int32_t* vals_sorted;
     // sort / scan scratch
     int32_t* hist;             // [256 * max_tiles]
     int64_t* hist_off;         // [256 * max_tiles]
     int64_t* scan_tmp;         // block sums
     int64_t scan_tmp_len;
     // counters (device): 0 n_cells, 1 n_wall_pairs, 2 n_cell_pairs, 3 n_pairs
-    // (capped), 4 n_contacts, 5 n_records, 6 n_cells of the previous step,
+    // (capped), 4 n_contacts, 5 n_records, 6 unused,
     // 7 n_pairs before capping, 8 simplex slots claimed this step, 9 radix
     // histogram entries (scratch), 10 sphere-test survivors, 11 bucket
     // overflow records
@@ -236,11 +247,11 @@ int mwd_launch_cell_keys(const double* com, int64_t n, const double* cell, int64
 // simplex slot (counter *slots, zero on entry) and stores it in sidx.
 int mwd_launch_sphere(const BodyView& A, const BodyView& B, int64_t n_cap, const int64_t* n_dev, int32_t* collision,
                       int32_t* surv, int64_t* n_surv, hipStream_t s, hipEvent_t e0 = nullptr,
-                      hipEvent_t e1 = nullptr);
+                      hipEvent_t e1 = nullptr, const PairTune& tu = PairTune());
 int mwd_launch_gjk_surv(const BodyView& A, const BodyView& B, int64_t n_cap, int32_t* collision,
                         const int32_t* surv, const int64_t* n_surv, Simplex* simplex, int32_t* sidx, int64_t* slots,
                         int64_t slot_cap, uint32_t* flags, hipStream_t s, hipEvent_t e0 = nullptr,
-                        hipEvent_t e1 = nullptr);
+                        hipEvent_t e1 = nullptr, const PairTune& tu = PairTune());
 // Both of the above.
 int mwd_launch_gjk(const BodyView& A, const BodyView& B, int64_t n_cap, const int64_t* n_dev,
                    int32_t* collision, int32_t* surv, int64_t* n_surv, Simplex* simplex,

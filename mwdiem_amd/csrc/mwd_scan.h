@@ -18,12 +18,16 @@
 // nothing to clear between calls.
 //
 // IO is a small struct with
-//   __device__ int32_t load(int64_t i) const;             count of item i
-//   __device__ void store(int64_t i, int64_t off, int32_t v) const;
+//   __device__ V load(int64_t i) const;                   count of item i
+//   __device__ void store(int64_t i, int64_t off, V v) const;
 //                                                         its exclusive offset
 //   __device__ void finish(int64_t total) const;          once, with the total
-// load/store are called for the live items i < n only.
+// load/store are called for the live items i < n only. V is int32_t or
+// int64_t (an int64 count may pack two counts, as hi << 32 | lo, whose sums
+// stay below 2^62).
 #pragma once
+#include <type_traits>
+#include <utility>
 #include "mwd_internal.h"
 
 namespace {
@@ -87,12 +91,13 @@ __device__ __forceinline__ int scan_pad(int i) { return i + (i >> 5); }
 template <class IO, int SCAN_ITEMS>
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_io(IO io, int64_t n_cap, const int64_t* n_dev, int64_t* total,
                                                           uint64_t* ctr, uint64_t* status2, int64_t S) {
+    using V = std::decay_t<decltype(io.load(int64_t(0)))>;
     constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
     constexpr int SCAN_LDS = SCAN_TILE + SCAN_TILE / 32;
     __shared__ int64_t lw[SCAN_WAVES];
     __shared__ int64_t s_prefix;
     __shared__ uint64_t s_claim;
-    __shared__ int32_t s_v[SCAN_LDS];
+    __shared__ V s_v[SCAN_LDS];
     __shared__ int64_t s_off[SCAN_LDS];
     const int64_t n = live_n(n_cap, n_dev);
     int64_t ntl = (n + SCAN_TILE - 1) / SCAN_TILE;
@@ -118,7 +123,7 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_io(IO io, int64_t n_cap, 
             s_v[scan_pad(il)] = (i < n) ? io.load(i) : 0;
         }
         __syncthreads();
-        int32_t v[SCAN_ITEMS];
+        V v[SCAN_ITEMS];
         int64_t acc = 0;
 #pragma unroll
         for (int k = 0; k < SCAN_ITEMS; ++k) {  // blocked values
@@ -192,11 +197,14 @@ struct ScanIO {
     __device__ __forceinline__ void finish(int64_t) const {}
 };
 
-template <class IO>
+// ITEMS: items per thread; 0 picks by length (16 for long scans, 4 below a
+// million items). An IO whose load does much latency-bound work per item
+// asks for 1, so that each thread has one item and more blocks run.
+template <class IO, int ITEMS = 0>
 int mwd_scan_io(const IO& io, int64_t n_cap, const int64_t* n_dev, int64_t* total, int64_t* tmp, int64_t tmp_len,
                 hipStream_t s) {
-    const bool shrt = n_cap <= SCAN_SHORT_MAX;
-    const int64_t tile = SCAN_THREADS * (shrt ? SCAN_ITEMS_SHORT : SCAN_ITEMS_LONG);
+    const int items = ITEMS > 0 ? ITEMS : (n_cap <= SCAN_SHORT_MAX ? SCAN_ITEMS_SHORT : SCAN_ITEMS_LONG);
+    const int64_t tile = (int64_t)SCAN_THREADS * items;
     const int64_t nb = n_cap > 0 ? (n_cap + tile - 1) / tile : 1;
     // scratch layout: [0] call/ticket word, then two status buffers of S words
     const int64_t S = (tmp_len - 1) / 2;
@@ -206,12 +214,18 @@ int mwd_scan_io(const IO& io, int64_t n_cap, const int64_t* n_dev, int64_t* tota
     }
     uint64_t* ctr = reinterpret_cast<uint64_t*>(tmp);
     uint64_t* st2 = reinterpret_cast<uint64_t*>(tmp + 1);
-    if (shrt)
-        hipLaunchKernelGGL((k_scan_io<IO, SCAN_ITEMS_SHORT>), dim3((unsigned)nb), dim3(SCAN_THREADS), 0, s, io,
-                           n_cap > 0 ? n_cap : 0, n_dev, total, ctr, st2, S);
-    else
-        hipLaunchKernelGGL((k_scan_io<IO, SCAN_ITEMS_LONG>), dim3((unsigned)nb), dim3(SCAN_THREADS), 0, s, io,
-                           n_cap > 0 ? n_cap : 0, n_dev, total, ctr, st2, S);
+    const int64_t n = n_cap > 0 ? n_cap : 0;
+    if constexpr (ITEMS > 0) {
+        hipLaunchKernelGGL((k_scan_io<IO, ITEMS>), dim3((unsigned)nb), dim3(SCAN_THREADS), 0, s, io, n, n_dev, total,
+                           ctr, st2, S);
+    } else {
+        if (items == SCAN_ITEMS_SHORT)
+            hipLaunchKernelGGL((k_scan_io<IO, SCAN_ITEMS_SHORT>), dim3((unsigned)nb), dim3(SCAN_THREADS), 0, s, io, n,
+                               n_dev, total, ctr, st2, S);
+        else
+            hipLaunchKernelGGL((k_scan_io<IO, SCAN_ITEMS_LONG>), dim3((unsigned)nb), dim3(SCAN_THREADS), 0, s, io, n,
+                               n_dev, total, ctr, st2, S);
+    }
     MWD_LAUNCH_CHECK();
     return MWD_OK;
 }

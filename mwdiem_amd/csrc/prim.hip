@@ -128,7 +128,7 @@ __global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
 }  // namespace
 
 int64_t mwd_scan_tmp_len(int64_t n_cap) {
-    const int64_t smallest_tile = SCAN_THREADS * SCAN_ITEMS_SHORT;
+    const int64_t smallest_tile = SCAN_THREADS;  // one item per thread
     return 2 * ((n_cap + smallest_tile - 1) / smallest_tile + 1) + 1;
 }
 int64_t mwd_sort_tiles(int64_t n_cap) { return (n_cap + SORT_TILE - 1) / SORT_TILE; }

@@ -199,6 +199,23 @@ class Stepper:
             else:
                 self.step(n - done)
 
+    def tuning(self) -> dict:
+        """Launch choices of the workspace (cell_pass, gjk_group, gjk_blocks, sphere_block)."""
+        t = _lib.Tuning()
+        _lib.check(_lib.load().mwd_get_tuning(self.ws, C.byref(t)), "mwd_get_tuning")
+        return {name: int(getattr(t, name)) for name, _ in _lib.Tuning._fields_}
+
+    def set_tuning(self, **choices):
+        """Change launch choices for later steps; results are unchanged bit for
+        bit. Graphs captured before keep their launches, so they are dropped."""
+        cur = self.tuning()
+        unknown = set(choices) - set(cur)
+        if unknown:
+            raise ValueError(f"unknown tuning keys {sorted(unknown)}")
+        cur.update(choices)
+        _lib.check(_lib.load().mwd_set_tuning(self.ws, C.byref(_lib.Tuning(**cur))), "mwd_set_tuning")
+        self._graph = None
+
     def set_timing(self, on: bool = True):
         _lib.check(_lib.load().mwd_set_timing(self.ws, 1 if on else 0), "mwd_set_timing")
 

@@ -50,3 +50,10 @@ def impact_scene():
     """60 deposited hulls (touching, not overlapping) on the physical law."""
     from mwdiem_amd import scene
     return scene.make_scene(60, (6.0, 6.0, 16.0), 21, law=scene.Law4(mode=scene.PHYSICAL), name="impact-60")
+
+
+@pytest.fixture(scope="session")
+def large_scene():
+    """The 100,000-hull settle scene (8-20 hull points, 200x200x400 mm box)."""
+    from mwdiem_amd import scene
+    return scene.config_large()

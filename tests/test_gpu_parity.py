@@ -93,6 +93,124 @@ def test_candidates_identical(gpu, small_scene, dense_scene, dense_phys, impact_
     assert s["n_wall_pairs"] == nw
 
 
+def _crowded(sc, factor=2.0):
+    """The scene on a grid of cells `factor` times wider, so cells hold many
+    members (the set-order emulation beyond four members, sorting of the
+    members claimed in any order)."""
+    from dataclasses import replace
+    from mwdiem_amd import scene
+    cell, ncx, ncy, ncz, kmin, kspan = scene.grid_for(sc.box, factor * sc.max_particlesize)
+    return replace(sc.copy(), cell=cell, ncx=ncx, ncy=ncy, ncz=ncz, kmin=kmin, kspan=kspan)
+
+
+@pytest.mark.parametrize("which", ["dense", "dense_phys", "crowded", "crowded_phys"])
+def test_cell_passes_agree(gpu, dense_scene, dense_phys, which):
+    """Cells from the key tables (the default) and from the radix sort of the
+    keys give the same candidate list, order included, step after step while
+    particles change cells; the first list also matches the oracle."""
+    from mwdiem_amd import _lib
+    sc = {"dense": dense_scene, "dense_phys": dense_phys, "crowded": _crowded(dense_scene),
+          "crowded_phys": _crowded(dense_phys)}[which]
+    st = random_motion(O.initial_state(sc), sc, 21, vscale=3000.0, wscale=300.0)
+    A, B = _stepper(sc), _stepper(sc)
+    assert A.tuning()["cell_pass"] == _lib.CELLS_TABLE
+    B.set_tuning(cell_pass=_lib.CELLS_SORTED)
+    assert B.tuning()["cell_pass"] == _lib.CELLS_SORTED
+    A.set_state(st)
+    B.set_state(st)
+    A.candidates()
+    assert np.array_equal(A.pairs(), O.candidates(st, sc))
+    if which.startswith("crowded"):
+        assert A.status()["n_cells"] * 4 < sc.n_particles  # many members per cell
+    # the same workspaces on shifted copies of the bed: every particle changes
+    # cell, so the key tables are reused with other keys occupied
+    W = sc.n_walls
+    rng = np.random.Generator(np.random.PCG64(7))
+    for i in range(4):
+        d = sc.cell * rng.uniform(-0.5, 0.5, size=3)
+        si = {k: v.copy() for k, v in st.items()}
+        si["com"][W:] += d
+        si["verts"][W:] += d
+        A.set_state(si)
+        B.set_state(si)
+        A.candidates()
+        B.candidates()
+        sa, sb = A.check(), B.check()
+        assert sa["n_cells"] == sb["n_cells"], i
+        assert np.array_equal(A.pairs(), B.pairs()), i
+        if i == 0:
+            assert np.array_equal(A.pairs(), O.candidates(si, sc))
+    for i in range(8):
+        A.step(1)
+        B.step(1)
+        A.candidates()
+        B.candidates()
+        sa, sb = A.check(), B.check()
+        assert sa["n_cells"] == sb["n_cells"], i
+        assert np.array_equal(A.pairs(), B.pairs()), i
+    a, b = A.numpy_state(), B.numpy_state()
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_large_scene_cells_and_candidates(gpu, large_scene):
+    """At full size (100,000 hulls, 890k candidates): the candidate list of
+    the key-table cells equals the oracle's, order included, and the sorted
+    cells give the same list and the same state after 10 steps."""
+    from mwdiem_amd import _lib
+    sc = large_scene
+    st = O.initial_state(sc)
+    A = _stepper(sc)
+    A.candidates()
+    A.check()
+    ref = O.candidates(st, sc)
+    got = A.pairs()
+    assert got.shape == ref.shape and np.array_equal(got, ref)
+    B = _stepper(sc)
+    B.set_tuning(cell_pass=_lib.CELLS_SORTED)
+    B.candidates()
+    B.check()
+    assert np.array_equal(B.pairs(), ref)
+    A.step(10)
+    B.step(10)
+    A.check()
+    B.check()
+    assert np.array_equal(A.pairs(), B.pairs())
+    a, b = A.numpy_state(), B.numpy_state()
+    for k in ("com", "q", "v", "w", "verts"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_tuning_choices_change_nothing(gpu, dense_phys):
+    """Every launch choice of the tuning interface gives the same state, bit
+    for bit; out-of-range choices are refused."""
+    from mwdiem_amd import _lib
+    sc = dense_phys
+    st = random_motion(O.initial_state(sc), sc, 5, vscale=2000.0, wscale=200.0)
+    ref = _stepper(sc)
+    ref.set_state(st)
+    ref.step(20)
+    ref.check()
+    want = ref.numpy_state()
+    assert ref.status()["n_contacts"] > 0
+    variants = [dict(gjk_group=4), dict(gjk_group=16), dict(gjk_blocks=64), dict(gjk_blocks=8192),
+                dict(sphere_block=64), dict(sphere_block=1024), dict(cell_pass=_lib.CELLS_SORTED, gjk_group=16)]
+    for v in variants:
+        S = _stepper(sc)
+        S.set_tuning(**v)
+        S.set_state(st)
+        S.step(20)
+        S.check()
+        got = S.numpy_state()
+        for k in want:
+            assert np.array_equal(got[k], want[k]), (v, k)
+        S.close()
+    S = _stepper(sc)
+    for bad in (dict(gjk_group=5), dict(gjk_blocks=8), dict(sphere_block=100), dict(cell_pass=7)):
+        with pytest.raises(_lib.MwdError):
+            S.set_tuning(**bad)
+
+
 @pytest.mark.parametrize("seed,phys", [(3, False), (4, False), (3, True)])
 def test_fine_contact_matches_oracle(gpu, dense_scene, dense_phys, seed, phys):
     sc = dense_phys if phys else dense_scene
