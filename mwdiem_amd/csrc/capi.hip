@@ -48,11 +48,14 @@ void free_ws(mwd_workspace* ws) {
                     ws->pairs, ws->collision, ws->simplex, ws->sidx, ws->surv, ws->contact_pair,
                     ws->depth, ws->normal, ws->cpA, ws->cpB, ws->Fn, ws->tA, ws->tB, ws->epa_status, ws->rec_cnt,
                     ws->rec_bkt, ws->rec_ovf, ws->hist,
-                    ws->hist_off, ws->scan_tmp, ws->counters, ws->flags, ws->laws, ws->scene_wt_off, ws->ctan,
-                    ws->law_tans};
+                    ws->hist_off, ws->scan_tmp, ws->scan_tmp_side, ws->counters, ws->flags, ws->laws,
+                    ws->scene_wt_off, ws->ctan, ws->law_tans};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     destroy_events(ws);
+    if (ws->ev_fork) (void)hipEventDestroy(ws->ev_fork);
+    if (ws->ev_join) (void)hipEventDestroy(ws->ev_join);
+    if (ws->side) (void)hipStreamDestroy(ws->side);
     delete[] ws->scenes_host;
     delete ws;
 }
@@ -399,6 +402,8 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     A(hist, hist_len);
     A(hist_off, hist_len);
     A(scan_tmp, ws->scan_tmp_len);
+    ws->scan_tmp_side_len = mwd_scan_tmp_len(std::max<int64_t>(ws->n_wall_tests, 1));
+    A(scan_tmp_side, ws->scan_tmp_side_len);
     A(counters, 16);
     A(flags, 1);
     A(laws, 2 * n_scenes);
@@ -427,9 +432,18 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
         hipMemsetAsync(ws->counters, 0, sizeof(int64_t) * 16, s) != hipSuccess ||
         hipMemsetAsync(ws->rec_cnt, 0, sizeof(int32_t) * n_rows, s) != hipSuccess ||
         hipMemsetAsync(ws->scan_tmp, 0, sizeof(int64_t) * ws->scan_tmp_len, s) != hipSuccess ||
+        hipMemsetAsync(ws->scan_tmp_side, 0, sizeof(int64_t) * ws->scan_tmp_side_len, s) != hipSuccess ||
         hipMemsetAsync(ws->flags, 0, sizeof(uint32_t), s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
         mwd_set_error("mwd_workspace_create: initialisation failed");
+        free_ws(ws);
+        return MWD_E_HIP;
+    }
+    // side stream (default priority) and the events that fork to it and join back
+    if (hipStreamCreateWithFlags(&ws->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ws->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ws->ev_join, hipEventDisableTiming) != hipSuccess) {
+        mwd_set_error("mwd_workspace_create: side stream set-up failed");
         free_ws(ws);
         return MWD_E_HIP;
     }

@@ -574,13 +574,28 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
         hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(64), 0, s, ctr, 4);
         MWD_LAUNCH_CHECK();
     }
-    // -- wall slab pairs: test, count and write in one pass
+    // -- wall slab pairs: test, count and write in one pass, on the side
+    // stream; the cell pass runs meanwhile and joins before it needs the wall
+    // pair count (the offset of the cell pairs)
+    bool joined = true;
     if (walls) {
         const WallIO io{st->com, st->wall_n, st->wall_c, ws->scenes_dev, ws->n_scenes, ws->scene_wt_off,
                         ws->pairs, ws->pair_cap};
-        rc = mwd_scan_io(io, ws->n_wall_tests, nullptr, ctr + 1, ws->scan_tmp, ws->scan_tmp_len, s);
+        MWD_HIP_CHECK(hipEventRecord(ws->ev_fork, s));
+        MWD_HIP_CHECK(hipStreamWaitEvent(ws->side, ws->ev_fork, 0));
+        rc = mwd_scan_io(io, ws->n_wall_tests, nullptr, ctr + 1, ws->scan_tmp_side, ws->scan_tmp_side_len,
+                         ws->side);
         if (rc) return rc;
+        MWD_HIP_CHECK(hipEventRecord(ws->ev_join, ws->side));
+        joined = false;
     }
+    auto join = [&]() -> int {
+        if (!joined) {
+            MWD_HIP_CHECK(hipStreamWaitEvent(s, ws->ev_join, 0));
+            joined = true;
+        }
+        return MWD_OK;
+    };
     // -- linked-cell pairs
     if (M > 0) {
         mwd_mark(ws, 1);
@@ -596,6 +611,7 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
             hipLaunchKernelGGL(k_key_scatter, dim3(blocks_for(M)), dim3(256), 0, s, ws->plist, M, ws->row_key,
                                ws->key_cnt, ws->cell_of, ws->cell_start, ws->vals);
             MWD_LAUNCH_CHECK();
+            if ((rc = join())) return rc;
             rc = mwd_scan_io<CellInfoIO, 1>(CellInfoIO{ws->vals, ws->cell_start, ws->cell_key, ws->cell_of, ws->scene_of,
                                         ws->scenes_dev, ws->perm, ws->cell_self, ws->cell_seg, ws->cell_off, ctr,
                                         ws->pair_cap, ws->flags},
@@ -624,6 +640,7 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
                                ws->cell_key, ctr + 0, ws->scene_of, ws->scenes_dev, ws->row_rank, ws->perm,
                                ws->cell_self, ws->cell_seg, ws->cell_cnt, ws->flags);
             MWD_LAUNCH_CHECK();
+            if ((rc = join())) return rc;
             rc = mwd_scan_io(CellCountIO{ws->cell_cnt, ws->cell_off, ctr, ws->pair_cap, ws->flags}, M, ctr + 0,
                              ctr + 2, ws->scan_tmp, ws->scan_tmp_len, s);
             if (rc) return rc;
@@ -634,6 +651,7 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
                       ws->cell_self, ws->cell_seg, ws->cell_off, ctr, ws->pairs, ws->pair_cap);
         MWD_LAUNCH_CHECK();
     }
+    if ((rc = join())) return rc;
     mwd_mark(ws, 4);
     if (M <= 0) {  // no cell-count scan closed the candidate count
         hipLaunchKernelGGL(k_finish_pairs, dim3(1), dim3(1), 0, s, ctr, ws->pair_cap, ws->flags);

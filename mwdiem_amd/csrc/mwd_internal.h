@@ -167,6 +167,11 @@ struct mwd_workspace {
     int64_t* hist_off;         // [256 * max_tiles]
     int64_t* scan_tmp;         // block sums
     int64_t scan_tmp_len;
+    // the wall slab scan runs on a side stream, beside the cell pass
+    hipStream_t side;
+    hipEvent_t ev_fork, ev_join;
+    int64_t* scan_tmp_side;    // scratch of the scan on the side stream
+    int64_t scan_tmp_side_len;
     // counters (device): 0 n_cells, 1 n_wall_pairs, 2 n_cell_pairs, 3 n_pairs
     // (capped), 4 n_contacts, 5 n_records, 6 unused,
     // 7 n_pairs before capping, 8 simplex slots claimed this step, 9 radix
