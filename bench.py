@@ -99,25 +99,37 @@ def cpu_baseline(sc, budget_s: float = 12.0):
                       f"(oracle/mwdiem_oracle.cpp, g++ -O2, one thread), {dt:.1f} s"}
 
 
-def bench_batched(n_scenes: int, steps: int, plain_steps: int, graph_steps: int):
+def bench_batched(n_scenes: int, steps: int, plain_steps: int, graph_steps: int, tune: bool = True):
     """The batched form: n_scenes hopper-discharge scenes (particle counts
     log-uniform 500-20,000, per-scene friction, stiffness and restitution,
     physical law) in one Stepper -- one launch per stage over all scenes,
-    graph-captured -- against the same scenes run one after another, each in
-    its own graph-captured Stepper."""
+    graph-captured, cell size and launch choices picked by the autotuner --
+    against the same scenes (same cells) run one after another, each in its
+    own graph-captured Stepper with the default launch choices."""
     import torch
 
     from mwdiem_amd import scene
     from mwdiem_amd.stepper import Stepper
+    from mwdiem_amd.tune import autotune, regrid, tune_cell_scale
 
     t0 = time.perf_counter()
     scs = scene.config_batched(n_scenes=n_scenes)
     n_tot = sum(sc.n_particles for sc in scs)
     print(f"# batched: {n_scenes} scenes, {n_tot} particles, generated in {time.perf_counter() - t0:.1f} s",
           file=sys.stderr, flush=True)
+    tuned = None
+    if tune:
+        scale, scale_ms = tune_cell_scale(scs, steps=5)
+        scs = [regrid(sc, scale) for sc in scs]
+        print(f"# batched cell scale {scale} ({scale_ms})", file=sys.stderr, flush=True)
     B = Stepper(scs)
     B.step(5)
     B.check()
+    if tune:
+        tuned = autotune(B, steps=5)
+        tuned = {"cell_scale": scale, "cell_scale_ms_per_step": scale_ms, "tuning": tuned["tuning"],
+                 "ms_per_step_eager": tuned["ms_per_step"], "default_ms_per_step_eager": tuned["default_ms_per_step"]}
+        print(f"# batched autotune: {tuned['tuning']}", file=sys.stderr, flush=True)
     B.capture(graph_steps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -146,7 +158,8 @@ def bench_batched(n_scenes: int, steps: int, plain_steps: int, graph_steps: int)
     print(f"# batched {batched:.4g} vs plain {plain:.4g} particle-steps/s", file=sys.stderr, flush=True)
     return {"scenes": n_scenes, "particles": n_tot, "steps": steps, "ms_per_step": 1e3 * tb / steps,
             "particle_steps_per_s": batched, "pairs_per_step": sb["n_pairs"], "contacts_per_step": sb["n_contacts"],
-            "plain_steps_per_scene": plain_steps, "plain_particle_steps_per_s": plain, "batched_vs_plain": batched / plain}
+            "plain_steps_per_scene": plain_steps, "plain_particle_steps_per_s": plain, "batched_vs_plain": batched / plain,
+            "autotune": tuned}
 
 
 def bench_diff(steps: int, graph_steps: int, n: int = 4096, short: int = 200):
@@ -240,6 +253,7 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=200, help="eager steps timed per kernel with events")
     ap.add_argument("--diff-steps", type=int, default=2000, help="steps of the differentiable-form measurement")
     ap.add_argument("--no-diff", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch choices (no autotuner)")
     ap.add_argument("--copy-probe", type=int, default=0, help="only run a device copy of this many bytes (counter "
                                                                  "calibration)")
     args = ap.parse_args()
@@ -265,6 +279,14 @@ def main():
     for _ in range(args.warmup):
         S.step(1)
     S.check()
+    tuned = None
+    if not args.no_tune:
+        # launch choices by timing (results are bit-identical across them;
+        # the state is restored after the trials)
+        from mwdiem_amd.tune import autotune
+        tuned = autotune(S, steps=10)
+        print(f"# autotune: {tuned['tuning']} {tuned['ms_per_step']:.4f} ms/step (defaults "
+              f"{tuned['default_ms_per_step']:.4f}), {len(tuned['trials'])} trials", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     # Kernel durations: the first steps of the run after warm-up, launched
     # eagerly on the stepper's stream with HIP events -- stage boundaries,
@@ -315,7 +337,8 @@ def main():
     torch.cuda.empty_cache()
     batched = None
     if not args.no_batched:
-        batched = bench_batched(args.batched_scenes, args.batched_steps, args.plain_steps, args.graph_steps)
+        batched = bench_batched(args.batched_scenes, args.batched_steps, args.plain_steps, args.graph_steps,
+                                tune=not args.no_tune)
 
     diff = None
     if not args.no_diff:
@@ -333,7 +356,10 @@ def main():
                                "walled box, FP64, dt=1e-6 s, Kn=1e4, e=0.5, mu=0.3, cell=2*r_circ,max",
                    "particles": sc.n_particles, "pairs_per_step": stats["n_pairs"],
                    "contacts_per_step": stats["n_contacts"], "pair_tests_per_s": stats["n_pairs"] * args.steps / wall,
-                   "stage_ms_per_step": {k: round(v, 5) for k, v in stage_ms.items()}},
+                   "stage_ms_per_step": {k: round(v, 5) for k, v in stage_ms.items()},
+                   "tuning": None if tuned is None else tuned["tuning"],
+                   "untuned_ms_per_step_eager": None if tuned is None else tuned["default_ms_per_step"],
+                   "tuned_ms_per_step_eager": None if tuned is None else tuned["ms_per_step"]},
         "roofline": roof,
         "cpu_baseline": cpu,
         "batched": batched,

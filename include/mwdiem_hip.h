@@ -155,6 +155,8 @@ int mwd_workspace_destroy(mwd_workspace* ws);
  *   gjk_blocks   grid cap of the grid-strided GJK launch, 64..65536 (1024).
  *   sphere_block threads per block of the bounding-sphere test, a multiple
  *                of 64 in 64..1024 (256).
+ *   verlet_block particles per block of the fused force reduction and
+ *                integrator: 64, 128 or 256 (256).
  * mwd_set_tuning applies to later steps (graphs captured before keep their
  * launches) and returns MWD_E_INVALID for a choice out of range. */
 #define MWD_CELLS_TABLE 0
@@ -164,6 +166,7 @@ typedef struct mwd_tuning {
     int32_t gjk_group;
     int32_t gjk_blocks;
     int32_t sphere_block;
+    int32_t verlet_block;
 } mwd_tuning;
 int mwd_get_tuning(const mwd_workspace* ws, mwd_tuning* out);
 int mwd_set_tuning(mwd_workspace* ws, const mwd_tuning* t);

@@ -98,11 +98,12 @@ struct BodyTan {
     TArr<MWD_NT> com, verts, vel, angvel, quat;
 };
 
-// Launch shapes of the pair tests (results do not depend on them).
+// Launch shapes (results do not depend on them).
 struct PairTune {
     int gjk_group = 8;       // lanes sharing one pair's support map: 4, 8 or 16
     int gjk_blocks = 1024;   // grid cap of the grid-strided GJK launch
     int sphere_block = 256;  // threads per block of the sphere test
+    int verlet_block = 256;  // particles per block of the fused reduction + integrator: 64, 128 or 256
 };
 
 // --------------------------------------------------------------- workspace

@@ -138,13 +138,14 @@ constexpr int VBLOCK = 256;
 // vertex is transformed by exactly the operations of the per-lane loop; the
 // bounding radius takes the maximum through an LDS atomic on the bits of the
 // non-negative squared distances (NaNs skipped, as fmax skips them).
+template <int VB>
 __device__ __forceinline__ void vertex_pass(double* __restrict__ verts, const int64_t* rows, const VertJob* job,
                                             unsigned long long* r2bits, int nb, int vmax, double* radius) {
     const int tid = threadIdx.x;
     r2bits[tid] = 0ull;
     __syncthreads();
     const int total = nb * vmax;
-    for (int idx = tid; idx < total; idx += VBLOCK) {
+    for (int idx = tid; idx < total; idx += VB) {
         const int j = idx / vmax;
         const int i = idx - j * vmax;
         const VertJob& J = job[j];
@@ -179,7 +180,7 @@ __global__ void __launch_bounds__(VBLOCK) k_verlet(VState a, const double* __res
         verlet_row<double>(a, P, ld3(F + 3 * P), ld3(tau + 3 * P), vmax, nrot ? nrot[P] : vmax, dt, box.b, 0,
                            &job[threadIdx.x]);
     }
-    vertex_pass(a.verts.v, rows, job, r2bits, nb, vmax, nullptr);
+    vertex_pass<VBLOCK>(a.verts.v, rows, job, r2bits, nb, vmax, nullptr);
 }
 
 // Record r of a particle: side A (r even) or B (r odd) of contact r >> 1.
@@ -253,8 +254,9 @@ __device__ __forceinline__ void reduce_verlet_row(const VState& a, const int32_t
 }
 
 // The double path: force assembly and integrator per lane, then the block's
-// vertex pass.
-__global__ void __launch_bounds__(VBLOCK) k_reduce_verlet_v(VState a, const int32_t* __restrict__ plist, int64_t M,
+// vertex pass; VB particles per block.
+template <int VB>
+__global__ void __launch_bounds__(VB) k_reduce_verlet_v(VState a, const int32_t* __restrict__ plist, int64_t M,
                                                             const int32_t* __restrict__ scene_of,
                                                             const SceneDev* __restrict__ scenes,
                                                             int32_t* __restrict__ rec_cnt,
@@ -264,11 +266,11 @@ __global__ void __launch_bounds__(VBLOCK) k_reduce_verlet_v(VState a, const int3
                                                             const double* __restrict__ fext,
                                                             const double* __restrict__ text,
                                                             const int32_t* __restrict__ nverts, int vmax, double dt) {
-    __shared__ VertJob job[VBLOCK];
-    __shared__ int64_t rows[VBLOCK];
-    __shared__ unsigned long long r2bits[VBLOCK];
-    const int64_t first = (int64_t)blockIdx.x * VBLOCK;
-    const int nb = (int)(M - first < VBLOCK ? M - first : VBLOCK);
+    __shared__ VertJob job[VB];
+    __shared__ int64_t rows[VB];
+    __shared__ unsigned long long r2bits[VB];
+    const int64_t first = (int64_t)blockIdx.x * VB;
+    const int nb = (int)(M - first < VB ? M - first : VB);
     const int64_t i = first + threadIdx.x;
     if (i < M) {
         rows[threadIdx.x] = plist[i];
@@ -276,7 +278,7 @@ __global__ void __launch_bounds__(VBLOCK) k_reduce_verlet_v(VState a, const int3
                                   text, nverts,
                                   vmax, dt, &job[threadIdx.x]);
     }
-    vertex_pass(a.verts.v, rows, job, r2bits, nb, vmax, a.radius);
+    vertex_pass<VB>(a.verts.v, rows, job, r2bits, nb, vmax, a.radius);
 }
 
 // The tangent path: one lane per particle, vertices and their tangents
@@ -368,6 +370,15 @@ int mwd_launch_verlet(double* v, double* v12, double* w12bf, double* q12, double
     return MWD_OK;
 }
 
+template <int VB>
+static void launch_reduce_verlet_v(mwd_workspace* ws, const mwd_state* st, const VState& a, const ContactOut& co) {
+    const int64_t M = ws->n_moving;
+    MWD_LAUNCH_EV(mwd_kev(ws, MWD_KT_VERLET, 0), mwd_kev(ws, MWD_KT_VERLET, 1), k_reduce_verlet_v<VB>,
+                  dim3((unsigned)((M + VB - 1) / VB)), dim3(VB), 0, ws->stream, a, ws->plist, M, ws->scene_of,
+                  ws->scenes_dev, ws->rec_cnt, ws->rec_bkt, ws->rec_ovf, ws->counters + 11, 2 * ws->contact_cap, co,
+                  st->fext, st->text, st->nverts, (int)st->vmax, ws->dt);
+}
+
 int mwd_launch_reduce_verlet(mwd_workspace* ws, mwd_state* st, double* const* tan, const ContactOut& co) {
     if (ws->n_moving <= 0) return MWD_OK;
     const VState a = mwd_vstate(st, tan);
@@ -378,10 +389,16 @@ int mwd_launch_reduce_verlet(mwd_workspace* ws, mwd_state* st, double* const* ta
                            ws->counters + 11, 2 * ws->contact_cap, co, st->fext, st->text, st->nverts, (int)st->vmax,
                            ws->dt);
     else
-        MWD_LAUNCH_EV(mwd_kev(ws, MWD_KT_VERLET, 0), mwd_kev(ws, MWD_KT_VERLET, 1), k_reduce_verlet_v,
-                      dim3((unsigned)((M + VBLOCK - 1) / VBLOCK)), dim3(VBLOCK), 0, ws->stream, a, ws->plist, M,
-                      ws->scene_of, ws->scenes_dev, ws->rec_cnt, ws->rec_bkt, ws->rec_ovf, ws->counters + 11,
-                      2 * ws->contact_cap, co, st->fext, st->text, st->nverts, (int)st->vmax, ws->dt);
+        switch (ws->tune.verlet_block) {
+            case 64:
+                launch_reduce_verlet_v<64>(ws, st, a, co);
+                break;
+            case 128:
+                launch_reduce_verlet_v<128>(ws, st, a, co);
+                break;
+            default:
+                launch_reduce_verlet_v<256>(ws, st, a, co);
+        }
     MWD_LAUNCH_CHECK();
     return MWD_OK;
 }

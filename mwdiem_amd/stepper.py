@@ -200,7 +200,7 @@ class Stepper:
                 self.step(n - done)
 
     def tuning(self) -> dict:
-        """Launch choices of the workspace (cell_pass, gjk_group, gjk_blocks, sphere_block)."""
+        """Launch choices of the workspace (cell_pass, gjk_group, gjk_blocks, sphere_block, verlet_block)."""
         t = _lib.Tuning()
         _lib.check(_lib.load().mwd_get_tuning(self.ws, C.byref(t)), "mwd_get_tuning")
         return {name: int(getattr(t, name)) for name, _ in _lib.Tuning._fields_}
@@ -331,6 +331,20 @@ class Stepper:
 
     def scene_rows(self, s: int) -> slice:
         return slice(int(self.row0[s]), int(self.row0[s]) + self.scenes[s].n_rows)
+
+    def snapshot(self) -> dict:
+        """Device copies of the state tensors (and radii), for restore()."""
+        self._enter()
+        with torch.cuda.stream(self.stream):
+            snap = {k: self.t[k].clone() for k in STATE_KEYS + ("radius",)}
+        return snap
+
+    def restore(self, snap: dict):
+        """Put back the state of a snapshot() (stream-ordered, on the device)."""
+        self._enter()
+        with torch.cuda.stream(self.stream):
+            for k, v in snap.items():
+                self.t[k].copy_(v)
 
     def set_state(self, st: dict):
         """Overwrite state tensors from numpy arrays (rows of all scenes)."""

@@ -139,3 +139,14 @@ def test_law_tangent_matches_central_differences():
             fd = (getattr(Lp, n) - getattr(Lm, n)) / (2 * h)
             got = getattr(T, n)[k]
             assert abs(got - fd) <= 1e-6 * max(1.0, abs(fd)), (n, k, got, fd)
+
+
+def test_cell_scale_tuning_refuses_reference_law(small_scene):
+    """The reference law's candidate order depends on its grid, so the cell
+    size is not tuned there (refused before any GPU work)."""
+    from mwdiem_amd.tune import regrid, tune_cell_scale
+    with pytest.raises(ValueError):
+        tune_cell_scale([small_scene])
+    wide = regrid(small_scene, 1.5)
+    assert wide.ncx < small_scene.ncx and np.all(wide.cell >= 1.5 * small_scene.max_particlesize)
+    assert wide.kspan < small_scene.kspan
