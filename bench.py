@@ -216,10 +216,25 @@ def bench_diff(steps: int, graph_steps: int, n: int = 4096, short: int = 200):
         checks[name] = out
     print(f"# diff: tangent {1e3 * times['tangent']:.4f} ms/step vs plain {1e3 * times['plain']:.4f}; {checks}",
           file=sys.stderr, flush=True)
+    # reverse mode: the same pour's gradient by the adjoint, checkpoints every
+    # 100 steps (forward, then per segment a recomputation and the reverse sweep)
+    from mwdiem_amd.adjoint import adjoint_run
+    every = 100
+    tm = {}
+    La, ga = adjoint_run(sc, v0, theta, steps, every=every, steps_per_graph=graph_steps, timings=tm)
+    gt = np.array(checks[f"pour_1m_s_{steps}_steps"]["grad"])
+    adj = {"checkpoint_every": every, "outcome": La, "grad": [float(x) for x in ga],
+           "rel_to_tangents": [float(x) for x in np.abs(ga - gt) / np.maximum(np.abs(gt), 1e-300)],
+           "forward_ms_per_step": 1e3 * tm["forward_s"] / steps,
+           "reverse_ms_per_step": 1e3 * tm["reverse_s"] / steps,
+           "adjoint_vs_forward": tm["reverse_s"] / steps / times["plain"],
+           "gradient_vs_forward": (tm["forward_s"] + tm["reverse_s"]) / steps / times["plain"]}
+    print(f"# diff: adjoint {adj}", file=sys.stderr, flush=True)
     return {"particles": n, "steps": steps, "ms_per_step_plain": 1e3 * times["plain"],
             "ms_per_step_tangent": 1e3 * times["tangent"], "tangent_vs_forward": times["tangent"] / times["plain"],
+            "adjoint": adj, "adjoint_vs_forward": adj["adjoint_vs_forward"],
             "gradient_checks": checks, "contacts_last_step": st["n_contacts"], "theta": ["Kn", "en", "mu"],
-            "mode": "forward tangents (3 directions)"}
+            "mode": "forward tangents (3 directions); reverse adjoint with step checkpoints"}
 
 
 def copy_probe(nbytes: int, reps: int = 5):
@@ -366,6 +381,7 @@ def main():
         "batched_vs_plain": None if batched is None else batched["batched_vs_plain"],
         "differentiable": diff,
         "tangent_vs_forward": None if diff is None else diff["tangent_vs_forward"],
+        "adjoint_vs_forward": None if diff is None else diff["adjoint_vs_forward"],
     }
     print(json.dumps(out))
 

@@ -303,6 +303,18 @@ int mwd_set_law_tangents(mwd_workspace* ws, const mwd_law_tan* laws2);
  * face's support vertices. Requires mwd_set_law_tangents. */
 int mwd_step_tangent(mwd_workspace* ws, mwd_state* state, double* const* tan);
 
+/* Reverse-mode adjoint of one step (no counterpart in the reference, which
+ * has no gradients; the differentiable form's reverse sweep). state holds
+ * the step's input state s_n; lam[MWD_TAN_*] (MWD_N_TAN_ARRAYS device
+ * arrays laid out as the state arrays) hold on entry the adjoint of the
+ * step's output s_{n+1} and on return that of s_n. dtheta (device, 3
+ * doubles) accumulates d(lam . s_{n+1})/d(theta_k) for the parameters whose
+ * law-coefficient derivatives mwd_set_law_tangents gave. The step's contact
+ * stages rerun on s_n; topology (touching pairs, EPA's final face, capped
+ * friction) is held fixed at what the values give. Scenes on the physical
+ * law only (MWD_E_INVALID otherwise). Walls receive no adjoint. */
+int mwd_step_adjoint(mwd_workspace* ws, mwd_state* state, double* const* lam, double* dtheta);
+
 #ifdef __cplusplus
 }
 #endif

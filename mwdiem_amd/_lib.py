@@ -126,6 +126,7 @@ _SIGS = {
                                   C.POINTER(C.c_double)]),
     "mwd_set_law_tangents": (C.c_int, [_P, C.POINTER(LawTan)]),
     "mwd_step_tangent": (C.c_int, [_P, C.POINTER(State), C.POINTER(_P)]),
+    "mwd_step_adjoint": (C.c_int, [_P, C.POINTER(State), C.POINTER(_P), _P]),
 }
 
 N_KTIMED = 5

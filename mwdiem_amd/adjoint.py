@@ -1,0 +1,100 @@
+"""Reverse-mode gradients of bed outcomes, with step checkpointing.
+
+The forward run keeps a device copy of the state every `every` steps. The
+reverse sweep takes the segments last to first: it restores a segment's
+checkpoint, recomputes the segment's steps keeping each step's input state,
+then reverses those steps one at a time with mwd_step_adjoint -- the
+hand-written adjoint kernels of the Verlet step, the physical contact law
+and EPA's final-face geometry; the step's contact stages rerun on the
+restored state, so the topology is the forward's. Memory: the checkpoints
+plus one segment of states. The result equals the forward-mode tangents'
+(mwdiem_amd.diff) to rounding; the adjoint costs one recomputed forward
+step and one reverse step per step, whatever the number of parameters.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from .diff import _with_theta, bed_outcome
+from .stepper import Stepper
+
+
+def _set_init(S: Stepper, init):
+    if isinstance(init, dict):
+        S.set_state(init)
+    elif init is not None:
+        S.set_state({"v": init, "v12": init})
+
+
+def adjoint_run(sc, init, theta, steps: int, every: int = 100, top_frac: float = 0.1, seed=None,
+                steps_per_graph: int = 0, timings: dict | None = None):
+    """Run `steps` steps of the scene with law parameters theta = (Kn, en,
+    mu) from `init` (None, initial velocities or a state dict) and return
+    (outcome, d outcome/d theta) by the adjoint. With steps_per_graph the
+    forward and recomputed segments replay captured graphs. timings, if
+    given, receives the forward and reverse wall times (s)."""
+    import time
+    S = Stepper(_with_theta(sc, theta))
+    _set_init(S, init)
+    S.set_law_seeds(seed)
+    W = sc.n_walls
+    t0 = time.perf_counter()
+    if steps_per_graph:
+        S.capture(steps_per_graph)
+    ckpt = []
+    for n0 in range(0, steps, every):
+        ckpt.append(S.snapshot())
+        S.run(min(every, steps - n0))
+    S.check()
+    torch.cuda.synchronize(S.device)
+    t1 = time.perf_counter()
+    with torch.enable_grad():
+        com = S.t["com"].detach().clone().requires_grad_(True)
+        L = bed_outcome(com, W, top_frac)
+        (g,) = torch.autograd.grad(L, com)
+    lam = {k: torch.zeros_like(S.t[k]) for k in _lib.TAN_ARRAYS}
+    lam["com"].copy_(g)
+    ptrs = (C.c_void_p * len(_lib.TAN_ARRAYS))(*[lam[k].data_ptr() for k in _lib.TAN_ARRAYS])
+    dth = torch.zeros(_lib.N_TAN, dtype=torch.float64, device=S.device)
+    for j in reversed(range(len(ckpt))):
+        n0 = j * every
+        n1 = min(n0 + every, steps)
+        S.restore(ckpt[j])
+        states = []
+        for _ in range(n0, n1):
+            states.append(S.snapshot())
+            S.step(1)
+        for k in reversed(range(n1 - n0)):
+            S.restore(states[k])
+            S.step_adjoint(ptrs, dth)
+        del states
+    S.check()
+    torch.cuda.synchronize(S.device)
+    t2 = time.perf_counter()
+    if timings is not None:
+        timings["forward_s"] = t1 - t0
+        timings["reverse_s"] = t2 - t1
+    out = float(L.detach().cpu()), dth.cpu().numpy()
+    S.close()
+    return out
+
+
+class BedOutcomeAdjoint(torch.autograd.Function):
+    """outcome = BedOutcomeAdjoint.apply(theta, scene, init, steps, every):
+    theta a float64 tensor (Kn, en, mu); backward returns grad_output * the
+    adjoint gradient."""
+
+    @staticmethod
+    def forward(ctx, theta, sc, init, steps, every=100):
+        L, g = adjoint_run(sc, init, theta.detach().cpu().numpy(), int(steps), int(every))
+        ctx.save_for_backward(torch.from_numpy(np.asarray(g)).to(theta.dtype))
+        return theta.new_tensor(L)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (g,) = ctx.saved_tensors
+        return grad_out * g.to(grad_out.device), None, None, None, None

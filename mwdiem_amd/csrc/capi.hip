@@ -49,7 +49,8 @@ void free_ws(mwd_workspace* ws) {
                     ws->depth, ws->normal, ws->cpA, ws->cpB, ws->Fn, ws->tA, ws->tB, ws->epa_status, ws->rec_cnt,
                     ws->rec_bkt, ws->rec_ovf, ws->hist,
                     ws->hist_off, ws->scan_tmp, ws->scan_tmp_side, ws->counters, ws->flags, ws->laws,
-                    ws->scene_wt_off, ws->ctan, ws->law_tans};
+                    ws->scene_wt_off, ws->ctan, ws->law_tans, ws->adj_F, ws->adj_T, ws->adj_lamF,
+                    ws->adj_lamT, ws->adj_face, ws->adj_c, ws->adj_cth};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     destroy_events(ws);
@@ -550,6 +551,7 @@ static ContactOut contact_out(mwd_workspace* ws, bool tangents) {
     co.Fn = tarr_of(ws->Fn, tb, stride, 0);
     co.tA = tarr_of(ws->tA, tb, stride, 1);
     co.tB = tarr_of(ws->tB, tb, stride, 2);
+    if (!tangents && ws->emit_faces) co.face = ws->adj_face;
     return co;
 }
 
@@ -731,6 +733,47 @@ MWD_API int mwd_step_tangent(mwd_workspace* ws, mwd_state* st, double* const* ta
     if (rc) return rc;
     mwd_mark_end(ws);
     return MWD_OK;
+}
+
+MWD_API int mwd_step_adjoint(mwd_workspace* ws, mwd_state* st, double* const* lam, double* dtheta) {
+    if (!ws || !st || !lam || !dtheta || st->vmax != ws->vmax) {
+        mwd_set_error("mwd_step_adjoint: invalid argument");
+        return MWD_E_INVALID;
+    }
+    for (int i = 0; i < MWD_N_TAN_ARRAYS; ++i)
+        if (!lam[i]) {
+            mwd_set_error("mwd_step_adjoint: adjoint array " + std::to_string(i) + " is NULL");
+            return MWD_E_INVALID;
+        }
+    if (!ws->law_tans) {
+        mwd_set_error("mwd_step_adjoint: call mwd_set_law_tangents first (the parameter derivatives of the laws)");
+        return MWD_E_INVALID;
+    }
+    for (int s = 0; s < ws->n_scenes; ++s)
+        if (ws->scenes_host[s].law_p.mode != MWD_LAW_PHYSICAL || ws->scenes_host[s].law_w.mode != MWD_LAW_PHYSICAL) {
+            mwd_set_error("mwd_step_adjoint: scene " + std::to_string(s) + " is not on the physical law");
+            return MWD_E_INVALID;
+        }
+    MWD_HIP_CHECK(hipSetDevice(ws->device));
+    if (!ws->adj_F) {
+        const size_t r3 = sizeof(double) * 3 * (size_t)ws->n_rows;
+        MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_F, r3));
+        MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_T, r3));
+        MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_lamF, r3));
+        MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_lamT, r3));
+        MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_face, sizeof(int32_t) * 7 * (size_t)ws->contact_cap));
+        MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_c, sizeof(double) * 36 * (size_t)ws->contact_cap));
+        MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_cth, sizeof(double) * 3 * (size_t)ws->contact_cap));
+    }
+    int rc = mwd_candidates(ws, st, nullptr);
+    if (rc) return rc;
+    ws->emit_faces = 1;
+    rc = fine_contact(ws, st, nullptr);
+    ws->emit_faces = 0;
+    if (rc) return rc;
+    rc = mwd_launch_step_adjoint(ws, st, state_view(ws, st, 0), state_view(ws, st, 1), lam, dtheta);
+    ws->bkt_dirty = 0;  // the gather emptied the buckets
+    return rc;
 }
 
 MWD_API int mwd_radius_init(const double* com, const double* verts, const int32_t* nverts, int64_t n, int32_t vmax,

@@ -25,10 +25,6 @@
 namespace {
 
 constexpr int EPA_FACES = MWD_EPA_MAX_FACES;
-// Face distances at or above this are EPA's placeholders (1e30 for a face
-// whose plane misses the origin's projection, 3e20 for a deleted face), not
-// penetration depths.
-constexpr double EPA_PLACEHOLDER = 1e20;
 constexpr int EPA_VERTS = 32;
 
 struct Side {
@@ -36,9 +32,6 @@ struct Side {
     int n;
 };
 
-__device__ __forceinline__ int64_t view_row(const BodyView& b, int64_t p) {
-    return b.rows ? (int64_t)b.rows[p * b.row_stride] : p;
-}
 
 // Support map (Support_funtion, .cu:55-101): first index of the strict maximum
 // of d.a over A and of (-d).b over B; returns a-b.
@@ -816,7 +809,10 @@ __global__ void __launch_bounds__(256) k_epa_force(BodyView A, BodyView B, BodyT
         const Side sa{A.verts + (int64_t)A.vmax * 3 * ra, A.nverts[ra]};
         const Side sb{B.verts + (int64_t)B.vmax * 3 * rb, B.nverts[rb]};
         const int32_t slot = sidx[p];
-        if (slot < 0 || slot >= contact_cap) continue;  // simplex storage overflowed (flagged by k_gjk)
+        if (slot < 0 || slot >= contact_cap) {  // simplex storage overflowed (flagged by k_gjk)
+            if (co.face) co.face[7 * c + 6] = 4;   // no face: nothing to reverse
+            continue;
+        }
         const EpaResult e = epa(simplex[slot], sa, sb);
         const int wall = (A.is_wall[ra] || B.is_wall[rb]) ? 1 : 0;
         const int scene = A.scene ? A.scene[ra] : 0;
@@ -857,6 +853,24 @@ __global__ void __launch_bounds__(256) k_epa_force(BodyView A, BodyView B, BodyT
                                          body3<T>(A.angvel, bt.angvel, ra), body3<T>(B.angvel, bt.angvel, rb), dt_in,
                                          bodyq<T>(A.quat, bt.quat, ra), A.isize[ra], B.isize[rb], imA, imB, dep, nrm,
                                          ca, cb, L);
+        if constexpr (std::is_same<T, double>::value) {
+            if (co.face) {  // the final face, for the adjoint of this step
+                TArr<MWD_NT> va{}, vb{};
+                va.v = const_cast<double*>(A.verts);
+                vb.v = const_cast<double*>(B.verts);
+                const FaceGeo<double> g = face_geometry<double>(va, (int64_t)A.vmax * ra, vb, (int64_t)B.vmax * rb,
+                                                                e.fa, e.fb, e.flip);
+                const bool same = g.depth == e.depth && g.n.x == e.n.x && g.n.y == e.n.y && g.n.z == e.n.z &&
+                                  g.cpA.x == e.cpA.x && g.cpA.y == e.cpA.y && g.cpA.z == e.cpA.z &&
+                                  g.cpB.x == e.cpB.x && g.cpB.y == e.cpB.y && g.cpB.z == e.cpB.z;
+                int32_t* f = co.face + 7 * c;
+                for (int k = 0; k < 3; ++k) {
+                    f[k] = e.fa[k];
+                    f[3 + k] = e.fb[k];
+                }
+                f[6] = (e.flip ? 1 : 0) | (same ? 0 : 2);
+            }
+        }
         depth[c] = e.depth;
         st3(normal + 3 * c, e.n);
         st3(cpA + 3 * c, e.cpA);

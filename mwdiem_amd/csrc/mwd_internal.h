@@ -92,6 +92,11 @@ struct VState {
 // Per-contact force on A, torques on A and B (with tangents in the tangent path).
 struct ContactOut {
     TArr<MWD_NT> Fn, tA, tB;
+    // optional (forward path): EPA's final face per contact, 7 ints: support
+    // indices on A (3) and B (3), then bit 0 = AddTriangle swapped its last
+    // two vertices, bit 1 = re-evaluating the face does not give EPA's values,
+    // bit 2 = the contact was not resolved (no face)
+    int32_t* face = nullptr;
 };
 // Tangents of the body arrays the contact stage reads.
 struct BodyTan {
@@ -105,6 +110,57 @@ struct PairTune {
     int sphere_block = 256;  // threads per block of the sphere test
     int verlet_block = 256;  // particles per block of the fused reduction + integrator: 64, 128 or 256
 };
+
+// Face distances at or above this are EPA's placeholders (1e30 for a face
+// whose plane misses the origin's projection, 3e20 for a deleted face), not
+// penetration depths.
+constexpr double EPA_PLACEHOLDER = 1e20;
+
+// Row of pair p on one side of a view.
+__device__ __forceinline__ int64_t view_row(const BodyView& b, int64_t p) {
+    return b.rows ? (int64_t)b.rows[p * b.row_stride] : p;
+}
+
+// The contact records of row P in ascending record order (2c + side for
+// contact c, so in the order of the candidate pairs): its bucket of at most
+// MWD_ROW_CONTACTS records sorted in registers, or, past a full bucket, the
+// bucket plus the row's entries of the overflow list by repeated selection.
+// fn(record) is called once per record.
+template <class Fn>
+__device__ __forceinline__ void for_row_records(int64_t P, int n, const int32_t* __restrict__ rec_bkt,
+                                                const int32_t* __restrict__ rec_ovf, const int64_t* n_ovf,
+                                                int64_t ovf_cap, Fn&& fn) {
+    if (n <= 0) return;
+    const int32_t* b = rec_bkt + P * MWD_ROW_CONTACTS;
+    if (n <= MWD_ROW_CONTACTS) {
+        int32_t r[MWD_ROW_CONTACTS];
+        for (int k = 0; k < n; ++k) {  // insertion sort; records are distinct
+            const int32_t x = b[k];
+            int j = k;
+            while (j > 0 && r[j - 1] > x) {
+                r[j] = r[j - 1];
+                --j;
+            }
+            r[j] = x;
+        }
+        for (int k = 0; k < n; ++k) fn(r[k]);
+    } else {
+        int64_t no = *n_ovf;
+        if (no > ovf_cap) no = ovf_cap;
+        int32_t prev = -1;
+        for (int t = 0; t < n; ++t) {
+            int32_t best = 0x7fffffff;
+            for (int k = 0; k < MWD_ROW_CONTACTS; ++k)
+                if (b[k] > prev && b[k] < best) best = b[k];
+            for (int64_t o = 0; o < no; ++o)
+                if (rec_ovf[2 * o] == (int32_t)P && rec_ovf[2 * o + 1] > prev && rec_ovf[2 * o + 1] < best)
+                    best = rec_ovf[2 * o + 1];
+            if (best == 0x7fffffff) break;  // lost to a full overflow list (flagged)
+            fn(best);
+            prev = best;
+        }
+    }
+}
 
 // --------------------------------------------------------------- workspace
 struct mwd_workspace {
@@ -156,7 +212,14 @@ struct mwd_workspace {
     double *depth, *normal, *cpA, *cpB, *Fn, *tA, *tB;
     int32_t* epa_status;       // [contact_cap]
     double* ctan;              // [3 arrays * MWD_NT * contact_cap * 3] tangents of Fn, tA, tB (tangent step)
-    mwd_law_tan* law_tans;     // [2*n_scenes] (tangent step)
+    mwd_law_tan* law_tans;     // [2*n_scenes] (tangent step, adjoint step)
+    // adjoint step (allocated at its first call)
+    double *adj_F, *adj_T;         // [n_rows*3] the step's force and torque per row
+    double *adj_lamF, *adj_lamT;   // [n_rows*3] their adjoints
+    int32_t* adj_face;             // [7*contact_cap] EPA's final face per contact
+    double* adj_c;                 // [2*18*contact_cap] per contact side: adjoint contributions
+    double* adj_cth;               // [3*contact_cap] per contact: d/d theta
+    int emit_faces;                // host: the next fine-contact pass writes adj_face
     // per moving row, the records (2c for side A of contact c, 2c+1 for side
     // B) of this step's contacts, in arrival order; the reduction sorts them
     int32_t* rec_cnt;          // [n_rows], zero between steps
@@ -244,6 +307,10 @@ int mwd_launch_reduce_verlet(mwd_workspace* ws, mwd_state* st, double* const* ta
 int mwd_launch_radius(const double* com, const double* verts, const int32_t* nverts, int64_t n, int32_t vmax,
                       double* radius, hipStream_t s);
 int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t* keys);
+// Adjoint of one step after its contact stages have run on the step's input
+// state (adj_face written): lam in/out, dtheta (device, 3) accumulated.
+int mwd_launch_step_adjoint(mwd_workspace* ws, const mwd_state* st, const BodyView& A, const BodyView& B,
+                            double* const* lam, double* dtheta);
 int mwd_launch_cell_keys(const double* com, int64_t n, const double* cell, int64_t ncx, int64_t ncy,
                          int64_t* keys, hipStream_t s);
 // GJK over pairs [0, n) (n from n_dev), in two launches: a bounding-sphere

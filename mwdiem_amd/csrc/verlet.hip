@@ -214,37 +214,7 @@ __device__ __forceinline__ void reduce_verlet_row(const VState& a, const int32_t
     // (the order of the candidate pairs); the bucket is emptied for the next step
     const int n = rec_cnt[P];
     if (n > 0) {
-        const int32_t* b = rec_bkt + P * MWD_ROW_CONTACTS;
-        if (n <= MWD_ROW_CONTACTS) {
-            int32_t r[MWD_ROW_CONTACTS];
-            for (int k = 0; k < n; ++k) {  // insertion sort; records are distinct
-                const int32_t x = b[k];
-                int j = k;
-                while (j > 0 && r[j - 1] > x) {
-                    r[j] = r[j - 1];
-                    --j;
-                }
-                r[j] = x;
-            }
-            for (int k = 0; k < n; ++k) add_record<T>(co, r[k], F, Tq);
-        } else {
-            // a full bucket plus the row's records in the overflow list, taken
-            // in ascending order by repeated selection (rare)
-            int64_t no = *n_ovf;
-            if (no > ovf_cap) no = ovf_cap;
-            int32_t prev = -1;
-            for (int t = 0; t < n; ++t) {
-                int32_t best = 0x7fffffff;
-                for (int k = 0; k < MWD_ROW_CONTACTS; ++k)
-                    if (b[k] > prev && b[k] < best) best = b[k];
-                for (int64_t o = 0; o < no; ++o)
-                    if (rec_ovf[2 * o] == (int32_t)P && rec_ovf[2 * o + 1] > prev && rec_ovf[2 * o + 1] < best)
-                        best = rec_ovf[2 * o + 1];
-                if (best == 0x7fffffff) break;  // lost to a full overflow list (flagged)
-                add_record<T>(co, best, F, Tq);
-                prev = best;
-            }
-        }
+        for_row_records(P, n, rec_bkt, rec_ovf, n_ovf, ovf_cap, [&](int32_t r) { add_record<T>(co, r, F, Tq); });
         rec_cnt[P] = 0;
     }
     F = F + mk3<T>(T(fext[3 * P]), T(fext[3 * P + 1]), T(fext[3 * P + 2]));

@@ -235,13 +235,11 @@ class Stepper:
         return dict(steps=n.value, ms=dict(zip(_lib.STAGE_NAMES, list(ms))))
 
     # ------------------------------------------------------------ tangents
-    def enable_tangents(self, seed=None):
-        """Carry d(state)/d(theta) for three law parameters theta_k.
-
-        seed[i][k] = d(input_i)/d(theta_k) for the law inputs (Kn, en, et, mu),
-        applied to both laws (particle and wall) of every scene. The default
-        theta is (Kn, en, mu), with et held fixed. Tangents start at zero: the
-        initial state does not depend on theta."""
+    def set_law_seeds(self, seed=None):
+        """Give the workspace the law coefficients' derivatives with respect to
+        three parameters theta_k: seed[i][k] = d(input_i)/d(theta_k) for the
+        law inputs (Kn, en, et, mu), on both laws of every scene; the default
+        theta is (Kn, en, mu). The tangent and the adjoint steps read them."""
         lib = _lib.load()
         if seed is None:
             seed = [[1, 0, 0], [0, 1, 0], [0, 0, 0], [0, 0, 1]]
@@ -252,6 +250,23 @@ class Stepper:
         self._enter()
         torch.cuda.synchronize(self.device)
         _lib.check(lib.mwd_set_law_tangents(self.ws, tans), "mwd_set_law_tangents")
+
+    def step_adjoint(self, lam_ptrs, dtheta: torch.Tensor):
+        """Reverse the step whose input state the stepper holds: lam_ptrs (11
+        device pointers, state layout) go from the adjoint of the step's output
+        to that of its input; dtheta (device, 3) accumulates d/d theta."""
+        self._enter()
+        _lib.check(_lib.load().mwd_step_adjoint(self.ws, C.byref(self.state), lam_ptrs, dtheta.data_ptr()),
+                   "mwd_step_adjoint")
+
+    def enable_tangents(self, seed=None):
+        """Carry d(state)/d(theta) for three law parameters theta_k.
+
+        seed[i][k] = d(input_i)/d(theta_k) for the law inputs (Kn, en, et, mu),
+        applied to both laws (particle and wall) of every scene. The default
+        theta is (Kn, en, mu), with et held fixed. Tangents start at zero: the
+        initial state does not depend on theta."""
+        self.set_law_seeds(seed)
         self.tan = {k: torch.zeros((_lib.N_TAN,) + tuple(self.t[k].shape), dtype=torch.float64, device=self.device)
                     for k in _lib.TAN_ARRAYS}
         arr = (C.c_void_p * (len(_lib.TAN_ARRAYS) * _lib.N_TAN))()
