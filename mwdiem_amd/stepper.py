@@ -53,7 +53,11 @@ class Stepper:
         self.n_rows = int(sum(rows))
         n_mov = sum(sc.n_particles for sc in self.scenes)
         n_wall_tests = sum(sc.n_walls * sc.n_particles for sc in self.scenes)
-        self.pair_cap = int(pair_cap or max(4096, 32 * n_mov + n_wall_tests))
+        # about 32 candidates per particle on a grid of cells one particle
+        # extent wide; a wider grid (physical law) holds more per cell
+        cand = sum(sc.n_particles * 32 * max(1.0, float(np.prod(sc.cell)) / sc.max_particlesize ** 3)
+                   for sc in self.scenes)
+        self.pair_cap = int(pair_cap or max(4096, int(cand) + n_wall_tests))
         self.contact_cap = int(contact_cap or max(1024, 8 * n_mov))
         dev = self.device
 
