@@ -107,7 +107,8 @@ __global__ void __launch_bounds__(256) k_force_sum(const int32_t* __restrict__ p
                                                    const int32_t* __restrict__ rec_cnt,
                                                    const int32_t* __restrict__ rec_bkt,
                                                    const int32_t* __restrict__ rec_ovf, const int64_t* n_ovf,
-                                                   int64_t ovf_cap, const double* __restrict__ Fn,
+                                                   int64_t ovf_cap, const int32_t* __restrict__ sidx,
+                                                   const double* __restrict__ Fn,
                                                    const double* __restrict__ tA, const double* __restrict__ tB,
                                                    const double* __restrict__ fext, const double* __restrict__ text,
                                                    double* __restrict__ F, double* __restrict__ T) {
@@ -116,7 +117,7 @@ __global__ void __launch_bounds__(256) k_force_sum(const int32_t* __restrict__ p
     const int64_t P = plist[i];
     d3 f = z3(), t = z3();
     for_row_records(P, rec_cnt[P], rec_bkt, rec_ovf, n_ovf, ovf_cap, [&](int32_t r) {
-        const int64_t c = r >> 1;
+        const int64_t c = sidx[r >> 1];
         if (r & 1) {
             f = f + neg3(ld3(Fn + 3 * c));
             t = t + ld3(tB + 3 * c);
@@ -505,7 +506,8 @@ __global__ void __launch_bounds__(256) k_gather_adj(const int32_t* __restrict__ 
                                                     int32_t* __restrict__ rec_cnt,
                                                     const int32_t* __restrict__ rec_bkt,
                                                     const int32_t* __restrict__ rec_ovf, const int64_t* n_ovf,
-                                                    int64_t ovf_cap, const double* __restrict__ cadj,
+                                                    int64_t ovf_cap, const int32_t* __restrict__ sidx,
+                                                    const double* __restrict__ cadj,
                                                     const int32_t* __restrict__ face, Lam lam, int vmax) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
@@ -513,7 +515,7 @@ __global__ void __launch_bounds__(256) k_gather_adj(const int32_t* __restrict__ 
     const int n = rec_cnt[P];
     if (n <= 0) return;
     for_row_records(P, n, rec_bkt, rec_ovf, n_ovf, ovf_cap, [&](int32_t r) {
-        const int64_t c = r >> 1;
+        const int64_t c = sidx[r >> 1];
         const int side = r & 1;
         const double* o = cadj + (2 * c + side) * CADJ;
         add3(lam.com + 3 * P, ld3(o));
@@ -568,8 +570,8 @@ int mwd_launch_step_adjoint(mwd_workspace* ws, const mwd_state* st, const BodyVi
     const VState a = mwd_vstate(st, nullptr);
     if (M > 0) {
         hipLaunchKernelGGL(k_force_sum, dim3(blocks_of(M, 1ll << 30)), dim3(256), 0, s, ws->plist, M, ws->rec_cnt,
-                           ws->rec_bkt, ws->rec_ovf, ctr + 11, 2 * ws->contact_cap, ws->Fn, ws->tA, ws->tB, st->fext,
-                           st->text, ws->adj_F, ws->adj_T);
+                           ws->rec_bkt, ws->rec_ovf, ctr + 11, 2 * ws->contact_cap, ws->sidx, ws->Fn, ws->tA,
+                           ws->tB, st->fext, st->text, ws->adj_F, ws->adj_T);
         MWD_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_verlet_adj, dim3(blocks_of(M, 1ll << 30)), dim3(256), 0, s, a, ws->adj_F, ws->adj_T, lam,
                            ws->adj_lamF, ws->adj_lamT, ws->plist, M, ws->scene_of, ws->scenes_dev, st->nverts,
@@ -582,8 +584,8 @@ int mwd_launch_step_adjoint(mwd_workspace* ws, const mwd_state* st, const BodyVi
     MWD_LAUNCH_CHECK();
     if (M > 0) {
         hipLaunchKernelGGL(k_gather_adj, dim3(blocks_of(M, 1ll << 30)), dim3(256), 0, s, ws->plist, M, ws->rec_cnt,
-                           ws->rec_bkt, ws->rec_ovf, ctr + 11, 2 * ws->contact_cap, ws->adj_c, ws->adj_face, lam,
-                           (int)st->vmax);
+                           ws->rec_bkt, ws->rec_ovf, ctr + 11, 2 * ws->contact_cap, ws->sidx, ws->adj_c,
+                           ws->adj_face, lam, (int)st->vmax);
         MWD_LAUNCH_CHECK();
     }
     hipLaunchKernelGGL(k_theta_sum, dim3(1), dim3(TSUM), 0, s, ws->adj_cth, ctr + 4, ws->contact_cap, dtheta);

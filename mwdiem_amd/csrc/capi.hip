@@ -552,6 +552,7 @@ static ContactOut contact_out(mwd_workspace* ws, bool tangents) {
     co.tA = tarr_of(ws->tA, tb, stride, 1);
     co.tB = tarr_of(ws->tB, tb, stride, 2);
     if (!tangents && ws->emit_faces) co.face = ws->adj_face;
+    co.sidx = ws->sidx;
     return co;
 }
 
@@ -569,8 +570,8 @@ static int fine_contact(mwd_workspace* ws, const mwd_state* st, double* const* t
     int64_t* ctr = ws->counters;
     mwd_mark(ws, 5);
     int rc = MWD_OK;
-    if (!ws->fresh_pairs) {  // a second pass over the same candidates: restart the slot and survivor counters
-        if ((rc = mwd_set_i64(ctr + 8, 0, s)) || (rc = mwd_set_i64(ctr + 10, 0, s)) ||
+    if (!ws->fresh_pairs) {  // a second pass over the same candidates: restart the contact and survivor counters
+        if ((rc = mwd_set_i64(ctr + 4, 0, s)) || (rc = mwd_set_i64(ctr + 10, 0, s)) ||
             (rc = mwd_set_i64(ctr + 11, 0, s)))
             return rc;
     }
@@ -579,14 +580,14 @@ static int fine_contact(mwd_workspace* ws, const mwd_state* st, double* const* t
                            mwd_kev(ws, MWD_KT_SPHERE, 0), mwd_kev(ws, MWD_KT_SPHERE, 1), ws->tune);
     if (rc) return rc;
     mwd_mark(ws, 6);
+    // a colliding pair claims contact slot c (counter 4) and writes
+    // contact_pair[c]: contacts are in no set order; the per-row sums key
+    // their records by pair index, so they are summed in candidate order
     rc = mwd_launch_gjk_surv(A, B, ws->pair_cap, ws->collision, ws->surv, ctr + 10, ws->simplex,
-                             ws->sidx, ctr + 8, ws->contact_cap, ws->flags, s, mwd_kev(ws, MWD_KT_GJK, 0),
-                             mwd_kev(ws, MWD_KT_GJK, 1), ws->tune);
+                             ws->sidx, ctr + 4, ws->contact_cap, ws->flags, s, mwd_kev(ws, MWD_KT_GJK, 0),
+                             mwd_kev(ws, MWD_KT_GJK, 1), ws->tune, ws->contact_pair);
     if (rc) return rc;
-    mwd_mark(ws, 7);
-    rc = mwd_launch_compact_contacts(ws->collision, ws->pair_cap, ctr + 3, ws->contact_pair, ws->contact_cap, ctr + 4,
-                                     ws->flags, ws->scan_tmp, ws->scan_tmp_len, s);
-    if (rc) return rc;
+    mwd_mark(ws, 7);  // (no compaction stage: kept for the stage numbering)
     mwd_mark(ws, 8);
     if (ws->bkt_dirty) {  // the previous contacts were not consumed by a reduction
         if ((rc = mwd_fill_i32(ws->rec_cnt, 0, ws->n_rows, s)) || (rc = mwd_set_i64(ctr + 11, 0, s))) return rc;

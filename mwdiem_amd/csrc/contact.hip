@@ -365,7 +365,7 @@ __global__ void __launch_bounds__(256) k_gjk(BodyView A, BodyView B, int64_t n_c
                                              const int64_t* n_surv, int32_t* __restrict__ collision,
                                              Simplex* __restrict__ simplex,
                                              int32_t* __restrict__ sidx, int64_t* slots, int64_t slot_cap,
-                                             uint32_t* flags) {
+                                             uint32_t* flags, int32_t* __restrict__ contact_pair) {
     int64_t m = *n_surv;
     if (m > n_cap) m = n_cap;
     constexpr int GG = K > 0 ? G : 1;
@@ -397,6 +397,7 @@ __global__ void __launch_bounds__(256) k_gjk(BodyView A, BodyView B, int64_t n_c
                 atomicOr(flags, MWD_F_CONTACT_OVERFLOW);
                 continue;
             }
+            if (contact_pair) contact_pair[slot] = (int32_t)p;  // contacts in slot order
             Simplex& x = simplex[slot];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
@@ -892,9 +893,11 @@ __global__ void __launch_bounds__(256) k_epa_force(BodyView A, BodyView B, BodyT
         }
         if (rec_cnt) {
             // buckets of the moving rows (walls do not integrate); a full
-            // bucket spills into the shared overflow list as (row, record)
-            if (!A.is_wall[ra]) fl |= push_record(rec_cnt, rec_bkt, rec_ovf, n_ovf, ovf_cap, ra, (int32_t)(2 * c));
-            if (!B.is_wall[rb]) fl |= push_record(rec_cnt, rec_bkt, rec_ovf, n_ovf, ovf_cap, rb, (int32_t)(2 * c + 1));
+            // bucket spills into the shared overflow list as (row, record).
+            // Records are keyed by pair index (2p + side), so a row's sum
+            // runs in candidate order whatever order the contacts came in
+            if (!A.is_wall[ra]) fl |= push_record(rec_cnt, rec_bkt, rec_ovf, n_ovf, ovf_cap, ra, (int32_t)(2 * p));
+            if (!B.is_wall[rb]) fl |= push_record(rec_cnt, rec_bkt, rec_ovf, n_ovf, ovf_cap, rb, (int32_t)(2 * p + 1));
         } else {
             rec_key[2 * c] = (uint32_t)(idA ? idA[p] : ra);
             rec_val[2 * c] = (int32_t)(2 * c);
@@ -1013,31 +1016,33 @@ constexpr int GJK_MAX_K = 6;
 template <int G, int K>
 void launch_gjk_k(int k, dim3 grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const BodyView& A,
                   const BodyView& B, int64_t n_cap, const int32_t* surv, const int64_t* n_surv, int32_t* collision,
-                  Simplex* simplex, int32_t* sidx, int64_t* slots, int64_t slot_cap, uint32_t* flags) {
+                  Simplex* simplex, int32_t* sidx, int64_t* slots, int64_t slot_cap, uint32_t* flags,
+                  int32_t* contact_pair) {
     if constexpr (K > GJK_MAX_K) {
         MWD_LAUNCH_EV(e0, e1, (k_gjk<G, 0>), grid, dim3(256), 0, s, A, B, n_cap, surv, n_surv, collision, simplex,
-                      sidx, slots, slot_cap, flags);
+                      sidx, slots, slot_cap, flags, contact_pair);
     } else {
         if (k == K)
             MWD_LAUNCH_EV(e0, e1, (k_gjk<G, K>), grid, dim3(256), 0, s, A, B, n_cap, surv, n_surv, collision,
-                          simplex, sidx, slots, slot_cap, flags);
+                          simplex, sidx, slots, slot_cap, flags, contact_pair);
         else
             launch_gjk_k<G, K + 1>(k, grid, s, e0, e1, A, B, n_cap, surv, n_surv, collision, simplex, sidx, slots,
-                                   slot_cap, flags);
+                                   slot_cap, flags, contact_pair);
     }
 }
 
 template <int G>
 void launch_gjk_g(const BodyView& A, const BodyView& B, int64_t n_cap, int32_t* collision, const int32_t* surv,
                   const int64_t* n_surv, Simplex* simplex, int32_t* sidx, int64_t* slots, int64_t slot_cap,
-                  uint32_t* flags, hipStream_t s, hipEvent_t e0, hipEvent_t e1, int max_blocks) {
+                  uint32_t* flags, hipStream_t s, hipEvent_t e0, hipEvent_t e1, int max_blocks,
+                  int32_t* contact_pair) {
     const int K = (A.vmax + G - 1) / G;
     const bool grouped = K <= GJK_MAX_K;
     // the survivors are few next to the candidates: a grid-strided launch
     // with a capped grid
     const dim3 grid(grid_for(n_cap * (grouped ? G : 1), 256, max_blocks));
     launch_gjk_k<G, 1>(grouped ? K : GJK_MAX_K + 1, grid, s, e0, e1, A, B, n_cap, surv, n_surv, collision, simplex,
-                       sidx, slots, slot_cap, flags);
+                       sidx, slots, slot_cap, flags, contact_pair);
 }
 
 }  // namespace
@@ -1045,20 +1050,20 @@ void launch_gjk_g(const BodyView& A, const BodyView& B, int64_t n_cap, int32_t* 
 int mwd_launch_gjk_surv(const BodyView& A, const BodyView& B, int64_t n_cap, int32_t* collision,
                         const int32_t* surv, const int64_t* n_surv, Simplex* simplex, int32_t* sidx, int64_t* slots,
                         int64_t slot_cap, uint32_t* flags, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
-                        const PairTune& tu) {
+                        const PairTune& tu, int32_t* contact_pair) {
     if (n_cap <= 0) return MWD_OK;
     switch (tu.gjk_group) {
         case 4:
             launch_gjk_g<4>(A, B, n_cap, collision, surv, n_surv, simplex, sidx, slots, slot_cap, flags, s, e0, e1,
-                            tu.gjk_blocks);
+                            tu.gjk_blocks, contact_pair);
             break;
         case 16:
             launch_gjk_g<16>(A, B, n_cap, collision, surv, n_surv, simplex, sidx, slots, slot_cap, flags, s, e0, e1,
-                             tu.gjk_blocks);
+                            tu.gjk_blocks, contact_pair);
             break;
         default:
             launch_gjk_g<8>(A, B, n_cap, collision, surv, n_surv, simplex, sidx, slots, slot_cap, flags, s, e0, e1,
-                            tu.gjk_blocks);
+                            tu.gjk_blocks, contact_pair);
     }
     MWD_LAUNCH_CHECK();
     return MWD_OK;

@@ -496,8 +496,9 @@ __device__ __forceinline__ void close_pairs(int64_t* counters, int64_t cell_pair
     const int64_t P = counters[1] + cell_pairs;
     counters[3] = P < cap ? P : cap;
     counters[7] = P;
-    counters[8] = 0;   // simplex slots of the fine-contact stage
-    counters[10] = 0;  // its sphere-test survivors
+    counters[4] = 0;   // contacts (slots claimed by GJK) of the fine-contact stage
+    counters[8] = 0;   // simplex slots of the per-pair fine-contact entry point
+    counters[10] = 0;  // sphere-test survivors
     counters[11] = 0;  // and its bucket overflow records
     if (P > cap) atomicOr(flags, MWD_F_PAIR_OVERFLOW);
 }

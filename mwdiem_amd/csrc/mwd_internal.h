@@ -97,6 +97,8 @@ struct ContactOut {
     // two vertices, bit 1 = re-evaluating the face does not give EPA's values,
     // bit 2 = the contact was not resolved (no face)
     int32_t* face = nullptr;
+    // pair -> contact (the GJK slot); bucket records are keyed by pair index
+    const int32_t* sidx = nullptr;
 };
 // Tangents of the body arrays the contact stage reads.
 struct BodyTan {
@@ -121,8 +123,9 @@ __device__ __forceinline__ int64_t view_row(const BodyView& b, int64_t p) {
     return b.rows ? (int64_t)b.rows[p * b.row_stride] : p;
 }
 
-// The contact records of row P in ascending record order (2c + side for
-// contact c, so in the order of the candidate pairs): its bucket of at most
+// The contact records of row P in ascending record order (2p + side for the
+// contact of candidate pair p, so in the order of the candidate pairs,
+// whatever order the contacts were found in): its bucket of at most
 // MWD_ROW_CONTACTS records sorted in registers, or, past a full bucket, the
 // bucket plus the row's entries of the overflow list by repeated selection.
 // fn(record) is called once per record.
@@ -321,10 +324,11 @@ int mwd_launch_cell_keys(const double* com, int64_t n, const double* cell, int64
 int mwd_launch_sphere(const BodyView& A, const BodyView& B, int64_t n_cap, const int64_t* n_dev, int32_t* collision,
                       int32_t* surv, int64_t* n_surv, hipStream_t s, hipEvent_t e0 = nullptr,
                       hipEvent_t e1 = nullptr, const PairTune& tu = PairTune());
+// contact_pair (may be NULL): slot -> pair index, written with the slot.
 int mwd_launch_gjk_surv(const BodyView& A, const BodyView& B, int64_t n_cap, int32_t* collision,
                         const int32_t* surv, const int64_t* n_surv, Simplex* simplex, int32_t* sidx, int64_t* slots,
                         int64_t slot_cap, uint32_t* flags, hipStream_t s, hipEvent_t e0 = nullptr,
-                        hipEvent_t e1 = nullptr, const PairTune& tu = PairTune());
+                        hipEvent_t e1 = nullptr, const PairTune& tu = PairTune(), int32_t* contact_pair = nullptr);
 // Both of the above.
 int mwd_launch_gjk(const BodyView& A, const BodyView& B, int64_t n_cap, const int64_t* n_dev,
                    int32_t* collision, int32_t* surv, int64_t* n_surv, Simplex* simplex,

@@ -183,10 +183,11 @@ __global__ void __launch_bounds__(VBLOCK) k_verlet(VState a, const double* __res
     vertex_pass<VBLOCK>(a.verts.v, rows, job, r2bits, nb, vmax, nullptr);
 }
 
-// Record r of a particle: side A (r even) or B (r odd) of contact r >> 1.
+// Record r of a particle: side A (r even) or B (r odd) of the contact of
+// candidate pair r >> 1.
 template <class T>
 __device__ __forceinline__ void add_record(const ContactOut& co, int32_t r, V3<T>& F, V3<T>& Tq) {
-    const int64_t c = r >> 1;
+    const int64_t c = co.sidx[r >> 1];
     if (r & 1) {
         F = F + neg3(ldv<T>(co.Fn, c));
         Tq = Tq + ldv<T>(co.tB, c);

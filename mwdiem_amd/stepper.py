@@ -330,6 +330,9 @@ class Stepper:
         return _copy_from_device(p.value, n * 2, np.int32).reshape(n, 2)
 
     def contacts(self) -> dict:
+        """This step's contacts in candidate-pair order: pair index, EPA depth,
+        normal and contact points, the law's force on A and torques, EPA
+        status; plus the per-pair collision codes."""
         ptrs = [C.c_void_p() for _ in range(11)]
         _lib.check(_lib.load().mwd_contacts_view(self.ws, *[C.byref(x) for x in ptrs]), "mwd_contacts_view")
         s = self.status()
@@ -342,6 +345,11 @@ class Stepper:
         for i, name in ((3, "normal"), (4, "cpA"), (5, "cpB"), (6, "Fn"), (7, "tauA"), (8, "tauB")):
             out[name] = _copy_from_device(ptrs[i].value, 3 * K, np.float64).reshape(K, 3)
         out["epa_status"] = _copy_from_device(ptrs[9].value, K, np.int32)
+        # contacts are stored in the order GJK found them; present them in
+        # candidate-pair order
+        order = np.argsort(out["pair"], kind="stable")
+        for k in ("pair", "depth", "normal", "cpA", "cpB", "Fn", "tauA", "tauB", "epa_status"):
+            out[k] = out[k][order]
         return out
 
     def numpy_state(self) -> dict:
