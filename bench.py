@@ -237,6 +237,47 @@ def bench_diff(steps: int, graph_steps: int, n: int = 4096, short: int = 200):
             "mode": "forward tangents (3 directions); reverse adjoint with step checkpoints"}
 
 
+def bench_polydisperse(steps: int, graph_steps: int):
+    """The polydisperse stress configuration: 1,000,000 hulls (circumradius
+    log-uniform 0.5-2.5 mm, 6-32 points) in a 400 mm box whose ceiling slides
+    along x at 0.1 m/s, physical law, FP64 GJK/EPA; `steps` graph-replayed
+    steps with the FP64 and with the FP32 contact reduction, and the largest
+    position difference between the two runs."""
+    import numpy as np
+    import torch
+
+    from mwdiem_amd import scene
+    from mwdiem_amd.stepper import Stepper
+
+    t0 = time.perf_counter()
+    sc = scene.config_polydisperse()
+    print(f"# polydisperse: {sc.n_particles} hulls, vmax {sc.vmax}, generated in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    out, com = {}, {}
+    for fp32 in (False, True):
+        S = Stepper(sc, reduce_fp32=fp32)
+        S.step(2)
+        S.capture(graph_steps)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        S.run(steps)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        st = S.check()
+        com[fp32] = S.numpy_state()["com"][sc.n_walls:]
+        out["fp32" if fp32 else "fp64"] = {"ms_per_step": 1e3 * dt / steps,
+                                           "particle_steps_per_s": sc.n_particles * steps / dt,
+                                           "pairs_per_step": st["n_pairs"], "contacts_per_step": st["n_contacts"]}
+        S.close()
+        del S
+        torch.cuda.empty_cache()
+    disp = float(np.abs(com[False] - sc.com[sc.n_walls:]).max())
+    out.update({"particles": sc.n_particles, "steps": steps + 2, "max_abs_dx_fp32_vs_fp64_mm": float(
+        np.abs(com[True] - com[False]).max()), "max_displacement_mm": disp})
+    print(f"# polydisperse: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def copy_probe(nbytes: int, reps: int = 5):
     """A plain device copy of nbytes, reps times: the calibration run for the
     FETCH_SIZE / WRITE_SIZE counters."""
@@ -269,6 +310,8 @@ def main():
     ap.add_argument("--diff-steps", type=int, default=2000, help="steps of the differentiable-form measurement")
     ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch choices (no autotuner)")
+    ap.add_argument("--poly-steps", type=int, default=200, help="steps of the polydisperse stress configuration")
+    ap.add_argument("--no-poly", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, help="only run a device copy of this many bytes (counter "
                                                                  "calibration)")
     args = ap.parse_args()
@@ -359,6 +402,10 @@ def main():
     if not args.no_diff:
         diff = bench_diff(args.diff_steps, args.graph_steps)
 
+    poly = None
+    if not args.no_poly:
+        poly = bench_polydisperse(args.poly_steps, args.graph_steps)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sc, args.cpu_budget)
@@ -380,6 +427,7 @@ def main():
         "batched": batched,
         "batched_vs_plain": None if batched is None else batched["batched_vs_plain"],
         "differentiable": diff,
+        "polydisperse": poly,
         "tangent_vs_forward": None if diff is None else diff["tangent_vs_forward"],
         "adjoint_vs_forward": None if diff is None else diff["adjoint_vs_forward"],
     }

@@ -142,6 +142,21 @@ int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scene* scene
                          void* stream);
 int mwd_workspace_destroy(mwd_workspace* ws);
 
+/* Moving walls (no counterpart in the reference, whose walls are static;
+ * for the sheared-box configuration): with `on`, each mwd_step (and tangent
+ * step) ends by translating every wall row r by v[r]*dt -- its centre, its
+ * nverts vertices and its slab centre wall_c (which is then written). A wall's
+ * velocity v[r] also enters the contact law as the body's velocity. */
+int mwd_set_wall_motion(mwd_workspace* ws, int32_t on);
+
+/* Precision of the per-particle contact sums (a15-a16): MWD_REDUCE_FP64
+ * (default) or MWD_REDUCE_FP32, the precision of the reference's force and
+ * torque arrays (gpu_fine_contact_detection.cu:1304-1319, FP32 atomics), still
+ * summed in candidate order. Contact forces themselves stay FP64. */
+#define MWD_REDUCE_FP64 0
+#define MWD_REDUCE_FP32 1
+int mwd_set_reduction(mwd_workspace* ws, int32_t mode);
+
 /* Launch choices of a workspace. No counterpart in the reference (its
  * kernels have fixed launches, gpu_fine_contact_detection.py:64-73); every
  * choice gives the same results, bit for bit, and the autotuner

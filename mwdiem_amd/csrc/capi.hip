@@ -41,7 +41,7 @@ int dalloc(T** p, int64_t count) {
 
 void free_ws(mwd_workspace* ws) {
     if (!ws) return;
-    void* ptrs[] = {ws->scenes_dev, ws->cell_of, ws->plist, ws->scene_of, ws->is_wall, ws->keys, ws->keys_alt, ws->vals,
+    void* ptrs[] = {ws->scenes_dev, ws->cell_of, ws->plist, ws->wlist, ws->scene_of, ws->is_wall, ws->keys, ws->keys_alt, ws->vals,
                     ws->vals_alt, ws->cell_start, ws->cell_key, ws->cell_self, ws->cell_seg,
                     ws->row_first, ws->row_rank, ws->cell_cnt, ws->cell_off, ws->perm, ws->key_cnt,
                     ws->key_min, ws->row_key,
@@ -279,7 +279,7 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     std::vector<SceneDev> sd(n_scenes);
     std::vector<int32_t> scene_of(n_rows, 0);
     std::vector<uint8_t> is_wall(n_rows, 0);
-    std::vector<int32_t> plist;
+    std::vector<int32_t> plist, wlist;
     std::vector<int64_t> wt(n_scenes + 1, 0);
     std::vector<mwd_law> laws(2 * n_scenes);
     int64_t kbase = 0, prev_end = 0;
@@ -293,7 +293,10 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
         }
         const int64_t row_end = (s + 1 < n_scenes) ? scenes[s + 1].row0 : n_rows;
         for (int64_t r = sc.row0; r < row_end; ++r) scene_of[r] = s;
-        for (int64_t r = sc.row0; r < sc.row0 + sc.n_walls; ++r) is_wall[r] = 1;
+        for (int64_t r = sc.row0; r < sc.row0 + sc.n_walls; ++r) {
+            is_wall[r] = 1;
+            wlist.push_back((int32_t)r);
+        }
         SceneDev& d = sd[s];
         d.row0 = sc.row0;
         d.n_walls = sc.n_walls;
@@ -358,6 +361,8 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     }
     A(scenes_dev, n_scenes);
     A(plist, M);
+    ws->n_wall_rows = (int64_t)wlist.size();
+    A(wlist, std::max<int64_t>(ws->n_wall_rows, 1));
     A(scene_of, n_rows);
     A(is_wall, n_rows);
     A(keys, M);
@@ -420,6 +425,7 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     }
     H2D(scenes_dev, sd.data(), n_scenes);
     if (M) H2D(plist, plist.data(), M);
+    if (ws->n_wall_rows) H2D(wlist, wlist.data(), ws->n_wall_rows);
     H2D(scene_of, scene_of.data(), n_rows);
     H2D(is_wall, is_wall.data(), n_rows);
     H2D(laws, laws.data(), 2 * n_scenes);
@@ -652,7 +658,26 @@ MWD_API int mwd_step(mwd_workspace* ws, mwd_state* st) {
     if (rc) return rc;
     rc = mwd_reduce_verlet(ws, st);
     if (rc) return rc;
+    if (ws->wall_motion && (rc = mwd_launch_move_walls(ws, st))) return rc;
     mwd_mark_end(ws);
+    return MWD_OK;
+}
+
+MWD_API int mwd_set_wall_motion(mwd_workspace* ws, int32_t on) {
+    if (!ws) {
+        mwd_set_error("mwd_set_wall_motion: null workspace");
+        return MWD_E_INVALID;
+    }
+    ws->wall_motion = on ? 1 : 0;
+    return MWD_OK;
+}
+
+MWD_API int mwd_set_reduction(mwd_workspace* ws, int32_t mode) {
+    if (!ws || (mode != MWD_REDUCE_FP64 && mode != MWD_REDUCE_FP32)) {
+        mwd_set_error("mwd_set_reduction: null workspace or unknown mode");
+        return MWD_E_INVALID;
+    }
+    ws->reduce_fp32 = mode == MWD_REDUCE_FP32 ? 1 : 0;
     return MWD_OK;
 }
 
@@ -732,6 +757,8 @@ MWD_API int mwd_step_tangent(mwd_workspace* ws, mwd_state* st, double* const* ta
     ws->bkt_dirty = 0;
     rc = mwd_launch_reduce_verlet(ws, st, tan, contact_out(ws, true));
     if (rc) return rc;
+    // walls move independently of theta: their tangents stay zero
+    if (ws->wall_motion && (rc = mwd_launch_move_walls(ws, st))) return rc;
     mwd_mark_end(ws);
     return MWD_OK;
 }

@@ -181,6 +181,10 @@ struct mwd_workspace {
     int key_bits, row_bits;    // significant bits of the sort keys
     // per-row tables
     int32_t* plist;            // [M] moving rows, row order
+    int32_t* wlist;            // [n_wall_rows] wall rows, row order
+    int64_t n_wall_rows;
+    int wall_motion;           // walls translate by v dt each step (mwd_set_wall_motion)
+    int reduce_fp32;           // per-particle contact sums in single precision (mwd_set_reduction)
     int32_t* scene_of;         // [n_rows]
     uint8_t* is_wall;          // [n_rows]
     // coarse stage
@@ -307,6 +311,7 @@ int mwd_launch_verlet(double* v, double* v12, double* w12bf, double* q12, double
 VState mwd_vstate(const mwd_state* st, double* const* tan);
 // Force assembly + Verlet over the moving rows; tan != NULL runs the tangent path.
 int mwd_launch_reduce_verlet(mwd_workspace* ws, mwd_state* st, double* const* tan, const ContactOut& co);
+int mwd_launch_move_walls(mwd_workspace* ws, mwd_state* st);
 int mwd_launch_radius(const double* com, const double* verts, const int32_t* nverts, int64_t n, int32_t vmax,
                       double* radius, hipStream_t s);
 int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t* keys);

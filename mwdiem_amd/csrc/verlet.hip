@@ -197,7 +197,21 @@ __device__ __forceinline__ void add_record(const ContactOut& co, int32_t r, V3<T
     }
 }
 
-// Force assembly fused with the integrator for moving particle i.
+// The same record summed in single precision (the precision of the
+// reference's force arrays and atomics), still in candidate order.
+__device__ __forceinline__ void add_record_f32(const ContactOut& co, int32_t r, float* F, float* Tq) {
+    const int64_t c = co.sidx[r >> 1];
+    const double* f = co.Fn.v + 3 * c;
+    const double* t = (r & 1) ? co.tB.v + 3 * c : co.tA.v + 3 * c;
+    const float sg = (r & 1) ? -1.0f : 1.0f;
+    for (int k = 0; k < 3; ++k) {
+        F[k] += sg * (float)f[k];
+        Tq[k] += (float)t[k];
+    }
+}
+
+// Force assembly fused with the integrator for moving particle i. fp32: the
+// contact sums are taken in single precision (double path only).
 template <class T>
 __device__ __forceinline__ void reduce_verlet_row(const VState& a, const int32_t* __restrict__ plist, int64_t i,
                                                   const int32_t* __restrict__ scene_of,
@@ -208,14 +222,26 @@ __device__ __forceinline__ void reduce_verlet_row(const VState& a, const int32_t
                                                   int64_t ovf_cap, const ContactOut& co,
                                                   const double* __restrict__ fext, const double* __restrict__ text,
                                                   const int32_t* __restrict__ nverts, int vmax, double dt,
-                                                  VertJob* job) {
+                                                  VertJob* job, bool fp32 = false) {
     const int64_t P = plist[i];
     V3<T> F = zero3<T>(), Tq = zero3<T>();
     // this step's records of the particle, summed in ascending contact order
     // (the order of the candidate pairs); the bucket is emptied for the next step
     const int n = rec_cnt[P];
     if (n > 0) {
-        for_row_records(P, n, rec_bkt, rec_ovf, n_ovf, ovf_cap, [&](int32_t r) { add_record<T>(co, r, F, Tq); });
+        bool done = false;
+        if constexpr (std::is_same<T, double>::value) {
+            if (fp32) {
+                float f[3] = {0.0f, 0.0f, 0.0f}, t[3] = {0.0f, 0.0f, 0.0f};
+                for_row_records(P, n, rec_bkt, rec_ovf, n_ovf, ovf_cap,
+                                [&](int32_t r) { add_record_f32(co, r, f, t); });
+                F = mk3<T>((double)f[0], (double)f[1], (double)f[2]);
+                Tq = mk3<T>((double)t[0], (double)t[1], (double)t[2]);
+                done = true;
+            }
+        }
+        if (!done)
+            for_row_records(P, n, rec_bkt, rec_ovf, n_ovf, ovf_cap, [&](int32_t r) { add_record<T>(co, r, F, Tq); });
         rec_cnt[P] = 0;
     }
     F = F + mk3<T>(T(fext[3 * P]), T(fext[3 * P + 1]), T(fext[3 * P + 2]));
@@ -236,7 +262,8 @@ __global__ void __launch_bounds__(VB) k_reduce_verlet_v(VState a, const int32_t*
                                                             const int64_t* n_ovf, int64_t ovf_cap, ContactOut co,
                                                             const double* __restrict__ fext,
                                                             const double* __restrict__ text,
-                                                            const int32_t* __restrict__ nverts, int vmax, double dt) {
+                                                            const int32_t* __restrict__ nverts, int vmax, double dt,
+                                                            int fp32) {
     __shared__ VertJob job[VB];
     __shared__ int64_t rows[VB];
     __shared__ unsigned long long r2bits[VB];
@@ -246,8 +273,7 @@ __global__ void __launch_bounds__(VB) k_reduce_verlet_v(VState a, const int32_t*
     if (i < M) {
         rows[threadIdx.x] = plist[i];
         reduce_verlet_row<double>(a, plist, i, scene_of, scenes, rec_cnt, rec_bkt, rec_ovf, n_ovf, ovf_cap, co, fext,
-                                  text, nverts,
-                                  vmax, dt, &job[threadIdx.x]);
+                                  text, nverts, vmax, dt, &job[threadIdx.x], fp32 != 0);
     }
     vertex_pass<VB>(a.verts.v, rows, job, r2bits, nb, vmax, a.radius);
 }
@@ -347,7 +373,7 @@ static void launch_reduce_verlet_v(mwd_workspace* ws, const mwd_state* st, const
     MWD_LAUNCH_EV(mwd_kev(ws, MWD_KT_VERLET, 0), mwd_kev(ws, MWD_KT_VERLET, 1), k_reduce_verlet_v<VB>,
                   dim3((unsigned)((M + VB - 1) / VB)), dim3(VB), 0, ws->stream, a, ws->plist, M, ws->scene_of,
                   ws->scenes_dev, ws->rec_cnt, ws->rec_bkt, ws->rec_ovf, ws->counters + 11, 2 * ws->contact_cap, co,
-                  st->fext, st->text, st->nverts, (int)st->vmax, ws->dt);
+                  st->fext, st->text, st->nverts, (int)st->vmax, ws->dt, ws->reduce_fp32);
 }
 
 int mwd_launch_reduce_verlet(mwd_workspace* ws, mwd_state* st, double* const* tan, const ContactOut& co) {
@@ -370,6 +396,36 @@ int mwd_launch_reduce_verlet(mwd_workspace* ws, mwd_state* st, double* const* ta
             default:
                 launch_reduce_verlet_v<256>(ws, st, a, co);
         }
+    MWD_LAUNCH_CHECK();
+    return MWD_OK;
+}
+
+namespace {
+// Walls moved at their prescribed velocity (a kinematic boundary; the
+// reference's walls are static): centre, vertices and slab centre translate
+// by v dt at the end of the step.
+__global__ void k_move_walls(const int32_t* __restrict__ wlist, int64_t nw, double* __restrict__ com,
+                             double* __restrict__ verts, const int32_t* __restrict__ nverts, int vmax,
+                             const double* __restrict__ v, double* __restrict__ wall_c, double dt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nw) return;
+    const int64_t r = wlist[i];
+    const d3 d = ld3(v + 3 * r) * dt;
+    if (zero3(d)) return;
+    st3(com + 3 * r, ld3(com + 3 * r) + d);
+    if (wall_c) st3(wall_c + 3 * r, ld3(wall_c + 3 * r) + d);
+    for (int k = 0; k < nverts[r]; ++k) {
+        double* p = verts + 3 * ((int64_t)vmax * r + k);
+        st3(p, ld3(p) + d);
+    }
+}
+}  // namespace
+
+int mwd_launch_move_walls(mwd_workspace* ws, mwd_state* st) {
+    if (ws->n_wall_rows <= 0) return MWD_OK;
+    hipLaunchKernelGGL(k_move_walls, dim3((unsigned)((ws->n_wall_rows + 63) / 64)), dim3(64), 0, ws->stream,
+                       ws->wlist, ws->n_wall_rows, st->com, st->verts, st->nverts, (int)st->vmax, st->v,
+                       const_cast<double*>(st->wall_c), ws->dt);
     MWD_LAUNCH_CHECK();
     return MWD_OK;
 }

@@ -70,6 +70,11 @@ class SceneData:
     dt: float = 1e-6
     name: str = ""
     inner: tuple = (0.0, 0.0, 0.0)   # inner box extent (Lx, Ly, Lz)
+    wall_v: np.ndarray | None = None  # [N,3] prescribed wall velocities (walls' rows); None: static walls
+
+    @property
+    def moving_walls(self) -> bool:
+        return self.wall_v is not None and bool(np.any(self.wall_v[:self.n_walls] != 0.0))
 
     @property
     def n_rows(self) -> int:
@@ -620,6 +625,108 @@ def hopper_scene(n_particles: int, seed: int, pool, law: Law4, orifice: float = 
                      isize=isize, fext=fext, text=np.zeros((N, 3)), wall_n=wall_n, wall_c=wall_c, box=box,
                      cell=cell, ncx=ncx, ncy=ncy, ncz=ncz, kmin=kmin, kspan=kspan, max_particlesize=ms,
                      law_p=law, law_w=law, dt=dt, name=name, inner=(w, w, Lz))
+
+
+def config_polydisperse(seed: int = 3, n: int = 1_000_000, L: float = 400.0, r_range=(0.5, 2.5), npts=(6, 32),
+                        v_top: float = 100.0, n_shapes: int = 4096, law: Law4 | None = None,
+                        density: float = RHO_KG_MM3, dt: float = 1e-6) -> SceneData:
+    """The polydisperse stress test: n hulls (the convex hull of 6-32 points on
+    an ellipsoid with semi-axes 1.0/0.8/0.6, shapes from a pool, each scaled
+    to a circumradius log-uniform over r_range mm, random orientation) in an
+    L^3 mm box of six walls whose ceiling moves along +x at v_top mm/s, under
+    gravity, on the physical law.
+
+    Placement: n must be a cube m^3; the hulls sit on an m^3 cubic lattice
+    filling the box, the larger half of the radii on the sites of one parity
+    and the smaller half on the other, so that two lattice neighbours never
+    have meeting bounding spheres (checked; ValueError otherwise). The cell
+    size is twice the largest circumradius."""
+    m = int(round(n ** (1.0 / 3.0)))
+    if m ** 3 != n:
+        raise ValueError("n must be a cube (the lattice has m^3 sites)")
+    law = law or Law4(mode=PHYSICAL)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    pool = shape_pool(n_shapes, seed, npts=npts, size=(1.0, 1.0), density=density)
+    prad = np.array([float(np.max(np.linalg.norm(p[0], axis=1))) for p in pool])
+    idx = rng.integers(0, len(pool), size=n)
+    r = np.exp(rng.uniform(math.log(r_range[0]), math.log(r_range[1]), size=n))
+    order = np.argsort(-r, kind="stable")            # largest first
+    g = np.arange(m, dtype=np.float64)
+    margin = r_range[1] + 0.1
+    spacing = (L - 2.0 * margin) / (m - 1) if m > 1 else 0.0
+    ii, jj, kk = np.meshgrid(g, g, g, indexing="ij")
+    par = ((ii + jj + kk).astype(np.int64) % 2).ravel()
+    sites = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], axis=1) * spacing + margin
+    even, odd = np.flatnonzero(par == 0), np.flatnonzero(par == 1)
+    big, small = order[:even.size], order[even.size:]
+    site_of = np.empty(n, dtype=np.int64)
+    site_of[big] = even
+    site_of[small] = odd
+    if m > 1 and r[big].max() + r[small].max() >= spacing:
+        raise ValueError("bounding spheres of lattice neighbours would meet; enlarge L or narrow r_range")
+    if m > 1 and 2.0 * r.max() >= spacing * math.sqrt(2.0):
+        raise ValueError("bounding spheres of same-parity neighbours would meet")
+    scale = r / prad[idx]
+    ms = 2.0 * float(r.max())
+    T = 10.0 * ms
+    walls = walls_for_box((L, L, L), T, density, False)
+    W = len(walls)
+    N = W + n
+    pv = max(p[0].shape[0] for p in pool)
+    vmax = max([wl[0].shape[0] for wl in walls] + [pv])
+    com = np.zeros((N, 3))
+    verts = np.zeros((N, vmax, 3))
+    nverts = np.zeros(N, dtype=np.int32)
+    mass = np.zeros(N)
+    I = np.zeros((N, 3, 3))
+    Iinv = np.zeros((N, 3, 3))
+    isize = np.zeros(N)
+    wall_n = np.zeros((N, 3))
+    wall_c = np.zeros((N, 3))
+    for k, (V, nrm, c, mw, Ik, Iik, isz) in enumerate(walls):
+        com[k] = c
+        verts[k, :V.shape[0]] = V
+        nverts[k] = V.shape[0]
+        mass[k], I[k], Iinv[k], isize[k] = mw, Ik, Iik, isz
+        wall_n[k], wall_c[k] = nrm, c
+    Vp = np.zeros((len(pool), pv, 3))
+    npool = np.array([p[0].shape[0] for p in pool], dtype=np.int32)
+    for j, p in enumerate(pool):
+        Vp[j, :p[0].shape[0]] = p[0]
+    Ip = np.stack([p[3] for p in pool])
+    mp = np.array([p[1] for p in pool])
+    ip = np.array([p[2] for p in pool])
+    pos = sites[site_of]
+    # in chunks: rotated, scaled body vertices and tensors (mass ~ s^3, inertia ~ s^5)
+    for a in range(0, n, 1 << 16):
+        b = min(n, a + (1 << 16))
+        R = random_rotations(rng, b - a)
+        s3 = scale[a:b]
+        Vw = np.einsum("nij,nkj->nki", R, Vp[idx[a:b]]) * s3[:, None, None]
+        Iw = np.einsum("nij,njk,nlk->nil", R, Ip[idx[a:b]], R) * (s3 ** 5)[:, None, None]
+        pad = np.arange(pv)[None, :] >= npool[idx[a:b]][:, None]
+        Vw = Vw + pos[a:b, None, :]
+        Vw[pad] = 0.0
+        rows = slice(W + a, W + b)
+        verts[rows, :pv] = Vw
+        I[rows] = Iw
+        Iinv[rows] = np.linalg.inv(Iw)
+    com[W:] = pos
+    nverts[W:] = npool[idx]
+    mass[W:] = mp[idx] * scale ** 3
+    isize[W:] = ip[idx] * scale ** 5
+    allv = np.concatenate([wl[0] for wl in walls])
+    box = np.array([allv[:, 0].max(), allv[:, 0].min(), allv[:, 1].max(), allv[:, 1].min(),
+                    allv[:, 2].max(), allv[:, 2].min()])
+    cell, ncx, ncy, ncz, kmin, kspan = grid_for(box, ms)
+    fext = mass[:, None] * np.array([0.0, 0.0, -G_MM])[None, :]
+    fext[:W] = 0.0
+    wall_v = np.zeros((N, 3))
+    wall_v[1] = (v_top, 0.0, 0.0)  # the ceiling (walls_for_box order: floor, ceiling, x-, x+, y-, y+)
+    return SceneData(n_walls=W, vmax=vmax, com=com, verts=verts, nverts=nverts, mass=mass, I=I, Iinv=Iinv,
+                     isize=isize, fext=fext, text=np.zeros((N, 3)), wall_n=wall_n, wall_c=wall_c, box=box,
+                     cell=cell, ncx=ncx, ncy=ncy, ncz=ncz, kmin=kmin, kspan=kspan, max_particlesize=ms,
+                     law_p=law, law_w=law, dt=dt, name=f"polydisperse-{n}", inner=(L, L, L), wall_v=wall_v)
 
 
 def config_batched(n_scenes: int = 256, seed: int = 4, n_min: int = 500, n_max: int = 20_000,

@@ -36,7 +36,8 @@ class Stepper:
     """
 
     def __init__(self, scenes: SceneData | Sequence[SceneData], device: int = 0, pair_cap: int | None = None,
-                 contact_cap: int | None = None, stream: torch.cuda.Stream | None = None, sphere_test: bool = True):
+                 contact_cap: int | None = None, stream: torch.cuda.Stream | None = None, sphere_test: bool = True,
+                 reduce_fp32: bool = False):
         _require_gpu()
         if isinstance(scenes, SceneData):
             scenes = [scenes]
@@ -107,6 +108,15 @@ class Stepper:
                                                 self.stream.cuda_stream or None), "mwd_workspace_create")
         self.ws = ws
         self.sphere_test = sphere_test
+        # walls with a prescribed velocity (their rows of v) move every step
+        if any(sc.moving_walls for sc in self.scenes):
+            for s, sc in enumerate(self.scenes):
+                if sc.wall_v is not None:
+                    r0, W = int(self.row0[s]), sc.n_walls
+                    self.t["v"][r0:r0 + W] = torch.from_numpy(np.ascontiguousarray(sc.wall_v[:W])).to(dev)
+            _lib.check(lib.mwd_set_wall_motion(ws, 1), "mwd_set_wall_motion")
+        if reduce_fp32:
+            _lib.check(lib.mwd_set_reduction(ws, _lib.REDUCE_FP32), "mwd_set_reduction")
         self.t["radius"] = torch.zeros(self.n_rows, dtype=torch.float64, device=dev)
         self.state = _lib.State()
         self._fill_state()
@@ -363,7 +373,7 @@ class Stepper:
         """Device copies of the state tensors (and radii), for restore()."""
         self._enter()
         with torch.cuda.stream(self.stream):
-            snap = {k: self.t[k].clone() for k in STATE_KEYS + ("radius",)}
+            snap = {k: self.t[k].clone() for k in STATE_KEYS + ("radius", "wall_c")}
         return snap
 
     def restore(self, snap: dict):

@@ -222,6 +222,9 @@ def initial_state(sc) -> dict:
         st[k][:, 0] = 1.0
     st["com"] = sc.com.copy()
     st["verts"] = sc.verts.copy()
+    if getattr(sc, "wall_v", None) is not None:  # prescribed wall velocities
+        st["v"][:sc.n_walls] = sc.wall_v[:sc.n_walls]
+        st["wall_c"] = sc.wall_c.copy()
     return st
 
 
@@ -244,7 +247,7 @@ def verlet(st, F, T, sc, pmin, pmax, nrot=None, rot_mode=None):
 def candidates(st, sc) -> np.ndarray:
     """Wall slab pairs then Linked_cell_hash pairs (MWDiem.py:807-817)."""
     W, N = sc.n_walls, sc.n_rows
-    wp = wall_pairs(st["com"], W, N, sc.wall_n, sc.wall_c, sc.wall_thr)
+    wp = wall_pairs(st["com"], W, N, sc.wall_n, st.get("wall_c", sc.wall_c), sc.wall_thr)
     keys = cell_keys(st["com"], sc.cell, sc.ncx, sc.ncy)
     hp = linked_cell_hash(keys, W, N, sc.ncx, sc.ncy, stencil=int(getattr(sc.law_p, "mode", 0)))
     return np.concatenate([wp, hp]).astype(np.int32).reshape(-1, 2)
@@ -258,7 +261,22 @@ def step(st, sc):
     F = F + sc.fext
     T = T + sc.text
     verlet(st, F, T, sc, sc.n_walls, sc.n_rows, nrot=sc.nverts)
+    if getattr(sc, "wall_v", None) is not None:
+        move_walls(st, sc)
     return pairs, coll
+
+
+def move_walls(st, sc):
+    """Walls with a prescribed velocity translate by v dt at the end of the
+    step: centre, vertices, slab centre (a kinematic boundary; the
+    reference's walls are static)."""
+    W = sc.n_walls
+    d = st["v"][:W] * sc.dt
+    st["com"][:W] += d
+    if "wall_c" in st:
+        st["wall_c"][:W] += d
+    for r in range(W):
+        st["verts"][r, :int(sc.nverts[r])] += d[r]
 
 
 def quat_mul(a, b) -> np.ndarray:
