@@ -149,6 +149,19 @@ int mwd_workspace_destroy(mwd_workspace* ws);
  * velocity v[r] also enters the contact law as the body's velocity. */
 int mwd_set_wall_motion(mwd_workspace* ws, int32_t on);
 
+/* Tangential history (a5, MWDiem.py:649,872-880,953-966). MWD_HISTORY_RESET
+ * (default): the reference as it runs -- the history is dropped every step
+ * (Q1), so every contact starts from a zero tangential displacement.
+ * MWD_HISTORY_PERSISTENT: the history the reference intends -- each
+ * contact's new displacement (.cu:770-811) is kept, keyed min(a,b)+max(a,b)^2
+ * as the driver keys it, and read back next step if the pair is still in
+ * contact; pairs that part lose it. Held in two open-addressing tables on the
+ * device (2x contact_cap slots each, allocated on first use). One
+ * fine-contact pass per step; not with the tangent or adjoint step. */
+#define MWD_HISTORY_RESET 0
+#define MWD_HISTORY_PERSISTENT 1
+int mwd_set_tangential_history(mwd_workspace* ws, int32_t mode);
+
 /* Precision of the per-particle contact sums (a15-a16): MWD_REDUCE_FP64
  * (default) or MWD_REDUCE_FP32, the precision of the reference's force and
  * torque arrays (gpu_fine_contact_detection.cu:1304-1319, FP32 atomics), still

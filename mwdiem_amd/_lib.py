@@ -90,6 +90,8 @@ class Tuning(C.Structure):
                 ("sphere_block", C.c_int32), ("verlet_block", C.c_int32)]
 
 
+HISTORY_RESET = 0
+HISTORY_PERSISTENT = 1
 REDUCE_FP64 = 0
 REDUCE_FP32 = 1
 CELLS_TABLE = 0
@@ -108,6 +110,7 @@ _SIGS = {
     "mwd_workspace_destroy": (C.c_int, [_P]),
     "mwd_set_wall_motion": (C.c_int, [_P, C.c_int32]),
     "mwd_set_reduction": (C.c_int, [_P, C.c_int32]),
+    "mwd_set_tangential_history": (C.c_int, [_P, C.c_int32]),
     "mwd_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "mwd_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "mwd_verlet": (C.c_int, [_P] * 16 + [C.c_int32, _P, C.c_double, _P, C.c_int64, C.c_int64, _P]),

@@ -54,6 +54,12 @@ void free_ws(mwd_workspace* ws) {
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     destroy_events(ws);
+    for (int t = 0; t < 2; ++t) {
+        if (ws->thist.keys[t]) (void)hipFree(ws->thist.keys[t]);
+        if (ws->thist.val[t]) (void)hipFree(ws->thist.val[t]);
+        if (ws->thist.used[t]) (void)hipFree(ws->thist.used[t]);
+    }
+    if (ws->thist.idx) (void)hipFree(ws->thist.idx);
     if (ws->ev_fork) (void)hipEventDestroy(ws->ev_fork);
     if (ws->ev_join) (void)hipEventDestroy(ws->ev_join);
     if (ws->side) (void)hipStreamDestroy(ws->side);
@@ -559,6 +565,7 @@ static ContactOut contact_out(mwd_workspace* ws, bool tangents) {
     co.tB = tarr_of(ws->tB, tb, stride, 2);
     if (!tangents && ws->emit_faces) co.face = ws->adj_face;
     co.sidx = ws->sidx;
+    if (!tangents && ws->hist_on) co.hist = ws->thist;
     return co;
 }
 
@@ -595,6 +602,7 @@ static int fine_contact(mwd_workspace* ws, const mwd_state* st, double* const* t
     if (rc) return rc;
     mwd_mark(ws, 7);  // (no compaction stage: kept for the stage numbering)
     mwd_mark(ws, 8);
+    if (ws->hist_on && !tan && (rc = mwd_launch_hist_prep(ws->thist, ws->contact_cap, s))) return rc;
     if (ws->bkt_dirty) {  // the previous contacts were not consumed by a reduction
         if ((rc = mwd_fill_i32(ws->rec_cnt, 0, ws->n_rows, s)) || (rc = mwd_set_i64(ctr + 11, 0, s))) return rc;
     }
@@ -619,6 +627,7 @@ static int fine_contact(mwd_workspace* ws, const mwd_state* st, double* const* t
                                        mwd_kev(ws, MWD_KT_EPA, 1));
     }
     if (rc) return rc;
+    if (ws->hist_on && !tan && (rc = mwd_launch_hist_flip(ws->thist, s))) return rc;
     ws->bkt_dirty = 1;
     return MWD_OK;
 }
@@ -660,6 +669,32 @@ MWD_API int mwd_step(mwd_workspace* ws, mwd_state* st) {
     if (rc) return rc;
     if (ws->wall_motion && (rc = mwd_launch_move_walls(ws, st))) return rc;
     mwd_mark_end(ws);
+    return MWD_OK;
+}
+
+MWD_API int mwd_set_tangential_history(mwd_workspace* ws, int32_t mode) {
+    if (!ws || (mode != MWD_HISTORY_RESET && mode != MWD_HISTORY_PERSISTENT)) {
+        mwd_set_error("mwd_set_tangential_history: null workspace or unknown mode");
+        return MWD_E_INVALID;
+    }
+    if (mode == MWD_HISTORY_PERSISTENT && !ws->thist.keys[0]) {
+        MWD_HIP_CHECK(hipSetDevice(ws->device));
+        int64_t H = 1024;
+        while (H < 2 * ws->contact_cap) H <<= 1;  // load factor at most 1/2
+        for (int t = 0; t < 2; ++t) {
+            MWD_HIP_CHECK(hipMalloc((void**)&ws->thist.keys[t], sizeof(int64_t) * H));
+            MWD_HIP_CHECK(hipMalloc((void**)&ws->thist.val[t], sizeof(double) * 4 * H));
+            MWD_HIP_CHECK(hipMalloc((void**)&ws->thist.used[t], sizeof(int32_t) * ws->contact_cap));
+            MWD_HIP_CHECK(hipMemsetAsync(ws->thist.keys[t], 0xFF, sizeof(int64_t) * H, ws->stream));
+            ws->thist.count[t] = ws->counters + 12 + t;
+        }
+        MWD_HIP_CHECK(hipMalloc((void**)&ws->thist.idx, sizeof(int32_t)));
+        MWD_HIP_CHECK(hipMemsetAsync(ws->thist.idx, 0, sizeof(int32_t), ws->stream));
+        MWD_HIP_CHECK(hipMemsetAsync(ws->counters + 12, 0, 2 * sizeof(int64_t), ws->stream));
+        MWD_HIP_CHECK(hipStreamSynchronize(ws->stream));
+        ws->thist.mask = H - 1;
+    }
+    ws->hist_on = mode == MWD_HISTORY_PERSISTENT ? 1 : 0;
     return MWD_OK;
 }
 
@@ -744,6 +779,10 @@ MWD_API int mwd_step_tangent(mwd_workspace* ws, mwd_state* st, double* const* ta
         mwd_set_error("mwd_step_tangent: call mwd_set_law_tangents first");
         return MWD_E_INVALID;
     }
+    if (ws->hist_on) {
+        mwd_set_error("mwd_step_tangent: not with a persistent tangential history");
+        return MWD_E_INVALID;
+    }
     for (int i = 0; i < MWD_N_TAN_ARRAYS * MWD_NT; ++i)
         if (!tan[i]) {
             mwd_set_error("mwd_step_tangent: tangent array " + std::to_string(i) + " is NULL");
@@ -775,6 +814,10 @@ MWD_API int mwd_step_adjoint(mwd_workspace* ws, mwd_state* st, double* const* la
         }
     if (!ws->law_tans) {
         mwd_set_error("mwd_step_adjoint: call mwd_set_law_tangents first (the parameter derivatives of the laws)");
+        return MWD_E_INVALID;
+    }
+    if (ws->hist_on) {
+        mwd_set_error("mwd_step_adjoint: not with a persistent tangential history");
         return MWD_E_INVALID;
     }
     for (int s = 0; s < ws->n_scenes; ++s)

@@ -739,8 +739,14 @@ __device__ LawOut<T> force_law(V3<T> comA, V3<T> comB, V3<T> vA, V3<T> vB, V3<T>
     if (Ftn > muF) Ft = ((t * Fnn) * -1.0) * L.mu;
     if (Ftn <= muF) {
         dnew = qrot(qA, dispt, qconj(qA)) + ut * L.dt;
-    } else {
+    } else if (!phys) {
+        // .cu:800-806 as written: the slip displacement makes the next step's
+        // spring force point along the sliding (the sign of the mu term)
         dnew = ((((t * Fnn) * L.mu) * -1.0) - ut * Gt) / Kt;
+    } else {
+        // physical law: the displacement whose spring-dashpot force is the
+        // Coulomb limit opposing the sliding, -Kt d - Gt ut = -mu |Fn| t
+        dnew = (((t * Fnn) * L.mu) - ut * Gt) / Kt;
     }
     const V3<T> Ff = Ft + Fn;
     o.Fn = phys ? Ff : Fn;
@@ -748,6 +754,36 @@ __device__ LawOut<T> force_law(V3<T> comA, V3<T> comB, V3<T> vA, V3<T> vB, V3<T>
     o.tB = cross3(cp - comB, Ff * -1.0);
     o.disp = dnew;
     return o;
+}
+
+// Tangential history table: multiplicative hash, linear probing.
+__device__ __forceinline__ int64_t hist_key(int64_t a, int64_t b) {
+    const int64_t lo = a < b ? a : b, hi = a < b ? b : a;
+    return lo + hi * hi;  // MWDiem.py:872-880
+}
+__device__ __forceinline__ int64_t hist_slot(int64_t key, int64_t mask) {
+    return (int64_t)(((uint64_t)key * 0x9E3779B97F4A7C15ull) >> 17) & mask;
+}
+__device__ __forceinline__ d3 hist_find(const HistView& h, int t, int64_t key) {
+    for (int64_t i = hist_slot(key, h.mask), n = 0; n <= h.mask; i = (i + 1) & h.mask, ++n) {
+        const int64_t k = h.keys[t][i];
+        if (k == key) return ld3(h.val[t] + 4 * i);
+        if (k < 0) break;
+    }
+    return mk3(0.0, 0.0, 0.0);
+}
+__device__ __forceinline__ void hist_put(const HistView& h, int t, int64_t key, const double* v4) {
+    for (int64_t i = hist_slot(key, h.mask), n = 0; n <= h.mask; i = (i + 1) & h.mask, ++n) {
+        const unsigned long long prev =
+            atomicCAS((unsigned long long*)&h.keys[t][i], ~0ull, (unsigned long long)key);
+        if (prev == ~0ull) {  // claimed
+            for (int k = 0; k < 4; ++k) h.val[t][4 * i + k] = v4[k];
+            const int64_t u = (int64_t)atomicAdd((unsigned long long*)h.count[t], 1ull);
+            h.used[t][u] = (int32_t)i;
+            return;
+        }
+        if ((int64_t)prev == key) return;  // the pair listed twice (a one-cell-wide grid): first one kept
+    }
 }
 
 __device__ __forceinline__ uint32_t push_record(int32_t* rec_cnt, int32_t* rec_bkt, int32_t* rec_ovf, int64_t* n_ovf,
@@ -818,7 +854,16 @@ __global__ void __launch_bounds__(256) k_epa_force(BodyView A, BodyView B, BodyT
         const int wall = (A.is_wall[ra] || B.is_wall[rb]) ? 1 : 0;
         const int scene = A.scene ? A.scene[ra] : 0;
         const LawT<T> L = law_of(laws2[2 * scene + wall], lawt2 ? lawt2 + 2 * scene + wall : nullptr, (T*)nullptr);
-        const V3<T> dt_in = dispt ? mk3<T>(T(dispt[4 * p]), T(dispt[4 * p + 1]), T(dispt[4 * p + 2])) : zero3<T>();
+        V3<T> dt_in = dispt ? mk3<T>(T(dispt[4 * p]), T(dispt[4 * p + 1]), T(dispt[4 * p + 2])) : zero3<T>();
+        int hist_t = -1;
+        int64_t hkey = 0;
+        if constexpr (std::is_same<T, double>::value) {
+            if (co.hist.keys[0]) {  // persistent history: last step's displacement of this pair
+                hist_t = *co.hist.idx;
+                hkey = hist_key(ra, rb);
+                dt_in = hist_find(co.hist, hist_t, hkey);
+            }
+        }
         const double imA = (A.mass && !A.is_wall[ra]) ? 1.0 / A.mass[ra] : 0.0;
         const double imB = (B.mass && !B.is_wall[rb]) ? 1.0 / B.mass[rb] : 0.0;
         T dep;
@@ -890,6 +935,10 @@ __global__ void __launch_bounds__(256) k_epa_force(BodyView A, BodyView B, BodyT
             dp[1] = val(f.disp.y);
             dp[2] = val(f.disp.z);
             dp[3] = 1.0;
+        }
+        if (hist_t >= 0) {  // this step's displacement, for the next step
+            const double d4[4] = {val(f.disp.x), val(f.disp.y), val(f.disp.z), 1.0};
+            hist_put(co.hist, 1 - hist_t, hkey, d4);
         }
         if (rec_cnt) {
             // buckets of the moving rows (walls do not integrate); a full
@@ -1065,6 +1114,33 @@ int mwd_launch_gjk_surv(const BodyView& A, const BodyView& B, int64_t n_cap, int
             launch_gjk_g<8>(A, B, n_cap, collision, surv, n_surv, simplex, sidx, slots, slot_cap, flags, s, e0, e1,
                             tu.gjk_blocks, contact_pair);
     }
+    MWD_LAUNCH_CHECK();
+    return MWD_OK;
+}
+
+namespace {
+__global__ void k_hist_prep(HistView h, int64_t cap) {
+    const int t = 1 - *h.idx;  // the table this step writes
+    int64_t n = *h.count[t];
+    if (n > cap) n = cap;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        h.keys[t][h.used[t][i]] = -1;
+}
+__global__ void k_hist_count0(HistView h) { *h.count[1 - *h.idx] = 0; }
+__global__ void k_hist_flip(HistView h) { *h.idx = 1 - *h.idx; }
+}  // namespace
+
+int mwd_launch_hist_prep(const HistView& h, int64_t contact_cap, hipStream_t s) {
+    hipLaunchKernelGGL(k_hist_prep, dim3(grid_for(contact_cap, 256, 2048)), dim3(256), 0, s, h, contact_cap);
+    MWD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_hist_count0, dim3(1), dim3(1), 0, s, h);
+    MWD_LAUNCH_CHECK();
+    return MWD_OK;
+}
+
+int mwd_launch_hist_flip(const HistView& h, hipStream_t s) {
+    hipLaunchKernelGGL(k_hist_flip, dim3(1), dim3(1), 0, s, h);
     MWD_LAUNCH_CHECK();
     return MWD_OK;
 }

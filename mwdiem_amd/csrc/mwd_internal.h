@@ -89,6 +89,20 @@ struct VState {
     const double *mass, *Iinv, *I;
     double* radius;
 };
+// Persistent tangential history (mwd_set_tangential_history): two
+// open-addressing tables of (pair key min(a,b) + max(a,b)^2 -> displacement
+// [4]), -1 for an empty key. *idx names the table the previous step wrote,
+// which this step reads; this step writes the other one, listing the slots it
+// fills (used, count) so that the table can be emptied slot by slot later.
+struct HistView {
+    int64_t* keys[2] = {nullptr, nullptr};
+    double* val[2] = {nullptr, nullptr};
+    int32_t* used[2] = {nullptr, nullptr};
+    int64_t* count[2] = {nullptr, nullptr};
+    int32_t* idx = nullptr;
+    int64_t mask = 0;
+};
+
 // Per-contact force on A, torques on A and B (with tangents in the tangent path).
 struct ContactOut {
     TArr<MWD_NT> Fn, tA, tB;
@@ -99,6 +113,8 @@ struct ContactOut {
     int32_t* face = nullptr;
     // pair -> contact (the GJK slot); bucket records are keyed by pair index
     const int32_t* sidx = nullptr;
+    // forward path: tangential history read from and written to these tables
+    HistView hist;
 };
 // Tangents of the body arrays the contact stage reads.
 struct BodyTan {
@@ -185,6 +201,8 @@ struct mwd_workspace {
     int64_t n_wall_rows;
     int wall_motion;           // walls translate by v dt each step (mwd_set_wall_motion)
     int reduce_fp32;           // per-particle contact sums in single precision (mwd_set_reduction)
+    HistView thist;            // persistent tangential history tables (allocated when first enabled)
+    int hist_on;
     int32_t* scene_of;         // [n_rows]
     uint8_t* is_wall;          // [n_rows]
     // coarse stage
@@ -247,7 +265,7 @@ struct mwd_workspace {
     // (capped), 4 n_contacts, 5 n_records, 6 unused,
     // 7 n_pairs before capping, 8 simplex slots claimed this step, 9 radix
     // histogram entries (scratch), 10 sphere-test survivors, 11 bucket
-    // overflow records
+    // overflow records, 12-13 slots filled in the tangential history tables
     int64_t* counters;
     uint32_t* flags;
     mwd_law* laws;             // [2*n_scenes] (particle law, wall law)
@@ -312,6 +330,10 @@ VState mwd_vstate(const mwd_state* st, double* const* tan);
 // Force assembly + Verlet over the moving rows; tan != NULL runs the tangent path.
 int mwd_launch_reduce_verlet(mwd_workspace* ws, mwd_state* st, double* const* tan, const ContactOut& co);
 int mwd_launch_move_walls(mwd_workspace* ws, mwd_state* st);
+// Tangential history: empty the table this step writes (start of the fine
+// contact stage); make it the one read next step (end of the step).
+int mwd_launch_hist_prep(const HistView& h, int64_t contact_cap, hipStream_t s);
+int mwd_launch_hist_flip(const HistView& h, hipStream_t s);
 int mwd_launch_radius(const double* com, const double* verts, const int32_t* nverts, int64_t n, int32_t vmax,
                       double* radius, hipStream_t s);
 int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t* keys);

@@ -37,7 +37,7 @@ class Stepper:
 
     def __init__(self, scenes: SceneData | Sequence[SceneData], device: int = 0, pair_cap: int | None = None,
                  contact_cap: int | None = None, stream: torch.cuda.Stream | None = None, sphere_test: bool = True,
-                 reduce_fp32: bool = False):
+                 reduce_fp32: bool = False, persistent_history: bool = False):
         _require_gpu()
         if isinstance(scenes, SceneData):
             scenes = [scenes]
@@ -117,6 +117,8 @@ class Stepper:
             _lib.check(lib.mwd_set_wall_motion(ws, 1), "mwd_set_wall_motion")
         if reduce_fp32:
             _lib.check(lib.mwd_set_reduction(ws, _lib.REDUCE_FP32), "mwd_set_reduction")
+        if persistent_history:  # keep each contact's tangential displacement (the reference drops it, Q1)
+            _lib.check(lib.mwd_set_tangential_history(ws, _lib.HISTORY_PERSISTENT), "mwd_set_tangential_history")
         self.t["radius"] = torch.zeros(self.n_rows, dtype=torch.float64, device=dev)
         self.state = _lib.State()
         self._fill_state()

@@ -446,8 +446,14 @@ ForceOut<T> force_law(const PairIn<T>& in, T depth, V3<T> n, V3<T> cpA, V3<T> cp
     if (Ftn <= muF) {
         V3<T> r = qtripl(in.qA, in.dispt, qinv(in.qA));
         dnew = r + ut * T(L.dt);
-    } else {
+    } else if (!phys) {
+        // .cu:800-806 as written: the slip displacement makes the next step's
+        // spring force point along the sliding (the sign of the mu term)
         dnew = ((((t * Fnn) * T(L.mu)) * T(-1.0)) - ut * Gt) / Kt;
+    } else {
+        // physical law: the displacement whose spring-dashpot force is the
+        // Coulomb limit opposing the sliding, -Kt d - Gt ut = -mu |Fn| t
+        dnew = (((t * Fnn) * T(L.mu)) - ut * Gt) / Kt;
     }
     V3<T> Ff = Ft + Fn;
     o.Fn = phys ? Ff : Fn;
@@ -891,7 +897,8 @@ ORC_EXPORT int64_t orc_contact_state(int64_t P, const int32_t* pa, const int32_t
                                      const double* isize, const double* mass, const double* law_p, const double* law_w,
                                      int64_t W, double dt, int32_t* collision, double* depth,
                                      double* normal, double* cpA, double* cpB, double* force,
-                                     double* torque, int32_t* epa_status) {
+                                     double* torque, int32_t* epa_status, const double* dispt,
+                                     double* disp_out) {
     Law Lp = make_law(law_p[0], law_p[1], law_p[2], law_p[3], dt, (int)law_p[4]);
     Law Lw = make_law(law_w[0], law_w[1], law_w[2], law_w[3], dt, (int)law_w[4]);
     int64_t K = 0;
@@ -903,7 +910,8 @@ ORC_EXPORT int64_t orc_contact_state(int64_t P, const int32_t* pa, const int32_t
         in.comA = load3<double>(com + 3 * a); in.comB = load3<double>(com + 3 * b);
         in.vA = load3<double>(vel + 3 * a); in.vB = load3<double>(vel + 3 * b);
         in.wA = load3<double>(angvel + 3 * a); in.wB = load3<double>(angvel + 3 * b);
-        in.dispt = mk<double>(0.0, 0.0, 0.0);
+        // tangential history of the pair ([P,4] per pair; none: reset, Q1)
+        in.dispt = dispt ? mk<double>(dispt[4 * p], dispt[4 * p + 1], dispt[4 * p + 2]) : mk<double>(0.0, 0.0, 0.0);
         in.qA = qload<double>(q + 4 * a);
         in.isA = isize[a]; in.isB = isize[b];
         in.imA = a < W ? 0.0 : 1.0 / mass[a]; in.imB = b < W ? 0.0 : 1.0 / mass[b];
@@ -917,6 +925,8 @@ ORC_EXPORT int64_t orc_contact_state(int64_t P, const int32_t* pa, const int32_t
             if (cpA) cpA[3 * p + k] = r.cpA[k];
             if (cpB) cpB[3 * p + k] = r.cpB[k];
         }
+        if (disp_out && r.collision == 1)
+            for (int k = 0; k < 4; ++k) disp_out[4 * p + k] = r.disp[k];
         if (r.collision == 1) {
             ++K;
             for (int k = 0; k < 3; ++k) {

@@ -162,9 +162,11 @@ def _laws(sc):
     return _law5(sc.law_p), _law5(sc.law_w)
 
 
-def contact_state(pairs, st, sc):
+def contact_state(pairs, st, sc, dispt=None, with_disp=False):
     """Fine contact over a pair list indexed into state arrays; returns
-    (collision, depth, normal, cpA, cpB, force_sum, torque_sum, epa_status)."""
+    (collision, depth, normal, cpA, cpB, force_sum, torque_sum, epa_status),
+    and the pairs' new tangential displacements [P,4] with with_disp. dispt
+    [P,4]: the pairs' tangential history (None: zero, the reference's reset)."""
     pairs = _i32(pairs).reshape(-1, 2)
     P = pairs.shape[0]
     pa, pb = np.ascontiguousarray(pairs[:, 0]), np.ascontiguousarray(pairs[:, 1])
@@ -177,11 +179,16 @@ def contact_state(pairs, st, sc):
     cb = np.zeros((P, 3))
     F = np.zeros((N, 3))
     T = np.zeros((N, 3))
+    dispt = None if dispt is None else _f64(dispt)
+    disp = np.zeros((P, 4)) if with_disp else None
     lp, lw = _laws(sc)
     lib().orc_contact_state(C.c_int64(P), _p(pa), _p(pb), C.c_int(sc.vmax), _p(_f64(st["com"])),
                             _p(_f64(st["verts"])), _p(_i32(sc.nverts)), _p(_f64(st["v"])), _p(_f64(st["w"])),
                             _p(_f64(st["q"])), _p(_f64(sc.isize)), _p(_f64(sc.mass)), _p(lp), _p(lw),
-                            C.c_int64(sc.n_walls), C.c_double(sc.dt), _p(coll), _p(depth), _p(nrm), _p(ca), _p(cb), _p(F), _p(T), _p(est))
+                            C.c_int64(sc.n_walls), C.c_double(sc.dt), _p(coll), _p(depth), _p(nrm), _p(ca), _p(cb), _p(F), _p(T), _p(est),
+                            _p(dispt), _p(disp))
+    if disp is not None:
+        return coll, depth, nrm, ca, cb, F, T, est, disp
     return coll, depth, nrm, ca, cb, F, T, est
 
 
@@ -253,11 +260,32 @@ def candidates(st, sc) -> np.ndarray:
     return np.concatenate([wp, hp]).astype(np.int32).reshape(-1, 2)
 
 
-def step(st, sc):
+def pair_keys(pairs) -> np.ndarray:
+    """The driver's tangential-history key min + max^2 (MWDiem.py:872-880)."""
+    a = np.minimum(pairs[:, 0], pairs[:, 1]).astype(np.int64)
+    b = np.maximum(pairs[:, 0], pairs[:, 1]).astype(np.int64)
+    return a + b * b
+
+
+def step(st, sc, history: dict | None = None):
     """One iteration of the reference time loop (MWDiem.py:805-1001) in place;
-    returns (pairs, collision)."""
+    returns (pairs, collision). With `history` (a dict key -> [4]), the
+    tangential history persists: each pair reads its entry and the dict is
+    replaced by this step's contacts' new displacements (the history the
+    driver intends; as it runs, it drops every row, Q1)."""
     pairs = candidates(st, sc)
-    coll, _, _, _, _, F, T, _ = contact_state(pairs, st, sc)
+    if history is None:
+        coll, _, _, _, _, F, T, _ = contact_state(pairs, st, sc)
+    else:
+        keys = pair_keys(pairs)
+        dispt = np.zeros((pairs.shape[0], 4))
+        for i, k in enumerate(keys.tolist()):
+            if k in history:
+                dispt[i] = history[k]
+        coll, _, _, _, _, F, T, _, disp = contact_state(pairs, st, sc, dispt=dispt, with_disp=True)
+        history.clear()
+        for i in np.flatnonzero(coll == 1).tolist():
+            history.setdefault(int(keys[i]), disp[i].copy())
     F = F + sc.fext
     T = T + sc.text
     verlet(st, F, T, sc, sc.n_walls, sc.n_rows, nrot=sc.nverts)

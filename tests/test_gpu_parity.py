@@ -440,3 +440,33 @@ def test_batched_equals_separate(gpu, small_scene, dense_scene):
         gs = s.numpy_state()
         for k in ("com", "q", "v", "w", "verts"):
             assert np.array_equal(gb[k][rows], gs[k]), (i, k)
+
+
+def test_persistent_tangential_history_matches_oracle(gpu, dense_phys):
+    """Persistent tangential history (the history the driver means to keep,
+    keyed min+max^2): 60 steps of the dense bed with random motion against
+    the oracle carrying the same history, within 1e-8; the history changes
+    the result (the reference's reset mode gives another state)."""
+    from mwdiem_amd.stepper import Stepper
+    sc = dense_phys
+    st0 = random_motion(O.initial_state(sc), sc, 8, vscale=800.0, wscale=100.0)
+    st = {k: v.copy() for k, v in st0.items()}
+    hist = {}
+    S = Stepper(sc, persistent_history=True)
+    R = Stepper(sc)
+    S.set_state(st0)
+    R.set_state(st0)
+    n_kept = 0
+    for i in range(60):
+        O.step(st, sc, history=hist)
+        n_kept += len(hist)
+        S.step(1)
+        R.step(1)
+    S.check()
+    R.check()
+    got, reset = S.numpy_state(), R.numpy_state()
+    W = sc.n_walls
+    assert n_kept > 100
+    for k in ("com", "q", "v", "w"):
+        assert max_rel(got[k][W:], st[k][W:]) <= 1e-8, k
+    assert np.abs(got["v"][W:] - reset["v"][W:]).max() > 1e-6 * np.abs(reset["v"][W:]).max()

@@ -375,3 +375,51 @@ def test_full_stencil_is_the_half_shell():
             if np.all(np.abs(xyz[a] - xyz[b]) <= 1):
                 ref[frozenset((a, b))] += 1
     assert got == ref
+
+
+# ------------------------------------------------------------ tangential history
+def test_tangential_history_stick_then_slip():
+    """Two overlapping boxes held in place, B sliding past A along y at v;
+    the history fed back each step (the persistent mode), physical law.
+    While sticking the stored displacement grows as k*v*dt and the
+    tangential force stays under mu*|Fn|; from the first slip on the
+    tangential force equals mu*|Fn| and opposes the sliding, and the stored
+    displacement stays within one step's travel of the slip value (at the
+    Coulomb limit a step may count as sticking by rounding and advance it by
+    v*dt, the next one resets it) (.cu:770-811 with qA = 1; the slip
+    displacement with the sign of the physical law)."""
+    from mwdiem_amd import scene
+    dt, v, mu = 1e-6, 1000.0, 0.3
+    sc = _Sc(2, dt=dt, I=(1e30, 1e30, 1e30), mass=2e-6, vmax=8, mode=1)
+    sc.com = np.zeros((3, 3))
+    sc.verts = np.stack([_box((-10, -10, -60), (10, 10, -40)), _box((0, 0, 0), (1, 1, 1)),
+                         _box((0.95, 0.13, 0.21), (1.95, 1.13, 1.21))])
+    sc.com = sc.verts.mean(axis=1)
+    sc.isize = np.full(3, 1e30)
+    sc.law_p = scene.Law4(Kn=1e4, en=0.5, et=0.5, mu=mu, mode=1)
+    sc.law_w = sc.law_p
+    st = _state(sc)
+    st["v"][2] = [0.0, -v, 0.0]   # relative velocity of A against B: +v along y
+    pairs = np.array([[1, 2]], np.int32)
+    disp = np.zeros((1, 4))
+    slip_at = None
+    prev = None
+    for k in range(1, 41):
+        coll, _, _, _, _, F, _, _, new = O.contact_state(pairs, st, sc, dispt=disp, with_disp=True)
+        assert coll[0] == 1
+        Fn, Ft = abs(F[1, 0]), np.hypot(F[1, 1], F[1, 2])
+        if slip_at is None and Ft < mu * Fn * (1 - 1e-12):
+            assert abs(new[0, 1] - k * v * dt) <= 1e-12 * k * v * dt
+            assert abs(new[0, 0]) < 1e-15 and abs(new[0, 2]) < 1e-15
+        else:
+            slip_at = slip_at or k
+            assert abs(Ft - mu * Fn) <= 1e-12 * Fn
+            assert F[1, 1] < 0.0  # on A, against A's sliding along +y
+            if prev is None:
+                prev = new[0, :3].copy()  # the slip displacement
+            assert np.all(np.abs(new[0, :3] - prev) <= v * dt * (1 + 1e-9))
+        disp = new
+    assert slip_at is not None and 3 < slip_at < 40
+    # without history (the reference as it runs) the same step gives a smaller tangential force
+    _, _, _, _, _, F0, _, _ = O.contact_state(pairs, st, sc)
+    assert np.hypot(F0[1, 1], F0[1, 2]) < np.hypot(F[1, 1], F[1, 2])
