@@ -60,8 +60,11 @@ def stage_bytes(sc_list, stats):
     S = stats["n_sphere_pass"]
     P_cell = P - stats["n_wall_pairs"]
     return {
-        "emit": 8 * P_cell + C * (4 + 8 + 52 + 8) + 4 * M,
-        "sphere": 16 * P + 32 * N,
+        # emission with the bounding-sphere test folded in (pairs read back
+        # from registers; 2 x 16 B centres and radii per pair, a collision
+        # code and a survivor index written)
+        "emit": 8 * P_cell + C * (4 + 8 + 52 + 8) + 4 * M + 16 * P + 32 * N + 4 * P + 4 * S,
+        "sphere": 0,
         "gjk": S * (4 + 8 + 48 + 8 + 2 * 24 * V + 12) + 128 * K,
         "epa+law": K * (136 + 224 + 48 * V + 8 + 168 + 16),
         "reduce+verlet": M * (448 + 272 + 48 * V) + 2 * K * 52 + 4 * N,
@@ -433,7 +436,8 @@ def main():
         "config": {"workload": "plain large settle: 100,000 convex polyhedra (8-20 hull points), 200x200x400 mm "
                                "walled box, FP64, dt=1e-6 s, Kn=1e4, e=0.5, mu=0.3, cell=2*r_circ,max",
                    "particles": sc.n_particles, "pairs_per_step": stats["n_pairs"],
-                   "contacts_per_step": stats["n_contacts"], "pair_tests_per_s": stats["n_pairs"] * args.steps / wall,
+                   "contacts_per_step": stats["n_contacts"], "sphere_survivors_per_step": stats["n_sphere_pass"],
+                   "pair_tests_per_s": stats["n_pairs"] * args.steps / wall,
                    "stage_ms_per_step": {k: round(v, 5) for k, v in stage_ms.items()},
                    "tuning": None if tuned is None else tuned["tuning"],
                    "untuned_ms_per_step_eager": None if tuned is None else tuned["default_ms_per_step"],

@@ -588,10 +588,15 @@ static int fine_contact(mwd_workspace* ws, const mwd_state* st, double* const* t
             (rc = mwd_set_i64(ctr + 11, 0, s)))
             return rc;
     }
+    // the candidate pass already sphere-tested these pairs (unless this is a
+    // second pass over them)
+    const bool tested = ws->fresh_pairs && ws->sphere_folded;
     ws->fresh_pairs = 0;
-    rc = mwd_launch_sphere(A, B, ws->pair_cap, ctr + 3, ws->collision, ws->surv, ctr + 10, s,
-                           mwd_kev(ws, MWD_KT_SPHERE, 0), mwd_kev(ws, MWD_KT_SPHERE, 1), ws->tune);
-    if (rc) return rc;
+    if (!tested) {
+        rc = mwd_launch_sphere(A, B, ws->pair_cap, ctr + 3, ws->collision, ws->surv, ctr + 10, s,
+                               mwd_kev(ws, MWD_KT_SPHERE, 0), mwd_kev(ws, MWD_KT_SPHERE, 1), ws->tune);
+        if (rc) return rc;
+    }
     mwd_mark(ws, 6);
     // a colliding pair claims contact slot c (counter 4) and writes
     // contact_pair[c]: contacts are in no set order; the per-row sums key

@@ -311,7 +311,11 @@ __global__ void __launch_bounds__(FILL_THREADS) k_cell_fill(const int32_t* __res
                                                           const int2* __restrict__ cell_seg,
                                                           const int64_t* __restrict__ cell_off,
                                                           const int64_t* __restrict__ counters,
-                                                          int32_t* __restrict__ pairs, int64_t cap) {
+                                                          int32_t* __restrict__ pairs, int64_t cap,
+                                                          const double* __restrict__ com,
+                                                          const double* __restrict__ radius,
+                                                          int32_t* __restrict__ surv, int64_t* n_surv,
+                                                          int32_t* __restrict__ collision) {
     __shared__ int64_t s_off[FILL_CELLS + 1];
     __shared__ int2 s_self[FILL_CELLS];
     __shared__ int2 s_seg[FILL_CELLS * MWD_MAX_OFF];
@@ -330,49 +334,64 @@ __global__ void __launch_bounds__(FILL_THREADS) k_cell_fill(const int32_t* __res
     __syncthreads();
     const int64_t total = s_off[nc];
     const int64_t pos0 = counters[1] + base;
-    for (int64_t q = tid; q < total; q += FILL_THREADS) {
-        // cell i: s_off[i] <= q < s_off[i + 1]
-        int lo = 0, hi = nc;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (s_off[mid] <= q) lo = mid;
-            else hi = mid;
-        }
-        const int i = lo;
-        const int32_t s0 = s_self[i].x, n = s_self[i].y;
-        int64_t t = q - s_off[i];
-        const int64_t tri = (int64_t)n * (n - 1) / 2;
-        int32_t a, b;
-        if (t < tri) {
-            int j = 0;
-            int64_t row = n - 1;  // pairs of member j with the later members
-            while (t >= row) {
-                t -= row;
-                ++j;
-                --row;
+    // a uniform trip count over the block, so that the survivors' ballot
+    // sees every lane of a wave
+    for (int64_t q0 = 0; q0 < total; q0 += FILL_THREADS) {
+        const int64_t q = q0 + tid;
+        bool have = q < total;
+        int32_t a = 0, b = 0;
+        if (have) {
+            // cell i: s_off[i] <= q < s_off[i + 1]
+            int lo = 0, hi = nc;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_off[mid] <= q) lo = mid;
+                else hi = mid;
             }
-            a = perm[s0 + j];
-            b = perm[s0 + j + 1 + (int32_t)t];
-        } else {
-            t -= tri;
-            int o = 0;
-            for (; o < MWD_MAX_OFF; ++o) {
-                const int64_t sz = (int64_t)n * s_seg[i * MWD_MAX_OFF + o].y;
-                if (t < sz) break;
-                t -= sz;
+            const int i = lo;
+            const int32_t s0 = s_self[i].x, n = s_self[i].y;
+            int64_t t = q - s_off[i];
+            const int64_t tri = (int64_t)n * (n - 1) / 2;
+            if (t < tri) {
+                int j = 0;
+                int64_t row = n - 1;  // pairs of member j with the later members
+                while (t >= row) {
+                    t -= row;
+                    ++j;
+                    --row;
+                }
+                a = perm[s0 + j];
+                b = perm[s0 + j + 1 + (int32_t)t];
+            } else {
+                t -= tri;
+                int o = 0;
+                for (; o < MWD_MAX_OFF; ++o) {
+                    const int64_t sz = (int64_t)n * s_seg[i * MWD_MAX_OFF + o].y;
+                    if (t < sz) break;
+                    t -= sz;
+                }
+                if (o == MWD_MAX_OFF) {
+                    have = false;  // not reached: q lies inside cell i's range
+                } else {
+                    const int2 sg = s_seg[i * MWD_MAX_OFF + o];
+                    const int32_t j = (int32_t)(t / sg.y), k = (int32_t)(t - (int64_t)j * sg.y);
+                    a = perm[s0 + j];
+                    b = perm[sg.x + k];
+                }
             }
-            if (o == MWD_MAX_OFF) continue;  // not reached: q lies inside cell i's range
-            const int2 sg = s_seg[i * MWD_MAX_OFF + o];
-            const int32_t j = (int32_t)(t / sg.y), k = (int32_t)(t - (int64_t)j * sg.y);
-            a = perm[s0 + j];
-            b = perm[sg.x + k];
         }
         const int64_t pos = pos0 + q;
-        if (pos < cap) {
+        have = have && pos < cap;
+        if (have) {
             int2 v;
             v.x = a;
             v.y = b;
             reinterpret_cast<int2*>(pairs)[pos] = v;
+        }
+        if (radius) {  // apart: decided (collision 0); the others go to GJK, which writes theirs
+            const bool keep = have && spheres_meet(com, radius, a, b);
+            if (have && !keep) collision[pos] = 0;
+            append_survivor(keep, pos, surv, n_surv, cap);
         }
     }
 }
@@ -400,6 +419,8 @@ struct WallIO {
     const int64_t* wt_off;
     int32_t* pairs;
     int64_t cap;
+    int32_t* surv;  // non-NULL: every wall pair goes to GJK as survivor p (a wall's bounding sphere,
+                    // half the plate's diagonal, meets nearly every particle in its slab)
     __device__ __forceinline__ void decode(int64_t t, const SceneDev** sc, int64_t* rj, int64_t* rk) const {
         const int s = scene_of_test(wt_off, S, t);
         *sc = &scenes[s];
@@ -427,6 +448,7 @@ struct WallIO {
         decode(t, &sc, &rj, &rk);
         pairs[2 * pos] = (int32_t)rj;
         pairs[2 * pos + 1] = (int32_t)rk;
+        if (surv) surv[pos] = (int32_t)pos;
     }
     __device__ __forceinline__ void finish(int64_t) const {}
 };
@@ -492,13 +514,16 @@ struct CellTableIO {
 // candidate count (counters 1 wall pairs + 2 cell pairs -> 3 capped, 7
 // uncapped) and restarts the counters of the fine-contact stage that
 // consumes the pairs.
-__device__ __forceinline__ void close_pairs(int64_t* counters, int64_t cell_pairs, int64_t cap, uint32_t* flags) {
+__device__ __forceinline__ void close_pairs(int64_t* counters, int64_t cell_pairs, int64_t cap, uint32_t* flags,
+                                            bool folded) {
     const int64_t P = counters[1] + cell_pairs;
+    // survivors of the sphere test: the wall pairs listed so far when the
+    // emission appends the rest, else none yet
+    counters[10] = folded ? (counters[1] < cap ? counters[1] : cap) : 0;
     counters[3] = P < cap ? P : cap;
     counters[7] = P;
     counters[4] = 0;   // contacts (slots claimed by GJK) of the fine-contact stage
     counters[8] = 0;   // simplex slots of the per-pair fine-contact entry point
-    counters[10] = 0;  // sphere-test survivors
     counters[11] = 0;  // and its bucket overflow records
     if (P > cap) atomicOr(flags, MWD_F_PAIR_OVERFLOW);
 }
@@ -509,9 +534,10 @@ struct CellCountIO {  // sorted path: counts written by k_cell_info
     int64_t* counters;
     int64_t cap;
     uint32_t* flags;
+    bool folded;
     __device__ __forceinline__ int32_t load(int64_t i) const { return cnt[i]; }
     __device__ __forceinline__ void store(int64_t i, int64_t o, int32_t) const { off[i] = o; }
-    __device__ __forceinline__ void finish(int64_t total) const { close_pairs(counters, total, cap, flags); }
+    __device__ __forceinline__ void finish(int64_t total) const { close_pairs(counters, total, cap, flags, folded); }
 };
 
 struct CellInfoIO {  // key-table path: the per-cell work runs as the scan's load
@@ -528,12 +554,13 @@ struct CellInfoIO {  // key-table path: the per-cell work runs as the scan's loa
     int64_t* counters;
     int64_t cap;
     uint32_t* flags;
+    bool folded;
     __device__ __forceinline__ int32_t load(int64_t c) const {
         return cell_work(c, c, false, members, cell_start, cell_key, scene_of, scenes, perm, cell_self, cell_seg,
                          flags, NbTable{cell_of, cell_key, counters[0]});
     }
     __device__ __forceinline__ void store(int64_t i, int64_t o, int32_t) const { off[i] = o; }
-    __device__ __forceinline__ void finish(int64_t total) const { close_pairs(counters, total, cap, flags); }
+    __device__ __forceinline__ void finish(int64_t total) const { close_pairs(counters, total, cap, flags, folded); }
 };
 
 // Sorted path, before the cells of this step: clear the row flags.
@@ -542,8 +569,8 @@ __global__ void k_cells_prep(int32_t* __restrict__ row_first, int64_t n_rows) {
     if (i < n_rows) row_first[i] = 0;
 }
 
-__global__ void k_finish_pairs(int64_t* counters, int64_t cap, uint32_t* flags) {
-    close_pairs(counters, counters[2], cap, flags);
+__global__ void k_finish_pairs(int64_t* counters, int64_t cap, uint32_t* flags, int folded) {
+    close_pairs(counters, counters[2], cap, flags, folded != 0);
 }
 
 __global__ void k_zero_i64(int64_t* p, int n) {
@@ -575,13 +602,17 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
         hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(64), 0, s, ctr, 4);
         MWD_LAUNCH_CHECK();
     }
+    // with bounding radii the survivors of the sphere test are listed as the
+    // pairs are written: the wall pairs (all of them) at their own indices,
+    // the cell pairs whose spheres meet appended after them by the emission
+    ws->sphere_folded = st->radius != nullptr;
     // -- wall slab pairs: test, count and write in one pass, on the side
     // stream; the cell pass runs meanwhile and joins before it needs the wall
     // pair count (the offset of the cell pairs)
     bool joined = true;
     if (walls) {
-        const WallIO io{st->com, st->wall_n, st->wall_c, ws->scenes_dev, ws->n_scenes, ws->scene_wt_off,
-                        ws->pairs, ws->pair_cap};
+        const WallIO io{st->com,   st->wall_n,    st->wall_c, ws->scenes_dev, ws->n_scenes, ws->scene_wt_off,
+                        ws->pairs, ws->pair_cap, st->radius ? ws->surv : nullptr};
         MWD_HIP_CHECK(hipEventRecord(ws->ev_fork, s));
         MWD_HIP_CHECK(hipStreamWaitEvent(ws->side, ws->ev_fork, 0));
         rc = mwd_scan_io(io, ws->n_wall_tests, nullptr, ctr + 1, ws->scan_tmp_side, ws->scan_tmp_side_len,
@@ -615,7 +646,7 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
             if ((rc = join())) return rc;
             rc = mwd_scan_io<CellInfoIO, 1>(CellInfoIO{ws->vals, ws->cell_start, ws->cell_key, ws->cell_of, ws->scene_of,
                                         ws->scenes_dev, ws->perm, ws->cell_self, ws->cell_seg, ws->cell_off, ctr,
-                                        ws->pair_cap, ws->flags},
+                                        ws->pair_cap, ws->flags, ws->sphere_folded != 0},
                              M, ctr + 0, ctr + 2, ws->scan_tmp, ws->scan_tmp_len, s);
             if (rc) return rc;
         } else {
@@ -642,20 +673,24 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
                                ws->cell_self, ws->cell_seg, ws->cell_cnt, ws->flags);
             MWD_LAUNCH_CHECK();
             if ((rc = join())) return rc;
-            rc = mwd_scan_io(CellCountIO{ws->cell_cnt, ws->cell_off, ctr, ws->pair_cap, ws->flags}, M, ctr + 0,
+            rc = mwd_scan_io(CellCountIO{ws->cell_cnt, ws->cell_off, ctr, ws->pair_cap, ws->flags,
+                                         ws->sphere_folded != 0},
+                             M, ctr + 0,
                              ctr + 2, ws->scan_tmp, ws->scan_tmp_len, s);
             if (rc) return rc;
         }
         mwd_mark(ws, 3);
         MWD_LAUNCH_EV(mwd_kev(ws, MWD_KT_EMIT, 0), mwd_kev(ws, MWD_KT_EMIT, 1), k_cell_fill,
                       dim3((unsigned)((M + FILL_CELLS - 1) / FILL_CELLS)), dim3(FILL_THREADS), 0, s, ws->perm,
-                      ws->cell_self, ws->cell_seg, ws->cell_off, ctr, ws->pairs, ws->pair_cap);
+                      ws->cell_self, ws->cell_seg, ws->cell_off, ctr, ws->pairs, ws->pair_cap, st->com, st->radius,
+                      ws->surv, ctr + 10, ws->collision);
         MWD_LAUNCH_CHECK();
     }
     if ((rc = join())) return rc;
     mwd_mark(ws, 4);
     if (M <= 0) {  // no cell-count scan closed the candidate count
-        hipLaunchKernelGGL(k_finish_pairs, dim3(1), dim3(1), 0, s, ctr, ws->pair_cap, ws->flags);
+        hipLaunchKernelGGL(k_finish_pairs, dim3(1), dim3(1), 0, s, ctr, ws->pair_cap, ws->flags,
+                           ws->sphere_folded);
         MWD_LAUNCH_CHECK();
     }
     return MWD_OK;

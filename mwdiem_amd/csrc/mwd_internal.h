@@ -139,6 +139,31 @@ __device__ __forceinline__ int64_t view_row(const BodyView& b, int64_t p) {
     return b.rows ? (int64_t)b.rows[p * b.row_stride] : p;
 }
 
+// Bounding spheres of rows a and b meet (the test that decides most
+// candidate pairs before GJK; its 'apart' answer is the reference's first
+// support test's, .cu:1013).
+__device__ __forceinline__ bool spheres_meet(const double* com, const double* radius, int64_t a, int64_t b) {
+    const d3 d = ld3(com + 3 * b) - ld3(com + 3 * a);
+    const double rr = radius[a] + radius[b];
+    return !(nrm2(d) > rr * rr);
+}
+
+// Append pair p to the sphere-test survivors when keep, one atomic per wave
+// for the lanes that call this together.
+__device__ __forceinline__ void append_survivor(bool keep, int64_t p, int32_t* surv, int64_t* n_surv, int64_t cap) {
+    const uint64_t m = __ballot(keep);
+    if (!m) return;
+    const uint64_t act = __ballot(1);
+    const int lane = threadIdx.x & (MWD_WAVE - 1);
+    const int leader = __ffsll((long long)act) - 1;
+    int64_t base = 0;
+    if (lane == leader) base = (int64_t)atomicAdd((unsigned long long*)n_surv, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader, MWD_WAVE);
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (MWD_WAVE - lane));
+    const int64_t at = base + __popcll(m & lt);
+    if (keep && at < cap) surv[at] = (int32_t)p;
+}
+
 // The contact records of row P in ascending record order (2p + side for the
 // contact of candidate pair p, so in the order of the candidate pairs,
 // whatever order the contacts were found in): its bucket of at most
@@ -219,6 +244,7 @@ struct mwd_workspace {
     int32_t* perm;             // [M] members of each cell in set order
     // key-table path (cell_table = 1): cells numbered by rank directly, no sort
     int cell_table;            // key-table path in use (MWD_CELLS_TABLE)
+    int sphere_folded;         // host: the last candidate pass also did the bounding-sphere test
     int cell_table_ok;         // its tables exist
     PairTune tune;
     int32_t* cell_of;          // [key_span_total] cell of a key; valid only where cell_key[cell_of[k]] == k
