@@ -240,7 +240,7 @@ def bench_diff(steps: int, graph_steps: int, n: int = 4096, short: int = 200):
             "mode": "forward tangents (3 directions); reverse adjoint with step checkpoints"}
 
 
-def bench_polydisperse(steps: int, graph_steps: int, v_rms: float = 1000.0):
+def bench_polydisperse(steps: int, graph_steps: int, v_rms: float = 1000.0, tune: bool = True):
     """The polydisperse stress configuration: 1,000,000 hulls (circumradius
     log-uniform 0.5-2.5 mm, 6-32 points) in a 400 mm box whose ceiling slides
     along x at 0.1 m/s, physical law, FP64 GJK/EPA. The hulls start from
@@ -264,10 +264,18 @@ def bench_polydisperse(steps: int, graph_steps: int, v_rms: float = 1000.0):
     v[W:] = rng.normal(size=(sc.n_particles, 3)) * v_rms
     v[:W] = sc.wall_v[:W]
     out, com = {}, {}
+    tuning = None
     for fp32 in (False, True):
         S = Stepper(sc, reduce_fp32=fp32)
         S.set_state({"v": v, "v12": v})
         S.step(2)
+        if tune:
+            if tuning is None:  # choices timed on the FP64 run, used for both
+                from mwdiem_amd.tune import autotune
+                tuning = autotune(S, steps=5)["tuning"]
+                print(f"# polydisperse autotune: {tuning}", file=sys.stderr, flush=True)
+            else:
+                S.set_tuning(**tuning)
         S.capture(graph_steps)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -289,7 +297,7 @@ def bench_polydisperse(steps: int, graph_steps: int, v_rms: float = 1000.0):
     # a granular gas is chaotic: compare the two reductions statistically
     # (kinetic energy, contact count, how many hulls have parted company)
     dx = np.linalg.norm(com[True] - com[False], axis=1)
-    out.update({"particles": sc.n_particles, "steps": steps + 2, "v_rms_mm_s": v_rms,
+    out.update({"particles": sc.n_particles, "steps": steps + 2, "v_rms_mm_s": v_rms, "tuning": tuning,
                 "kinetic_energy_rel_diff": abs(out["fp32"]["kinetic_energy"] / out["fp64"]["kinetic_energy"] - 1.0),
                 "median_dx_fp32_vs_fp64_mm": float(np.median(dx)), "frac_dx_above_1um": float(np.mean(dx > 1e-3)),
                 "max_dx_fp32_vs_fp64_mm": float(dx.max()), "max_displacement_mm": disp})
@@ -423,7 +431,7 @@ def main():
 
     poly = None
     if not args.no_poly:
-        poly = bench_polydisperse(args.poly_steps, args.graph_steps)
+        poly = bench_polydisperse(args.poly_steps, args.graph_steps, tune=not args.no_tune)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -185,6 +185,9 @@ int mwd_set_reduction(mwd_workspace* ws, int32_t mode);
  *                of 64 in 64..1024 (256).
  *   verlet_block particles per block of the fused force reduction and
  *                integrator: 64, 128 or 256 (256).
+ *   cell_items   key-table cell pass: cells per thread of the scan that
+ *                also does the per-cell work (1, 2 or 4; 1), or 0 for a
+ *                separate per-cell kernel followed by the pair-count scan.
  * mwd_set_tuning applies to later steps (graphs captured before keep their
  * launches) and returns MWD_E_INVALID for a choice out of range. */
 #define MWD_CELLS_TABLE 0
@@ -195,6 +198,7 @@ typedef struct mwd_tuning {
     int32_t gjk_blocks;
     int32_t sphere_block;
     int32_t verlet_block;
+    int32_t cell_items;
 } mwd_tuning;
 int mwd_get_tuning(const mwd_workspace* ws, mwd_tuning* out);
 int mwd_set_tuning(mwd_workspace* ws, const mwd_tuning* t);

@@ -475,6 +475,7 @@ MWD_API int mwd_get_tuning(const mwd_workspace* ws, mwd_tuning* out) {
     out->gjk_blocks = ws->tune.gjk_blocks;
     out->sphere_block = ws->tune.sphere_block;
     out->verlet_block = ws->tune.verlet_block;
+    out->cell_items = ws->tune.cell_items;
     return MWD_OK;
 }
 
@@ -488,9 +489,11 @@ MWD_API int mwd_set_tuning(mwd_workspace* ws, const mwd_tuning* t) {
     const bool blocks_ok = t->gjk_blocks >= 64 && t->gjk_blocks <= 65536;
     const bool sphere_ok = t->sphere_block >= 64 && t->sphere_block <= 1024 && t->sphere_block % 64 == 0;
     const bool verlet_ok = t->verlet_block == 64 || t->verlet_block == 128 || t->verlet_block == 256;
-    if (!pass_ok || !group_ok || !blocks_ok || !sphere_ok || !verlet_ok) {
+    const bool items_ok = t->cell_items == 0 || t->cell_items == 1 || t->cell_items == 2 || t->cell_items == 4;
+    if (!pass_ok || !group_ok || !blocks_ok || !sphere_ok || !verlet_ok || !items_ok) {
         mwd_set_error(!pass_ok ? "mwd_set_tuning: cell_pass unknown, or MWD_CELLS_TABLE without key tables"
-                               : "mwd_set_tuning: gjk_group, gjk_blocks, sphere_block or verlet_block out of range");
+                               : "mwd_set_tuning: gjk_group, gjk_blocks, sphere_block, verlet_block or cell_items "
+                                 "out of range");
         return MWD_E_INVALID;
     }
     ws->cell_table = t->cell_pass == MWD_CELLS_TABLE;
@@ -498,6 +501,7 @@ MWD_API int mwd_set_tuning(mwd_workspace* ws, const mwd_tuning* t) {
     ws->tune.gjk_blocks = t->gjk_blocks;
     ws->tune.sphere_block = t->sphere_block;
     ws->tune.verlet_block = t->verlet_block;
+    ws->tune.cell_items = t->cell_items;
     return MWD_OK;
 }
 

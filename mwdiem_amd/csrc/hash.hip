@@ -295,6 +295,21 @@ __global__ void k_cell_info(int32_t* __restrict__ vals, const int32_t* __restric
                             flags, NbSorted{cell_key, C});
 }
 
+// Key-table path, the per-cell work as a kernel of its own (one cell per
+// thread; the pair-count scan follows), instead of inside the scan's load.
+__global__ void k_cell_info_table(int32_t* __restrict__ members, const int32_t* __restrict__ cell_start,
+                                  const uint32_t* __restrict__ cell_key, const int32_t* __restrict__ cell_of,
+                                  const int64_t* __restrict__ n_cells, const int32_t* __restrict__ scene_of,
+                                  const SceneDev* __restrict__ scenes, int32_t* __restrict__ perm,
+                                  int2* __restrict__ cell_self, int2* __restrict__ cell_seg,
+                                  int32_t* __restrict__ cell_cnt, uint32_t* flags) {
+    const int64_t C = *n_cells;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    cell_cnt[c] = cell_work(c, c, false, members, cell_start, cell_key, scene_of, scenes, perm, cell_self, cell_seg,
+                            flags, NbTable{cell_of, cell_key, C});
+}
+
 // Pair emission, one output pair per thread: a block takes 64 cells by rank,
 // stages their pair offsets, member ranges and neighbour ranges in LDS
 // (coalesced loads of the rank-ordered tables), and its threads walk the
@@ -644,10 +659,29 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
                                ws->key_cnt, ws->cell_of, ws->cell_start, ws->vals);
             MWD_LAUNCH_CHECK();
             if ((rc = join())) return rc;
-            rc = mwd_scan_io<CellInfoIO, 1>(CellInfoIO{ws->vals, ws->cell_start, ws->cell_key, ws->cell_of, ws->scene_of,
-                                        ws->scenes_dev, ws->perm, ws->cell_self, ws->cell_seg, ws->cell_off, ctr,
-                                        ws->pair_cap, ws->flags, ws->sphere_folded != 0},
-                             M, ctr + 0, ctr + 2, ws->scan_tmp, ws->scan_tmp_len, s);
+            const CellInfoIO io{ws->vals,      ws->cell_start, ws->cell_key,  ws->cell_of, ws->scene_of,
+                                ws->scenes_dev, ws->perm,      ws->cell_self, ws->cell_seg, ws->cell_off,
+                                ctr,           ws->pair_cap,   ws->flags,     ws->sphere_folded != 0};
+            switch (ws->tune.cell_items) {
+                case 0:  // the per-cell work as its own kernel, then the pair-count scan
+                    hipLaunchKernelGGL(k_cell_info_table, dim3(blocks_for(M)), dim3(256), 0, s, ws->vals,
+                                       ws->cell_start, ws->cell_key, ws->cell_of, ctr + 0, ws->scene_of,
+                                       ws->scenes_dev, ws->perm, ws->cell_self, ws->cell_seg, ws->cell_cnt,
+                                       ws->flags);
+                    MWD_LAUNCH_CHECK();
+                    rc = mwd_scan_io(CellCountIO{ws->cell_cnt, ws->cell_off, ctr, ws->pair_cap, ws->flags,
+                                                 ws->sphere_folded != 0},
+                                     M, ctr + 0, ctr + 2, ws->scan_tmp, ws->scan_tmp_len, s);
+                    break;
+                case 2:
+                    rc = mwd_scan_io<CellInfoIO, 2>(io, M, ctr + 0, ctr + 2, ws->scan_tmp, ws->scan_tmp_len, s);
+                    break;
+                case 4:
+                    rc = mwd_scan_io<CellInfoIO, 4>(io, M, ctr + 0, ctr + 2, ws->scan_tmp, ws->scan_tmp_len, s);
+                    break;
+                default:  // one cell per thread inside the scan
+                    rc = mwd_scan_io<CellInfoIO, 1>(io, M, ctr + 0, ctr + 2, ws->scan_tmp, ws->scan_tmp_len, s);
+            }
             if (rc) return rc;
         } else {
             hipLaunchKernelGGL(k_sort_keys, dim3(blocks_for(M)), dim3(256), 0, s, st->com, keys_in, ws->plist, M,

@@ -127,6 +127,7 @@ struct PairTune {
     int gjk_blocks = 1024;   // grid cap of the grid-strided GJK launch
     int sphere_block = 256;  // threads per block of the sphere test
     int verlet_block = 256;  // particles per block of the fused reduction + integrator: 64, 128 or 256
+    int cell_items = 1;      // key-table cell pass: cells per thread of the fused cell scan (1, 2, 4), 0 = split
 };
 
 // Face distances at or above this are EPA's placeholders (1e30 for a face

@@ -216,7 +216,8 @@ class Stepper:
                 self.step(n - done)
 
     def tuning(self) -> dict:
-        """Launch choices of the workspace (cell_pass, gjk_group, gjk_blocks, sphere_block, verlet_block)."""
+        """Launch choices of the workspace (cell_pass, gjk_group, gjk_blocks, sphere_block, verlet_block,
+        cell_items)."""
         t = _lib.Tuning()
         _lib.check(_lib.load().mwd_get_tuning(self.ws, C.byref(t)), "mwd_get_tuning")
         return {name: int(getattr(t, name)) for name, _ in _lib.Tuning._fields_}

@@ -33,6 +33,7 @@ SPACE = {
     "gjk_blocks": (256, 1024, 4096),
     "sphere_block": (128, 256, 512),
     "cell_pass": (_lib.CELLS_TABLE, _lib.CELLS_SORTED),
+    "cell_items": (0, 1, 2, 4),
 }
 
 

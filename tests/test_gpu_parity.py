@@ -195,6 +195,7 @@ def test_tuning_choices_change_nothing(gpu, dense_phys):
     assert ref.status()["n_contacts"] > 0
     variants = [dict(gjk_group=4), dict(gjk_group=16), dict(gjk_blocks=64), dict(gjk_blocks=8192),
                 dict(sphere_block=64), dict(sphere_block=1024), dict(verlet_block=64), dict(verlet_block=128),
+                dict(cell_items=0), dict(cell_items=2), dict(cell_items=4),
                 dict(cell_pass=_lib.CELLS_SORTED, gjk_group=16)]
     for v in variants:
         S = _stepper(sc)
@@ -208,7 +209,7 @@ def test_tuning_choices_change_nothing(gpu, dense_phys):
         S.close()
     S = _stepper(sc)
     for bad in (dict(gjk_group=5), dict(gjk_blocks=8), dict(sphere_block=100), dict(cell_pass=7),
-                dict(verlet_block=96)):
+                dict(verlet_block=96), dict(cell_items=3)):
         with pytest.raises(_lib.MwdError):
             S.set_tuning(**bad)
 
