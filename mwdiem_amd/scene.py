@@ -659,13 +659,18 @@ def config_polydisperse(seed: int = 3, n: int = 1_000_000, L: float = 400.0, r_r
     sites = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], axis=1) * spacing + margin
     even, odd = np.flatnonzero(par == 0), np.flatnonzero(par == 1)
     big, small = order[:even.size], order[even.size:]
-    site_of = np.empty(n, dtype=np.int64)
-    site_of[big] = even
-    site_of[small] = odd
     if m > 1 and r[big].max() + r[small].max() >= spacing:
         raise ValueError("bounding spheres of lattice neighbours would meet; enlarge L or narrow r_range")
     if m > 1 and 2.0 * r.max() >= spacing * math.sqrt(2.0):
         raise ValueError("bounding spheres of same-parity neighbours would meet")
+    # rows in lattice order (neighbouring rows are neighbours in space, as a
+    # generator that fills a box layer by layer numbers them): the radius of
+    # row i is the one its site's parity received
+    r_site = np.empty(n)
+    r_site[even] = r[big]
+    r_site[odd] = r[small]
+    r = r_site
+    site_of = np.arange(n, dtype=np.int64)
     scale = r / prad[idx]
     ms = 2.0 * float(r.max())
     T = 10.0 * ms
