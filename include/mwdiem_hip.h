@@ -188,6 +188,12 @@ int mwd_set_reduction(mwd_workspace* ws, int32_t mode);
  *   cell_items   key-table cell pass: cells per thread of the scan that
  *                also does the per-cell work (1, 2 or 4; 1), or 0 for a
  *                separate per-cell kernel followed by the pair-count scan.
+ *   verlet_waves minimum waves per SIMD that the register allocation of the
+ *                fused reduction + integrator must allow: 1 (the compiler's
+ *                choice; default), 3 or 4 (fewer registers, possibly spills).
+ *   vertex_pack  1: the vertex pass walks the block's real vertices only
+ *                (prefix sum of the vertex counts); 0 (default): every vmax
+ *                slot, skipping the padding.
  * mwd_set_tuning applies to later steps (graphs captured before keep their
  * launches) and returns MWD_E_INVALID for a choice out of range. */
 #define MWD_CELLS_TABLE 0
@@ -199,6 +205,8 @@ typedef struct mwd_tuning {
     int32_t sphere_block;
     int32_t verlet_block;
     int32_t cell_items;
+    int32_t verlet_waves;
+    int32_t vertex_pack;
 } mwd_tuning;
 int mwd_get_tuning(const mwd_workspace* ws, mwd_tuning* out);
 int mwd_set_tuning(mwd_workspace* ws, const mwd_tuning* t);

@@ -476,6 +476,8 @@ MWD_API int mwd_get_tuning(const mwd_workspace* ws, mwd_tuning* out) {
     out->sphere_block = ws->tune.sphere_block;
     out->verlet_block = ws->tune.verlet_block;
     out->cell_items = ws->tune.cell_items;
+    out->verlet_waves = ws->tune.verlet_waves;
+    out->vertex_pack = ws->tune.vertex_pack;
     return MWD_OK;
 }
 
@@ -490,10 +492,12 @@ MWD_API int mwd_set_tuning(mwd_workspace* ws, const mwd_tuning* t) {
     const bool sphere_ok = t->sphere_block >= 64 && t->sphere_block <= 1024 && t->sphere_block % 64 == 0;
     const bool verlet_ok = t->verlet_block == 64 || t->verlet_block == 128 || t->verlet_block == 256;
     const bool items_ok = t->cell_items == 0 || t->cell_items == 1 || t->cell_items == 2 || t->cell_items == 4;
-    if (!pass_ok || !group_ok || !blocks_ok || !sphere_ok || !verlet_ok || !items_ok) {
+    const bool waves_ok = t->verlet_waves == 1 || t->verlet_waves == 3 || t->verlet_waves == 4;
+    const bool pack_ok = t->vertex_pack == 0 || t->vertex_pack == 1;
+    if (!pass_ok || !group_ok || !blocks_ok || !sphere_ok || !verlet_ok || !items_ok || !waves_ok || !pack_ok) {
         mwd_set_error(!pass_ok ? "mwd_set_tuning: cell_pass unknown, or MWD_CELLS_TABLE without key tables"
-                               : "mwd_set_tuning: gjk_group, gjk_blocks, sphere_block, verlet_block or cell_items "
-                                 "out of range");
+                               : "mwd_set_tuning: gjk_group, gjk_blocks, sphere_block, verlet_block, cell_items, "
+                                 "verlet_waves or vertex_pack out of range");
         return MWD_E_INVALID;
     }
     ws->cell_table = t->cell_pass == MWD_CELLS_TABLE;
@@ -502,6 +506,8 @@ MWD_API int mwd_set_tuning(mwd_workspace* ws, const mwd_tuning* t) {
     ws->tune.sphere_block = t->sphere_block;
     ws->tune.verlet_block = t->verlet_block;
     ws->tune.cell_items = t->cell_items;
+    ws->tune.verlet_waves = t->verlet_waves;
+    ws->tune.vertex_pack = t->vertex_pack;
     return MWD_OK;
 }
 

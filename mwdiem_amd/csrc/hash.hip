@@ -334,6 +334,11 @@ __global__ void __launch_bounds__(FILL_THREADS) k_cell_fill(const int32_t* __res
     __shared__ int64_t s_off[FILL_CELLS + 1];
     __shared__ int2 s_self[FILL_CELLS];
     __shared__ int2 s_seg[FILL_CELLS * MWD_MAX_OFF];
+    // per cell, where each part of its pair range starts: [0] the first
+    // neighbour pair (after the same-cell pairs), [o + 1] the pairs after
+    // neighbour o. Saturated at UINT32_MAX; a cell's pair count (and so every
+    // offset decoded below) stays under 2^31.
+    __shared__ uint32_t s_cum[FILL_CELLS][MWD_MAX_OFF + 1];
     const int64_t C = counters[0];
     const int64_t r0 = (int64_t)blockIdx.x * FILL_CELLS;
     if (r0 >= C) return;  // uniform per block
@@ -346,6 +351,16 @@ __global__ void __launch_bounds__(FILL_THREADS) k_cell_fill(const int32_t* __res
         s_off[tid] = cell_off[r0 + tid] - base;
     }
     if (tid == 0) s_off[nc] = (r0 + nc < C ? cell_off[r0 + nc] : counters[2]) - base;  // counters[2]: all cell pairs
+    __syncthreads();
+    if (tid < nc) {
+        const uint64_t n = (uint64_t)s_self[tid].y;
+        uint64_t acc = n * (n - (n > 0)) / 2;
+        s_cum[tid][0] = acc < UINT32_MAX ? (uint32_t)acc : UINT32_MAX;
+        for (int o = 0; o < MWD_MAX_OFF; ++o) {
+            acc += n * (uint64_t)s_seg[tid * MWD_MAX_OFF + o].y;
+            s_cum[tid][o + 1] = acc < UINT32_MAX ? (uint32_t)acc : UINT32_MAX;
+        }
+    }
     __syncthreads();
     const int64_t total = s_off[nc];
     const int64_t pos0 = counters[1] + base;
@@ -365,11 +380,10 @@ __global__ void __launch_bounds__(FILL_THREADS) k_cell_fill(const int32_t* __res
             }
             const int i = lo;
             const int32_t s0 = s_self[i].x, n = s_self[i].y;
-            int64_t t = q - s_off[i];
-            const int64_t tri = (int64_t)n * (n - 1) / 2;
-            if (t < tri) {
+            uint32_t t = (uint32_t)(q - s_off[i]);
+            if (t < s_cum[i][0]) {
                 int j = 0;
-                int64_t row = n - 1;  // pairs of member j with the later members
+                uint32_t row = (uint32_t)n - 1u;  // pairs of member j with the later members
                 while (t >= row) {
                     t -= row;
                     ++j;
@@ -378,21 +392,17 @@ __global__ void __launch_bounds__(FILL_THREADS) k_cell_fill(const int32_t* __res
                 a = perm[s0 + j];
                 b = perm[s0 + j + 1 + (int32_t)t];
             } else {
-                t -= tri;
+                // the last neighbour o whose pairs start at or before t (empty
+                // neighbours start where the next one does, so they are passed)
                 int o = 0;
-                for (; o < MWD_MAX_OFF; ++o) {
-                    const int64_t sz = (int64_t)n * s_seg[i * MWD_MAX_OFF + o].y;
-                    if (t < sz) break;
-                    t -= sz;
-                }
-                if (o == MWD_MAX_OFF) {
-                    have = false;  // not reached: q lies inside cell i's range
-                } else {
-                    const int2 sg = s_seg[i * MWD_MAX_OFF + o];
-                    const int32_t j = (int32_t)(t / sg.y), k = (int32_t)(t - (int64_t)j * sg.y);
-                    a = perm[s0 + j];
-                    b = perm[sg.x + k];
-                }
+#pragma unroll
+                for (int step = 8; step > 0; step >>= 1)
+                    if (o + step < MWD_MAX_OFF && s_cum[i][o + step] <= t) o += step;
+                const int2 sg = s_seg[i * MWD_MAX_OFF + o];
+                const uint32_t tt = t - s_cum[i][o];
+                const uint32_t j = tt / (uint32_t)sg.y, k = tt - j * (uint32_t)sg.y;
+                a = perm[s0 + (int32_t)j];
+                b = perm[sg.x + (int32_t)k];
             }
         }
         const int64_t pos = pos0 + q;

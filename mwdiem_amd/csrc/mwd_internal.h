@@ -128,6 +128,8 @@ struct PairTune {
     int sphere_block = 256;  // threads per block of the sphere test
     int verlet_block = 256;  // particles per block of the fused reduction + integrator: 64, 128 or 256
     int cell_items = 1;      // key-table cell pass: cells per thread of the fused cell scan (1, 2, 4), 0 = split
+    int verlet_waves = 1;    // minimum waves per SIMD asked of the reduction + integrator's registers: 1, 3 or 4
+    int vertex_pack = 0;     // vertex pass over real vertices only (1) or over every vmax slot (0)
 };
 
 // Face distances at or above this are EPA's placeholders (1e30 for a face

@@ -138,31 +138,101 @@ constexpr int VBLOCK = 256;
 // vertex is transformed by exactly the operations of the per-lane loop; the
 // bounding radius takes the maximum through an LDS atomic on the bits of the
 // non-negative squared distances (NaNs skipped, as fmax skips them).
-template <int VB>
+// New position of one vertex: exactly the operations of the per-lane loop.
+__device__ __forceinline__ d3 vertex_move(const VertJob& J, d3 p) {
+    const q4 qr = J.qr;
+    const d3 r = qrot(qr, p - J.com, qconj(qr));
+    d3 nv = (r + J.com) + J.dx;
+    if (J.out) nv = nv + J.dx * -1.0;
+    return nv;
+}
+
+// PACK false: thread idx takes slot (idx / vmax, idx % vmax) of the block's
+// nb x vmax slots and skips slots at or above the particle's count. PACK
+// true: the block's vertex counts are prefix-summed first (vbase) and thread
+// idx takes the idx-th real vertex, so no lane idles on padding slots.
+template <int VB, bool PACK>
 __device__ __forceinline__ void vertex_pass(double* __restrict__ verts, const int64_t* rows, const VertJob* job,
-                                            unsigned long long* r2bits, int nb, int vmax, double* radius) {
+                                            unsigned long long* r2bits, int* vbase, int nb, int vmax, double* radius) {
     const int tid = threadIdx.x;
     r2bits[tid] = 0ull;
-    __syncthreads();
-    const int total = nb * vmax;
-    for (int idx = tid; idx < total; idx += VB) {
-        const int j = idx / vmax;
-        const int i = idx - j * vmax;
-        const VertJob& J = job[j];
-        if (i >= J.nrot) continue;
-        double* pv = verts + ((int64_t)vmax * rows[j] + i) * 3;
-        const q4 qr = J.qr;
-        const d3 r = qrot(qr, ld3(pv) - J.com, qconj(qr));
-        d3 nv = (r + J.com) + J.dx;
-        if (J.out) nv = nv + J.dx * -1.0;
-        st3(pv, nv);
-        if (radius) {
-            const double r2 = nrm2(nv - J.cnew);
-            if (r2 == r2) atomicMax(&r2bits[j], (unsigned long long)__double_as_longlong(r2));
+    if constexpr (!PACK) {
+        __syncthreads();
+        const int total = nb * vmax;
+        for (int idx = tid; idx < total; idx += VB) {
+            const int j = idx / vmax;
+            const int i = idx - j * vmax;
+            if (i >= job[j].nrot) continue;
+            const d3 nv = vertex_move(job[j], ld3(verts + ((int64_t)vmax * rows[j] + i) * 3));
+            st3(verts + ((int64_t)vmax * rows[j] + i) * 3, nv);
+            if (radius) {
+                const double r2 = nrm2(nv - job[j].cnew);
+                if (r2 == r2) atomicMax(&r2bits[j], (unsigned long long)__double_as_longlong(r2));
+            }
+        }
+    } else {
+        // block-wide exclusive scan of the vertex counts (wave scans, then
+        // the totals of the waves before)
+        constexpr int NW = VB / MWD_WAVE;
+        __shared__ int wsum[NW];
+        const int lane = tid & (MWD_WAVE - 1);
+        const int n = tid < nb ? job[tid].nrot : 0;
+        int x = n;
+#pragma unroll
+        for (int o = 1; o < MWD_WAVE; o <<= 1) {
+            const int y = __shfl_up(x, o, MWD_WAVE);
+            if (lane >= o) x += y;
+        }
+        if (lane == MWD_WAVE - 1) wsum[tid / MWD_WAVE] = x;
+        __syncthreads();
+        int before = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            if (w < tid / MWD_WAVE) before += wsum[w];
+            total += wsum[w];
+        }
+        vbase[tid] = before + x - n;
+        __syncthreads();
+        // VU vertices per thread and round: a thread issues the loads of VU
+        // vertices before it stores any of them (the compiler would not move
+        // a later round's loads above a store to the same array by itself)
+        constexpr int VU = 4;
+        for (int base = tid; base < total; base += VB * VU) {
+            int jj[VU];
+            double* pv[VU];
+            d3 p[VU];
+#pragma unroll
+            for (int u = 0; u < VU; ++u) {
+                const int idx = base + u * VB;
+                // the last particle whose first vertex is at or before idx
+                int lo = 0, hi = nb - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (vbase[mid] <= idx)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                jj[u] = idx < total ? lo : -1;
+                pv[u] = verts + ((int64_t)vmax * rows[lo] + (idx - vbase[lo])) * 3;
+                if (jj[u] >= 0) p[u] = ld3(pv[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < VU; ++u) {
+                const int j = jj[u];
+                if (j < 0) continue;
+                const d3 nv = vertex_move(job[j], p[u]);
+                st3(pv[u], nv);
+                if (radius) {
+                    const double r2 = nrm2(nv - job[j].cnew);
+                    if (r2 == r2) atomicMax(&r2bits[j], (unsigned long long)__double_as_longlong(r2));
+                }
+            }
         }
     }
     __syncthreads();
-    if (radius && tid < nb) radius[rows[tid]] = inflate_radius(__longlong_as_double((long long)r2bits[tid]), job[tid].cnew);
+    if (radius && tid < nb)
+        radius[rows[tid]] = inflate_radius(__longlong_as_double((long long)r2bits[tid]), job[tid].cnew);
 }
 
 __global__ void __launch_bounds__(VBLOCK) k_verlet(VState a, const double* __restrict__ F,
@@ -180,7 +250,7 @@ __global__ void __launch_bounds__(VBLOCK) k_verlet(VState a, const double* __res
         verlet_row<double>(a, P, ld3(F + 3 * P), ld3(tau + 3 * P), vmax, nrot ? nrot[P] : vmax, dt, box.b, 0,
                            &job[threadIdx.x]);
     }
-    vertex_pass<VBLOCK>(a.verts.v, rows, job, r2bits, nb, vmax, nullptr);
+    vertex_pass<VBLOCK, false>(a.verts.v, rows, job, r2bits, nullptr, nb, vmax, nullptr);
 }
 
 // Record r of a particle: side A (r even) or B (r odd) of the contact of
@@ -251,22 +321,19 @@ __device__ __forceinline__ void reduce_verlet_row(const VState& a, const int32_t
 }
 
 // The double path: force assembly and integrator per lane, then the block's
-// vertex pass; VB particles per block.
-template <int VB>
-__global__ void __launch_bounds__(VB) k_reduce_verlet_v(VState a, const int32_t* __restrict__ plist, int64_t M,
-                                                            const int32_t* __restrict__ scene_of,
-                                                            const SceneDev* __restrict__ scenes,
-                                                            int32_t* __restrict__ rec_cnt,
-                                                            const int32_t* __restrict__ rec_bkt,
-                                                            const int32_t* __restrict__ rec_ovf,
-                                                            const int64_t* n_ovf, int64_t ovf_cap, ContactOut co,
-                                                            const double* __restrict__ fext,
-                                                            const double* __restrict__ text,
-                                                            const int32_t* __restrict__ nverts, int vmax, double dt,
-                                                            int fp32) {
+// vertex pass; VB particles per block. WPE: the minimum waves per SIMD the
+// register allocation must allow (1 leaves the compiler's choice).
+template <int VB, int WPE, bool PACK>
+__global__ void __launch_bounds__(VB) __attribute__((amdgpu_waves_per_eu(WPE)))
+k_reduce_verlet_v(VState a, const int32_t* __restrict__ plist, int64_t M, const int32_t* __restrict__ scene_of,
+                  const SceneDev* __restrict__ scenes, int32_t* __restrict__ rec_cnt,
+                  const int32_t* __restrict__ rec_bkt, const int32_t* __restrict__ rec_ovf, const int64_t* n_ovf,
+                  int64_t ovf_cap, ContactOut co, const double* __restrict__ fext, const double* __restrict__ text,
+                  const int32_t* __restrict__ nverts, int vmax, double dt, int fp32) {
     __shared__ VertJob job[VB];
     __shared__ int64_t rows[VB];
     __shared__ unsigned long long r2bits[VB];
+    __shared__ int vbase[PACK ? VB : 1];
     const int64_t first = (int64_t)blockIdx.x * VB;
     const int nb = (int)(M - first < VB ? M - first : VB);
     const int64_t i = first + threadIdx.x;
@@ -275,7 +342,7 @@ __global__ void __launch_bounds__(VB) k_reduce_verlet_v(VState a, const int32_t*
         reduce_verlet_row<double>(a, plist, i, scene_of, scenes, rec_cnt, rec_bkt, rec_ovf, n_ovf, ovf_cap, co, fext,
                                   text, nverts, vmax, dt, &job[threadIdx.x], fp32 != 0);
     }
-    vertex_pass<VB>(a.verts.v, rows, job, r2bits, nb, vmax, a.radius);
+    vertex_pass<VB, PACK>(a.verts.v, rows, job, r2bits, vbase, nb, vmax, a.radius);
 }
 
 // The tangent path: one lane per particle, vertices and their tangents
@@ -367,13 +434,35 @@ int mwd_launch_verlet(double* v, double* v12, double* w12bf, double* q12, double
     return MWD_OK;
 }
 
-template <int VB>
+template <int VB, int WPE, bool PACK>
 static void launch_reduce_verlet_v(mwd_workspace* ws, const mwd_state* st, const VState& a, const ContactOut& co) {
     const int64_t M = ws->n_moving;
-    MWD_LAUNCH_EV(mwd_kev(ws, MWD_KT_VERLET, 0), mwd_kev(ws, MWD_KT_VERLET, 1), k_reduce_verlet_v<VB>,
+    MWD_LAUNCH_EV(mwd_kev(ws, MWD_KT_VERLET, 0), mwd_kev(ws, MWD_KT_VERLET, 1), (k_reduce_verlet_v<VB, WPE, PACK>),
                   dim3((unsigned)((M + VB - 1) / VB)), dim3(VB), 0, ws->stream, a, ws->plist, M, ws->scene_of,
                   ws->scenes_dev, ws->rec_cnt, ws->rec_bkt, ws->rec_ovf, ws->counters + 11, 2 * ws->contact_cap, co,
                   st->fext, st->text, st->nverts, (int)st->vmax, ws->dt, ws->reduce_fp32);
+}
+
+template <int VB, int WPE>
+static void launch_reduce_verlet_p(mwd_workspace* ws, const mwd_state* st, const VState& a, const ContactOut& co) {
+    if (ws->tune.vertex_pack)
+        launch_reduce_verlet_v<VB, WPE, true>(ws, st, a, co);
+    else
+        launch_reduce_verlet_v<VB, WPE, false>(ws, st, a, co);
+}
+
+template <int VB>
+static void launch_reduce_verlet_b(mwd_workspace* ws, const mwd_state* st, const VState& a, const ContactOut& co) {
+    switch (ws->tune.verlet_waves) {
+        case 3:
+            launch_reduce_verlet_p<VB, 3>(ws, st, a, co);
+            break;
+        case 4:
+            launch_reduce_verlet_p<VB, 4>(ws, st, a, co);
+            break;
+        default:
+            launch_reduce_verlet_p<VB, 1>(ws, st, a, co);
+    }
 }
 
 int mwd_launch_reduce_verlet(mwd_workspace* ws, mwd_state* st, double* const* tan, const ContactOut& co) {
@@ -388,13 +477,13 @@ int mwd_launch_reduce_verlet(mwd_workspace* ws, mwd_state* st, double* const* ta
     else
         switch (ws->tune.verlet_block) {
             case 64:
-                launch_reduce_verlet_v<64>(ws, st, a, co);
+                launch_reduce_verlet_b<64>(ws, st, a, co);
                 break;
             case 128:
-                launch_reduce_verlet_v<128>(ws, st, a, co);
+                launch_reduce_verlet_b<128>(ws, st, a, co);
                 break;
             default:
-                launch_reduce_verlet_v<256>(ws, st, a, co);
+                launch_reduce_verlet_b<256>(ws, st, a, co);
         }
     MWD_LAUNCH_CHECK();
     return MWD_OK;

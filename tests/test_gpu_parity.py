@@ -196,7 +196,8 @@ def test_tuning_choices_change_nothing(gpu, dense_phys):
     variants = [dict(gjk_group=4), dict(gjk_group=16), dict(gjk_blocks=64), dict(gjk_blocks=8192),
                 dict(sphere_block=64), dict(sphere_block=1024), dict(verlet_block=64), dict(verlet_block=128),
                 dict(cell_items=0), dict(cell_items=2), dict(cell_items=4),
-                dict(cell_pass=_lib.CELLS_SORTED, gjk_group=16)]
+                dict(cell_pass=_lib.CELLS_SORTED, gjk_group=16), dict(verlet_waves=3), dict(verlet_waves=4),
+                dict(vertex_pack=1), dict(vertex_pack=1, verlet_block=64, verlet_waves=4)]
     for v in variants:
         S = _stepper(sc)
         S.set_tuning(**v)
@@ -209,9 +210,36 @@ def test_tuning_choices_change_nothing(gpu, dense_phys):
         S.close()
     S = _stepper(sc)
     for bad in (dict(gjk_group=5), dict(gjk_blocks=8), dict(sphere_block=100), dict(cell_pass=7),
-                dict(verlet_block=96), dict(cell_items=3)):
+                dict(verlet_block=96), dict(cell_items=3), dict(verlet_waves=2), dict(vertex_pack=2)):
         with pytest.raises(_lib.MwdError):
             S.set_tuning(**bad)
+
+
+@pytest.mark.parametrize("pack", [0, 1])
+def test_graph_replay_equals_eager(gpu, dense_phys, pack):
+    """Steps replayed from a captured graph give the eager steps' state bit
+    for bit, replay after replay (with eager work and host reads between the
+    replays); the graph holds the wall scan on the side stream and its
+    join."""
+    sc = dense_phys
+    st = random_motion(O.initial_state(sc), sc, 7, vscale=2000.0, wscale=200.0)
+    A = _stepper(sc)
+    B = _stepper(sc)
+    for S in (A, B):
+        S.set_tuning(vertex_pack=pack)
+        S.set_state(st)
+    B.capture(5)
+    seen = 0
+    for r in range(4):
+        A.step(5)
+        B.run(5)
+        A.check()
+        B.check()
+        seen += A.status()["n_contacts"]
+        a, b = A.numpy_state(), B.numpy_state()
+        for k in a:
+            assert np.array_equal(a[k], b[k]), (r, k)
+    assert seen > 0
 
 
 @pytest.mark.parametrize("seed,phys", [(3, False), (4, False), (3, True)])
