@@ -140,104 +140,116 @@ struct GroupSup {
     }
 };
 
+// The simplex tests below are pure functions of their points. Each one
+// evaluates all of its sub-tests unconditionally and then picks the outcome
+// in the order of the reference's early returns, which gives the same code
+// and direction as running them one after another (a sub-test whose outcome
+// is not picked has no effect), while the independent sub-tests interleave
+// instead of forming one long chain of dependent FP64 operations.
+struct SimplexOut {
+    int c;  // -1 go on, 0 apart, 1 origin enclosed
+    d3 d;   // the direction the reference leaves behind
+};
+
 // Check_linesegment, .cu:103-124
-__device__ __forceinline__ void check_segment(d3 P1, d3 P2, int* coll, d3* d) {
-    *coll = -1;
-    *d = mk3(0.0, 0.0, 0.0);
+__device__ __forceinline__ SimplexOut check_segment(d3 P1, d3 P2) {
     const d3 V21 = P2 - P1;
-    if (dot3(V21, P2) < 0.0) {
-        *coll = 0;
-        return;
-    }
-    if (zero3(cross3(V21, P2))) {
-        *coll = 1;
-        return;
-    }
+    const bool apart = dot3(V21, P2) < 0.0;
+    const bool on = zero3(cross3(V21, P2));
     const d3 Vt = cross3(V21, neg3(P2));
-    *d = cross3(Vt, V21);
+    const d3 dd = cross3(Vt, V21);
+    SimplexOut o;
+    o.c = apart ? 0 : (on ? 1 : -1);
+    o.d = (apart || on) ? mk3(0.0, 0.0, 0.0) : dd;
+    return o;
 }
 
 // Check_triangle, .cu:127-186
-__device__ __forceinline__ void check_triangle(d3 P1, d3 P2, d3 P3, int* coll, d3* d) {
-    *coll = -1;
-    *d = mk3(0.0, 0.0, 0.0);
-    int c2, c3;
-    check_segment(P2, P3, &c2, d);
-    if (c2 == 1) {
-        *coll = 1;
-        return;
-    }
-    check_segment(P1, P3, &c3, d);
-    if (c3 == 1) {
-        *coll = 1;
-        return;
-    }
-    if (c2 + c3 > -1) {
-        *coll = 0;
-        return;
-    }
+__device__ __forceinline__ SimplexOut check_triangle(d3 P1, d3 P2, d3 P3) {
+    const SimplexOut s2 = check_segment(P2, P3);
+    const SimplexOut s3 = check_segment(P1, P3);
     const d3 V32 = P3 - P2;
     const d3 V13 = P1 - P3;
     const d3 n = cross3(V32, V13);
     const d3 n2 = cross3(V32, n);
     const d3 n3 = cross3(V13, n);
-    if (dot3(n2, P2) < 0.0) {
-        *d = n2;
-        return;
+    SimplexOut o;
+    if (s2.c == 1) {
+        o.c = 1;
+        o.d = s2.d;
+    } else if (s3.c == 1) {
+        o.c = 1;
+        o.d = s3.d;
+    } else if (s2.c + s3.c > -1) {
+        o.c = 0;
+        o.d = s3.d;
+    } else if (dot3(n2, P2) < 0.0) {
+        o.c = -1;
+        o.d = n2;
+    } else if (dot3(n3, P3) < 0.0) {
+        o.c = -1;
+        o.d = n3;
+    } else if (dot3(P3, n) == 0.0) {
+        o.c = 1;
+        o.d = s3.d;
+    } else {
+        o.c = -1;
+        o.d = (dot3(n, P3) > 0.0) ? neg3(n) : n;
     }
-    if (dot3(n3, P3) < 0.0) {
-        *d = n3;
-        return;
-    }
-    if (dot3(P3, n) == 0.0) {
-        *coll = 1;
-        return;
-    }
-    *d = (dot3(n, P3) > 0.0) ? neg3(n) : n;
+    return o;
+}
+
+__device__ __forceinline__ void check_segment(d3 P1, d3 P2, int* coll, d3* d) {
+    const SimplexOut o = check_segment(P1, P2);
+    *coll = o.c;
+    *d = o.d;
+}
+
+__device__ __forceinline__ void check_triangle(d3 P1, d3 P2, d3 P3, int* coll, d3* d) {
+    const SimplexOut o = check_triangle(P1, P2, P3);
+    *coll = o.c;
+    *d = o.d;
 }
 
 // Check_Tetrahedron, .cu:188-254
 __device__ __forceinline__ void check_tetra(d3 P1, d3 P2, d3 P3, d3 P4, int* coll, d3* d, int* todel) {
-    *coll = -1;
-    *d = mk3(0.0, 0.0, 0.0);
-    *todel = 0;
-    int c1, c2, c3;
-    check_triangle(P1, P2, P4, &c1, d);
-    if (c1 == 1) {
-        *coll = 1;
-        return;
-    }
-    check_triangle(P2, P3, P4, &c2, d);
-    if (c2 == 1) {
-        *coll = 1;
-        return;
-    }
-    check_triangle(P3, P1, P4, &c3, d);
-    if (c3 == 1) {
-        *coll = 1;
-        return;
-    }
-    if (c1 + c2 + c3 > -2) {
-        *coll = 0;
-        return;
-    }
+    const SimplexOut t1 = check_triangle(P1, P2, P4);
+    const SimplexOut t2 = check_triangle(P2, P3, P4);
+    const SimplexOut t3 = check_triangle(P3, P1, P4);
     const d3 V13 = P1 - P3, V21 = P2 - P1, V32 = P3 - P2, V41 = P4 - P1;
     const d3 n2 = cross3(V21, V41);
     const d3 V42 = P4 - P2;
     const d3 n3 = cross3(V32, V42);
     const d3 V43 = P4 - P3;
     const d3 n4 = cross3(V13, V43);
-    if (dot3(n2, P4) < 0.0) {
+    *todel = 0;
+    if (t1.c == 1) {
+        *coll = 1;
+        *d = t1.d;
+    } else if (t2.c == 1) {
+        *coll = 1;
+        *d = t2.d;
+    } else if (t3.c == 1) {
+        *coll = 1;
+        *d = t3.d;
+    } else if (t1.c + t2.c + t3.c > -2) {
+        *coll = 0;
+        *d = t3.d;
+    } else if (dot3(n2, P4) < 0.0) {
+        *coll = -1;
         *d = n2;
         *todel = 3;
     } else if (dot3(n3, P4) < 0.0) {
+        *coll = -1;
         *d = n3;
         *todel = 1;
     } else if (dot3(n4, P4) < 0.0) {
+        *coll = -1;
         *d = n4;
         *todel = 2;
     } else {
         *coll = 1;
+        *d = t3.d;
     }
 }
 
