@@ -44,7 +44,7 @@ void free_ws(mwd_workspace* ws) {
     void* ptrs[] = {ws->scenes_dev, ws->cell_of, ws->plist, ws->wlist, ws->scene_of, ws->is_wall, ws->keys, ws->keys_alt, ws->vals,
                     ws->vals_alt, ws->cell_start, ws->cell_key, ws->cell_self, ws->cell_seg,
                     ws->row_first, ws->row_rank, ws->cell_cnt, ws->cell_off, ws->perm, ws->key_cnt,
-                    ws->key_min, ws->row_key,
+                    ws->key_min, ws->row_key, ws->sph,
                     ws->pairs, ws->collision, ws->simplex, ws->sidx, ws->surv, ws->contact_pair,
                     ws->depth, ws->normal, ws->cpA, ws->cpB, ws->Fn, ws->tA, ws->tB, ws->epa_status, ws->rec_cnt,
                     ws->rec_bkt, ws->rec_ovf, ws->hist,
@@ -394,6 +394,7 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
         A(key_min, ws->key_span_total);
     }
     A(row_key, n_rows);
+    A(sph, 4 * n_rows);
     A(pairs, 2 * pair_cap);
     A(collision, pair_cap);
     A(simplex, contact_cap);

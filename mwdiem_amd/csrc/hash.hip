@@ -69,12 +69,14 @@ __device__ __forceinline__ uint32_t global_key(const double* __restrict__ com, c
 __global__ void k_sort_keys(const double* __restrict__ com, const int64_t* __restrict__ keys_in,
                             const int32_t* __restrict__ plist, int64_t M, const int32_t* __restrict__ scene_of,
                             const SceneDev* __restrict__ scenes, uint32_t* __restrict__ keys,
-                            int32_t* __restrict__ vals, uint32_t* flags) {
+                            int32_t* __restrict__ vals, uint32_t* flags, const double* __restrict__ radius,
+                            double* __restrict__ sph) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
     const int32_t row = plist[i];
     keys[i] = global_key(com, keys_in, row, scenes[scene_of[row]], flags);
     vals[i] = row;
+    if (sph) pack_sphere(sph, com, radius, row);
 }
 
 // Key-table path, one thread per moving particle: its key, the member count
@@ -84,10 +86,12 @@ constexpr int32_t KEY_MIN_EMPTY = 0x7F7F7F7F;  // above every row; the byte-fill
 __global__ void k_key_count(const double* __restrict__ com, const int64_t* __restrict__ keys_in,
                             const int32_t* __restrict__ plist, int64_t M, const int32_t* __restrict__ scene_of,
                             const SceneDev* __restrict__ scenes, int32_t* __restrict__ row_key,
-                            int32_t* __restrict__ key_cnt, int32_t* __restrict__ key_min, uint32_t* flags) {
+                            int32_t* __restrict__ key_cnt, int32_t* __restrict__ key_min, uint32_t* flags,
+                            const double* __restrict__ radius, double* __restrict__ sph) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
     const int32_t row = plist[i];
+    if (sph) pack_sphere(sph, com, radius, row);
     const uint32_t k = global_key(com, keys_in, row, scenes[scene_of[row]], flags);
     row_key[row] = (int32_t)k;
     atomicAdd(&key_cnt[k], 1);
@@ -310,6 +314,18 @@ __global__ void k_cell_info_table(int32_t* __restrict__ members, const int32_t* 
                             flags, NbTable{cell_of, cell_key, C});
 }
 
+// Blocks are dispatched to the 8 XCDs in turn (block b to XCD b % 8), and
+// each XCD has its own L2. For the first `live` blocks this maps block b to
+// a position such that every XCD takes one contiguous run of positions, so
+// neighbouring cells (which share particles) meet in one L2. A bijection of
+// [0, live).
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t live) {
+    constexpr int NX = 8;
+    const int64_t q = live / NX, r = live % NX;
+    const int64_t x = b % NX, k = b / NX;
+    return x * q + (x < r ? x : r) + k;
+}
+
 // Pair emission, one output pair per thread: a block takes 64 cells by rank,
 // stages their pair offsets, member ranges and neighbour ranges in LDS
 // (coalesced loads of the rank-ordered tables), and its threads walk the
@@ -327,8 +343,7 @@ __global__ void __launch_bounds__(FILL_THREADS) k_cell_fill(const int32_t* __res
                                                           const int64_t* __restrict__ cell_off,
                                                           const int64_t* __restrict__ counters,
                                                           int32_t* __restrict__ pairs, int64_t cap,
-                                                          const double* __restrict__ com,
-                                                          const double* __restrict__ radius,
+                                                          const double* __restrict__ sph,
                                                           int32_t* __restrict__ surv, int64_t* n_surv,
                                                           int32_t* __restrict__ collision) {
     __shared__ int64_t s_off[FILL_CELLS + 1];
@@ -340,8 +355,9 @@ __global__ void __launch_bounds__(FILL_THREADS) k_cell_fill(const int32_t* __res
     // offset decoded below) stays under 2^31.
     __shared__ uint32_t s_cum[FILL_CELLS][MWD_MAX_OFF + 1];
     const int64_t C = counters[0];
-    const int64_t r0 = (int64_t)blockIdx.x * FILL_CELLS;
-    if (r0 >= C) return;  // uniform per block
+    const int64_t live = (C + FILL_CELLS - 1) / FILL_CELLS;
+    if ((int64_t)blockIdx.x >= live) return;  // uniform per block
+    const int64_t r0 = xcd_block(blockIdx.x, live) * FILL_CELLS;
     const int nc = (int)(C - r0 < FILL_CELLS ? C - r0 : FILL_CELLS);
     const int tid = threadIdx.x;
     const int64_t base = cell_off[r0];
@@ -413,8 +429,8 @@ __global__ void __launch_bounds__(FILL_THREADS) k_cell_fill(const int32_t* __res
             v.y = b;
             reinterpret_cast<int2*>(pairs)[pos] = v;
         }
-        if (radius) {  // apart: decided (collision 0); the others go to GJK, which writes theirs
-            const bool keep = have && spheres_meet(com, radius, a, b);
+        if (sph) {  // apart: decided (collision 0); the others go to GJK, which writes theirs
+            const bool keep = have && spheres_meet_packed(sph, a, b);
             if (have && !keep) collision[pos] = 0;
             append_survivor(keep, pos, surv, n_surv, cap);
         }
@@ -658,7 +674,8 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
         mwd_mark(ws, 1);
         if (ws->cell_table) {
             hipLaunchKernelGGL(k_key_count, dim3(blocks_for(M)), dim3(256), 0, s, st->com, keys_in, ws->plist, M,
-                               ws->scene_of, ws->scenes_dev, ws->row_key, ws->key_cnt, ws->key_min, ws->flags);
+                               ws->scene_of, ws->scenes_dev, ws->row_key, ws->key_cnt, ws->key_min, ws->flags,
+                               st->radius, st->radius ? ws->sph : nullptr);
             MWD_LAUNCH_CHECK();
             mwd_mark(ws, 2);
             rc = mwd_scan_io(CellTableIO{ws->row_key, ws->key_cnt, ws->key_min, ws->cell_start, ws->cell_key,
@@ -695,7 +712,8 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
             if (rc) return rc;
         } else {
             hipLaunchKernelGGL(k_sort_keys, dim3(blocks_for(M)), dim3(256), 0, s, st->com, keys_in, ws->plist, M,
-                               ws->scene_of, ws->scenes_dev, ws->keys, ws->vals, ws->flags);
+                               ws->scene_of, ws->scenes_dev, ws->keys, ws->vals, ws->flags, st->radius,
+                               st->radius ? ws->sph : nullptr);
             MWD_LAUNCH_CHECK();
             uint32_t* ks;
             int32_t* vs;
@@ -726,8 +744,8 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
         mwd_mark(ws, 3);
         MWD_LAUNCH_EV(mwd_kev(ws, MWD_KT_EMIT, 0), mwd_kev(ws, MWD_KT_EMIT, 1), k_cell_fill,
                       dim3((unsigned)((M + FILL_CELLS - 1) / FILL_CELLS)), dim3(FILL_THREADS), 0, s, ws->perm,
-                      ws->cell_self, ws->cell_seg, ws->cell_off, ctr, ws->pairs, ws->pair_cap, st->com, st->radius,
-                      ws->surv, ctr + 10, ws->collision);
+                      ws->cell_self, ws->cell_seg, ws->cell_off, ctr, ws->pairs, ws->pair_cap,
+                      st->radius ? ws->sph : nullptr, ws->surv, ctr + 10, ws->collision);
         MWD_LAUNCH_CHECK();
     }
     if ((rc = join())) return rc;

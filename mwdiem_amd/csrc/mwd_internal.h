@@ -151,6 +151,25 @@ __device__ __forceinline__ bool spheres_meet(const double* com, const double* ra
     return !(nrm2(d) > rr * rr);
 }
 
+// Row r's centre and bounding radius packed as 4 doubles (two 16-byte
+// halves), so that the sphere test loads each row with two vector loads.
+__device__ __forceinline__ void pack_sphere(double* __restrict__ sph, const double* __restrict__ com,
+                                            const double* __restrict__ radius, int64_t r) {
+    double2* p = reinterpret_cast<double2*>(sph + 4 * r);
+    p[0] = make_double2(com[3 * r], com[3 * r + 1]);
+    p[1] = make_double2(com[3 * r + 2], radius[r]);
+}
+
+// spheres_meet on the packed rows: the same operations on the same values.
+__device__ __forceinline__ bool spheres_meet_packed(const double* __restrict__ sph, int64_t a, int64_t b) {
+    const double2* pa = reinterpret_cast<const double2*>(sph + 4 * a);
+    const double2* pb = reinterpret_cast<const double2*>(sph + 4 * b);
+    const double2 a0 = pa[0], a1 = pa[1], b0 = pb[0], b1 = pb[1];
+    const d3 d = mk3(b0.x - a0.x, b0.y - a0.y, b1.x - a1.x);
+    const double rr = a1.y + b1.y;
+    return !(nrm2(d) > rr * rr);
+}
+
 // Append pair p to the sphere-test survivors when keep, one atomic per wave
 // for the lanes that call this together.
 __device__ __forceinline__ void append_survivor(bool keep, int64_t p, int32_t* surv, int64_t* n_surv, int64_t cap) {
@@ -254,6 +273,8 @@ struct mwd_workspace {
     int32_t* key_cnt;          // [key_span_total] members per key, zero between steps
     int32_t* key_min;          // [key_span_total] lowest row per key, INT32_MAX between steps
     int32_t* row_key;          // [n_rows] key of each moving row this step, -1 for walls
+    double* sph;               // [n_rows*4] (centre, bounding radius) of the moving rows, packed by the key
+                               // pass when radii are kept, for the sphere test inside the emission
     // candidate pairs
     int32_t* pairs;            // [pair_cap*2]
     int32_t* collision;        // [pair_cap]
