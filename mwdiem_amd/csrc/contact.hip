@@ -25,6 +25,10 @@
 namespace {
 
 constexpr int EPA_FACES = MWD_EPA_MAX_FACES;
+// Grid cap of the grid-strided EPA launch (one contact per wave): 2,048
+// blocks of 4 waves, about what the polytopes' LDS lets stay resident (7
+// blocks per CU); more contacts loop.
+constexpr int64_t EPA_MAX_BLOCKS = 2048;
 constexpr int EPA_VERTS = 32;
 
 struct Side {
@@ -440,44 +444,6 @@ struct ContactIO {
 };
 
 // ------------------------------------------------------------------- EPA
-struct Poly {
-    d3 V[EPA_VERTS];
-    d3 n[EPA_FACES];
-    double dist[EPA_FACES];
-    uint8_t tr[EPA_FACES][3];
-    int8_t valid[EPA_FACES];
-    uint8_t flip[EPA_FACES];
-    uint8_t opA[EPA_VERTS], opB[EPA_VERTS];
-    uint8_t vis[EPA_FACES];
-};
-
-// AddTriangle, .cu:270-325
-__device__ __forceinline__ void add_triangle(Poly& P, int e0, int e1, int e2, int id) {
-    d3 nn = cross3(P.V[e1] - P.V[e0], P.V[e2] - P.V[e1]);
-    const double len = sqrt(nrm2(nn));
-    nn = nn / len;
-    double dd = dot3(nn, P.V[e0]);
-    P.flip[id] = 0;
-    if (dd < 0.0) {
-        const int o = e2;
-        e2 = e1;
-        e1 = o;
-        dd = dd * -1.0;
-        nn = nn * -1.0;
-        P.flip[id] = 1;
-    }
-    P.tr[id][0] = (uint8_t)e0;
-    P.tr[id][1] = (uint8_t)e1;
-    P.tr[id][2] = (uint8_t)e2;
-    const d3 a = P.V[e0], b = P.V[e1], c = P.V[e2];
-    const bool inside = dot3(cross3(b - a, nn), neg3(a)) < 0.0 && dot3(cross3(c - b, nn), neg3(b)) < 0.0 &&
-                        dot3(cross3(a - c, nn), neg3(c)) < 0.0;
-    P.valid[id] = inside ? 1 : -2;
-    if (!inside) dd = 1e30;
-    P.n[id] = nn;
-    P.dist[id] = dd;
-}
-
 struct EpaResult {
     double depth;
     d3 n, cpA, cpB;
@@ -487,99 +453,250 @@ struct EpaResult {
     int fa[3], fb[3], flip;
 };
 
-// EPA, .cu:328-603. status 1: expansion cap; 2: face storage exhausted.
-__device__ EpaResult epa(const Simplex& sx, const Side& A, const Side& B) {
-    Poly P;
+// ------------------------------------------------- EPA, one wave per contact
+// EPA (.cu:328-603) run by the 64 lanes of a wave on a polytope held in LDS
+// (structure of arrays, 5.7 KB per wave): every loop of the reference over
+// faces or vertices is spread over the lanes, and every sequential choice is
+// reproduced exactly. The closest face is the first index of the minimum
+// (the reference's strict-less scan from face 0); the support point is the
+// first strict maximum (support()); faces that see the new point are listed
+// in index order; the horizon edges (.cu:470-540) are taken in (visible face,
+// edge) order and each new face gets its slot by a prefix count over that
+// order, so faces land where the reference's sequential loop puts them.
+// status 1: expansion cap; 2: face storage exhausted.
+static_assert(EPA_FACES == 2 * MWD_WAVE, "a lane holds faces lane and lane + 64");
+struct PolyW {
+    double vx[EPA_VERTS], vy[EPA_VERTS], vz[EPA_VERTS];
+    double nx[EPA_FACES], ny[EPA_FACES], nz[EPA_FACES];
+    double dist[EPA_FACES];
+    uint8_t t0[EPA_FACES], t1[EPA_FACES], t2[EPA_FACES];
+    int8_t valid[EPA_FACES];
+    uint8_t flip[EPA_FACES];
+    uint8_t vis[EPA_FACES];
+    uint8_t opA[EPA_VERTS], opB[EPA_VERTS];
+};
+
+// Orders the LDS accesses of one wave (its lanes run in lockstep).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ d3 pw_v(const PolyW& P, int k) { return mk3(P.vx[k], P.vy[k], P.vz[k]); }
+__device__ __forceinline__ d3 pw_n(const PolyW& P, int k) { return mk3(P.nx[k], P.ny[k], P.nz[k]); }
+
+// AddTriangle (.cu:270-325) into slot id.
+__device__ __forceinline__ void add_triangle_w(PolyW& P, int e0, int e1, int e2, int id) {
+    const d3 V0 = pw_v(P, e0), V1 = pw_v(P, e1), V2 = pw_v(P, e2);
+    d3 nn = cross3(V1 - V0, V2 - V1);
+    const double len = sqrt(nrm2(nn));
+    nn = nn / len;
+    double dd = dot3(nn, V0);
+    uint8_t fl = 0;
+    if (dd < 0.0) {
+        const int o = e2;
+        e2 = e1;
+        e1 = o;
+        dd = dd * -1.0;
+        nn = nn * -1.0;
+        fl = 1;
+    }
+    P.flip[id] = fl;
+    P.t0[id] = (uint8_t)e0;
+    P.t1[id] = (uint8_t)e1;
+    P.t2[id] = (uint8_t)e2;
+    const d3 a = pw_v(P, e0), b = pw_v(P, e1), c = pw_v(P, e2);
+    const bool inside = dot3(cross3(b - a, nn), neg3(a)) < 0.0 && dot3(cross3(c - b, nn), neg3(b)) < 0.0 &&
+                        dot3(cross3(a - c, nn), neg3(c)) < 0.0;
+    P.valid[id] = inside ? 1 : -2;
+    if (!inside) dd = 1e30;
+    P.nx[id] = nn.x;
+    P.ny[id] = nn.y;
+    P.nz[id] = nn.z;
+    P.dist[id] = dd;
+}
+
+__device__ __forceinline__ double wave_min(double x) {
+#pragma unroll
+    for (int o = 1; o < MWD_WAVE; o <<= 1) {
+        const double y = __shfl_xor(x, o, MWD_WAVE);
+        x = y < x ? y : x;
+    }
+    return x;
+}
+__device__ __forceinline__ double wave_max(double x) {
+#pragma unroll
+    for (int o = 1; o < MWD_WAVE; o <<= 1) {
+        const double y = __shfl_xor(x, o, MWD_WAVE);
+        x = y > x ? y : x;
+    }
+    return x;
+}
+__device__ __forceinline__ int wave_min_i(int x) {
+#pragma unroll
+    for (int o = 1; o < MWD_WAVE; o <<= 1) {
+        const int y = __shfl_xor(x, o, MWD_WAVE);
+        x = y < x ? y : x;
+    }
+    return x;
+}
+
+// Index of the first strict maximum of d.v over a body (support(): best
+// starts at -1e30, index 0 when nothing beats it).
+__device__ __forceinline__ int support_index_w(const Side& S, d3 d, int lane) {
+    double best = -1e30;
+    int bi = 0x7fffffff;
+    for (int i = lane; i < S.n; i += MWD_WAVE) {
+        const double t = dot3(d, ld3(S.v + 3 * i));
+        if (t > best) {
+            best = t;
+            bi = i;
+        }
+    }
+    const double m = wave_max(best);
+    if (!(m > -1e30)) return 0;
+    return wave_min_i(best == m ? bi : 0x7fffffff);
+}
+
+// First index of the minimum distance over faces [0, nt) (epa()'s scan: a
+// NaN never wins, unless it is face 0's and so the starting value).
+__device__ __forceinline__ int closest_face_w(const PolyW& P, int nt, int lane) {
+    const double inf = __builtin_inf();
+    double x0 = inf, x1 = inf;
+    if (lane < nt) {
+        x0 = P.dist[lane];
+        if (x0 != x0) x0 = inf;
+    }
+    if (lane + MWD_WAVE < nt) {
+        x1 = P.dist[lane + MWD_WAVE];
+        if (x1 != x1) x1 = inf;
+    }
+    const double d0 = P.dist[0];
+    if (d0 != d0) return 0;
+    const double m = wave_min(x0 < x1 ? x0 : x1);
+    const uint64_t lo = __ballot(lane < nt && x0 == m);
+    if (lo) return __ffsll((long long)lo) - 1;
+    const uint64_t hi = __ballot(lane + MWD_WAVE < nt && x1 == m);
+    return MWD_WAVE + __ffsll((long long)hi) - 1;
+}
+
+__device__ EpaResult epa_wave(const Simplex& sx, const Side& A, const Side& B, PolyW& P, int lane) {
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (MWD_WAVE - lane));
     double xmin_old = 1e29;
     double proj = 2.0 * xmin_old + 1.0;
     int counter = 0, nv = 4, nt = 0, indmin = 0;
     double xmin_new = 0.0;
     int st = 0;
-    for (int k = 0; k < 4; ++k) {
-        P.V[k] = ld3(sx.s[k]);
-        P.opA[k] = (uint8_t)sx.ia[k];
-        P.opB[k] = (uint8_t)sx.ib[k];
+    if (lane < 4) {
+        P.vx[lane] = sx.s[lane][0];
+        P.vy[lane] = sx.s[lane][1];
+        P.vz[lane] = sx.s[lane][2];
+        P.opA[lane] = (uint8_t)sx.ia[lane];
+        P.opB[lane] = (uint8_t)sx.ib[lane];
     }
-    add_triangle(P, 0, 1, 2, 0);
-    add_triangle(P, 1, 2, 3, 1);
-    add_triangle(P, 2, 0, 3, 2);
-    add_triangle(P, 0, 1, 3, 3);
+    wave_sync();
+    if (lane < 4) {
+        const int e0 = lane == 2 ? 2 : (lane == 1 ? 1 : 0);
+        const int e1 = lane == 0 ? 1 : (lane == 1 ? 2 : (lane == 2 ? 0 : 1));
+        const int e2 = lane == 0 ? 2 : 3;
+        add_triangle_w(P, e0, e1, e2, lane);
+    }
     nt = 4;
+    wave_sync();
     while (xmin_old < proj) {
-        indmin = 0;
-        double xm = P.dist[0];
-        for (int k = 0; k < nt; ++k)
-            if (P.dist[k] < xm) {
-                xm = P.dist[k];
-                indmin = k;
-            }
+        indmin = closest_face_w(P, nt, lane);
         xmin_new = P.dist[indmin];
         xmin_old = xmin_new;
-        int a, b;
-        const d3 Suj = support(A, B, P.n[indmin], &a, &b);
-        proj = dot3(P.n[indmin], Suj);
-        P.opA[nv] = (uint8_t)a;
-        P.opB[nv] = (uint8_t)b;
+        const d3 nrm = pw_n(P, indmin);
+        const int a = support_index_w(A, nrm, lane);
+        const int b = support_index_w(B, neg3(nrm), lane);
+        const d3 Suj = ld3(A.v + 3 * a) - ld3(B.v + 3 * b);
+        proj = dot3(nrm, Suj);
+        if (lane == 0) {
+            P.opA[nv] = (uint8_t)a;
+            P.opB[nv] = (uint8_t)b;
+        }
         if (xmin_old > proj || (xmin_old * 1.00001 > proj && xmin_old * 0.99999 < proj)) break;
         if (counter > 49) {
             st |= 1;
             break;
         }
         counter = counter + 1;
-        P.V[nv] = Suj;
-        nv = nv + 1;
-        // faces that see the new point
-        int nvis = 0;
-        for (int i = 0; i < nt; ++i) {
-            if ((P.valid[i] == 1 || P.valid[i] == -2) && dot3(P.V[nv - 1] - P.V[P.tr[i][0]], P.n[i]) >= 0.0) {
-                P.valid[i] = -1;
-                P.vis[nvis++] = (uint8_t)i;
-            }
+        if (lane == 0) {
+            P.vx[nv] = Suj.x;
+            P.vy[nv] = Suj.y;
+            P.vz[nv] = Suj.z;
         }
+        nv = nv + 1;
+        wave_sync();
+        // faces that see the new point, listed in index order
+        int nvis = 0;
+#pragma unroll
+        for (int h = 0; h < EPA_FACES / MWD_WAVE; ++h) {
+            const int i = h * MWD_WAVE + lane;
+            bool see = false;
+            if (i < nt && (P.valid[i] == 1 || P.valid[i] == -2)) {
+                const int t = P.t0[i];
+                see = dot3(Suj - pw_v(P, t), pw_n(P, i)) >= 0.0;
+            }
+            const uint64_t m = __ballot(see);
+            if (see) {
+                P.valid[i] = -1;
+                P.vis[nvis + __popcll(m & lt)] = (uint8_t)i;
+            }
+            nvis += __popcll(m);
+        }
+        wave_sync();
         // horizon: an edge of a visible face that no other visible face holds
-        // (the reference's delpair count < 2, .cu:470-540)
-        bool overflow = false;
+        // (.cu:470-540), in (visible face, edge) order
         const int nt0 = nt;
-        for (int vi = 0; vi < nvis && !overflow; ++vi) {
-            const int i = P.vis[vi];
-            for (int e = 0; e < 3; ++e) {
-                const int u = P.tr[i][e], v = P.tr[i][e == 2 ? 0 : e + 1];
+        int added = 0;
+        for (int base = 0; base < 3 * nvis; base += MWD_WAVE) {
+            const int q = base + lane;
+            bool isnew = false;
+            int u = 0, v = 0;
+            if (q < 3 * nvis) {
+                const int vi = q / 3, e = q - 3 * vi;
+                const int i = P.vis[vi];
+                const int x[3] = {P.t0[i], P.t1[i], P.t2[i]};
+                u = x[e];
+                v = x[e == 2 ? 0 : e + 1];
                 int cnt = 0;
                 for (int vj = 0; vj < nvis; ++vj) {
                     const int j = P.vis[vj];
-                    const int x0 = P.tr[j][0], x1 = P.tr[j][1], x2 = P.tr[j][2];
+                    const int x0 = P.t0[j], x1 = P.t1[j], x2 = P.t2[j];
                     cnt += ((x0 == u && x1 == v) || (x0 == v && x1 == u));
                     cnt += ((x1 == u && x2 == v) || (x1 == v && x2 == u));
                     cnt += ((x2 == u && x0 == v) || (x2 == v && x0 == u));
                 }
-                if (cnt < 2) {
-                    if (nt == EPA_FACES) {
-                        overflow = true;
-                        break;
-                    }
-                    add_triangle(P, u, v, nv - 1, nt);
-                    nt = nt + 1;
-                }
+                isnew = cnt < 2;
             }
+            const uint64_t m = __ballot(isnew);
+            const int id = nt0 + added + __popcll(m & lt);
+            if (isnew && id < EPA_FACES) add_triangle_w(P, u, v, nv - 1, id);
+            added += __popcll(m);
         }
-        (void)nt0;
-        if (overflow) {
+        wave_sync();
+        if (added > EPA_FACES - nt0) {  // the sequential loop stops at a full face store
             st |= 2;
             break;
         }
-        for (int vi = 0; vi < nvis; ++vi) {
+        nt = nt0 + added;
+        for (int vi = lane; vi < nvis; vi += MWD_WAVE) {
             const int i = P.vis[vi];
             P.valid[i] = 0;
             P.dist[i] = 3e20;
         }
         counter = counter + 1;
+        wave_sync();
     }
+    wave_sync();
     EpaResult r;
     r.depth = xmin_new;
-    const int a = P.tr[indmin][0], b = P.tr[indmin][1], c = P.tr[indmin][2];
-    const d3 nrm = P.n[indmin];
+    const int a = P.t0[indmin], b = P.t1[indmin], c = P.t2[indmin];
+    const d3 nrm = pw_n(P, indmin);
     const d3 Proj = nrm * xmin_new;
-    const d3 Aa = P.V[a], Bb = P.V[b], Cc = P.V[c];
+    const d3 Aa = pw_v(P, a), Bb = pw_v(P, b), Cc = pw_v(P, c);
     d3 v0 = Bb - Aa, v1 = Cc - Aa;
     const d3 v2 = Proj - Aa;
     if (dot3(cross3(v0, v1), nrm) < 0.0) {
@@ -603,6 +720,7 @@ __device__ EpaResult epa(const Simplex& sx, const Side& A, const Side& B) {
     r.fb[1] = P.opB[b];
     r.fb[2] = P.opB[c];
     r.flip = P.flip[indmin];
+    wave_sync();  // the polytope is rewritten by the wave's next contact
     return r;
 }
 
@@ -827,7 +945,8 @@ __device__ __forceinline__ Q4<T> bodyq(const double* p, const TArr<MWD_NT>& t, i
         return ldqT<T>(t, r);
 }
 
-// EPA + law per contact. T = double: the forward path (any pair layout).
+// EPA + law per contact, one contact per wave: EPA by the whole wave, then
+// the law by its first lane. T = double: the forward path (any pair layout).
 // T = Dual: the tangent path, where A and B index one state whose tangents bt
 // carries; EPA runs on values and the final face is re-evaluated in T.
 template <class T>
@@ -846,23 +965,28 @@ __global__ void __launch_bounds__(256) k_epa_force(BodyView A, BodyView B, BodyT
                                                    int64_t* n_records, int32_t* __restrict__ rec_cnt,
                                                    int32_t* __restrict__ rec_bkt, int32_t* __restrict__ rec_ovf,
                                                    int64_t* n_ovf, int64_t ovf_cap, uint32_t* flags) {
+    __shared__ PolyW polys[256 / MWD_WAVE];
     int64_t K = *n_contacts;
     if (K > contact_cap) K = contact_cap;
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid == 0 && n_records) *n_records = 2 * K;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    // one contact per wave: EPA by all 64 lanes, then the law by lane 0
+    const int lane = threadIdx.x & (MWD_WAVE - 1);
+    PolyW& P = polys[threadIdx.x / MWD_WAVE];
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / MWD_WAVE);
     uint32_t fl = 0;
-    for (int64_t c = gid; c < K; c += stride) {
+    for (int64_t c = gid / MWD_WAVE; c < K; c += nw) {
         const int64_t p = contact_pair[c];
         const int64_t ra = view_row(A, p), rb = view_row(B, p);
         const Side sa{A.verts + (int64_t)A.vmax * 3 * ra, A.nverts[ra]};
         const Side sb{B.verts + (int64_t)B.vmax * 3 * rb, B.nverts[rb]};
         const int32_t slot = sidx[p];
         if (slot < 0 || slot >= contact_cap) {  // simplex storage overflowed (flagged by k_gjk)
-            if (co.face) co.face[7 * c + 6] = 4;   // no face: nothing to reverse
+            if (co.face && lane == 0) co.face[7 * c + 6] = 4;   // no face: nothing to reverse
             continue;
         }
-        const EpaResult e = epa(simplex[slot], sa, sb);
+        const EpaResult e = epa_wave(simplex[slot], sa, sb, P, lane);
+        if (lane != 0) continue;
         const int wall = (A.is_wall[ra] || B.is_wall[rb]) ? 1 : 0;
         const int scene = A.scene ? A.scene[ra] : 0;
         const LawT<T> L = law_of(laws2[2 * scene + wall], lawt2 ? lawt2 + 2 * scene + wall : nullptr, (T*)nullptr);
@@ -1184,7 +1308,7 @@ int mwd_launch_epa_force(const BodyView& A, const BodyView& B, const mwd_law* la
     co.Fn = plain(Fn);
     co.tA = plain(tA);
     co.tB = plain(tB);
-    hipLaunchKernelGGL(k_epa_force<double>, dim3(grid_for(contact_cap, 256, 4096)), dim3(256), 0, s, A, B, bt, laws2,
+    hipLaunchKernelGGL(k_epa_force<double>, dim3(grid_for(contact_cap * MWD_WAVE, 256, EPA_MAX_BLOCKS)), dim3(256), 0, s, A, B, bt, laws2,
                        (const mwd_law_tan*)nullptr, contact_pair, simplex, sidx, contact_cap, n_contacts, dispt, idA,
                        idB, depth, normal, cpA, cpB, co, disp_out, epa_status, rec_key, rec_val, n_records,
                        (int32_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr, (int64_t*)nullptr, (int64_t)0, flags);
@@ -1199,7 +1323,7 @@ int mwd_launch_epa_force_rows(const BodyView& A, const BodyView& B, const BodyTa
                               int32_t* rec_cnt, int32_t* rec_bkt, int32_t* rec_ovf, int64_t* n_ovf,
                               int64_t ovf_cap, uint32_t* flags, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     if (contact_cap <= 0) return MWD_OK;
-    const dim3 grid(grid_for(contact_cap, 256, 4096)), blk(256);
+    const dim3 grid(grid_for(contact_cap * MWD_WAVE, 256, EPA_MAX_BLOCKS)), blk(256);
     if (bt) {
         MWD_LAUNCH_EV(e0, e1, k_epa_force<Dual<MWD_NT>>, grid, blk, 0, s, A, B, *bt, laws2, lawt2, contact_pair, simplex,
                            sidx, contact_cap, n_contacts, (const double*)nullptr, (const int32_t*)nullptr,
