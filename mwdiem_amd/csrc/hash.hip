@@ -336,6 +336,7 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t live) {
 // every member k of that cell.
 constexpr int FILL_CELLS = 64;    // cells per block
 constexpr int FILL_THREADS = 256;
+constexpr int FILL_SURV = 2048;   // survivors gathered in LDS before a global append
 
 __global__ void __launch_bounds__(FILL_THREADS) k_cell_fill(const int32_t* __restrict__ perm,
                                                           const int2* __restrict__ cell_self,
@@ -354,6 +355,14 @@ __global__ void __launch_bounds__(FILL_THREADS) k_cell_fill(const int32_t* __res
     // neighbour o. Saturated at UINT32_MAX; a cell's pair count (and so every
     // offset decoded below) stays under 2^31.
     __shared__ uint32_t s_cum[FILL_CELLS][MWD_MAX_OFF + 1];
+    // sphere-test survivors gathered in LDS and appended to the global list
+    // in batches: one global atomic per batch instead of one per wave and
+    // round (in crowded steps, with hundreds of thousands of survivors, the
+    // appends to one counter serialise); their order there is free (contacts
+    // are summed by pair index)
+    __shared__ int32_t s_surv[FILL_SURV];
+    __shared__ int s_nsurv;
+    __shared__ int64_t s_gbase;
     const int64_t C = counters[0];
     const int64_t live = (C + FILL_CELLS - 1) / FILL_CELLS;
     if ((int64_t)blockIdx.x >= live) return;  // uniform per block
@@ -366,7 +375,10 @@ __global__ void __launch_bounds__(FILL_THREADS) k_cell_fill(const int32_t* __res
         s_self[tid] = cell_self[r0 + tid];
         s_off[tid] = cell_off[r0 + tid] - base;
     }
-    if (tid == 0) s_off[nc] = (r0 + nc < C ? cell_off[r0 + nc] : counters[2]) - base;  // counters[2]: all cell pairs
+    if (tid == 0) {
+        s_off[nc] = (r0 + nc < C ? cell_off[r0 + nc] : counters[2]) - base;  // counters[2]: all cell pairs
+        s_nsurv = 0;
+    }
     __syncthreads();
     if (tid < nc) {
         const uint64_t n = (uint64_t)s_self[tid].y;
@@ -382,6 +394,7 @@ __global__ void __launch_bounds__(FILL_THREADS) k_cell_fill(const int32_t* __res
     const int64_t pos0 = counters[1] + base;
     // a uniform trip count over the block, so that the survivors' ballot
     // sees every lane of a wave
+    int rounds = 0;
     for (int64_t q0 = 0; q0 < total; q0 += FILL_THREADS) {
         const int64_t q = q0 + tid;
         bool have = q < total;
@@ -432,7 +445,33 @@ __global__ void __launch_bounds__(FILL_THREADS) k_cell_fill(const int32_t* __res
         if (sph) {  // apart: decided (collision 0); the others go to GJK, which writes theirs
             const bool keep = have && spheres_meet_packed(sph, a, b);
             if (have && !keep) collision[pos] = 0;
-            append_survivor(keep, pos, surv, n_surv, cap);
+            const uint64_t m = __ballot(keep);
+            if (m) {
+                const int lane = tid & (MWD_WAVE - 1);
+                const int leader = __ffsll((long long)m) - 1;
+                int at = 0;
+                if (lane == leader) at = atomicAdd(&s_nsurv, __popcll(m));
+                at = __shfl(at, leader, MWD_WAVE);
+                const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (MWD_WAVE - lane));
+                if (keep) s_surv[at + __popcll(m & lt)] = (int32_t)pos;
+            }
+            // append after FILL_SURV / FILL_THREADS rounds (the buffer is then
+            // at most full) and after the last; the round count is uniform,
+            // so only the appending rounds pay the barriers
+            if (++rounds == FILL_SURV / FILL_THREADS || q0 + FILL_THREADS >= total) {
+                rounds = 0;
+                __syncthreads();
+                const int cnt = s_nsurv;
+                if (tid == 0 && cnt > 0)
+                    s_gbase = (int64_t)atomicAdd((unsigned long long*)n_surv, (unsigned long long)cnt);
+                __syncthreads();
+                const int64_t gb = s_gbase;
+                for (int k = tid; k < cnt; k += FILL_THREADS)
+                    if (gb + k < cap) surv[gb + k] = s_surv[k];
+                __syncthreads();  // every lane has read the count and copied before the next round appends
+                if (tid == 0) s_nsurv = 0;
+                __syncthreads();
+            }
         }
     }
 }
