@@ -170,22 +170,6 @@ __device__ __forceinline__ bool spheres_meet_packed(const double* __restrict__ s
     return !(nrm2(d) > rr * rr);
 }
 
-// Append pair p to the sphere-test survivors when keep, one atomic per wave
-// for the lanes that call this together.
-__device__ __forceinline__ void append_survivor(bool keep, int64_t p, int32_t* surv, int64_t* n_surv, int64_t cap) {
-    const uint64_t m = __ballot(keep);
-    if (!m) return;
-    const uint64_t act = __ballot(1);
-    const int lane = threadIdx.x & (MWD_WAVE - 1);
-    const int leader = __ffsll((long long)act) - 1;
-    int64_t base = 0;
-    if (lane == leader) base = (int64_t)atomicAdd((unsigned long long*)n_surv, (unsigned long long)__popcll(m));
-    base = __shfl(base, leader, MWD_WAVE);
-    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (MWD_WAVE - lane));
-    const int64_t at = base + __popcll(m & lt);
-    if (keep && at < cap) surv[at] = (int32_t)p;
-}
-
 // The contact records of row P in ascending record order (2p + side for the
 // contact of candidate pair p, so in the order of the candidate pairs,
 // whatever order the contacts were found in): its bucket of at most
