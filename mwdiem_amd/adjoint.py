@@ -20,7 +20,9 @@ import torch
 
 from . import _lib
 from .diff import _with_theta, bed_outcome
-from .stepper import Stepper
+from .stepper import STATE_KEYS, Stepper
+
+SNAP_KEYS = STATE_KEYS + ("radius", "wall_c")
 
 
 def _set_init(S: Stepper, init):
@@ -60,18 +62,37 @@ def adjoint_run(sc, init, theta, steps: int, every: int = 100, top_frac: float =
     lam["com"].copy_(g)
     ptrs = (C.c_void_p * len(_lib.TAN_ARRAYS))(*[lam[k].data_ptr() for k in _lib.TAN_ARRAYS])
     dth = torch.zeros(_lib.N_TAN, dtype=torch.float64, device=S.device)
+    # one segment's step inputs, saved and put back with one fused copy each;
+    # with steps_per_graph the recomputed step and the adjoint step replay
+    # graphs of one step each (captured once, on the stepper's fixed buffers)
+    live = [S.t[k] for k in SNAP_KEYS]
+    slots = [[torch.empty_like(t) for t in live] for _ in range(min(every, steps))]
+    g_step = g_adj = None
     for j in reversed(range(len(ckpt))):
         n0 = j * every
         n1 = min(n0 + every, steps)
         S.restore(ckpt[j])
-        states = []
-        for _ in range(n0, n1):
-            states.append(S.snapshot())
-            S.step(1)
-        for k in reversed(range(n1 - n0)):
-            S.restore(states[k])
-            S.step_adjoint(ptrs, dth)
-        del states
+        for i in range(n1 - n0):
+            with torch.cuda.stream(S.stream):
+                torch._foreach_copy_(slots[i], live)
+            if steps_per_graph:
+                if g_step is None:
+                    g_step = S.capture(1)
+                with torch.cuda.stream(S.stream):
+                    g_step.replay()
+            else:
+                S.step(1)
+        for i in reversed(range(n1 - n0)):
+            with torch.cuda.stream(S.stream):
+                torch._foreach_copy_(live, slots[i])
+            if g_adj is not None:
+                with torch.cuda.stream(S.stream):
+                    g_adj.replay()
+            else:
+                S.step_adjoint(ptrs, dth)
+                if steps_per_graph:
+                    g_adj = S.capture_adjoint(ptrs, dth)
+    del slots
     S.check()
     torch.cuda.synchronize(S.device)
     t2 = time.perf_counter()

@@ -841,8 +841,8 @@ MWD_API int mwd_step_adjoint(mwd_workspace* ws, mwd_state* st, double* const* la
             mwd_set_error("mwd_step_adjoint: scene " + std::to_string(s) + " is not on the physical law");
             return MWD_E_INVALID;
         }
-    MWD_HIP_CHECK(hipSetDevice(ws->device));
-    if (!ws->adj_F) {
+    if (!ws->adj_F) {  // the first call allocates (so later calls can be captured in a graph)
+        MWD_HIP_CHECK(hipSetDevice(ws->device));
         const size_t r3 = sizeof(double) * 3 * (size_t)ws->n_rows;
         MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_F, r3));
         MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_T, r3));

@@ -276,6 +276,20 @@ class Stepper:
         _lib.check(_lib.load().mwd_step_adjoint(self.ws, C.byref(self.state), lam_ptrs, dtheta.data_ptr()),
                    "mwd_step_adjoint")
 
+    def capture_adjoint(self, lam_ptrs, dtheta: torch.Tensor):
+        """A HIP graph of one step_adjoint on these adjoint arrays (capture
+        runs nothing). step_adjoint must have run once before (its first call
+        allocates)."""
+        self._enter()
+        torch.cuda.synchronize(self.device)
+        g = torch.cuda.CUDAGraph()
+        lib = _lib.load()
+        with torch.cuda.graph(g, stream=self.stream):
+            _lib.check(lib.mwd_step_adjoint(self.ws, C.byref(self.state), lam_ptrs, dtheta.data_ptr()),
+                       "mwd_step_adjoint")
+        torch.cuda.synchronize(self.device)
+        return g
+
     def enable_tangents(self, seed=None):
         """Carry d(state)/d(theta) for three law parameters theta_k.
 

@@ -105,6 +105,10 @@ def test_adjoint_gradient_matches_tangents(gpu):
     print(f"outcome {L_a} vs {L_t}; adjoint {g_a} tangents {g_t} rel {rel}")
     assert L_a == L_t
     assert np.all(rel <= 1e-9)
+    # the same sweep with the forward, recomputed and adjoint steps replayed
+    # from graphs (fused state copies between the replays): bit for bit
+    L_g, g_g = adjoint_run(sc, v0, theta, 400, every=64, steps_per_graph=16)
+    assert L_g == L_a and np.array_equal(g_g, g_a), (g_g, g_a)
     th = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
     out = BedOutcomeAdjoint.apply(th, sc, v0, 400, 128)
     out.backward()
