@@ -298,6 +298,14 @@ int mwd_kernel_times(mwd_workspace* ws, double* ms, int64_t* steps);
 int mwd_radius_init(const double* com, const double* verts, const int32_t* nverts, int64_t n, int32_t vmax,
                     double* radius, void* stream);
 
+/* dst[i] <- src[i] for i < n (n <= MWD_MAX_COPY arrays, bytes[i] a multiple
+ * of 8, 8-byte aligned device pointers) in one kernel launch on `stream`.
+ * No counterpart in the reference: the reverse sweep uses it to save and
+ * restore a step's input state (a dozen arrays) in one launch instead of one
+ * per array. */
+#define MWD_MAX_COPY 16
+int mwd_copy_arrays(int32_t n, void* const* dst, const void* const* src, const int64_t* bytes, void* stream);
+
 /* Counts and flags of the last step (synchronises the workspace stream). */
 int mwd_step_status(mwd_workspace* ws, mwd_status* out);
 int mwd_clear_flags(mwd_workspace* ws);

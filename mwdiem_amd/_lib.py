@@ -128,6 +128,7 @@ _SIGS = {
     "mwd_clear_flags": (C.c_int, [_P]),
     "mwd_set_timing": (C.c_int, [_P, C.c_int]),
     "mwd_radius_init": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int32, _P, _P]),
+    "mwd_copy_arrays": (C.c_int, [C.c_int32, C.POINTER(_P), C.POINTER(_P), C.POINTER(C.c_int64), _P]),
     "mwd_stage_times": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "mwd_kernel_times": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "mwd_law_tangent": (C.c_int, [C.POINTER(LawTan), C.c_double, C.c_double, C.c_double, C.c_double,
@@ -204,6 +205,25 @@ def stream_handle(torch_stream=None) -> int | None:
     import torch
     s = torch_stream if torch_stream is not None else torch.cuda.current_stream()
     return s.cuda_stream or None
+
+
+MAX_COPY = 16
+
+
+def copy_arrays(dst, src, stream=None) -> None:
+    """dst[i].copy_(src[i]) for lists of contiguous device tensors of equal
+    sizes (at most MAX_COPY), in one kernel launch on `stream` (a HIP stream
+    handle; the current stream when None)."""
+    n = len(dst)
+    if n != len(src) or n > MAX_COPY:
+        raise ValueError(f"copy_arrays: {n} destinations for {len(src)} sources (at most {MAX_COPY})")
+    for d, s in zip(dst, src):
+        if d.nbytes != s.nbytes or not d.is_contiguous() or not s.is_contiguous():
+            raise ValueError("copy_arrays: arrays must be contiguous and of equal byte sizes")
+    D = (_P * n)(*[d.data_ptr() for d in dst])
+    S = (_P * n)(*[s.data_ptr() for s in src])
+    B = (C.c_int64 * n)(*[d.nbytes for d in dst])
+    check(load().mwd_copy_arrays(n, D, S, B, stream if stream is not None else stream_handle()), "mwd_copy_arrays")
 
 
 def loaded_path() -> str:

@@ -62,19 +62,19 @@ def adjoint_run(sc, init, theta, steps: int, every: int = 100, top_frac: float =
     lam["com"].copy_(g)
     ptrs = (C.c_void_p * len(_lib.TAN_ARRAYS))(*[lam[k].data_ptr() for k in _lib.TAN_ARRAYS])
     dth = torch.zeros(_lib.N_TAN, dtype=torch.float64, device=S.device)
-    # one segment's step inputs, saved and put back with one fused copy each;
+    # one segment's step inputs, saved and put back with one copy launch each;
     # with steps_per_graph the recomputed step and the adjoint step replay
     # graphs of one step each (captured once, on the stepper's fixed buffers)
     live = [S.t[k] for k in SNAP_KEYS]
     slots = [[torch.empty_like(t) for t in live] for _ in range(min(every, steps))]
+    stream = S.stream.cuda_stream
     g_step = g_adj = None
     for j in reversed(range(len(ckpt))):
         n0 = j * every
         n1 = min(n0 + every, steps)
         S.restore(ckpt[j])
         for i in range(n1 - n0):
-            with torch.cuda.stream(S.stream):
-                torch._foreach_copy_(slots[i], live)
+            _lib.copy_arrays(slots[i], live, stream)
             if steps_per_graph:
                 if g_step is None:
                     g_step = S.capture(1)
@@ -83,8 +83,7 @@ def adjoint_run(sc, init, theta, steps: int, every: int = 100, top_frac: float =
             else:
                 S.step(1)
         for i in reversed(range(n1 - n0)):
-            with torch.cuda.stream(S.stream):
-                torch._foreach_copy_(live, slots[i])
+            _lib.copy_arrays(live, slots[i], stream)
             if g_adj is not None:
                 with torch.cuda.stream(S.stream):
                     g_adj.replay()

@@ -24,7 +24,9 @@
 //   k_gather_adj   per moving row: its contacts' contributions added in the
 //                  row's record order (deterministic, no atomics), face-vertex
 //                  adjoints into the row's vertex slots; buckets emptied;
-//   k_theta_sum    one block: the contacts' d/d theta summed in a fixed order.
+//                  each contact's d/d theta added to the row holding its A
+//                  side (B when A is a wall);
+//   k_theta_sum    one block: the rows' d/d theta summed in a fixed order.
 // A square root of zero (a body at rest) takes a zero derivative, as the
 // dual-number path does; a face whose re-evaluation does not reproduce EPA's
 // values (a placeholder face) keeps its geometry without derivative.
@@ -508,15 +510,26 @@ __global__ void __launch_bounds__(256) k_gather_adj(const int32_t* __restrict__ 
                                                     const int32_t* __restrict__ rec_ovf, const int64_t* n_ovf,
                                                     int64_t ovf_cap, const int32_t* __restrict__ sidx,
                                                     const double* __restrict__ cadj,
-                                                    const int32_t* __restrict__ face, Lam lam, int vmax) {
+                                                    const int32_t* __restrict__ face, Lam lam, int vmax,
+                                                    const int32_t* __restrict__ pairs,
+                                                    const uint8_t* __restrict__ is_wall,
+                                                    const double* __restrict__ cdth, double* __restrict__ rowth) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
     const int64_t P = plist[i];
     const int n = rec_cnt[P];
-    if (n <= 0) return;
+    // the row's share of d/d theta: the contacts it holds the A side of (the
+    // B side when A is a wall), in the row's record order
+    double th[MWD_N_TAN] = {};
+    if (n <= 0) {
+        for (int k = 0; k < MWD_N_TAN; ++k) rowth[MWD_N_TAN * P + k] = 0.0;
+        return;
+    }
     for_row_records(P, n, rec_bkt, rec_ovf, n_ovf, ovf_cap, [&](int32_t r) {
         const int64_t c = sidx[r >> 1];
         const int side = r & 1;
+        if (side == (is_wall[pairs[2 * (int64_t)(r >> 1)]] ? 1 : 0))
+            for (int k = 0; k < MWD_N_TAN; ++k) th[k] += cdth[MWD_N_TAN * c + k];
         const double* o = cadj + (2 * c + side) * CADJ;
         add3(lam.com + 3 * P, ld3(o));
         add3(lam.v + 3 * P, ld3(o + 3));
@@ -526,19 +539,20 @@ __global__ void __launch_bounds__(256) k_gather_adj(const int32_t* __restrict__ 
             if (f[k] >= 0 && f[k] < vmax) add3(lam.verts + 3 * ((int64_t)vmax * P + f[k]), ld3(o + 9 + 3 * k));
     });
     rec_cnt[P] = 0;
+    for (int k = 0; k < MWD_N_TAN; ++k) rowth[MWD_N_TAN * P + k] = th[k];
 }
 
-// The contacts' d/d theta, summed in a fixed order by one block and added to
-// the accumulator.
+// The rows' shares of d/d theta, summed by one block in a fixed order (thread
+// t takes the moving rows t, t + TSUM, ..., then a fixed tree) and added to
+// the accumulator: the same sum whatever order GJK gave the contact slots.
 constexpr int TSUM = 256;
-__global__ void __launch_bounds__(TSUM) k_theta_sum(const double* __restrict__ cdth, const int64_t* n_contacts,
-                                                    int64_t contact_cap, double* __restrict__ dtheta) {
+__global__ void __launch_bounds__(TSUM) k_theta_sum(const double* __restrict__ rowth,
+                                                    const int32_t* __restrict__ plist, int64_t M,
+                                                    double* __restrict__ dtheta) {
     __shared__ double s[MWD_N_TAN][TSUM];
-    int64_t K = *n_contacts;
-    if (K > contact_cap) K = contact_cap;
     double acc[MWD_N_TAN] = {};
-    for (int64_t c = threadIdx.x; c < K; c += TSUM)
-        for (int k = 0; k < MWD_N_TAN; ++k) acc[k] += cdth[MWD_N_TAN * c + k];
+    for (int64_t i = threadIdx.x; i < M; i += TSUM)
+        for (int k = 0; k < MWD_N_TAN; ++k) acc[k] += rowth[MWD_N_TAN * (int64_t)plist[i] + k];
     for (int k = 0; k < MWD_N_TAN; ++k) s[k][threadIdx.x] = acc[k];
     __syncthreads();
     for (int h = TSUM / 2; h > 0; h >>= 1) {
@@ -585,10 +599,10 @@ int mwd_launch_step_adjoint(mwd_workspace* ws, const mwd_state* st, const BodyVi
     if (M > 0) {
         hipLaunchKernelGGL(k_gather_adj, dim3(blocks_of(M, 1ll << 30)), dim3(256), 0, s, ws->plist, M, ws->rec_cnt,
                            ws->rec_bkt, ws->rec_ovf, ctr + 11, 2 * ws->contact_cap, ws->sidx, ws->adj_c,
-                           ws->adj_face, lam, (int)st->vmax);
+                           ws->adj_face, lam, (int)st->vmax, ws->pairs, ws->is_wall, ws->adj_cth, ws->adj_rowth);
+        MWD_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_theta_sum, dim3(1), dim3(TSUM), 0, s, ws->adj_rowth, ws->plist, M, dtheta);
         MWD_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(k_theta_sum, dim3(1), dim3(TSUM), 0, s, ws->adj_cth, ctr + 4, ws->contact_cap, dtheta);
-    MWD_LAUNCH_CHECK();
     return MWD_OK;
 }

@@ -50,7 +50,7 @@ void free_ws(mwd_workspace* ws) {
                     ws->rec_bkt, ws->rec_ovf, ws->hist,
                     ws->hist_off, ws->scan_tmp, ws->scan_tmp_side, ws->counters, ws->flags, ws->laws,
                     ws->scene_wt_off, ws->ctan, ws->law_tans, ws->adj_F, ws->adj_T, ws->adj_lamF,
-                    ws->adj_lamT, ws->adj_face, ws->adj_c, ws->adj_cth};
+                    ws->adj_lamT, ws->adj_face, ws->adj_c, ws->adj_cth, ws->adj_rowth};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     destroy_events(ws);
@@ -851,6 +851,7 @@ MWD_API int mwd_step_adjoint(mwd_workspace* ws, mwd_state* st, double* const* la
         MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_face, sizeof(int32_t) * 7 * (size_t)ws->contact_cap));
         MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_c, sizeof(double) * 36 * (size_t)ws->contact_cap));
         MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_cth, sizeof(double) * 3 * (size_t)ws->contact_cap));
+        MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_rowth, r3));
     }
     int rc = mwd_candidates(ws, st, nullptr);
     if (rc) return rc;
@@ -861,6 +862,24 @@ MWD_API int mwd_step_adjoint(mwd_workspace* ws, mwd_state* st, double* const* la
     rc = mwd_launch_step_adjoint(ws, st, state_view(ws, st, 0), state_view(ws, st, 1), lam, dtheta);
     ws->bkt_dirty = 0;  // the gather emptied the buckets
     return rc;
+}
+
+MWD_API int mwd_copy_arrays(int32_t n, void* const* dst, const void* const* src, const int64_t* bytes, void* stream) {
+    if (n < 0 || n > MWD_MAX_COPY || (n > 0 && (!dst || !src || !bytes))) {
+        mwd_set_error("mwd_copy_arrays: n must be 0.." + std::to_string(MWD_MAX_COPY) + " with arrays given");
+        return MWD_E_INVALID;
+    }
+    int64_t words[MWD_MAX_COPY];
+    for (int i = 0; i < n; ++i) {
+        if (bytes[i] < 0 || bytes[i] % 8 || (bytes[i] && (!dst[i] || !src[i])) ||
+            ((uintptr_t)dst[i] | (uintptr_t)src[i]) % 8) {
+            mwd_set_error("mwd_copy_arrays: array " + std::to_string(i) +
+                          ": sizes must be multiples of 8 bytes and pointers 8-byte aligned");
+            return MWD_E_INVALID;
+        }
+        words[i] = bytes[i] / 8;
+    }
+    return mwd_launch_copy_arrays(n, dst, src, words, (hipStream_t)stream);
 }
 
 MWD_API int mwd_radius_init(const double* com, const double* verts, const int32_t* nverts, int64_t n, int32_t vmax,

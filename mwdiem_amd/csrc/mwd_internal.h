@@ -278,6 +278,7 @@ struct mwd_workspace {
     int32_t* adj_face;             // [7*contact_cap] EPA's final face per contact
     double* adj_c;                 // [2*18*contact_cap] per contact side: adjoint contributions
     double* adj_cth;               // [3*contact_cap] per contact: d/d theta
+    double* adj_rowth;             // [3*n_rows] per moving row: its contacts' share of d/d theta
     int emit_faces;                // host: the next fine-contact pass writes adj_face
     // per moving row, the records (2c for side A of contact c, 2c+1 for side
     // B) of this step's contacts, in arrival order; the reduction sorts them
@@ -371,6 +372,8 @@ int mwd_launch_hist_flip(const HistView& h, hipStream_t s);
 int mwd_launch_radius(const double* com, const double* verts, const int32_t* nverts, int64_t n, int32_t vmax,
                       double* radius, hipStream_t s);
 int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t* keys);
+// n arrays of 8-byte words: dst[i][0, words[i]) <- src[i][0, words[i]).
+int mwd_launch_copy_arrays(int n, void* const* dst, const void* const* src, const int64_t* words, hipStream_t s);
 // Adjoint of one step after its contact stages have run on the step's input
 // state (adj_face written): lam in/out, dtheta (device, 3) accumulated.
 int mwd_launch_step_adjoint(mwd_workspace* ws, const mwd_state* st, const BodyView& A, const BodyView& B,

@@ -125,7 +125,45 @@ __global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
     for (; i < n; i += st) p[i] = v;
 }
 
+struct CopyJob {
+    const uint64_t* src[MWD_MAX_COPY];
+    uint64_t* dst[MWD_MAX_COPY];
+    int64_t end[MWD_MAX_COPY];  // running word counts: array i holds words [end[i-1], end[i])
+    int n;
+};
+
+// All arrays of the job as one grid-strided range of words.
+__global__ void k_copy_arrays(CopyJob job) {
+    const int64_t total = job.end[job.n - 1];
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += st) {
+        int i = 0;
+        while (w >= job.end[i]) ++i;
+        const int64_t o = w - (i ? job.end[i - 1] : 0);
+        job.dst[i][o] = job.src[i][o];
+    }
+}
+
 }  // namespace
+
+int mwd_launch_copy_arrays(int n, void* const* dst, const void* const* src, const int64_t* words, hipStream_t s) {
+    if (n <= 0) return MWD_OK;
+    CopyJob job;
+    int64_t acc = 0;
+    for (int i = 0; i < n; ++i) {
+        job.src[i] = static_cast<const uint64_t*>(src[i]);
+        job.dst[i] = static_cast<uint64_t*>(dst[i]);
+        acc += words[i];
+        job.end[i] = acc;
+    }
+    job.n = n;
+    if (acc == 0) return MWD_OK;
+    int64_t blocks = (acc + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    hipLaunchKernelGGL(k_copy_arrays, dim3((unsigned)blocks), dim3(256), 0, s, job);
+    MWD_LAUNCH_CHECK();
+    return MWD_OK;
+}
 
 int64_t mwd_scan_tmp_len(int64_t n_cap) {
     const int64_t smallest_tile = SCAN_THREADS;  // one item per thread
