@@ -194,6 +194,8 @@ int mwd_set_reduction(mwd_workspace* ws, int32_t mode);
  *   vertex_pack  1: the vertex pass walks the block's real vertices only
  *                (prefix sum of the vertex counts); 0 (default): every vmax
  *                slot, skipping the padding.
+ *   epa_blocks   grid cap of the grid-strided EPA launch (4 contacts per
+ *                block at a time), 16..65536 (2048).
  * mwd_set_tuning applies to later steps (graphs captured before keep their
  * launches) and returns MWD_E_INVALID for a choice out of range. */
 #define MWD_CELLS_TABLE 0
@@ -207,6 +209,7 @@ typedef struct mwd_tuning {
     int32_t cell_items;
     int32_t verlet_waves;
     int32_t vertex_pack;
+    int32_t epa_blocks;
 } mwd_tuning;
 int mwd_get_tuning(const mwd_workspace* ws, mwd_tuning* out);
 int mwd_set_tuning(mwd_workspace* ws, const mwd_tuning* t);

@@ -479,6 +479,7 @@ MWD_API int mwd_get_tuning(const mwd_workspace* ws, mwd_tuning* out) {
     out->cell_items = ws->tune.cell_items;
     out->verlet_waves = ws->tune.verlet_waves;
     out->vertex_pack = ws->tune.vertex_pack;
+    out->epa_blocks = ws->tune.epa_blocks;
     return MWD_OK;
 }
 
@@ -495,10 +496,12 @@ MWD_API int mwd_set_tuning(mwd_workspace* ws, const mwd_tuning* t) {
     const bool items_ok = t->cell_items == 0 || t->cell_items == 1 || t->cell_items == 2 || t->cell_items == 4;
     const bool waves_ok = t->verlet_waves == 1 || t->verlet_waves == 3 || t->verlet_waves == 4;
     const bool pack_ok = t->vertex_pack == 0 || t->vertex_pack == 1;
-    if (!pass_ok || !group_ok || !blocks_ok || !sphere_ok || !verlet_ok || !items_ok || !waves_ok || !pack_ok) {
+    const bool epa_ok = t->epa_blocks >= 16 && t->epa_blocks <= 65536;
+    if (!pass_ok || !group_ok || !blocks_ok || !sphere_ok || !verlet_ok || !items_ok || !waves_ok || !pack_ok ||
+        !epa_ok) {
         mwd_set_error(!pass_ok ? "mwd_set_tuning: cell_pass unknown, or MWD_CELLS_TABLE without key tables"
                                : "mwd_set_tuning: gjk_group, gjk_blocks, sphere_block, verlet_block, cell_items, "
-                                 "verlet_waves or vertex_pack out of range");
+                                 "verlet_waves, vertex_pack or epa_blocks out of range");
         return MWD_E_INVALID;
     }
     ws->cell_table = t->cell_pass == MWD_CELLS_TABLE;
@@ -509,6 +512,7 @@ MWD_API int mwd_set_tuning(mwd_workspace* ws, const mwd_tuning* t) {
     ws->tune.cell_items = t->cell_items;
     ws->tune.verlet_waves = t->verlet_waves;
     ws->tune.vertex_pack = t->vertex_pack;
+    ws->tune.epa_blocks = t->epa_blocks;
     return MWD_OK;
 }
 
@@ -640,7 +644,7 @@ static int fine_contact(mwd_workspace* ws, const mwd_state* st, double* const* t
                                        ws->contact_cap, ctr + 4, ws->depth, ws->normal, ws->cpA, ws->cpB,
                                        contact_out(ws, false), ws->epa_status, ws->rec_cnt, ws->rec_bkt, ws->rec_ovf,
                                        ctr + 11, 2 * ws->contact_cap, ws->flags, s, mwd_kev(ws, MWD_KT_EPA, 0),
-                                       mwd_kev(ws, MWD_KT_EPA, 1));
+                                       mwd_kev(ws, MWD_KT_EPA, 1), ws->tune.epa_blocks);
     }
     if (rc) return rc;
     if (ws->hist_on && !tan && (rc = mwd_launch_hist_flip(ws->thist, s))) return rc;

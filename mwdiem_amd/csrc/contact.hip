@@ -25,9 +25,11 @@
 namespace {
 
 constexpr int EPA_FACES = MWD_EPA_MAX_FACES;
-// Grid cap of the grid-strided EPA launch (one contact per wave): 2,048
-// blocks of 4 waves, about what the polytopes' LDS lets stay resident (7
-// blocks per CU); more contacts loop.
+// Grid cap of the grid-strided EPA launch of the per-pair entry point (one
+// contact per wave): 2,048 blocks of 4 waves, about what the polytopes' LDS
+// lets stay resident (7 blocks per CU); more contacts loop. The stepper's
+// launch takes its cap from the tuning (a small grid costs less to dispatch
+// when contacts are few).
 constexpr int64_t EPA_MAX_BLOCKS = 2048;
 constexpr int EPA_VERTS = 32;
 
@@ -1321,9 +1323,10 @@ int mwd_launch_epa_force_rows(const BodyView& A, const BodyView& B, const BodyTa
                               const int32_t* sidx, int64_t contact_cap, const int64_t* n_contacts, double* depth,
                               double* normal, double* cpA, double* cpB, const ContactOut& co, int32_t* epa_status,
                               int32_t* rec_cnt, int32_t* rec_bkt, int32_t* rec_ovf, int64_t* n_ovf,
-                              int64_t ovf_cap, uint32_t* flags, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+                              int64_t ovf_cap, uint32_t* flags, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                              int64_t max_blocks) {
     if (contact_cap <= 0) return MWD_OK;
-    const dim3 grid(grid_for(contact_cap * MWD_WAVE, 256, EPA_MAX_BLOCKS)), blk(256);
+    const dim3 grid(grid_for(contact_cap * MWD_WAVE, 256, max_blocks)), blk(256);
     if (bt) {
         MWD_LAUNCH_EV(e0, e1, k_epa_force<Dual<MWD_NT>>, grid, blk, 0, s, A, B, *bt, laws2, lawt2, contact_pair, simplex,
                            sidx, contact_cap, n_contacts, (const double*)nullptr, (const int32_t*)nullptr,

@@ -130,6 +130,7 @@ struct PairTune {
     int cell_items = 1;      // key-table cell pass: cells per thread of the fused cell scan (1, 2, 4), 0 = split
     int verlet_waves = 1;    // minimum waves per SIMD asked of the reduction + integrator's registers: 1, 3 or 4
     int vertex_pack = 0;     // vertex pass over real vertices only (1) or over every vmax slot (0)
+    int epa_blocks = 2048;   // grid cap of the EPA launch (blocks of 4 waves, one contact per wave)
 };
 
 // Face distances at or above this are EPA's placeholders (1e30 for a face
@@ -419,7 +420,7 @@ int mwd_launch_epa_force_rows(const BodyView& A, const BodyView& B, const BodyTa
                               double* normal, double* cpA, double* cpB, const ContactOut& co, int32_t* epa_status,
                               int32_t* rec_cnt, int32_t* rec_bkt, int32_t* rec_ovf, int64_t* n_ovf,
                               int64_t ovf_cap, uint32_t* flags, hipStream_t s, hipEvent_t e0 = nullptr,
-                              hipEvent_t e1 = nullptr);
+                              hipEvent_t e1 = nullptr, int64_t max_blocks = 2048);
 // Zero the per-pair outputs, then copy per-contact results to their pairs.
 int mwd_launch_zero_pairs(int64_t P, double* depth, double* normal, double* cpA, double* cpB, int32_t* epa_status,
                           double* disp, hipStream_t s);

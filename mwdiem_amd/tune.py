@@ -4,9 +4,10 @@ The knobs are those of mwd_tuning (include/mwdiem_hip.h): how the cells are
 built (key tables or a radix sort of the keys), GJK lanes per pair (so pairs
 per wave), the GJK grid cap, the sphere-test block, the particles per block
 of the fused force reduction + integrator, the occupancy asked of its
-register allocation and whether its vertex pass skips padding slots. None
-of them changes a bit of the results, so the tuner times them on the live state, restoring it
-before every trial, and leaves the stepper on the fastest choice.
+register allocation, whether its vertex pass skips padding slots and the
+EPA grid cap. None of them changes a bit of the results, so the tuner
+times them on the live state, restoring it before every trial, and leaves
+the stepper on the fastest choice.
 
 For scenes on the physical law the cell size is a free parameter as well:
 any cell at least the largest particle extent finds every contact with the
@@ -37,6 +38,7 @@ SPACE = {
     "cell_items": (0, 1, 2, 4),
     "verlet_waves": (1, 3, 4),
     "vertex_pack": (0, 1),
+    "epa_blocks": (128, 512, 2048),
 }
 
 
