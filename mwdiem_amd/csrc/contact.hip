@@ -560,8 +560,9 @@ __device__ __forceinline__ int support_index_w(const Side& S, d3 d, int lane) {
     return wave_min_i(best == m ? bi : 0x7fffffff);
 }
 
-// First index of the minimum distance over faces [0, nt) (epa()'s scan: a
-// NaN never wins, unless it is face 0's and so the starting value).
+// First index of the minimum distance over faces [0, nt) (array_min_finder,
+// .cu:41-52: a strict-less scan from face 0, so a NaN never wins unless it is
+// face 0's and so the starting value).
 __device__ __forceinline__ int closest_face_w(const PolyW& P, int nt, int lane) {
     const double inf = __builtin_inf();
     double x0 = inf, x1 = inf;
