@@ -93,16 +93,33 @@ struct GroupSup {
             vb[k] = i < B.n ? ld3(B.v + 3 * i) : mk3(0.0, 0.0, 0.0);
         }
     }
-    __device__ __forceinline__ static void merge(double& best, int& bi) {
-#pragma unroll
-        for (int off = 1; off < G; off <<= 1) {
-            const double ob = __shfl_xor(best, off, G);
-            const int oi = __shfl_xor(bi, off, G);
-            if (ob > best || (ob == best && oi < bi)) {
-                best = ob;
-                bi = oi;
-            }
+    // One exchange of the merge: each lane meets the lane that the DPP
+    // permutation CTRL names (within its quad, 8-lane half or 16-lane row) and
+    // keeps the better (value, index).
+    template <int CTRL>
+    __device__ __forceinline__ static void meet(double& best, int& bi) {
+        const int2 w = *reinterpret_cast<const int2*>(&best);
+        int2 ow;
+        ow.x = __builtin_amdgcn_update_dpp(0, w.x, CTRL, 0xF, 0xF, false);
+        ow.y = __builtin_amdgcn_update_dpp(0, w.y, CTRL, 0xF, 0xF, false);
+        const double ob = *reinterpret_cast<const double*>(&ow);
+        const int oi = __builtin_amdgcn_update_dpp(0, bi, CTRL, 0xF, 0xF, false);
+        if (ob > best || (ob == best && oi < bi)) {
+            best = ob;
+            bi = oi;
         }
+    }
+    // The group's larger value, and on a tie its smaller index, in every lane
+    // of the group. The choice does not depend on the order of the exchanges,
+    // so DPP permutations (register-to-register, no LDS round trip) pair the
+    // lanes: swap neighbours, swap pairs (quad_perm), then mirror the 8-lane
+    // halves (row_half_mirror) and the 16-lane rows (row_mirror).
+    __device__ __forceinline__ static void merge(double& best, int& bi) {
+        static_assert(G == 4 || G == 8 || G == 16, "groups of 4, 8 or 16 lanes");
+        meet<0xB1>(best, bi);  // quad_perm [1,0,3,2]
+        meet<0x4E>(best, bi);  // quad_perm [2,3,0,1]
+        if constexpr (G >= 8) meet<0x141>(best, bi);  // row_half_mirror
+        if constexpr (G >= 16) meet<0x140>(best, bi);  // row_mirror
     }
     __device__ __forceinline__ d3 operator()(d3 d, int* ia, int* ib) const {
         constexpr int NONE = 0x7fffffff;
