@@ -535,29 +535,47 @@ __device__ __forceinline__ void add_triangle_w(PolyW& P, int e0, int e1, int e2,
     P.dist[id] = dd;
 }
 
-__device__ __forceinline__ double wave_min(double x) {
-#pragma unroll
-    for (int o = 1; o < MWD_WAVE; o <<= 1) {
-        const double y = __shfl_xor(x, o, MWD_WAVE);
-        x = y < x ? y : x;
-    }
+// Wave-wide min / max: exchanges within 16-lane rows by DPP permutes
+// (register to register), then across rows by shuffles. min and max do not
+// depend on the order of the exchanges.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double x) {
+    const int2 w = *reinterpret_cast<const int2*>(&x);
+    int2 o;
+    o.x = __builtin_amdgcn_update_dpp(0, w.x, CTRL, 0xF, 0xF, false);
+    o.y = __builtin_amdgcn_update_dpp(0, w.y, CTRL, 0xF, 0xF, false);
+    return *reinterpret_cast<const double*>(&o);
+}
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int x) {
+    return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, false);
+}
+template <class T, class Pick, class Dpp>
+__device__ __forceinline__ T wave_reduce(T x, Pick pick, Dpp) {
+    x = pick(x, Dpp::template get<0xB1>(x));   // quad_perm [1,0,3,2]
+    x = pick(x, Dpp::template get<0x4E>(x));   // quad_perm [2,3,0,1]
+    x = pick(x, Dpp::template get<0x141>(x));  // row_half_mirror
+    x = pick(x, Dpp::template get<0x140>(x));  // row_mirror
+    x = pick(x, __shfl_xor(x, 16, MWD_WAVE));
+    x = pick(x, __shfl_xor(x, 32, MWD_WAVE));
     return x;
+}
+struct DppD {
+    template <int CTRL>
+    __device__ static double get(double x) { return dpp_d<CTRL>(x); }
+};
+struct DppI {
+    template <int CTRL>
+    __device__ static int get(int x) { return dpp_i<CTRL>(x); }
+};
+__device__ __forceinline__ double wave_min(double x) {
+    return wave_reduce(x, [](double a, double b) { return b < a ? b : a; }, DppD{});
 }
 __device__ __forceinline__ double wave_max(double x) {
-#pragma unroll
-    for (int o = 1; o < MWD_WAVE; o <<= 1) {
-        const double y = __shfl_xor(x, o, MWD_WAVE);
-        x = y > x ? y : x;
-    }
-    return x;
+    return wave_reduce(x, [](double a, double b) { return b > a ? b : a; }, DppD{});
 }
 __device__ __forceinline__ int wave_min_i(int x) {
-#pragma unroll
-    for (int o = 1; o < MWD_WAVE; o <<= 1) {
-        const int y = __shfl_xor(x, o, MWD_WAVE);
-        x = y < x ? y : x;
-    }
-    return x;
+    return wave_reduce(x, [](int a, int b) { return b < a ? b : a; }, DppI{});
 }
 
 // Index of the first strict maximum of d.v over a body (support(): best
