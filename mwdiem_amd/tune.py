@@ -4,10 +4,10 @@ The knobs are those of mwd_tuning (include/mwdiem_hip.h): how the cells are
 built (key tables or a radix sort of the keys), GJK lanes per pair (so pairs
 per wave), the GJK grid cap, the sphere-test block, the particles per block
 of the fused force reduction + integrator, the occupancy asked of its
-register allocation, whether its vertex pass skips padding slots and the
-EPA grid cap. None of them changes a bit of the results, so the tuner
-times them on the live state, restoring it before every trial, and leaves
-the stepper on the fastest choice.
+register allocation and whether its vertex pass skips padding slots (the
+EPA grid cap is left to the caller, see below). None of them changes a bit
+of the results, so the tuner times them on the live state, restoring it
+before every trial, and leaves the stepper on the fastest choice.
 
 For scenes on the physical law the cell size is a free parameter as well:
 any cell at least the largest particle extent finds every contact with the
@@ -38,8 +38,11 @@ SPACE = {
     "cell_items": (0, 1, 2, 4),
     "verlet_waves": (1, 3, 4),
     "vertex_pack": (0, 1),
-    "epa_blocks": (128, 512, 2048),
 }
+# Not searched: epa_blocks (the EPA grid cap). Its best value follows the
+# contact count, which grows over a run (a pour starts with none), so a
+# choice timed on the first steps can starve the later ones; set it with
+# Stepper.set_tuning when the contact count is known.
 
 
 def time_steps(S: Stepper, steps: int) -> float:
