@@ -503,3 +503,45 @@ def test_persistent_tangential_history_matches_oracle(gpu, dense_phys):
     for k in ("com", "q", "v", "w"):
         assert max_rel(got[k][W:], st[k][W:]) <= 1e-8, k
     assert np.abs(got["v"][W:] - reset["v"][W:]).max() > 1e-6 * np.abs(reset["v"][W:]).max()
+
+
+def test_sphere_survivors_in_crowded_cells(gpu):
+    """Crowded cells give blocks of the pair emission thousands of rounds, so
+    the survivors of the sphere test go through the emission's buffered
+    appends many times: their count equals the wall pairs (all listed) plus
+    the cell pairs whose bounding spheres meet, recounted on the host from
+    the emitted candidates, the radii and the centres. A second pass over the
+    same candidates (the standalone sphere-test kernel, walls tested too)
+    counts the pairs whose spheres meet among all of them."""
+    from dataclasses import replace
+    from mwdiem_amd import scene
+    base = scene.make_scene(3000, (60.0, 60.0, 120.0), 5, name="crowd-3000")
+    sc = scene.compact_dense(base, gap=-0.3)
+    cell, ncx, ncy, ncz, kmin, kspan = scene.grid_for(sc.box, 2.0 * sc.max_particlesize)
+    sc = replace(sc.copy(), cell=cell, ncx=ncx, ncy=ncy, ncz=ncz, kmin=kmin, kspan=kspan)
+    st = random_motion(O.initial_state(sc), sc, 3, vscale=500.0, wscale=100.0)
+    S = _stepper(sc, pair_cap=4_000_000, contact_cap=200_000)
+    S.set_state(st)
+    S.candidates()
+    S.fine_contact()
+    stat = S.check()
+    pairs = S.pairs()
+    W = stat["n_wall_pairs"]
+    # cells of ~20 members: 64-cell blocks of the emission hold hundreds of
+    # 256-pair rounds each
+    assert (len(pairs) - W) / stat["n_cells"] > 1024
+    com = S.numpy_state()["com"]
+    rad = S.t["radius"].cpu().numpy()
+
+    def meet(pp):
+        a, b = pp[:, 0], pp[:, 1]
+        d = com[b] - com[a]
+        n2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        rr = rad[a] + rad[b]
+        return ~(n2 > rr * rr)
+
+    cell_meet = int(meet(pairs[W:]).sum())
+    assert cell_meet > 1000
+    assert stat["n_sphere_pass"] == W + cell_meet
+    S.fine_contact()  # a second pass: k_sphere over the stored candidates
+    assert S.check()["n_sphere_pass"] == int(meet(pairs).sum())
