@@ -196,6 +196,10 @@ int mwd_set_reduction(mwd_workspace* ws, int32_t mode);
  *                slot, skipping the padding.
  *   epa_blocks   grid cap of the grid-strided EPA launch (4 contacts per
  *                block at a time), 16..65536 (2048).
+ *   epa_fused    1 (default): the step runs EPA and the law inside the GJK
+ *                launch, each wave on the colliding pairs it found (bodies
+ *                of at most 4 * gjk_group vertices, forward steps); 0: a
+ *                separate EPA launch over all contacts.
  * mwd_set_tuning applies to later steps (graphs captured before keep their
  * launches) and returns MWD_E_INVALID for a choice out of range. */
 #define MWD_CELLS_TABLE 0
@@ -210,6 +214,7 @@ typedef struct mwd_tuning {
     int32_t verlet_waves;
     int32_t vertex_pack;
     int32_t epa_blocks;
+    int32_t epa_fused;
 } mwd_tuning;
 int mwd_get_tuning(const mwd_workspace* ws, mwd_tuning* out);
 int mwd_set_tuning(mwd_workspace* ws, const mwd_tuning* t);

@@ -480,6 +480,7 @@ MWD_API int mwd_get_tuning(const mwd_workspace* ws, mwd_tuning* out) {
     out->verlet_waves = ws->tune.verlet_waves;
     out->vertex_pack = ws->tune.vertex_pack;
     out->epa_blocks = ws->tune.epa_blocks;
+    out->epa_fused = ws->tune.epa_fused;
     return MWD_OK;
 }
 
@@ -496,12 +497,12 @@ MWD_API int mwd_set_tuning(mwd_workspace* ws, const mwd_tuning* t) {
     const bool items_ok = t->cell_items == 0 || t->cell_items == 1 || t->cell_items == 2 || t->cell_items == 4;
     const bool waves_ok = t->verlet_waves == 1 || t->verlet_waves == 3 || t->verlet_waves == 4;
     const bool pack_ok = t->vertex_pack == 0 || t->vertex_pack == 1;
-    const bool epa_ok = t->epa_blocks >= 16 && t->epa_blocks <= 65536;
+    const bool epa_ok = t->epa_blocks >= 16 && t->epa_blocks <= 65536 && (t->epa_fused == 0 || t->epa_fused == 1);
     if (!pass_ok || !group_ok || !blocks_ok || !sphere_ok || !verlet_ok || !items_ok || !waves_ok || !pack_ok ||
         !epa_ok) {
         mwd_set_error(!pass_ok ? "mwd_set_tuning: cell_pass unknown, or MWD_CELLS_TABLE without key tables"
                                : "mwd_set_tuning: gjk_group, gjk_blocks, sphere_block, verlet_block, cell_items, "
-                                 "verlet_waves, vertex_pack or epa_blocks out of range");
+                                 "verlet_waves, vertex_pack, epa_blocks or epa_fused out of range");
         return MWD_E_INVALID;
     }
     ws->cell_table = t->cell_pass == MWD_CELLS_TABLE;
@@ -513,6 +514,7 @@ MWD_API int mwd_set_tuning(mwd_workspace* ws, const mwd_tuning* t) {
     ws->tune.verlet_waves = t->verlet_waves;
     ws->tune.vertex_pack = t->vertex_pack;
     ws->tune.epa_blocks = t->epa_blocks;
+    ws->tune.epa_fused = t->epa_fused;
     return MWD_OK;
 }
 
@@ -613,20 +615,38 @@ static int fine_contact(mwd_workspace* ws, const mwd_state* st, double* const* t
         if (rc) return rc;
     }
     mwd_mark(ws, 6);
-    // a colliding pair claims contact slot c (counter 4) and writes
-    // contact_pair[c]: contacts are in no set order; the per-row sums key
-    // their records by pair index, so they are summed in candidate order
-    rc = mwd_launch_gjk_surv(A, B, ws->pair_cap, ws->collision, ws->surv, ctr + 10, ws->simplex,
-                             ws->sidx, ctr + 4, ws->contact_cap, ws->flags, s, mwd_kev(ws, MWD_KT_GJK, 0),
-                             mwd_kev(ws, MWD_KT_GJK, 1), ws->tune, ws->contact_pair);
-    if (rc) return rc;
-    mwd_mark(ws, 7);  // (no compaction stage: kept for the stage numbering)
-    mwd_mark(ws, 8);
     if (ws->hist_on && !tan && (rc = mwd_launch_hist_prep(ws->thist, ws->contact_cap, s))) return rc;
     if (ws->bkt_dirty) {  // the previous contacts were not consumed by a reduction
         if ((rc = mwd_fill_i32(ws->rec_cnt, 0, ws->n_rows, s)) || (rc = mwd_set_i64(ctr + 11, 0, s))) return rc;
     }
-    if (tan) {
+    // a colliding pair claims contact slot c (counter 4) and writes
+    // contact_pair[c]: contacts are in no set order; the per-row sums key
+    // their records by pair index, so they are summed in candidate order
+    bool fused = false;
+    if (!tan && ws->tune.epa_fused) {
+        rc = mwd_launch_gjk_epa(A, B, ws->pair_cap, ws->collision, ws->surv, ctr + 10, ws->simplex, ws->sidx,
+                                ctr + 4, ws->contact_cap, ws->contact_pair, ws->laws, ws->depth, ws->normal, ws->cpA,
+                                ws->cpB, contact_out(ws, false), ws->epa_status, ws->rec_cnt, ws->rec_bkt, ws->rec_ovf,
+                                ctr + 11, 2 * ws->contact_cap, ws->flags, s, mwd_kev(ws, MWD_KT_GJK, 0),
+                                mwd_kev(ws, MWD_KT_GJK, 1), ws->tune);
+        if (rc < 0) return rc;
+        fused = rc == 0;
+    }
+    if (!fused) {
+        rc = mwd_launch_gjk_surv(A, B, ws->pair_cap, ws->collision, ws->surv, ctr + 10, ws->simplex, ws->sidx,
+                                 ctr + 4, ws->contact_cap, ws->flags, s, mwd_kev(ws, MWD_KT_GJK, 0),
+                                 mwd_kev(ws, MWD_KT_GJK, 1), ws->tune, ws->contact_pair);
+        if (rc) return rc;
+    }
+    mwd_mark(ws, 7);  // (no compaction stage: kept for the stage numbering)
+    mwd_mark(ws, 8);
+    if (fused) {  // EPA ran inside the GJK launch: its timer pair spans nothing
+        hipEvent_t e0 = mwd_kev(ws, MWD_KT_EPA, 0);
+        if (e0 && (hipEventRecord(e0, s) != hipSuccess || hipEventRecord(mwd_kev(ws, MWD_KT_EPA, 1), s) != hipSuccess)) {
+            ws->timing_errors += 1;
+            (void)hipGetLastError();
+        }
+    } else if (tan) {
         const VState vs = mwd_vstate(st, tan);
         BodyTan bt;
         bt.com = vs.com;

@@ -395,6 +395,24 @@ __global__ void __launch_bounds__(1024) k_sphere(BodyView A, BodyView B, int64_t
 // (GroupSup); every lane of a group follows the same path on the same
 // values, and its first lane writes the results. K = 0 selects one lane per
 // pair with the vertices read from memory (bodies of more than 6G vertices).
+// GJK of candidate pair p by lane g of its group (K = 0: one lane, the
+// vertices read from memory).
+template <int G, int K>
+__device__ __forceinline__ int gjk_of(const BodyView& A, const BodyView& B, int64_t p, int g, d3* S, int* ia,
+                                      int* ib) {
+    const int64_t ra = view_row(A, p), rb = view_row(B, p);
+    const d3 ca = ld3(A.com + 3 * ra), cb = ld3(B.com + 3 * rb);
+    const Side sa{A.verts + (int64_t)A.vmax * 3 * ra, A.nverts[ra]};
+    const Side sb{B.verts + (int64_t)B.vmax * 3 * rb, B.nverts[rb]};
+    if constexpr (K > 0) {
+        GroupSup<G, K> sup;
+        sup.load(sa, sb, g);
+        return gjk_pair(sup, ca, cb, S, ia, ib);
+    } else {
+        return gjk_pair(SerialSup{sa, sb}, ca, cb, S, ia, ib);
+    }
+}
+
 template <int G, int K>
 __global__ void __launch_bounds__(256) k_gjk(BodyView A, BodyView B, int64_t n_cap, const int32_t* __restrict__ surv,
                                              const int64_t* n_surv, int32_t* __restrict__ collision,
@@ -409,20 +427,9 @@ __global__ void __launch_bounds__(256) k_gjk(BodyView A, BodyView B, int64_t n_c
     const int64_t stride = (int64_t)gridDim.x * blockDim.x / GG;
     for (int64_t s = t / GG; s < m; s += stride) {
         const int64_t p = surv[s];
-        const int64_t ra = view_row(A, p), rb = view_row(B, p);
-        const d3 ca = ld3(A.com + 3 * ra), cb = ld3(B.com + 3 * rb);
-        const Side sa{A.verts + (int64_t)A.vmax * 3 * ra, A.nverts[ra]};
-        const Side sb{B.verts + (int64_t)B.vmax * 3 * rb, B.nverts[rb]};
         d3 S[4];
         int ia[4] = {0, 0, 0, 0}, ib[4] = {0, 0, 0, 0};
-        int c;
-        if constexpr (K > 0) {
-            GroupSup<G, K> sup;
-            sup.load(sa, sb, g);
-            c = gjk_pair(sup, ca, cb, S, ia, ib);
-        } else {
-            c = gjk_pair(SerialSup{sa, sb}, ca, cb, S, ia, ib);
-        }
+        const int c = gjk_of<G, K>(A, B, p, g, S, ia, ib);
         if (g != 0) continue;
         collision[p] = c;
         if (c == 1) {
@@ -983,26 +990,174 @@ __device__ __forceinline__ Q4<T> bodyq(const double* p, const TArr<MWD_NT>& t, i
         return ldqT<T>(t, r);
 }
 
+// What a contact needs besides its pair and simplex: the law tables, the
+// inputs of the pair layout, and where the results go.
+struct ContactSink {
+    const mwd_law* laws2;
+    const mwd_law_tan* lawt2;
+    const double* dispt;
+    const int32_t* idA;
+    const int32_t* idB;
+    double* depth;
+    double* normal;
+    double* cpA;
+    double* cpB;
+    ContactOut co;
+    double* disp_out;
+    int32_t* epa_status;
+    uint32_t* rec_key;
+    int32_t* rec_val;
+    int32_t* rec_cnt;
+    int32_t* rec_bkt;
+    int32_t* rec_ovf;
+    int64_t* n_ovf;
+    int64_t ovf_cap;
+};
+
+// Contact c (pair p, GJK simplex sx): EPA by all 64 lanes of the wave, then
+// the law and the writes by its first lane.
+template <class T>
+__device__ __forceinline__ void contact_wave(int64_t c, int64_t p, const Simplex& sx, const BodyView& A,
+                                             const BodyView& B, const BodyTan& bt, const ContactSink& k, PolyW& P,
+                                             int lane, uint32_t& fl) {
+    const mwd_law* laws2 = k.laws2;
+    const mwd_law_tan* lawt2 = k.lawt2;
+    const double* dispt = k.dispt;
+    const int32_t* idA = k.idA;
+    const int32_t* idB = k.idB;
+    double* depth = k.depth;
+    double* normal = k.normal;
+    double* cpA = k.cpA;
+    double* cpB = k.cpB;
+    const ContactOut& co = k.co;
+    double* disp_out = k.disp_out;
+    int32_t* epa_status = k.epa_status;
+    uint32_t* rec_key = k.rec_key;
+    int32_t* rec_val = k.rec_val;
+    int32_t* rec_cnt = k.rec_cnt;
+    int32_t* rec_bkt = k.rec_bkt;
+    int32_t* rec_ovf = k.rec_ovf;
+    int64_t* n_ovf = k.n_ovf;
+    const int64_t ovf_cap = k.ovf_cap;
+    const int64_t ra = view_row(A, p), rb = view_row(B, p);
+    const Side sa{A.verts + (int64_t)A.vmax * 3 * ra, A.nverts[ra]};
+    const Side sb{B.verts + (int64_t)B.vmax * 3 * rb, B.nverts[rb]};
+    const EpaResult e = epa_wave(sx, sa, sb, P, lane);
+    if (lane != 0) return;
+    const int wall = (A.is_wall[ra] || B.is_wall[rb]) ? 1 : 0;
+    const int scene = A.scene ? A.scene[ra] : 0;
+    const LawT<T> L = law_of(laws2[2 * scene + wall], lawt2 ? lawt2 + 2 * scene + wall : nullptr, (T*)nullptr);
+    V3<T> dt_in = dispt ? mk3<T>(T(dispt[4 * p]), T(dispt[4 * p + 1]), T(dispt[4 * p + 2])) : zero3<T>();
+    int hist_t = -1;
+    int64_t hkey = 0;
+    if constexpr (std::is_same<T, double>::value) {
+        if (co.hist.keys[0]) {  // persistent history: last step's displacement of this pair
+            hist_t = *co.hist.idx;
+            hkey = hist_key(ra, rb);
+            dt_in = hist_find(co.hist, hist_t, hkey);
+        }
+    }
+    const double imA = (A.mass && !A.is_wall[ra]) ? 1.0 / A.mass[ra] : 0.0;
+    const double imB = (B.mass && !B.is_wall[rb]) ? 1.0 / B.mass[rb] : 0.0;
+    T dep;
+    V3<T> nrm, ca, cb;
+    if constexpr (std::is_same<T, double>::value) {
+        dep = e.depth;
+        nrm = e.n;
+        ca = e.cpA;
+        cb = e.cpB;
+    } else {
+        const FaceGeo<T> g = face_geometry<T>(bt.verts, (int64_t)A.vmax * ra, bt.verts, (int64_t)B.vmax * rb,
+                                              e.fa, e.fb, e.flip);
+        // A face EPA left as "outside" (or deleted) carries a placeholder
+        // distance instead of its plane's: keep EPA's values, without
+        // derivatives, whenever the re-evaluation does not reproduce them.
+        const bool same = g.depth.v == e.depth && g.n.x.v == e.n.x && g.n.y.v == e.n.y && g.n.z.v == e.n.z &&
+                          g.cpA.x.v == e.cpA.x && g.cpA.y.v == e.cpA.y && g.cpA.z.v == e.cpA.z &&
+                          g.cpB.x.v == e.cpB.x && g.cpB.y.v == e.cpB.y && g.cpB.z.v == e.cpB.z;
+        if (same) {
+            dep = g.depth;
+            nrm = g.n;
+            ca = g.cpA;
+            cb = g.cpB;
+        } else {
+            dep = T(e.depth);
+            nrm = mk3<T>(T(e.n.x), T(e.n.y), T(e.n.z));
+            ca = mk3<T>(T(e.cpA.x), T(e.cpA.y), T(e.cpA.z));
+            cb = mk3<T>(T(e.cpB.x), T(e.cpB.y), T(e.cpB.z));
+        }
+    }
+    const LawOut<T> f = force_law<T>(body3<T>(A.com, bt.com, ra), body3<T>(B.com, bt.com, rb),
+                                     body3<T>(A.vel, bt.vel, ra), body3<T>(B.vel, bt.vel, rb),
+                                     body3<T>(A.angvel, bt.angvel, ra), body3<T>(B.angvel, bt.angvel, rb), dt_in,
+                                     bodyq<T>(A.quat, bt.quat, ra), A.isize[ra], B.isize[rb], imA, imB, dep, nrm,
+                                     ca, cb, L);
+    if constexpr (std::is_same<T, double>::value) {
+        if (co.face) {  // the final face, for the adjoint of this step
+            TArr<MWD_NT> va{}, vb{};
+            va.v = const_cast<double*>(A.verts);
+            vb.v = const_cast<double*>(B.verts);
+            const FaceGeo<double> g = face_geometry<double>(va, (int64_t)A.vmax * ra, vb, (int64_t)B.vmax * rb,
+                                                            e.fa, e.fb, e.flip);
+            const bool same = g.depth == e.depth && g.n.x == e.n.x && g.n.y == e.n.y && g.n.z == e.n.z &&
+                              g.cpA.x == e.cpA.x && g.cpA.y == e.cpA.y && g.cpA.z == e.cpA.z &&
+                              g.cpB.x == e.cpB.x && g.cpB.y == e.cpB.y && g.cpB.z == e.cpB.z;
+            int32_t* f = co.face + 7 * c;
+            for (int k = 0; k < 3; ++k) {
+                f[k] = e.fa[k];
+                f[3 + k] = e.fb[k];
+            }
+            f[6] = (e.flip ? 1 : 0) | (same ? 0 : 2);
+        }
+    }
+    depth[c] = e.depth;
+    st3(normal + 3 * c, e.n);
+    st3(cpA + 3 * c, e.cpA);
+    st3(cpB + 3 * c, e.cpB);
+    stv(co.Fn, c, f.Fn);
+    stv(co.tA, c, f.tA);
+    stv(co.tB, c, f.tB);
+    const int est = e.status | (e.depth < EPA_PLACEHOLDER ? 0 : 4);
+    if (epa_status) epa_status[c] = est;
+    if (est & 1) fl |= MWD_F_EPA_CAP;
+    if (est & 2) fl |= MWD_F_EPA_FACES;
+    if (est & 4) fl |= MWD_F_EPA_UNRESOLVED;
+    if (disp_out) {
+        double* dp = disp_out + 4 * p;
+        dp[0] = val(f.disp.x);
+        dp[1] = val(f.disp.y);
+        dp[2] = val(f.disp.z);
+        dp[3] = 1.0;
+    }
+    if (hist_t >= 0) {  // this step's displacement, for the next step
+        const double d4[4] = {val(f.disp.x), val(f.disp.y), val(f.disp.z), 1.0};
+        hist_put(co.hist, 1 - hist_t, hkey, d4);
+    }
+    if (rec_cnt) {
+        // buckets of the moving rows (walls do not integrate); a full
+        // bucket spills into the shared overflow list as (row, record).
+        // Records are keyed by pair index (2p + side), so a row's sum
+        // runs in candidate order whatever order the contacts came in
+        if (!A.is_wall[ra]) fl |= push_record(rec_cnt, rec_bkt, rec_ovf, n_ovf, ovf_cap, ra, (int32_t)(2 * p));
+        if (!B.is_wall[rb]) fl |= push_record(rec_cnt, rec_bkt, rec_ovf, n_ovf, ovf_cap, rb, (int32_t)(2 * p + 1));
+    } else {
+        rec_key[2 * c] = (uint32_t)(idA ? idA[p] : ra);
+        rec_val[2 * c] = (int32_t)(2 * c);
+        rec_key[2 * c + 1] = (uint32_t)(idB ? idB[p] : rb);
+        rec_val[2 * c + 1] = (int32_t)(2 * c + 1);
+    }
+}
+
 // EPA + law per contact, one contact per wave: EPA by the whole wave, then
 // the law by its first lane. T = double: the forward path (any pair layout).
 // T = Dual: the tangent path, where A and B index one state whose tangents bt
 // carries; EPA runs on values and the final face is re-evaluated in T.
 template <class T>
-__global__ void __launch_bounds__(256) k_epa_force(BodyView A, BodyView B, BodyTan bt,
-                                                   const mwd_law* __restrict__ laws2,
-                                                   const mwd_law_tan* __restrict__ lawt2,
+__global__ void __launch_bounds__(256) k_epa_force(BodyView A, BodyView B, BodyTan bt, ContactSink sink,
                                                    const int32_t* __restrict__ contact_pair,
                                                    const Simplex* __restrict__ simplex,
                                                    const int32_t* __restrict__ sidx, int64_t contact_cap,
-                                                   const int64_t* n_contacts, const double* __restrict__ dispt,
-                                                   const int32_t* __restrict__ idA, const int32_t* __restrict__ idB,
-                                                   double* __restrict__ depth, double* __restrict__ normal,
-                                                   double* __restrict__ cpA, double* __restrict__ cpB, ContactOut co,
-                                                   double* __restrict__ disp_out, int32_t* __restrict__ epa_status,
-                                                   uint32_t* __restrict__ rec_key, int32_t* __restrict__ rec_val,
-                                                   int64_t* n_records, int32_t* __restrict__ rec_cnt,
-                                                   int32_t* __restrict__ rec_bkt, int32_t* __restrict__ rec_ovf,
-                                                   int64_t* n_ovf, int64_t ovf_cap, uint32_t* flags) {
+                                                   const int64_t* n_contacts, int64_t* n_records, uint32_t* flags) {
     __shared__ PolyW polys[256 / MWD_WAVE];
     int64_t K = *n_contacts;
     if (K > contact_cap) K = contact_cap;
@@ -1015,117 +1170,85 @@ __global__ void __launch_bounds__(256) k_epa_force(BodyView A, BodyView B, BodyT
     uint32_t fl = 0;
     for (int64_t c = gid / MWD_WAVE; c < K; c += nw) {
         const int64_t p = contact_pair[c];
-        const int64_t ra = view_row(A, p), rb = view_row(B, p);
-        const Side sa{A.verts + (int64_t)A.vmax * 3 * ra, A.nverts[ra]};
-        const Side sb{B.verts + (int64_t)B.vmax * 3 * rb, B.nverts[rb]};
         const int32_t slot = sidx[p];
         if (slot < 0 || slot >= contact_cap) {  // simplex storage overflowed (flagged by k_gjk)
-            if (co.face && lane == 0) co.face[7 * c + 6] = 4;   // no face: nothing to reverse
+            if (sink.co.face && lane == 0) sink.co.face[7 * c + 6] = 4;   // no face: nothing to reverse
             continue;
         }
-        const EpaResult e = epa_wave(simplex[slot], sa, sb, P, lane);
-        if (lane != 0) continue;
-        const int wall = (A.is_wall[ra] || B.is_wall[rb]) ? 1 : 0;
-        const int scene = A.scene ? A.scene[ra] : 0;
-        const LawT<T> L = law_of(laws2[2 * scene + wall], lawt2 ? lawt2 + 2 * scene + wall : nullptr, (T*)nullptr);
-        V3<T> dt_in = dispt ? mk3<T>(T(dispt[4 * p]), T(dispt[4 * p + 1]), T(dispt[4 * p + 2])) : zero3<T>();
-        int hist_t = -1;
-        int64_t hkey = 0;
-        if constexpr (std::is_same<T, double>::value) {
-            if (co.hist.keys[0]) {  // persistent history: last step's displacement of this pair
-                hist_t = *co.hist.idx;
-                hkey = hist_key(ra, rb);
-                dt_in = hist_find(co.hist, hist_t, hkey);
-            }
-        }
-        const double imA = (A.mass && !A.is_wall[ra]) ? 1.0 / A.mass[ra] : 0.0;
-        const double imB = (B.mass && !B.is_wall[rb]) ? 1.0 / B.mass[rb] : 0.0;
-        T dep;
-        V3<T> nrm, ca, cb;
-        if constexpr (std::is_same<T, double>::value) {
-            dep = e.depth;
-            nrm = e.n;
-            ca = e.cpA;
-            cb = e.cpB;
-        } else {
-            const FaceGeo<T> g = face_geometry<T>(bt.verts, (int64_t)A.vmax * ra, bt.verts, (int64_t)B.vmax * rb,
-                                                  e.fa, e.fb, e.flip);
-            // A face EPA left as "outside" (or deleted) carries a placeholder
-            // distance instead of its plane's: keep EPA's values, without
-            // derivatives, whenever the re-evaluation does not reproduce them.
-            const bool same = g.depth.v == e.depth && g.n.x.v == e.n.x && g.n.y.v == e.n.y && g.n.z.v == e.n.z &&
-                              g.cpA.x.v == e.cpA.x && g.cpA.y.v == e.cpA.y && g.cpA.z.v == e.cpA.z &&
-                              g.cpB.x.v == e.cpB.x && g.cpB.y.v == e.cpB.y && g.cpB.z.v == e.cpB.z;
-            if (same) {
-                dep = g.depth;
-                nrm = g.n;
-                ca = g.cpA;
-                cb = g.cpB;
-            } else {
-                dep = T(e.depth);
-                nrm = mk3<T>(T(e.n.x), T(e.n.y), T(e.n.z));
-                ca = mk3<T>(T(e.cpA.x), T(e.cpA.y), T(e.cpA.z));
-                cb = mk3<T>(T(e.cpB.x), T(e.cpB.y), T(e.cpB.z));
-            }
-        }
-        const LawOut<T> f = force_law<T>(body3<T>(A.com, bt.com, ra), body3<T>(B.com, bt.com, rb),
-                                         body3<T>(A.vel, bt.vel, ra), body3<T>(B.vel, bt.vel, rb),
-                                         body3<T>(A.angvel, bt.angvel, ra), body3<T>(B.angvel, bt.angvel, rb), dt_in,
-                                         bodyq<T>(A.quat, bt.quat, ra), A.isize[ra], B.isize[rb], imA, imB, dep, nrm,
-                                         ca, cb, L);
-        if constexpr (std::is_same<T, double>::value) {
-            if (co.face) {  // the final face, for the adjoint of this step
-                TArr<MWD_NT> va{}, vb{};
-                va.v = const_cast<double*>(A.verts);
-                vb.v = const_cast<double*>(B.verts);
-                const FaceGeo<double> g = face_geometry<double>(va, (int64_t)A.vmax * ra, vb, (int64_t)B.vmax * rb,
-                                                                e.fa, e.fb, e.flip);
-                const bool same = g.depth == e.depth && g.n.x == e.n.x && g.n.y == e.n.y && g.n.z == e.n.z &&
-                                  g.cpA.x == e.cpA.x && g.cpA.y == e.cpA.y && g.cpA.z == e.cpA.z &&
-                                  g.cpB.x == e.cpB.x && g.cpB.y == e.cpB.y && g.cpB.z == e.cpB.z;
-                int32_t* f = co.face + 7 * c;
-                for (int k = 0; k < 3; ++k) {
-                    f[k] = e.fa[k];
-                    f[3 + k] = e.fb[k];
+        contact_wave<T>(c, p, simplex[slot], A, B, bt, sink, P, lane, fl);
+    }
+    if (fl) atomicOr(flags, fl);
+}
+
+// GJK and, for the pairs it finds colliding, EPA and the law in the same
+// wave (the plain FP64 step, grouped support): each round the wave runs GJK
+// on 64 / G survivors, the group leaders of the colliding ones claim contact
+// slots and park their simplices in LDS, and then the whole wave runs EPA
+// and the law on each of them in lane order. Results are those of k_gjk
+// followed by k_epa_force<double> (records are keyed by pair index, so the
+// per-row sums do not depend on which wave ran a contact).
+template <int G, int K>
+__global__ void __launch_bounds__(256) k_gjk_epa(BodyView A, BodyView B, BodyTan bt, int64_t n_cap,
+                                                 const int32_t* __restrict__ surv, const int64_t* n_surv,
+                                                 int32_t* __restrict__ collision, Simplex* __restrict__ simplex,
+                                                 int32_t* __restrict__ sidx, int64_t* slots, int64_t slot_cap,
+                                                 uint32_t* flags, int32_t* __restrict__ contact_pair,
+                                                 ContactSink sink) {
+    static_assert(K > 0 && MWD_WAVE % G == 0, "grouped support only");
+    constexpr int PW = MWD_WAVE / G;  // pairs per wave and round
+    __shared__ PolyW polys[256 / MWD_WAVE];
+    __shared__ Simplex parked[256 / MWD_WAVE][PW];
+    int64_t m = *n_surv;
+    if (m > n_cap) m = n_cap;
+    const int lane = threadIdx.x & (MWD_WAVE - 1);
+    const int w = threadIdx.x / MWD_WAVE;
+    const int g = lane % G;
+    PolyW& P = polys[w];
+    const int64_t stride = (int64_t)gridDim.x * (blockDim.x / G);
+    uint32_t fl = 0;
+    // s0, the wave's first survivor of a round, is the same on every lane, so
+    // the whole wave reaches each ballot and each EPA
+    for (int64_t s0 = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x - lane)) / G; s0 < m; s0 += stride) {
+        const int64_t s = s0 + lane / G;
+        int32_t p = 0;
+        int64_t slot = -1;
+        if (s < m) {
+            p = surv[s];
+            d3 S[4];
+            int ia[4] = {0, 0, 0, 0}, ib[4] = {0, 0, 0, 0};
+            const int c = gjk_of<G, K>(A, B, p, g, S, ia, ib);
+            if (g == 0) {
+                collision[p] = c;
+                if (c == 1) {
+                    slot = (int64_t)atomicAdd((unsigned long long*)slots, 1ull);
+                    sidx[p] = (int32_t)slot;
+                    if (slot >= slot_cap) {
+                        fl |= MWD_F_CONTACT_OVERFLOW;
+                    } else {
+                        contact_pair[slot] = p;
+                        Simplex& x = simplex[slot];
+                        Simplex& y = parked[w][lane / G];
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            x.s[k][0] = y.s[k][0] = S[k].x;
+                            x.s[k][1] = y.s[k][1] = S[k].y;
+                            x.s[k][2] = y.s[k][2] = S[k].z;
+                            x.ia[k] = y.ia[k] = ia[k];
+                            x.ib[k] = y.ib[k] = ib[k];
+                        }
+                    }
                 }
-                f[6] = (e.flip ? 1 : 0) | (same ? 0 : 2);
             }
         }
-        depth[c] = e.depth;
-        st3(normal + 3 * c, e.n);
-        st3(cpA + 3 * c, e.cpA);
-        st3(cpB + 3 * c, e.cpB);
-        stv(co.Fn, c, f.Fn);
-        stv(co.tA, c, f.tA);
-        stv(co.tB, c, f.tB);
-        const int est = e.status | (e.depth < EPA_PLACEHOLDER ? 0 : 4);
-        if (epa_status) epa_status[c] = est;
-        if (est & 1) fl |= MWD_F_EPA_CAP;
-        if (est & 2) fl |= MWD_F_EPA_FACES;
-        if (est & 4) fl |= MWD_F_EPA_UNRESOLVED;
-        if (disp_out) {
-            double* dp = disp_out + 4 * p;
-            dp[0] = val(f.disp.x);
-            dp[1] = val(f.disp.y);
-            dp[2] = val(f.disp.z);
-            dp[3] = 1.0;
-        }
-        if (hist_t >= 0) {  // this step's displacement, for the next step
-            const double d4[4] = {val(f.disp.x), val(f.disp.y), val(f.disp.z), 1.0};
-            hist_put(co.hist, 1 - hist_t, hkey, d4);
-        }
-        if (rec_cnt) {
-            // buckets of the moving rows (walls do not integrate); a full
-            // bucket spills into the shared overflow list as (row, record).
-            // Records are keyed by pair index (2p + side), so a row's sum
-            // runs in candidate order whatever order the contacts came in
-            if (!A.is_wall[ra]) fl |= push_record(rec_cnt, rec_bkt, rec_ovf, n_ovf, ovf_cap, ra, (int32_t)(2 * p));
-            if (!B.is_wall[rb]) fl |= push_record(rec_cnt, rec_bkt, rec_ovf, n_ovf, ovf_cap, rb, (int32_t)(2 * p + 1));
-        } else {
-            rec_key[2 * c] = (uint32_t)(idA ? idA[p] : ra);
-            rec_val[2 * c] = (int32_t)(2 * c);
-            rec_key[2 * c + 1] = (uint32_t)(idB ? idB[p] : rb);
-            rec_val[2 * c + 1] = (int32_t)(2 * c + 1);
+        uint64_t todo = __ballot(slot >= 0 && slot < slot_cap);
+        if (todo) wave_sync();  // the parked simplices before any lane reads them
+        while (todo) {
+            const int L = __ffsll((unsigned long long)todo) - 1;
+            todo &= todo - 1;
+            const int32_t pl = __shfl(p, L, MWD_WAVE);
+            const int32_t cl = __shfl((int32_t)slot, L, MWD_WAVE);
+            contact_wave<double>(cl, pl, parked[w][L / G], A, B, bt, sink, P, lane, fl);
+            wave_sync();  // this contact's polytope before the next one overwrites it
         }
     }
     if (fl) atomicOr(flags, fl);
@@ -1346,10 +1469,10 @@ int mwd_launch_epa_force(const BodyView& A, const BodyView& B, const mwd_law* la
     co.Fn = plain(Fn);
     co.tA = plain(tA);
     co.tB = plain(tB);
-    hipLaunchKernelGGL(k_epa_force<double>, dim3(grid_for(contact_cap * MWD_WAVE, 256, EPA_MAX_BLOCKS)), dim3(256), 0, s, A, B, bt, laws2,
-                       (const mwd_law_tan*)nullptr, contact_pair, simplex, sidx, contact_cap, n_contacts, dispt, idA,
-                       idB, depth, normal, cpA, cpB, co, disp_out, epa_status, rec_key, rec_val, n_records,
-                       (int32_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr, (int64_t*)nullptr, (int64_t)0, flags);
+    const ContactSink sink{laws2,    nullptr,    dispt,   idA,     idB,     depth,   normal,  cpA,     cpB,   co,
+                           disp_out, epa_status, rec_key, rec_val, nullptr, nullptr, nullptr, nullptr, 0};
+    hipLaunchKernelGGL(k_epa_force<double>, dim3(grid_for(contact_cap * MWD_WAVE, 256, EPA_MAX_BLOCKS)), dim3(256),
+                       0, s, A, B, bt, sink, contact_pair, simplex, sidx, contact_cap, n_contacts, n_records, flags);
     MWD_LAUNCH_CHECK();
     return MWD_OK;
 }
@@ -1363,23 +1486,83 @@ int mwd_launch_epa_force_rows(const BodyView& A, const BodyView& B, const BodyTa
                               int64_t max_blocks) {
     if (contact_cap <= 0) return MWD_OK;
     const dim3 grid(grid_for(contact_cap * MWD_WAVE, 256, max_blocks)), blk(256);
+    const ContactSink sink{laws2,   bt ? lawt2 : nullptr, nullptr, nullptr, nullptr, depth,   normal,  cpA,   cpB, co,
+                           nullptr, epa_status,           nullptr, nullptr, rec_cnt, rec_bkt, rec_ovf, n_ovf, ovf_cap};
     if (bt) {
-        MWD_LAUNCH_EV(e0, e1, k_epa_force<Dual<MWD_NT>>, grid, blk, 0, s, A, B, *bt, laws2, lawt2, contact_pair, simplex,
-                           sidx, contact_cap, n_contacts, (const double*)nullptr, (const int32_t*)nullptr,
-                           (const int32_t*)nullptr, depth, normal, cpA, cpB, co, (double*)nullptr, epa_status,
-                           (uint32_t*)nullptr, (int32_t*)nullptr, (int64_t*)nullptr, rec_cnt, rec_bkt, rec_ovf, n_ovf,
-                           ovf_cap, flags);
+        MWD_LAUNCH_EV(e0, e1, k_epa_force<Dual<MWD_NT>>, grid, blk, 0, s, A, B, *bt, sink, contact_pair, simplex,
+                      sidx, contact_cap, n_contacts, (int64_t*)nullptr, flags);
     } else {
         BodyTan none;
         memset(&none, 0, sizeof(none));
-        MWD_LAUNCH_EV(e0, e1, k_epa_force<double>, grid, blk, 0, s, A, B, none, laws2, (const mwd_law_tan*)nullptr,
-                           contact_pair, simplex, sidx, contact_cap, n_contacts, (const double*)nullptr,
-                           (const int32_t*)nullptr, (const int32_t*)nullptr, depth, normal, cpA, cpB, co,
-                           (double*)nullptr, epa_status, (uint32_t*)nullptr, (int32_t*)nullptr, (int64_t*)nullptr,
-                           rec_cnt, rec_bkt, rec_ovf, n_ovf, ovf_cap, flags);
+        MWD_LAUNCH_EV(e0, e1, k_epa_force<double>, grid, blk, 0, s, A, B, none, sink, contact_pair, simplex, sidx,
+                      contact_cap, n_contacts, (int64_t*)nullptr, flags);
     }
     MWD_LAUNCH_CHECK();
     return MWD_OK;
+}
+
+namespace {
+
+template <int G, int K>
+void launch_gjk_epa_k(int k, dim3 grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const BodyView& A,
+                      const BodyView& B, const BodyTan& bt, int64_t n_cap, const int32_t* surv,
+                      const int64_t* n_surv, int32_t* collision, Simplex* simplex, int32_t* sidx, int64_t* slots,
+                      int64_t slot_cap, uint32_t* flags, int32_t* contact_pair, const ContactSink& sink) {
+    if constexpr (K <= 4) {
+        if (k == K)
+            MWD_LAUNCH_EV(e0, e1, (k_gjk_epa<G, K>), grid, dim3(256), 0, s, A, B, bt, n_cap, surv, n_surv,
+                          collision, simplex, sidx, slots, slot_cap, flags, contact_pair, sink);
+        else
+            launch_gjk_epa_k<G, K + 1>(k, grid, s, e0, e1, A, B, bt, n_cap, surv, n_surv, collision, simplex, sidx,
+                                       slots, slot_cap, flags, contact_pair, sink);
+    }
+}
+
+template <int G>
+int launch_gjk_epa_g(const BodyView& A, const BodyView& B, int64_t n_cap, const int32_t* surv, const int64_t* n_surv,
+                     int32_t* collision, Simplex* simplex, int32_t* sidx, int64_t* slots, int64_t slot_cap,
+                     uint32_t* flags, int32_t* contact_pair, const ContactSink& sink, hipStream_t s, hipEvent_t e0,
+                     hipEvent_t e1, int max_blocks) {
+    // beyond 4 vertices per lane the fused kernel's registers allow one wave
+    // per SIMD: the two launches are faster there
+    const int K = (A.vmax + G - 1) / G;
+    if (K < 1 || K > 4) return 1;
+    BodyTan none;
+    memset(&none, 0, sizeof(none));
+    launch_gjk_epa_k<G, 1>(K, dim3(grid_for(n_cap * G, 256, max_blocks)), s, e0, e1, A, B, none, n_cap, surv, n_surv,
+                           collision, simplex, sidx, slots, slot_cap, flags, contact_pair, sink);
+    return 0;
+}
+
+}  // namespace
+
+int mwd_launch_gjk_epa(const BodyView& A, const BodyView& B, int64_t n_cap, int32_t* collision, const int32_t* surv,
+                       const int64_t* n_surv, Simplex* simplex, int32_t* sidx, int64_t* slots, int64_t slot_cap,
+                       int32_t* contact_pair, const mwd_law* laws2, double* depth, double* normal, double* cpA,
+                       double* cpB, const ContactOut& co, int32_t* epa_status, int32_t* rec_cnt, int32_t* rec_bkt,
+                       int32_t* rec_ovf, int64_t* n_ovf, int64_t ovf_cap, uint32_t* flags, hipStream_t s,
+                       hipEvent_t e0, hipEvent_t e1, const PairTune& tu) {
+    if (A.vmax != B.vmax || !contact_pair || !rec_cnt) return 1;
+    if (n_cap <= 0) return 0;
+    const ContactSink sink{laws2,   nullptr, nullptr, nullptr, nullptr, depth,   normal,  cpA,   cpB,    co,
+                           nullptr, epa_status, nullptr, nullptr, rec_cnt, rec_bkt, rec_ovf, n_ovf, ovf_cap};
+    int r;
+    switch (tu.gjk_group) {
+        case 4:
+            r = launch_gjk_epa_g<4>(A, B, n_cap, surv, n_surv, collision, simplex, sidx, slots, slot_cap, flags,
+                                    contact_pair, sink, s, e0, e1, tu.gjk_blocks);
+            break;
+        case 16:
+            r = launch_gjk_epa_g<16>(A, B, n_cap, surv, n_surv, collision, simplex, sidx, slots, slot_cap, flags,
+                                     contact_pair, sink, s, e0, e1, tu.gjk_blocks);
+            break;
+        default:
+            r = launch_gjk_epa_g<8>(A, B, n_cap, surv, n_surv, collision, simplex, sidx, slots, slot_cap, flags,
+                                    contact_pair, sink, s, e0, e1, tu.gjk_blocks);
+    }
+    if (r) return r;
+    MWD_LAUNCH_CHECK();
+    return 0;
 }
 
 int mwd_launch_segments(const uint32_t* rec_key, int64_t rec_cap, const int64_t* n_rec, int32_t* seg_begin,

@@ -131,6 +131,7 @@ struct PairTune {
     int verlet_waves = 1;    // minimum waves per SIMD asked of the reduction + integrator's registers: 1, 3 or 4
     int vertex_pack = 0;     // vertex pass over real vertices only (1) or over every vmax slot (0)
     int epa_blocks = 2048;   // grid cap of the EPA launch (blocks of 4 waves, one contact per wave)
+    int epa_fused = 1;       // EPA and the law inside the GJK launch (1) or as their own launch (0)
 };
 
 // Face distances at or above this are EPA's placeholders (1e30 for a face
@@ -414,6 +415,17 @@ int mwd_launch_epa_force(const BodyView& A, const BodyView& B, const mwd_law* la
 // rows only) instead of a list; bt (tangent path) carries the state's
 // tangents, lawt2 the law derivatives per scene like laws2, and co receives
 // values (and tangents) of Fn, tA, tB.
+// GJK, EPA and the law in one launch, for the plain FP64 step with bucketed
+// records: the results of mwd_launch_gjk_surv followed by
+// mwd_launch_epa_force_rows. Returns 1, launching nothing, when the bodies
+// have more than 4 * gjk_group vertices or the arguments do not fit this
+// form; a negative code on a launch error.
+int mwd_launch_gjk_epa(const BodyView& A, const BodyView& B, int64_t n_cap, int32_t* collision, const int32_t* surv,
+                       const int64_t* n_surv, Simplex* simplex, int32_t* sidx, int64_t* slots, int64_t slot_cap,
+                       int32_t* contact_pair, const mwd_law* laws2, double* depth, double* normal, double* cpA,
+                       double* cpB, const ContactOut& co, int32_t* epa_status, int32_t* rec_cnt, int32_t* rec_bkt,
+                       int32_t* rec_ovf, int64_t* n_ovf, int64_t ovf_cap, uint32_t* flags, hipStream_t s,
+                       hipEvent_t e0, hipEvent_t e1, const PairTune& tu);
 int mwd_launch_epa_force_rows(const BodyView& A, const BodyView& B, const BodyTan* bt, const mwd_law* laws2,
                               const mwd_law_tan* lawt2, const int32_t* contact_pair, const Simplex* simplex,
                               const int32_t* sidx, int64_t contact_cap, const int64_t* n_contacts, double* depth,

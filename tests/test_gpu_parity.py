@@ -198,7 +198,8 @@ def test_tuning_choices_change_nothing(gpu, dense_phys):
                 dict(cell_items=0), dict(cell_items=2), dict(cell_items=4),
                 dict(cell_pass=_lib.CELLS_SORTED, gjk_group=16), dict(verlet_waves=3), dict(verlet_waves=4),
                 dict(vertex_pack=1), dict(vertex_pack=1, verlet_block=64, verlet_waves=4), dict(epa_blocks=16),
-                dict(epa_blocks=128)]
+                dict(epa_blocks=128), dict(epa_fused=0), dict(epa_fused=0, gjk_group=16),
+                dict(gjk_group=16, gjk_blocks=64)]
     for v in variants:
         S = _stepper(sc)
         S.set_tuning(**v)
@@ -212,7 +213,7 @@ def test_tuning_choices_change_nothing(gpu, dense_phys):
     S = _stepper(sc)
     for bad in (dict(gjk_group=5), dict(gjk_blocks=8), dict(sphere_block=100), dict(cell_pass=7),
                 dict(verlet_block=96), dict(cell_items=3), dict(verlet_waves=2), dict(vertex_pack=2),
-                dict(epa_blocks=8)):
+                dict(epa_blocks=8), dict(epa_fused=2)):
         with pytest.raises(_lib.MwdError):
             S.set_tuning(**bad)
 
