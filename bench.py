@@ -265,17 +265,27 @@ def bench_polydisperse(steps: int, graph_steps: int, v_rms: float = 1000.0, tune
     v[:W] = sc.wall_v[:W]
     out, com, speed = {}, {}, {}
     tuning = None
+    if tune:
+        # launch choices timed half-way through the run, once the bed has
+        # lost its lattice order and the contacts have formed (on the first
+        # steps there are none, and the cell-pass and EPA choices timed there
+        # do not hold later), then used for both timed runs from the start
+        from mwdiem_amd.tune import SPACE, autotune
+        S = Stepper(sc)
+        S.set_state({"v": v, "v12": v})
+        S.step(max(2, steps // 2))
+        tuning = autotune(S, steps=5, space=dict(SPACE, epa_fused=(0, 1)))["tuning"]
+        print(f"# polydisperse autotune (after {max(2, steps // 2)} steps, {S.status()['n_contacts']} contacts): "
+              f"{tuning}", file=sys.stderr, flush=True)
+        S.close()
+        del S
+        torch.cuda.empty_cache()
     for fp32 in (False, True):
         S = Stepper(sc, reduce_fp32=fp32)
         S.set_state({"v": v, "v12": v})
         S.step(2)
-        if tune:
-            if tuning is None:  # choices timed on the FP64 run, used for both
-                from mwdiem_amd.tune import autotune
-                tuning = autotune(S, steps=5)["tuning"]
-                print(f"# polydisperse autotune: {tuning}", file=sys.stderr, flush=True)
-            else:
-                S.set_tuning(**tuning)
+        if tuning is not None:
+            S.set_tuning(**tuning)
         S.capture(graph_steps)
         torch.cuda.synchronize()
         t0 = time.perf_counter()

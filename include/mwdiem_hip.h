@@ -296,8 +296,10 @@ int mwd_stage_times(mwd_workspace* ws, double* ms, int64_t* steps);
  * steps, of the MWD_N_KTIMED single-kernel stages (pair emission,
  * bounding-sphere test, GJK, EPA and law, force assembly and Verlet): with
  * timing on they are launched with start/stop events that the runtime stamps
- * at the kernel's own start and end, so launch gaps are excluded. Call before
- * mwd_stage_times (which resets both). */
+ * at the kernel's own start and end, so launch gaps are excluded. When EPA
+ * and the law run inside the GJK launch (epa_fused), the GJK time holds them
+ * and the EPA time of that step is 0. Call before mwd_stage_times (which
+ * resets both). */
 #define MWD_N_KTIMED 5
 int mwd_kernel_times(mwd_workspace* ws, double* ms, int64_t* steps);
 

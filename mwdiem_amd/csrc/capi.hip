@@ -116,7 +116,7 @@ static bool timing_active(mwd_workspace* ws) {
 
 hipEvent_t mwd_kev(mwd_workspace* ws, int kt, int stop) {
     if (!timing_active(ws)) return nullptr;
-    ws->kt_used |= 1u << kt;
+    ws->kt_slot[ws->ring_slot] |= 1u << kt;
     return ev_at(ws, ws->ring_slot, MWD_N_STAGES + 1 + 2 * kt + (stop ? 1 : 0));
 }
 
@@ -132,7 +132,7 @@ static void flush_timing(mwd_workspace* ws) {
                 ws->timing_errors += 1;
         }
         for (int k = 0; k < MWD_N_KTIMED; ++k) {
-            if (!(ws->kt_used & (1u << k))) continue;
+            if (!(ws->kt_slot[sl] & (1u << k))) continue;
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, ev_at(ws, sl, MWD_N_STAGES + 1 + 2 * k),
                                     ev_at(ws, sl, MWD_N_STAGES + 2 + 2 * k)) == hipSuccess)
@@ -140,6 +140,7 @@ static void flush_timing(mwd_workspace* ws) {
             else
                 ws->timing_errors += 1;
         }
+        ws->kt_slot[sl] = 0;
         ws->stage_calls += 1;
         ws->kernel_calls += 1;
     }
@@ -183,7 +184,7 @@ MWD_API int mwd_set_timing(mwd_workspace* ws, int enable) {
     ws->timing = enable ? 1 : 0;
     ws->ring_slot = 0;
     ws->timing_errors = 0;
-    ws->kt_used = 0;
+    for (int k = 0; k < MWD_TIMING_RING; ++k) ws->kt_slot[k] = 0;
     for (int k = 0; k < MWD_N_STAGES; ++k) ws->stage_ms[k] = 0.0;
     for (int k = 0; k < MWD_N_KTIMED; ++k) ws->kernel_ms[k] = 0.0;
     ws->stage_calls = 0;
@@ -640,12 +641,8 @@ static int fine_contact(mwd_workspace* ws, const mwd_state* st, double* const* t
     }
     mwd_mark(ws, 7);  // (no compaction stage: kept for the stage numbering)
     mwd_mark(ws, 8);
-    if (fused) {  // EPA ran inside the GJK launch: its timer pair spans nothing
-        hipEvent_t e0 = mwd_kev(ws, MWD_KT_EPA, 0);
-        if (e0 && (hipEventRecord(e0, s) != hipSuccess || hipEventRecord(mwd_kev(ws, MWD_KT_EPA, 1), s) != hipSuccess)) {
-            ws->timing_errors += 1;
-            (void)hipGetLastError();
-        }
+    if (fused) {
+        // EPA ran inside the GJK launch: no EPA kernel time for this step
     } else if (tan) {
         const VState vs = mwd_vstate(st, tan);
         BodyTan bt;

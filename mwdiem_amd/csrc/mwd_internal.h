@@ -314,7 +314,7 @@ struct mwd_workspace {
     int64_t timing_errors;     // event records or elapsed-time queries that failed
     double kernel_ms[MWD_N_KTIMED];
     int64_t kernel_calls;
-    uint32_t kt_used;          // timed kernels launched with events since timing was enabled
+    uint32_t kt_slot[MWD_TIMING_RING];  // per ring slot: the timed kernels launched with events in that step
     double stage_ms[MWD_N_STAGES];
     int64_t stage_calls;
 };

@@ -335,15 +335,24 @@ def test_reduction_with_more_contacts_than_a_bucket(gpu):
 def test_timing_mode_changes_nothing(gpu, dense_phys):
     """With timing on, the single-kernel stages are launched with start/stop
     events (hipExtLaunchKernelGGL): the results stay bit-identical and the
-    kernel times are positive."""
+    kernel times are positive. Steps with EPA inside the GJK launch add no
+    EPA time, also when they alternate with steps that launch it."""
     A = _stepper(dense_phys)
     B = _stepper(dense_phys)
     B.set_timing(True)
     A.step(30)
-    B.step(30)
+    B.step(15)
+    B.set_tuning(epa_fused=0)
+    B.step(10)
+    B.set_tuning(epa_fused=1)
+    B.step(5)
     kt = B.kernel_times()
     st = B.stage_times()
+    B.step(4)
+    kt_fused = B.kernel_times()
+    B.stage_times()
     B.set_timing(False)
+    A.step(4)
     A.check()
     B.check()
     a, b = A.numpy_state(), B.numpy_state()
@@ -353,6 +362,7 @@ def test_timing_mode_changes_nothing(gpu, dense_phys):
     # the sphere test runs inside the emission when bounding radii are kept
     assert kt["ms"]["k_sphere"] == 0.0
     assert all(v > 0 for k, v in kt["ms"].items() if k != "k_sphere"), kt
+    assert kt_fused["steps"] == 4 and kt_fused["ms"]["k_epa_force"] == 0.0 and kt_fused["ms"]["k_gjk"] > 0, kt_fused
 
 
 def _run_both(sc, st, steps, check_pairs=5):
