@@ -486,11 +486,13 @@ def test_batched_equals_separate(gpu, small_scene, dense_scene):
             assert np.array_equal(gb[k][rows], gs[k]), (i, k)
 
 
-def test_persistent_tangential_history_matches_oracle(gpu, dense_phys):
+@pytest.mark.parametrize("fused", [1, 0])
+def test_persistent_tangential_history_matches_oracle(gpu, dense_phys, fused):
     """Persistent tangential history (the history the driver means to keep,
     keyed min+max^2): 60 steps of the dense bed with random motion against
-    the oracle carrying the same history, within 1e-8; the history changes
-    the result (the reference's reset mode gives another state)."""
+    the oracle carrying the same history, within 1e-8, with EPA inside the
+    GJK launch and as its own; the history changes the result (the
+    reference's reset mode gives another state)."""
     from mwdiem_amd.stepper import Stepper
     sc = dense_phys
     st0 = random_motion(O.initial_state(sc), sc, 8, vscale=800.0, wscale=100.0)
@@ -498,6 +500,7 @@ def test_persistent_tangential_history_matches_oracle(gpu, dense_phys):
     hist = {}
     S = Stepper(sc, persistent_history=True)
     R = Stepper(sc)
+    S.set_tuning(epa_fused=fused)
     S.set_state(st0)
     R.set_state(st0)
     n_kept = 0
