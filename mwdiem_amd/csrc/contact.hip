@@ -1,5 +1,4 @@
-// Fine contact: GJK, EPA and the normal/tangential contact law (FP64), one
-// pair per lane.
+// Fine contact: GJK, EPA and the normal/tangential contact law (FP64).
 //
 // Follows the control flow of the reference's contact_detection kernel
 // (LIBRARIES/gpu_fine_contact_detection.cu:874-1322) and its helpers
@@ -10,11 +9,14 @@
 // point (cpA-cpB)/2, mu in the damping time, normal force only on the linear
 // momentum). Evaluated in FP64 with a fixed operator order.
 //
-// The stage is split in two kernels so that pairs which separate early do not
-// hold a wavefront hostage to EPA: k_gjk decides every candidate pair and
-// stores the final simplex of colliding ones; after compaction k_epa_force
-// runs EPA and the law over contacts only, writing per-contact force/torque
-// and two reduction records (row of A, row of B) per contact.
+// GJK runs G lanes per pair (k_gjk); a colliding pair claims a contact slot
+// and stores its final simplex. EPA and the law run one contact per wave,
+// all 64 lanes on one polytope in LDS, writing per-contact force/torque and
+// two reduction records (row of A, row of B) per contact: either in a launch
+// of their own over all contacts (k_epa_force), or in the GJK launch, each
+// wave on the contacts it found (k_gjk_epa), which saves a launch and a
+// grid-wide dependency per step but leaves the EPA work where the contacts
+// fell.
 #include <string.h>
 
 #include <type_traits>
