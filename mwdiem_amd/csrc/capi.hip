@@ -44,7 +44,7 @@ void free_ws(mwd_workspace* ws) {
     void* ptrs[] = {ws->scenes_dev, ws->cell_of, ws->plist, ws->wlist, ws->scene_of, ws->is_wall, ws->keys, ws->keys_alt, ws->vals,
                     ws->vals_alt, ws->cell_start, ws->cell_key, ws->cell_self, ws->cell_seg,
                     ws->row_first, ws->row_rank, ws->cell_cnt, ws->cell_off, ws->perm, ws->key_cnt,
-                    ws->key_min, ws->row_key, ws->sph,
+                    ws->row_key, ws->sph,
                     ws->pairs, ws->collision, ws->simplex, ws->sidx, ws->surv, ws->contact_pair,
                     ws->depth, ws->normal, ws->cpA, ws->cpB, ws->Fn, ws->tA, ws->tB, ws->epa_status, ws->rec_cnt,
                     ws->rec_bkt, ws->rec_ovf, ws->hist,
@@ -391,8 +391,8 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     ws->cell_table = ws->cell_table_ok;
     if (ws->cell_table_ok) {
         A(cell_of, ws->key_span_total);
-        A(key_cnt, ws->key_span_total);
-        A(key_min, ws->key_span_total);
+        A(key_cnt, 2 * ws->key_span_total);  // (count, lowest row) interleaved: one cache line per key
+        ws->key_min = ws->key_cnt + 1;
     }
     A(row_key, n_rows);
     A(sph, 4 * n_rows);
@@ -441,8 +441,7 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
 #undef H2D
     const size_t kbytes = sizeof(int32_t) * (size_t)ws->key_span_total;
     if ((ws->cell_table_ok && (hipMemsetAsync(ws->cell_of, 0xFF, kbytes, s) != hipSuccess ||
-                            hipMemsetAsync(ws->key_cnt, 0, kbytes, s) != hipSuccess ||
-                            hipMemsetAsync(ws->key_min, 0x7F, kbytes, s) != hipSuccess)) ||
+                            mwd_fill_key_table(ws->key_cnt, ws->key_span_total, s) != MWD_OK)) ||
         hipMemsetAsync(ws->row_key, 0xFF, sizeof(int32_t) * n_rows, s) != hipSuccess ||
         hipMemsetAsync(ws->counters, 0, sizeof(int64_t) * 16, s) != hipSuccess ||
         hipMemsetAsync(ws->rec_cnt, 0, sizeof(int32_t) * n_rows, s) != hipSuccess ||

@@ -81,7 +81,7 @@ __global__ void k_sort_keys(const double* __restrict__ com, const int64_t* __res
 
 // Key-table path, one thread per moving particle: its key, the member count
 // of that key and the key's lowest row.
-constexpr int32_t KEY_MIN_EMPTY = 0x7F7F7F7F;  // above every row; the byte-fill value of an empty table
+constexpr int32_t KEY_MIN_EMPTY = 0x7F7F7F7F;  // above every row: the mark of an empty key
 
 __global__ void k_key_count(const double* __restrict__ com, const int64_t* __restrict__ keys_in,
                             const int32_t* __restrict__ plist, int64_t M, const int32_t* __restrict__ scene_of,
@@ -94,8 +94,8 @@ __global__ void k_key_count(const double* __restrict__ com, const int64_t* __res
     if (sph) pack_sphere(sph, com, radius, row);
     const uint32_t k = global_key(com, keys_in, row, scenes[scene_of[row]], flags);
     row_key[row] = (int32_t)k;
-    atomicAdd(&key_cnt[k], 1);
-    atomicMin(&key_min[k], row);
+    atomicAdd(&key_cnt[2 * k], 1);
+    atomicMin(&key_min[2 * k], row);
 }
 
 // Key-table path: each particle takes a slot in its cell's member range. The
@@ -108,7 +108,7 @@ __global__ void k_key_scatter(const int32_t* __restrict__ plist, int64_t M, cons
     if (i >= M) return;
     const int32_t row = plist[i];
     const int32_t k = row_key[row];
-    const int32_t slot = atomicSub(&key_cnt[k], 1) - 1;
+    const int32_t slot = atomicSub(&key_cnt[2 * k], 1) - 1;
     members[cell_start[cell_of[k]] + slot] = row;
 }
 
@@ -572,8 +572,10 @@ struct CellTableIO {
     int64_t* n_cells;
     __device__ __forceinline__ int64_t load(int64_t r) const {
         const int32_t k = row_key[r];
-        if (k < 0 || key_min[k] != (int32_t)r) return 0;
-        return (int64_t(1) << 32) | (int64_t)key_cnt[k];
+        if (k < 0) return 0;
+        const int2 cm = *reinterpret_cast<const int2*>(key_cnt + 2 * (int64_t)k);  // (members, lowest row)
+        if (cm.y != (int32_t)r) return 0;
+        return (int64_t(1) << 32) | (int64_t)cm.x;
     }
     __device__ __forceinline__ void store(int64_t r, int64_t off, int64_t v) const {
         if (!v) return;
@@ -582,7 +584,7 @@ struct CellTableIO {
         cell_start[c] = (int32_t)(off & 0xffffffffll);
         cell_key[c] = (uint32_t)k;
         cell_of[k] = (int32_t)c;
-        key_min[k] = KEY_MIN_EMPTY;
+        key_min[2 * (int64_t)k] = KEY_MIN_EMPTY;
     }
     __device__ __forceinline__ void finish(int64_t total) const {
         *n_cells = total >> 32;
@@ -659,7 +661,19 @@ __global__ void k_zero_i64(int64_t* p, int n) {
 
 unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256 > 0 ? (n + 255) / 256 : 1); }
 
+__global__ void k_fill_key_table(int2* key_cm, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) key_cm[i] = make_int2(0, KEY_MIN_EMPTY);
+}
+
 }  // namespace
+
+int mwd_fill_key_table(int32_t* key_cm, int64_t n, hipStream_t s) {
+    if (n <= 0) return MWD_OK;
+    hipLaunchKernelGGL(k_fill_key_table, dim3(blocks_for(n)), dim3(256), 0, s, reinterpret_cast<int2*>(key_cm), n);
+    MWD_LAUNCH_CHECK();
+    return MWD_OK;
+}
 
 int mwd_launch_cell_keys(const double* com, int64_t n, const double* cell, int64_t ncx, int64_t ncy, int64_t* keys,
                          hipStream_t s) {

@@ -256,8 +256,8 @@ struct mwd_workspace {
     int cell_table_ok;         // its tables exist
     PairTune tune;
     int32_t* cell_of;          // [key_span_total] cell of a key; valid only where cell_key[cell_of[k]] == k
-    int32_t* key_cnt;          // [key_span_total] members per key, zero between steps
-    int32_t* key_min;          // [key_span_total] lowest row per key, INT32_MAX between steps
+    int32_t* key_cnt;          // [2 * key_span_total] per key k: members at 2k (zero between steps) and
+    int32_t* key_min;          // lowest row at 2k + 1 (KEY_MIN_EMPTY between steps); key_min = key_cnt + 1
     int32_t* row_key;          // [n_rows] key of each moving row this step, -1 for walls
     double* sph;               // [n_rows*4] (centre, bounding radius) of the moving rows, packed by the key
                                // pass when radii are kept, for the sphere test inside the emission
@@ -354,6 +354,8 @@ int mwd_radix_sort(uint32_t* keys, int32_t* vals, uint32_t* keys_alt, int32_t* v
                    int64_t* tmp, int64_t tmp_len, hipStream_t s, uint32_t** keys_out, int32_t** vals_out);
 int64_t mwd_sort_tiles(int64_t n_cap);
 int mwd_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
+// The empty key table: (0 members, no lowest row) for each of n keys.
+int mwd_fill_key_table(int32_t* key_cm, int64_t n, hipStream_t s);
 int mwd_fill2_i32(int32_t* p, int32_t* q, int32_t v, int64_t n, hipStream_t s);
 int mwd_set_i64(int64_t* p, int64_t v, hipStream_t s);
 
