@@ -559,3 +559,25 @@ def test_sphere_survivors_in_crowded_cells(gpu):
     assert stat["n_sphere_pass"] == W + cell_meet
     S.fine_contact()  # a second pass: k_sphere over the stored candidates
     assert S.check()["n_sphere_pass"] == int(meet(pairs).sum())
+
+
+@pytest.mark.parametrize("fused", [1, 0])
+def test_contact_overflow_is_reported(gpu, dense_phys, fused):
+    """More contacts than contact_cap: the contacts past the cap are dropped
+    without a write out of bounds and the step reports MWD_E_CAPACITY, with
+    EPA inside the GJK launch and as its own."""
+    from mwdiem_amd import _lib
+    sc = dense_phys
+    st = random_motion(O.initial_state(sc), sc, 5, vscale=2000.0, wscale=200.0)
+    ref = _stepper(sc)
+    ref.set_state(st)
+    ref.step(20)
+    need = ref.status()["n_contacts"]
+    assert need > 2
+    S = _stepper(sc, contact_cap=2)
+    S.set_tuning(epa_fused=fused)
+    S.set_state(st)
+    with pytest.raises(_lib.MwdError, match="contact_cap"):
+        for _ in range(20):
+            S.step(1)
+            S.check()
