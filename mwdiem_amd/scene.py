@@ -227,13 +227,28 @@ def _particle_chunk(args):
     return out
 
 
+def gpu_runtime_started() -> bool:
+    """True once this process has initialised the GPU runtime (torch's HIP
+    context, or the HIP library loaded by mwdiem_amd._lib). A process in that
+    state must not fork workers: the child inherits the runtime's locks and
+    device mappings without its threads."""
+    import sys
+    torch = sys.modules.get("torch")
+    if torch is not None and torch.cuda.is_initialized():
+        return True
+    lib = sys.modules.get("mwdiem_amd._lib")
+    return lib is not None and getattr(lib, "_lib", None) is not None
+
+
 def make_particles(n: int, seed: int, npts, axes, size, density, ref_inertia, consistent=False,
                    workers: int | None = None):
     """n particles in chunks of 2048, chunk c drawn from PCG64([seed, c]): the
-    result does not depend on the number of worker processes."""
+    result does not depend on the number of worker processes. Workers are
+    forked only while the GPU runtime is not started in this process (after
+    that the chunks are made in this process, with the same result)."""
     jobs = [(seed, c, min(_CHUNK, n - c * _CHUNK), tuple(npts), tuple(axes), tuple(size), density, ref_inertia,
              consistent) for c in range((n + _CHUNK - 1) // _CHUNK)]
-    if len(jobs) <= 1:
+    if len(jobs) <= 1 or gpu_runtime_started():
         return [p for j in jobs for p in _particle_chunk(j)]
     import multiprocessing as mp
     import os
