@@ -471,6 +471,7 @@ struct PairResult {
     int iters;
     int face[6];
     int gjk_iters;  // tetrahedron iterations of GJK (.cu:1104-1288)
+    int flat;       // CorrectFlat re-supported the fourth vertex (.cu:1119-1128)
     double depth, n[3], cpA[3], cpB[3], Fn[3], tA[3], tB[3], disp[4];
 };
 
@@ -523,6 +524,7 @@ PairResult contact_one(const Body<T>& A, const Body<T>& B, const PairIn<T>& in, 
         }
         if (hit) {
             if (correct_flat(S[0], S[1], S[2], S[3])) {
+                r.flat = 1;
                 d4 = cross(S[0], S[1]);
                 S[3] = support(A, B, d4, &ia[3], &ib[3]);
             }
@@ -979,6 +981,32 @@ ORC_EXPORT void orc_gjk_iters(int64_t P, const int32_t* pa, const int32_t* pb, i
         PairResult r = contact_one(A, B, in, L);
         collision[p] = r.collision;
         iters[p] = r.gjk_iters;
+    }
+}
+
+// GJK/EPA path of each pair of a list (test helper): collision code,
+// tetrahedron iterations, whether CorrectFlat fired, EPA status and EPA
+// expansions.
+ORC_EXPORT void orc_gjk_trace(int64_t P, const int32_t* pa, const int32_t* pb, int vmax, const double* com,
+                              const double* verts, const int32_t* nverts, int32_t* collision, int32_t* iters,
+                              int32_t* flat, int32_t* epa_status, int32_t* epa_iters) {
+    Law L = make_law(1.0, 0.5, 0.5, 0.5, 1.0);
+    for (int64_t p = 0; p < P; ++p) {
+        int a = pa[p], b = pb[p];
+        Body<double> A{verts + (int64_t)vmax * 3 * a, nverts[a]};
+        Body<double> B{verts + (int64_t)vmax * 3 * b, nverts[b]};
+        PairIn<double> in;
+        std::memset(&in, 0, sizeof(in));
+        in.comA = load3<double>(com + 3 * a);
+        in.comB = load3<double>(com + 3 * b);
+        in.qA.w = 1.0;
+        in.isA = 1.0; in.isB = 1.0;
+        PairResult r = contact_one(A, B, in, L);
+        collision[p] = r.collision;
+        iters[p] = r.gjk_iters;
+        flat[p] = r.flat;
+        epa_status[p] = r.epa_status;
+        epa_iters[p] = r.iters;
     }
 }
 

@@ -24,11 +24,14 @@ Parity status of each function (what pins it; see tests/test_oracle_*.py):
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
 
-LIB = Path(__file__).resolve().parent / "libmwdiem_oracle.so"
+# MWD_ORACLE_LIB: another build of the oracle (for checking that the pins
+# catch planted mistakes in a modified copy of the source)
+LIB = Path(os.environ.get("MWD_ORACLE_LIB") or Path(__file__).resolve().parent / "libmwdiem_oracle.so")
 _lib = None
 
 
@@ -205,6 +208,19 @@ def contact_faces(pairs, st, sc):
     lib().orc_contact_faces(C.c_int64(P), _p(pa), _p(pb), C.c_int(sc.vmax), _p(_f64(st["com"])),
                             _p(_f64(st["verts"])), _p(_i32(sc.nverts)), _p(coll), _p(face))
     return coll, face
+
+
+def gjk_trace(pairs, com, verts, nverts):
+    """Path of GJK/EPA for each pair of a list: (collision, tetrahedron
+    iterations, CorrectFlat fired (0/1), EPA status, EPA expansions)."""
+    pairs = _i32(pairs).reshape(-1, 2)
+    P = pairs.shape[0]
+    pa, pb = np.ascontiguousarray(pairs[:, 0]), np.ascontiguousarray(pairs[:, 1])
+    verts = _f64(verts)
+    out = [np.zeros(P, np.int32) for _ in range(5)]
+    lib().orc_gjk_trace(C.c_int64(P), _p(pa), _p(pb), C.c_int(verts.shape[1]), _p(_f64(com)), _p(verts),
+                        _p(_i32(nverts)), *[_p(x) for x in out])
+    return tuple(out)
 
 
 def count_flops(pairs, st, sc) -> int:

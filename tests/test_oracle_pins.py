@@ -219,8 +219,14 @@ def test_gjk_epa_against_minkowski_hull():
             true = float(np.min(-off))
             assert true / (1 + 1.1e-5) <= depth <= true * (1 + 1e-12), (trial, depth, true)
             assert abs(np.linalg.norm(n) - 1) < 1e-12
-            # cpA - cpB is the penetration vector along the normal (up to sign)
-            assert np.allclose(np.abs((cpA - cpB) @ n), depth, rtol=1e-6, atol=1e-9) or True
+            # the contact points realise the penetration: cpA - cpB is the
+            # Minkowski point depth*n; cpA lies on A's support plane along n,
+            # cpB on B's support plane along -n, each inside its own hull
+            assert np.allclose(cpA - cpB, depth * n, rtol=0, atol=1e-12)
+            assert abs(cpA @ n - (A @ n).max()) <= 1e-12 and abs(cpB @ n - (B @ n).min()) <= 1e-12
+            for P_, H in ((cpA, A), (cpB, B)):
+                eq = ConvexHull(H).equations
+                assert np.all(eq[:, :3] @ P_ + eq[:, 3] <= 1e-12), trial
     assert n_hit >= 20
 
 
@@ -423,3 +429,247 @@ def test_tangential_history_stick_then_slip():
     # without history (the reference as it runs) the same step gives a smaller tangential force
     _, _, _, _, _, F0, _, _ = O.contact_state(pairs, st, sc)
     assert np.hypot(F0[1, 1], F0[1, 2]) < np.hypot(F[1, 1], F[1, 2])
+
+
+# ------------------------------------------------------------ contact points, torques, tangential law
+def _tip_into_face(delta, tip=(0.3, -0.2)):
+    """A tetrahedron standing on its tip, pressed delta deep into the top
+    face (z = 0) of a wide box. The minimum translation is along z: EPA's
+    contact point on A is the tip, on B the tip's projection onto the face."""
+    tx, ty = tip
+    A = np.array([[tx, ty, -delta],
+                  [tx + 1.0, ty, 1.0 - delta],
+                  [tx - 0.5, ty + 0.866, 1.0 - delta],
+                  [tx - 0.5, ty - 0.866, 1.0 - delta]])
+    B = _box((-2.0, -2.0, -1.0), (2.0, 2.0, 0.0))
+    return A, B
+
+
+def _edge_on_face(delta, x0=-0.4, x1=0.7, y=0.15):
+    """A wedge whose lowest edge (x0..x1 at height -delta, y fixed) is pressed
+    into the same box face."""
+    A = np.array([[x0, y, -delta], [x1, y, -delta],
+                  [x0, y - 0.5, 0.8 - delta], [x1, y - 0.5, 0.8 - delta],
+                  [x0, y + 0.5, 0.8 - delta], [x1, y + 0.5, 0.8 - delta]])
+    B = _box((-2.0, -2.0, -1.0), (2.0, 2.0, 0.0))
+    return A, B
+
+
+@pytest.mark.parametrize("delta", [0.02, 0.1, 0.3])
+def test_contact_points_tip_into_face(delta):
+    A, B = _tip_into_face(delta)
+    c, depth, n, cpA, cpB, st = O.gjk_epa(A, B, A.mean(0), B.mean(0))
+    assert c == 1 and st == 0
+    assert abs(depth - delta) <= 1e-5 * delta
+    assert np.allclose(n, [0, 0, -1], atol=1e-12)
+    assert np.allclose(cpA, A[0], rtol=0, atol=1e-12)                 # the tip
+    assert np.allclose(cpB, [A[0, 0], A[0, 1], 0.0], rtol=0, atol=1e-12)  # its projection on the face
+
+
+@pytest.mark.parametrize("delta", [0.05, 0.2])
+def test_contact_points_edge_on_face(delta):
+    x0, x1, y = -0.4, 0.7, 0.15
+    A, B = _edge_on_face(delta, x0, x1, y)
+    c, depth, n, cpA, cpB, st = O.gjk_epa(A, B, A.mean(0), B.mean(0))
+    assert c == 1 and st == 0
+    assert abs(depth - delta) <= 1e-5 * delta
+    assert np.allclose(n, [0, 0, -1], atol=1e-12)
+    # on the edge segment, and straight above it on the face
+    assert abs(cpA[1] - y) < 1e-12 and abs(cpA[2] + delta) < 1e-12 and x0 - 1e-12 <= cpA[0] <= x1 + 1e-12
+    assert np.allclose(cpB, [cpA[0], cpA[1], 0.0], rtol=0, atol=1e-12)
+
+
+def _pair_scene(A, B, comA, comB, law, isA=1.0, isB=1.0, mA=1.0, mB=1.0):
+    """Two bodies as rows 1 and 2 of a scene (row 0: an unused wall) for
+    O.contact_state; state at rest, q = 1."""
+    from mwdiem_amd import scene
+    vmax = max(len(A), len(B))
+    sc = _Sc(2, I=(1.0, 1.0, 1.0), mass=1.0, vmax=vmax, mode=law.mode)
+    sc.verts = np.zeros((3, vmax, 3))
+    sc.verts[1, :len(A)] = A
+    sc.verts[2, :len(B)] = B
+    sc.nverts = np.array([vmax, len(A), len(B)], np.int32)
+    sc.com = np.array([[0.0, 0.0, 100.0], comA, comB], float)
+    sc.isize = np.array([1.0, isA, isB])
+    sc.mass = np.array([1.0, mA, mB])
+    sc.law_p = law
+    sc.law_w = scene.Law4(mode=law.mode)
+    return sc, _state(sc)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_contact_torque_closed_form(mode):
+    """Normal impact at rest off A's centre: F_A = Kn*delta pushing A away
+    from the face, torques (c - x) x F with the contact point c of each law:
+    (cpA - cpB)/2 in the reference law (Q3, .cu:677, a lever from near the
+    origin), (cpA + cpB)/2 in the physical one (.cu:815-821)."""
+    from mwdiem_amd import scene
+    delta, Kn = 0.05, 1e4
+    A, B = _tip_into_face(delta, tip=(0.3, -0.2))
+    comA, comB = np.array([1.1, 0.4, 0.9]), np.array([-0.3, 0.2, -0.6])
+    sc, st = _pair_scene(A, B, comA, comB, scene.Law4(Kn=Kn, en=0.5, et=0.5, mu=0.3, mode=mode), mA=2e-3, mB=3e-3)
+    coll, depth, nrm, ca, cb, F, T, est = O.contact_state(np.array([[1, 2]]), st, sc)
+    assert coll[0] == 1
+    cpA, cpB = A[0], np.array([A[0, 0], A[0, 1], 0.0])
+    d = depth[0]  # EPA's depth, within 1e-5 of delta
+    assert abs(d - delta) <= 1e-5 * delta
+    Fa = np.array([0.0, 0.0, Kn * d])
+    c = (cpA - cpB) / 2 if mode == 0 else (cpA + cpB) / 2
+    assert np.allclose(F[1], Fa, rtol=1e-12, atol=1e-9) and np.allclose(F[2], -Fa, rtol=1e-12, atol=1e-9)
+    assert np.allclose(T[1], np.cross(c - comA, Fa), rtol=1e-10, atol=1e-9)
+    assert np.allclose(T[2], np.cross(c - comB, -Fa), rtol=1e-10, atol=1e-9)
+    # the two laws' levers differ: the reference's contact point is near the origin
+    assert np.linalg.norm(c) < 0.05 if mode == 0 else np.linalg.norm(c - cpA) < 0.05
+
+
+def test_epa_face_inside_test_is_strict():
+    """AddTriangle keeps a face only if the origin's projection lies strictly
+    inside all three edges (.cu:300-318). With the tip straight above the
+    face's centre, the projection falls on the diagonal that splits the
+    Minkowski face into two triangles, both are set aside (distance 1e30),
+    and EPA ends on the far face: depth 2 - delta along +z instead of delta."""
+    delta = 0.05
+    A, B = _tip_into_face(delta, tip=(0.0, 0.0))
+    c, depth, n, cpA, cpB, st = O.gjk_epa(A, B, A.mean(0), B.mean(0))
+    assert c == 1 and st == 0
+    assert abs(depth - (2.0 - delta)) <= 1e-12 and np.allclose(n, [0, 0, 1], atol=1e-12)
+    A2, B2 = _tip_into_face(delta, tip=(1e-3, 0.0))
+    assert abs(O.gjk_epa(A2, B2, A2.mean(0), B2.mean(0))[1] - delta) <= 1e-5 * delta
+
+
+def _ref_tangential_scene(vA=(0.0, 0.0, 0.0)):
+    """The tip contact with hand-picked numbers for the reference law's
+    tangential coefficients (.cu:685-701): mu = 1/2, en = et = exp(-pi) (so
+    ln en = ln et = -pi and the two restitution factors are equal), the
+    contact point c = (cpA - cpB)/2 = (0, 0, -delta/2), centres 2 above and
+    3 below it and inertia sizes 8 and 18, so that
+    C = 1/mu + |c - xA|^2/IA + |c - xB|^2/IB = 2 + 4/8 + 9/18 = 3,
+    Kt = Kn (pi^2 + ln^2 et)/(pi^2 + ln^2 en) / C = Kn/3 and
+    Gt = -2 ln(et)/C = 2 pi/3."""
+    from mwdiem_amd import scene
+    delta, Kn = 0.05, 1e4
+    A, B = _tip_into_face(delta, tip=(0.3, -0.2))
+    c_ref = np.array([0.0, 0.0, -delta / 2])
+    comA, comB = c_ref + [0, 0, 2.0], c_ref - [0, 0, 3.0]
+    e = math.exp(-math.pi)
+    sc, st = _pair_scene(A, B, comA, comB, scene.Law4(Kn=Kn, en=e, et=e, mu=0.5, mode=0), isA=8.0, isB=18.0)
+    st["v"][1] = vA
+    return sc, st, Kn, delta
+
+
+def test_reference_law_tangential_damping_hand_computed():
+    # sliding along x at 10 mm/s: Ft = -Gt u_t = -(2 pi/3)*10, under the
+    # Coulomb cap mu*Kn*delta = 250; only Fn acts on the linear momentum (Q5),
+    # Ft enters the torques: tau_A = (0,0,-2) x (Ft,0,Fn) = (0, -2 Ft, 0),
+    # tau_B = (0,0,3) x -(Ft,0,Fn) = (0, -3 Ft, 0)
+    sc, st, Kn, delta = _ref_tangential_scene(vA=(10.0, 0.0, 0.0))
+    coll, depth, *_r, F, T, est = O.contact_state(np.array([[1, 2]]), st, sc)
+    assert coll[0] == 1
+    d = depth[0]
+    Ft = -(2 * math.pi / 3) * 10.0
+    assert np.allclose(F[1], [0, 0, Kn * d], rtol=1e-12, atol=1e-9)
+    assert np.allclose(T[1], [0, -2 * Ft, 0], rtol=1e-12, atol=1e-9)
+    assert np.allclose(T[2], [0, -3 * Ft, 0], rtol=1e-12, atol=1e-9)
+
+
+def test_reference_law_tangential_stiffness_hand_computed():
+    # at rest with a stored tangential displacement xi = 0.01 along x (the
+    # persistent history): Ft = -Kt xi = -(Kn/3)*0.01, under the cap; it sticks,
+    # so the stored displacement comes back as xi (q_A = 1, u_t = 0)
+    sc, st, Kn, delta = _ref_tangential_scene()
+    xi = 0.01
+    coll, depth, *_r, F, T, est, new = O.contact_state(np.array([[1, 2]]), st, sc,
+                                                       dispt=np.array([[xi, 0, 0, 0]]), with_disp=True)
+    Ft = -(Kn / 3) * xi
+    assert np.allclose(T[1], [0, -2 * Ft, 0], rtol=1e-12, atol=1e-9)
+    assert np.allclose(new[0, :3], [xi, 0, 0], rtol=1e-14, atol=0)
+
+
+def test_reference_law_coulomb_cap_and_slip_displacement():
+    # sliding at 1000 mm/s: Gt u_t = 2094 > mu Kn delta, so Ft = -mu |Fn| t;
+    # the stored slip displacement is the reference's (-mu|Fn| t - Gt u_t)/Kt
+    # (.cu:800-806 as written, the sign noted as Q19)
+    sc, st, Kn, delta = _ref_tangential_scene(vA=(1000.0, 0.0, 0.0))
+    coll, depth, *_r, F, T, est, new = O.contact_state(np.array([[1, 2]]), st, sc, with_disp=True)
+    d = depth[0]
+    Ft = -0.5 * Kn * d
+    assert np.allclose(T[1], [0, -2 * Ft, 0], rtol=1e-12, atol=1e-9)
+    slip = (-0.5 * Kn * d - (2 * math.pi / 3) * 1000.0) / (Kn / 3)
+    assert np.allclose(new[0, :3], [slip, 0, 0], rtol=1e-12, atol=0)
+
+
+# ------------------------------------------------------------ Verlet torque and gyroscopic path
+def _qmul(a, b):
+    w1, v1 = a[0], np.asarray(a[1:])
+    w2, v2 = b[0], np.asarray(b[1:])
+    return np.concatenate([[w1 * w2 - v1 @ v2], w1 * v2 + w2 * v1 + np.cross(v1, v2)])
+
+
+def _axis_quat(axis, angle):
+    axis = np.asarray(axis, float) / np.linalg.norm(axis)
+    return np.concatenate([[math.cos(angle / 2)], math.sin(angle / 2) * axis])
+
+
+def test_verlet_constant_torque_about_a_principal_axis():
+    """From rest, a constant world torque along the body's z axis (body turned
+    by q0, so body and world frames differ): alpha = tau/Iz, and the
+    staggered scheme gives w12bf = k alpha dt e_z, wbf = (k + 1/2) alpha dt e_z,
+    q12 = q0 (x) rot_z(alpha dt^2 (k^2-1)/2) and
+    q = q0 (x) rot_z(alpha dt^2 (k^2 + k - 3/4)/2) after k steps
+    (gpu_Verlet.cu:171-271: tau_bf = q^-1 tau q, alpha = I^-1 (tau_bf - w x I w),
+    the half-step turns use the previous step's world w)."""
+    dt, Iz, tau = 1e-3, 3.0, 0.6
+    sc = _Sc(1, dt=dt, I=(1.0, 2.0, Iz))
+    st = _state(sc)
+    q0 = _axis_quat([0.3, -0.5, 0.8], 1.1)
+    for k in ("q", "q12", "qall"):
+        st[k][1] = q0
+    R0 = _rotmat(q0)
+    T = np.zeros((2, 3))
+    T[1] = R0 @ np.array([0.0, 0.0, tau])
+    F = np.zeros((2, 3))
+    alpha = tau / Iz
+    for k in range(1, 41):
+        O.verlet(st, F, T, sc, 1, 2)
+        assert np.allclose(st["w12bf"][1], [0, 0, k * alpha * dt], rtol=1e-12, atol=1e-15)
+        assert np.allclose(st["wbf"][1], [0, 0, (k + 0.5) * alpha * dt], rtol=1e-12, atol=1e-15)
+        q12 = _qmul(q0, _axis_quat([0, 0, 1], alpha * dt * dt * (k * k - 1) / 2))
+        q = _qmul(q0, _axis_quat([0, 0, 1], alpha * dt * dt * (k * k + k - 0.75) / 2))
+        assert np.allclose(st["q12"][1], q12, rtol=0, atol=1e-14), k
+        assert np.allclose(st["q"][1], q, rtol=0, atol=1e-14), k
+        assert np.allclose(st["w"][1], R0 @ st["wbf"][1], rtol=1e-12, atol=1e-15)
+
+
+def test_verlet_torque_free_asymmetric_body_conserves_momentum_and_energy():
+    """A torque-free body with three different moments (full tensor, not
+    diagonal in the body frame) spinning off its principal axes: the world
+    angular momentum R(q) I w_bf and the energy w_bf.I w_bf / 2 stay constant
+    to the scheme's O(dt^2); without the gyroscopic term w x I w, or with its
+    sign or I and I^-1 swapped, the momentum turns or the energy drifts by
+    O(1)."""
+    rng = np.random.default_rng(2)
+    Rb = _rotmat(rng.normal(size=4))
+    I = Rb @ np.diag([1.0, 2.5, 4.0]) @ Rb.T
+    dt = 2e-4
+    sc = _Sc(1, dt=dt)
+    sc.I[1] = I
+    sc.Iinv[1] = np.linalg.inv(I)
+    st = _state(sc)
+    q0 = _axis_quat([1.0, 2.0, -0.5], 0.7)
+    wbf = np.array([3.0, -1.5, 2.0])
+    for k in ("q", "q12", "qall"):
+        st[k][1] = q0
+    st["wbf"][1] = st["w12bf"][1] = wbf
+    st["w"][1] = _rotmat(q0) @ wbf
+    F = np.zeros((2, 3))
+    L0 = _rotmat(q0) @ I @ wbf
+    E0 = 0.5 * wbf @ I @ wbf
+    for _ in range(10000):  # 2 s, about 8 turns
+        O.verlet(st, F, F, sc, 1, 2)
+    q, w = st["q"][1], st["wbf"][1]
+    L = _rotmat(q) @ I @ w
+    E = 0.5 * w @ I @ w
+    assert np.linalg.norm(L - L0) / np.linalg.norm(L0) < 1e-4
+    assert abs(E / E0 - 1) < 1e-4
+    # the body did turn, and not about a fixed body axis
+    assert np.linalg.norm(w - wbf) > 0.5
