@@ -121,8 +121,12 @@ def test_gpu_fp32_reduction_close_to_fp64(gpu):
 def test_gpu_full_polydisperse_configuration(gpu):
     """1,000,000 hulls, radii 0.5-2.5 mm, 6-32 points, 400 mm box, sliding
     ceiling: the first candidate list equals the oracle's, order included;
-    then 40 steps in FP64 and in FP32 reductions, which agree to 1e-4 of the
-    displacement."""
+    then the hulls are thrown about (5 m/s rms, so lattice neighbours meet
+    within the run) for 300 steps in FP64 and in FP32 reductions: both runs
+    see thousands of contacts, the FP32 sums change the result (the path is
+    taken), the median hull moves by less than 1e-9 of the displacement
+    between the two and fewer than 0.1 % of the hulls part by more than
+    1 um."""
     import time
     from mwdiem_amd import scene
     from mwdiem_amd.stepper import Stepper
@@ -137,14 +141,29 @@ def test_gpu_full_polydisperse_configuration(gpu):
     print(f"pairs {got.shape[0]} (oracle {ref.shape[0]}), cells {s0['n_cells']}", flush=True)
     assert got.shape == ref.shape and np.array_equal(got, ref)
     del ref, got
+    S.close()
+    W = sc.n_walls
+    rng = np.random.Generator(np.random.PCG64(3))
+    v = np.zeros((sc.n_rows, 3))
+    v[W:] = rng.normal(size=(sc.n_particles, 3)) * 5000.0
+    v[:W] = sc.wall_v[:W]
     out = {}
     for fp32 in (False, True):
         S = Stepper(sc, reduce_fp32=fp32)
-        S.step(40)
+        S.set_state({"v": v, "v12": v})
+        S.step(300)
         out[fp32] = (S.numpy_state()["com"], S.check())
         S.close()
-    W = sc.n_walls
+        print(f"fp32={fp32}: {out[fp32][1]}", flush=True)
     a, b = out[True][0][W:], out[False][0][W:]
     disp = np.abs(b - sc.com[W:]).max()
-    print(f"status {out[False][1]}; fp32 vs fp64 max |dx| {np.abs(a - b).max():.3e} of {disp:.3e}")
-    assert np.abs(a - b).max() <= 1e-4 * disp
+    dx = np.linalg.norm(a - b, axis=1)
+    q = np.quantile(dx, [0.5, 0.99, 0.999])
+    print(f"fp32 vs fp64 |dx|: median {q[0]:.3e}, 99% {q[1]:.3e}, 99.9% {q[2]:.3e}, max {dx.max():.3e}, "
+          f"above 1 um {np.mean(dx > 1e-3):.2e}; displacement {disp:.3e}")
+    assert out[False][1]["n_contacts"] > 1000 and out[True][1]["n_contacts"] > 1000
+    # the FP32 sums change the result (the path is taken); a colliding hull
+    # gas is chaotic, so the two runs are compared by the share of hulls
+    # that have parted company (a few collide violently enough to diverge)
+    assert dx.max() > 0.0
+    assert q[0] <= 1e-9 * disp and np.mean(dx > 1e-3) < 1e-3
