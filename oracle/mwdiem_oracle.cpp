@@ -35,6 +35,9 @@
 //
 // Compiled with -ffp-contract=off so every expression rounds as written.
 
+#include <omp.h>
+
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -51,7 +54,7 @@ const int EPA_MAX_VERTS = 32;
 // ---------------------------------------------------------------- flop counter
 // CD is a double that counts every floating-point add/sub/mul/div/sqrt it
 // executes (FMA is never formed: the library is built with -ffp-contract=off).
-static uint64_t g_flops = 0;
+static thread_local uint64_t g_flops = 0;  // per thread; orc_count_flops sums them
 struct CD {
     double v;
     CD() : v(0) {}
@@ -861,7 +864,11 @@ ORC_EXPORT uint64_t orc_count_flops(int64_t P, const int32_t* pa, const int32_t*
                                     int64_t W, double dt) {
     Law Lp = make_law(law_p[0], law_p[1], law_p[2], law_p[3], dt, (int)law_p[4]);
     Law Lw = make_law(law_w[0], law_w[1], law_w[2], law_w[3], dt, (int)law_w[4]);
+    uint64_t total = 0;
+#pragma omp parallel reduction(+ : total)
+    {
     g_flops = 0;
+#pragma omp for schedule(dynamic, 64)
     for (int64_t p = 0; p < P; ++p) {
         int a = pa[p], b = pb[p];
         Body<CD> A{verts + (int64_t)vmax * 3 * a, nverts[a]};
@@ -877,7 +884,9 @@ ORC_EXPORT uint64_t orc_count_flops(int64_t P, const int32_t* pa, const int32_t*
         in.wall = (a < W || b < W) ? 1 : 0;
         contact_one(A, B, in, in.wall ? Lw : Lp);
     }
-    return g_flops;
+    total += g_flops;
+    }
+    return total;
 }
 
 // Verlet over rows [pmin,pmax) of the reference's arrays (gpu_Verlet.py:53-92).
@@ -888,6 +897,7 @@ ORC_EXPORT void orc_verlet(double* v, double* v12, double* w12bf, double* q12, d
                            double* dxall, int vmax, const int32_t* nrot, double dt, const double* box,
                            int64_t pmin, int64_t pmax, int rot_mode) {
     StateView s{v, v12, w12bf, q12, q, qall, wbf, w, F, tau, verts, com, mass, Iinv, I, dxall};
+#pragma omp parallel for schedule(static)
     for (int64_t P = pmin; P < pmax; ++P) verlet_one(s, P, vmax, nrot ? nrot[P] : vmax, dt, box, rot_mode);
 }
 
@@ -904,38 +914,52 @@ ORC_EXPORT int64_t orc_contact_state(int64_t P, const int32_t* pa, const int32_t
     Law Lp = make_law(law_p[0], law_p[1], law_p[2], law_p[3], dt, (int)law_p[4]);
     Law Lw = make_law(law_w[0], law_w[1], law_w[2], law_w[3], dt, (int)law_w[4]);
     int64_t K = 0;
-    for (int64_t p = 0; p < P; ++p) {
-        int a = pa[p], b = pb[p];
-        Body<double> A{verts + (int64_t)vmax * 3 * a, nverts[a]};
-        Body<double> B{verts + (int64_t)vmax * 3 * b, nverts[b]};
-        PairIn<double> in;
-        in.comA = load3<double>(com + 3 * a); in.comB = load3<double>(com + 3 * b);
-        in.vA = load3<double>(vel + 3 * a); in.vB = load3<double>(vel + 3 * b);
-        in.wA = load3<double>(angvel + 3 * a); in.wB = load3<double>(angvel + 3 * b);
-        // tangential history of the pair ([P,4] per pair; none: reset, Q1)
-        in.dispt = dispt ? mk<double>(dispt[4 * p], dispt[4 * p + 1], dispt[4 * p + 2]) : mk<double>(0.0, 0.0, 0.0);
-        in.qA = qload<double>(q + 4 * a);
-        in.isA = isize[a]; in.isB = isize[b];
-        in.imA = a < W ? 0.0 : 1.0 / mass[a]; in.imB = b < W ? 0.0 : 1.0 / mass[b];
-        in.wall = (a < W || b < W) ? 1 : 0;
-        PairResult r = contact_one(A, B, in, in.wall ? Lw : Lp);
-        collision[p] = r.collision;
-        if (epa_status) epa_status[p] = r.epa_status;
-        if (depth) depth[p] = r.depth;
-        for (int k = 0; k < 3; ++k) {
-            if (normal) normal[3 * p + k] = r.n[k];
-            if (cpA) cpA[3 * p + k] = r.cpA[k];
-            if (cpB) cpB[3 * p + k] = r.cpB[k];
-        }
-        if (disp_out && r.collision == 1)
-            for (int k = 0; k < 4; ++k) disp_out[4 * p + k] = r.disp[k];
-        if (r.collision == 1) {
-            ++K;
+    // pairs are independent: each chunk is evaluated in parallel (OpenMP),
+    // then its contacts are added to the particle sums one by one in pair
+    // order, so the sums do not depend on the thread count
+    const int64_t CH = int64_t(1) << 18;
+    std::vector<double> fb(9 * (size_t)std::min<int64_t>(P, CH));
+    for (int64_t p0 = 0; p0 < P; p0 += CH) {
+        const int64_t p1 = std::min<int64_t>(P, p0 + CH);
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int64_t p = p0; p < p1; ++p) {
+            int a = pa[p], b = pb[p];
+            Body<double> A{verts + (int64_t)vmax * 3 * a, nverts[a]};
+            Body<double> B{verts + (int64_t)vmax * 3 * b, nverts[b]};
+            PairIn<double> in;
+            in.comA = load3<double>(com + 3 * a); in.comB = load3<double>(com + 3 * b);
+            in.vA = load3<double>(vel + 3 * a); in.vB = load3<double>(vel + 3 * b);
+            in.wA = load3<double>(angvel + 3 * a); in.wB = load3<double>(angvel + 3 * b);
+            // tangential history of the pair ([P,4] per pair; none: reset, Q1)
+            in.dispt = dispt ? mk<double>(dispt[4 * p], dispt[4 * p + 1], dispt[4 * p + 2]) : mk<double>(0.0, 0.0, 0.0);
+            in.qA = qload<double>(q + 4 * a);
+            in.isA = isize[a]; in.isB = isize[b];
+            in.imA = a < W ? 0.0 : 1.0 / mass[a]; in.imB = b < W ? 0.0 : 1.0 / mass[b];
+            in.wall = (a < W || b < W) ? 1 : 0;
+            PairResult r = contact_one(A, B, in, in.wall ? Lw : Lp);
+            collision[p] = r.collision;
+            if (epa_status) epa_status[p] = r.epa_status;
+            if (depth) depth[p] = r.depth;
             for (int k = 0; k < 3; ++k) {
-                force[3 * a + k] = force[3 * a + k] + r.Fn[k];
-                force[3 * b + k] = force[3 * b + k] + (-r.Fn[k]);
-                torque[3 * a + k] = torque[3 * a + k] + r.tA[k];
-                torque[3 * b + k] = torque[3 * b + k] + r.tB[k];
+                if (normal) normal[3 * p + k] = r.n[k];
+                if (cpA) cpA[3 * p + k] = r.cpA[k];
+                if (cpB) cpB[3 * p + k] = r.cpB[k];
+            }
+            if (disp_out && r.collision == 1)
+                for (int k = 0; k < 4; ++k) disp_out[4 * p + k] = r.disp[k];
+            double* f = fb.data() + 9 * (p - p0);
+            for (int k = 0; k < 3; ++k) { f[k] = r.Fn[k]; f[3 + k] = r.tA[k]; f[6 + k] = r.tB[k]; }
+        }
+        for (int64_t p = p0; p < p1; ++p) {
+            if (collision[p] != 1) continue;
+            ++K;
+            const int a = pa[p], b = pb[p];
+            const double* f = fb.data() + 9 * (p - p0);
+            for (int k = 0; k < 3; ++k) {
+                force[3 * a + k] = force[3 * a + k] + f[k];
+                force[3 * b + k] = force[3 * b + k] + (-f[k]);
+                torque[3 * a + k] = torque[3 * a + k] + f[3 + k];
+                torque[3 * b + k] = torque[3 * b + k] + f[6 + k];
             }
         }
     }
@@ -1009,6 +1033,9 @@ ORC_EXPORT void orc_gjk_trace(int64_t P, const int32_t* pa, const int32_t* pb, i
         epa_iters[p] = r.iters;
     }
 }
+
+// Threads the parallel loops of the oracle use (OpenMP).
+ORC_EXPORT int orc_threads() { return omp_get_max_threads(); }
 
 // ------------------------------------------------------------ test helpers
 ORC_EXPORT void orc_quat_mul(const double* a, const double* b, double* out) {

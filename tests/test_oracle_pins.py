@@ -5,6 +5,7 @@ swapped operand) fails one of them."""
 from __future__ import annotations
 
 import math
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -673,3 +674,32 @@ def test_verlet_torque_free_asymmetric_body_conserves_momentum_and_energy():
     assert abs(E / E0 - 1) < 1e-4
     # the body did turn, and not about a fixed body axis
     assert np.linalg.norm(w - wbf) > 0.5
+
+
+def test_oracle_results_do_not_depend_on_its_thread_count():
+    """The oracle evaluates pairs and particles in parallel (OpenMP) and adds
+    contact sums in pair order: one thread and all threads give the same
+    state bit for bit after steps with contacts."""
+    import hashlib
+    import os
+    import subprocess
+    import sys
+    code = (
+        "import hashlib, numpy as np\n"
+        "from mwdiem_amd import scene\n"
+        "from oracle import oracle as O\n"
+        "sc = scene.compact_dense(scene.make_scene(40, (8.0, 8.0, 16.0), 7, law=scene.Law4(mode=1)), gap=-0.55)\n"
+        "st = O.initial_state(sc)\n"
+        "n = 0\n"
+        "for _ in range(5):\n"
+        "    n += int((O.step(st, sc)[1] == 1).sum())\n"
+        "h = hashlib.sha256(b''.join(st[k].tobytes() for k in O.STATE_KEYS)).hexdigest()\n"
+        "print(n, h)\n")
+    root = Path(__file__).resolve().parents[1]
+    outs = []
+    for threads in ("1", "4"):
+        env = dict(os.environ, OMP_NUM_THREADS=threads, PYTHONPATH=str(root))
+        r = subprocess.run([sys.executable, "-c", code], env=env, cwd=root, capture_output=True, text=True,
+                           check=True)
+        outs.append(r.stdout.split())
+    assert int(outs[0][0]) > 20 and outs[0] == outs[1]
