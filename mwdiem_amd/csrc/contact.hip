@@ -69,6 +69,8 @@ __device__ __forceinline__ d3 support(const Side& A, const Side& B, d3 d, int* i
 }
 
 struct SerialSup {
+    static constexpr bool kQuad = false;  // one lane per pair
+    __device__ __forceinline__ int quad_lane() const { return 0; }
     Side A, B;
     __device__ __forceinline__ d3 operator()(d3 d, int* ia, int* ib) const { return support(A, B, d, ia, ib); }
 };
@@ -81,6 +83,8 @@ struct SerialSup {
 // coordinates come from their owner lanes by shuffle (no memory round trip).
 template <int G, int K>
 struct GroupSup {
+    static constexpr bool kQuad = true;  // groups of 4, 8 or 16 lanes: whole quads
+    __device__ __forceinline__ int quad_lane() const { return g & 3; }
     Side A, B;
     d3 va[K], vb[K];
     int g;
@@ -278,6 +282,78 @@ __device__ __forceinline__ void check_tetra(d3 P1, d3 P2, d3 P3, d3 P4, int* col
     }
 }
 
+// Check_Tetrahedron by the four lanes of a quad that share one pair (the
+// grouped support map's lanes hold the same simplex): lane q evaluates
+// triangle q of the three the reference tests -- (P1,P2,P4), (P2,P3,P4),
+// (P3,P1,P4); lane 3 repeats the third -- and the quad exchanges the
+// outcomes by DPP broadcasts, so each lane runs one triangle test instead of
+// three. The outcome is check_tetra's, bit for bit (the same arithmetic on
+// the same values, picked in the same order). Every lane of the quad must be
+// active.
+template <int J>
+__device__ __forceinline__ int quad_bcast(int x) {
+    return __builtin_amdgcn_update_dpp(0, x, J * 0x55, 0xF, 0xF, false);  // quad_perm [J,J,J,J]
+}
+template <int J>
+__device__ __forceinline__ double quad_bcast(double x) {
+    const int2 w = *reinterpret_cast<const int2*>(&x);
+    int2 o;
+    o.x = quad_bcast<J>(w.x);
+    o.y = quad_bcast<J>(w.y);
+    return *reinterpret_cast<const double*>(&o);
+}
+template <int J>
+__device__ __forceinline__ SimplexOut quad_bcast(const SimplexOut& t) {
+    SimplexOut o;
+    o.c = quad_bcast<J>(t.c);
+    o.d = mk3(quad_bcast<J>(t.d.x), quad_bcast<J>(t.d.y), quad_bcast<J>(t.d.z));
+    return o;
+}
+
+__device__ __forceinline__ void check_tetra_quad(d3 P1, d3 P2, d3 P3, d3 P4, int q, int* coll, d3* d, int* todel) {
+    const d3 X = q == 0 ? P1 : (q == 1 ? P2 : P3);
+    const d3 Y = q == 0 ? P2 : (q == 1 ? P3 : P1);
+    const SimplexOut mine = check_triangle(X, Y, P4);
+    const SimplexOut t1 = quad_bcast<0>(mine);
+    const SimplexOut t2 = quad_bcast<1>(mine);
+    const SimplexOut t3 = quad_bcast<2>(mine);
+    const d3 V13 = P1 - P3, V21 = P2 - P1, V32 = P3 - P2, V41 = P4 - P1;
+    const d3 n2 = cross3(V21, V41);
+    const d3 V42 = P4 - P2;
+    const d3 n3 = cross3(V32, V42);
+    const d3 V43 = P4 - P3;
+    const d3 n4 = cross3(V13, V43);
+    *todel = 0;
+    if (t1.c == 1) {
+        *coll = 1;
+        *d = t1.d;
+    } else if (t2.c == 1) {
+        *coll = 1;
+        *d = t2.d;
+    } else if (t3.c == 1) {
+        *coll = 1;
+        *d = t3.d;
+    } else if (t1.c + t2.c + t3.c > -2) {
+        *coll = 0;
+        *d = t3.d;
+    } else if (dot3(n2, P4) < 0.0) {
+        *coll = -1;
+        *d = n2;
+        *todel = 3;
+    } else if (dot3(n3, P4) < 0.0) {
+        *coll = -1;
+        *d = n3;
+        *todel = 1;
+    } else if (dot3(n4, P4) < 0.0) {
+        *coll = -1;
+        *d = n4;
+        *todel = 2;
+    } else {
+        *coll = 1;
+        *d = t3.d;
+    }
+}
+
 // CorrectFlat, .cu:256-268
 __device__ __forceinline__ bool correct_flat(d3 S1, d3 S2, d3 S3, d3 S4) {
     const double det1 = S1.x * (S2.y * S3.z - S3.y * S2.z) - S2.x * (S1.y * S3.z - S3.y * S1.z) +
@@ -319,7 +395,10 @@ __device__ __forceinline__ int gjk_pair(const Sup& support, d3 comA, d3 comB, d3
         bool hit = zero3(S3);
         if (!hit) {
             int todel;
-            check_tetra(S0, S1, S2, S3, &coll, &d4, &todel);
+            if constexpr (Sup::kQuad)
+                check_tetra_quad(S0, S1, S2, S3, support.quad_lane(), &coll, &d4, &todel);
+            else
+                check_tetra(S0, S1, S2, S3, &coll, &d4, &todel);
             if (coll == 0) return 0;
             if (coll == 1) {
                 hit = true;
