@@ -115,13 +115,18 @@ class AsyncWriter:
         self.worker.start()
 
     def _snapshot(self, keys):
+        """Device copies of the state arrays taken in the stepper's stream
+        order (so the next step cannot overwrite them first), then copied to
+        pinned host buffers on the side stream while the stepper goes on."""
         import torch
         ev = torch.cuda.Event()
+        with torch.cuda.stream(self.S.stream):
+            staged = {k: self.S.t[k].clone() for k in keys}
         self.side.wait_stream(self.S.stream)
         host = {}
         with torch.cuda.stream(self.side):
-            for k in keys:
-                t = self.S.t[k]
+            for k, t in staged.items():
+                t.record_stream(self.side)  # its memory is reused only after the side stream's copy
                 h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
                 h.copy_(t, non_blocking=True)
                 host[k] = h
