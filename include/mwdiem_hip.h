@@ -245,6 +245,27 @@ int mwd_contact_pairs(int64_t P, int32_t vmax, const double* comA, const double*
                       int32_t* epa_status, double* depth, double* normal, double* cpA, double* cpB, double* disp,
                       double* force, double* torque, int64_t n_particles, void* stream);
 
+/* mwd_contact_pairs without allocation. No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid hipMalloc). Same arguments, results and
+ * synchronisation as mwd_contact_pairs; the call's temporaries live in a
+ * fixed device array of MWD_NOALLOC_SCRATCH_BYTES on each GPU the library
+ * runs on (held until the process ends). Limits: P <= MWD_NOALLOC_MAX_PAIRS
+ * and n_particles <= MWD_NOALLOC_MAX_PARTICLES, else MWD_E_INVALID (nothing
+ * launched). Every call on one GPU uses that GPU's one array: calls that
+ * overlap on one GPU (other threads, other streams) race. */
+#define MWD_NOALLOC_MAX_PAIRS 65536
+#define MWD_NOALLOC_MAX_PARTICLES 262144
+#define MWD_NOALLOC_SCRATCH_BYTES (32u << 20)
+int mwd_contact_pairs_noalloc(int64_t P, int32_t vmax, const double* comA, const double* comB, const double* vertsA,
+                              const double* vertsB, const double* vA, const double* vB, const double* wA,
+                              const double* wB, const int32_t* nA, const int32_t* nB, const int32_t* idA,
+                              const int32_t* idB, const double* dispt, const double* qA, const double* isA,
+                              const double* isB, const mwd_law* law_p, const mwd_law* law_w, int64_t n_walls,
+                              int32_t* collision, int32_t* epa_status, double* depth, double* normal, double* cpA,
+                              double* cpB, double* disp, double* force, double* torque, int64_t n_particles,
+                              void* stream);
+
 /* Cell keys (MWDiem.py:813-816) for rows [0,n): trunc(com/cell) combined as
  * xc + yc*ncx + zc*ncx*ncy, int64. */
 int mwd_cell_keys(const double* com, int64_t n, const double* cell, int64_t ncx, int64_t ncy, int64_t* keys,

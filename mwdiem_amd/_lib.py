@@ -33,6 +33,9 @@ ROW_CONTACTS = 32
 EPA_MAX_FACES = 128
 MAX_CELL = 76
 
+NOALLOC_MAX_PAIRS = 65536
+NOALLOC_MAX_PARTICLES = 262144
+
 
 class MwdError(RuntimeError):
     def __init__(self, code: int, where: str, msg: str = ""):
@@ -118,6 +121,8 @@ _SIGS = {
     "mwd_verlet": (C.c_int, [_P] * 16 + [C.c_int32, _P, C.c_double, _P, C.c_int64, C.c_int64, _P]),
     "mwd_contact_pairs": (C.c_int, [C.c_int64, C.c_int32] + [_P] * 16 + [_P, _P, C.c_int64] + [_P] * 9
                           + [C.c_int64, _P]),
+    "mwd_contact_pairs_noalloc": (C.c_int, [C.c_int64, C.c_int32] + [_P] * 16 + [_P, _P, C.c_int64] + [_P] * 9
+                                  + [C.c_int64, _P]),
     "mwd_cell_keys": (C.c_int, [_P, C.c_int64, _P, C.c_int64, C.c_int64, _P, _P]),
     "mwd_candidates": (C.c_int, [_P, C.POINTER(State), _P]),
     "mwd_pairs_view": (C.c_int, [_P, C.POINTER(_P), C.POINTER(_P)]),

@@ -3,7 +3,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "mwd_internal.h"
@@ -933,6 +935,172 @@ MWD_API int mwd_clear_flags(mwd_workspace* ws) {
     return MWD_OK;
 }
 
+// ------------------------------------------------------- per-pair fine contact
+namespace {
+
+__global__ void k_put_laws(mwd_law* dst, mwd_law a, mwd_law b) {
+    if (threadIdx.x == 0) {
+        dst[0] = a;
+        dst[1] = b;
+    }
+}
+
+// Temporaries of one per-pair fine-contact call, carved from one device
+// buffer: per pair the collision code, contact slot, EPA status, simplex and
+// per-contact results; per record (two per pair) the sort keys and values;
+// the radix histograms, the scan scratch, counters and the two laws.
+struct PairScratch {
+    int32_t *coll, *cpair, *est, *rval, *rval2, *sb, *se, *hist, *sidx, *surv;
+    uint8_t *wallA, *wallB;
+    Simplex* sx;
+    int64_t *ctr, *hist_off, *tmp;
+    double *dep, *nrm, *ca, *cb, *fn, *ta, *tb;
+    uint32_t *rkey, *rkey2, *flags;
+    mwd_law* laws;
+    int64_t tmp_len;
+};
+
+// Lays the temporaries for P pairs and n_particles force rows out from
+// `base` (each buffer 256-byte aligned) and returns the bytes they take;
+// with base == nullptr it only counts.
+size_t carve_pair_scratch(char* base, int64_t P, int64_t n_particles, bool with_force, PairScratch* ps) {
+    size_t off = 0;
+    auto take = [&](auto** p, int64_t count) {
+        using T = typename std::remove_pointer<typename std::remove_pointer<decltype(p)>::type>::type;
+        off = (off + 255) & ~size_t(255);
+        *p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+        off += sizeof(T) * (size_t)std::max<int64_t>(count, 1);
+    };
+    const int64_t rec = 2 * P;
+    const int64_t tiles = mwd_sort_tiles(rec);
+    const int64_t hist_len = (int64_t(1) << MWD_SORT_MAX_BITS) * tiles;
+    ps->tmp_len = mwd_scan_tmp_len(std::max({P, rec, hist_len, n_particles}));
+    take(&ps->coll, P);
+    take(&ps->cpair, P);
+    take(&ps->est, P);
+    take(&ps->rval, rec);
+    take(&ps->rval2, rec);
+    take(&ps->rkey, rec);
+    take(&ps->rkey2, rec);
+    take(&ps->wallA, P);
+    take(&ps->wallB, P);
+    take(&ps->sx, P);
+    take(&ps->sidx, P);
+    take(&ps->surv, P);
+    take(&ps->ctr, 16);
+    take(&ps->hist, hist_len);
+    take(&ps->hist_off, hist_len);
+    take(&ps->tmp, ps->tmp_len);
+    take(&ps->dep, P);
+    take(&ps->nrm, 3 * P);
+    take(&ps->ca, 3 * P);
+    take(&ps->cb, 3 * P);
+    take(&ps->fn, 3 * P);
+    take(&ps->ta, 3 * P);
+    take(&ps->tb, 3 * P);
+    take(&ps->flags, 1);
+    take(&ps->laws, 2);
+    ps->sb = ps->se = nullptr;
+    if (with_force) {
+        take(&ps->sb, n_particles);
+        take(&ps->se, n_particles);
+    }
+    return off;
+}
+
+int check_pair_args(const char* fn, int64_t P, int32_t vmax, const mwd_law* law_p, const mwd_law* law_w,
+                    int64_t n_particles) {
+    if (P < 0 || P > INT32_MAX / 4 || vmax <= 0 || vmax > 255 || !law_p || !law_w || n_particles < 0) {
+        mwd_set_error(std::string(fn) + ": invalid argument");
+        return MWD_E_INVALID;
+    }
+    return MWD_OK;
+}
+
+int check_pair_arrays(const char* fn, const double* comA, const double* comB, const double* vertsA,
+                      const double* vertsB, const double* vA, const double* vB, const double* wA, const double* wB,
+                      const int32_t* nA, const int32_t* nB, const int32_t* idA, const int32_t* idB, const double* qA,
+                      const double* isA, const double* isB, const mwd_law* law_p, const mwd_law* law_w,
+                      const double* force, const double* torque, int64_t n_particles) {
+    if (!comA || !comB || !vertsA || !vertsB || !vA || !vB || !wA || !wB || !nA || !nB || !idA || !idB || !qA ||
+        !isA || !isB || ((force || torque) && (!force || !torque || n_particles <= 0))) {
+        mwd_set_error(std::string(fn) + ": null array");
+        return MWD_E_INVALID;
+    }
+    if (law_p->mode != MWD_LAW_REF || law_w->mode != MWD_LAW_REF) {
+        mwd_set_error(std::string(fn) + ": the per-pair layout carries no masses; use MWD_LAW_REF");
+        return MWD_E_INVALID;
+    }
+    return MWD_OK;
+}
+
+// GJK, EPA and the law over P pairs in the reference's per-pair layout, the
+// per-pair outputs scattered back to pair order and (with force) the
+// per-particle contact sums added in ascending pair order. Every temporary
+// is in `ps`; nothing is allocated. Stream-ordered (no synchronisation).
+int contact_pairs_core(const PairScratch& ps, int64_t P, int32_t vmax, const double* comA, const double* comB,
+                       const double* vertsA, const double* vertsB, const double* vA, const double* vB,
+                       const double* wA, const double* wB, const int32_t* nA, const int32_t* nB, const int32_t* idA,
+                       const int32_t* idB, const double* dispt, const double* qA, const double* isA,
+                       const double* isB, const mwd_law* law_p, const mwd_law* law_w, int64_t n_walls,
+                       int32_t* collision, int32_t* epa_status, double* depth, double* normal, double* cpA,
+                       double* cpB, double* disp, double* force, double* torque, int64_t n_particles,
+                       hipStream_t s) {
+    int rc;
+    const int64_t rec = 2 * P;
+    // counters, scan scratch and flags start at zero (fill kernels, no memset nodes)
+    if ((rc = mwd_fill_i32(reinterpret_cast<int32_t*>(ps.ctr), 0, 32, s))) return rc;
+    if ((rc = mwd_fill_i32(reinterpret_cast<int32_t*>(ps.tmp), 0, 2 * ps.tmp_len, s))) return rc;
+    if ((rc = mwd_fill_i32(reinterpret_cast<int32_t*>(ps.flags), 0, 1, s))) return rc;
+    hipLaunchKernelGGL(k_put_laws, dim3(1), dim3(64), 0, s, ps.laws, *law_p, *law_w);
+    MWD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_wall_from_ids, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, idA, idB, n_walls, P,
+                       ps.wallA, ps.wallB);
+    MWD_LAUNCH_CHECK();
+    BodyView A{comA, vertsA, nA, vA, wA, qA, isA, nullptr, nullptr, ps.wallA, nullptr, nullptr, 1, vmax};
+    BodyView B{comB, vertsB, nB, vB, wB, qA, isB, nullptr, nullptr, ps.wallB, nullptr, nullptr, 1, vmax};
+    int64_t* ctr = ps.ctr;
+    if ((rc = mwd_set_i64(ctr + 3, P, s))) return rc;
+    if ((rc = mwd_launch_zero_pairs(P, depth, normal, cpA, cpB, epa_status, disp, s))) return rc;
+    if ((rc = mwd_launch_gjk(A, B, P, ctr + 3, ps.coll, ps.surv, ctr + 10, ps.sx, ps.sidx, ctr + 8, P, ps.flags, s)))
+        return rc;
+    if ((rc = mwd_launch_compact_contacts(ps.coll, P, ctr + 3, ps.cpair, P, ctr + 4, ps.flags, ps.tmp, ps.tmp_len,
+                                          s)))
+        return rc;
+    if ((rc = mwd_launch_epa_force(A, B, ps.laws, ps.cpair, ps.sx, ps.sidx, P, ctr + 4, dispt, idA, idB, ps.dep,
+                                   ps.nrm, ps.ca, ps.cb, ps.fn, ps.ta, ps.tb, disp, ps.est, ps.rkey, ps.rval,
+                                   ctr + 5, ps.flags, s)))
+        return rc;
+    // per-pair outputs scattered back from contact order
+    if (collision) MWD_HIP_CHECK(hipMemcpyAsync(collision, ps.coll, sizeof(int32_t) * P, hipMemcpyDeviceToDevice, s));
+    if ((rc = mwd_launch_scatter_contacts(ps.cpair, P, ctr + 4, ps.dep, ps.nrm, ps.ca, ps.cb, ps.est, P, depth,
+                                          normal, cpA, cpB, epa_status, disp, s)))
+        return rc;
+    if (force) {
+        uint32_t* ks;
+        int32_t* vs;
+        if ((rc = mwd_radix_sort(ps.rkey, ps.rval, ps.rkey2, ps.rval2, rec, ctr + 5, bits_for(n_particles), ps.hist,
+                                 ps.hist_off, ctr + 9, ps.tmp, ps.tmp_len, s, &ks, &vs)))
+            return rc;
+        if ((rc = mwd_fill_i32(ps.sb, 0, n_particles, s))) return rc;
+        if ((rc = mwd_fill_i32(ps.se, 0, n_particles, s))) return rc;
+        if ((rc = mwd_launch_segments(ks, rec, ctr + 5, ps.sb, ps.se, s))) return rc;
+        if ((rc = mwd_launch_add_sums(ps.sb, ps.se, vs, ps.fn, ps.ta, ps.tb, force, torque, n_particles, s)))
+            return rc;
+    }
+    return MWD_OK;
+}
+
+int finish_pair_call(const char* fn, int rc, hipStream_t s) {
+    if (hipStreamSynchronize(s) != hipSuccess && !rc) {
+        mwd_set_error(std::string(fn) + ": kernel failure");
+        rc = MWD_E_HIP;
+    }
+    return rc;
+}
+
+}  // namespace
+
 MWD_API int mwd_contact_pairs(int64_t P, int32_t vmax, const double* comA, const double* comB, const double* vertsA,
                               const double* vertsB, const double* vA, const double* vB, const double* wA,
                               const double* wB, const int32_t* nA, const int32_t* nB, const int32_t* idA,
@@ -941,131 +1109,93 @@ MWD_API int mwd_contact_pairs(int64_t P, int32_t vmax, const double* comA, const
                               int32_t* collision, int32_t* epa_status, double* depth, double* normal, double* cpA,
                               double* cpB, double* disp, double* force, double* torque, int64_t n_particles,
                               void* stream) {
-    if (P < 0 || P > INT32_MAX / 4 || vmax <= 0 || vmax > 255 || !law_p || !law_w || n_particles < 0) {
-        mwd_set_error("mwd_contact_pairs: invalid argument");
+    const char* fn = "mwd_contact_pairs";
+    int rc = check_pair_args(fn, P, vmax, law_p, law_w, n_particles);
+    if (rc || P == 0) return rc;
+    if ((rc = check_pair_arrays(fn, comA, comB, vertsA, vertsB, vA, vB, wA, wB, nA, nB, idA, idB, qA, isA, isB, law_p,
+                                law_w, force, torque, n_particles)))
+        return rc;
+    hipStream_t s = (hipStream_t)stream;
+    PairScratch ps;
+    const size_t bytes = carve_pair_scratch(nullptr, P, n_particles, force != nullptr, &ps);
+    char* base = nullptr;
+    if ((rc = dalloc(&base, (int64_t)bytes))) return rc;
+    carve_pair_scratch(base, P, n_particles, force != nullptr, &ps);
+    rc = contact_pairs_core(ps, P, vmax, comA, comB, vertsA, vertsB, vA, vB, wA, wB, nA, nB, idA, idB, dispt, qA, isA,
+                            isB, law_p, law_w, n_walls, collision, epa_status, depth, normal, cpA, cpB, disp, force,
+                            torque, n_particles, s);
+    rc = finish_pair_call(fn, rc, s);
+    (void)hipFree(base);
+    return rc;
+}
+
+// ---------------------------------------------- the allocation-free variant
+// The temporaries of mwd_contact_pairs_noalloc live in this array (one copy
+// per GPU, resident while the library is loaded). Host code never uses the
+// name itself as an address: each device's copy is looked up once with
+// hipGetSymbolAddress and that pointer is what the kernels receive.
+static __device__ __align__(256) unsigned char scratch[MWD_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxDevices] = {};
+
+// Address of the current device's scratch array (looked up on its first use
+// on that device).
+int scratch_for_current_device(char** out) {
+    int dev = -1;
+    MWD_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        mwd_set_error("mwd_contact_pairs_noalloc: device index out of range");
+        return MWD_E_INVALID;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            mwd_set_error(std::string("mwd_contact_pairs_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+            return MWD_E_HIP;
+        }
+        g_scratch_ptr[dev] = static_cast<char*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return MWD_OK;
+}
+}  // namespace
+
+MWD_API int mwd_contact_pairs_noalloc(int64_t P, int32_t vmax, const double* comA, const double* comB,
+                                      const double* vertsA, const double* vertsB, const double* vA, const double* vB,
+                                      const double* wA, const double* wB, const int32_t* nA, const int32_t* nB,
+                                      const int32_t* idA, const int32_t* idB, const double* dispt, const double* qA,
+                                      const double* isA, const double* isB, const mwd_law* law_p,
+                                      const mwd_law* law_w, int64_t n_walls, int32_t* collision, int32_t* epa_status,
+                                      double* depth, double* normal, double* cpA, double* cpB, double* disp,
+                                      double* force, double* torque, int64_t n_particles, void* stream) {
+    const char* fn = "mwd_contact_pairs_noalloc";
+    int rc = check_pair_args(fn, P, vmax, law_p, law_w, n_particles);
+    if (rc) return rc;
+    if (P > MWD_NOALLOC_MAX_PAIRS || n_particles > MWD_NOALLOC_MAX_PARTICLES) {
+        mwd_set_error("mwd_contact_pairs_noalloc: more than MWD_NOALLOC_MAX_PAIRS pairs or "
+                      "MWD_NOALLOC_MAX_PARTICLES particles");
         return MWD_E_INVALID;
     }
     if (P == 0) return MWD_OK;
-    if (!comA || !comB || !vertsA || !vertsB || !vA || !vB || !wA || !wB || !nA || !nB || !idA || !idB || !qA ||
-        !isA || !isB || ((force || torque) && (!force || !torque || n_particles <= 0))) {
-        mwd_set_error("mwd_contact_pairs: null array");
+    if ((rc = check_pair_arrays(fn, comA, comB, vertsA, vertsB, vA, vB, wA, wB, nA, nB, idA, idB, qA, isA, isB, law_p,
+                                law_w, force, torque, n_particles)))
+        return rc;
+    PairScratch ps;
+    if (carve_pair_scratch(nullptr, P, n_particles, force != nullptr, &ps) > sizeof(scratch)) {
+        mwd_set_error("mwd_contact_pairs_noalloc: temporaries exceed MWD_NOALLOC_SCRATCH_BYTES");
         return MWD_E_INVALID;
     }
+    char* base = nullptr;
+    if ((rc = scratch_for_current_device(&base))) return rc;
+    carve_pair_scratch(base, P, n_particles, force != nullptr, &ps);
     hipStream_t s = (hipStream_t)stream;
-    // temporaries of this call
-    int32_t *coll = nullptr, *cpair = nullptr, *est = nullptr, *rval = nullptr, *rval2 = nullptr;
-    int32_t *sb = nullptr, *se = nullptr, *hist = nullptr, *sidx = nullptr, *surv = nullptr;
-    uint8_t *wallA = nullptr, *wallB = nullptr;
-    Simplex* sx = nullptr;
-    int64_t *ctr = nullptr, *hist_off = nullptr, *tmp = nullptr;
-    double *dep = nullptr, *nrm = nullptr, *ca = nullptr, *cb = nullptr, *fn = nullptr, *ta = nullptr, *tb = nullptr;
-    uint32_t *rkey = nullptr, *rkey2 = nullptr, *flags = nullptr;
-    mwd_law* laws = nullptr;
-    const int64_t rec = 2 * P;
-    const int64_t tiles = mwd_sort_tiles(rec);
-    const int64_t hist_len = (int64_t(1) << MWD_SORT_MAX_BITS) * tiles;
-    const int64_t tmp_len = mwd_scan_tmp_len(std::max({P, rec, hist_len, n_particles}));
-    int rc = MWD_OK;
-    std::vector<void*> owned;
-    auto al = [&](auto** p, int64_t n) {
-        if (rc) return;
-        rc = dalloc(p, n);
-        if (!rc) owned.push_back((void*)*p);
-    };
-    al(&coll, P);
-    al(&cpair, P);
-    al(&est, P);
-    al(&rval, rec);
-    al(&rval2, rec);
-    al(&rkey, rec);
-    al(&rkey2, rec);
-    al(&wallA, P);
-    al(&wallB, P);
-    al(&sx, P);
-    al(&sidx, P);
-    al(&surv, P);
-    al(&ctr, 16);
-    al(&hist, hist_len);
-    al(&hist_off, hist_len);
-    al(&tmp, tmp_len);
-    al(&dep, P);
-    al(&nrm, 3 * P);
-    al(&ca, 3 * P);
-    al(&cb, 3 * P);
-    al(&fn, 3 * P);
-    al(&ta, 3 * P);
-    al(&tb, 3 * P);
-    al(&flags, 1);
-    al(&laws, 2);
-    if (force) {
-        al(&sb, n_particles);
-        al(&se, n_particles);
-    }
-    auto cleanup = [&]() {
-        (void)hipStreamSynchronize(s);
-        for (void* p : owned) (void)hipFree(p);
-    };
-    if (rc) {
-        cleanup();
-        return rc;
-    }
-    mwd_law lw[2] = {*law_p, *law_w};
-    do {
-        if (hipMemcpyAsync(laws, lw, sizeof(lw), hipMemcpyHostToDevice, s) != hipSuccess ||
-            hipMemsetAsync(ctr, 0, 16 * sizeof(int64_t), s) != hipSuccess ||
-            hipMemsetAsync(tmp, 0, sizeof(int64_t) * tmp_len, s) != hipSuccess ||
-            hipMemsetAsync(flags, 0, sizeof(uint32_t), s) != hipSuccess) {
-            mwd_set_error("mwd_contact_pairs: upload failed");
-            rc = MWD_E_HIP;
-            break;
-        }
-        hipLaunchKernelGGL(k_wall_from_ids, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, idA, idB, n_walls, P,
-                           wallA, wallB);
-        if (hipGetLastError() != hipSuccess) {
-            mwd_set_error("mwd_contact_pairs: launch failed");
-            rc = MWD_E_HIP;
-            break;
-        }
-        if (law_p->mode != MWD_LAW_REF || law_w->mode != MWD_LAW_REF) {
-            mwd_set_error("mwd_contact_pairs: the per-pair layout carries no masses; use MWD_LAW_REF");
-            rc = MWD_E_INVALID;
-            break;
-        }
-        BodyView A{comA, vertsA, nA, vA, wA, qA, isA, nullptr, nullptr, wallA, nullptr, nullptr, 1, vmax};
-        BodyView B{comB, vertsB, nB, vB, wB, qA, isB, nullptr, nullptr, wallB, nullptr, nullptr, 1, vmax};
-        if ((rc = mwd_set_i64(ctr + 3, P, s))) break;
-        if ((rc = mwd_launch_zero_pairs(P, depth, normal, cpA, cpB, epa_status, disp, s))) break;
-        if ((rc = mwd_launch_gjk(A, B, P, ctr + 3, coll, surv, ctr + 10, sx, sidx, ctr + 8, P, flags, s))) break;
-        if ((rc = mwd_launch_compact_contacts(coll, P, ctr + 3, cpair, P, ctr + 4, flags, tmp, tmp_len, s))) break;
-        if ((rc = mwd_launch_epa_force(A, B, laws, cpair, sx, sidx, P, ctr + 4, dispt, idA, idB, dep, nrm, ca, cb, fn, ta,
-                                       tb, disp, est, rkey, rval, ctr + 5, flags, s)))
-            break;
-        // per-pair outputs scattered back from contact order
-        if (collision &&
-            hipMemcpyAsync(collision, coll, sizeof(int32_t) * P, hipMemcpyDeviceToDevice, s) != hipSuccess) {
-            mwd_set_error("mwd_contact_pairs: copy failed");
-            rc = MWD_E_HIP;
-            break;
-        }
-        if ((rc = mwd_launch_scatter_contacts(cpair, P, ctr + 4, dep, nrm, ca, cb, est, P, depth, normal, cpA, cpB,
-                                              epa_status, disp, s)))
-            break;
-        if (force) {
-            uint32_t* ks;
-            int32_t* vs;
-            if ((rc = mwd_radix_sort(rkey, rval, rkey2, rval2, rec, ctr + 5, bits_for(n_particles), hist, hist_off,
-                                     ctr + 9, tmp, tmp_len, s, &ks, &vs)))
-                break;
-            if ((rc = mwd_fill_i32(sb, 0, n_particles, s))) break;
-            if ((rc = mwd_fill_i32(se, 0, n_particles, s))) break;
-            if ((rc = mwd_launch_segments(ks, rec, ctr + 5, sb, se, s))) break;
-            if ((rc = mwd_launch_add_sums(sb, se, vs, fn, ta, tb, force, torque, n_particles, s))) break;
-        }
-    } while (0);
-    if (hipStreamSynchronize(s) != hipSuccess && !rc) {
-        mwd_set_error("mwd_contact_pairs: kernel failure");
-        rc = MWD_E_HIP;
-    }
-    cleanup();
-    return rc;
+    rc = contact_pairs_core(ps, P, vmax, comA, comB, vertsA, vertsB, vA, vB, wA, wB, nA, nB, idA, idB, dispt, qA, isA,
+                            isB, law_p, law_w, n_walls, collision, epa_status, depth, normal, cpA, cpB, disp, force,
+                            torque, n_particles, s);
+    return finish_pair_call(fn, rc, s);
 }

@@ -41,7 +41,7 @@ def max_rel(a, b, scale=None):
 
 
 def gpu_contact_pairs(verts, nverts, com, pairs, law_p=(1e4, 0.5, 0.5, 0.3, 0), law_w=None, n_walls=0, dt=1e-6,
-                      vel=None, angvel=None, q=None, isize=None):
+                      vel=None, angvel=None, q=None, isize=None, entry="mwd_contact_pairs", n_rows=None):
     """mwd_contact_pairs (the per-pair entry point, through the C ABI) on a
     pair list indexed into per-body arrays: gathers each pair's rows into the
     reference's per-pair layout and returns the per-pair outputs and the
@@ -53,7 +53,8 @@ def gpu_contact_pairs(verts, nverts, com, pairs, law_p=(1e4, 0.5, 0.5, 0.3, 0), 
     from mwdiem_amd import _lib
     dev = torch.device("cuda", 0)
     pairs = np.asarray(pairs, np.int64).reshape(-1, 2)
-    P, N, vmax = pairs.shape[0], verts.shape[0], verts.shape[1]
+    P, vmax = pairs.shape[0], verts.shape[1]
+    N = verts.shape[0] if n_rows is None else n_rows
     a, b = pairs[:, 0], pairs[:, 1]
     z3 = np.zeros((N, 3))
     vel = z3 if vel is None else vel
@@ -79,12 +80,12 @@ def gpu_contact_pairs(verts, nverts, com, pairs, law_p=(1e4, 0.5, 0.5, 0.3, 0), 
     law_w = law_p if law_w is None else law_w
     lp = _lib.law(*law_p[:4], dt, int(law_p[4]))
     lw = _lib.law(*law_w[:4], dt, int(law_w[4]))
-    rc = _lib.load().mwd_contact_pairs(P, vmax, *[x.data_ptr() for x in args], C.byref(lp), C.byref(lw), n_walls,
+    rc = getattr(_lib.load(), entry)(P, vmax, *[x.data_ptr() for x in args], C.byref(lp), C.byref(lw), n_walls,
                                        out["collision"].data_ptr(), out["epa_status"].data_ptr(),
                                        out["depth"].data_ptr(), out["normal"].data_ptr(), out["cpA"].data_ptr(),
                                        out["cpB"].data_ptr(), out["disp"].data_ptr(), F.data_ptr(), T.data_ptr(),
                                        N, None)
-    _lib.check(rc, "mwd_contact_pairs")
+    _lib.check(rc, entry)
     torch.cuda.synchronize()
     res = {k: v.cpu().numpy() for k, v in out.items()}
     res["force"], res["torque"] = F.cpu().numpy(), T.cpu().numpy()
