@@ -149,6 +149,19 @@ int mwd_workspace_destroy(mwd_workspace* ws);
  * velocity v[r] also enters the contact law as the body's velocity. */
 int mwd_set_wall_motion(mwd_workspace* ws, int32_t on);
 
+/* Spawning (MWDiem.py:685-714: the driver creates particles at the start of
+ * every 5th step and widens the active range [num_particles_wall,
+ * num_particles_temp) that the hash and the Verlet step cover). For a
+ * workspace of one scene: from the next step on only rows [row0, row0 +
+ * n_active) take part; at the start of every `every`-th step (the first
+ * included) `per` more of the scene's rows become active, in row order, up
+ * to the rows the scene was created with. The rows waiting to spawn hold
+ * their particles as they will enter (the reference writes them at spawn
+ * time); nothing reads or moves them before. The counts live on the device,
+ * so spawning steps are captured in graphs like any others. Key-table cell
+ * pass only; not with mwd_step_adjoint. */
+int mwd_set_spawn(mwd_workspace* ws, int64_t n_active, int64_t every, int64_t per);
+
 /* Tangential history (a5, MWDiem.py:649,872-880,953-966). MWD_HISTORY_RESET
  * (default): the reference as it runs -- the history is dropped every step
  * (Q1), so every contact starts from a zero tangential displacement.
@@ -200,6 +213,8 @@ int mwd_set_reduction(mwd_workspace* ws, int32_t mode);
  *                launch, each wave on the colliding pairs it found (bodies
  *                of at most 4 * gjk_group vertices, forward steps); 0: a
  *                separate EPA launch over all contacts.
+ *   fill_cells   cells per block of the pair emission: 64 (default), 16 or
+ *                4 (fewer for beds packed many hulls to a cell).
  * mwd_set_tuning applies to later steps (graphs captured before keep their
  * launches) and returns MWD_E_INVALID for a choice out of range. */
 #define MWD_CELLS_TABLE 0
@@ -215,6 +230,7 @@ typedef struct mwd_tuning {
     int32_t vertex_pack;
     int32_t epa_blocks;
     int32_t epa_fused;
+    int32_t fill_cells;
 } mwd_tuning;
 int mwd_get_tuning(const mwd_workspace* ws, mwd_tuning* out);
 int mwd_set_tuning(mwd_workspace* ws, const mwd_tuning* t);

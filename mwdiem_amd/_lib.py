@@ -92,7 +92,7 @@ class Tuning(C.Structure):
     _fields_ = [("cell_pass", C.c_int32), ("gjk_group", C.c_int32), ("gjk_blocks", C.c_int32),
                 ("sphere_block", C.c_int32), ("verlet_block", C.c_int32), ("cell_items", C.c_int32),
                 ("verlet_waves", C.c_int32), ("vertex_pack", C.c_int32), ("epa_blocks", C.c_int32),
-                ("epa_fused", C.c_int32)]
+                ("epa_fused", C.c_int32), ("fill_cells", C.c_int32)]
 
 
 HISTORY_RESET = 0
@@ -114,6 +114,7 @@ _SIGS = {
                                        C.c_int64, C.c_int64, C.c_double, _P]),
     "mwd_workspace_destroy": (C.c_int, [_P]),
     "mwd_set_wall_motion": (C.c_int, [_P, C.c_int32]),
+    "mwd_set_spawn": (C.c_int, [_P, C.c_int64, C.c_int64, C.c_int64]),
     "mwd_set_reduction": (C.c_int, [_P, C.c_int32]),
     "mwd_set_tangential_history": (C.c_int, [_P, C.c_int32]),
     "mwd_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),

@@ -2,6 +2,8 @@
 #include <math.h>
 #include <string.h>
 
+#include <stddef.h>
+
 #include <algorithm>
 #include <mutex>
 #include <string>
@@ -420,7 +422,7 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     A(scan_tmp, ws->scan_tmp_len);
     ws->scan_tmp_side_len = mwd_scan_tmp_len(std::max<int64_t>(ws->n_wall_tests, 1));
     A(scan_tmp_side, ws->scan_tmp_side_len);
-    A(counters, 16);
+    A(counters, 20);
     A(flags, 1);
     A(laws, 2 * n_scenes);
     A(scene_wt_off, n_scenes + 1);
@@ -445,7 +447,7 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     if ((ws->cell_table_ok && (hipMemsetAsync(ws->cell_of, 0xFF, kbytes, s) != hipSuccess ||
                             mwd_fill_key_table(ws->key_cnt, ws->key_span_total, s) != MWD_OK)) ||
         hipMemsetAsync(ws->row_key, 0xFF, sizeof(int32_t) * n_rows, s) != hipSuccess ||
-        hipMemsetAsync(ws->counters, 0, sizeof(int64_t) * 16, s) != hipSuccess ||
+        hipMemsetAsync(ws->counters, 0, sizeof(int64_t) * 20, s) != hipSuccess ||
         hipMemsetAsync(ws->rec_cnt, 0, sizeof(int32_t) * n_rows, s) != hipSuccess ||
         hipMemsetAsync(ws->scan_tmp, 0, sizeof(int64_t) * ws->scan_tmp_len, s) != hipSuccess ||
         hipMemsetAsync(ws->scan_tmp_side, 0, sizeof(int64_t) * ws->scan_tmp_side_len, s) != hipSuccess ||
@@ -454,6 +456,17 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
         mwd_set_error("mwd_workspace_create: initialisation failed");
         free_ws(ws);
         return MWD_E_HIP;
+    }
+    // every moving row and wall slab test is live until a spawn schedule says otherwise
+    ws->m_live = ws->counters + 14;
+    ws->wt_live = ws->counters + 15;
+    {
+        const int64_t live[2] = {M, ws->n_wall_tests};
+        if (hipMemcpy(ws->m_live, live, sizeof(live), hipMemcpyHostToDevice) != hipSuccess) {
+            mwd_set_error("mwd_workspace_create: initialisation failed");
+            free_ws(ws);
+            return MWD_E_HIP;
+        }
     }
     // side stream (default priority) and the events that fork to it and join back
     if (hipStreamCreateWithFlags(&ws->side, hipStreamNonBlocking) != hipSuccess ||
@@ -483,6 +496,7 @@ MWD_API int mwd_get_tuning(const mwd_workspace* ws, mwd_tuning* out) {
     out->vertex_pack = ws->tune.vertex_pack;
     out->epa_blocks = ws->tune.epa_blocks;
     out->epa_fused = ws->tune.epa_fused;
+    out->fill_cells = ws->tune.fill_cells;
     return MWD_OK;
 }
 
@@ -491,7 +505,8 @@ MWD_API int mwd_set_tuning(mwd_workspace* ws, const mwd_tuning* t) {
         mwd_set_error("mwd_set_tuning: null argument");
         return MWD_E_INVALID;
     }
-    const bool pass_ok = t->cell_pass == MWD_CELLS_SORTED || (t->cell_pass == MWD_CELLS_TABLE && ws->cell_table_ok);
+    const bool pass_ok = (t->cell_pass == MWD_CELLS_SORTED && !ws->spawn_on) ||
+                         (t->cell_pass == MWD_CELLS_TABLE && ws->cell_table_ok);
     const bool group_ok = t->gjk_group == 4 || t->gjk_group == 8 || t->gjk_group == 16;
     const bool blocks_ok = t->gjk_blocks >= 64 && t->gjk_blocks <= 65536;
     const bool sphere_ok = t->sphere_block >= 64 && t->sphere_block <= 1024 && t->sphere_block % 64 == 0;
@@ -500,11 +515,13 @@ MWD_API int mwd_set_tuning(mwd_workspace* ws, const mwd_tuning* t) {
     const bool waves_ok = t->verlet_waves == 1 || t->verlet_waves == 3 || t->verlet_waves == 4;
     const bool pack_ok = t->vertex_pack == 0 || t->vertex_pack == 1;
     const bool epa_ok = t->epa_blocks >= 16 && t->epa_blocks <= 65536 && (t->epa_fused == 0 || t->epa_fused == 1);
+    const bool fill_ok = t->fill_cells == 4 || t->fill_cells == 16 || t->fill_cells == 64;
     if (!pass_ok || !group_ok || !blocks_ok || !sphere_ok || !verlet_ok || !items_ok || !waves_ok || !pack_ok ||
-        !epa_ok) {
-        mwd_set_error(!pass_ok ? "mwd_set_tuning: cell_pass unknown, or MWD_CELLS_TABLE without key tables"
+        !epa_ok || !fill_ok) {
+        mwd_set_error(!pass_ok ? "mwd_set_tuning: cell_pass unknown, MWD_CELLS_TABLE without key tables, or "
+                                 "MWD_CELLS_SORTED with a spawn schedule"
                                : "mwd_set_tuning: gjk_group, gjk_blocks, sphere_block, verlet_block, cell_items, "
-                                 "verlet_waves, vertex_pack, epa_blocks or epa_fused out of range");
+                                 "verlet_waves, vertex_pack, epa_blocks, epa_fused or fill_cells out of range");
         return MWD_E_INVALID;
     }
     ws->cell_table = t->cell_pass == MWD_CELLS_TABLE;
@@ -517,6 +534,7 @@ MWD_API int mwd_set_tuning(mwd_workspace* ws, const mwd_tuning* t) {
     ws->tune.vertex_pack = t->vertex_pack;
     ws->tune.epa_blocks = t->epa_blocks;
     ws->tune.epa_fused = t->epa_fused;
+    ws->tune.fill_cells = t->fill_cells;
     return MWD_OK;
 }
 
@@ -698,8 +716,65 @@ MWD_API int mwd_reduce_verlet(mwd_workspace* ws, mwd_state* st) {
     return mwd_launch_reduce_verlet(ws, st, nullptr, contact_out(ws, false));
 }
 
+namespace {
+// Spawn schedule (MWDiem.py:685-714 creates particles every 5th step, at the
+// start of the step): at the start of every spawn_every-th step, per more of
+// the scene's rows become active, up to cap; the live counts that the
+// kernels read follow. One thread.
+__global__ void k_spawn(SceneDev* sc, int64_t* counters, int64_t every, int64_t per, int64_t cap) {
+    if (threadIdx.x != 0) return;
+    const int64_t i = counters[16];
+    counters[16] = i + 1;
+    if (i % every != 0) return;
+    int64_t n = sc->n_active;
+    if (n >= cap) return;
+    n = n + per < cap ? n + per : cap;
+    sc->n_active = n;
+    counters[14] = n - sc->n_walls;
+    counters[15] = sc->n_walls * (n - sc->n_walls);
+}
+
+int spawn_begin(mwd_workspace* ws) {
+    if (!ws->spawn_on) return MWD_OK;
+    hipLaunchKernelGGL(k_spawn, dim3(1), dim3(64), 0, ws->stream, ws->scenes_dev, ws->counters, ws->spawn_every,
+                       ws->spawn_per, ws->spawn_cap);
+    MWD_LAUNCH_CHECK();
+    return MWD_OK;
+}
+}  // namespace
+
+MWD_API int mwd_set_spawn(mwd_workspace* ws, int64_t n_active, int64_t every, int64_t per) {
+    if (!ws || ws->n_scenes != 1 || every < 1 || per < 1) {
+        mwd_set_error("mwd_set_spawn: a workspace of one scene, every >= 1 and per >= 1");
+        return MWD_E_INVALID;
+    }
+    const mwd_scene& sc = ws->scenes_host[0];
+    const int64_t cap = sc.n_active;  // the scene's rows as created
+    if (n_active < sc.n_walls || n_active > cap) {
+        mwd_set_error("mwd_set_spawn: n_active outside [n_walls, the scene's rows]");
+        return MWD_E_INVALID;
+    }
+    if (!ws->cell_table_ok) {
+        mwd_set_error("mwd_set_spawn: needs the key-table cell pass");
+        return MWD_E_INVALID;
+    }
+    const int64_t live[3] = {n_active - sc.n_walls, sc.n_walls * (n_active - sc.n_walls), 0};
+    MWD_HIP_CHECK(hipStreamSynchronize(ws->stream));
+    MWD_HIP_CHECK(hipMemcpy(ws->m_live, live, sizeof(live), hipMemcpyHostToDevice));
+    MWD_HIP_CHECK(hipMemcpy((char*)ws->scenes_dev + offsetof(SceneDev, n_active), &n_active, sizeof(int64_t),
+                            hipMemcpyHostToDevice));
+    ws->spawn_on = 1;
+    ws->spawn_every = every;
+    ws->spawn_per = per;
+    ws->spawn_cap = cap;
+    ws->cell_table = true;
+    return MWD_OK;
+}
+
 MWD_API int mwd_step(mwd_workspace* ws, mwd_state* st) {
-    int rc = mwd_candidates(ws, st, nullptr);
+    int rc = spawn_begin(ws);
+    if (rc) return rc;
+    rc = mwd_candidates(ws, st, nullptr);
     if (rc) return rc;
     rc = mwd_fine_contact(ws, st);
     if (rc) return rc;
@@ -826,7 +901,9 @@ MWD_API int mwd_step_tangent(mwd_workspace* ws, mwd_state* st, double* const* ta
             mwd_set_error("mwd_step_tangent: tangent array " + std::to_string(i) + " is NULL");
             return MWD_E_INVALID;
         }
-    int rc = mwd_candidates(ws, st, nullptr);
+    int rc = spawn_begin(ws);
+    if (rc) return rc;
+    rc = mwd_candidates(ws, st, nullptr);
     if (rc) return rc;
     rc = fine_contact(ws, st, tan);
     if (rc) return rc;
@@ -843,6 +920,10 @@ MWD_API int mwd_step_tangent(mwd_workspace* ws, mwd_state* st, double* const* ta
 MWD_API int mwd_step_adjoint(mwd_workspace* ws, mwd_state* st, double* const* lam, double* dtheta) {
     if (!ws || !st || !lam || !dtheta || st->vmax != ws->vmax) {
         mwd_set_error("mwd_step_adjoint: invalid argument");
+        return MWD_E_INVALID;
+    }
+    if (ws->spawn_on) {
+        mwd_set_error("mwd_step_adjoint: not with a spawn schedule");
         return MWD_E_INVALID;
     }
     for (int i = 0; i < MWD_N_TAN_ARRAYS; ++i)

@@ -84,12 +84,13 @@ __global__ void k_sort_keys(const double* __restrict__ com, const int64_t* __res
 constexpr int32_t KEY_MIN_EMPTY = 0x7F7F7F7F;  // above every row: the mark of an empty key
 
 __global__ void k_key_count(const double* __restrict__ com, const int64_t* __restrict__ keys_in,
-                            const int32_t* __restrict__ plist, int64_t M, const int32_t* __restrict__ scene_of,
+                            const int32_t* __restrict__ plist, int64_t M_cap, const int64_t* m_live,
+                            const int32_t* __restrict__ scene_of,
                             const SceneDev* __restrict__ scenes, int32_t* __restrict__ row_key,
                             int32_t* __restrict__ key_cnt, int32_t* __restrict__ key_min, uint32_t* flags,
                             const double* __restrict__ radius, double* __restrict__ sph) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M) return;
+    if (i >= (*m_live < M_cap ? *m_live : M_cap)) return;
     const int32_t row = plist[i];
     if (sph) pack_sphere(sph, com, radius, row);
     const uint32_t k = global_key(com, keys_in, row, scenes[scene_of[row]], flags);
@@ -101,11 +102,12 @@ __global__ void k_key_count(const double* __restrict__ com, const int64_t* __res
 // Key-table path: each particle takes a slot in its cell's member range. The
 // decrements bring the key counts back to zero; slots come in no set order,
 // so the cell pass sorts its members.
-__global__ void k_key_scatter(const int32_t* __restrict__ plist, int64_t M, const int32_t* __restrict__ row_key,
+__global__ void k_key_scatter(const int32_t* __restrict__ plist, int64_t M_cap, const int64_t* m_live,
+                              const int32_t* __restrict__ row_key,
                               int32_t* __restrict__ key_cnt, const int32_t* __restrict__ cell_of,
                               const int32_t* __restrict__ cell_start, int32_t* __restrict__ members) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M) return;
+    if (i >= (*m_live < M_cap ? *m_live : M_cap)) return;
     const int32_t row = plist[i];
     const int32_t k = row_key[row];
     const int32_t slot = atomicSub(&key_cnt[2 * k], 1) - 1;
@@ -334,10 +336,14 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t live) {
 // (LIBRARY_Coarse_contact_detection.py:84-118): same-cell pairs (j < k,
 // members in set order), then per neighbour offset o every member j against
 // every member k of that cell.
-constexpr int FILL_CELLS = 64;    // cells per block
+// The cells a block takes (FILL_CELLS, a launch choice: 64, 16 or 4) set the
+// balance: with about one hull per cell 64 cells give a block a few hundred
+// pairs; a packed bed holds many hulls per cell and thousands of pairs per
+// cell, and fewer cells per block spread them over more blocks.
 constexpr int FILL_THREADS = 256;
 constexpr int FILL_SURV = 2048;   // survivors gathered in LDS before a global append
 
+template <int FILL_CELLS>
 __global__ void __launch_bounds__(FILL_THREADS) k_cell_fill(const int32_t* __restrict__ perm,
                                                           const int2* __restrict__ cell_self,
                                                           const int2* __restrict__ cell_seg,
@@ -709,7 +715,7 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
                         ws->pairs, ws->pair_cap, st->radius ? ws->surv : nullptr};
         MWD_HIP_CHECK(hipEventRecord(ws->ev_fork, s));
         MWD_HIP_CHECK(hipStreamWaitEvent(ws->side, ws->ev_fork, 0));
-        rc = mwd_scan_io(io, ws->n_wall_tests, nullptr, ctr + 1, ws->scan_tmp_side, ws->scan_tmp_side_len,
+        rc = mwd_scan_io(io, ws->n_wall_tests, ws->wt_live, ctr + 1, ws->scan_tmp_side, ws->scan_tmp_side_len,
                          ws->side);
         if (rc) return rc;
         MWD_HIP_CHECK(hipEventRecord(ws->ev_join, ws->side));
@@ -727,7 +733,7 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
         mwd_mark(ws, 1);
         if (ws->cell_table) {
             hipLaunchKernelGGL(k_key_count, dim3(blocks_for(M)), dim3(256), 0, s, st->com, keys_in, ws->plist, M,
-                               ws->scene_of, ws->scenes_dev, ws->row_key, ws->key_cnt, ws->key_min, ws->flags,
+                               ws->m_live, ws->scene_of, ws->scenes_dev, ws->row_key, ws->key_cnt, ws->key_min, ws->flags,
                                st->radius, st->radius ? ws->sph : nullptr);
             MWD_LAUNCH_CHECK();
             mwd_mark(ws, 2);
@@ -735,7 +741,8 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
                                          ws->cell_of, ctr + 0},
                              ws->n_rows, nullptr, nullptr, ws->scan_tmp, ws->scan_tmp_len, s);
             if (rc) return rc;
-            hipLaunchKernelGGL(k_key_scatter, dim3(blocks_for(M)), dim3(256), 0, s, ws->plist, M, ws->row_key,
+            hipLaunchKernelGGL(k_key_scatter, dim3(blocks_for(M)), dim3(256), 0, s, ws->plist, M, ws->m_live,
+                               ws->row_key,
                                ws->key_cnt, ws->cell_of, ws->cell_start, ws->vals);
             MWD_LAUNCH_CHECK();
             if ((rc = join())) return rc;
@@ -795,10 +802,23 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
             if (rc) return rc;
         }
         mwd_mark(ws, 3);
-        MWD_LAUNCH_EV(mwd_kev(ws, MWD_KT_EMIT, 0), mwd_kev(ws, MWD_KT_EMIT, 1), k_cell_fill,
-                      dim3((unsigned)((M + FILL_CELLS - 1) / FILL_CELLS)), dim3(FILL_THREADS), 0, s, ws->perm,
-                      ws->cell_self, ws->cell_seg, ws->cell_off, ctr, ws->pairs, ws->pair_cap,
-                      st->radius ? ws->sph : nullptr, ws->surv, ctr + 10, ws->collision);
+        const hipEvent_t e0 = mwd_kev(ws, MWD_KT_EMIT, 0), e1 = mwd_kev(ws, MWD_KT_EMIT, 1);
+        double* sph = st->radius ? ws->sph : nullptr;
+#define MWD_FILL(CELLS)                                                                                      \
+    MWD_LAUNCH_EV(e0, e1, k_cell_fill<CELLS>, dim3((unsigned)((M + (CELLS)-1) / (CELLS))), dim3(FILL_THREADS), \
+                  0, s, ws->perm, ws->cell_self, ws->cell_seg, ws->cell_off, ctr, ws->pairs, ws->pair_cap, sph, \
+                  ws->surv, ctr + 10, ws->collision)
+        switch (ws->tune.fill_cells) {
+            case 4:
+                MWD_FILL(4);
+                break;
+            case 16:
+                MWD_FILL(16);
+                break;
+            default:
+                MWD_FILL(64);
+        }
+#undef MWD_FILL
         MWD_LAUNCH_CHECK();
     }
     if ((rc = join())) return rc;

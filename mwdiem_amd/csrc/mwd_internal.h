@@ -132,6 +132,7 @@ struct PairTune {
     int vertex_pack = 0;     // vertex pass over real vertices only (1) or over every vmax slot (0)
     int epa_blocks = 2048;   // grid cap of the EPA launch (blocks of 4 waves, one contact per wave)
     int epa_fused = 1;       // EPA and the law inside the GJK launch (1) or as their own launch (0)
+    int fill_cells = 64;     // cells per block of the pair emission: 64, 16 or 4
 };
 
 // Face distances at or above this are EPA's placeholders (1e30 for a face
@@ -302,8 +303,16 @@ struct mwd_workspace {
     // (capped), 4 n_contacts, 5 n_records, 6 unused,
     // 7 n_pairs before capping, 8 simplex slots claimed this step, 9 radix
     // histogram entries (scratch), 10 sphere-test survivors, 11 bucket
-    // overflow records, 12-13 slots filled in the tangential history tables
+    // overflow records, 12-13 slots filled in the tangential history tables,
+    // 14 moving rows active now (m_live), 15 wall slab tests now (wt_live),
+    // 16 steps taken since the spawn schedule was set
     int64_t* counters;
+    int64_t* m_live;           // counters + 14: every kernel over the moving rows reads its count here
+    int64_t* wt_live;          // counters + 15
+    // spawning (one scene): rows [row0 + n_active, spawn_cap) are activated
+    // spawn_per at a time every spawn_every steps, at the start of a step
+    int spawn_on;
+    int64_t spawn_every, spawn_per, spawn_cap;
     uint32_t* flags;
     mwd_law* laws;             // [2*n_scenes] (particle law, wall law)
     int64_t* scene_wt_off;     // [n_scenes+1] wall-test prefix (device)

@@ -325,7 +325,8 @@ __device__ __forceinline__ void reduce_verlet_row(const VState& a, const int32_t
 // register allocation must allow (1 leaves the compiler's choice).
 template <int VB, int WPE, bool PACK>
 __global__ void __launch_bounds__(VB) __attribute__((amdgpu_waves_per_eu(WPE)))
-k_reduce_verlet_v(VState a, const int32_t* __restrict__ plist, int64_t M, const int32_t* __restrict__ scene_of,
+k_reduce_verlet_v(VState a, const int32_t* __restrict__ plist, int64_t M_cap, const int64_t* m_live,
+                  const int32_t* __restrict__ scene_of,
                   const SceneDev* __restrict__ scenes, int32_t* __restrict__ rec_cnt,
                   const int32_t* __restrict__ rec_bkt, const int32_t* __restrict__ rec_ovf, const int64_t* n_ovf,
                   int64_t ovf_cap, ContactOut co, const double* __restrict__ fext, const double* __restrict__ text,
@@ -335,6 +336,8 @@ k_reduce_verlet_v(VState a, const int32_t* __restrict__ plist, int64_t M, const 
     __shared__ unsigned long long r2bits[VB];
     __shared__ int vbase[PACK ? VB : 1];
     const int64_t first = (int64_t)blockIdx.x * VB;
+    const int64_t M = *m_live < M_cap ? *m_live : M_cap;  // the moving rows active now
+    if (first >= M) return;  // the whole block: none of its rows is active
     const int nb = (int)(M - first < VB ? M - first : VB);
     const int64_t i = first + threadIdx.x;
     if (i < M) {
@@ -347,7 +350,8 @@ k_reduce_verlet_v(VState a, const int32_t* __restrict__ plist, int64_t M, const 
 
 // The tangent path: one lane per particle, vertices and their tangents
 // updated in the lane's own loop.
-__global__ void __launch_bounds__(256) k_reduce_verlet_t(VState a, const int32_t* __restrict__ plist, int64_t M,
+__global__ void __launch_bounds__(256) k_reduce_verlet_t(VState a, const int32_t* __restrict__ plist, int64_t M_cap,
+                                                         const int64_t* m_live,
                                                          const int32_t* __restrict__ scene_of,
                                                          const SceneDev* __restrict__ scenes,
                                                          int32_t* __restrict__ rec_cnt,
@@ -358,6 +362,7 @@ __global__ void __launch_bounds__(256) k_reduce_verlet_t(VState a, const int32_t
                                                          const double* __restrict__ text,
                                                          const int32_t* __restrict__ nverts, int vmax, double dt) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t M = *m_live < M_cap ? *m_live : M_cap;
     if (i >= M) return;
     reduce_verlet_row<Dual<MWD_NT>>(a, plist, i, scene_of, scenes, rec_cnt, rec_bkt, rec_ovf, n_ovf, ovf_cap, co,
                                     fext, text,
@@ -438,7 +443,8 @@ template <int VB, int WPE, bool PACK>
 static void launch_reduce_verlet_v(mwd_workspace* ws, const mwd_state* st, const VState& a, const ContactOut& co) {
     const int64_t M = ws->n_moving;
     MWD_LAUNCH_EV(mwd_kev(ws, MWD_KT_VERLET, 0), mwd_kev(ws, MWD_KT_VERLET, 1), (k_reduce_verlet_v<VB, WPE, PACK>),
-                  dim3((unsigned)((M + VB - 1) / VB)), dim3(VB), 0, ws->stream, a, ws->plist, M, ws->scene_of,
+                  dim3((unsigned)((M + VB - 1) / VB)), dim3(VB), 0, ws->stream, a, ws->plist, M, ws->m_live,
+                  ws->scene_of,
                   ws->scenes_dev, ws->rec_cnt, ws->rec_bkt, ws->rec_ovf, ws->counters + 11, 2 * ws->contact_cap, co,
                   st->fext, st->text, st->nverts, (int)st->vmax, ws->dt, ws->reduce_fp32);
 }
@@ -471,7 +477,7 @@ int mwd_launch_reduce_verlet(mwd_workspace* ws, mwd_state* st, double* const* ta
     const int64_t M = ws->n_moving;
     if (tan)
         hipLaunchKernelGGL(k_reduce_verlet_t, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, ws->stream, a,
-                           ws->plist, M, ws->scene_of, ws->scenes_dev, ws->rec_cnt, ws->rec_bkt, ws->rec_ovf,
+                           ws->plist, M, ws->m_live, ws->scene_of, ws->scenes_dev, ws->rec_cnt, ws->rec_bkt, ws->rec_ovf,
                            ws->counters + 11, 2 * ws->contact_cap, co, st->fext, st->text, st->nverts, (int)st->vmax,
                            ws->dt);
     else

@@ -54,14 +54,17 @@ def _with_theta(sc: SceneData, theta) -> SceneData:
     return out
 
 
-def form_pile(sc: SceneData, v0: np.ndarray | None = None, steps: int = 20_000, dt: float = 2e-6,
-              steps_per_graph: int = 50) -> dict:
-    """Let the scene's particles fall and settle into a pile (its own law, a
-    coarser time step dt) and return the state (numpy arrays of the state
-    keys) to start runs from."""
+def form_pile(sc: SceneData, v0: np.ndarray | None = None, steps: int = 20_000, dt: float | None = None,
+              steps_per_graph: int = 50, spawn=None) -> dict:
+    """Let the scene's particles fall and settle into a pile (its own law;
+    dt, if given, replaces the scene's time step) and return the state (numpy
+    arrays of the state keys) to start runs from. With spawn (scene.Spawn,
+    e.g. from scene.config_pour) the particles enter on that schedule: the
+    pile is poured."""
     f = sc.copy()
-    f.dt = dt
-    S = Stepper(f)
+    if dt is not None:
+        f.dt = dt
+    S = Stepper(f, spawn=spawn)
     if v0 is not None:
         S.set_state({"v": v0, "v12": v0})
     S.capture(steps_per_graph)

@@ -415,10 +415,13 @@ def grid_for(box: np.ndarray, max_particlesize: float):
 def make_scene(n_particles: int, box_mm, seed: int, npts=(12, 12), axes=(1.0, 0.8, 0.6), size=(0.8, 1.2),
                density: float = RHO_KG_MM3, law: Law4 | None = None, wall_law: Law4 | None = None,
                dt: float = 1e-6, g=(0.0, 0.0, -G_MM), wall_thickness_factor: float = 10.0,
-               spacing_factor: float = 1.05, ref_inertia: bool = False, name: str = "") -> SceneData:
+               spacing_factor: float = 1.05, ref_inertia: bool = False, name: str = "",
+               site_fn=None) -> SceneData:
     """A bed of `n_particles` hulls of npts[0]..npts[1] points on an ellipsoid
     with semi-axes axes*U(size) in a walled box (the configurations' generator).
-    The cell size (max_particlesize) is 2x the largest circumradius."""
+    The cell size (max_particlesize) is 2x the largest circumradius.
+    site_fn(rmax, rng) -> [n, 3], if given, places the particles instead of
+    the jittered lattice."""
     rng = np.random.Generator(np.random.PCG64(seed))
     law = law or Law4()
     wall_law = wall_law or law
@@ -427,13 +430,16 @@ def make_scene(n_particles: int, box_mm, seed: int, npts=(12, 12), axes=(1.0, 0.
     rmax = max(float(np.max(np.linalg.norm(p[0], axis=1))) for p in parts) if parts else 1.0
     ms = 2.0 * rmax
     spacing = spacing_factor * ms
-    try:
-        sites = bcc_sites(box_mm, spacing, rmax, n_particles)
-        jit = (spacing - ms) / 2.0 / math.sqrt(3.0)
-        sites = sites + rng.uniform(-jit, jit, size=sites.shape)
-    except ValueError:
-        # too tight for bounding spheres: deposit with an exact separation test
-        sites = deposit_sites([p[0] for p in parts], [p[5] for p in parts], box_mm, rng)
+    if site_fn is not None:
+        sites = site_fn(rmax, rng)
+    else:
+        try:
+            sites = bcc_sites(box_mm, spacing, rmax, n_particles)
+            jit = (spacing - ms) / 2.0 / math.sqrt(3.0)
+            sites = sites + rng.uniform(-jit, jit, size=sites.shape)
+        except ValueError:
+            # too tight for bounding spheres: deposit with an exact separation test
+            sites = deposit_sites([p[0] for p in parts], [p[5] for p in parts], box_mm, rng)
     T = wall_thickness_factor * ms
     walls = walls_for_box(box_mm, T, density, ref_inertia)
     W = len(walls)
@@ -499,6 +505,56 @@ def config_pile(seed: int = 5, n: int = 4096, v_pour: float = 1000.0, law: Law4 
     v = np.zeros((sc.n_rows, 3))
     v[sc.n_walls:] = rng.normal(0.0, v_pour / math.sqrt(3.0), size=(n, 3))
     return sc, v
+
+
+@dataclass
+class Spawn:
+    """A spawn schedule: rows [n_walls, n_active) take part from the start;
+    at the start of every `every`-th step (the first included) `per` more
+    rows join, in row order (MWDiem.py:685-714 creates particles at the start
+    of every 5th step)."""
+    n_active: int
+    every: int
+    per: int
+
+    def active_at(self, step: int, n_rows: int) -> int:
+        """Rows [0, n) active during step `step` (counted from 0)."""
+        return min(n_rows, self.n_active + (step // self.every + 1) * self.per)
+
+
+def config_pour(seed: int = 5, n: int = 4096, L=(40.0, 40.0, 60.0), v_pour: float = 2000.0, every: int = 1000,
+                law: Law4 | None = None, npts=(8, 20), size=(0.6, 0.9), dt: float = 1e-6):
+    """A pile formed by pouring: n hulls (the pile's generator: 8-20 points on
+    ellipsoids of semi-axes (1.0, 0.8, 0.6) x U(0.6, 0.9) mm) enter the top of
+    an L mm box in batches of one layer -- a square grid of sites 2.1 x the
+    largest circumradius apart, 1.5 circumradii under the ceiling, random
+    orientation, a small jitter -- moving down at v_pour mm/s (2 m/s by
+    default); a batch every `every` steps, by which time the previous one has
+    fallen clear of the layer. Physical law Kn=1e4, en=et=0.5, mu=0.3.
+    Returns (scene, initial velocities [n_rows, 3], Spawn schedule): rows
+    wait in place until their batch spawns."""
+    law = law or Law4(Kn=1e4, en=0.5, et=0.5, mu=0.3, mode=PHYSICAL)
+    Lx, Ly, Lz = L
+    batch = {}
+
+    def sites(rmax, rng):
+        a = 2.1 * rmax
+        xs = np.arange(rmax * 1.1, Lx - rmax * 1.1 + 1e-9, a)
+        ys = np.arange(rmax * 1.1, Ly - rmax * 1.1 + 1e-9, a)
+        X, Y = np.meshgrid(xs, ys, indexing="ij")
+        layer = np.stack([X.ravel(), Y.ravel(), np.full(X.size, Lz - 1.5 * rmax)], axis=1)
+        B = layer.shape[0]
+        batch["per"] = B
+        out = np.concatenate([layer[rng.permutation(B)] for _ in range((n + B - 1) // B)])[:n]
+        jit = 0.04 * rmax
+        return out + rng.uniform(-jit, jit, size=out.shape)
+
+    sc = make_scene(n, L, seed, npts=npts, size=size, law=law, dt=dt, site_fn=sites, name=f"pour-{n}")
+    rng = np.random.Generator(np.random.PCG64([seed, 12]))
+    v = np.zeros((sc.n_rows, 3))
+    v[sc.n_walls:] = rng.normal(0.0, 0.02 * v_pour, size=(n, 3))
+    v[sc.n_walls:, 2] -= v_pour
+    return sc, v, Spawn(n_active=sc.n_walls, every=every, per=batch["per"])
 
 
 def compact_dense(sc: SceneData, gap: float = -0.02, seed: int = 0) -> SceneData:

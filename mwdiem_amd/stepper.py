@@ -32,12 +32,14 @@ class Stepper:
     """Device-resident DEM step over one or more scenes.
 
     pair_cap / contact_cap bound the candidate and contact buffers; a step
-    that exceeds them raises (after the step) from check().
+    that exceeds them raises (after the step) from check(). spawn (a
+    scene.Spawn, one scene only): the scene's rows join the run on that
+    schedule, counted on the device from the first step.
     """
 
     def __init__(self, scenes: SceneData | Sequence[SceneData], device: int = 0, pair_cap: int | None = None,
                  contact_cap: int | None = None, stream: torch.cuda.Stream | None = None, sphere_test: bool = True,
-                 reduce_fp32: bool = False, persistent_history: bool = False):
+                 reduce_fp32: bool = False, persistent_history: bool = False, spawn=None):
         _require_gpu()
         if isinstance(scenes, SceneData):
             scenes = [scenes]
@@ -121,6 +123,10 @@ class Stepper:
             _lib.check(lib.mwd_set_reduction(ws, _lib.REDUCE_FP32), "mwd_set_reduction")
         if persistent_history:  # keep each contact's tangential displacement (the reference drops it, Q1)
             _lib.check(lib.mwd_set_tangential_history(ws, _lib.HISTORY_PERSISTENT), "mwd_set_tangential_history")
+        if spawn is not None:  # rows join on the device's schedule (scene.Spawn; one scene)
+            _lib.check(lib.mwd_set_spawn(ws, int(self.row0[0]) + int(spawn.n_active), int(spawn.every),
+                                         int(spawn.per)), "mwd_set_spawn")
+        self.spawn = spawn
         self.t["radius"] = torch.zeros(self.n_rows, dtype=torch.float64, device=dev)
         self.state = _lib.State()
         self._fill_state()
@@ -219,7 +225,7 @@ class Stepper:
 
     def tuning(self) -> dict:
         """Launch choices of the workspace (cell_pass, gjk_group, gjk_blocks, sphere_block, verlet_block,
-        cell_items, verlet_waves, vertex_pack, epa_blocks, epa_fused)."""
+        cell_items, verlet_waves, vertex_pack, epa_blocks, epa_fused, fill_cells)."""
         t = _lib.Tuning()
         _lib.check(_lib.load().mwd_get_tuning(self.ws, C.byref(t)), "mwd_get_tuning")
         return {name: int(getattr(t, name)) for name, _ in _lib.Tuning._fields_}

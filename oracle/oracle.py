@@ -267,9 +267,12 @@ def verlet(st, F, T, sc, pmin, pmax, nrot=None, rot_mode=None):
                      C.c_int64(pmin), C.c_int64(pmax), C.c_int(rot_mode))
 
 
-def candidates(st, sc) -> np.ndarray:
-    """Wall slab pairs then Linked_cell_hash pairs (MWDiem.py:807-817)."""
-    W, N = sc.n_walls, sc.n_rows
+def candidates(st, sc, n_active: int | None = None) -> np.ndarray:
+    """Wall slab pairs then Linked_cell_hash pairs (MWDiem.py:807-817) over
+    the active rows [n_walls, n_active) (num_particles_temp; all rows by
+    default)."""
+    W = sc.n_walls
+    N = sc.n_rows if n_active is None else n_active
     wp = wall_pairs(st["com"], W, N, sc.wall_n, st.get("wall_c", sc.wall_c), sc.wall_thr)
     keys = cell_keys(st["com"], sc.cell, sc.ncx, sc.ncy)
     hp = linked_cell_hash(keys, W, N, sc.ncx, sc.ncy, stencil=int(getattr(sc.law_p, "mode", 0)))
@@ -283,13 +286,16 @@ def pair_keys(pairs) -> np.ndarray:
     return a + b * b
 
 
-def step(st, sc, history: dict | None = None):
+def step(st, sc, history: dict | None = None, n_active: int | None = None):
     """One iteration of the reference time loop (MWDiem.py:805-1001) in place;
     returns (pairs, collision). With `history` (a dict key -> [4]), the
     tangential history persists: each pair reads its entry and the dict is
     replaced by this step's contacts' new displacements (the history the
-    driver intends; as it runs, it drops every row, Q1)."""
-    pairs = candidates(st, sc)
+    driver intends; as it runs, it drops every row, Q1). n_active: rows
+    [n_walls, n_active) take part (num_particles_temp, MWDiem.py:685-714,
+    808-817, 1001); all rows by default."""
+    N = sc.n_rows if n_active is None else n_active
+    pairs = candidates(st, sc, N)
     if history is None:
         coll, _, _, _, _, F, T, _ = contact_state(pairs, st, sc)
     else:
@@ -304,7 +310,7 @@ def step(st, sc, history: dict | None = None):
             history.setdefault(int(keys[i]), disp[i].copy())
     F = F + sc.fext
     T = T + sc.text
-    verlet(st, F, T, sc, sc.n_walls, sc.n_rows, nrot=sc.nverts)
+    verlet(st, F, T, sc, sc.n_walls, N, nrot=sc.nverts)
     if getattr(sc, "wall_v", None) is not None:
         move_walls(st, sc)
     return pairs, coll
