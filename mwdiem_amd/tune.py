@@ -4,8 +4,9 @@ The knobs are those of mwd_tuning (include/mwdiem_hip.h): how the cells are
 built (key tables or a radix sort of the keys), GJK lanes per pair (so pairs
 per wave), the GJK grid cap, the sphere-test block, the particles per block
 of the fused force reduction + integrator, the occupancy asked of its
-register allocation and whether its vertex pass skips padding slots (the
-EPA grid cap is left to the caller, see below). None of them changes a bit
+register allocation, whether its vertex pass skips padding slots and the
+cells per block of the pair emission (the EPA grid cap is left to the
+caller, see below). None of them changes a bit
 of the results, so the tuner times them on the live state, restoring it
 before every trial, and leaves the stepper on the fastest choice.
 
@@ -38,6 +39,7 @@ SPACE = {
     "cell_items": (0, 1, 2, 4),
     "verlet_waves": (1, 3, 4),
     "vertex_pack": (0, 1),
+    "fill_cells": (64, 16, 4),
 }
 # Not searched: epa_blocks (the EPA grid cap) and epa_fused (EPA inside the
 # GJK launch). Their best values follow the contact count, which grows over
