@@ -84,25 +84,233 @@ def measured_traffic(kernel: str, workload: str):
     return None
 
 
-def cpu_baseline(sc, budget_s: float = 12.0):
-    """The oracle (FP64 C++ port of the reference path, one core) on the same
-    scene, for as many whole steps as fit in about budget_s seconds."""
+def host_cpu() -> dict:
+    """Host cores as the CPU baseline sees them: logical CPUs of the machine,
+    those this process may run on, and the model name."""
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "model": model}
+
+
+def cpu_baseline(sc, budget_s: float = 12.0, gjk_samples: dict | None = None):
+    """The oracle (the FP64 C++ restatement of the reference path, OpenMP over
+    pairs and particles, contact sums in pair order) on the same scene, for
+    as many whole steps as fit in about budget_s seconds. It also counts, with
+    its instrumented double (add/sub/mul/div/sqrt = 1 flop), the FP64 work of
+    GJK, EPA and the contact law on each of gjk_samples' pair lists -- the
+    pairs the GPU's GJK launch processed (sphere-test survivors) and their
+    state -- the algorithmic flops of that launch."""
+    import ctypes
+
     from oracle import oracle as O
+    O.lib().orc_threads.restype = ctypes.c_int
+    threads = int(O.lib().orc_threads())
     st = O.initial_state(sc)
     t0 = time.perf_counter()
     n = 0
     while True:
         O.step(st, sc)
         n += 1
-        if time.perf_counter() - t0 > budget_s or n >= 50:
+        if time.perf_counter() - t0 > budget_s or n >= 200:
+            break
+    dt = time.perf_counter() - t0
+    out = {"value": sc.n_particles * n / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+           "sample": f"{n} full steps of the same {sc.n_particles}-particle scene from its initial state "
+                     f"(oracle/mwdiem_oracle.cpp, g++ -O2 -fopenmp, {threads} threads), {dt:.1f} s",
+           "host": host_cpu()}
+    flops = {}
+    for name, smp in (gjk_samples or {}).items():
+        t1 = time.perf_counter()
+        flops[name] = O.count_flops(smp["pairs"], smp["state"], smp["scene"])
+        print(f"# flops of {name}: {flops[name]} over {len(smp['pairs'])} pairs ({time.perf_counter() - t1:.1f} s)",
+              file=sys.stderr, flush=True)
+    return out, flops
+
+
+def cpu_baseline_python(sc, budget_s: float = 10.0):
+    """The reference's host loop in pure Python (oracle/ref_structure.py: dict
+    and set cell hash, per-pair gather, each pair's GJK/EPA and law one after
+    another, each particle's Verlet step), one core, on the small scene from
+    its initial state, for as many whole steps as fit in about budget_s."""
+    from oracle import oracle as O
+    from oracle import ref_structure as R
+    st = O.initial_state(sc)
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        R.step(st, sc)
+        n += 1
+        if time.perf_counter() - t0 > budget_s or n >= 1000:
             break
     dt = time.perf_counter() - t0
     return {"value": sc.n_particles * n / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{n} full steps of the same {sc.n_particles}-particle scene from its initial state "
-                      f"(oracle/mwdiem_oracle.cpp, g++ -O2, one thread), {dt:.1f} s"}
+            "sample": f"{n} of the 1,000 steps of the small scene ({sc.n_particles} hulls of 12 points), the "
+                      f"reference's host loop restated in pure Python (oracle/ref_structure.py), one core, "
+                      f"{dt:.1f} s", "host": host_cpu()}
 
 
-def bench_batched(n_scenes: int, steps: int, plain_steps: int, graph_steps: int, tune: bool = True):
+def gjk_sample(S, sc):
+    """The pair list the next step's GJK launch takes (candidate pairs of the
+    current state whose bounding spheres meet, wall pairs all), with the
+    state, for counting that launch's algorithmic flops."""
+    import numpy as np
+    S.candidates()
+    stt = S.check()
+    pairs = S.pairs()
+    st = S.numpy_state()
+    r = S.t["radius"].cpu().numpy()
+    nw = stt["n_wall_pairs"]
+    cell = pairs[nw:]
+    d = st["com"][cell[:, 1]] - st["com"][cell[:, 0]]
+    meet = np.einsum("ij,ij->i", d, d) <= (r[cell[:, 0]] + r[cell[:, 1]]) ** 2
+    surv = np.concatenate([pairs[:nw], cell[meet]]).astype(np.int32)
+    return {"pairs": surv, "state": st, "scene": sc, "candidates": int(pairs.shape[0])}
+
+
+def roofline_fp64(flops, launch_ms, kernel, workload, timed_over):
+    """FP64 roofline of the GJK launch: algorithmic flops per launch over its
+    mean duration against MI355X's FP64 vector peak; the PMC figures of the
+    same launch (FP64 VALU instructions per launch) when a summary exists."""
+    if flops is None or not launch_ms:
+        return None
+    ach = flops / (launch_ms * 1e-3) / 1e12
+    return {"bound": "fp64_valu", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": ach / FP64_PEAK_TFLOPS, "algorithmic_flops": flops, "kernel": kernel, "launch_ms": launch_ms,
+            "timed_over": timed_over, "pmc": measured_fp64(kernel, workload)}
+
+
+def measured_fp64(kernel: str, workload: str):
+    """FP64 VALU instructions per launch of `kernel` from the committed PMC
+    summary profiles/<kernel>_<workload>_fp64_pmc.md (lines 'name: value')."""
+    f = ROOT / "profiles" / f"{kernel}_{workload}_fp64_pmc.md"
+    if not f.exists():
+        return None
+    out = {}
+    for line in f.read_text().splitlines():
+        for key in ("fp64_instructions_per_launch", "fp64_lane_flops_per_launch", "dispatches"):
+            if line.startswith(key + ":"):
+                out[key] = float(line.split(":", 1)[1])
+    return out or None
+
+
+def plain_leg(S, sc, args, workload: str, label: str, tune_space=None):
+    """One plain-method measurement on a Stepper: warm-up, launch choices by
+    the autotuner, kernel durations over eager steps with start/stop events,
+    the pair list of the next GJK launch (for its flop count), then K
+    graph-replayed steps timed."""
+    import torch
+
+    from mwdiem_amd import _lib
+    for _ in range(args.warmup):
+        S.step(1)
+    S.check()
+    tuned = None
+    if not args.no_tune:
+        # launch choices by timing (results are bit-identical across them;
+        # the state is restored after the trials)
+        from mwdiem_amd.tune import autotune
+        tuned = autotune(S, steps=10, space=tune_space)
+        print(f"# {label} autotune: {tuned['tuning']} {tuned['ms_per_step']:.4f} ms/step (defaults "
+              f"{tuned['default_ms_per_step']:.4f}), {len(tuned['trials'])} trials", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    # Kernel durations: eager steps on the stepper's stream with HIP events --
+    # stage boundaries, and for the single-kernel stages start/stop events
+    # stamped with the kernel's own start and end (hipExtLaunchKernelGGL), so
+    # launch gaps are excluded.
+    n_time = min(args.steps, args.timing_steps)
+    S.set_timing(True)
+    S.step(n_time)
+    kt = S.kernel_times()
+    st = S.stage_times()
+    stats_t = S.check()  # counts of the last timed-kernel step
+    S.set_timing(False)
+    timing_src = f"{kt['steps']} eager steps after warm-up, kernel start/stop events"
+    sample = gjk_sample(S, sc)
+    if not args.no_graph:
+        S.capture(args.graph_steps)   # records graph_steps steps; capture runs nothing
+    # timed region: K full steps
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    S.run(args.steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    stats = S.check()
+    stage_ms = {k: v / max(st["steps"], 1) for k, v in st["ms"].items()}
+    kernel_ms = {k: v / max(kt["steps"], 1) for k, v in kt["ms"].items()}
+    sbytes = stage_bytes([sc], stats_t)
+    stage_of = dict(zip(_lib.KTIMED_NAMES, _lib.KTIMED_STAGE))
+    # the dominant kernel: the slowest of the single-kernel stages
+    kname = max(kernel_ms, key=kernel_ms.get)
+    dom = stage_of[kname]
+    ach = sbytes[dom] / (kernel_ms[kname] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": measured_traffic(kname, workload), "kernel": kname, "launch_ms": kernel_ms[kname],
+            "algorithmic_bytes": sbytes[dom], "timed_over": timing_src,
+            "kernel_ms": {k: round(v, 5) for k, v in kernel_ms.items()},
+            "kernel_gbs": {k: round(sbytes[stage_of[k]] / (v * 1e-3) / 1e9, 1) for k, v in kernel_ms.items() if v > 0}}
+    print(f"# {label} stages (ms/step): {json.dumps({k: round(v, 4) for k, v in stage_ms.items()})}",
+          file=sys.stderr)
+    print(f"# {label}: pairs {stats['n_pairs']} GJK pairs {stats['n_sphere_pass']} contacts {stats['n_contacts']} "
+          f"cells {stats['n_cells']}; {1e3 * wall / args.steps:.4f} ms/step", file=sys.stderr, flush=True)
+    return {"wall": wall, "stats": stats, "stats_timed": stats_t, "stage_ms": stage_ms, "kernel_ms": kernel_ms,
+            "roofline": roof, "tuned": tuned, "timing_src": timing_src, "sample": sample}
+
+
+def profile_leg(args):
+    """One workload stepped eagerly, for a rocprofv3 run: --warmup steps,
+    then --steps steps, default launch choices, then the counts."""
+    import numpy as np
+
+    from mwdiem_amd import scene
+    from mwdiem_amd.stepper import Stepper
+    kw = {}
+    v = None
+    if args.profile_leg == "poly":
+        sc = scene.config_polydisperse()
+        rng = np.random.Generator(np.random.PCG64(3))
+        v = np.zeros((sc.n_rows, 3))
+        v[sc.n_walls:] = rng.normal(size=(sc.n_particles, 3)) * 1000.0
+        v[:sc.n_walls] = sc.wall_v[:sc.n_walls]
+    else:
+        sc = scene.config_large(seed=args.seed, n=args.particles)
+        if args.profile_leg == "contacting":
+            sc = contacting_scene(sc)
+            kw["pair_cap"] = 20_000_000
+    S = Stepper(sc, **kw)
+    if v is not None:
+        S.set_state({"v": v, "v12": v})
+    print(f"# profile {args.profile_leg}: {sc.n_particles} hulls, {args.warmup} + {args.steps} steps",
+          file=sys.stderr, flush=True)
+    S.step(args.warmup)
+    S.check()
+    S.step(args.steps)
+    print(json.dumps({"profile_leg": args.profile_leg, "steps": args.steps, "warmup": args.warmup,
+                      "status": S.check()}))
+    S.close()
+
+
+def contacting_scene(sc, gap: float = -0.6):
+    """A contact-bearing state of the large settle's hulls: the same 100,000
+    hulls moved onto a body-centred-cubic lattice from the floor up whose
+    spacing is (1 + gap) times twice the largest circumradius, so that
+    neighbours interpenetrate (about 1.5 contacts per hull, 13 M candidate
+    pairs), on the physical law (the reference law's damping is unstable once
+    hulls touch)."""
+    from mwdiem_amd import scene
+    out = scene.compact_dense(sc, gap=gap)
+    out.law_p = scene.Law4(Kn=sc.law_p.Kn, en=sc.law_p.en, et=sc.law_p.et, mu=sc.law_p.mu, mode=scene.PHYSICAL)
+    out.law_w = scene.Law4(Kn=sc.law_w.Kn, en=sc.law_w.en, et=sc.law_w.et, mu=sc.law_w.mu, mode=scene.PHYSICAL)
+    out.name = f"{sc.name}-compacted"
+    return out
+
+
+def bench_batched(scs, steps: int, plain_steps: int, graph_steps: int, tune: bool = True):
     """The batched form: n_scenes hopper-discharge scenes (particle counts
     log-uniform 500-20,000, per-scene friction, stiffness and restitution,
     physical law) in one Stepper -- one launch per stage over all scenes,
@@ -111,15 +319,12 @@ def bench_batched(n_scenes: int, steps: int, plain_steps: int, graph_steps: int,
     own graph-captured Stepper with the default launch choices."""
     import torch
 
-    from mwdiem_amd import scene
     from mwdiem_amd.stepper import Stepper
     from mwdiem_amd.tune import autotune, regrid, tune_cell_scale
 
-    t0 = time.perf_counter()
-    scs = scene.config_batched(n_scenes=n_scenes)
+    n_scenes = len(scs)
     n_tot = sum(sc.n_particles for sc in scs)
-    print(f"# batched: {n_scenes} scenes, {n_tot} particles, generated in {time.perf_counter() - t0:.1f} s",
-          file=sys.stderr, flush=True)
+    print(f"# batched: {n_scenes} scenes, {n_tot} particles", file=sys.stderr, flush=True)
     tuned = None
     if tune:
         scale, scale_ms = tune_cell_scale(scs, steps=5)
@@ -165,7 +370,7 @@ def bench_batched(n_scenes: int, steps: int, plain_steps: int, graph_steps: int,
             "autotune": tuned}
 
 
-def bench_diff(steps: int, graph_steps: int, n: int = 4096, short: int = 200):
+def bench_diff(pile, steps: int, graph_steps: int, short: int = 200):
     """The differentiable form on the poured pile (n hulls poured at 1 m/s rms
     into a 40x40x60 mm box, physical law Kn=1e4, en=0.5, mu=0.3): cost of a
     tangent step (state plus its derivatives with respect to Kn, en, mu)
@@ -179,11 +384,11 @@ def bench_diff(steps: int, graph_steps: int, n: int = 4096, short: int = 200):
     import numpy as np
     import torch
 
-    from mwdiem_amd import scene
     from mwdiem_amd.diff import _with_theta, run_outcome
     from mwdiem_amd.stepper import Stepper
 
-    sc, v0 = scene.config_pile(n=n)
+    (sc, v0), (_, v_fast) = pile
+    n = sc.n_particles
     theta = np.array([sc.law_p.Kn, sc.law_p.en, sc.law_p.mu])
     times = {}
     for mode in ("plain", "tangent"):
@@ -199,7 +404,6 @@ def bench_diff(steps: int, graph_steps: int, n: int = 4096, short: int = 200):
         times[mode] = (time.perf_counter() - t0) / steps
         st = S.check()
         S.close()
-    _, v_fast = scene.config_pile(n=n, v_pour=3000.0)
     checks = {}
     for name, init, horizon, hs in ((f"pour_3m_s_{short}_steps", v_fast, short, (1e-4,)),
                                     (f"pour_1m_s_{steps}_steps", v0, steps, (1e-4, 1e-7))):
@@ -240,7 +444,7 @@ def bench_diff(steps: int, graph_steps: int, n: int = 4096, short: int = 200):
             "mode": "forward tangents (3 directions); reverse adjoint with step checkpoints"}
 
 
-def bench_polydisperse(steps: int, graph_steps: int, v_rms: float = 1000.0, tune: bool = True):
+def bench_polydisperse(sc, steps: int, graph_steps: int, v_rms: float = 1000.0, tune: bool = True):
     """The polydisperse stress configuration: 1,000,000 hulls (circumradius
     log-uniform 0.5-2.5 mm, 6-32 points) in a 400 mm box whose ceiling slides
     along x at 0.1 m/s, physical law, FP64 GJK/EPA. The hulls start from
@@ -251,13 +455,9 @@ def bench_polydisperse(steps: int, graph_steps: int, v_rms: float = 1000.0, tune
     import numpy as np
     import torch
 
-    from mwdiem_amd import scene
     from mwdiem_amd.stepper import Stepper
 
-    t0 = time.perf_counter()
-    sc = scene.config_polydisperse()
-    print(f"# polydisperse: {sc.n_particles} hulls, vmax {sc.vmax}, generated in {time.perf_counter() - t0:.1f} s",
-          file=sys.stderr, flush=True)
+    print(f"# polydisperse: {sc.n_particles} hulls, vmax {sc.vmax}", file=sys.stderr, flush=True)
     W = sc.n_walls
     rng = np.random.Generator(np.random.PCG64(3))
     v = np.zeros((sc.n_rows, 3))
@@ -294,6 +494,18 @@ def bench_polydisperse(steps: int, graph_steps: int, v_rms: float = 1000.0, tune
         dt = time.perf_counter() - t0
         st = S.check()
         fin = S.numpy_state()
+        if not fp32:
+            # kernel durations at the end of the run (contacts formed) and the
+            # pair list of the next GJK launch, for its flop count
+            S.set_timing(True)
+            S.step(20)
+            kt = S.kernel_times()
+            S.stage_times()
+            S.set_timing(False)
+            kernel_ms = {k: v / max(kt["steps"], 1) for k, v in kt["ms"].items()}
+            sample = gjk_sample(S, sc)
+            sample["kernel_ms"] = kernel_ms
+            sample["stats"] = S.check()
         com[fp32] = fin["com"][W:]
         speed[fp32] = np.linalg.norm(fin["v"][W:], axis=1)
         ke = 0.5 * float(np.sum(sc.mass[W:] * speed[fp32] ** 2))
@@ -319,8 +531,10 @@ def bench_polydisperse(steps: int, graph_steps: int, v_rms: float = 1000.0, tune
                 "calm_hulls": int(calm.sum()), "calm_kinetic_energy_rel_diff": abs(ke_calm[1] / ke_calm[0] - 1.0),
                 "median_dx_fp32_vs_fp64_mm": float(np.median(dx)), "frac_dx_above_1um": float(np.mean(dx > 1e-3)),
                 "max_dx_fp32_vs_fp64_mm": float(dx.max()), "max_displacement_mm": disp})
+    out["kernel_ms_at_end"] = {k: round(v, 5) for k, v in sample["kernel_ms"].items()}
+    out["gjk_pairs_at_end"] = int(sample["pairs"].shape[0])
     print(f"# polydisperse: {out}", file=sys.stderr, flush=True)
-    return out
+    return out, sample
 
 
 def copy_probe(nbytes: int, reps: int = 5):
@@ -357,6 +571,11 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch choices (no autotuner)")
     ap.add_argument("--poly-steps", type=int, default=2000, help="steps of the polydisperse stress configuration")
     ap.add_argument("--no-poly", action="store_true")
+    ap.add_argument("--no-contacting", action="store_true", help="skip the compacted (contact-bearing) C2 bed")
+    ap.add_argument("--profile-leg", choices=("c2", "contacting", "poly"), default=None,
+                    help="only step one workload eagerly (warm-up plus --steps steps) for a profiler run: the "
+                         "large settle, its compacted bed, or the polydisperse configuration thrown about at "
+                         "1 m/s (--warmup steps first, then --steps steps)")
     ap.add_argument("--copy-probe", type=int, default=0, help="only run a device copy of this many bytes (counter "
                                                                  "calibration)")
     args = ap.parse_args()
@@ -368,108 +587,121 @@ def main():
               "MI355X; sharding across GPUs is a design, not a measured path.", file=sys.stderr)
         sys.exit(2)
 
-    import torch
-
+    # every scene is made before the GPU runtime starts in this process
+    # (the generator forks worker processes only before that)
     from mwdiem_amd import scene
-    from mwdiem_amd.stepper import Stepper
-
+    if args.profile_leg:
+        profile_leg(args)
+        return
     t_gen = time.perf_counter()
     sc = scene.config_large(seed=args.seed, n=args.particles)
-    t_gen = time.perf_counter() - t_gen
     print(f"# scene: {sc.n_particles} particles, vmax {sc.vmax}, cell {sc.max_particlesize:.3f} mm, "
-          f"generated in {t_gen:.1f} s", file=sys.stderr, flush=True)
+          f"generated in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
+    sc_dense = None if args.no_contacting else contacting_scene(sc)
+    scs_batched = None if args.no_batched else scene.config_batched(n_scenes=args.batched_scenes)
+    pile = None if args.no_diff else (scene.config_pile(n=4096), scene.config_pile(n=4096, v_pour=3000.0))
+    sc_poly = None if args.no_poly else scene.config_polydisperse()
+    sc_small = None if args.no_cpu_baseline else scene.config_small()
+    print(f"# all scenes generated in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
+
+    import torch
+
+    from mwdiem_amd.stepper import Stepper
+    from mwdiem_amd.tune import SPACE
+
     S = Stepper(sc, device=0)
-    for _ in range(args.warmup):
-        S.step(1)
-    S.check()
-    tuned = None
-    if not args.no_tune:
-        # launch choices by timing (results are bit-identical across them;
-        # the state is restored after the trials)
-        from mwdiem_amd.tune import autotune
-        tuned = autotune(S, steps=10)
-        print(f"# autotune: {tuned['tuning']} {tuned['ms_per_step']:.4f} ms/step (defaults "
-              f"{tuned['default_ms_per_step']:.4f}), {len(tuned['trials'])} trials", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    # Kernel durations: the first steps of the run after warm-up, launched
-    # eagerly on the stepper's stream with HIP events -- stage boundaries,
-    # and for the single-kernel stages start/stop events stamped with the
-    # kernel's own start and end (hipExtLaunchKernelGGL), so launch gaps are
-    # excluded. (The same steps as a short profiled run from the initial state.)
-    n_time = min(args.steps, args.timing_steps)
-    S.set_timing(True)
-    S.step(n_time)
-    kt = S.kernel_times()
-    st = S.stage_times()
-    stats_t = S.check()  # counts of the last timed-kernel step
-    S.set_timing(False)
-    timing_src = f"{kt['steps']} eager steps after warm-up, kernel start/stop events"
-    if not args.no_graph:
-        S.capture(args.graph_steps)   # records graph_steps steps; capture runs nothing
-
-    # timed region: K full steps
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    S.run(args.steps)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    stats = S.check()
-    ms_step = 1e3 * wall / args.steps
-    value = sc.n_particles * args.steps / wall
-    stage_ms = {k: v / max(st["steps"], 1) for k, v in st["ms"].items()}
-    kernel_ms = {k: v / max(kt["steps"], 1) for k, v in kt["ms"].items()}
-    sbytes = stage_bytes([sc], stats_t)
-    # the dominant kernel: the slowest of the single-kernel stages
-    from mwdiem_amd import _lib
-    stage_of = dict(zip(_lib.KTIMED_NAMES, _lib.KTIMED_STAGE))
-    kname = max(kernel_ms, key=kernel_ms.get)
-    dom = stage_of[kname]
-    ach = sbytes[dom] / (kernel_ms[kname] * 1e-3) / 1e9
-    traffic = measured_traffic(kname, "c2_100k")
-    roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-            "traffic": traffic, "kernel": kname, "launch_ms": kernel_ms[kname], "algorithmic_bytes": sbytes[dom],
-            "timed_over": timing_src,
-            "kernel_ms": {k: round(v, 5) for k, v in kernel_ms.items()},
-            "kernel_gbs": {k: round(sbytes[stage_of[k]] / (v * 1e-3) / 1e9, 1) for k, v in kernel_ms.items() if v > 0}}
-    print(f"# stages (ms/step): {json.dumps({k: round(v, 4) for k, v in stage_ms.items()})}", file=sys.stderr)
-    print(f"# pairs {stats['n_pairs']} contacts {stats['n_contacts']} cells {stats['n_cells']}", file=sys.stderr,
-          flush=True)
-
+    main_leg = plain_leg(S, sc, args, "c2_100k", "C2")
     S.close()
     del S
     torch.cuda.empty_cache()
+    wall, stats = main_leg["wall"], main_leg["stats"]
+    value = sc.n_particles * args.steps / wall
+    samples = {"c2_100k": main_leg["sample"]}
+
+    contacting = None
+    if sc_dense is not None:
+        S = Stepper(sc_dense, device=0, pair_cap=20_000_000)
+        leg = plain_leg(S, sc_dense, args, "c2_100k_contacting", "C2 compacted",
+                        tune_space=dict(SPACE, epa_fused=(0, 1)))
+        S.close()
+        del S
+        torch.cuda.empty_cache()
+        samples["c2_100k_contacting"] = leg["sample"]
+        contacting = {"workload": "the large settle's 100,000 hulls compacted onto a lattice 0.4x twice the largest "
+                                  "circumradius (neighbours interpenetrate), physical law, same Kn, e, mu, dt",
+                      "ms_per_step": 1e3 * leg["wall"] / args.steps,
+                      "particle_steps_per_s": sc_dense.n_particles * args.steps / leg["wall"],
+                      "candidates_per_step": leg["stats"]["n_pairs"],
+                      "gjk_pairs_per_step": leg["stats"]["n_sphere_pass"],
+                      "contacts_per_step": leg["stats"]["n_contacts"],
+                      "pair_tests_per_s": leg["stats"]["n_sphere_pass"] * args.steps / leg["wall"],
+                      "stage_ms_per_step": {k: round(v, 5) for k, v in leg["stage_ms"].items()},
+                      "roofline": leg["roofline"], "tuning": None if leg["tuned"] is None else leg["tuned"]["tuning"],
+                      "kernel_ms": leg["kernel_ms"]}
+
     batched = None
-    if not args.no_batched:
-        batched = bench_batched(args.batched_scenes, args.batched_steps, args.plain_steps, args.graph_steps,
+    if scs_batched is not None:
+        batched = bench_batched(scs_batched, args.batched_steps, args.plain_steps, args.graph_steps,
                                 tune=not args.no_tune)
 
     diff = None
-    if not args.no_diff:
-        diff = bench_diff(args.diff_steps, args.graph_steps)
+    if pile is not None:
+        diff = bench_diff(pile, args.diff_steps, args.graph_steps)
 
     poly = None
-    if not args.no_poly:
-        poly = bench_polydisperse(args.poly_steps, args.graph_steps, tune=not args.no_tune)
+    if sc_poly is not None:
+        poly, samples["c3_1m"] = bench_polydisperse(sc_poly, args.poly_steps, args.graph_steps,
+                                                    tune=not args.no_tune)
 
-    cpu = None
+    cpu, flops, cpu_py = None, {}, None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(sc, args.cpu_budget)
+        cpu, flops = cpu_baseline(sc, args.cpu_budget, samples)
+        cpu_py = cpu_baseline_python(sc_small, args.cpu_budget)
+        print(f"# cpu baselines: {cpu['value']:.4g} (oracle, {cpu['cores']} threads), {cpu_py['value']:.4g} "
+              f"(reference structure in Python, 1 core) particle-steps/s", file=sys.stderr, flush=True)
+
+    # FP64 roofline of the GJK launch (GJK, and with epa_fused EPA and the law)
+    from mwdiem_amd import _lib
+    gjk_name = _lib.KTIMED_NAMES[_lib.KTIMED_STAGE.index("gjk")]
+    ms = main_leg["kernel_ms"]
+    roof_gjk = {"c2_100k": roofline_fp64(flops.get("c2_100k"), ms.get(gjk_name, 0.0) + ms.get("k_epa_force", 0.0),
+                                         "k_gjk_epa", "c2_100k", main_leg["timing_src"])}
+    if contacting is not None:
+        ms = contacting["kernel_ms"]
+        # with EPA in its own launch the GJK+EPA work spans both kernels
+        both = ms.get(gjk_name, 0.0) + ms.get("k_epa_force", 0.0)
+        roof_gjk["c2_100k_contacting"] = roofline_fp64(flops.get("c2_100k_contacting"), both, "k_gjk_epa",
+                                                       "c2_100k_contacting", "eager steps, kernel start/stop events")
+        contacting["kernel_ms"] = {k: round(v, 5) for k, v in ms.items()}
+    if poly is not None:
+        ms = samples["c3_1m"]["kernel_ms"]
+        both = ms.get(gjk_name, 0.0) + ms.get("k_epa_force", 0.0)
+        roof_gjk["c3_1m"] = roofline_fp64(flops.get("c3_1m"), both, "k_gjk_epa", "c3_1m",
+                                          "20 eager steps after the timed run, kernel start/stop events")
+        poly["gjk_pairs_per_s_at_end"] = samples["c3_1m"]["stats"]["n_sphere_pass"] / (both * 1e-3) if both else None
 
     out = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "plain large settle: 100,000 convex polyhedra (8-20 hull points), 200x200x400 mm "
                                "walled box, FP64, dt=1e-6 s, Kn=1e4, e=0.5, mu=0.3, cell=2*r_circ,max",
-                   "particles": sc.n_particles, "pairs_per_step": stats["n_pairs"],
-                   "contacts_per_step": stats["n_contacts"], "sphere_survivors_per_step": stats["n_sphere_pass"],
-                   "pair_tests_per_s": stats["n_pairs"] * args.steps / wall,
-                   "stage_ms_per_step": {k: round(v, 5) for k, v in stage_ms.items()},
-                   "tuning": None if tuned is None else tuned["tuning"],
-                   "untuned_ms_per_step_eager": None if tuned is None else tuned["default_ms_per_step"],
-                   "tuned_ms_per_step_eager": None if tuned is None else tuned["ms_per_step"]},
-        "roofline": roof,
+                   "particles": sc.n_particles, "candidates_per_step": stats["n_pairs"],
+                   "gjk_pairs_per_step": stats["n_sphere_pass"], "contacts_per_step": stats["n_contacts"],
+                   # pair tests: candidate pairs through GJK (the bounding-sphere
+                   # test decides the others without GJK)
+                   "pair_tests_per_s": stats["n_sphere_pass"] * args.steps / wall,
+                   "candidates_per_s": stats["n_pairs"] * args.steps / wall,
+                   "stage_ms_per_step": {k: round(v, 5) for k, v in main_leg["stage_ms"].items()},
+                   "tuning": None if main_leg["tuned"] is None else main_leg["tuned"]["tuning"],
+                   "untuned_ms_per_step_eager": None if main_leg["tuned"] is None
+                   else main_leg["tuned"]["default_ms_per_step"],
+                   "tuned_ms_per_step_eager": None if main_leg["tuned"] is None else main_leg["tuned"]["ms_per_step"]},
+        "roofline": main_leg["roofline"],
+        "roofline_gjk": roof_gjk,
         "cpu_baseline": cpu,
+        "cpu_baseline_reference_structure": cpu_py,
+        "contacting": contacting,
         "batched": batched,
         "batched_vs_plain": None if batched is None else batched["batched_vs_plain"],
         "differentiable": diff,
