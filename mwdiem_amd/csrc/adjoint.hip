@@ -283,7 +283,8 @@ __global__ void __launch_bounds__(256) k_contact_adj(BodyView A, BodyView B, con
         double* ob = cadj + (2 * c + 1) * CADJ;
         for (int k = 0; k < CADJ; ++k) oa[k] = ob[k] = 0.0;
         cdth[3 * c] = cdth[3 * c + 1] = cdth[3 * c + 2] = 0.0;
-        if (!(depth_e[c] < EPA_PLACEHOLDER)) continue;  // no depth resolved: no force, nothing to reverse
+        // no depth resolved, or a signed depth that is not positive: no force, nothing to reverse
+        if (!(depth_e[c] < EPA_PLACEHOLDER && depth_e[c] > 0.0)) continue;
         const int64_t p = contact_pair[c];
         const int64_t ra = view_row(A, p), rb = view_row(B, p);
         const bool wallA = A.is_wall[ra], wallB = B.is_wall[rb];

@@ -87,6 +87,7 @@ BodyView state_view(const mwd_workspace* ws, const mwd_state* st, int side) {
     b.rows = ws->pairs + side;
     b.row_stride = 2;
     b.vmax = st->vmax;
+    b.laws2 = ws->laws;
     return b;
 }
 

@@ -366,10 +366,39 @@ __device__ __forceinline__ bool correct_flat(d3 S1, d3 S2, d3 S3, d3 S4) {
     return false;
 }
 
+// The physical law's flatness test of a GJK tetrahedron: six times its
+// volume at most 1e-9 of the cube of its longest edge (squared: v6^2 <=
+// 1e-18 L2^3). It measures the tetrahedron, not where the origin lies (not
+// in the reference).
+__device__ __forceinline__ bool flat_tetra(d3 S0, d3 S1, d3 S2, d3 S3) {
+    const d3 e1 = S1 - S0, e2 = S2 - S0, e3 = S3 - S0;
+    const double v6 = dot3(cross3(e1, e2), e3);
+    double L2 = nrm2(e1);
+    const double c2 = nrm2(e2), c3 = nrm2(e3), c4 = nrm2(S2 - S1), c5 = nrm2(S3 - S1), c6 = nrm2(S3 - S2);
+    if (c2 > L2) L2 = c2;
+    if (c3 > L2) L2 = c3;
+    if (c4 > L2) L2 = c4;
+    if (c5 > L2) L2 = c5;
+    if (c6 > L2) L2 = c6;
+    return v6 * v6 <= 1e-18 * ((L2 * L2) * L2);
+}
+
+// Whether pair (ra, rb) takes the physical law's GJK/EPA variant.
+__device__ __forceinline__ bool robust_pair(const BodyView& A, const BodyView& B, int64_t ra, int64_t rb) {
+    if (!A.laws2) return false;
+    const int wall = (A.is_wall[ra] || B.is_wall[rb]) ? 1 : 0;
+    const int scene = A.scene ? A.scene[ra] : 0;
+    return A.laws2[2 * scene + wall].mode == MWD_LAW_PHYSICAL;
+}
+
 // GJK of one pair (.cu:1009-1288). Returns -1 (tetrahedron cap), 0 or 1; on
-// 1 the simplex handed to EPA (after CorrectFlat) is in S/ia/ib.
+// 1 the simplex handed to EPA (after CorrectFlat) is in S/ia/ib. robust (the
+// physical law's pairs; not in the reference): instead of CorrectFlat, a flat
+// tetrahedron gets its fourth vertex from the normal of the first three, on
+// the side where the Minkowski difference reaches farther.
 template <class Sup>
-__device__ __forceinline__ int gjk_pair(const Sup& support, d3 comA, d3 comB, d3* S, int* ia, int* ib) {
+__device__ __forceinline__ int gjk_pair(const Sup& support, d3 comA, d3 comB, d3* S, int* ia, int* ib,
+                                        bool robust) {
     // the simplex lives in named scalars (registers); the arrays are written
     // once, on a collision
     d3 S0, S1, S2, S3;
@@ -422,7 +451,19 @@ __device__ __forceinline__ int gjk_pair(const Sup& support, d3 comA, d3 comB, d3
             }
         }
         if (hit) {
-            if (correct_flat(S0, S1, S2, S3)) {
+            if (robust) {
+                if (flat_tetra(S0, S1, S2, S3)) {
+                    const d3 nT = cross3(S1 - S0, S2 - S0);
+                    int ap, bp, am, bm;
+                    const d3 Sp = support(nT, &ap, &bp);
+                    const d3 Sm = support(neg3(nT), &am, &bm);
+                    if (dot3(nT, Sp - S0) >= dot3(nT, S0 - Sm)) {
+                        S3 = Sp; a3 = ap; b3 = bp;
+                    } else {
+                        S3 = Sm; a3 = am; b3 = bm;
+                    }
+                }
+            } else if (correct_flat(S0, S1, S2, S3)) {
                 d4 = cross3(S0, S1);
                 S3 = support(d4, &a3, &b3);
             }
@@ -485,12 +526,13 @@ __device__ __forceinline__ int gjk_of(const BodyView& A, const BodyView& B, int6
     const d3 ca = ld3(A.com + 3 * ra), cb = ld3(B.com + 3 * rb);
     const Side sa{A.verts + (int64_t)A.vmax * 3 * ra, A.nverts[ra]};
     const Side sb{B.verts + (int64_t)B.vmax * 3 * rb, B.nverts[rb]};
+    const bool robust = robust_pair(A, B, ra, rb);
     if constexpr (K > 0) {
         GroupSup<G, K> sup;
         sup.load(sa, sb, g);
-        return gjk_pair(sup, ca, cb, S, ia, ib);
+        return gjk_pair(sup, ca, cb, S, ia, ib, robust);
     } else {
-        return gjk_pair(SerialSup{sa, sb}, ca, cb, S, ia, ib);
+        return gjk_pair(SerialSup{sa, sb}, ca, cb, S, ia, ib, robust);
     }
 }
 
@@ -592,15 +634,17 @@ __device__ __forceinline__ void wave_sync() {
 __device__ __forceinline__ d3 pw_v(const PolyW& P, int k) { return mk3(P.vx[k], P.vy[k], P.vz[k]); }
 __device__ __forceinline__ d3 pw_n(const PolyW& P, int k) { return mk3(P.nx[k], P.ny[k], P.nz[k]); }
 
-// AddTriangle (.cu:270-325) into slot id.
-__device__ __forceinline__ void add_triangle_w(PolyW& P, int e0, int e1, int e2, int id) {
+// AddTriangle (.cu:270-325) into slot id. robust (the physical law's EPA):
+// the face is turned away from the interior point `inner` instead of away
+// from the origin, and dd keeps its sign (negative: the origin is outside).
+__device__ __forceinline__ void add_triangle_w(PolyW& P, int e0, int e1, int e2, int id, bool robust, d3 inner) {
     const d3 V0 = pw_v(P, e0), V1 = pw_v(P, e1), V2 = pw_v(P, e2);
     d3 nn = cross3(V1 - V0, V2 - V1);
     const double len = sqrt(nrm2(nn));
     nn = nn / len;
     double dd = dot3(nn, V0);
     uint8_t fl = 0;
-    if (dd < 0.0) {
+    if (robust ? dot3(nn, V0 - inner) < 0.0 : dd < 0.0) {
         const int o = e2;
         e2 = e1;
         e1 = o;
@@ -706,7 +750,14 @@ __device__ __forceinline__ int closest_face_w(const PolyW& P, int nt, int lane) 
     return MWD_WAVE + __ffsll((long long)hi) - 1;
 }
 
-__device__ EpaResult epa_wave(const Simplex& sx, const Side& A, const Side& B, PolyW& P, int lane) {
+// robust (the physical law; not in the reference): faces are turned away from
+// the starting tetrahedron's centroid, an interior point of every polytope
+// EPA grows from it, so a face whose plane passes through or beside the
+// origin (a contact of near-zero depth) keeps its outward normal and signed
+// distance; the expansion also stops when the support point gains at most
+// 1e-5 |d| + 1e-10 R over the closest face (R: the largest distance of a
+// starting vertex from the origin).
+__device__ EpaResult epa_wave(const Simplex& sx, const Side& A, const Side& B, PolyW& P, int lane, bool robust) {
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (MWD_WAVE - lane));
     double xmin_old = 1e29;
     double proj = 2.0 * xmin_old + 1.0;
@@ -720,12 +771,20 @@ __device__ EpaResult epa_wave(const Simplex& sx, const Side& A, const Side& B, P
         P.opA[lane] = (uint8_t)sx.ia[lane];
         P.opB[lane] = (uint8_t)sx.ib[lane];
     }
+    const d3 s0 = mk3(sx.s[0][0], sx.s[0][1], sx.s[0][2]), s1 = mk3(sx.s[1][0], sx.s[1][1], sx.s[1][2]);
+    const d3 s2 = mk3(sx.s[2][0], sx.s[2][1], sx.s[2][2]), s3 = mk3(sx.s[3][0], sx.s[3][1], sx.s[3][2]);
+    const d3 inner = (((s0 + s1) + s2) + s3) * 0.25;
+    double R2 = nrm2(s0);
+    if (nrm2(s1) > R2) R2 = nrm2(s1);
+    if (nrm2(s2) > R2) R2 = nrm2(s2);
+    if (nrm2(s3) > R2) R2 = nrm2(s3);
+    const double tol_abs = 1e-10 * sqrt(R2);
     wave_sync();
     if (lane < 4) {
         const int e0 = lane == 2 ? 2 : (lane == 1 ? 1 : 0);
         const int e1 = lane == 0 ? 1 : (lane == 1 ? 2 : (lane == 2 ? 0 : 1));
         const int e2 = lane == 0 ? 2 : 3;
-        add_triangle_w(P, e0, e1, e2, lane);
+        add_triangle_w(P, e0, e1, e2, lane, robust, inner);
     }
     nt = 4;
     wave_sync();
@@ -743,6 +802,7 @@ __device__ EpaResult epa_wave(const Simplex& sx, const Side& A, const Side& B, P
             P.opB[nv] = (uint8_t)b;
         }
         if (xmin_old > proj || (xmin_old * 1.00001 > proj && xmin_old * 0.99999 < proj)) break;
+        if (robust && proj - xmin_old <= 1e-5 * fabs(xmin_old) + tol_abs) break;
         if (counter > 49) {
             st |= 1;
             break;
@@ -799,7 +859,7 @@ __device__ EpaResult epa_wave(const Simplex& sx, const Side& A, const Side& B, P
             }
             const uint64_t m = __ballot(isnew);
             const int id = nt0 + added + __popcll(m & lt);
-            if (isnew && id < EPA_FACES) add_triangle_w(P, u, v, nv - 1, id);
+            if (isnew && id < EPA_FACES) add_triangle_w(P, u, v, nv - 1, id, robust, inner);
             added += __popcll(m);
         }
         wave_sync();
@@ -950,10 +1010,12 @@ __device__ LawOut<T> force_law(V3<T> comA, V3<T> comB, V3<T> vA, V3<T> vB, V3<T>
                                const LawT<T>& L) {
     LawOut<T> o;
     const bool phys = L.mode == MWD_LAW_PHYSICAL;
-    if (phys && !(depth < EPA_PLACEHOLDER)) {
+    if (phys && !(depth < EPA_PLACEHOLDER && depth > 0.0)) {
         // EPA ended on a face whose distance is its "outside" or "deleted"
-        // placeholder (1e30 / 3e20): no depth was resolved; the physical law
-        // applies no force for the step (the reference law takes the value)
+        // placeholder (1e30 / 3e20): no depth was resolved; or the signed
+        // depth is not positive (the bodies touch without overlapping): the
+        // physical law applies no force for the step (the reference law
+        // takes the value)
         o.Fn = o.tA = o.tB = o.disp = zero3<T>();
         return o;
     }
@@ -1123,7 +1185,7 @@ __device__ __forceinline__ void contact_wave(int64_t c, int64_t p, const Simplex
     const int64_t ra = view_row(A, p), rb = view_row(B, p);
     const Side sa{A.verts + (int64_t)A.vmax * 3 * ra, A.nverts[ra]};
     const Side sb{B.verts + (int64_t)B.vmax * 3 * rb, B.nverts[rb]};
-    const EpaResult e = epa_wave(sx, sa, sb, P, lane);
+    const EpaResult e = epa_wave(sx, sa, sb, P, lane, robust_pair(A, B, ra, rb));
     if (lane != 0) return;
     const int wall = (A.is_wall[ra] || B.is_wall[rb]) ? 1 : 0;
     const int scene = A.scene ? A.scene[ra] : 0;

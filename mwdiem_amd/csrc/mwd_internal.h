@@ -79,6 +79,8 @@ struct BodyView {
     const int32_t* rows;        // NULL: identity
     int32_t row_stride;         // pairs[p*row_stride] (2 for the interleaved pair list)
     int32_t vmax;
+    const mwd_law* laws2;       // per scene (particle law, wall law); NULL: every pair takes the
+                                // reference's GJK/EPA (the physical law's pairs take its variant)
 };
 
 // A state array and its MWD_N_TAN tangents (tangent pointers NULL in the

@@ -245,6 +245,24 @@ inline int correct_flat(V3<T> S1, V3<T> S2, V3<T> S3, V3<T> S4) {
     return 0;
 }
 
+// Physical law: a tetrahedron is flat when six times its volume is at most
+// 1e-9 of the cube of its longest edge (squared: v6^2 <= 1e-18 L2^3). Unlike
+// CorrectFlat's test, this measures the tetrahedron and not where the origin
+// lies (not in the reference).
+template <typename T>
+inline bool flat_tetra(V3<T> S0, V3<T> S1, V3<T> S2, V3<T> S3) {
+    const V3<T> e1 = S1 - S0, e2 = S2 - S0, e3 = S3 - S0;
+    const T v6 = dot(cross(e1, e2), e3);
+    T L2 = norm2(e1);
+    const T c2 = norm2(e2), c3 = norm2(e3), c4 = norm2(S2 - S1), c5 = norm2(S3 - S1), c6 = norm2(S3 - S2);
+    if (c2 > L2) L2 = c2;
+    if (c3 > L2) L2 = c3;
+    if (c4 > L2) L2 = c4;
+    if (c5 > L2) L2 = c5;
+    if (c6 > L2) L2 = c6;
+    return v6 * v6 <= T(1e-18) * ((L2 * L2) * L2);
+}
+
 // ------------------------------------------------------------------------ EPA
 // Face distances at or above this are EPA's placeholders (1e30 for a face
 // whose plane misses the origin's projection, 3e20 for a deleted face), not
@@ -259,16 +277,19 @@ template <typename T> struct Poly {
     int valid[EPA_MAX_FACES];
 };
 
-// AddTriangle, .cu:270-325
+// AddTriangle, .cu:270-325. With `inner` (the physical law's EPA) the face
+// is turned to face away from that interior point instead of away from the
+// origin, and dd keeps its sign (negative: the origin is outside the face).
 template <typename T>
-inline void add_triangle(Poly<T>& P, int e0, int e1, int e2, int id) {
+inline void add_triangle(Poly<T>& P, int e0, int e1, int e2, int id, const V3<T>* inner = nullptr) {
     P.Tr[id][0] = e0; P.Tr[id][1] = e1; P.Tr[id][2] = e2;
     V3<T> nn = cross(P.V[e1] - P.V[e0], P.V[e2] - P.V[e1]);
     T len = sqrt_(norm2(nn));
     nn = nn / len;
     T dd = dot(nn, P.V[e0]);
     P.valid[id] = 1;
-    if (dd < T(0.0)) {
+    const bool turn = inner ? dot(nn, P.V[e0] - *inner) < T(0.0) : dd < T(0.0);
+    if (turn) {
         int o = P.Tr[id][2]; P.Tr[id][2] = P.Tr[id][1]; P.Tr[id][1] = o;
         dd = dd * T(-1.0);
         nn = nn * T(-1.0);
@@ -288,10 +309,17 @@ struct EpaOut { double depth; double n[3]; double cpA[3]; double cpB[3]; int sta
 
 // EPA, .cu:328-603. status bit 1: expansion cap reached (.cu:443-447);
 // bit 2: face storage exhausted (reference: undefined behaviour).
+// robust (the physical law; not in the reference): faces are turned away from
+// the starting tetrahedron's centroid, an interior point of every polytope
+// EPA grows from it, so that a face whose plane passes through or beside the
+// origin (a contact of near-zero depth) keeps its outward normal and its
+// signed distance; and the expansion also stops when the support point gains
+// at most 1e-5 |d| + 1e-10 R over the closest face (R: the largest distance
+// of a starting vertex from the origin).
 template <typename T>
 void epa(const V3<T> S[4], const int ia[4], const int ib[4], const Body<T>& A, const Body<T>& B,
          T* depth, V3<T>* cpA, V3<T>* cpB, V3<T>* normal, int* status, int* faces_used, int* iters,
-         int* face = nullptr) {
+         int* face = nullptr, bool robust = false) {
     Poly<T> P;
     T xmin_old = T(1e29);
     T proj = T(2.0) * xmin_old + T(1.0);
@@ -299,10 +327,15 @@ void epa(const V3<T> S[4], const int ia[4], const int ib[4], const Body<T>& A, c
     T xmin_new = T(0.0);
     int st = 0;
     for (int k = 0; k < 4; ++k) { P.V[k] = S[k]; P.opA[k] = ia[k]; P.opB[k] = ib[k]; }
-    add_triangle(P, 0, 1, 2, 0);
-    add_triangle(P, 1, 2, 3, 1);
-    add_triangle(P, 2, 0, 3, 2);
-    add_triangle(P, 0, 1, 3, 3);
+    const V3<T> ctr = (((S[0] + S[1]) + S[2]) + S[3]) * T(0.25);
+    const V3<T>* inner = robust ? &ctr : nullptr;
+    T R2 = norm2(S[0]);
+    for (int k = 1; k < 4; ++k) if (norm2(S[k]) > R2) R2 = norm2(S[k]);
+    const T tol_abs = T(1e-10) * sqrt_(R2);
+    add_triangle(P, 0, 1, 2, 0, inner);
+    add_triangle(P, 1, 2, 3, 1, inner);
+    add_triangle(P, 2, 0, 3, 2, inner);
+    add_triangle(P, 0, 1, 3, 3, inner);
     nt = 4;
     int it = 0;
     while (xmin_old < proj) {
@@ -318,6 +351,7 @@ void epa(const V3<T> S[4], const int ia[4], const int ib[4], const Body<T>& A, c
         P.opA[nv] = a;
         P.opB[nv] = b;
         if (xmin_old > proj || (xmin_old * T(1.00001) > proj && xmin_old * T(0.99999) < proj)) break;
+        if (robust && proj - xmin_old <= T(1e-5) * fabs_(xmin_old) + tol_abs) break;
         if (counter > 49) { st |= 1; break; }
         counter = counter + 1;
         P.V[nv] = Suj;
@@ -350,7 +384,7 @@ void epa(const V3<T> S[4], const int ia[4], const int ib[4], const Body<T>& A, c
             for (int e = 0; e < 3; ++e) {
                 if (!horizon[i][e]) continue;
                 if (nt == EPA_MAX_FACES) { overflow = true; break; }
-                add_triangle(P, P.Tr[i][e], P.Tr[i][(e + 1) % 3], nv - 1, nt);
+                add_triangle(P, P.Tr[i][e], P.Tr[i][(e + 1) % 3], nv - 1, nt, inner);
                 nt = nt + 1;
             }
             if (overflow) break;
@@ -401,10 +435,12 @@ template <typename T>
 ForceOut<T> force_law(const PairIn<T>& in, T depth, V3<T> n, V3<T> cpA, V3<T> cpB, const Law& L) {
     ForceOut<T> o;
     const bool phys = L.mode == 1;
-    if (phys && !(depth < T(EPA_PLACEHOLDER))) {
+    if (phys && !(depth < T(EPA_PLACEHOLDER) && depth > T(0.0))) {
         // EPA ended on a face whose distance is its "outside" or "deleted"
-        // placeholder (1e30 / 3e20): no depth was resolved; the physical law
-        // applies no force for the step (the reference law takes the value)
+        // placeholder (1e30 / 3e20): no depth was resolved; or the signed
+        // depth is not positive (the bodies touch without overlapping): the
+        // physical law applies no force for the step (the reference law
+        // takes the value)
         o.Fn = o.tA = o.tB = o.disp = mk<T>(T(0.0), T(0.0), T(0.0));
         return o;
     }
@@ -526,7 +562,22 @@ PairResult contact_one(const Body<T>& A, const Body<T>& B, const PairIn<T>& in, 
             }
         }
         if (hit) {
-            if (correct_flat(S[0], S[1], S[2], S[3])) {
+            const bool robust = L.mode == 1;
+            if (robust) {
+                // physical law: a flat tetrahedron gets its fourth vertex from
+                // the normal of the first three, on the side where the
+                // Minkowski difference reaches farther (so EPA starts from a
+                // solid; not in the reference)
+                if (flat_tetra(S[0], S[1], S[2], S[3])) {
+                    r.flat = 1;
+                    const V3<T> nT = cross(S[1] - S[0], S[2] - S[0]);
+                    int ap, bp, am, bm;
+                    const V3<T> Sp = support(A, B, nT, &ap, &bp);
+                    const V3<T> Sm = support(A, B, neg(nT), &am, &bm);
+                    if (dot(nT, Sp - S[0]) >= dot(nT, S[0] - Sm)) { S[3] = Sp; ia[3] = ap; ib[3] = bp; }
+                    else { S[3] = Sm; ia[3] = am; ib[3] = bm; }
+                }
+            } else if (correct_flat(S[0], S[1], S[2], S[3])) {
                 r.flat = 1;
                 d4 = cross(S[0], S[1]);
                 S[3] = support(A, B, d4, &ia[3], &ib[3]);
@@ -534,7 +585,7 @@ PairResult contact_one(const Body<T>& A, const Body<T>& B, const PairIn<T>& in, 
             r.collision = 1;
             T depth;
             V3<T> cpA, cpB, nrm;
-            epa(S, ia, ib, A, B, &depth, &cpA, &cpB, &nrm, &r.epa_status, &r.faces, &r.iters, r.face);
+            epa(S, ia, ib, A, B, &depth, &cpA, &cpB, &nrm, &r.epa_status, &r.faces, &r.iters, r.face, robust);
             if (!(val(depth) < EPA_PLACEHOLDER)) r.epa_status |= 4;
             ForceOut<T> f = force_law(in, depth, nrm, cpA, cpB, L);
             r.depth = val(depth);
@@ -1013,8 +1064,8 @@ ORC_EXPORT void orc_gjk_iters(int64_t P, const int32_t* pa, const int32_t* pb, i
 // expansions.
 ORC_EXPORT void orc_gjk_trace(int64_t P, const int32_t* pa, const int32_t* pb, int vmax, const double* com,
                               const double* verts, const int32_t* nverts, int32_t* collision, int32_t* iters,
-                              int32_t* flat, int32_t* epa_status, int32_t* epa_iters) {
-    Law L = make_law(1.0, 0.5, 0.5, 0.5, 1.0);
+                              int32_t* flat, int32_t* epa_status, int32_t* epa_iters, int robust) {
+    Law L = make_law(1.0, 0.5, 0.5, 0.5, 1.0, robust ? 1 : 0);
     for (int64_t p = 0; p < P; ++p) {
         int a = pa[p], b = pb[p];
         Body<double> A{verts + (int64_t)vmax * 3 * a, nverts[a]};
@@ -1049,9 +1100,10 @@ ORC_EXPORT void orc_quat_tripl(const double* q1, const double* v, const double* 
 }
 
 // GJK + EPA of two vertex sets with given centres (contact_detection's GJK
-// driver and EPA, no force law). out: [depth, n(3), cpA(3), cpB(3)].
+// driver and EPA, no force law; robust: the physical law's variant).
+// out: [depth, n(3), cpA(3), cpB(3)].
 ORC_EXPORT int orc_gjk_epa(const double* vA, int nA, const double* vB, int nB, const double* comA, const double* comB,
-                           double* out, int* status) {
+                           double* out, int* status, int robust) {
     Body<double> A{vA, nA};
     Body<double> B{vB, nB};
     PairIn<double> in;
@@ -1060,7 +1112,7 @@ ORC_EXPORT int orc_gjk_epa(const double* vA, int nA, const double* vB, int nB, c
     in.comB = load3<double>(comB);
     in.qA.w = 1.0;
     in.isA = 1.0; in.isB = 1.0;
-    Law L = make_law(1.0, 0.5, 0.5, 0.5, 1.0);
+    Law L = make_law(1.0, 0.5, 0.5, 0.5, 1.0, robust ? 1 : 0);
     PairResult r = contact_one(A, B, in, L);
     out[0] = r.depth;
     for (int k = 0; k < 3; ++k) { out[1 + k] = r.n[k]; out[4 + k] = r.cpA[k]; out[7 + k] = r.cpB[k]; }

@@ -210,16 +210,18 @@ def contact_faces(pairs, st, sc):
     return coll, face
 
 
-def gjk_trace(pairs, com, verts, nverts):
+def gjk_trace(pairs, com, verts, nverts, robust: bool = False):
     """Path of GJK/EPA for each pair of a list: (collision, tetrahedron
-    iterations, CorrectFlat fired (0/1), EPA status, EPA expansions)."""
+    iterations, CorrectFlat fired (0/1), EPA status, EPA expansions).
+    robust: the physical law's variant (flat tetrahedra re-supported along
+    the face normal, EPA faces turned away from an interior point)."""
     pairs = _i32(pairs).reshape(-1, 2)
     P = pairs.shape[0]
     pa, pb = np.ascontiguousarray(pairs[:, 0]), np.ascontiguousarray(pairs[:, 1])
     verts = _f64(verts)
     out = [np.zeros(P, np.int32) for _ in range(5)]
     lib().orc_gjk_trace(C.c_int64(P), _p(pa), _p(pb), C.c_int(verts.shape[1]), _p(_f64(com)), _p(verts),
-                        _p(_i32(nverts)), *[_p(x) for x in out])
+                        _p(_i32(nverts)), *[_p(x) for x in out], C.c_int(1 if robust else 0))
     return tuple(out)
 
 
@@ -341,13 +343,14 @@ def quat_tripl(q1, v, q2) -> np.ndarray:
     return out
 
 
-def gjk_epa(VA, VB, comA, comB):
-    """(collision, depth, normal, cpA, cpB, epa_status) of two vertex sets."""
+def gjk_epa(VA, VB, comA, comB, robust: bool = False):
+    """(collision, depth, normal, cpA, cpB, epa_status) of two vertex sets;
+    robust: the physical law's GJK/EPA variant."""
     VA, VB = _f64(VA), _f64(VB)
     out = np.zeros(10)
     st = C.c_int(0)
     c = lib().orc_gjk_epa(_p(VA), C.c_int(VA.shape[0]), _p(VB), C.c_int(VB.shape[0]), _p(_f64(comA)),
-                          _p(_f64(comB)), _p(out), C.byref(st))
+                          _p(_f64(comB)), _p(out), C.byref(st), C.c_int(1 if robust else 0))
     return c, out[0], out[1:4], out[4:7], out[7:10], st.value
 
 

@@ -255,7 +255,7 @@ def step(st, sc, law_p, law_w):
     pairs = O.candidates(npst, sc)
     coll, depth, _, _, _, _, _, _ = O.contact_state(pairs, npst, sc)
     _, face = O.contact_faces(pairs, npst, sc)
-    hit = (coll == 1) & (depth < EPA_PLACEHOLDER)
+    hit = (coll == 1) & (depth < EPA_PLACEHOLDER) & (depth > 0.0)
     a, b = pairs[hit, 0], pairs[hit, 1]
     F, T = contact_forces(st, sc, law_p, law_w, a, b, face[hit])
     F = F + torch.from_numpy(sc.fext)

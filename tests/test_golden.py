@@ -74,11 +74,25 @@ def test_oracle_flags_unresolved_epa():
     assert c == int(d["collision"]) and depth == float(d["depth"]) and st == int(d["status"])
 
 
+def test_oracle_physical_variant_resolves_that_pair():
+    """The physical law's GJK/EPA variant (faces turned away from an interior
+    point) resolves the same pair: depth and normal of the Minkowski
+    difference's nearest facet (scipy Qhull), status 0."""
+    from scipy.spatial import ConvexHull
+    d = np.load(GOLD / "epa_unresolved.npz")
+    A, B = d["VA"], d["VB"]
+    eq = ConvexHull((A[:, None, :] - B[None, :, :]).reshape(-1, 3)).equations
+    i = int(np.argmax(eq[:, 3]))
+    c, depth, n, ca, cb, st = O.gjk_epa(A, B, d["comA"], d["comB"], robust=True)
+    assert c == 1 and st == 0
+    assert abs(depth + eq[i, 3]) <= 1e-12 and np.abs(n - eq[i, :3]).max() <= 1e-6
+
+
 @pytest.mark.gpu
-def test_gpu_physical_law_skips_unresolved_epa(gpu):
-    """The same pair in a two-hull scene on the physical law: one GPU step
-    equals the oracle's (no force from the unresolved contact) and the
-    workspace reports MWD_F_EPA_UNRESOLVED."""
+def test_gpu_physical_law_resolves_the_unresolved_pair(gpu):
+    """The same pair in a two-hull scene on the physical law: the GPU's
+    variant resolves it as the oracle's does (no MWD_F_EPA_UNRESOLVED) and one
+    GPU step equals the oracle's."""
     from mwdiem_amd import _lib, scene
     from mwdiem_amd.stepper import Stepper
     from tests.helpers import max_rel
@@ -102,7 +116,8 @@ def test_gpu_physical_law_skips_unresolved_epa(gpu):
     pairs, coll = O.step(ref, sc)
     S.step(1)
     s = S.check()
-    assert s["flags"] & _lib.F_EPA_UNRESOLVED
+    assert not s["flags"] & _lib.F_EPA_UNRESOLVED
+    assert np.count_nonzero(coll == 1) >= 1
     got = S.numpy_state()
     for k in ("com", "v", "w", "q"):
         assert max_rel(got[k][W:], ref[k][W:]) <= 1e-12, k
