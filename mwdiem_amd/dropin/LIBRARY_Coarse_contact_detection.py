@@ -1,10 +1,10 @@
 """Drop-in for LIBRARIES/LIBRARY_Coarse_contact_detection.py: Linked_cell_hash
 with the reference signature (:84) and behaviour -- it appends the candidate
 pairs [A, B] of particles num_part_min..num_part_max-1 to the caller's list
-and returns that list -- computed on the GPU (radix sort of cell keys, cell
-tables, emission) in exactly the reference's order: cells in order of their
-lowest id, members in CPython set order, same-cell pairs then the 7 forward
-offsets.
+and returns that list -- computed on the GPU (cell keys counted into a
+hash table of occupied cells, per-cell member lists, pair emission) in
+exactly the reference's order: cells in order of their lowest id, members in
+CPython set order, same-cell pairs then the 7 forward offsets.
 
 Like the reference it reads only hashh_arr[i] and assumes
 particle[i].particle_id == i.

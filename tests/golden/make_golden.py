@@ -4,9 +4,12 @@
 
 Each fixture holds a small seeded scene (inputs, as plain arrays) and the
 oracle's outputs on it: the candidate list, collision codes, EPA results and
-the state after a few full steps. The reference itself cannot run here (it
-needs pycuda, nvcc and a CUDA device), so these vectors come from the oracle;
-they pin the oracle and the GPU path against silent changes.
+the state after a few full steps. The reference's contact and Verlet kernels
+cannot run here (they need pycuda, nvcc and a CUDA device; its pure-Python
+modules -- the cell hash, the writers, the particle class -- do import), so
+these vectors come from the oracle, which the pins and the reference-derived
+fixtures check; they hold the oracle and the GPU path against silent
+changes.
 """
 from __future__ import annotations
 
