@@ -13,9 +13,11 @@
   force vectors (MWDiem.py:732-734 and :1015 are commented out), so they are
   written as its initial integer zeros ("0").
 
-Triangles come from the convex hull of each row's vertices (their rigid
-motion leaves the triangulation unchanged), oriented outward as
-Particle_class.py:109-113 orients faces, computed once per Stepper.
+Triangles are the scene's (SceneData.faces: each hull's triangles as Qhull
+listed them when the generator built it, Particle_class.py:72-87, which the
+reference writes unchanged); for a scene without them, the convex hull of
+each row's vertices, oriented outward, computed once per writer (a rigid
+motion leaves the triangulation unchanged).
 
 AsyncWriter copies the state a snapshot needs into pinned host buffers on a
 side stream (after the work already queued on the stepper's stream) and
@@ -107,7 +109,10 @@ class AsyncWriter:
         self.side = torch.cuda.Stream(device=S.device)  # default priority
         nv = S.t["nverts"].cpu().numpy()
         self.nverts = nv
-        self.faces = hull_faces(S.t["verts"].cpu().numpy(), nv)
+        if all(sc.faces is not None for sc in S.scenes):
+            self.faces = [np.asarray(f, dtype=np.int64) for sc in S.scenes for f in sc.faces]
+        else:
+            self.faces = hull_faces(S.t["verts"].cpu().numpy(), nv)
         self.W = S.scenes[0].n_walls
         self.jobs: queue.Queue = queue.Queue()
         self.error = None

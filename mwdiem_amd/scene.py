@@ -71,6 +71,8 @@ class SceneData:
     name: str = ""
     inner: tuple = (0.0, 0.0, 0.0)   # inner box extent (Lx, Ly, Lz)
     wall_v: np.ndarray | None = None  # [N,3] prescribed wall velocities (walls' rows); None: static walls
+    pairs_per_particle: float | None = None  # candidate pairs to provide for per particle (None: the stepper's default)
+    faces: list | None = None  # per row, the hull's triangles in Qhull order (None: the writer takes the hull's)
 
     @property
     def moving_walls(self) -> bool:
@@ -141,14 +143,18 @@ def random_rotation(rng: np.random.Generator) -> np.ndarray:
     ])
 
 
-def hull_body(points: np.ndarray):
+def hull_body(points: np.ndarray, qhull_faces: bool = False):
     """Hull vertices (in scipy's vertex order), outward-oriented triangles as
-    vertex indices, and the hull volume."""
+    vertex indices, and the hull volume. With qhull_faces, also the triangles
+    as Qhull lists them (facet order, each facet's own vertex order), mapped
+    to the vertex indices: the face ids of Particle_class.py:72-87, which the
+    VTK writer prints."""
     h = ConvexHull(points)
     vidx = h.vertices
     remap = -np.ones(points.shape[0], dtype=np.int64)
     remap[vidx] = np.arange(vidx.size)
     tri = remap[h.simplices]
+    qtri = tri.astype(np.int32)
     V = points[vidx]
     # orient each face outward w.r.t. the vertex mean (Particle_class.py:109-113)
     cm = V.mean(axis=0)
@@ -156,6 +162,8 @@ def hull_body(points: np.ndarray):
     nrm = np.cross(d1 - d0, d2 - d1)
     flip = np.einsum("ij,ij->i", nrm, d0 - cm) < 0
     tri[flip] = tri[flip][:, [0, 2, 1]]
+    if qhull_faces:
+        return V, tri, float(h.volume), qtri
     return V, tri, float(h.volume)
 
 
@@ -199,7 +207,7 @@ def make_particle(rng, npts: int, axes, size: float, density: float, ref_inertia
     expressed in the world frame of the rotated vertices (the frame in which
     q = 1 at t = 0), so geometry and inertia agree."""
     pts = ellipsoid_points(rng, npts, np.asarray(axes) * size)
-    V, tri, vol = hull_body(pts)
+    V, tri, vol, qtri = hull_body(pts, qhull_faces=True)
     cm = V.mean(axis=0)
     V = V - cm
     I, isize = polyhedron_inertia(V, tri, density, ref_inertia)
@@ -210,7 +218,7 @@ def make_particle(rng, npts: int, axes, size: float, density: float, ref_inertia
         Ibf, Iinv = Iw, np.linalg.inv(Iw)
     else:
         Ibf, Iinv = body_frame(I)
-    return Vw, density * vol, Ibf, Iinv, isize, face_normals(Vw, tri)
+    return Vw, density * vol, Ibf, Iinv, isize, face_normals(Vw, tri), qtri
 
 
 _CHUNK = 2048
@@ -268,7 +276,7 @@ def box_corners(lo, hi):
 def walls_for_box(L, T: float, density: float, ref_inertia: bool):
     """Six wall boxes of thickness T around [0,Lx]x[0,Ly]x[0,Lz]:
     floor, ceiling, x-, x+, y-, y+. Returns per-wall (vertices, normal,
-    centre, mass, I, Iinv, isize)."""
+    centre, mass, I, Iinv, isize, Qhull-order faces)."""
     Lx, Ly, Lz = L
     specs = [
         ((-T, -T, -T), (Lx + T, Ly + T, 0.0), (0.0, 0.0, 1.0)),
@@ -281,11 +289,11 @@ def walls_for_box(L, T: float, density: float, ref_inertia: bool):
     out = []
     for lo, hi, n in specs:
         P = box_corners(lo, hi)
-        V, tri, vol = hull_body(P)
+        V, tri, vol, qtri = hull_body(P, qhull_faces=True)
         c = V.mean(axis=0)
         I, isize = polyhedron_inertia(V - c, tri, density, ref_inertia)
         Ibf, Iinv = body_frame(I)
-        out.append((V, np.array(n), c, density * vol, Ibf, Iinv, isize))
+        out.append((V, np.array(n), c, density * vol, Ibf, Iinv, isize, qtri))
     return out
 
 
@@ -454,13 +462,16 @@ def make_scene(n_particles: int, box_mm, seed: int, npts=(12, 12), axes=(1.0, 0.
     isize = np.zeros(N)
     wall_n = np.zeros((N, 3))
     wall_c = np.zeros((N, 3))
-    for k, (V, n, c, m, Ibf, Iiv, isz) in enumerate(walls):
+    faces = []
+    for k, (V, n, c, m, Ibf, Iiv, isz, qf) in enumerate(walls):
+        faces.append(qf)
         com[k] = c
         verts[k, :V.shape[0]] = V
         nverts[k] = V.shape[0]
         mass[k], I[k], Iinv[k], isize[k] = m, Ibf, Iiv, isz
         wall_n[k], wall_c[k] = n, c
-    for i, (V, m, Ibf, Iiv, isz, _) in enumerate(parts):
+    for i, (V, m, Ibf, Iiv, isz, _, qf) in enumerate(parts):
+        faces.append(qf)
         r = W + i
         com[r] = sites[i]
         verts[r, :V.shape[0]] = V + sites[i]
@@ -475,7 +486,8 @@ def make_scene(n_particles: int, box_mm, seed: int, npts=(12, 12), axes=(1.0, 0.
     return SceneData(n_walls=W, vmax=vmax, com=com, verts=verts, nverts=nverts, mass=mass, I=I, Iinv=Iinv,
                      isize=isize, fext=fext, text=np.zeros((N, 3)), wall_n=wall_n, wall_c=wall_c, box=box,
                      cell=cell, ncx=ncx, ncy=ncy, ncz=ncz, kmin=kmin, kspan=kspan, max_particlesize=ms,
-                     law_p=law, law_w=wall_law, dt=dt, name=name, inner=tuple(float(x) for x in box_mm))
+                     law_p=law, law_w=wall_law, dt=dt, name=name, inner=tuple(float(x) for x in box_mm),
+                     faces=faces)
 
 
 # ------------------------------------------------------- the configurations
@@ -522,17 +534,17 @@ class Spawn:
         return min(n_rows, self.n_active + (step // self.every + 1) * self.per)
 
 
-def config_pour(seed: int = 5, n: int = 4096, L=(40.0, 40.0, 60.0), v_pour: float = 2000.0, every: int = 1000,
-                law: Law4 | None = None, npts=(8, 20), size=(0.6, 0.9), dt: float = 1e-6):
-    """A pile formed by pouring: n hulls (the pile's generator: 8-20 points on
-    ellipsoids of semi-axes (1.0, 0.8, 0.6) x U(0.6, 0.9) mm) enter the top of
-    an L mm box in batches of one layer -- a square grid of sites 2.1 x the
-    largest circumradius apart, 1.5 circumradii under the ceiling, random
-    orientation, a small jitter -- moving down at v_pour mm/s (2 m/s by
-    default); a batch every `every` steps, by which time the previous one has
-    fallen clear of the layer. Physical law Kn=1e4, en=et=0.5, mu=0.3.
-    Returns (scene, initial velocities [n_rows, 3], Spawn schedule): rows
-    wait in place until their batch spawns."""
+def config_pour(seed: int = 5, n: int = 4096, L=(40.0, 40.0, 60.0), v_pour: float = 2000.0, every: int | None = None,
+                law: Law4 | None = None, npts=(8, 20), size=(0.8, 1.2), dt: float = 1e-6):
+    """A pile formed by pouring: n hulls (the configurations' generator: 8-20
+    points on ellipsoids of semi-axes (1.0, 0.8, 0.6) x U(0.8, 1.2) mm) enter
+    the top of an L mm box in batches of one layer -- a square grid of sites
+    2.1 x the largest circumradius apart, 1.5 circumradii under the ceiling,
+    random orientation, a small jitter -- moving down at v_pour mm/s (2 m/s by
+    default); a batch every `every` steps (by default the steps the previous
+    batch takes to fall 1.05 grid spacings, clear of the layer). Physical law
+    Kn=1e4, en=et=0.5, mu=0.3. Returns (scene, initial velocities [n_rows, 3],
+    Spawn schedule): rows wait in place until their batch spawns."""
     law = law or Law4(Kn=1e4, en=0.5, et=0.5, mu=0.3, mode=PHYSICAL)
     Lx, Ly, Lz = L
     batch = {}
@@ -554,6 +566,11 @@ def config_pour(seed: int = 5, n: int = 4096, L=(40.0, 40.0, 60.0), v_pour: floa
     v = np.zeros((sc.n_rows, 3))
     v[sc.n_walls:] = rng.normal(0.0, 0.02 * v_pour, size=(n, 3))
     v[sc.n_walls:, 2] -= v_pour
+    if every is None:
+        every = int(math.ceil(1.05 * 2.1 * (sc.max_particlesize / 2) / (v_pour * dt)))
+    # a packed pile of these hulls holds about 100 candidate pairs per hull
+    # on cells twice the largest circumradius wide
+    sc.pairs_per_particle = 200.0
     return sc, v, Spawn(n_active=sc.n_walls, every=every, per=batch["per"])
 
 
@@ -660,7 +677,7 @@ def hopper_scene(n_particles: int, seed: int, pool, law: Law4, orifice: float = 
     isize = np.zeros(N)
     wall_n = np.zeros((N, 3))
     wall_c = np.zeros((N, 3))
-    for k, (V, nrm, c, m, Ik, Iik, isz) in enumerate(walls):
+    for k, (V, nrm, c, m, Ik, Iik, isz, *_) in enumerate(walls):
         com[k] = c
         verts[k, :V.shape[0]] = V
         nverts[k] = V.shape[0]
@@ -759,7 +776,7 @@ def config_polydisperse(seed: int = 3, n: int = 1_000_000, L: float = 400.0, r_r
     isize = np.zeros(N)
     wall_n = np.zeros((N, 3))
     wall_c = np.zeros((N, 3))
-    for k, (V, nrm, c, mw, Ik, Iik, isz) in enumerate(walls):
+    for k, (V, nrm, c, mw, Ik, Iik, isz, *_) in enumerate(walls):
         com[k] = c
         verts[k, :V.shape[0]] = V
         nverts[k] = V.shape[0]

@@ -57,11 +57,14 @@ class Stepper:
         n_mov = sum(sc.n_particles for sc in self.scenes)
         n_wall_tests = sum(sc.n_walls * sc.n_particles for sc in self.scenes)
         # about 32 candidates per particle on a grid of cells one particle
-        # extent wide with the reference's 7 neighbour offsets; the physical
-        # law's 13 offsets visit 13/7 as many neighbour cells, and a wider
-        # grid holds more per cell
-        cand = sum(sc.n_particles * 32 * (13.0 / 7.0 if sc.law_p.mode != 0 else 1.0)
-                   * max(1.0, float(np.prod(sc.cell)) / sc.max_particlesize ** 3) for sc in self.scenes)
+        # extent wide with the reference's 7 neighbour offsets (48 leaves some
+        # room); the physical law's 13 offsets visit 13/7 as many neighbour
+        # cells, and a wider grid holds more per cell. A scene may say what
+        # it needs (scene.pairs_per_particle, e.g. a packed pile).
+        cand = sum(sc.n_particles * (sc.pairs_per_particle if sc.pairs_per_particle else
+                                     48 * (13.0 / 7.0 if sc.law_p.mode != 0 else 1.0)
+                                     * max(1.0, float(np.prod(sc.cell)) / sc.max_particlesize ** 3))
+                   for sc in self.scenes)
         self.pair_cap = int(pair_cap or max(4096, int(cand) + n_wall_tests))
         self.contact_cap = int(contact_cap or max(1024, 8 * n_mov))
         dev = self.device

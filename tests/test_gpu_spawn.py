@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 def _pour(n=300):
     from mwdiem_amd import scene
     # a narrow box poured fast: batches of one layer every 300 steps
-    return scene.config_pour(n=n, L=(12.0, 12.0, 14.0), v_pour=10000.0, every=300)
+    return scene.config_pour(n=n, L=(12.0, 12.0, 14.0), v_pour=10000.0, every=300, size=(0.6, 0.9))
 
 
 def test_spawned_rows_join_on_schedule_and_match_the_oracle(gpu):
