@@ -215,6 +215,11 @@ int mwd_set_reduction(mwd_workspace* ws, int32_t mode);
  *                separate EPA launch over all contacts.
  *   fill_cells   cells per block of the pair emission: 64 (default), 16 or
  *                4 (fewer for beds packed many hulls to a cell).
+ *   gjk_buckets  1: the sphere-test survivors are first sorted by
+ *                ceil(max(nA, nB) / gjk_group) and GJK runs one launch per
+ *                class, each with the vertices-per-lane its class needs
+ *                (uneven hull sizes); 0 (default): one launch sized for the
+ *                largest hull.
  * mwd_set_tuning applies to later steps (graphs captured before keep their
  * launches) and returns MWD_E_INVALID for a choice out of range. */
 #define MWD_CELLS_TABLE 0
@@ -231,6 +236,7 @@ typedef struct mwd_tuning {
     int32_t epa_blocks;
     int32_t epa_fused;
     int32_t fill_cells;
+    int32_t gjk_buckets;
 } mwd_tuning;
 int mwd_get_tuning(const mwd_workspace* ws, mwd_tuning* out);
 int mwd_set_tuning(mwd_workspace* ws, const mwd_tuning* t);

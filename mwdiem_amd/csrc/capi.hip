@@ -49,7 +49,8 @@ void free_ws(mwd_workspace* ws) {
                     ws->vals_alt, ws->cell_start, ws->cell_key, ws->cell_self, ws->cell_seg,
                     ws->row_first, ws->row_rank, ws->cell_cnt, ws->cell_off, ws->perm, ws->key_cnt,
                     ws->row_key, ws->sph,
-                    ws->pairs, ws->collision, ws->simplex, ws->sidx, ws->surv, ws->contact_pair,
+                    ws->pairs, ws->collision, ws->simplex, ws->sidx, ws->surv, ws->surv_cls, ws->bucket,
+                    ws->contact_pair,
                     ws->depth, ws->normal, ws->cpA, ws->cpB, ws->Fn, ws->tA, ws->tB, ws->epa_status, ws->rec_cnt,
                     ws->rec_bkt, ws->rec_ovf, ws->hist,
                     ws->hist_off, ws->scan_tmp, ws->scan_tmp_side, ws->counters, ws->flags, ws->laws,
@@ -406,6 +407,8 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     A(simplex, contact_cap);
     A(sidx, pair_cap);
     A(surv, pair_cap);
+    A(surv_cls, pair_cap);
+    A(bucket, 4 * MWD_GJK_BUCKETS);
     A(contact_pair, contact_cap);
     A(depth, contact_cap);
     A(normal, 3 * contact_cap);
@@ -498,6 +501,7 @@ MWD_API int mwd_get_tuning(const mwd_workspace* ws, mwd_tuning* out) {
     out->epa_blocks = ws->tune.epa_blocks;
     out->epa_fused = ws->tune.epa_fused;
     out->fill_cells = ws->tune.fill_cells;
+    out->gjk_buckets = ws->tune.gjk_buckets;
     return MWD_OK;
 }
 
@@ -517,12 +521,14 @@ MWD_API int mwd_set_tuning(mwd_workspace* ws, const mwd_tuning* t) {
     const bool pack_ok = t->vertex_pack == 0 || t->vertex_pack == 1;
     const bool epa_ok = t->epa_blocks >= 16 && t->epa_blocks <= 65536 && (t->epa_fused == 0 || t->epa_fused == 1);
     const bool fill_ok = t->fill_cells == 4 || t->fill_cells == 16 || t->fill_cells == 64;
+    const bool buckets_ok = t->gjk_buckets == 0 || t->gjk_buckets == 1;
     if (!pass_ok || !group_ok || !blocks_ok || !sphere_ok || !verlet_ok || !items_ok || !waves_ok || !pack_ok ||
-        !epa_ok || !fill_ok) {
+        !epa_ok || !fill_ok || !buckets_ok) {
         mwd_set_error(!pass_ok ? "mwd_set_tuning: cell_pass unknown, MWD_CELLS_TABLE without key tables, or "
                                  "MWD_CELLS_SORTED with a spawn schedule"
                                : "mwd_set_tuning: gjk_group, gjk_blocks, sphere_block, verlet_block, cell_items, "
-                                 "verlet_waves, vertex_pack, epa_blocks, epa_fused or fill_cells out of range");
+                                 "verlet_waves, vertex_pack, epa_blocks, epa_fused, fill_cells or gjk_buckets "
+                                 "out of range");
         return MWD_E_INVALID;
     }
     ws->cell_table = t->cell_pass == MWD_CELLS_TABLE;
@@ -536,6 +542,7 @@ MWD_API int mwd_set_tuning(mwd_workspace* ws, const mwd_tuning* t) {
     ws->tune.epa_blocks = t->epa_blocks;
     ws->tune.epa_fused = t->epa_fused;
     ws->tune.fill_cells = t->fill_cells;
+    ws->tune.gjk_buckets = t->gjk_buckets;
     return MWD_OK;
 }
 
@@ -643,21 +650,43 @@ static int fine_contact(mwd_workspace* ws, const mwd_state* st, double* const* t
     // a colliding pair claims contact slot c (counter 4) and writes
     // contact_pair[c]: contacts are in no set order; the per-row sums key
     // their records by pair index, so they are summed in candidate order
+    // GJK per class of vertex count (gjk_buckets): the survivors sorted by
+    // class, then one launch per class over its range of the sorted list,
+    // each with the vertices per lane its class needs; results do not depend
+    // on the order in which pairs are processed
+    const int G = ws->tune.gjk_group;
+    const int Kv = (ws->vmax + G - 1) / G;
+    const int nb = ws->tune.gjk_buckets ? (Kv < MWD_GJK_BUCKETS ? Kv : MWD_GJK_BUCKETS) : 1;
+    const int32_t* surv = ws->surv;
+    if (nb > 1) {
+        if ((rc = mwd_launch_gjk_buckets(A, B, ws->surv, ctr + 10, ws->pair_cap, G, nb, ws->bucket, ws->surv_cls, s)))
+            return rc;
+        surv = ws->surv_cls;
+    }
+    const int64_t* ofs = ws->bucket + 2 * MWD_GJK_BUCKETS;
+    hipEvent_t g0 = mwd_kev(ws, MWD_KT_GJK, 0), g1 = mwd_kev(ws, MWD_KT_GJK, 1);
     bool fused = false;
-    if (!tan && ws->tune.epa_fused) {
-        rc = mwd_launch_gjk_epa(A, B, ws->pair_cap, ws->collision, ws->surv, ctr + 10, ws->simplex, ws->sidx,
-                                ctr + 4, ws->contact_cap, ws->contact_pair, ws->laws, ws->depth, ws->normal, ws->cpA,
-                                ws->cpB, contact_out(ws, false), ws->epa_status, ws->rec_cnt, ws->rec_bkt, ws->rec_ovf,
-                                ctr + 11, 2 * ws->contact_cap, ws->flags, s, mwd_kev(ws, MWD_KT_GJK, 0),
-                                mwd_kev(ws, MWD_KT_GJK, 1), ws->tune);
-        if (rc < 0) return rc;
+    if (!tan && ws->tune.epa_fused && Kv <= 4) {
+        for (int b = 0; b < nb; ++b) {
+            rc = mwd_launch_gjk_epa(A, B, ws->pair_cap, ws->collision, surv, nb > 1 ? ofs + b + 1 : ctr + 10,
+                                    ws->simplex, ws->sidx, ctr + 4, ws->contact_cap, ws->contact_pair, ws->laws,
+                                    ws->depth, ws->normal, ws->cpA, ws->cpB, contact_out(ws, false), ws->epa_status,
+                                    ws->rec_cnt, ws->rec_bkt, ws->rec_ovf, ctr + 11, 2 * ws->contact_cap, ws->flags,
+                                    s, b == 0 ? g0 : nullptr, b == nb - 1 ? g1 : nullptr, ws->tune,
+                                    nb > 1 ? ofs + b : nullptr, nb > 1 ? (b == nb - 1 ? Kv : b + 1) : 0);
+            if (rc < 0) return rc;
+            if (rc > 0) break;  // not this form (nothing launched: b == 0)
+        }
         fused = rc == 0;
     }
     if (!fused) {
-        rc = mwd_launch_gjk_surv(A, B, ws->pair_cap, ws->collision, ws->surv, ctr + 10, ws->simplex, ws->sidx,
-                                 ctr + 4, ws->contact_cap, ws->flags, s, mwd_kev(ws, MWD_KT_GJK, 0),
-                                 mwd_kev(ws, MWD_KT_GJK, 1), ws->tune, ws->contact_pair);
-        if (rc) return rc;
+        for (int b = 0; b < nb; ++b) {
+            rc = mwd_launch_gjk_surv(A, B, ws->pair_cap, ws->collision, surv, nb > 1 ? ofs + b + 1 : ctr + 10,
+                                     ws->simplex, ws->sidx, ctr + 4, ws->contact_cap, ws->flags, s,
+                                     b == 0 ? g0 : nullptr, b == nb - 1 ? g1 : nullptr, ws->tune, ws->contact_pair,
+                                     nb > 1 ? ofs + b : nullptr, nb > 1 ? (b == nb - 1 ? Kv : b + 1) : 0);
+            if (rc) return rc;
+        }
     }
     mwd_mark(ws, 7);  // (no compaction stage: kept for the stage numbering)
     mwd_mark(ws, 8);

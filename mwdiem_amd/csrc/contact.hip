@@ -513,6 +513,89 @@ __global__ void __launch_bounds__(1024) k_sphere(BodyView A, BodyView B, int64_t
     }
 }
 
+// Survivors [b, e) of a GJK launch: [*lo, *hi) (lo NULL: from 0), within
+// the list's capacity.
+__device__ __forceinline__ void surv_range(const int64_t* lo, const int64_t* hi, int64_t n_cap, int64_t* b,
+                                           int64_t* e) {
+    int64_t x = *hi;
+    if (x > n_cap) x = n_cap;
+    int64_t y = lo ? *lo : 0;
+    if (y > x) y = x;
+    *b = y;
+    *e = x;
+}
+
+// GJK class of a survivor: ceil(max(nA, nB) / G) - 1 vertices-per-lane
+// steps, the last class taking the rest.
+__device__ __forceinline__ int gjk_class(const BodyView& A, const BodyView& B, int64_t p, int G, int nb) {
+    const int32_t na = A.nverts[view_row(A, p)], nb_ = B.nverts[view_row(B, p)];
+    const int v = na > nb_ ? na : nb_;
+    int k = (v + G - 1) / G - 1;
+    if (k < 0) k = 0;
+    if (k > nb - 1) k = nb - 1;
+    return k;
+}
+
+// Survivor counts per GJK class (LDS histogram, one global add per class and
+// block). cnt[nb] zero on entry.
+__global__ void __launch_bounds__(256) k_bucket_count(BodyView A, BodyView B, const int32_t* __restrict__ surv,
+                                                      const int64_t* n_surv, int64_t n_cap, int G, int nb,
+                                                      int64_t* cnt) {
+    __shared__ int c[MWD_GJK_BUCKETS];
+    if (threadIdx.x < MWD_GJK_BUCKETS) c[threadIdx.x] = 0;
+    __syncthreads();
+    int64_t n = *n_surv;
+    if (n > n_cap) n = n_cap;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        atomicAdd(&c[gjk_class(A, B, surv[i], G, nb)], 1);
+    __syncthreads();
+    if ((int)threadIdx.x < nb && c[threadIdx.x]) atomicAdd((unsigned long long*)&cnt[threadIdx.x], (unsigned long long)c[threadIdx.x]);
+}
+
+// Survivors written class after class into out (any order within a class):
+// each block ranks its chunk's members per class in LDS and reserves one
+// range per class with a global add on cur[nb] (zero on entry). ofs[nb + 1]
+// receives the classes' start offsets (block 0).
+__global__ void __launch_bounds__(256) k_bucket_scatter(BodyView A, BodyView B, const int32_t* __restrict__ surv,
+                                                        const int64_t* n_surv, int64_t n_cap, int G, int nb,
+                                                        const int64_t* cnt, int64_t* cur, int64_t* ofs,
+                                                        int32_t* __restrict__ out) {
+    __shared__ int64_t off[MWD_GJK_BUCKETS + 1];
+    __shared__ int64_t base[MWD_GJK_BUCKETS];
+    __shared__ int lc[MWD_GJK_BUCKETS];
+    if (threadIdx.x == 0) {
+        int64_t a = 0;
+        for (int k = 0; k < nb; ++k) {
+            off[k] = a;
+            a += cnt[k];
+        }
+        off[nb] = a;
+        if (blockIdx.x == 0)
+            for (int k = 0; k <= nb; ++k) ofs[k] = off[k];
+    }
+    int64_t n = *n_surv;
+    if (n > n_cap) n = n_cap;
+    const int tid = threadIdx.x;
+    // a uniform trip count over the block (its barriers)
+    for (int64_t s0 = (int64_t)blockIdx.x * blockDim.x; s0 < n; s0 += (int64_t)gridDim.x * blockDim.x) {
+        if (tid < MWD_GJK_BUCKETS) lc[tid] = 0;
+        __syncthreads();
+        int k = -1, r = 0;
+        int32_t p = 0;
+        if (s0 + tid < n) {
+            p = surv[s0 + tid];
+            k = gjk_class(A, B, p, G, nb);
+            r = atomicAdd(&lc[k], 1);
+        }
+        __syncthreads();
+        if (tid < nb) base[tid] = lc[tid] ? off[tid] + (int64_t)atomicAdd((unsigned long long*)&cur[tid], (unsigned long long)lc[tid]) : 0;
+        __syncthreads();
+        if (k >= 0) out[base[k] + r] = p;
+        __syncthreads();
+    }
+}
+
 // GJK over the survivors (.cu:983-1322, without EPA), G lanes per pair
 // (GroupSup); every lane of a group follows the same path on the same
 // values, and its first lane writes the results. K = 0 selects one lane per
@@ -538,17 +621,17 @@ __device__ __forceinline__ int gjk_of(const BodyView& A, const BodyView& B, int6
 
 template <int G, int K>
 __global__ void __launch_bounds__(256) k_gjk(BodyView A, BodyView B, int64_t n_cap, const int32_t* __restrict__ surv,
-                                             const int64_t* n_surv, int32_t* __restrict__ collision,
-                                             Simplex* __restrict__ simplex,
+                                             const int64_t* lo, const int64_t* n_surv,
+                                             int32_t* __restrict__ collision, Simplex* __restrict__ simplex,
                                              int32_t* __restrict__ sidx, int64_t* slots, int64_t slot_cap,
                                              uint32_t* flags, int32_t* __restrict__ contact_pair) {
-    int64_t m = *n_surv;
-    if (m > n_cap) m = n_cap;
+    int64_t b0, m;
+    surv_range(lo, n_surv, n_cap, &b0, &m);
     constexpr int GG = K > 0 ? G : 1;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int g = (int)(t % GG);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x / GG;
-    for (int64_t s = t / GG; s < m; s += stride) {
+    for (int64_t s = b0 + t / GG; s < m; s += stride) {
         const int64_t p = surv[s];
         d3 S[4];
         int ia[4] = {0, 0, 0, 0}, ib[4] = {0, 0, 0, 0};
@@ -1332,17 +1415,17 @@ __global__ void __launch_bounds__(256) k_epa_force(BodyView A, BodyView B, BodyT
 // per-row sums do not depend on which wave ran a contact).
 template <int G, int K>
 __global__ void __launch_bounds__(256) k_gjk_epa(BodyView A, BodyView B, BodyTan bt, int64_t n_cap,
-                                                 const int32_t* __restrict__ surv, const int64_t* n_surv,
-                                                 int32_t* __restrict__ collision, Simplex* __restrict__ simplex,
-                                                 int32_t* __restrict__ sidx, int64_t* slots, int64_t slot_cap,
-                                                 uint32_t* flags, int32_t* __restrict__ contact_pair,
-                                                 ContactSink sink) {
+                                                 const int32_t* __restrict__ surv, const int64_t* lo,
+                                                 const int64_t* n_surv, int32_t* __restrict__ collision,
+                                                 Simplex* __restrict__ simplex, int32_t* __restrict__ sidx,
+                                                 int64_t* slots, int64_t slot_cap, uint32_t* flags,
+                                                 int32_t* __restrict__ contact_pair, ContactSink sink) {
     static_assert(K > 0 && MWD_WAVE % G == 0, "grouped support only");
     constexpr int PW = MWD_WAVE / G;  // pairs per wave and round
     __shared__ PolyW polys[256 / MWD_WAVE];
     __shared__ Simplex parked[256 / MWD_WAVE][PW];
-    int64_t m = *n_surv;
-    if (m > n_cap) m = n_cap;
+    int64_t b0, m;
+    surv_range(lo, n_surv, n_cap, &b0, &m);
     const int lane = threadIdx.x & (MWD_WAVE - 1);
     const int w = threadIdx.x / MWD_WAVE;
     const int g = lane % G;
@@ -1351,7 +1434,7 @@ __global__ void __launch_bounds__(256) k_gjk_epa(BodyView A, BodyView B, BodyTan
     uint32_t fl = 0;
     // s0, the wave's first survivor of a round, is the same on every lane, so
     // the whole wave reaches each ballot and each EPA
-    for (int64_t s0 = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x - lane)) / G; s0 < m; s0 += stride) {
+    for (int64_t s0 = b0 + ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x - lane)) / G; s0 < m; s0 += stride) {
         const int64_t s = s0 + lane / G;
         int32_t p = 0;
         int64_t slot = -1;
@@ -1504,34 +1587,34 @@ constexpr int GJK_MAX_K = 6;
 
 template <int G, int K>
 void launch_gjk_k(int k, dim3 grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const BodyView& A,
-                  const BodyView& B, int64_t n_cap, const int32_t* surv, const int64_t* n_surv, int32_t* collision,
-                  Simplex* simplex, int32_t* sidx, int64_t* slots, int64_t slot_cap, uint32_t* flags,
-                  int32_t* contact_pair) {
+                  const BodyView& B, int64_t n_cap, const int32_t* surv, const int64_t* lo, const int64_t* n_surv,
+                  int32_t* collision, Simplex* simplex, int32_t* sidx, int64_t* slots, int64_t slot_cap,
+                  uint32_t* flags, int32_t* contact_pair) {
     if constexpr (K > GJK_MAX_K) {
-        MWD_LAUNCH_EV(e0, e1, (k_gjk<G, 0>), grid, dim3(256), 0, s, A, B, n_cap, surv, n_surv, collision, simplex,
-                      sidx, slots, slot_cap, flags, contact_pair);
+        MWD_LAUNCH_EV(e0, e1, (k_gjk<G, 0>), grid, dim3(256), 0, s, A, B, n_cap, surv, lo, n_surv, collision,
+                      simplex, sidx, slots, slot_cap, flags, contact_pair);
     } else {
         if (k == K)
-            MWD_LAUNCH_EV(e0, e1, (k_gjk<G, K>), grid, dim3(256), 0, s, A, B, n_cap, surv, n_surv, collision,
+            MWD_LAUNCH_EV(e0, e1, (k_gjk<G, K>), grid, dim3(256), 0, s, A, B, n_cap, surv, lo, n_surv, collision,
                           simplex, sidx, slots, slot_cap, flags, contact_pair);
         else
-            launch_gjk_k<G, K + 1>(k, grid, s, e0, e1, A, B, n_cap, surv, n_surv, collision, simplex, sidx, slots,
-                                   slot_cap, flags, contact_pair);
+            launch_gjk_k<G, K + 1>(k, grid, s, e0, e1, A, B, n_cap, surv, lo, n_surv, collision, simplex, sidx,
+                                   slots, slot_cap, flags, contact_pair);
     }
 }
 
 template <int G>
 void launch_gjk_g(const BodyView& A, const BodyView& B, int64_t n_cap, int32_t* collision, const int32_t* surv,
-                  const int64_t* n_surv, Simplex* simplex, int32_t* sidx, int64_t* slots, int64_t slot_cap,
-                  uint32_t* flags, hipStream_t s, hipEvent_t e0, hipEvent_t e1, int max_blocks,
-                  int32_t* contact_pair) {
-    const int K = (A.vmax + G - 1) / G;
+                  const int64_t* lo, const int64_t* n_surv, Simplex* simplex, int32_t* sidx, int64_t* slots,
+                  int64_t slot_cap, uint32_t* flags, hipStream_t s, hipEvent_t e0, hipEvent_t e1, int max_blocks,
+                  int32_t* contact_pair, int K_class) {
+    const int K = K_class > 0 ? K_class : (A.vmax + G - 1) / G;
     const bool grouped = K <= GJK_MAX_K;
     // the survivors are few next to the candidates: a grid-strided launch
     // with a capped grid
     const dim3 grid(grid_for(n_cap * (grouped ? G : 1), 256, max_blocks));
-    launch_gjk_k<G, 1>(grouped ? K : GJK_MAX_K + 1, grid, s, e0, e1, A, B, n_cap, surv, n_surv, collision, simplex,
-                       sidx, slots, slot_cap, flags, contact_pair);
+    launch_gjk_k<G, 1>(grouped ? K : GJK_MAX_K + 1, grid, s, e0, e1, A, B, n_cap, surv, lo, n_surv, collision,
+                       simplex, sidx, slots, slot_cap, flags, contact_pair);
 }
 
 }  // namespace
@@ -1539,20 +1622,20 @@ void launch_gjk_g(const BodyView& A, const BodyView& B, int64_t n_cap, int32_t* 
 int mwd_launch_gjk_surv(const BodyView& A, const BodyView& B, int64_t n_cap, int32_t* collision,
                         const int32_t* surv, const int64_t* n_surv, Simplex* simplex, int32_t* sidx, int64_t* slots,
                         int64_t slot_cap, uint32_t* flags, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
-                        const PairTune& tu, int32_t* contact_pair) {
+                        const PairTune& tu, int32_t* contact_pair, const int64_t* lo, int K_class) {
     if (n_cap <= 0) return MWD_OK;
     switch (tu.gjk_group) {
         case 4:
-            launch_gjk_g<4>(A, B, n_cap, collision, surv, n_surv, simplex, sidx, slots, slot_cap, flags, s, e0, e1,
-                            tu.gjk_blocks, contact_pair);
+            launch_gjk_g<4>(A, B, n_cap, collision, surv, lo, n_surv, simplex, sidx, slots, slot_cap, flags, s, e0,
+                            e1, tu.gjk_blocks, contact_pair, K_class);
             break;
         case 16:
-            launch_gjk_g<16>(A, B, n_cap, collision, surv, n_surv, simplex, sidx, slots, slot_cap, flags, s, e0, e1,
-                            tu.gjk_blocks, contact_pair);
+            launch_gjk_g<16>(A, B, n_cap, collision, surv, lo, n_surv, simplex, sidx, slots, slot_cap, flags, s, e0,
+                             e1, tu.gjk_blocks, contact_pair, K_class);
             break;
         default:
-            launch_gjk_g<8>(A, B, n_cap, collision, surv, n_surv, simplex, sidx, slots, slot_cap, flags, s, e0, e1,
-                            tu.gjk_blocks, contact_pair);
+            launch_gjk_g<8>(A, B, n_cap, collision, surv, lo, n_surv, simplex, sidx, slots, slot_cap, flags, s, e0,
+                            e1, tu.gjk_blocks, contact_pair, K_class);
     }
     MWD_LAUNCH_CHECK();
     return MWD_OK;
@@ -1648,32 +1731,33 @@ namespace {
 
 template <int G, int K>
 void launch_gjk_epa_k(int k, dim3 grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const BodyView& A,
-                      const BodyView& B, const BodyTan& bt, int64_t n_cap, const int32_t* surv,
+                      const BodyView& B, const BodyTan& bt, int64_t n_cap, const int32_t* surv, const int64_t* lo,
                       const int64_t* n_surv, int32_t* collision, Simplex* simplex, int32_t* sidx, int64_t* slots,
                       int64_t slot_cap, uint32_t* flags, int32_t* contact_pair, const ContactSink& sink) {
     if constexpr (K <= 4) {
         if (k == K)
-            MWD_LAUNCH_EV(e0, e1, (k_gjk_epa<G, K>), grid, dim3(256), 0, s, A, B, bt, n_cap, surv, n_surv,
+            MWD_LAUNCH_EV(e0, e1, (k_gjk_epa<G, K>), grid, dim3(256), 0, s, A, B, bt, n_cap, surv, lo, n_surv,
                           collision, simplex, sidx, slots, slot_cap, flags, contact_pair, sink);
         else
-            launch_gjk_epa_k<G, K + 1>(k, grid, s, e0, e1, A, B, bt, n_cap, surv, n_surv, collision, simplex, sidx,
-                                       slots, slot_cap, flags, contact_pair, sink);
+            launch_gjk_epa_k<G, K + 1>(k, grid, s, e0, e1, A, B, bt, n_cap, surv, lo, n_surv, collision, simplex,
+                                       sidx, slots, slot_cap, flags, contact_pair, sink);
     }
 }
 
 template <int G>
-int launch_gjk_epa_g(const BodyView& A, const BodyView& B, int64_t n_cap, const int32_t* surv, const int64_t* n_surv,
-                     int32_t* collision, Simplex* simplex, int32_t* sidx, int64_t* slots, int64_t slot_cap,
-                     uint32_t* flags, int32_t* contact_pair, const ContactSink& sink, hipStream_t s, hipEvent_t e0,
-                     hipEvent_t e1, int max_blocks) {
+int launch_gjk_epa_g(const BodyView& A, const BodyView& B, int64_t n_cap, const int32_t* surv, const int64_t* lo,
+                     const int64_t* n_surv, int32_t* collision, Simplex* simplex, int32_t* sidx, int64_t* slots,
+                     int64_t slot_cap, uint32_t* flags, int32_t* contact_pair, const ContactSink& sink, hipStream_t s,
+                     hipEvent_t e0, hipEvent_t e1, int max_blocks, int K_class) {
     // beyond 4 vertices per lane the fused kernel's registers allow one wave
     // per SIMD: the two launches are faster there
-    const int K = (A.vmax + G - 1) / G;
-    if (K < 1 || K > 4) return 1;
+    const int Kv = (A.vmax + G - 1) / G;
+    if (Kv < 1 || Kv > 4) return 1;
+    const int K = K_class > 0 ? K_class : Kv;
     BodyTan none;
     memset(&none, 0, sizeof(none));
-    launch_gjk_epa_k<G, 1>(K, dim3(grid_for(n_cap * G, 256, max_blocks)), s, e0, e1, A, B, none, n_cap, surv, n_surv,
-                           collision, simplex, sidx, slots, slot_cap, flags, contact_pair, sink);
+    launch_gjk_epa_k<G, 1>(K, dim3(grid_for(n_cap * G, 256, max_blocks)), s, e0, e1, A, B, none, n_cap, surv, lo,
+                           n_surv, collision, simplex, sidx, slots, slot_cap, flags, contact_pair, sink);
     return 0;
 }
 
@@ -1684,7 +1768,7 @@ int mwd_launch_gjk_epa(const BodyView& A, const BodyView& B, int64_t n_cap, int3
                        int32_t* contact_pair, const mwd_law* laws2, double* depth, double* normal, double* cpA,
                        double* cpB, const ContactOut& co, int32_t* epa_status, int32_t* rec_cnt, int32_t* rec_bkt,
                        int32_t* rec_ovf, int64_t* n_ovf, int64_t ovf_cap, uint32_t* flags, hipStream_t s,
-                       hipEvent_t e0, hipEvent_t e1, const PairTune& tu) {
+                       hipEvent_t e0, hipEvent_t e1, const PairTune& tu, const int64_t* lo, int K_class) {
     if (A.vmax != B.vmax || !contact_pair || !rec_cnt) return 1;
     if (n_cap <= 0) return 0;
     const ContactSink sink{laws2,   nullptr, nullptr, nullptr, nullptr, depth,   normal,  cpA,   cpB,    co,
@@ -1692,20 +1776,35 @@ int mwd_launch_gjk_epa(const BodyView& A, const BodyView& B, int64_t n_cap, int3
     int r;
     switch (tu.gjk_group) {
         case 4:
-            r = launch_gjk_epa_g<4>(A, B, n_cap, surv, n_surv, collision, simplex, sidx, slots, slot_cap, flags,
-                                    contact_pair, sink, s, e0, e1, tu.gjk_blocks);
+            r = launch_gjk_epa_g<4>(A, B, n_cap, surv, lo, n_surv, collision, simplex, sidx, slots, slot_cap, flags,
+                                    contact_pair, sink, s, e0, e1, tu.gjk_blocks, K_class);
             break;
         case 16:
-            r = launch_gjk_epa_g<16>(A, B, n_cap, surv, n_surv, collision, simplex, sidx, slots, slot_cap, flags,
-                                     contact_pair, sink, s, e0, e1, tu.gjk_blocks);
+            r = launch_gjk_epa_g<16>(A, B, n_cap, surv, lo, n_surv, collision, simplex, sidx, slots, slot_cap,
+                                     flags, contact_pair, sink, s, e0, e1, tu.gjk_blocks, K_class);
             break;
         default:
-            r = launch_gjk_epa_g<8>(A, B, n_cap, surv, n_surv, collision, simplex, sidx, slots, slot_cap, flags,
-                                    contact_pair, sink, s, e0, e1, tu.gjk_blocks);
+            r = launch_gjk_epa_g<8>(A, B, n_cap, surv, lo, n_surv, collision, simplex, sidx, slots, slot_cap, flags,
+                                    contact_pair, sink, s, e0, e1, tu.gjk_blocks, K_class);
     }
     if (r) return r;
     MWD_LAUNCH_CHECK();
     return 0;
+}
+
+int mwd_launch_gjk_buckets(const BodyView& A, const BodyView& B, const int32_t* surv, const int64_t* n_surv,
+                           int64_t n_cap, int G, int nb, int64_t* bucket, int32_t* out, hipStream_t s) {
+    if (nb < 1 || nb > MWD_GJK_BUCKETS) return MWD_E_INVALID;
+    int rc;
+    // counts and cursors start at zero (a fill kernel: no memset node in a graph)
+    if ((rc = mwd_fill_i32(reinterpret_cast<int32_t*>(bucket), 0, 4 * MWD_GJK_BUCKETS, s))) return rc;
+    const dim3 grid(grid_for(n_cap, 256, 1024));
+    hipLaunchKernelGGL(k_bucket_count, grid, dim3(256), 0, s, A, B, surv, n_surv, n_cap, G, nb, bucket);
+    MWD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_bucket_scatter, grid, dim3(256), 0, s, A, B, surv, n_surv, n_cap, G, nb, bucket,
+                       bucket + MWD_GJK_BUCKETS, bucket + 2 * MWD_GJK_BUCKETS, out);
+    MWD_LAUNCH_CHECK();
+    return MWD_OK;
 }
 
 int mwd_launch_segments(const uint32_t* rec_key, int64_t rec_cap, const int64_t* n_rec, int32_t* seg_begin,

@@ -505,8 +505,18 @@ struct WallIO {
     const int64_t* wt_off;
     int32_t* pairs;
     int64_t cap;
-    int32_t* surv;  // non-NULL: every wall pair goes to GJK as survivor p (a wall's bounding sphere,
-                    // half the plate's diagonal, meets nearly every particle in its slab)
+    // non-NULL (with bounding radii): the wall pairs go to GJK as survivors,
+    // listed in surv from its start, except pairs the wall's slab separates
+    // from the particle's bounding sphere, which get collision 0 here (a
+    // wall plate's bounding sphere, half its diagonal, meets nearly every
+    // particle in its slab, so the sphere test decides nothing for them)
+    int32_t* surv;
+    int32_t* collision;
+    const double* radius;
+    const double* verts;
+    const int32_t* nverts;
+    int32_t vmax;
+    int64_t* counters;  // [1] wall pairs, [17] wall survivors (written by finish)
     __device__ __forceinline__ void decode(int64_t t, const SceneDev** sc, int64_t* rj, int64_t* rk) const {
         const int s = scene_of_test(wt_off, S, t);
         *sc = &scenes[s];
@@ -519,24 +529,46 @@ struct WallIO {
         *rk = (*sc)->row0 + k;
         *rj = (*sc)->row0 + (*sc)->n_walls + j;
     }
-    __device__ __forceinline__ int32_t load(int64_t t) const {
+    // The wall's slab |n.(x - c)| <= h (h: the half-extent of its vertices
+    // along n) and the particle's bounding sphere are apart by more than
+    // 1e-6 (1 + |dd| + h): every particle vertex lies outside the slab, the
+    // bodies are separated and GJK would return 0.
+    __device__ __forceinline__ bool apart(int64_t rj, int64_t rk, d3 n, double dd) const {
+        const d3 c = ld3(wall_c + 3 * rk);
+        const double* v = verts + (int64_t)vmax * 3 * rk;
+        double h = 0.0;
+        for (int i = 0; i < nverts[rk]; ++i) h = fmax(h, fabs(dot3(n, ld3(v + 3 * i) - c)));
+        const double a = fabs(dd);
+        return a - h - radius[rj] > 1e-6 * (1.0 + a + h);
+    }
+    // packed: bit 0 the pair is listed, bit 32 it goes to GJK
+    __device__ __forceinline__ int64_t load(int64_t t) const {
         const SceneDev* sc;
         int64_t rj, rk;
         decode(t, &sc, &rj, &rk);
-        const d3 d = ld3(com + 3 * rj) - ld3(wall_c + 3 * rk);
-        const double dd = dot3(ld3(wall_n + 3 * rk), d);
-        return fabs(dd) < sc->wall_thr ? 1 : 0;
+        const d3 n = ld3(wall_n + 3 * rk);
+        const double dd = dot3(n, ld3(com + 3 * rj) - ld3(wall_c + 3 * rk));
+        if (!(fabs(dd) < sc->wall_thr)) return 0;
+        if (!surv || apart(rj, rk, n, dd)) return 1;
+        return (int64_t(1) << 32) | 1;
     }
-    __device__ __forceinline__ void store(int64_t t, int64_t pos, int32_t v) const {
+    __device__ __forceinline__ void store(int64_t t, int64_t off, int64_t v) const {
+        const int64_t pos = off & 0xffffffffll;
         if (!v || pos >= cap) return;
         const SceneDev* sc;
         int64_t rj, rk;
         decode(t, &sc, &rj, &rk);
         pairs[2 * pos] = (int32_t)rj;
         pairs[2 * pos + 1] = (int32_t)rk;
-        if (surv) surv[pos] = (int32_t)pos;
+        if (surv) {
+            if (v >> 32) surv[off >> 32] = (int32_t)pos;
+            else collision[pos] = 0;
+        }
     }
-    __device__ __forceinline__ void finish(int64_t) const {}
+    __device__ __forceinline__ void finish(int64_t total) const {
+        counters[1] = total & 0xffffffffll;
+        counters[17] = total >> 32;
+    }
 };
 
 // Sorted path: cell heads over the sorted keys fused with their numbering:
@@ -605,9 +637,9 @@ struct CellTableIO {
 __device__ __forceinline__ void close_pairs(int64_t* counters, int64_t cell_pairs, int64_t cap, uint32_t* flags,
                                             bool folded) {
     const int64_t P = counters[1] + cell_pairs;
-    // survivors of the sphere test: the wall pairs listed so far when the
-    // emission appends the rest, else none yet
-    counters[10] = folded ? (counters[1] < cap ? counters[1] : cap) : 0;
+    // survivors of the sphere test: the wall pairs listed for GJK so far
+    // when the emission appends the rest, else none yet
+    counters[10] = folded ? (counters[17] < cap ? counters[17] : cap) : 0;
     counters[3] = P < cap ? P : cap;
     counters[7] = P;
     counters[4] = 0;   // contacts (slots claimed by GJK) of the fine-contact stage
@@ -698,8 +730,10 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
     // counters 0..3 = n_cells, n_wall_pairs, n_cell_pairs, n_pairs
     mwd_mark(ws, 0);
     const bool walls = ws->n_wall_tests > 0 && st->wall_n && st->wall_c;
-    if (!walls || M <= 0) {  // otherwise the scans below write counters 0..3
+    if (!walls || M <= 0) {  // otherwise the scans below write counters 0..3 and 17
         hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(64), 0, s, ctr, 4);
+        MWD_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(64), 0, s, ctr + 17, 1);
         MWD_LAUNCH_CHECK();
     }
     // with bounding radii the survivors of the sphere test are listed as the
@@ -711,11 +745,13 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
     // pair count (the offset of the cell pairs)
     bool joined = true;
     if (walls) {
-        const WallIO io{st->com,   st->wall_n,    st->wall_c, ws->scenes_dev, ws->n_scenes, ws->scene_wt_off,
-                        ws->pairs, ws->pair_cap, st->radius ? ws->surv : nullptr};
+        const WallIO io{st->com,      st->wall_n,    st->wall_c,   ws->scenes_dev, ws->n_scenes,
+                        ws->scene_wt_off, ws->pairs, ws->pair_cap, st->radius ? ws->surv : nullptr,
+                        ws->collision, st->radius,   st->verts,    st->nverts,     st->vmax,
+                        ctr};
         MWD_HIP_CHECK(hipEventRecord(ws->ev_fork, s));
         MWD_HIP_CHECK(hipStreamWaitEvent(ws->side, ws->ev_fork, 0));
-        rc = mwd_scan_io(io, ws->n_wall_tests, ws->wt_live, ctr + 1, ws->scan_tmp_side, ws->scan_tmp_side_len,
+        rc = mwd_scan_io(io, ws->n_wall_tests, ws->wt_live, nullptr, ws->scan_tmp_side, ws->scan_tmp_side_len,
                          ws->side);
         if (rc) return rc;
         MWD_HIP_CHECK(hipEventRecord(ws->ev_join, ws->side));

@@ -135,6 +135,7 @@ struct PairTune {
     int epa_blocks = 2048;   // grid cap of the EPA launch (blocks of 4 waves, one contact per wave)
     int epa_fused = 1;       // EPA and the law inside the GJK launch (1) or as their own launch (0)
     int fill_cells = 64;     // cells per block of the pair emission: 64, 16 or 4
+    int gjk_buckets = 0;     // GJK launched per class of vertex count (survivors sorted first): 1, or one launch (0)
 };
 
 // Face distances at or above this are EPA's placeholders (1e30 for a face
@@ -270,6 +271,8 @@ struct mwd_workspace {
     Simplex* simplex;          // [contact_cap], slot claimed by GJK per colliding pair
     int32_t* sidx;             // [pair_cap] simplex slot of a colliding pair
     int32_t* surv;             // [pair_cap] pairs whose bounding spheres meet (any order)
+    int32_t* surv_cls;         // [pair_cap] the same sorted by GJK class (gjk_buckets)
+    int64_t* bucket;           // [4 * MWD_GJK_BUCKETS] class counts, cursors and offsets
     int fresh_pairs;           // host: candidates not yet consumed by a fine-contact stage
     // contacts
     int32_t* contact_pair;     // [contact_cap]
@@ -307,7 +310,8 @@ struct mwd_workspace {
     // histogram entries (scratch), 10 sphere-test survivors, 11 bucket
     // overflow records, 12-13 slots filled in the tangential history tables,
     // 14 moving rows active now (m_live), 15 wall slab tests now (wt_live),
-    // 16 steps taken since the spawn schedule was set
+    // 16 steps taken since the spawn schedule was set, 17 wall pairs listed
+    // for GJK (the rest are separated by the wall's slab)
     int64_t* counters;
     int64_t* m_live;           // counters + 14: every kernel over the moving rows reads its count here
     int64_t* wt_live;          // counters + 15
@@ -341,7 +345,7 @@ hipEvent_t mwd_kev(mwd_workspace* ws, int kt, int stop);
 // Launch with start/stop events when e0 is set, else a plain launch.
 #define MWD_LAUNCH_EV(e0, e1, kernel, grid, block, shmem, stream, ...)                                   \
     do {                                                                                                 \
-        if (e0)                                                                                          \
+        if (e0 || e1)                                                                                    \
             hipExtLaunchKernelGGL(kernel, grid, block, shmem, stream, e0, e1, 0u, __VA_ARGS__);          \
         else                                                                                             \
             hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);                         \
@@ -404,10 +408,20 @@ int mwd_launch_sphere(const BodyView& A, const BodyView& B, int64_t n_cap, const
                       int32_t* surv, int64_t* n_surv, hipStream_t s, hipEvent_t e0 = nullptr,
                       hipEvent_t e1 = nullptr, const PairTune& tu = PairTune());
 // contact_pair (may be NULL): slot -> pair index, written with the slot.
+// lo (may be NULL: 0): the survivors [*lo, *n_surv) only; K_class > 0: the
+// launch for survivors of at most K_class * gjk_group vertices per body.
 int mwd_launch_gjk_surv(const BodyView& A, const BodyView& B, int64_t n_cap, int32_t* collision,
                         const int32_t* surv, const int64_t* n_surv, Simplex* simplex, int32_t* sidx, int64_t* slots,
                         int64_t slot_cap, uint32_t* flags, hipStream_t s, hipEvent_t e0 = nullptr,
-                        hipEvent_t e1 = nullptr, const PairTune& tu = PairTune(), int32_t* contact_pair = nullptr);
+                        hipEvent_t e1 = nullptr, const PairTune& tu = PairTune(), int32_t* contact_pair = nullptr,
+                        const int64_t* lo = nullptr, int K_class = 0);
+// Survivors sorted by GJK class k = ceil(max(nA, nB) / G) - 1 (the last of
+// the nb classes takes the rest) into out, class after class. bucket: int64
+// scratch of 4 * MWD_GJK_BUCKETS (counts, cursors, then the nb + 1 class
+// start offsets at bucket + 2 * MWD_GJK_BUCKETS).
+constexpr int MWD_GJK_BUCKETS = 8;
+int mwd_launch_gjk_buckets(const BodyView& A, const BodyView& B, const int32_t* surv, const int64_t* n_surv,
+                           int64_t n_cap, int G, int nb, int64_t* bucket, int32_t* out, hipStream_t s);
 // Both of the above.
 int mwd_launch_gjk(const BodyView& A, const BodyView& B, int64_t n_cap, const int64_t* n_dev,
                    int32_t* collision, int32_t* surv, int64_t* n_surv, Simplex* simplex,
@@ -438,7 +452,8 @@ int mwd_launch_gjk_epa(const BodyView& A, const BodyView& B, int64_t n_cap, int3
                        int32_t* contact_pair, const mwd_law* laws2, double* depth, double* normal, double* cpA,
                        double* cpB, const ContactOut& co, int32_t* epa_status, int32_t* rec_cnt, int32_t* rec_bkt,
                        int32_t* rec_ovf, int64_t* n_ovf, int64_t ovf_cap, uint32_t* flags, hipStream_t s,
-                       hipEvent_t e0, hipEvent_t e1, const PairTune& tu);
+                       hipEvent_t e0, hipEvent_t e1, const PairTune& tu, const int64_t* lo = nullptr,
+                       int K_class = 0);
 int mwd_launch_epa_force_rows(const BodyView& A, const BodyView& B, const BodyTan* bt, const mwd_law* laws2,
                               const mwd_law_tan* lawt2, const int32_t* contact_pair, const Simplex* simplex,
                               const int32_t* sidx, int64_t contact_cap, const int64_t* n_contacts, double* depth,

@@ -200,7 +200,8 @@ def test_tuning_choices_change_nothing(gpu, dense_phys):
                 dict(vertex_pack=1), dict(vertex_pack=1, verlet_block=64, verlet_waves=4), dict(epa_blocks=16),
                 dict(epa_blocks=128), dict(epa_fused=0), dict(epa_fused=0, gjk_group=16),
                 dict(gjk_group=16, gjk_blocks=64), dict(fill_cells=16), dict(fill_cells=4),
-                dict(fill_cells=4, cell_pass=_lib.CELLS_SORTED)]
+                dict(fill_cells=4, cell_pass=_lib.CELLS_SORTED), dict(gjk_buckets=1),
+                dict(gjk_buckets=1, gjk_group=4), dict(gjk_buckets=1, epa_fused=0)]
     for v in variants:
         S = _stepper(sc)
         S.set_tuning(**v)
@@ -214,9 +215,34 @@ def test_tuning_choices_change_nothing(gpu, dense_phys):
     S = _stepper(sc)
     for bad in (dict(gjk_group=5), dict(gjk_blocks=8), dict(sphere_block=100), dict(cell_pass=7),
                 dict(verlet_block=96), dict(cell_items=3), dict(verlet_waves=2), dict(vertex_pack=2),
-                dict(epa_blocks=8), dict(epa_fused=2), dict(fill_cells=8)):
+                dict(epa_blocks=8), dict(epa_fused=2), dict(fill_cells=8), dict(gjk_buckets=2)):
         with pytest.raises(_lib.MwdError):
             S.set_tuning(**bad)
+
+
+@pytest.mark.parametrize("group,fused", [(4, 1), (4, 0), (8, 1), (8, 0)])
+def test_gjk_classes_change_nothing(gpu, group, fused):
+    """GJK per class of vertex count on hulls of 6 to 24 vertices (classes
+    of 4 or 8 vertices per lane step, fused with EPA or not): the same
+    state as one launch, bit for bit, after contact-bearing steps."""
+    from mwdiem_amd import scene
+    base = scene.make_scene(80, (12.0, 12.0, 30.0), 21, npts=(6, 24), law=scene.Law4(mode=scene.PHYSICAL),
+                            name="mixed-80")
+    sc = scene.compact_dense(base, gap=-0.55)
+    assert len(set(sc.nverts[sc.n_walls:].tolist())) > 8
+    st = random_motion(O.initial_state(sc), sc, 3, vscale=2000.0, wscale=200.0)
+    out = []
+    for buckets in (0, 1):
+        S = _stepper(sc)
+        S.set_tuning(gjk_group=group, epa_fused=fused, gjk_buckets=buckets)
+        S.set_state(st)
+        S.step(15)
+        stt = S.check()
+        assert stt["n_contacts"] > 0
+        out.append(S.numpy_state())
+        S.close()
+    for k in out[0]:
+        assert np.array_equal(out[0][k], out[1][k]), k
 
 
 @pytest.mark.parametrize("pack", [0, 1])
@@ -523,15 +549,16 @@ def test_persistent_tangential_history_matches_oracle(gpu, dense_phys, fused):
 def test_sphere_survivors_in_crowded_cells(gpu):
     """Crowded cells give blocks of the pair emission thousands of rounds, so
     the survivors of the sphere test go through the emission's buffered
-    appends many times: their count equals the wall pairs (all listed) plus
-    the cell pairs whose bounding spheres meet, recounted on the host from
-    the emitted candidates, the radii and the centres. A second pass over the
-    same candidates (the standalone sphere-test kernel, walls tested too)
-    counts the pairs whose spheres meet among all of them."""
+    appends many times: their count equals the wall pairs the wall's slab
+    does not separate from the particle's bounding sphere plus the cell pairs
+    whose bounding spheres meet, recounted on the host from the emitted
+    candidates, the radii, the centres and the walls' vertices. A second pass
+    over the same candidates (the standalone sphere-test kernel, walls tested
+    too) counts the pairs whose spheres meet among all of them."""
     from dataclasses import replace
     from mwdiem_amd import scene
     base = scene.make_scene(3000, (60.0, 60.0, 120.0), 5, name="crowd-3000")
-    sc = scene.compact_dense(base, gap=-0.3)
+    sc = scene.compact_dense(base, gap=-0.55)
     cell, ncx, ncy, ncz, kmin, kspan = scene.grid_for(sc.box, 2.0 * sc.max_particlesize)
     sc = replace(sc.copy(), cell=cell, ncx=ncx, ncy=ncy, ncz=ncz, kmin=kmin, kspan=kspan)
     st = random_motion(O.initial_state(sc), sc, 3, vscale=500.0, wscale=100.0)
@@ -557,7 +584,17 @@ def test_sphere_survivors_in_crowded_cells(gpu):
 
     cell_meet = int(meet(pairs[W:]).sum())
     assert cell_meet > 1000
-    assert stat["n_sphere_pass"] == W + cell_meet
+    # wall pairs: the particle's sphere and the wall's slab |n.(x - c)| <= h
+    # apart by more than 1e-6 (1 + |dd| + h) are decided in the wall scan
+    verts = S.numpy_state()["verts"]
+    wall_keep = 0
+    for j, k in pairs[:W]:
+        n, c = sc.wall_n[k], sc.wall_c[k]
+        dd = abs(float(n @ (com[j] - c)))
+        h = float(np.abs((verts[k, :sc.nverts[k]] - c) @ n).max())
+        wall_keep += not (dd - h - rad[j] > 1e-6 * (1.0 + dd + h))
+    assert 0 < wall_keep < W
+    assert stat["n_sphere_pass"] == wall_keep + cell_meet
     S.fine_contact()  # a second pass: k_sphere over the stored candidates
     assert S.check()["n_sphere_pass"] == int(meet(pairs).sum())
 
