@@ -310,13 +310,18 @@ def contacting_scene(sc, gap: float = -0.6):
     return out
 
 
-def bench_batched(scs, steps: int, plain_steps: int, graph_steps: int, tune: bool = True):
+def bench_batched(scs, steps: int, plain_steps: int, graph_steps: int, tune: bool = True, v_rms: float = 5000.0):
     """The batched form: n_scenes hopper-discharge scenes (particle counts
     log-uniform 500-20,000, per-scene friction, stiffness and restitution,
     physical law) in one Stepper -- one launch per stage over all scenes,
     graph-captured, cell size and launch choices picked by the autotuner --
     against the same scenes (same cells) run one after another, each in its
-    own graph-captured Stepper with the default launch choices."""
+    own graph-captured Stepper with the default launch choices. Every scene
+    starts with its hulls thrown about (v_rms mm/s per component, seeded per
+    scene), so contacts form from the first steps in every scene and GJK/EPA
+    work is uneven across them; kernel durations of the batched step come
+    from the library's kernel timers over 20 eager steps after the timed run."""
+    import numpy as np
     import torch
 
     from mwdiem_amd.stepper import Stepper
@@ -325,12 +330,19 @@ def bench_batched(scs, steps: int, plain_steps: int, graph_steps: int, tune: boo
     n_scenes = len(scs)
     n_tot = sum(sc.n_particles for sc in scs)
     print(f"# batched: {n_scenes} scenes, {n_tot} particles", file=sys.stderr, flush=True)
+    vel = []
+    for i, sc in enumerate(scs):
+        v = np.zeros((sc.n_rows, 3))
+        v[sc.n_walls:] = np.random.Generator(np.random.PCG64(1000 + i)).normal(size=(sc.n_particles, 3)) * v_rms
+        vel.append(v)
+    v_all = np.concatenate(vel)
     tuned = None
     if tune:
         scale, scale_ms = tune_cell_scale(scs, steps=5)
         scs = [regrid(sc, scale) for sc in scs]
         print(f"# batched cell scale {scale} ({scale_ms})", file=sys.stderr, flush=True)
     B = Stepper(scs)
+    B.set_state({"v": v_all, "v12": v_all})
     B.step(5)
     B.check()
     if tune:
@@ -346,13 +358,23 @@ def bench_batched(scs, steps: int, plain_steps: int, graph_steps: int, tune: boo
     tb = time.perf_counter() - t0
     sb = B.check()
     batched = n_tot * steps / tb
+    B.set_timing(True)
+    B.step(20)
+    kt = B.kernel_times()
+    B.stage_times()
+    B.set_timing(False)
+    kernel_ms = {k: v / max(kt["steps"], 1) for k, v in kt["ms"].items()}
+    sb_end = B.check()
+    print(f"# batched kernels (ms/step, library timers, after {steps} steps, {sb_end['n_contacts']} contacts): "
+          f"{kernel_ms}", file=sys.stderr, flush=True)
     B.close()
     del B
     torch.cuda.empty_cache()
-    # plain: every scene alone, one after another
+    # plain: every scene alone, one after another, from the same throw
     t_plain = 0.0
-    for sc in scs:
+    for sc, v in zip(scs, vel):
         S = Stepper(sc)
+        S.set_state({"v": v, "v12": v})
         S.step(2)
         S.capture(graph_steps)
         torch.cuda.synchronize()
@@ -366,34 +388,76 @@ def bench_batched(scs, steps: int, plain_steps: int, graph_steps: int, tune: boo
     print(f"# batched {batched:.4g} vs plain {plain:.4g} particle-steps/s", file=sys.stderr, flush=True)
     return {"scenes": n_scenes, "particles": n_tot, "steps": steps, "ms_per_step": 1e3 * tb / steps,
             "particle_steps_per_s": batched, "pairs_per_step": sb["n_pairs"], "contacts_per_step": sb["n_contacts"],
+            "v_rms_mm_s": v_rms, "kernel_ms_at_end": kernel_ms,
             "plain_steps_per_scene": plain_steps, "plain_particle_steps_per_s": plain, "batched_vs_plain": batched / plain,
             "autotune": tuned}
 
 
-def bench_diff(pile, steps: int, graph_steps: int, short: int = 200):
-    """The differentiable form on the poured pile (n hulls poured at 1 m/s rms
-    into a 40x40x60 mm box, physical law Kn=1e4, en=0.5, mu=0.3): cost of a
-    tangent step (state plus its derivatives with respect to Kn, en, mu)
-    against a plain step over `steps` steps, both graph-captured, and the bed
-    outcome's gradient against central differences. Two checks: the same
-    hulls poured at 3 m/s over `short` steps (the first impacts: the outcome
-    is smooth in the parameters) and the pour itself over `steps` steps,
-    where contacts open and close thousands of times, the outcome is only
-    piecewise smooth and differences across such events are not derivatives
-    (two step sizes are reported)."""
+def fd_richardson(outcome, theta, k, h_rel=1e-4):
+    """Central differences of outcome(theta) in parameter k at h, h/2 and h/4
+    (h = h_rel * theta_k) and their Richardson extrapolations (the central
+    difference's error is c h^2 + O(h^4)): returns (D(h), D(h/2), D(h/4),
+    R(h, h/2), R(h/2, h/4))."""
+    import numpy as np
+    D = []
+    for f in (1.0, 0.5, 0.25):
+        h = h_rel * f * theta[k]
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] += h
+        tm[k] -= h
+        D.append((outcome(tp) - outcome(tm)) / (2 * h))
+    return D + [(4 * D[1] - D[0]) / 3, (4 * D[2] - D[1]) / 3]
+
+
+def bench_diff(pour, steps: int, graph_steps: int, horizons=(10, 100, 1000, 2000)):
+    """The differentiable form on the poured pile: the pour configuration's
+    4,096 hulls enter a 40x40x60 mm box on their spawn schedule and settle
+    (physical law Kn=1e4, en=0.5, mu=0.3); from that pile, the cost of a
+    tangent step (state plus its derivatives with respect to Kn, en, mu) and
+    of the adjoint reverse sweep (checkpoints every 100 steps) against a plain
+    step over `steps` steps, graph-captured; and the bed outcome's gradient
+    (tangents) against central differences at h = 1e-4, 5e-5 and 2.5e-5
+    relative with Richardson extrapolation, at several horizons. The longest
+    horizon at which the differences converge (to 1e-6) and agree with the
+    gradient (to 1e-5) is reported; on the settled pile there is none (its
+    outcome moves by jumps as contacts among ~8,000 open and close), and the
+    gradient's check is the PyTorch-CPU reference in the GPU tests."""
     import numpy as np
     import torch
 
-    from mwdiem_amd.diff import _with_theta, run_outcome
+    from mwdiem_amd.diff import _with_theta, form_pile, run_outcome
     from mwdiem_amd.stepper import Stepper
 
-    (sc, v0), (_, v_fast) = pile
+    from mwdiem_amd.tune import autotune
+
+    sc, v_pour, spawn = pour
     n = sc.n_particles
+    W = sc.n_walls
+    n_pour = spawn.every * ((n + spawn.per - 1) // spawn.per) + 40_000
+    t0 = time.perf_counter()
+    # launch choices for a packed bed of few cells (results do not depend on them)
+    st0 = form_pile(sc, v_pour, steps=n_pour, spawn=spawn, steps_per_graph=graph_steps,
+                    tuning=dict(fill_cells=4, gjk_group=4))
+    t_pour = time.perf_counter() - t0
+    speed = np.linalg.norm(st0["v"][W:], axis=1)
+    pile = {"pour_steps": n_pour, "pour_s": t_pour, "batches": (n + spawn.per - 1) // spawn.per,
+            "per_batch": spawn.per, "batch_every_steps": spawn.every,
+            "median_speed_mm_s": float(np.median(speed)), "max_speed_mm_s": float(speed.max()),
+            "kinetic_energy": float(0.5 * np.sum(sc.mass[W:] * speed ** 2)),
+            "top10_mean_z_mm": float(np.sort(st0["com"][W:, 2])[-n // 10:].mean()),
+            "finite": bool(all(np.isfinite(a).all() for a in st0.values()))}
+    print(f"# diff: pile poured in {n_pour} steps ({t_pour:.1f} s): {pile}", file=sys.stderr, flush=True)
     theta = np.array([sc.law_p.Kn, sc.law_p.en, sc.law_p.mu])
+    S = Stepper(_with_theta(sc, theta))
+    S.set_state(st0)
+    tune = autotune(S, steps=10)["tuning"]
+    S.close()
+    print(f"# diff: pile autotune {tune}", file=sys.stderr, flush=True)
     times = {}
     for mode in ("plain", "tangent"):
         S = Stepper(_with_theta(sc, theta))
-        S.set_state({"v": v0, "v12": v0})
+        S.set_tuning(**tune)
+        S.set_state(st0)
         if mode == "tangent":
             S.enable_tangents()
         S.capture(graph_steps, tangent=(mode == "tangent"))
@@ -404,32 +468,40 @@ def bench_diff(pile, steps: int, graph_steps: int, short: int = 200):
         times[mode] = (time.perf_counter() - t0) / steps
         st = S.check()
         S.close()
-    checks = {}
-    for name, init, horizon, hs in ((f"pour_3m_s_{short}_steps", v_fast, short, (1e-4,)),
-                                    (f"pour_1m_s_{steps}_steps", v0, steps, (1e-4, 1e-7))):
-        L, g = run_outcome(sc, init, theta, horizon, True, graph_steps)
-        out = {"outcome": L, "grad": [float(x) for x in g]}
-        for hrel in hs:
-            fd = np.zeros(3)
-            for k in range(3):
-                h = hrel * theta[k]
-                tp, tm = theta.copy(), theta.copy()
-                tp[k] += h
-                tm[k] -= h
-                fd[k] = (run_outcome(sc, init, tp, horizon, False, graph_steps)[0]
-                         - run_outcome(sc, init, tm, horizon, False, graph_steps)[0]) / (2 * h)
-            rel = np.abs(g - fd) / np.maximum(np.abs(fd), 1e-300)
-            out[f"fd_h{hrel:g}"] = {"fd": [float(x) for x in fd], "rel": [float(x) for x in rel]}
-        checks[name] = out
-    print(f"# diff: tangent {1e3 * times['tangent']:.4f} ms/step vs plain {1e3 * times['plain']:.4f}; {checks}",
-          file=sys.stderr, flush=True)
-    # reverse mode: the same pour's gradient by the adjoint, checkpoints every
-    # 100 steps (forward, then per segment a recomputation and the reverse sweep)
+    print(f"# diff: tangent {1e3 * times['tangent']:.4f} ms/step vs plain {1e3 * times['plain']:.4f}; "
+          f"{st['n_contacts']} contacts", file=sys.stderr, flush=True)
+    # reverse mode: the gradient by the adjoint, checkpoints every 100 steps
+    # (forward, then per segment a recomputation and the reverse sweep)
     from mwdiem_amd.adjoint import adjoint_run
     every = 100
     tm = {}
-    La, ga = adjoint_run(sc, v0, theta, steps, every=every, steps_per_graph=graph_steps, timings=tm)
-    gt = np.array(checks[f"pour_1m_s_{steps}_steps"]["grad"])
+    La, ga = adjoint_run(sc, st0, theta, steps, every=every, steps_per_graph=graph_steps, timings=tm, tuning=tune)
+    checks = {}
+    longest = None
+    for H in horizons:
+        L, g = run_outcome(sc, st0, theta, H, True, graph_steps, tuning=tune)
+        out = {"outcome": L, "grad": [float(x) for x in g]}
+
+        def outcome(th, H=H):
+            return run_outcome(sc, st0, th, H, False, graph_steps, tuning=tune)[0]
+        conv, rel, fd = [], [], []
+        for k in range(3):
+            D0, D1, D2, R1, R2 = fd_richardson(outcome, theta, k)
+            fd.append([D0, D1, D2, R2])
+            conv.append(abs(R2 - R1) / max(abs(R2), 1e-300))
+            rel.append(abs(g[k] - R2) / max(abs(R2), 1e-300))
+        out.update({"fd_h_h2_h4_richardson": fd, "fd_convergence": conv, "rel_to_richardson": rel})
+        ok = max(conv) <= 1e-6 and max(rel) <= 1e-5
+        out["agrees"] = bool(ok)
+        if ok:
+            longest = H
+        checks[str(H)] = out
+        print(f"# diff: horizon {H}: conv {['%.1e' % c for c in conv]} rel {['%.1e' % r for r in rel]}",
+              file=sys.stderr, flush=True)
+    if steps == horizons[-1]:
+        gt = np.array(checks[str(steps)]["grad"])
+    else:
+        gt = run_outcome(sc, st0, theta, steps, True, graph_steps, tuning=tune)[1]
     adj = {"checkpoint_every": every, "outcome": La, "grad": [float(x) for x in ga],
            "rel_to_tangents": [float(x) for x in np.abs(ga - gt) / np.maximum(np.abs(gt), 1e-300)],
            "forward_ms_per_step": 1e3 * tm["forward_s"] / steps,
@@ -437,10 +509,11 @@ def bench_diff(pile, steps: int, graph_steps: int, short: int = 200):
            "adjoint_vs_forward": tm["reverse_s"] / steps / times["plain"],
            "gradient_vs_forward": (tm["forward_s"] + tm["reverse_s"]) / steps / times["plain"]}
     print(f"# diff: adjoint {adj}", file=sys.stderr, flush=True)
-    return {"particles": n, "steps": steps, "ms_per_step_plain": 1e3 * times["plain"],
+    return {"particles": n, "steps": steps, "pile": pile, "ms_per_step_plain": 1e3 * times["plain"],
             "ms_per_step_tangent": 1e3 * times["tangent"], "tangent_vs_forward": times["tangent"] / times["plain"],
-            "adjoint": adj, "adjoint_vs_forward": adj["adjoint_vs_forward"],
-            "gradient_checks": checks, "contacts_last_step": st["n_contacts"], "theta": ["Kn", "en", "mu"],
+            "adjoint": adj, "adjoint_vs_forward": adj["adjoint_vs_forward"], "gradient_checks": checks,
+            "longest_horizon_fd_agrees": longest, "contacts_last_step": st["n_contacts"],
+            "theta": ["Kn", "en", "mu"],
             "mode": "forward tangents (3 directions); reverse adjoint with step checkpoints"}
 
 
@@ -562,7 +635,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="launch every step eagerly instead of graph replays")
     ap.add_argument("--graph-steps", type=int, default=20, help="steps captured per HIP graph")
     ap.add_argument("--batched-scenes", type=int, default=256)
-    ap.add_argument("--batched-steps", type=int, default=1000)
+    ap.add_argument("--batched-steps", type=int, default=5000)
     ap.add_argument("--plain-steps", type=int, default=100)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=200, help="eager steps timed per kernel with events")
@@ -599,7 +672,7 @@ def main():
           f"generated in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
     sc_dense = None if args.no_contacting else contacting_scene(sc)
     scs_batched = None if args.no_batched else scene.config_batched(n_scenes=args.batched_scenes)
-    pile = None if args.no_diff else (scene.config_pile(n=4096), scene.config_pile(n=4096, v_pour=3000.0))
+    pour = None if args.no_diff else scene.config_pour(n=4096)
     sc_poly = None if args.no_poly else scene.config_polydisperse()
     sc_small = None if args.no_cpu_baseline else scene.config_small()
     print(f"# all scenes generated in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
@@ -645,8 +718,8 @@ def main():
                                 tune=not args.no_tune)
 
     diff = None
-    if pile is not None:
-        diff = bench_diff(pile, args.diff_steps, args.graph_steps)
+    if pour is not None:
+        diff = bench_diff(pour, args.diff_steps, args.graph_steps)
 
     poly = None
     if sc_poly is not None:

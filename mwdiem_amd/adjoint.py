@@ -33,14 +33,17 @@ def _set_init(S: Stepper, init):
 
 
 def adjoint_run(sc, init, theta, steps: int, every: int = 100, top_frac: float = 0.1, seed=None,
-                steps_per_graph: int = 0, timings: dict | None = None):
+                steps_per_graph: int = 0, timings: dict | None = None, tuning: dict | None = None):
     """Run `steps` steps of the scene with law parameters theta = (Kn, en,
     mu) from `init` (None, initial velocities or a state dict) and return
     (outcome, d outcome/d theta) by the adjoint. With steps_per_graph the
     forward and recomputed segments replay captured graphs. timings, if
-    given, receives the forward and reverse wall times (s)."""
+    given, receives the forward and reverse wall times (s); tuning: launch
+    choices."""
     import time
     S = Stepper(_with_theta(sc, theta))
+    if tuning:
+        S.set_tuning(**tuning)
     _set_init(S, init)
     S.set_law_seeds(seed)
     W = sc.n_walls

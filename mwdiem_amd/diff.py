@@ -55,16 +55,19 @@ def _with_theta(sc: SceneData, theta) -> SceneData:
 
 
 def form_pile(sc: SceneData, v0: np.ndarray | None = None, steps: int = 20_000, dt: float | None = None,
-              steps_per_graph: int = 50, spawn=None) -> dict:
+              steps_per_graph: int = 50, spawn=None, tuning: dict | None = None) -> dict:
     """Let the scene's particles fall and settle into a pile (its own law;
     dt, if given, replaces the scene's time step) and return the state (numpy
     arrays of the state keys) to start runs from. With spawn (scene.Spawn,
     e.g. from scene.config_pour) the particles enter on that schedule: the
-    pile is poured."""
+    pile is poured. tuning: launch choices (Stepper.set_tuning; results do
+    not depend on them)."""
     f = sc.copy()
     if dt is not None:
         f.dt = dt
     S = Stepper(f, spawn=spawn)
+    if tuning:
+        S.set_tuning(**tuning)
     if v0 is not None:
         S.set_state({"v": v0, "v12": v0})
     S.capture(steps_per_graph)
@@ -76,12 +79,14 @@ def form_pile(sc: SceneData, v0: np.ndarray | None = None, steps: int = 20_000, 
 
 
 def run_outcome(sc: SceneData, init, theta, steps: int, tangents: bool = True,
-                steps_per_graph: int = 0, top_frac: float = 0.1):
+                steps_per_graph: int = 0, top_frac: float = 0.1, tuning: dict | None = None):
     """Run `steps` steps of the scene with law parameters theta = (Kn, en, mu)
     (both laws) from `init` -- None (the scene's initial state), an array of
     initial velocities, or a state dict; return (outcome, d outcome/d theta)
-    (gradient None without tangents)."""
+    (gradient None without tangents). tuning: launch choices."""
     S = Stepper(_with_theta(sc, theta))
+    if tuning:
+        S.set_tuning(**tuning)
     if isinstance(init, dict):
         S.set_state(init)
     elif init is not None:

@@ -1019,11 +1019,15 @@ ORC_EXPORT int64_t orc_contact_state(int64_t P, const int32_t* pa, const int32_t
 
 // Contact topology of a pair list: collision codes and, for colliding pairs,
 // the support indices of EPA's final face (face[6p..6p+2] on A, +3..+5 on B).
+// mode_p / mode_w: the law modes of particle pairs and of wall pairs (rows
+// below n_walls), which select the GJK/EPA variant as in contact_state.
 ORC_EXPORT void orc_contact_faces(int64_t P, const int32_t* pa, const int32_t* pb, int vmax, const double* com,
-                                  const double* verts, const int32_t* nverts, int32_t* collision, int32_t* face) {
-    Law L = make_law(1.0, 0.5, 0.5, 0.5, 1.0);
+                                  const double* verts, const int32_t* nverts, int32_t* collision, int32_t* face,
+                                  int mode_p, int mode_w, int64_t n_walls) {
+    const Law Lp = make_law(1.0, 0.5, 0.5, 0.5, 1.0, mode_p), Lw = make_law(1.0, 0.5, 0.5, 0.5, 1.0, mode_w);
     for (int64_t p = 0; p < P; ++p) {
         int a = pa[p], b = pb[p];
+        const Law& L = (a < n_walls || b < n_walls) ? Lw : Lp;
         Body<double> A{verts + (int64_t)vmax * 3 * a, nverts[a]};
         Body<double> B{verts + (int64_t)vmax * 3 * b, nverts[b]};
         PairIn<double> in;

@@ -199,14 +199,17 @@ def contact_faces(pairs, st, sc):
     """Collision codes of a pair list and, for colliding pairs, EPA's final
     face: the support indices of its three Minkowski vertices on A
     (face[:, :3]) and on B (face[:, 3:]), in the face's stored order (-1 for
-    the others)."""
+    the others); each pair takes the GJK/EPA variant of its law's mode."""
     pairs = _i32(pairs).reshape(-1, 2)
     P = pairs.shape[0]
     pa, pb = np.ascontiguousarray(pairs[:, 0]), np.ascontiguousarray(pairs[:, 1])
     coll = np.zeros(P, np.int32)
     face = np.zeros((P, 6), np.int32)
     lib().orc_contact_faces(C.c_int64(P), _p(pa), _p(pb), C.c_int(sc.vmax), _p(_f64(st["com"])),
-                            _p(_f64(st["verts"])), _p(_i32(sc.nverts)), _p(coll), _p(face))
+                            _p(_f64(st["verts"])), _p(_i32(sc.nverts)), _p(coll), _p(face),
+                            C.c_int(int(getattr(getattr(sc, "law_p", None), "mode", 0))),
+                            C.c_int(int(getattr(getattr(sc, "law_w", None), "mode", 0))),
+                            C.c_int64(int(getattr(sc, "n_walls", 0))))
     return coll, face
 
 
