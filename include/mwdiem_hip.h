@@ -351,6 +351,20 @@ int mwd_kernel_times(mwd_workspace* ws, double* ms, int64_t* steps);
 int mwd_radius_init(const double* com, const double* verts, const int32_t* nverts, int64_t n, int32_t vmax,
                     double* radius, void* stream);
 
+/* The scene generator's hull bodies (replaces the hull construction of
+ * Particle_class.Spherical_convex_polygon_maker_3D / Random_convex_polygon_
+ * maker_3D, Particle_class.py:184-303, and Calc_inertia, :125-156): for each
+ * of n_hulls point sets (points [n_hulls, pmax, 3], npts points each,
+ * 4 <= npts <= pmax <= 64), its convex hull's vertices in input order centred
+ * on their mean (verts_out [n_hulls, pmax, 3], zero-padded; nverts_out), the
+ * mean (com_out [n_hulls, 3]), the volume, the inertia tensor about the mean
+ * at `density` (I_out [n_hulls, 3, 3], volume element D.(E x F)/6) and its
+ * size trace/3. status_out: 0, 1 (four points on a supporting plane: not
+ * triangulated, nverts 0) or 2 (npts out of range). Stream-ordered. */
+int mwd_hull_bodies(const double* points, const int32_t* npts, int64_t n_hulls, int32_t pmax, double density,
+                    double* verts_out, int32_t* nverts_out, double* com_out, double* vol_out, double* I_out,
+                    double* isize_out, int32_t* status_out, void* stream);
+
 /* dst[i] <- src[i] for i < n (n <= MWD_MAX_COPY arrays, bytes[i] a multiple
  * of 8, 8-byte aligned device pointers) in one kernel launch on `stream`.
  * No counterpart in the reference: the reverse sweep uses it to save and

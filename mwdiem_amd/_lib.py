@@ -136,6 +136,7 @@ _SIGS = {
     "mwd_clear_flags": (C.c_int, [_P]),
     "mwd_set_timing": (C.c_int, [_P, C.c_int]),
     "mwd_radius_init": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int32, _P, _P]),
+    "mwd_hull_bodies": (C.c_int, [_P, _P, C.c_int64, C.c_int32, C.c_double, _P, _P, _P, _P, _P, _P, _P, _P]),
     "mwd_copy_arrays": (C.c_int, [C.c_int32, C.POINTER(_P), C.POINTER(_P), C.POINTER(C.c_int64), _P]),
     "mwd_stage_times": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "mwd_kernel_times": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),

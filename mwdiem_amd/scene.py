@@ -224,6 +224,89 @@ def make_particle(rng, npts: int, axes, size: float, density: float, ref_inertia
 _CHUNK = 2048
 
 
+def draw_particles(n: int, seed: int, npts, axes, size):
+    """The generator's random draws alone, in make_particles' order (chunk c
+    from PCG64([seed, c]); per particle its point count, size factor, points
+    and rotation): a list of (points, rotation). make_particles and
+    make_particles_device build hulls from exactly these draws."""
+    out = []
+    for c in range((n + _CHUNK - 1) // _CHUNK):
+        rng = np.random.Generator(np.random.PCG64([seed, c]))
+        for _ in range(min(_CHUNK, n - c * _CHUNK)):
+            k = int(rng.integers(npts[0], npts[1] + 1))
+            s = float(rng.uniform(size[0], size[1]))
+            pts = ellipsoid_points(rng, k, np.asarray(axes) * s)
+            out.append((pts, random_rotation(rng)))
+    return out
+
+
+def hull_bodies_device(points: list, density: float, device: int = 0):
+    """Hull bodies of point sets on the GPU (mwd_hull_bodies): per set its
+    hull vertices in input order centred on their mean, the volume and the
+    inertia tensor about the mean (volume element D.(E x F)/6), and its size.
+    Raises for a set the kernel does not triangulate (four points on a
+    supporting plane, or a point count outside 4..64)."""
+    import ctypes as C
+
+    import torch
+
+    from . import _lib
+    H = len(points)
+    pmax = max(4, max(int(p.shape[0]) for p in points))
+    P = np.zeros((H, pmax, 3))
+    n = np.zeros(H, np.int32)
+    for i, p in enumerate(points):
+        P[i, :p.shape[0]] = p
+        n[i] = p.shape[0]
+    dev = torch.device("cuda", device)
+    tp = torch.from_numpy(P).to(dev)
+    tn = torch.from_numpy(n).to(dev)
+    out = {"verts": torch.zeros((H, pmax, 3), dtype=torch.float64, device=dev),
+           "nverts": torch.zeros(H, dtype=torch.int32, device=dev),
+           "com": torch.zeros((H, 3), dtype=torch.float64, device=dev),
+           "vol": torch.zeros(H, dtype=torch.float64, device=dev),
+           "I": torch.zeros((H, 3, 3), dtype=torch.float64, device=dev),
+           "isize": torch.zeros(H, dtype=torch.float64, device=dev),
+           "status": torch.zeros(H, dtype=torch.int32, device=dev)}
+    with torch.cuda.device(dev):
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        _lib.check(_lib.load().mwd_hull_bodies(tp.data_ptr(), tn.data_ptr(), H, pmax, C.c_double(density),
+                                               out["verts"].data_ptr(), out["nverts"].data_ptr(),
+                                               out["com"].data_ptr(), out["vol"].data_ptr(), out["I"].data_ptr(),
+                                               out["isize"].data_ptr(), out["status"].data_ptr(), stream or None),
+                   "mwd_hull_bodies")
+        torch.cuda.synchronize(dev)
+    res = {k: v.cpu().numpy() for k, v in out.items()}
+    bad = np.flatnonzero(res["status"] != 0)
+    if bad.size:
+        raise ValueError(f"mwd_hull_bodies: {bad.size} point sets not triangulated (first {int(bad[0])}, "
+                         f"status {int(res['status'][bad[0]])})")
+    return res
+
+
+def make_particles_device(n: int, seed: int, npts, axes, size, density, consistent=False, device: int = 0):
+    """make_particles with the hulls, volumes and inertia tensors built on the
+    GPU in one launch (the draws are make_particles' own). The vertices come
+    out in the same order; centres, volumes and tensors agree with the host's
+    to rounding (the host's centre is numpy's mean, the volume Qhull's). No
+    face lists (the output writer then triangulates each hull), and the
+    reference's volume element is not offered."""
+    draws = draw_particles(n, seed, npts, axes, size)
+    hb = hull_bodies_device([d[0] for d in draws], density, device)
+    out = []
+    for i, (_, R) in enumerate(draws):
+        V = hb["verts"][i, :hb["nverts"][i]]
+        I = hb["I"][i]
+        Vw = V @ R.T
+        if consistent:
+            Iw = R @ I @ R.T
+            Ibf, Iinv = Iw, np.linalg.inv(Iw)
+        else:
+            Ibf, Iinv = body_frame(I)
+        out.append((Vw, density * float(hb["vol"][i]), Ibf, Iinv, float(hb["isize"][i]), None, None))
+    return out
+
+
 def _particle_chunk(args):
     seed, chunk, n, npts, axes, size, density, ref_inertia, consistent = args
     rng = np.random.Generator(np.random.PCG64([seed, chunk]))
@@ -424,17 +507,24 @@ def make_scene(n_particles: int, box_mm, seed: int, npts=(12, 12), axes=(1.0, 0.
                density: float = RHO_KG_MM3, law: Law4 | None = None, wall_law: Law4 | None = None,
                dt: float = 1e-6, g=(0.0, 0.0, -G_MM), wall_thickness_factor: float = 10.0,
                spacing_factor: float = 1.05, ref_inertia: bool = False, name: str = "",
-               site_fn=None) -> SceneData:
+               site_fn=None, device_hulls: bool = False) -> SceneData:
     """A bed of `n_particles` hulls of npts[0]..npts[1] points on an ellipsoid
     with semi-axes axes*U(size) in a walled box (the configurations' generator).
     The cell size (max_particlesize) is 2x the largest circumradius.
     site_fn(rmax, rng) -> [n, 3], if given, places the particles instead of
-    the jittered lattice."""
+    the jittered lattice. device_hulls: hull bodies built on the GPU
+    (make_particles_device; the scene then has no face lists)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     law = law or Law4()
     wall_law = wall_law or law
-    parts = make_particles(n_particles, seed, npts, axes, size, density, ref_inertia,
-                           consistent=(law.mode == PHYSICAL))
+    if device_hulls:
+        if ref_inertia:
+            raise ValueError("device_hulls: the reference's volume element is built on the host only")
+        parts = make_particles_device(n_particles, seed, npts, axes, size, density,
+                                      consistent=(law.mode == PHYSICAL))
+    else:
+        parts = make_particles(n_particles, seed, npts, axes, size, density, ref_inertia,
+                               consistent=(law.mode == PHYSICAL))
     rmax = max(float(np.max(np.linalg.norm(p[0], axis=1))) for p in parts) if parts else 1.0
     ms = 2.0 * rmax
     spacing = spacing_factor * ms
@@ -447,6 +537,9 @@ def make_scene(n_particles: int, box_mm, seed: int, npts=(12, 12), axes=(1.0, 0.
             sites = sites + rng.uniform(-jit, jit, size=sites.shape)
         except ValueError:
             # too tight for bounding spheres: deposit with an exact separation test
+            if any(p[5] is None for p in parts):
+                raise ValueError("the lattice is too tight and device-built hulls carry no face normals "
+                                 "for the deposit; build the hulls on the host")
             sites = deposit_sites([p[0] for p in parts], [p[5] for p in parts], box_mm, rng)
     T = wall_thickness_factor * ms
     walls = walls_for_box(box_mm, T, density, ref_inertia)
@@ -487,7 +580,7 @@ def make_scene(n_particles: int, box_mm, seed: int, npts=(12, 12), axes=(1.0, 0.
                      isize=isize, fext=fext, text=np.zeros((N, 3)), wall_n=wall_n, wall_c=wall_c, box=box,
                      cell=cell, ncx=ncx, ncy=ncy, ncz=ncz, kmin=kmin, kspan=kspan, max_particlesize=ms,
                      law_p=law, law_w=wall_law, dt=dt, name=name, inner=tuple(float(x) for x in box_mm),
-                     faces=faces)
+                     faces=None if device_hulls else faces)
 
 
 # ------------------------------------------------------- the configurations
