@@ -128,8 +128,8 @@ def cpu_baseline(sc, budget_s: float = 12.0, gjk_samples: dict | None = None):
     for name, smp in (gjk_samples or {}).items():
         t1 = time.perf_counter()
         flops[name] = O.count_flops(smp["pairs"], smp["state"], smp["scene"])
-        print(f"# flops of {name}: {flops[name]} over {len(smp['pairs'])} pairs ({time.perf_counter() - t1:.1f} s)",
-              file=sys.stderr, flush=True)
+        print(f"# flops of {name}: {flops[name]} over {len(smp['pairs'])} pairs (the GPU's survivors: "
+              f"{smp.get('survivors_on_gpu')}; {time.perf_counter() - t1:.1f} s)", file=sys.stderr, flush=True)
     return out, flops
 
 
@@ -157,8 +157,10 @@ def cpu_baseline_python(sc, budget_s: float = 10.0):
 
 def gjk_sample(S, sc):
     """The pair list the next step's GJK launch takes (candidate pairs of the
-    current state whose bounding spheres meet, wall pairs all), with the
-    state, for counting that launch's algorithmic flops."""
+    current state whose bounding spheres meet; wall pairs whose particle's
+    bounding sphere and wall slab are not apart by more than 1e-6 (1 + |d| + h),
+    the wall scan's test), with the state, for counting that launch's
+    algorithmic flops. Its length equals the step's sphere-test survivor count."""
     import numpy as np
     S.candidates()
     stt = S.check()
@@ -169,15 +171,26 @@ def gjk_sample(S, sc):
     cell = pairs[nw:]
     d = st["com"][cell[:, 1]] - st["com"][cell[:, 0]]
     meet = np.einsum("ij,ij->i", d, d) <= (r[cell[:, 0]] + r[cell[:, 1]]) ** 2
-    surv = np.concatenate([pairs[:nw], cell[meet]]).astype(np.int32)
-    return {"pairs": surv, "state": st, "scene": sc, "candidates": int(pairs.shape[0])}
+    wall = pairs[:nw]
+    j, k = wall[:, 0], wall[:, 1]
+    wc = S.t["wall_c"].cpu().numpy() if "wall_c" in S.t else sc.wall_c
+    n = sc.wall_n[k]
+    dd = np.abs(np.einsum("ij,ij->i", n, st["com"][j] - wc[k]))
+    h = np.zeros(nw)
+    for w in np.unique(k):
+        V = st["verts"][w, :sc.nverts[w]] - wc[w]
+        h[k == w] = np.abs(V @ sc.wall_n[w]).max()
+    keep = ~(dd - h - r[j] > 1e-6 * (1.0 + dd + h))
+    surv = np.concatenate([wall[keep], cell[meet]]).astype(np.int32)
+    return {"pairs": surv, "state": st, "scene": sc, "candidates": int(pairs.shape[0]),
+            "survivors_on_gpu": int(stt["n_sphere_pass"])}
 
 
 def roofline_fp64(flops, launch_ms, kernel, workload, timed_over):
     """FP64 roofline of the GJK launch: algorithmic flops per launch over its
     mean duration against MI355X's FP64 vector peak; the PMC figures of the
     same launch (FP64 VALU instructions per launch) when a summary exists."""
-    if flops is None or not launch_ms:
+    if not flops or not launch_ms:
         return None
     ach = flops / (launch_ms * 1e-3) / 1e12
     return {"bound": "fp64_valu", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
