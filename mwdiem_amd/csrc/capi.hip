@@ -53,7 +53,7 @@ void free_ws(mwd_workspace* ws) {
                     ws->contact_pair,
                     ws->depth, ws->normal, ws->cpA, ws->cpB, ws->Fn, ws->tA, ws->tB, ws->epa_status, ws->rec_cnt,
                     ws->rec_bkt, ws->rec_ovf, ws->hist,
-                    ws->hist_off, ws->scan_tmp, ws->scan_tmp_side, ws->counters, ws->flags, ws->laws,
+                    ws->hist_off, ws->scan_tmp, ws->scan_tmp_w, ws->wseg_off, ws->wseg_scene, ws->word_seg, ws->hit_words, ws->gjk_words, ws->counters, ws->flags, ws->laws,
                     ws->scene_wt_off, ws->ctan, ws->law_tans, ws->adj_F, ws->adj_T, ws->adj_lamF,
                     ws->adj_lamT, ws->adj_face, ws->adj_c, ws->adj_cth, ws->adj_rowth};
     for (void* p : ptrs)
@@ -65,9 +65,6 @@ void free_ws(mwd_workspace* ws) {
         if (ws->thist.used[t]) (void)hipFree(ws->thist.used[t]);
     }
     if (ws->thist.idx) (void)hipFree(ws->thist.idx);
-    if (ws->ev_fork) (void)hipEventDestroy(ws->ev_fork);
-    if (ws->ev_join) (void)hipEventDestroy(ws->ev_join);
-    if (ws->side) (void)hipStreamDestroy(ws->side);
     delete[] ws->scenes_host;
     delete ws;
 }
@@ -356,6 +353,27 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     }
     const int64_t M = (int64_t)plist.size();
     ws->n_moving = M;
+    // wall-word segments: per scene and wall, the words covering the scene's
+    // moving-particle list entries [plist_off, plist_off + np)
+    std::vector<int64_t> wseg_off(1, 0);
+    std::vector<int32_t> wseg_scene;
+    for (int s = 0; s < n_scenes; ++s) {
+        sd[s].wseg0 = (int64_t)wseg_scene.size();
+        const int64_t np = scenes[s].n_active - scenes[s].n_walls;
+        const int64_t nw = np > 0 ? (sd[s].plist_off + np + 63) / 64 - sd[s].plist_off / 64 : 0;
+        for (int64_t k = 0; k < scenes[s].n_walls; ++k) {
+            wseg_scene.push_back(s);
+            wseg_off.push_back(wseg_off.back() + nw);
+        }
+    }
+    ws->n_wseg = (int64_t)wseg_scene.size();
+    ws->n_words = wseg_off.back();
+    std::vector<int32_t> word_seg((size_t)std::max<int64_t>(ws->n_words, 1), 0);
+    for (int64_t g = 0; g < ws->n_wseg; ++g)
+        for (int64_t w = wseg_off[g]; w < wseg_off[g + 1]; ++w) word_seg[w] = (int32_t)g;
+    ws->word_walls_ok = 1;
+    for (int s = 0; s < n_scenes; ++s)
+        if (scenes[s].n_walls > MWD_WORD_WALLS) ws->word_walls_ok = 0;
     ws->n_wall_tests = wt[n_scenes];
     ws->key_span_total = kbase;
     ws->key_bits = bits_for(kbase);
@@ -424,8 +442,13 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     A(hist, hist_len);
     A(hist_off, hist_len);
     A(scan_tmp, ws->scan_tmp_len);
-    ws->scan_tmp_side_len = mwd_scan_tmp_len(std::max<int64_t>(ws->n_wall_tests, 1));
-    A(scan_tmp_side, ws->scan_tmp_side_len);
+    ws->scan_tmp_w_len = mwd_scan_tmp_len(std::max<int64_t>(ws->n_words, 1));
+    A(scan_tmp_w, ws->scan_tmp_w_len);
+    A(wseg_off, ws->n_wseg + 1);
+    A(wseg_scene, std::max<int64_t>(ws->n_wseg, 1));
+    A(word_seg, std::max<int64_t>(ws->n_words, 1));
+    A(hit_words, std::max<int64_t>(ws->n_words, 1));
+    A(gjk_words, std::max<int64_t>(ws->n_words, 1));
     A(counters, 20);
     A(flags, 1);
     A(laws, 2 * n_scenes);
@@ -446,6 +469,9 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     H2D(is_wall, is_wall.data(), n_rows);
     H2D(laws, laws.data(), 2 * n_scenes);
     H2D(scene_wt_off, wt.data(), n_scenes + 1);
+    H2D(wseg_off, wseg_off.data(), ws->n_wseg + 1);
+    if (ws->n_wseg) H2D(wseg_scene, wseg_scene.data(), ws->n_wseg);
+    if (ws->n_words) H2D(word_seg, word_seg.data(), ws->n_words);
 #undef H2D
     const size_t kbytes = sizeof(int32_t) * (size_t)ws->key_span_total;
     if ((ws->cell_table_ok && (hipMemsetAsync(ws->cell_of, 0xFF, kbytes, s) != hipSuccess ||
@@ -454,7 +480,9 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
         hipMemsetAsync(ws->counters, 0, sizeof(int64_t) * 20, s) != hipSuccess ||
         hipMemsetAsync(ws->rec_cnt, 0, sizeof(int32_t) * n_rows, s) != hipSuccess ||
         hipMemsetAsync(ws->scan_tmp, 0, sizeof(int64_t) * ws->scan_tmp_len, s) != hipSuccess ||
-        hipMemsetAsync(ws->scan_tmp_side, 0, sizeof(int64_t) * ws->scan_tmp_side_len, s) != hipSuccess ||
+        hipMemsetAsync(ws->scan_tmp_w, 0, sizeof(int64_t) * ws->scan_tmp_w_len, s) != hipSuccess ||
+        hipMemsetAsync(ws->hit_words, 0, sizeof(uint64_t) * std::max<int64_t>(ws->n_words, 1), s) != hipSuccess ||
+        hipMemsetAsync(ws->gjk_words, 0, sizeof(uint64_t) * std::max<int64_t>(ws->n_words, 1), s) != hipSuccess ||
         hipMemsetAsync(ws->flags, 0, sizeof(uint32_t), s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
         mwd_set_error("mwd_workspace_create: initialisation failed");
@@ -472,15 +500,6 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
             return MWD_E_HIP;
         }
     }
-    // side stream (default priority) and the events that fork to it and join back
-    if (hipStreamCreateWithFlags(&ws->side, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&ws->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ws->ev_join, hipEventDisableTiming) != hipSuccess) {
-        mwd_set_error("mwd_workspace_create: side stream set-up failed");
-        free_ws(ws);
-        return MWD_E_HIP;
-    }
-
     *out = ws;
     return MWD_OK;
 }

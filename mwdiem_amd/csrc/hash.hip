@@ -30,8 +30,12 @@
 //     cell rank by lowest row (scan over rows) -> per-cell set order,
 //     neighbour lookup (binary search of the sorted cell keys) and pair
 //     count -> scan of counts in rank order.
-// Then emission, one pair per thread. Wall tests are a flag/scan/scatter over
-// the (scene, wall, particle) test space.
+// Then emission, one pair per thread. Wall slab tests: on the key-table path
+// the key pass tests each particle against its scene's walls and gathers the
+// results by ballot into 64-bit words laid out in the reference's test order
+// (scene, wall, particle); a scan over words, in the same launch as the scan
+// over rows, places the wall pairs. Otherwise one scan over the (scene, wall,
+// particle) test space tests, counts and writes them.
 #include "mwd_internal.h"
 #include "mwd_scan.h"
 
@@ -79,24 +83,136 @@ __global__ void k_sort_keys(const double* __restrict__ com, const int64_t* __res
     if (sph) pack_sphere(sph, com, radius, row);
 }
 
+// Wall slab test of particle row rj (centre x) against wall row rk
+// (MWDiem.py:807-811): the pair is listed when |dd| < thr, dd = n.(x - c).
+// With the survivor list a listed pair also goes to GJK unless the wall's
+// slab |n.(v - c)| <= h (h: the half-extent of the wall's vertices along n)
+// and the particle's bounding sphere are apart by more than
+// 1e-6 (1 + |dd| + h): then every particle vertex lies outside the slab, the
+// bodies are separated and GJK would return 0.
+__device__ __forceinline__ double wall_dd(d3 x, const double* __restrict__ wall_n, const double* __restrict__ wall_c,
+                                          int64_t rk) {
+    return dot3(ld3(wall_n + 3 * rk), x - ld3(wall_c + 3 * rk));
+}
+
+__device__ __forceinline__ bool wall_apart(double dd, const double* __restrict__ wall_n,
+                                           const double* __restrict__ wall_c, const double* __restrict__ radius,
+                                           const double* __restrict__ verts, const int32_t* __restrict__ nverts,
+                                           int32_t vmax, int64_t rj, int64_t rk) {
+    const d3 n = ld3(wall_n + 3 * rk);
+    const d3 c = ld3(wall_c + 3 * rk);
+    const double* v = verts + (int64_t)vmax * 3 * rk;
+    const int nv = nverts[rk];
+    double h = 0.0;
+    for (int i0 = 0; i0 < nv; i0 += 8) {  // eight vertex loads in flight
+        d3 p[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (i0 + u < nv) p[u] = ld3(v + 3 * (i0 + u));
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (i0 + u < nv) h = fmax(h, fabs(dot3(n, p[u] - c)));
+    }
+    const double a = fabs(dd);
+    return a - h - radius[rj] > 1e-6 * (1.0 + a + h);
+}
+
+// 0: not listed; 1: listed; (1 << 32) | 1: listed and goes to GJK.
+__device__ __forceinline__ int64_t wall_pair_bits(d3 x, const double* __restrict__ wall_n,
+                                                  const double* __restrict__ wall_c,
+                                                  const double* __restrict__ radius, const double* __restrict__ verts,
+                                                  const int32_t* __restrict__ nverts, int32_t vmax, int64_t rj,
+                                                  int64_t rk, double thr, bool with_surv) {
+    const double dd = wall_dd(x, wall_n, wall_c, rk);
+    if (!(fabs(dd) < thr)) return 0;
+    if (!with_surv || wall_apart(dd, wall_n, wall_c, radius, verts, nverts, vmax, rj, rk)) return 1;
+    return (int64_t(1) << 32) | 1;
+}
+
 // Key-table path, one thread per moving particle: its key, the member count
-// of that key and the key's lowest row.
+// of that key and the key's lowest row. With wall words, also the particle's
+// slab tests against every wall of its scene, gathered by ballots into the
+// 64-bit words of the tests (a wave is 64 consecutive list entries, so its
+// ballot for one scene and wall is that segment's word; a scan over the
+// words then places the pairs in the reference's order).
 constexpr int32_t KEY_MIN_EMPTY = 0x7F7F7F7F;  // above every row: the mark of an empty key
+
+struct WallWordsIn {  // what the slab tests read and write (hit_words NULL: no tests here)
+    const double* wall_n;
+    const double* wall_c;
+    const double* radius;
+    const double* verts;
+    const int32_t* nverts;
+    int32_t vmax;
+    bool with_surv;
+    const int64_t* wseg_off;
+    uint64_t* hit_words;
+    uint64_t* gjk_words;
+};
 
 __global__ void k_key_count(const double* __restrict__ com, const int64_t* __restrict__ keys_in,
                             const int32_t* __restrict__ plist, int64_t M_cap, const int64_t* m_live,
                             const int32_t* __restrict__ scene_of,
                             const SceneDev* __restrict__ scenes, int32_t* __restrict__ row_key,
                             int32_t* __restrict__ key_cnt, int32_t* __restrict__ key_min, uint32_t* flags,
-                            const double* __restrict__ radius, double* __restrict__ sph) {
+                            const double* __restrict__ radius, double* __restrict__ sph, WallWordsIn ww) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (*m_live < M_cap ? *m_live : M_cap)) return;
-    const int32_t row = plist[i];
-    if (sph) pack_sphere(sph, com, radius, row);
-    const uint32_t k = global_key(com, keys_in, row, scenes[scene_of[row]], flags);
-    row_key[row] = (int32_t)k;
-    atomicAdd(&key_cnt[2 * k], 1);
-    atomicMin(&key_min[2 * k], row);
+    const bool live = i < (*m_live < M_cap ? *m_live : M_cap);
+    if (!ww.hit_words && !live) return;
+    int32_t row = 0, s = -1;
+    if (live) {
+        row = plist[i];
+        s = scene_of[row];
+        const SceneDev& sc = scenes[s];
+        if (sph) pack_sphere(sph, com, radius, row);
+        const uint32_t k = global_key(com, keys_in, row, sc, flags);
+        row_key[row] = (int32_t)k;
+        atomicAdd(&key_cnt[2 * k], 1);
+        atomicMin(&key_min[2 * k], row);
+    }
+    if (!ww.hit_words) return;
+    // the particle's tests against its scene's walls (at most MWD_WORD_WALLS):
+    // first every plane distance (independent loads), then the slab-apart
+    // test of the listed pairs
+    uint32_t hm = 0, gm = 0;
+    if (live) {
+        const SceneDev& sc = scenes[s];
+        const d3 x = ld3(com + 3 * (int64_t)row);
+        const int W = (int)sc.n_walls;
+#pragma unroll 8
+        for (int k = 0; k < W; ++k)
+            if (fabs(wall_dd(x, ww.wall_n, ww.wall_c, sc.row0 + k)) < sc.wall_thr) hm |= 1u << k;
+        if (ww.with_surv) {
+            for (uint32_t t = hm; t; t &= t - 1) {
+                const int k = __ffs(t) - 1;
+                const double dd = wall_dd(x, ww.wall_n, ww.wall_c, sc.row0 + k);
+                if (!wall_apart(dd, ww.wall_n, ww.wall_c, ww.radius, ww.verts, ww.nverts, ww.vmax, row, sc.row0 + k))
+                    gm |= 1u << k;
+            }
+        }
+    }
+    // into the words, one scene of the wave's live lanes at a time (usually one)
+    const int lane = threadIdx.x & (MWD_WAVE - 1);
+    const int64_t w = i >> 6;  // the wave's word in the list entries' numbering
+    uint64_t todo = __ballot(live);
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int sl = __shfl(s, leader, MWD_WAVE);
+        const bool in = live && s == sl;
+        const SceneDev& sc = scenes[sl];
+        const int64_t w0 = sc.plist_off >> 6;
+        const int W = (int)sc.n_walls;
+        for (int k = 0; k < W; ++k) {
+            const uint64_t hw = __ballot(in && ((hm >> k) & 1u));
+            const uint64_t gw = __ballot(in && ((gm >> k) & 1u));
+            if (lane == leader) {
+                const int64_t at = ww.wseg_off[sc.wseg0 + k] + (w - w0);
+                ww.hit_words[at] = hw;
+                ww.gjk_words[at] = gw;
+            }
+        }
+        todo &= ~__ballot(in);
+    }
 }
 
 // Key-table path: each particle takes a slot in its cell's member range. The
@@ -492,31 +608,12 @@ __device__ __forceinline__ int scene_of_test(const int64_t* wt_off, int S, int64
     return lo;
 }
 
-// Wall slab test (MWDiem.py:807-811) fused with its compaction: test t of a
-// scene pairs wall k with particle j (walls outer, particles inner); a pair
-// whose centre lies within wall_thr of the wall's plane is written, as
-// [j, k], at its offset among the scene-ordered hits.
-struct WallIO {
-    const double* com;
-    const double* wall_n;
-    const double* wall_c;
+// Test t of the wall slab test space: scene s's tests are
+// [wt_off[s], wt_off[s+1]), wall k outer and particle j inner.
+struct WallTests {
     const SceneDev* scenes;
     int S;
     const int64_t* wt_off;
-    int32_t* pairs;
-    int64_t cap;
-    // non-NULL (with bounding radii): the wall pairs go to GJK as survivors,
-    // listed in surv from its start, except pairs the wall's slab separates
-    // from the particle's bounding sphere, which get collision 0 here (a
-    // wall plate's bounding sphere, half its diagonal, meets nearly every
-    // particle in its slab, so the sphere test decides nothing for them)
-    int32_t* surv;
-    int32_t* collision;
-    const double* radius;
-    const double* verts;
-    const int32_t* nverts;
-    int32_t vmax;
-    int64_t* counters;  // [1] wall pairs, [17] wall survivors (written by finish)
     __device__ __forceinline__ void decode(int64_t t, const SceneDev** sc, int64_t* rj, int64_t* rk) const {
         const int s = scene_of_test(wt_off, S, t);
         *sc = &scenes[s];
@@ -529,35 +626,20 @@ struct WallIO {
         *rk = (*sc)->row0 + k;
         *rj = (*sc)->row0 + (*sc)->n_walls + j;
     }
-    // The wall's slab |n.(x - c)| <= h (h: the half-extent of its vertices
-    // along n) and the particle's bounding sphere are apart by more than
-    // 1e-6 (1 + |dd| + h): every particle vertex lies outside the slab, the
-    // bodies are separated and GJK would return 0.
-    __device__ __forceinline__ bool apart(int64_t rj, int64_t rk, d3 n, double dd) const {
-        const d3 c = ld3(wall_c + 3 * rk);
-        const double* v = verts + (int64_t)vmax * 3 * rk;
-        double h = 0.0;
-        for (int i = 0; i < nverts[rk]; ++i) h = fmax(h, fabs(dot3(n, ld3(v + 3 * i) - c)));
-        const double a = fabs(dd);
-        return a - h - radius[rj] > 1e-6 * (1.0 + a + h);
-    }
-    // packed: bit 0 the pair is listed, bit 32 it goes to GJK
-    __device__ __forceinline__ int64_t load(int64_t t) const {
-        const SceneDev* sc;
-        int64_t rj, rk;
-        decode(t, &sc, &rj, &rk);
-        const d3 n = ld3(wall_n + 3 * rk);
-        const double dd = dot3(n, ld3(com + 3 * rj) - ld3(wall_c + 3 * rk));
-        if (!(fabs(dd) < sc->wall_thr)) return 0;
-        if (!surv || apart(rj, rk, n, dd)) return 1;
-        return (int64_t(1) << 32) | 1;
-    }
-    __device__ __forceinline__ void store(int64_t t, int64_t off, int64_t v) const {
+};
+
+// What a listed wall pair writes: [j, k] at its offset among the scene-ordered
+// hits; with the survivor list, either its slot there (bit 32 of the count:
+// it goes to GJK) or collision 0 (decided by the slab).
+struct WallOut {
+    int32_t* pairs;
+    int64_t cap;
+    int32_t* surv;
+    int32_t* collision;
+    int64_t* counters;  // [1] wall pairs, [17] wall survivors (written by finish)
+    __device__ __forceinline__ void put(int64_t rj, int64_t rk, int64_t off, int64_t v) const {
         const int64_t pos = off & 0xffffffffll;
         if (!v || pos >= cap) return;
-        const SceneDev* sc;
-        int64_t rj, rk;
-        decode(t, &sc, &rj, &rk);
         pairs[2 * pos] = (int32_t)rj;
         pairs[2 * pos + 1] = (int32_t)rk;
         if (surv) {
@@ -569,6 +651,103 @@ struct WallIO {
         counters[1] = total & 0xffffffffll;
         counters[17] = total >> 32;
     }
+};
+
+// Wall slab test (MWDiem.py:807-811) fused with its compaction: test t of a
+// scene pairs wall k with particle j (walls outer, particles inner); a pair
+// whose centre lies within wall_thr of the wall's plane is written, as
+// [j, k], at its offset among the scene-ordered hits. With bounding radii
+// the wall pairs go to GJK as survivors, listed in surv from its start,
+// except pairs the wall's slab separates from the particle's bounding sphere,
+// which get collision 0 here (a wall plate's bounding sphere, half its
+// diagonal, meets nearly every particle in its slab, so the sphere test
+// decides nothing for them). The count is packed: bit 0 the pair is listed,
+// bit 32 it goes to GJK.
+struct WallIO {
+    const double* com;
+    const double* wall_n;
+    const double* wall_c;
+    WallTests tests;
+    WallOut out;
+    const double* radius;
+    const double* verts;
+    const int32_t* nverts;
+    int32_t vmax;
+    __device__ __forceinline__ int64_t load(int64_t t) const {
+        const SceneDev* sc;
+        int64_t rj, rk;
+        tests.decode(t, &sc, &rj, &rk);
+        return wall_pair_bits(ld3(com + 3 * rj), wall_n, wall_c, radius, verts, nverts, vmax, rj, rk, sc->wall_thr,
+                              out.surv != nullptr);
+    }
+    __device__ __forceinline__ void store(int64_t t, int64_t off, int64_t v) const {
+        if (!v) return;
+        const SceneDev* sc;
+        int64_t rj, rk;
+        tests.decode(t, &sc, &rj, &rk);
+        out.put(rj, rk, off, v);
+    }
+    __device__ __forceinline__ void finish(int64_t total) const { out.finish(total); }
+};
+
+// The wall pairs placed from the key pass's bit words: a scan over words
+// (64 tests each, concatenated in the reference's test order) whose count is
+// the word's listed pairs | its pairs for GJK << 32, as WallIO's. Bits of
+// entries past the scene's live count (rows not yet spawned) are ignored.
+struct WallWordIO {
+    const uint64_t* hit_words;
+    const uint64_t* gjk_words;
+    const int64_t* wseg_off;
+    const int32_t* wseg_scene;
+    const int32_t* word_seg;
+    const SceneDev* scenes;
+    WallOut out;
+    // segment of word w, its scene, and the list entry of the word's bit 0;
+    // the bits that lie in the scene's live entries
+    __device__ __forceinline__ uint64_t locate(int64_t w, int64_t* seg, const SceneDev** sc, int64_t* e0) const {
+        const int64_t lo = word_seg[w];
+        *seg = lo;
+        *sc = &scenes[wseg_scene[lo]];
+        const int64_t first = (*sc)->plist_off, end = first + (*sc)->n_active - (*sc)->n_walls;
+        *e0 = ((first >> 6) + (w - wseg_off[lo])) << 6;
+        uint64_t m = ~0ull;
+        if (*e0 < first) m &= ~0ull << (first - *e0);
+        if (end <= *e0) m = 0;
+        else if (end - *e0 < 64) m &= ~0ull >> (64 - (end - *e0));
+        return m;
+    }
+    __device__ __forceinline__ int64_t load(int64_t w) const {
+        uint64_t h = hit_words[w], g = gjk_words[w];
+        if (!(h | g)) return 0;
+        int64_t seg, e0;
+        const SceneDev* sc;
+        const uint64_t m = locate(w, &seg, &sc, &e0);
+        h &= m;
+        g &= m;
+        return ((int64_t)__popcll(g) << 32) | (int64_t)__popcll(h);
+    }
+    // The set bits of word w in ascending order, each pair at the word's
+    // offset plus the set bits below it (one word per thread: a word holds
+    // few pairs, at most 64).
+    __device__ __forceinline__ void store(int64_t w, int64_t off, int64_t v) const {
+        if (!v) return;
+        int64_t seg, e0;
+        const SceneDev* sc;
+        const uint64_t m = locate(w, &seg, &sc, &e0);
+        uint64_t h = hit_words[w] & m;
+        const uint64_t g = gjk_words[w] & m;
+        const int64_t rk = sc->row0 + (seg - sc->wseg0);
+        const int64_t rj0 = sc->row0 + sc->n_walls - sc->plist_off + e0;  // row of list entry e0
+        int64_t o = off;
+        while (h) {
+            const int b = __ffsll((long long)h) - 1;
+            h &= h - 1;
+            const int64_t gb = (int64_t)((g >> b) & 1u) << 32;
+            out.put(rj0 + b, rk, o, gb | 1);
+            o += gb + 1;
+        }
+    }
+    __device__ __forceinline__ void finish(int64_t total) const { out.finish(total); }
 };
 
 // Sorted path: cell heads over the sorted keys fused with their numbering:
@@ -722,6 +901,20 @@ int mwd_launch_cell_keys(const double* com, int64_t n, const double* cell, int64
     return MWD_OK;
 }
 
+namespace {
+// The cell-table scan over rows and the wall words' scan in one launch (rows:
+// 4 items per thread up to a million, else 16; words: one per thread, as a
+// word's pairs are written one after another).
+int scan_cells_and_walls(const CellTableIO& ct, int64_t n_rows, int64_t* tmp, int64_t tmp_len, const WallWordIO& wio,
+                         int64_t n_words, int64_t* tmp_w, int64_t tmp_w_len, hipStream_t s) {
+    if (n_rows <= SCAN_SHORT_MAX)
+        return mwd_scan_io2<CellTableIO, SCAN_ITEMS_SHORT, WallWordIO, 1>(
+            ct, n_rows, nullptr, tmp, tmp_len, wio, n_words, nullptr, tmp_w, tmp_w_len, s);
+    return mwd_scan_io2<CellTableIO, SCAN_ITEMS_LONG, WallWordIO, 1>(
+        ct, n_rows, nullptr, tmp, tmp_len, wio, n_words, nullptr, tmp_w, tmp_w_len, s);
+}
+}  // namespace
+
 int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t* keys_in) {
     hipStream_t s = ws->stream;
     const int64_t M = ws->n_moving;
@@ -740,48 +933,50 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
     // pairs are written: the wall pairs (all of them) at their own indices,
     // the cell pairs whose spheres meet appended after them by the emission
     ws->sphere_folded = st->radius != nullptr;
-    // -- wall slab pairs: test, count and write in one pass, on the side
-    // stream; the cell pass runs meanwhile and joins before it needs the wall
-    // pair count (the offset of the cell pairs)
-    bool joined = true;
-    if (walls) {
-        const WallIO io{st->com,      st->wall_n,    st->wall_c,   ws->scenes_dev, ws->n_scenes,
-                        ws->scene_wt_off, ws->pairs, ws->pair_cap, st->radius ? ws->surv : nullptr,
-                        ws->collision, st->radius,   st->verts,    st->nverts,     st->vmax,
-                        ctr};
-        MWD_HIP_CHECK(hipEventRecord(ws->ev_fork, s));
-        MWD_HIP_CHECK(hipStreamWaitEvent(ws->side, ws->ev_fork, 0));
-        rc = mwd_scan_io(io, ws->n_wall_tests, ws->wt_live, nullptr, ws->scan_tmp_side, ws->scan_tmp_side_len,
-                         ws->side);
+    const WallTests wtests{ws->scenes_dev, ws->n_scenes, ws->scene_wt_off};
+    const WallOut wout{ws->pairs, ws->pair_cap, st->radius ? ws->surv : nullptr, ws->collision, ctr};
+    // -- wall slab pairs and cell keys. On the key-table path with every
+    // scene walled (at most MWD_MASK_WALLS walls each), the key pass also
+    // tests each particle against its scene's walls and leaves the results as
+    // bits, and the scan that places the wall pairs runs in the launch of the
+    // scan over rows. Otherwise the wall scan tests as it goes, first. Either
+    // way the step is one chain of launches on one stream (a parallel branch
+    // for the wall scan costs more in cross-queue waits inside a replayed
+    // graph than it overlaps).
+    const bool words = walls && M > 0 && ws->cell_table && ws->n_words > 0 && ws->word_walls_ok;
+    if (walls && !words) {
+        const WallIO io{st->com, st->wall_n, st->wall_c, wtests, wout, st->radius, st->verts, st->nverts, st->vmax};
+        rc = mwd_scan_io(io, ws->n_wall_tests, ws->wt_live, nullptr, ws->scan_tmp, ws->scan_tmp_len, s);
         if (rc) return rc;
-        MWD_HIP_CHECK(hipEventRecord(ws->ev_join, ws->side));
-        joined = false;
     }
-    auto join = [&]() -> int {
-        if (!joined) {
-            MWD_HIP_CHECK(hipStreamWaitEvent(s, ws->ev_join, 0));
-            joined = true;
-        }
-        return MWD_OK;
-    };
     // -- linked-cell pairs
     if (M > 0) {
         mwd_mark(ws, 1);
         if (ws->cell_table) {
+            const WallWordsIn ww{st->wall_n,     st->wall_c,   st->radius,
+                                 st->verts,      st->nverts,   st->vmax,
+                                 st->radius != nullptr,        ws->wseg_off,
+                                 words ? ws->hit_words : nullptr, ws->gjk_words};
             hipLaunchKernelGGL(k_key_count, dim3(blocks_for(M)), dim3(256), 0, s, st->com, keys_in, ws->plist, M,
-                               ws->m_live, ws->scene_of, ws->scenes_dev, ws->row_key, ws->key_cnt, ws->key_min, ws->flags,
-                               st->radius, st->radius ? ws->sph : nullptr);
+                               ws->m_live, ws->scene_of, ws->scenes_dev, ws->row_key, ws->key_cnt, ws->key_min,
+                               ws->flags, st->radius, st->radius ? ws->sph : nullptr, ww);
             MWD_LAUNCH_CHECK();
             mwd_mark(ws, 2);
-            rc = mwd_scan_io(CellTableIO{ws->row_key, ws->key_cnt, ws->key_min, ws->cell_start, ws->cell_key,
-                                         ws->cell_of, ctr + 0},
-                             ws->n_rows, nullptr, nullptr, ws->scan_tmp, ws->scan_tmp_len, s);
+            const CellTableIO ct{ws->row_key, ws->key_cnt, ws->key_min, ws->cell_start, ws->cell_key,
+                                 ws->cell_of, ctr + 0};
+            if (words) {
+                const WallWordIO wio{ws->hit_words, ws->gjk_words, ws->wseg_off, ws->wseg_scene, ws->word_seg,
+                                     ws->scenes_dev, wout};
+                rc = scan_cells_and_walls(ct, ws->n_rows, ws->scan_tmp, ws->scan_tmp_len, wio, ws->n_words,
+                                          ws->scan_tmp_w, ws->scan_tmp_w_len, s);
+            } else {
+                rc = mwd_scan_io(ct, ws->n_rows, nullptr, nullptr, ws->scan_tmp, ws->scan_tmp_len, s);
+            }
             if (rc) return rc;
             hipLaunchKernelGGL(k_key_scatter, dim3(blocks_for(M)), dim3(256), 0, s, ws->plist, M, ws->m_live,
                                ws->row_key,
                                ws->key_cnt, ws->cell_of, ws->cell_start, ws->vals);
             MWD_LAUNCH_CHECK();
-            if ((rc = join())) return rc;
             const CellInfoIO io{ws->vals,      ws->cell_start, ws->cell_key,  ws->cell_of, ws->scene_of,
                                 ws->scenes_dev, ws->perm,      ws->cell_self, ws->cell_seg, ws->cell_off,
                                 ctr,           ws->pair_cap,   ws->flags,     ws->sphere_folded != 0};
@@ -830,7 +1025,6 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
                                ws->cell_key, ctr + 0, ws->scene_of, ws->scenes_dev, ws->row_rank, ws->perm,
                                ws->cell_self, ws->cell_seg, ws->cell_cnt, ws->flags);
             MWD_LAUNCH_CHECK();
-            if ((rc = join())) return rc;
             rc = mwd_scan_io(CellCountIO{ws->cell_cnt, ws->cell_off, ctr, ws->pair_cap, ws->flags,
                                          ws->sphere_folded != 0},
                              M, ctr + 0,
@@ -857,7 +1051,6 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
 #undef MWD_FILL
         MWD_LAUNCH_CHECK();
     }
-    if ((rc = join())) return rc;
     mwd_mark(ws, 4);
     if (M <= 0) {  // no cell-count scan closed the candidate count
         hipLaunchKernelGGL(k_finish_pairs, dim3(1), dim3(1), 0, s, ctr, ws->pair_cap, ws->flags,

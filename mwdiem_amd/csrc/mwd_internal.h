@@ -48,6 +48,7 @@ struct SceneDev {
     int64_t kbase;      // first global (sorted) key of the scene
     int64_t wt_off;     // first wall-slab test index of the scene
     int64_t plist_off;  // first entry of the scene in the moving-particle list
+    int64_t wseg0;      // first wall-word segment of the scene (segment wseg0 + k: wall k)
     double wall_thr;
     mwd_law law_p, law_w;
     int32_t n_off;      // 7 (reference stencil) or 13 (full half shell)
@@ -55,6 +56,7 @@ struct SceneDev {
 };
 
 #define MWD_MAX_OFF 13
+#define MWD_WORD_WALLS 32  // walls per scene whose slab tests the key pass can gather into words
 
 // Final GJK simplex of a colliding pair (input of EPA).
 struct Simplex {
@@ -299,11 +301,24 @@ struct mwd_workspace {
     int64_t* hist_off;         // [256 * max_tiles]
     int64_t* scan_tmp;         // block sums
     int64_t scan_tmp_len;
-    // the wall slab scan runs on a side stream, beside the cell pass
-    hipStream_t side;
-    hipEvent_t ev_fork, ev_join;
-    int64_t* scan_tmp_side;    // scratch of the scan on the side stream
-    int64_t scan_tmp_side_len;
+    // wall slab tests as bit words, written by the key pass: segment g (one
+    // per scene and wall, scene-major) holds the words of its scene's
+    // moving-particle list entries, 64 per word (word w: entries 64w ..
+    // 64w+63; a scene's first and last words are shared with its
+    // neighbours' ranges and its bits there are its own). Concatenated, the
+    // words run in the reference's test order, so a scan over words places
+    // the pairs. The words' scan has its own scratch: it runs in the launch
+    // of the scan over rows.
+    int64_t n_wseg;            // segments (sum over scenes of their walls)
+    int64_t n_words;           // words over all segments
+    int64_t* wseg_off;         // [n_wseg + 1] first word of each segment
+    int32_t* wseg_scene;       // [n_wseg] scene of each segment
+    int32_t* word_seg;         // [n_words] segment of each word
+    int word_walls_ok;         // no scene has more than MWD_WORD_WALLS walls
+    uint64_t* hit_words;       // [n_words] bit set: the pair is listed
+    uint64_t* gjk_words;       // [n_words] bit set: the listed pair goes to GJK
+    int64_t* scan_tmp_w;
+    int64_t scan_tmp_w_len;
     // counters (device): 0 n_cells, 1 n_wall_pairs, 2 n_cell_pairs, 3 n_pairs
     // (capped), 4 n_contacts, 5 n_records, 6 unused,
     // 7 n_pairs before capping, 8 simplex slots claimed this step, 9 radix

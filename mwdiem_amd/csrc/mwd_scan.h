@@ -88,28 +88,38 @@ __device__ __forceinline__ void status_store(uint64_t* p, uint64_t v) {
 // single bank.
 __device__ __forceinline__ int scan_pad(int i) { return i + (i >> 5); }
 
-template <class IO, int SCAN_ITEMS>
-__global__ void __launch_bounds__(SCAN_THREADS) k_scan_io(IO io, int64_t n_cap, const int64_t* n_dev, int64_t* total,
-                                                          uint64_t* ctr, uint64_t* status2, int64_t S) {
-    using V = std::decay_t<decltype(io.load(int64_t(0)))>;
+// LDS of one block of a scan: the tile's counts and offsets (padded, see
+// scan_pad), the waves' partial sums and the tile's prefix and ticket.
+template <class V, int SCAN_ITEMS>
+struct ScanLds {
+    static constexpr int TILE = SCAN_THREADS * SCAN_ITEMS;
+    static constexpr int LEN = TILE + TILE / 32;
+    int64_t lw[SCAN_WAVES];
+    int64_t prefix;
+    uint64_t claim;
+    V v[LEN];
+    int64_t off[LEN];
+};
+
+// One block's part of a scan: claim a tile, scan it, look back, store. `bid`
+// is the block's index among the scan's blocks (those at or past the live
+// tile count return at once).
+template <class IO, int SCAN_ITEMS, class V>
+__device__ __forceinline__ void scan_block(const IO& io, int64_t n_cap, const int64_t* n_dev, int64_t* total,
+                                           uint64_t* ctr, uint64_t* status2, int64_t S, int64_t bid,
+                                           ScanLds<V, SCAN_ITEMS>& L) {
     constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
-    constexpr int SCAN_LDS = SCAN_TILE + SCAN_TILE / 32;
-    __shared__ int64_t lw[SCAN_WAVES];
-    __shared__ int64_t s_prefix;
-    __shared__ uint64_t s_claim;
-    __shared__ V s_v[SCAN_LDS];
-    __shared__ int64_t s_off[SCAN_LDS];
     const int64_t n = live_n(n_cap, n_dev);
     int64_t ntl = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (ntl < 1) ntl = 1;  // one block runs even when nothing is live: it clears and advances
-    if ((int64_t)blockIdx.x >= ntl) return;
+    if (bid >= ntl) return;
     if (threadIdx.x == 0) {
         const uint64_t c = atomicAdd((unsigned long long*)ctr, 1ull);
-        s_claim = c;
+        L.claim = c;
         if ((int64_t)(c & 0xffffffffull) == ntl - 1) status_store(ctr, ((c >> 32) + 1) << 32);  // last ticket
     }
     __syncthreads();
-    const uint64_t claim = s_claim;
+    const uint64_t claim = L.claim;
     const int64_t tile = (int64_t)(claim & 0xffffffffull);
     const int par = (int)((claim >> 32) & 1);
     uint64_t* status = status2 + par * S;
@@ -120,18 +130,18 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_io(IO io, int64_t n_cap, 
         for (int k = 0; k < SCAN_ITEMS; ++k) {  // striped loads
             const int il = k * SCAN_THREADS + threadIdx.x;
             const int64_t i = base + il;
-            s_v[scan_pad(il)] = (i < n) ? io.load(i) : 0;
+            L.v[scan_pad(il)] = (i < n) ? io.load(i) : 0;
         }
         __syncthreads();
         V v[SCAN_ITEMS];
         int64_t acc = 0;
 #pragma unroll
         for (int k = 0; k < SCAN_ITEMS; ++k) {  // blocked values
-            v[k] = s_v[scan_pad(threadIdx.x * SCAN_ITEMS + k)];
+            v[k] = L.v[scan_pad(threadIdx.x * SCAN_ITEMS + k)];
             acc += v[k];
         }
         int64_t agg;
-        const int64_t incl = block_inclusive_scan(acc, lw, &agg);
+        const int64_t incl = block_inclusive_scan(acc, L.lw, &agg);
         if (threadIdx.x < MWD_WAVE) {
             const int lane = threadIdx.x;
             int64_t prefix = 0;
@@ -159,7 +169,7 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_io(IO io, int64_t n_cap, 
                 if (lane == 0) status_store(&status[tile], ST_INC | (uint64_t)(prefix + agg));
             }
             if (lane == 0) {
-                s_prefix = prefix;
+                L.prefix = prefix;
                 if (base + SCAN_TILE >= n) {  // the last live tile
                     if (total) *total = prefix + agg;
                     io.finish(prefix + agg);
@@ -167,10 +177,10 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_io(IO io, int64_t n_cap, 
             }
         }
         __syncthreads();
-        int64_t run = incl - acc + s_prefix;
+        int64_t run = incl - acc + L.prefix;
 #pragma unroll
         for (int k = 0; k < SCAN_ITEMS; ++k) {  // blocked offsets
-            s_off[scan_pad(threadIdx.x * SCAN_ITEMS + k)] = run;
+            L.off[scan_pad(threadIdx.x * SCAN_ITEMS + k)] = run;
             run += v[k];
         }
         __syncthreads();
@@ -178,7 +188,7 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_io(IO io, int64_t n_cap, 
         for (int k = 0; k < SCAN_ITEMS; ++k) {  // striped stores
             const int il = k * SCAN_THREADS + threadIdx.x;
             const int64_t i = base + il;
-            if (i < n) io.store(i, s_off[scan_pad(il)], s_v[scan_pad(il)]);
+            if (i < n) io.store(i, L.off[scan_pad(il)], L.v[scan_pad(il)]);
         }
     } else if (threadIdx.x == 0) {  // nothing live (only tile 0 exists then)
         if (total) *total = 0;
@@ -186,6 +196,34 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_io(IO io, int64_t n_cap, 
     }
     // clear the other buffer for the next call
     for (int64_t j = tile * SCAN_THREADS + threadIdx.x; j < S; j += ntl * SCAN_THREADS) other[j] = 0;
+}
+
+template <class IO>
+using ScanV = std::decay_t<decltype(static_cast<const IO*>(nullptr)->load(int64_t(0)))>;
+
+template <class IO, int SCAN_ITEMS>
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_io(IO io, int64_t n_cap, const int64_t* n_dev, int64_t* total,
+                                                          uint64_t* ctr, uint64_t* status2, int64_t S) {
+    __shared__ ScanLds<ScanV<IO>, SCAN_ITEMS> L;
+    scan_block<IO, SCAN_ITEMS>(io, n_cap, n_dev, total, ctr, status2, S, (int64_t)blockIdx.x, L);
+}
+
+// Two independent scans in one launch: the first nbA blocks run scan A, the
+// others scan B, each with its own scratch. They share the launch (and the
+// LDS, as a union), so two short scans cost about one.
+template <class IOA, int IA, class IOB, int IB>
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_io2(IOA ioa, int64_t na_cap, const int64_t* na_dev,
+                                                           uint64_t* ctra, uint64_t* sta, int64_t Sa, int64_t nba,
+                                                           IOB iob, int64_t nb_cap, const int64_t* nb_dev,
+                                                           uint64_t* ctrb, uint64_t* stb, int64_t Sb) {
+    __shared__ union U {
+        ScanLds<ScanV<IOA>, IA> a;
+        ScanLds<ScanV<IOB>, IB> b;
+    } L;
+    if ((int64_t)blockIdx.x < nba)
+        scan_block<IOA, IA>(ioa, na_cap, na_dev, nullptr, ctra, sta, Sa, (int64_t)blockIdx.x, L.a);
+    else
+        scan_block<IOB, IB>(iob, nb_cap, nb_dev, nullptr, ctrb, stb, Sb, (int64_t)blockIdx.x - nba, L.b);
 }
 
 // out[i] = exclusive prefix of in[i]
@@ -226,6 +264,28 @@ int mwd_scan_io(const IO& io, int64_t n_cap, const int64_t* n_dev, int64_t* tota
             hipLaunchKernelGGL((k_scan_io<IO, SCAN_ITEMS_LONG>), dim3((unsigned)nb), dim3(SCAN_THREADS), 0, s, io, n,
                                n_dev, total, ctr, st2, S);
     }
+    MWD_LAUNCH_CHECK();
+    return MWD_OK;
+}
+
+// Scans A and B in one launch (k_scan_io2), A with IA items per thread and B
+// with IB; each takes its scratch as mwd_scan_io does.
+template <class IOA, int IA, class IOB, int IB>
+int mwd_scan_io2(const IOA& ioa, int64_t na_cap, const int64_t* na_dev, int64_t* tmpa, int64_t tmpa_len,
+                 const IOB& iob, int64_t nb_cap, const int64_t* nb_dev, int64_t* tmpb, int64_t tmpb_len,
+                 hipStream_t s) {
+    const int64_t ta = (int64_t)SCAN_THREADS * IA, tb = (int64_t)SCAN_THREADS * IB;
+    const int64_t nba = na_cap > 0 ? (na_cap + ta - 1) / ta : 1;
+    const int64_t nbb = nb_cap > 0 ? (nb_cap + tb - 1) / tb : 1;
+    const int64_t Sa = (tmpa_len - 1) / 2, Sb = (tmpb_len - 1) / 2;
+    if (nba > Sa || nbb > Sb) {
+        mwd_set_error("scan scratch too small");
+        return MWD_E_CAPACITY;
+    }
+    hipLaunchKernelGGL((k_scan_io2<IOA, IA, IOB, IB>), dim3((unsigned)(nba + nbb)), dim3(SCAN_THREADS), 0, s, ioa,
+                       na_cap > 0 ? na_cap : 0, na_dev, reinterpret_cast<uint64_t*>(tmpa),
+                       reinterpret_cast<uint64_t*>(tmpa + 1), Sa, nba, iob, nb_cap > 0 ? nb_cap : 0, nb_dev,
+                       reinterpret_cast<uint64_t*>(tmpb), reinterpret_cast<uint64_t*>(tmpb + 1), Sb);
     MWD_LAUNCH_CHECK();
     return MWD_OK;
 }
