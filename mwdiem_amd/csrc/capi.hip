@@ -45,7 +45,7 @@ int dalloc(T** p, int64_t count) {
 
 void free_ws(mwd_workspace* ws) {
     if (!ws) return;
-    void* ptrs[] = {ws->scenes_dev, ws->cell_of, ws->plist, ws->wlist, ws->scene_of, ws->is_wall, ws->keys, ws->keys_alt, ws->vals,
+    void* ptrs[] = {ws->scenes_dev, ws->key_seg, ws->plist, ws->wlist, ws->scene_of, ws->is_wall, ws->keys, ws->keys_alt, ws->vals,
                     ws->vals_alt, ws->cell_start, ws->cell_key, ws->cell_self, ws->cell_seg,
                     ws->row_first, ws->row_rank, ws->cell_cnt, ws->cell_off, ws->perm, ws->key_cnt,
                     ws->row_key, ws->sph,
@@ -414,7 +414,7 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     ws->cell_table_ok = ws->key_span_total <= (int64_t(1) << 28) && n_rows < 0x7F7F7F7F;
     ws->cell_table = ws->cell_table_ok;
     if (ws->cell_table_ok) {
-        A(cell_of, ws->key_span_total);
+        A(key_seg, ws->key_span_total);
         A(key_cnt, 2 * ws->key_span_total);  // (count, lowest row) interleaved: one cache line per key
         ws->key_min = ws->key_cnt + 1;
     }
@@ -473,8 +473,8 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     if (ws->n_wseg) H2D(wseg_scene, wseg_scene.data(), ws->n_wseg);
     if (ws->n_words) H2D(word_seg, word_seg.data(), ws->n_words);
 #undef H2D
-    const size_t kbytes = sizeof(int32_t) * (size_t)ws->key_span_total;
-    if ((ws->cell_table_ok && (hipMemsetAsync(ws->cell_of, 0xFF, kbytes, s) != hipSuccess ||
+    const size_t kbytes = sizeof(int4) * (size_t)ws->key_span_total;
+    if ((ws->cell_table_ok && (hipMemsetAsync(ws->key_seg, 0xFF, kbytes, s) != hipSuccess ||
                             mwd_fill_key_table(ws->key_cnt, ws->key_span_total, s) != MWD_OK)) ||
         hipMemsetAsync(ws->row_key, 0xFF, sizeof(int32_t) * n_rows, s) != hipSuccess ||
         hipMemsetAsync(ws->counters, 0, sizeof(int64_t) * 20, s) != hipSuccess ||

@@ -155,8 +155,10 @@ __global__ void k_key_count(const double* __restrict__ com, const int64_t* __res
                             const int32_t* __restrict__ scene_of,
                             const SceneDev* __restrict__ scenes, int32_t* __restrict__ row_key,
                             int32_t* __restrict__ key_cnt, int32_t* __restrict__ key_min, uint32_t* flags,
-                            const double* __restrict__ radius, double* __restrict__ sph, WallWordsIn ww) {
+                            const double* __restrict__ radius, double* __restrict__ sph, WallWordsIn ww,
+                            int64_t* __restrict__ epoch) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *epoch += 1;  // a new pass: key segments of earlier passes go stale
     const bool live = i < (*m_live < M_cap ? *m_live : M_cap);
     if (!ww.hit_words && !live) return;
     int32_t row = 0, s = -1;
@@ -220,14 +222,14 @@ __global__ void k_key_count(const double* __restrict__ com, const int64_t* __res
 // so the cell pass sorts its members.
 __global__ void k_key_scatter(const int32_t* __restrict__ plist, int64_t M_cap, const int64_t* m_live,
                               const int32_t* __restrict__ row_key,
-                              int32_t* __restrict__ key_cnt, const int32_t* __restrict__ cell_of,
-                              const int32_t* __restrict__ cell_start, int32_t* __restrict__ members) {
+                              int32_t* __restrict__ key_cnt, const int4* __restrict__ key_seg,
+                              int32_t* __restrict__ members) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (*m_live < M_cap ? *m_live : M_cap)) return;
     const int32_t row = plist[i];
     const int32_t k = row_key[row];
     const int32_t slot = atomicSub(&key_cnt[2 * k], 1) - 1;
-    members[cell_start[cell_of[k]] + slot] = row;
+    members[key_seg[k].y + slot] = row;
 }
 
 // CPython set iteration order of non-negative ints inserted in the given
@@ -310,6 +312,44 @@ __device__ void set_order(const int32_t* rows, int n, int64_t row0, int32_t* out
         if (tab[q] >= 0) out[o++] = (int32_t)(tab[q] + row0);
 }
 
+// set_order of n <= 4 members given in registers (the same table walk).
+__device__ __forceinline__ void set_order4(const int32_t (&rows)[4], int n, int64_t row0, int32_t* out) {
+    uint32_t occ = 0;
+    int slot[4] = {8, 8, 8, 8};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k < n) {
+            const uint64_t key = (uint64_t)(rows[k] - row0);
+            uint64_t perturb = key;
+            uint32_t i = (uint32_t)(key & 7u);
+            while ((occ >> i) & 1u) {
+                perturb >>= 5;
+                i = (uint32_t)((i * 5u + 1u + perturb) & 7u);
+            }
+            occ |= 1u << i;
+            slot[k] = (int)i;
+        }
+    }
+    int o = 0;
+    for (int sl = 0; sl < 8; ++sl) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (slot[k] == sl) out[o++] = rows[k];
+    }
+}
+
+// Scene of a global cell key: the last scene whose key range starts at or
+// below it (scenes are in key order).
+__device__ __forceinline__ int scene_of_key(const SceneDev* __restrict__ scenes, int S, uint32_t key) {
+    int lo = 0, hi = S;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (scenes[mid].kbase <= (int64_t)key) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 __device__ __forceinline__ int64_t find_cell(const uint32_t* cell_key, int64_t C, uint32_t key) {
     int64_t lo = 0, hi = C;
     while (lo < hi) {
@@ -324,57 +364,70 @@ __device__ __forceinline__ int64_t find_cell(const uint32_t* cell_key, int64_t C
 // cell's and its neighbours' (first slot, members) into the rank-ordered
 // tables, and the cell's pair count. Cell c holds member slots
 // [cell_start[c], cell_start[c+1]); `sorted` says whether they are already
-// in ascending row order. nb_of(t) gives the cell of global key t or -1.
-template <class NbOf>
+// in ascending row order. seg_of(t) gives (first slot, members) of the cell
+// of global key t, or (0, 0).
+template <class SegOf>
 __device__ __forceinline__ int32_t cell_work(int64_t c, int64_t r, bool sorted, int32_t* __restrict__ members,
                                              const int32_t* __restrict__ cell_start,
                                              const uint32_t* __restrict__ cell_key,
-                                             const int32_t* __restrict__ scene_of,
-                                             const SceneDev* __restrict__ scenes, int32_t* __restrict__ perm,
-                                             int2* __restrict__ cell_self, int2* __restrict__ cell_seg,
-                                             uint32_t* flags, const NbOf& nb_of) {
+                                             const SceneDev* __restrict__ scenes, int S,
+                                             int32_t* __restrict__ perm, int2* __restrict__ cell_self,
+                                             int2* __restrict__ cell_seg, uint32_t* flags, const SegOf& seg_of) {
     const int32_t start = cell_start[c], end = cell_start[c + 1];
+    const uint32_t key = cell_key[c];
     const int n = end - start;
     int32_t* m = members + start;
-    // every member of a cell lies in one scene (keys are offset per scene)
-    const SceneDev& sc = scenes[scene_of[m[0]]];
-    const int64_t lk = (int64_t)cell_key[c] - sc.kbase + sc.kmin;
-    // neighbour lookups, then their member ranges, then the stores: phases
-    // of independent loads, so their latencies overlap
-    int64_t nb[MWD_MAX_OFF];
+    // the scene from the key (keys are offset per scene), so that the
+    // neighbour lookups wait only for the key
+    const SceneDev& sc = scenes[scene_of_key(scenes, S, key)];
+    const int64_t lk = (int64_t)key - sc.kbase + sc.kmin;
+    int32_t v[4];
 #pragma unroll
-    for (int o = 0; o < MWD_MAX_OFF; ++o) {
-        nb[o] = -1;
-        if (o < sc.n_off) {
-            const int64_t t = lk + sc.off[o] - sc.kmin;
-            if (t >= 0 && t < sc.kspan) nb[o] = nb_of((uint32_t)(sc.kbase + t));
-        }
-    }
+    for (int k = 0; k < 4; ++k) v[k] = k < n ? m[k] : 0;
     int2 seg[MWD_MAX_OFF];
 #pragma unroll
     for (int o = 0; o < MWD_MAX_OFF; ++o) {
         seg[o] = make_int2(0, 0);
-        if (nb[o] >= 0) {
-            seg[o].x = cell_start[nb[o]];
-            seg[o].y = cell_start[nb[o] + 1] - seg[o].x;
+        if (o < sc.n_off) {
+            const int64_t t = lk + sc.off[o] - sc.kmin;
+            if (t >= 0 && t < sc.kspan) seg[o] = seg_of((uint32_t)(sc.kbase + t));
         }
     }
-    if (!sorted) {  // insertion sort; cells hold a few members
-        for (int k = 1; k < n; ++k) {
-            const int32_t x = m[k];
-            int j = k - 1;
-            while (j >= 0 && m[j] > x) {
-                m[j + 1] = m[j];
-                --j;
+    if (n <= 4) {  // members in registers
+        if (!sorted) {
+#pragma unroll
+            for (int k = 1; k < 4; ++k) {
+                if (k < n) {
+                    const int32_t x = v[k];
+                    int j = k - 1;
+                    while (j >= 0 && v[j] > x) {
+                        v[j + 1] = v[j];
+                        --j;
+                    }
+                    v[j + 1] = x;
+                }
             }
-            m[j + 1] = x;
         }
-    }
-    if (n > MWD_MAX_CELL) {
-        atomicOr(flags, MWD_F_CELL);
-        for (int k = 0; k < n; ++k) perm[start + k] = m[k];
+        if (n == 1) perm[start] = v[0];
+        else if (n > 1) set_order4(v, n, sc.row0, perm + start);
     } else {
-        set_order(m, n, sc.row0, perm + start);
+        if (!sorted) {  // insertion sort in place; cells hold a few members
+            for (int k = 1; k < n; ++k) {
+                const int32_t x = m[k];
+                int j = k - 1;
+                while (j >= 0 && m[j] > x) {
+                    m[j + 1] = m[j];
+                    --j;
+                }
+                m[j + 1] = x;
+            }
+        }
+        if (n > MWD_MAX_CELL) {
+            atomicOr(flags, MWD_F_CELL);
+            for (int k = 0; k < n; ++k) perm[start + k] = m[k];
+        } else {
+            set_order(m, n, sc.row0, perm + start);
+        }
     }
     cell_self[r] = make_int2(start, n);
     int64_t cnt = (int64_t)n * (n - 1) / 2;
@@ -386,26 +439,31 @@ __device__ __forceinline__ int32_t cell_work(int64_t c, int64_t r, bool sorted, 
     return cnt < INT32_MAX ? (int32_t)cnt : INT32_MAX;  // beyond only with MWD_F_CELL raised
 }
 
-struct NbSorted {  // binary search of the sorted cell keys
+struct SegSorted {  // binary search of the sorted cell keys
     const uint32_t* cell_key;
+    const int32_t* cell_start;
     int64_t C;
-    __device__ __forceinline__ int64_t operator()(uint32_t key) const { return find_cell(cell_key, C, key); }
+    __device__ __forceinline__ int2 operator()(uint32_t key) const {
+        const int64_t c = find_cell(cell_key, C, key);
+        if (c < 0) return make_int2(0, 0);
+        const int32_t a = cell_start[c];
+        return make_int2(a, cell_start[c + 1] - a);
+    }
 };
 
-struct NbTable {  // key -> cell table, whose stale entries fail the key check
-    const int32_t* cell_of;
-    const uint32_t* cell_key;
-    int64_t C;
-    __device__ __forceinline__ int64_t operator()(uint32_t key) const {
-        const int32_t c = cell_of[key];
-        return (c >= 0 && c < C && cell_key[c] == key) ? c : -1;
+struct SegTable {  // key -> (first slot, members) this pass; other epochs' entries are stale
+    const int4* key_seg;
+    int32_t epoch;
+    __device__ __forceinline__ int2 operator()(uint32_t key) const {
+        const int4 e = key_seg[key];
+        return e.x == epoch ? make_int2(e.y, e.z) : make_int2(0, 0);
     }
 };
 
 // Sorted path: one thread per cell (cells in key order, members sorted).
 __global__ void k_cell_info(int32_t* __restrict__ vals, const int32_t* __restrict__ cell_start,
                             const uint32_t* __restrict__ cell_key, const int64_t* __restrict__ n_cells,
-                            const int32_t* __restrict__ scene_of, const SceneDev* __restrict__ scenes,
+                            const SceneDev* __restrict__ scenes, int S,
                             const int64_t* __restrict__ row_rank, int32_t* __restrict__ perm,
                             int2* __restrict__ cell_self, int2* __restrict__ cell_seg,
                             int32_t* __restrict__ cell_cnt, uint32_t* flags) {
@@ -413,23 +471,22 @@ __global__ void k_cell_info(int32_t* __restrict__ vals, const int32_t* __restric
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C) return;
     const int64_t r = row_rank[vals[cell_start[c]]];
-    cell_cnt[r] = cell_work(c, r, true, vals, cell_start, cell_key, scene_of, scenes, perm, cell_self, cell_seg,
-                            flags, NbSorted{cell_key, C});
+    cell_cnt[r] = cell_work(c, r, true, vals, cell_start, cell_key, scenes, S, perm, cell_self, cell_seg, flags,
+                            SegSorted{cell_key, cell_start, C});
 }
 
 // Key-table path, the per-cell work as a kernel of its own (one cell per
 // thread; the pair-count scan follows), instead of inside the scan's load.
 __global__ void k_cell_info_table(int32_t* __restrict__ members, const int32_t* __restrict__ cell_start,
-                                  const uint32_t* __restrict__ cell_key, const int32_t* __restrict__ cell_of,
-                                  const int64_t* __restrict__ n_cells, const int32_t* __restrict__ scene_of,
-                                  const SceneDev* __restrict__ scenes, int32_t* __restrict__ perm,
-                                  int2* __restrict__ cell_self, int2* __restrict__ cell_seg,
-                                  int32_t* __restrict__ cell_cnt, uint32_t* flags) {
-    const int64_t C = *n_cells;
+                                  const uint32_t* __restrict__ cell_key, const int4* __restrict__ key_seg,
+                                  const int64_t* __restrict__ counters, const SceneDev* __restrict__ scenes, int S,
+                                  int32_t* __restrict__ perm, int2* __restrict__ cell_self,
+                                  int2* __restrict__ cell_seg, int32_t* __restrict__ cell_cnt, uint32_t* flags) {
+    const int64_t C = counters[0];
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C) return;
-    cell_cnt[c] = cell_work(c, c, false, members, cell_start, cell_key, scene_of, scenes, perm, cell_self, cell_seg,
-                            flags, NbTable{cell_of, cell_key, C});
+    cell_cnt[c] = cell_work(c, c, false, members, cell_start, cell_key, scenes, S, perm, cell_self, cell_seg, flags,
+                            SegTable{key_seg, (int32_t)counters[6]});
 }
 
 // Blocks are dispatched to the 8 XCDs in turn (block b to XCD b % 8), and
@@ -785,8 +842,8 @@ struct CellTableIO {
     int32_t* key_min;
     int32_t* cell_start;
     uint32_t* cell_key;
-    int32_t* cell_of;
-    int64_t* n_cells;
+    int4* key_seg;
+    int64_t* n_cells;  // counters + 0; counters[6] holds the pass's epoch
     __device__ __forceinline__ int64_t load(int64_t r) const {
         const int32_t k = row_key[r];
         if (k < 0) return 0;
@@ -798,9 +855,10 @@ struct CellTableIO {
         if (!v) return;
         const int32_t k = row_key[r];
         const int64_t c = off >> 32;
-        cell_start[c] = (int32_t)(off & 0xffffffffll);
+        const int32_t first = (int32_t)(off & 0xffffffffll);
+        cell_start[c] = first;
         cell_key[c] = (uint32_t)k;
-        cell_of[k] = (int32_t)c;
+        key_seg[k] = make_int4((int32_t)n_cells[6], first, (int32_t)(v & 0xffffffffll), 0);
         key_min[2 * (int64_t)k] = KEY_MIN_EMPTY;
     }
     __device__ __forceinline__ void finish(int64_t total) const {
@@ -843,8 +901,8 @@ struct CellInfoIO {  // key-table path: the per-cell work runs as the scan's loa
     int32_t* members;
     const int32_t* cell_start;
     const uint32_t* cell_key;
-    const int32_t* cell_of;
-    const int32_t* scene_of;
+    const int4* key_seg;
+    int S;
     const SceneDev* scenes;
     int32_t* perm;
     int2* cell_self;
@@ -855,8 +913,8 @@ struct CellInfoIO {  // key-table path: the per-cell work runs as the scan's loa
     uint32_t* flags;
     bool folded;
     __device__ __forceinline__ int32_t load(int64_t c) const {
-        return cell_work(c, c, false, members, cell_start, cell_key, scene_of, scenes, perm, cell_self, cell_seg,
-                         flags, NbTable{cell_of, cell_key, counters[0]});
+        return cell_work(c, c, false, members, cell_start, cell_key, scenes, S, perm, cell_self, cell_seg, flags,
+                         SegTable{key_seg, (int32_t)counters[6]});
     }
     __device__ __forceinline__ void store(int64_t i, int64_t o, int32_t) const { off[i] = o; }
     __device__ __forceinline__ void finish(int64_t total) const { close_pairs(counters, total, cap, flags, folded); }
@@ -959,11 +1017,11 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
                                  words ? ws->hit_words : nullptr, ws->gjk_words};
             hipLaunchKernelGGL(k_key_count, dim3(blocks_for(M)), dim3(256), 0, s, st->com, keys_in, ws->plist, M,
                                ws->m_live, ws->scene_of, ws->scenes_dev, ws->row_key, ws->key_cnt, ws->key_min,
-                               ws->flags, st->radius, st->radius ? ws->sph : nullptr, ww);
+                               ws->flags, st->radius, st->radius ? ws->sph : nullptr, ww, ctr + 6);
             MWD_LAUNCH_CHECK();
             mwd_mark(ws, 2);
             const CellTableIO ct{ws->row_key, ws->key_cnt, ws->key_min, ws->cell_start, ws->cell_key,
-                                 ws->cell_of, ctr + 0};
+                                 ws->key_seg, ctr + 0};
             if (words) {
                 const WallWordIO wio{ws->hit_words, ws->gjk_words, ws->wseg_off, ws->wseg_scene, ws->word_seg,
                                      ws->scenes_dev, wout};
@@ -975,16 +1033,16 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
             if (rc) return rc;
             hipLaunchKernelGGL(k_key_scatter, dim3(blocks_for(M)), dim3(256), 0, s, ws->plist, M, ws->m_live,
                                ws->row_key,
-                               ws->key_cnt, ws->cell_of, ws->cell_start, ws->vals);
+                               ws->key_cnt, ws->key_seg, ws->vals);
             MWD_LAUNCH_CHECK();
-            const CellInfoIO io{ws->vals,      ws->cell_start, ws->cell_key,  ws->cell_of, ws->scene_of,
+            const CellInfoIO io{ws->vals,      ws->cell_start, ws->cell_key,  ws->key_seg, ws->n_scenes,
                                 ws->scenes_dev, ws->perm,      ws->cell_self, ws->cell_seg, ws->cell_off,
                                 ctr,           ws->pair_cap,   ws->flags,     ws->sphere_folded != 0};
             switch (ws->tune.cell_items) {
                 case 0:  // the per-cell work as its own kernel, then the pair-count scan
                     hipLaunchKernelGGL(k_cell_info_table, dim3(blocks_for(M)), dim3(256), 0, s, ws->vals,
-                                       ws->cell_start, ws->cell_key, ws->cell_of, ctr + 0, ws->scene_of,
-                                       ws->scenes_dev, ws->perm, ws->cell_self, ws->cell_seg, ws->cell_cnt,
+                                       ws->cell_start, ws->cell_key, ws->key_seg, ctr, ws->scenes_dev,
+                                       ws->n_scenes, ws->perm, ws->cell_self, ws->cell_seg, ws->cell_cnt,
                                        ws->flags);
                     MWD_LAUNCH_CHECK();
                     rc = mwd_scan_io(CellCountIO{ws->cell_cnt, ws->cell_off, ctr, ws->pair_cap, ws->flags,
@@ -1022,7 +1080,7 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
                               ws->scan_tmp_len, s);
             if (rc) return rc;
             hipLaunchKernelGGL(k_cell_info, dim3(blocks_for(M)), dim3(256), 0, s, vs, ws->cell_start,
-                               ws->cell_key, ctr + 0, ws->scene_of, ws->scenes_dev, ws->row_rank, ws->perm,
+                               ws->cell_key, ctr + 0, ws->scenes_dev, ws->n_scenes, ws->row_rank, ws->perm,
                                ws->cell_self, ws->cell_seg, ws->cell_cnt, ws->flags);
             MWD_LAUNCH_CHECK();
             rc = mwd_scan_io(CellCountIO{ws->cell_cnt, ws->cell_off, ctr, ws->pair_cap, ws->flags,

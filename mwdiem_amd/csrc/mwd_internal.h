@@ -261,7 +261,8 @@ struct mwd_workspace {
     int sphere_folded;         // host: the last candidate pass also did the bounding-sphere test
     int cell_table_ok;         // its tables exist
     PairTune tune;
-    int32_t* cell_of;          // [key_span_total] cell of a key; valid only where cell_key[cell_of[k]] == k
+    int4* key_seg;             // [key_span_total] per key this pass: (epoch, first member slot, members, 0);
+                               // an entry whose epoch is not the pass's (counters[6]) has no cell
     int32_t* key_cnt;          // [2 * key_span_total] per key k: members at 2k (zero between steps) and
     int32_t* key_min;          // lowest row at 2k + 1 (KEY_MIN_EMPTY between steps); key_min = key_cnt + 1
     int32_t* row_key;          // [n_rows] key of each moving row this step, -1 for walls
@@ -320,7 +321,7 @@ struct mwd_workspace {
     int64_t* scan_tmp_w;
     int64_t scan_tmp_w_len;
     // counters (device): 0 n_cells, 1 n_wall_pairs, 2 n_cell_pairs, 3 n_pairs
-    // (capped), 4 n_contacts, 5 n_records, 6 unused,
+    // (capped), 4 n_contacts, 5 n_records, 6 epoch of the key-table cell pass,
     // 7 n_pairs before capping, 8 simplex slots claimed this step, 9 radix
     // histogram entries (scratch), 10 sphere-test survivors, 11 bucket
     // overflow records, 12-13 slots filled in the tangential history tables,
