@@ -45,7 +45,7 @@ int dalloc(T** p, int64_t count) {
 
 void free_ws(mwd_workspace* ws) {
     if (!ws) return;
-    void* ptrs[] = {ws->scenes_dev, ws->key_seg, ws->plist, ws->wlist, ws->scene_of, ws->is_wall, ws->keys, ws->keys_alt, ws->vals,
+    void* ptrs[] = {ws->scenes_dev, ws->key_seg, ws->key_small, ws->key_head, ws->row_next, ws->plist, ws->wlist, ws->scene_of, ws->is_wall, ws->keys, ws->keys_alt, ws->vals,
                     ws->vals_alt, ws->cell_start, ws->cell_key, ws->cell_self, ws->cell_seg,
                     ws->row_first, ws->row_rank, ws->cell_cnt, ws->cell_off, ws->perm, ws->key_cnt,
                     ws->row_key, ws->sph,
@@ -415,6 +415,9 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     ws->cell_table = ws->cell_table_ok;
     if (ws->cell_table_ok) {
         A(key_seg, ws->key_span_total);
+        A(key_small, ws->key_span_total);
+        A(key_head, ws->key_span_total);
+        A(row_next, n_rows);
         A(key_cnt, 2 * ws->key_span_total);  // (count, lowest row) interleaved: one cache line per key
         ws->key_min = ws->key_cnt + 1;
     }
@@ -475,6 +478,7 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
 #undef H2D
     const size_t kbytes = sizeof(int4) * (size_t)ws->key_span_total;
     if ((ws->cell_table_ok && (hipMemsetAsync(ws->key_seg, 0xFF, kbytes, s) != hipSuccess ||
+                            hipMemsetAsync(ws->key_head, 0xFF, sizeof(int32_t) * (size_t)ws->key_span_total, s) != hipSuccess ||
                             mwd_fill_key_table(ws->key_cnt, ws->key_span_total, s) != MWD_OK)) ||
         hipMemsetAsync(ws->row_key, 0xFF, sizeof(int32_t) * n_rows, s) != hipSuccess ||
         hipMemsetAsync(ws->counters, 0, sizeof(int64_t) * 20, s) != hipSuccess ||

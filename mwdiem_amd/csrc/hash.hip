@@ -19,12 +19,14 @@
 // order of their lowest row) that the emission reads:
 //   key table (default while the summed key span is at most 2^28): a
 //     counting sort whose single digit is the whole key. Per particle, count
-//     the key's members and keep its lowest row (atomics on a key-indexed
-//     table); one scan over rows numbers the cells by rank and places their
-//     member ranges (a packed two-count scan); per particle, claim a slot in
-//     its cell; per cell (fused into the scan of pair counts), sort the few
-//     members, put them in set order, look up the neighbour cells in the
-//     table and count pairs. The tables return to their empty state as they
+//     the key's members, keep its lowest row and file the row under the key
+//     (atomics on key-indexed tables: the first four members side by side,
+//     more in a list); one scan over rows numbers the cells by rank and
+//     places their member ranges (a packed two-count scan), stamping each
+//     key's (first slot, members) with the pass's epoch; per cell (fused
+//     into the scan of pair counts), gather and sort the few members, put
+//     them in set order, look up the neighbour cells (one stamped entry
+//     each) and count pairs. The tables return to their empty state as they
 //     are used, so nothing is cleared between steps.
 //   sorted keys: stable radix sort of (key, row) -> cell heads (scan) ->
 //     cell rank by lowest row (scan over rows) -> per-cell set order,
@@ -130,7 +132,10 @@ __device__ __forceinline__ int64_t wall_pair_bits(d3 x, const double* __restrict
 }
 
 // Key-table path, one thread per moving particle: its key, the member count
-// of that key and the key's lowest row. With wall words, also the particle's
+// of that key and the key's lowest row; the key's first four members go to
+// its 16-byte entry of key_small (in arrival order), any more to a list
+// (key_head, row_next), so that the cell pass finds a cell's members without
+// a scatter of rows into slots. With wall words, also the particle's
 // slab tests against every wall of its scene, gathered by ballots into the
 // 64-bit words of the tests (a wave is 64 consecutive list entries, so its
 // ballot for one scene and wall is that segment's word; a scan over the
@@ -156,7 +161,8 @@ __global__ void k_key_count(const double* __restrict__ com, const int64_t* __res
                             const SceneDev* __restrict__ scenes, int32_t* __restrict__ row_key,
                             int32_t* __restrict__ key_cnt, int32_t* __restrict__ key_min, uint32_t* flags,
                             const double* __restrict__ radius, double* __restrict__ sph, WallWordsIn ww,
-                            int64_t* __restrict__ epoch) {
+                            int64_t* __restrict__ epoch, int32_t* __restrict__ key_small,
+                            int32_t* __restrict__ key_head, int32_t* __restrict__ row_next) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) *epoch += 1;  // a new pass: key segments of earlier passes go stale
     const bool live = i < (*m_live < M_cap ? *m_live : M_cap);
@@ -169,8 +175,11 @@ __global__ void k_key_count(const double* __restrict__ com, const int64_t* __res
         if (sph) pack_sphere(sph, com, radius, row);
         const uint32_t k = global_key(com, keys_in, row, sc, flags);
         row_key[row] = (int32_t)k;
-        atomicAdd(&key_cnt[2 * k], 1);
+        const int slot = atomicAdd(&key_cnt[2 * k], 1);
         atomicMin(&key_min[2 * k], row);
+        // the key's first four members side by side, the others in a list
+        if (slot < 4) key_small[4 * (int64_t)k + slot] = row;
+        else row_next[row] = atomicExch(&key_head[k], row);
     }
     if (!ww.hit_words) return;
     // the particle's tests against its scene's walls (at most MWD_WORD_WALLS):
@@ -215,21 +224,6 @@ __global__ void k_key_count(const double* __restrict__ com, const int64_t* __res
         }
         todo &= ~__ballot(in);
     }
-}
-
-// Key-table path: each particle takes a slot in its cell's member range. The
-// decrements bring the key counts back to zero; slots come in no set order,
-// so the cell pass sorts its members.
-__global__ void k_key_scatter(const int32_t* __restrict__ plist, int64_t M_cap, const int64_t* m_live,
-                              const int32_t* __restrict__ row_key,
-                              int32_t* __restrict__ key_cnt, const int4* __restrict__ key_seg,
-                              int32_t* __restrict__ members) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (*m_live < M_cap ? *m_live : M_cap)) return;
-    const int32_t row = plist[i];
-    const int32_t k = row_key[row];
-    const int32_t slot = atomicSub(&key_cnt[2 * k], 1) - 1;
-    members[key_seg[k].y + slot] = row;
 }
 
 // CPython set iteration order of non-negative ints inserted in the given
@@ -366,9 +360,19 @@ __device__ __forceinline__ int64_t find_cell(const uint32_t* cell_key, int64_t C
 // [cell_start[c], cell_start[c+1]); `sorted` says whether they are already
 // in ascending row order. seg_of(t) gives (first slot, members) of the cell
 // of global key t, or (0, 0).
+// Where the key-table path keeps a key's members (k_key_count): the first
+// four in key_small, the rest in a list. The cell pass empties both (the
+// count returns to zero, the list head to -1) for the next pass.
+struct KeyLists {
+    const int4* key_small;
+    int32_t* key_head;
+    const int32_t* row_next;
+    int32_t* key_cnt;
+};
+
 template <class SegOf>
 __device__ __forceinline__ int32_t cell_work(int64_t c, int64_t r, bool sorted, int32_t* __restrict__ members,
-                                             const int32_t* __restrict__ cell_start,
+                                             const KeyLists* kl, const int32_t* __restrict__ cell_start,
                                              const uint32_t* __restrict__ cell_key,
                                              const SceneDev* __restrict__ scenes, int S,
                                              int32_t* __restrict__ perm, int2* __restrict__ cell_self,
@@ -382,8 +386,27 @@ __device__ __forceinline__ int32_t cell_work(int64_t c, int64_t r, bool sorted, 
     const SceneDev& sc = scenes[scene_of_key(scenes, S, key)];
     const int64_t lk = (int64_t)key - sc.kbase + sc.kmin;
     int32_t v[4];
+    if (kl) {  // key-table path: the members from the key's entry and list
+        const int4 sm = kl->key_small[key];
+        v[0] = sm.x;
+        v[1] = n > 1 ? sm.y : 0;
+        v[2] = n > 2 ? sm.z : 0;
+        v[3] = n > 3 ? sm.w : 0;
+        kl->key_cnt[2 * (int64_t)key] = 0;
+        if (n > 4) {  // the cell's slots take all of them, the list after the four
 #pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] = k < n ? m[k] : 0;
+            for (int k = 0; k < 4; ++k) m[k] = v[k];
+            int32_t row = kl->key_head[key];
+            for (int k = 4; k < n; ++k) {
+                m[k] = row;
+                row = kl->row_next[row];
+            }
+            kl->key_head[key] = -1;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = k < n ? m[k] : 0;
+    }
     int2 seg[MWD_MAX_OFF];
 #pragma unroll
     for (int o = 0; o < MWD_MAX_OFF; ++o) {
@@ -471,13 +494,13 @@ __global__ void k_cell_info(int32_t* __restrict__ vals, const int32_t* __restric
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C) return;
     const int64_t r = row_rank[vals[cell_start[c]]];
-    cell_cnt[r] = cell_work(c, r, true, vals, cell_start, cell_key, scenes, S, perm, cell_self, cell_seg, flags,
-                            SegSorted{cell_key, cell_start, C});
+    cell_cnt[r] = cell_work(c, r, true, vals, nullptr, cell_start, cell_key, scenes, S, perm, cell_self, cell_seg,
+                            flags, SegSorted{cell_key, cell_start, C});
 }
 
 // Key-table path, the per-cell work as a kernel of its own (one cell per
 // thread; the pair-count scan follows), instead of inside the scan's load.
-__global__ void k_cell_info_table(int32_t* __restrict__ members, const int32_t* __restrict__ cell_start,
+__global__ void k_cell_info_table(int32_t* __restrict__ members, KeyLists kl, const int32_t* __restrict__ cell_start,
                                   const uint32_t* __restrict__ cell_key, const int4* __restrict__ key_seg,
                                   const int64_t* __restrict__ counters, const SceneDev* __restrict__ scenes, int S,
                                   int32_t* __restrict__ perm, int2* __restrict__ cell_self,
@@ -485,8 +508,8 @@ __global__ void k_cell_info_table(int32_t* __restrict__ members, const int32_t* 
     const int64_t C = counters[0];
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C) return;
-    cell_cnt[c] = cell_work(c, c, false, members, cell_start, cell_key, scenes, S, perm, cell_self, cell_seg, flags,
-                            SegTable{key_seg, (int32_t)counters[6]});
+    cell_cnt[c] = cell_work(c, c, false, members, &kl, cell_start, cell_key, scenes, S, perm, cell_self, cell_seg,
+                            flags, SegTable{key_seg, (int32_t)counters[6]});
 }
 
 // Blocks are dispatched to the 8 XCDs in turn (block b to XCD b % 8), and
@@ -899,6 +922,7 @@ struct CellCountIO {  // sorted path: counts written by k_cell_info
 
 struct CellInfoIO {  // key-table path: the per-cell work runs as the scan's load
     int32_t* members;
+    KeyLists kl;
     const int32_t* cell_start;
     const uint32_t* cell_key;
     const int4* key_seg;
@@ -913,8 +937,8 @@ struct CellInfoIO {  // key-table path: the per-cell work runs as the scan's loa
     uint32_t* flags;
     bool folded;
     __device__ __forceinline__ int32_t load(int64_t c) const {
-        return cell_work(c, c, false, members, cell_start, cell_key, scenes, S, perm, cell_self, cell_seg, flags,
-                         SegTable{key_seg, (int32_t)counters[6]});
+        return cell_work(c, c, false, members, &kl, cell_start, cell_key, scenes, S, perm, cell_self, cell_seg,
+                         flags, SegTable{key_seg, (int32_t)counters[6]});
     }
     __device__ __forceinline__ void store(int64_t i, int64_t o, int32_t) const { off[i] = o; }
     __device__ __forceinline__ void finish(int64_t total) const { close_pairs(counters, total, cap, flags, folded); }
@@ -1017,7 +1041,8 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
                                  words ? ws->hit_words : nullptr, ws->gjk_words};
             hipLaunchKernelGGL(k_key_count, dim3(blocks_for(M)), dim3(256), 0, s, st->com, keys_in, ws->plist, M,
                                ws->m_live, ws->scene_of, ws->scenes_dev, ws->row_key, ws->key_cnt, ws->key_min,
-                               ws->flags, st->radius, st->radius ? ws->sph : nullptr, ww, ctr + 6);
+                               ws->flags, st->radius, st->radius ? ws->sph : nullptr, ww, ctr + 6,
+                               reinterpret_cast<int32_t*>(ws->key_small), ws->key_head, ws->row_next);
             MWD_LAUNCH_CHECK();
             mwd_mark(ws, 2);
             const CellTableIO ct{ws->row_key, ws->key_cnt, ws->key_min, ws->cell_start, ws->cell_key,
@@ -1031,16 +1056,13 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
                 rc = mwd_scan_io(ct, ws->n_rows, nullptr, nullptr, ws->scan_tmp, ws->scan_tmp_len, s);
             }
             if (rc) return rc;
-            hipLaunchKernelGGL(k_key_scatter, dim3(blocks_for(M)), dim3(256), 0, s, ws->plist, M, ws->m_live,
-                               ws->row_key,
-                               ws->key_cnt, ws->key_seg, ws->vals);
-            MWD_LAUNCH_CHECK();
-            const CellInfoIO io{ws->vals,      ws->cell_start, ws->cell_key,  ws->key_seg, ws->n_scenes,
+            const KeyLists kl{ws->key_small, ws->key_head, ws->row_next, ws->key_cnt};
+            const CellInfoIO io{ws->vals,      kl,             ws->cell_start, ws->cell_key,  ws->key_seg, ws->n_scenes,
                                 ws->scenes_dev, ws->perm,      ws->cell_self, ws->cell_seg, ws->cell_off,
                                 ctr,           ws->pair_cap,   ws->flags,     ws->sphere_folded != 0};
             switch (ws->tune.cell_items) {
                 case 0:  // the per-cell work as its own kernel, then the pair-count scan
-                    hipLaunchKernelGGL(k_cell_info_table, dim3(blocks_for(M)), dim3(256), 0, s, ws->vals,
+                    hipLaunchKernelGGL(k_cell_info_table, dim3(blocks_for(M)), dim3(256), 0, s, ws->vals, kl,
                                        ws->cell_start, ws->cell_key, ws->key_seg, ctr, ws->scenes_dev,
                                        ws->n_scenes, ws->perm, ws->cell_self, ws->cell_seg, ws->cell_cnt,
                                        ws->flags);
