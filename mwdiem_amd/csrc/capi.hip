@@ -53,7 +53,7 @@ void free_ws(mwd_workspace* ws) {
                     ws->contact_pair,
                     ws->depth, ws->normal, ws->cpA, ws->cpB, ws->Fn, ws->tA, ws->tB, ws->epa_status, ws->rec_cnt,
                     ws->rec_bkt, ws->rec_ovf, ws->hist,
-                    ws->hist_off, ws->scan_tmp, ws->scan_tmp_w, ws->wseg_off, ws->wseg_scene, ws->word_seg, ws->hit_words, ws->gjk_words, ws->counters, ws->flags, ws->laws,
+                    ws->hist_off, ws->scan_tmp, ws->scan_tmp_w, ws->wseg_off, ws->word_desc, ws->hit_words, ws->gjk_words, ws->counters, ws->flags, ws->laws,
                     ws->scene_wt_off, ws->ctan, ws->law_tans, ws->adj_F, ws->adj_T, ws->adj_lamF,
                     ws->adj_lamT, ws->adj_face, ws->adj_c, ws->adj_cth, ws->adj_rowth};
     for (void* p : ptrs)
@@ -368,9 +368,19 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     }
     ws->n_wseg = (int64_t)wseg_scene.size();
     ws->n_words = wseg_off.back();
-    std::vector<int32_t> word_seg((size_t)std::max<int64_t>(ws->n_words, 1), 0);
-    for (int64_t g = 0; g < ws->n_wseg; ++g)
-        for (int64_t w = wseg_off[g]; w < wseg_off[g + 1]; ++w) word_seg[w] = (int32_t)g;
+    // per word: its scene, wall row, the row of its bit 0 and that bit's
+    // entry in the scene's list (negative in a first word shared with the
+    // scene before)
+    std::vector<int4> word_desc((size_t)std::max<int64_t>(ws->n_words, 1), make_int4(0, 0, 0, 0));
+    for (int64_t g = 0; g < ws->n_wseg; ++g) {
+        const int s = wseg_scene[g];
+        const int64_t k = g - sd[s].wseg0;
+        for (int64_t w = wseg_off[g]; w < wseg_off[g + 1]; ++w) {
+            const int64_t e0 = ((sd[s].plist_off >> 6) + (w - wseg_off[g])) * 64 - sd[s].plist_off;
+            word_desc[w] = make_int4(s, (int32_t)(scenes[s].row0 + k), (int32_t)(scenes[s].row0 + scenes[s].n_walls + e0),
+                                     (int32_t)e0);
+        }
+    }
     ws->word_walls_ok = 1;
     for (int s = 0; s < n_scenes; ++s)
         if (scenes[s].n_walls > MWD_WORD_WALLS) ws->word_walls_ok = 0;
@@ -448,8 +458,7 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     ws->scan_tmp_w_len = mwd_scan_tmp_len(std::max<int64_t>(ws->n_words, 1));
     A(scan_tmp_w, ws->scan_tmp_w_len);
     A(wseg_off, ws->n_wseg + 1);
-    A(wseg_scene, std::max<int64_t>(ws->n_wseg, 1));
-    A(word_seg, std::max<int64_t>(ws->n_words, 1));
+    A(word_desc, std::max<int64_t>(ws->n_words, 1));
     A(hit_words, std::max<int64_t>(ws->n_words, 1));
     A(gjk_words, std::max<int64_t>(ws->n_words, 1));
     A(counters, 20);
@@ -473,8 +482,7 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     H2D(laws, laws.data(), 2 * n_scenes);
     H2D(scene_wt_off, wt.data(), n_scenes + 1);
     H2D(wseg_off, wseg_off.data(), ws->n_wseg + 1);
-    if (ws->n_wseg) H2D(wseg_scene, wseg_scene.data(), ws->n_wseg);
-    if (ws->n_words) H2D(word_seg, word_seg.data(), ws->n_words);
+    if (ws->n_words) H2D(word_desc, word_desc.data(), ws->n_words);
 #undef H2D
     const size_t kbytes = sizeof(int4) * (size_t)ws->key_span_total;
     if ((ws->cell_table_ok && (hipMemsetAsync(ws->key_seg, 0xFF, kbytes, s) != hipSuccess ||

@@ -777,31 +777,24 @@ struct WallIO {
 struct WallWordIO {
     const uint64_t* hit_words;
     const uint64_t* gjk_words;
-    const int64_t* wseg_off;
-    const int32_t* wseg_scene;
-    const int32_t* word_seg;
+    const int4* word_desc;  // per word: scene, wall row, row of bit 0, list entry of bit 0 within the scene
     const SceneDev* scenes;
     WallOut out;
-    // segment of word w, its scene, and the list entry of the word's bit 0;
-    // the bits that lie in the scene's live entries
-    __device__ __forceinline__ uint64_t locate(int64_t w, int64_t* seg, const SceneDev** sc, int64_t* e0) const {
-        const int64_t lo = word_seg[w];
-        *seg = lo;
-        *sc = &scenes[wseg_scene[lo]];
-        const int64_t first = (*sc)->plist_off, end = first + (*sc)->n_active - (*sc)->n_walls;
-        *e0 = ((first >> 6) + (w - wseg_off[lo])) << 6;
+    // the bits of a word (description d) that lie in its scene's live entries
+    __device__ __forceinline__ uint64_t live_bits(const int4& d) const {
+        const SceneDev& sc = scenes[d.x];
+        const int64_t np = sc.n_active - sc.n_walls;
         uint64_t m = ~0ull;
-        if (*e0 < first) m &= ~0ull << (first - *e0);
-        if (end <= *e0) m = 0;
-        else if (end - *e0 < 64) m &= ~0ull >> (64 - (end - *e0));
+        if (d.w < 0) m &= ~0ull << (-d.w);
+        if (np <= d.w) m = 0;
+        else if (np - d.w < 64) m &= ~0ull >> (64 - (np - d.w));
         return m;
     }
     __device__ __forceinline__ int64_t load(int64_t w) const {
         uint64_t h = hit_words[w], g = gjk_words[w];
+        const int4 d = word_desc[w];
         if (!(h | g)) return 0;
-        int64_t seg, e0;
-        const SceneDev* sc;
-        const uint64_t m = locate(w, &seg, &sc, &e0);
+        const uint64_t m = live_bits(d);
         h &= m;
         g &= m;
         return ((int64_t)__popcll(g) << 32) | (int64_t)__popcll(h);
@@ -811,19 +804,16 @@ struct WallWordIO {
     // few pairs, at most 64).
     __device__ __forceinline__ void store(int64_t w, int64_t off, int64_t v) const {
         if (!v) return;
-        int64_t seg, e0;
-        const SceneDev* sc;
-        const uint64_t m = locate(w, &seg, &sc, &e0);
+        const int4 d = word_desc[w];
+        const uint64_t m = live_bits(d);
         uint64_t h = hit_words[w] & m;
         const uint64_t g = gjk_words[w] & m;
-        const int64_t rk = sc->row0 + (seg - sc->wseg0);
-        const int64_t rj0 = sc->row0 + sc->n_walls - sc->plist_off + e0;  // row of list entry e0
         int64_t o = off;
         while (h) {
             const int b = __ffsll((long long)h) - 1;
             h &= h - 1;
             const int64_t gb = (int64_t)((g >> b) & 1u) << 32;
-            out.put(rj0 + b, rk, o, gb | 1);
+            out.put((int64_t)d.z + b, d.y, o, gb | 1);
             o += gb + 1;
         }
     }
@@ -1048,8 +1038,7 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
             const CellTableIO ct{ws->row_key, ws->key_cnt, ws->key_min, ws->cell_start, ws->cell_key,
                                  ws->key_seg, ctr + 0};
             if (words) {
-                const WallWordIO wio{ws->hit_words, ws->gjk_words, ws->wseg_off, ws->wseg_scene, ws->word_seg,
-                                     ws->scenes_dev, wout};
+                const WallWordIO wio{ws->hit_words, ws->gjk_words, ws->word_desc, ws->scenes_dev, wout};
                 rc = scan_cells_and_walls(ct, ws->n_rows, ws->scan_tmp, ws->scan_tmp_len, wio, ws->n_words,
                                           ws->scan_tmp_w, ws->scan_tmp_w_len, s);
             } else {

@@ -316,8 +316,7 @@ struct mwd_workspace {
     int64_t n_wseg;            // segments (sum over scenes of their walls)
     int64_t n_words;           // words over all segments
     int64_t* wseg_off;         // [n_wseg + 1] first word of each segment
-    int32_t* wseg_scene;       // [n_wseg] scene of each segment
-    int32_t* word_seg;         // [n_words] segment of each word
+    int4* word_desc;           // [n_words] per word: scene, wall row, row of bit 0, entry of bit 0 in the scene
     int word_walls_ok;         // no scene has more than MWD_WORD_WALLS walls
     uint64_t* hit_words;       // [n_words] bit set: the pair is listed
     uint64_t* gjk_words;       // [n_words] bit set: the listed pair goes to GJK
