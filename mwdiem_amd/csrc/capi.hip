@@ -368,17 +368,22 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     }
     ws->n_wseg = (int64_t)wseg_scene.size();
     ws->n_words = wseg_off.back();
-    // per word: its scene, wall row, the row of its bit 0 and that bit's
-    // entry in the scene's list (negative in a first word shared with the
-    // scene before)
+    // per word: its wall row, the row of its bit 0, its index among the words
+    // of the moving-entry list and the bits that are its scene's entries
+    // (first | end << 8; a scene's first and last words share entries with
+    // its neighbours)
     std::vector<int4> word_desc((size_t)std::max<int64_t>(ws->n_words, 1), make_int4(0, 0, 0, 0));
     for (int64_t g = 0; g < ws->n_wseg; ++g) {
         const int s = wseg_scene[g];
         const int64_t k = g - sd[s].wseg0;
+        const int64_t first = sd[s].plist_off, end = first + scenes[s].n_active - scenes[s].n_walls;
         for (int64_t w = wseg_off[g]; w < wseg_off[g + 1]; ++w) {
-            const int64_t e0 = ((sd[s].plist_off >> 6) + (w - wseg_off[g])) * 64 - sd[s].plist_off;
-            word_desc[w] = make_int4(s, (int32_t)(scenes[s].row0 + k), (int32_t)(scenes[s].row0 + scenes[s].n_walls + e0),
-                                     (int32_t)e0);
+            const int64_t wg = (first >> 6) + (w - wseg_off[g]);  // the word in the list's numbering
+            const int64_t e0 = wg * 64;
+            const int64_t b0 = first > e0 ? first - e0 : 0, b1 = end - e0 < 64 ? end - e0 : 64;
+            word_desc[w] = make_int4((int32_t)(scenes[s].row0 + k),
+                                     (int32_t)(scenes[s].row0 + scenes[s].n_walls + e0 - first), (int32_t)wg,
+                                     (int32_t)(b0 | (b1 << 8)));
         }
     }
     ws->word_walls_ok = 1;

@@ -777,24 +777,25 @@ struct WallIO {
 struct WallWordIO {
     const uint64_t* hit_words;
     const uint64_t* gjk_words;
-    const int4* word_desc;  // per word: scene, wall row, row of bit 0, list entry of bit 0 within the scene
-    const SceneDev* scenes;
+    // per word: wall row, row of bit 0, the word's index in the list of
+    // moving entries (bit b: entry 64 * index + b), and the bits that are
+    // the scene's entries (first | end << 8)
+    const int4* word_desc;
+    const int64_t* m_live;  // live moving entries (rows not yet spawned lie beyond)
     WallOut out;
-    // the bits of a word (description d) that lie in its scene's live entries
-    __device__ __forceinline__ uint64_t live_bits(const int4& d) const {
-        const SceneDev& sc = scenes[d.x];
-        const int64_t np = sc.n_active - sc.n_walls;
-        uint64_t m = ~0ull;
-        if (d.w < 0) m &= ~0ull << (-d.w);
-        if (np <= d.w) m = 0;
-        else if (np - d.w < 64) m &= ~0ull >> (64 - (np - d.w));
+    __device__ __forceinline__ uint64_t bits_of(const int4& d, int64_t live) const {
+        const int first = d.w & 0xff, end = d.w >> 8;
+        uint64_t m = (end >= 64 ? ~0ull : ((1ull << end) - 1)) & (~0ull << first);
+        const int64_t e0 = (int64_t)d.z << 6;
+        if (live <= e0) m = 0;
+        else if (live - e0 < 64) m &= ~0ull >> (64 - (live - e0));
         return m;
     }
     __device__ __forceinline__ int64_t load(int64_t w) const {
         uint64_t h = hit_words[w], g = gjk_words[w];
         const int4 d = word_desc[w];
-        if (!(h | g)) return 0;
-        const uint64_t m = live_bits(d);
+        const int64_t live = *m_live;
+        const uint64_t m = bits_of(d, live);
         h &= m;
         g &= m;
         return ((int64_t)__popcll(g) << 32) | (int64_t)__popcll(h);
@@ -805,7 +806,7 @@ struct WallWordIO {
     __device__ __forceinline__ void store(int64_t w, int64_t off, int64_t v) const {
         if (!v) return;
         const int4 d = word_desc[w];
-        const uint64_t m = live_bits(d);
+        const uint64_t m = bits_of(d, *m_live);
         uint64_t h = hit_words[w] & m;
         const uint64_t g = gjk_words[w] & m;
         int64_t o = off;
@@ -813,7 +814,7 @@ struct WallWordIO {
             const int b = __ffsll((long long)h) - 1;
             h &= h - 1;
             const int64_t gb = (int64_t)((g >> b) & 1u) << 32;
-            out.put((int64_t)d.z + b, d.y, o, gb | 1);
+            out.put((int64_t)d.y + b, d.x, o, gb | 1);
             o += gb + 1;
         }
     }
@@ -1038,7 +1039,7 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
             const CellTableIO ct{ws->row_key, ws->key_cnt, ws->key_min, ws->cell_start, ws->cell_key,
                                  ws->key_seg, ctr + 0};
             if (words) {
-                const WallWordIO wio{ws->hit_words, ws->gjk_words, ws->word_desc, ws->scenes_dev, wout};
+                const WallWordIO wio{ws->hit_words, ws->gjk_words, ws->word_desc, ws->m_live, wout};
                 rc = scan_cells_and_walls(ct, ws->n_rows, ws->scan_tmp, ws->scan_tmp_len, wio, ws->n_words,
                                           ws->scan_tmp_w, ws->scan_tmp_w_len, s);
             } else {
