@@ -20,7 +20,7 @@
 //   key table (default while the summed key span is at most 2^28): a
 //     counting sort whose single digit is the whole key. Per particle, count
 //     the key's members, keep its lowest row and file the row under the key
-//     (atomics on key-indexed tables: the first four members side by side,
+//     (atomics on key-indexed tables: the first eight members side by side,
 //     more in a list); one scan over rows numbers the cells by rank and
 //     places their member ranges (a packed two-count scan), stamping each
 //     key's (first slot, members) with the pass's epoch; per cell (fused
@@ -132,8 +132,8 @@ __device__ __forceinline__ int64_t wall_pair_bits(d3 x, const double* __restrict
 }
 
 // Key-table path, one thread per moving particle: its key, the member count
-// of that key and the key's lowest row; the key's first four members go to
-// its 16-byte entry of key_small (in arrival order), any more to a list
+// of that key and the key's lowest row; the key's first eight members go to
+// its 32-byte entry of key_small (in arrival order), any more to a list
 // (key_head, row_next), so that the cell pass finds a cell's members without
 // a scatter of rows into slots. With wall words, also the particle's
 // slab tests against every wall of its scene, gathered by ballots into the
@@ -141,6 +141,7 @@ __device__ __forceinline__ int64_t wall_pair_bits(d3 x, const double* __restrict
 // ballot for one scene and wall is that segment's word; a scan over the
 // words then places the pairs in the reference's order).
 constexpr int32_t KEY_MIN_EMPTY = 0x7F7F7F7F;  // above every row: the mark of an empty key
+constexpr int KEY_SMALL = 8;  // members of a key kept side by side (32 bytes); more go to a list
 
 struct WallWordsIn {  // what the slab tests read and write (hit_words NULL: no tests here)
     const double* wall_n;
@@ -177,8 +178,8 @@ __global__ void k_key_count(const double* __restrict__ com, const int64_t* __res
         row_key[row] = (int32_t)k;
         const int slot = atomicAdd(&key_cnt[2 * k], 1);
         atomicMin(&key_min[2 * k], row);
-        // the key's first four members side by side, the others in a list
-        if (slot < 4) key_small[4 * (int64_t)k + slot] = row;
+        // the key's first eight members side by side, the others in a list
+        if (slot < KEY_SMALL) key_small[KEY_SMALL * (int64_t)k + slot] = row;
         else row_next[row] = atomicExch(&key_head[k], row);
     }
     if (!ww.hit_words) return;
@@ -306,28 +307,64 @@ __device__ void set_order(const int32_t* rows, int n, int64_t row0, int32_t* out
         if (tab[q] >= 0) out[o++] = (int32_t)(tab[q] + row0);
 }
 
-// set_order of n <= 4 members given in registers (the same table walk).
-__device__ __forceinline__ void set_order4(const int32_t (&rows)[4], int n, int64_t row0, int32_t* out) {
-    uint32_t occ = 0;
-    int slot[4] = {8, 8, 8, 8};
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        if (k < n) {
-            const uint64_t key = (uint64_t)(rows[k] - row0);
-            uint64_t perturb = key;
-            uint32_t i = (uint32_t)(key & 7u);
-            while ((occ >> i) & 1u) {
-                perturb >>= 5;
-                i = (uint32_t)((i * 5u + 1u + perturb) & 7u);
-            }
-            occ |= 1u << i;
-            slot[k] = (int)i;
+// set_order of n <= 8 members given in registers: the same table walks
+// with the tables as occupancy bits. Up to four keys the 8-slot table is
+// never resized (perturbation only: with mask 7 no linear probe fits); the
+// fifth insertion fills it to 5 of 8 (5*5 >= 7*3), so the five keys move,
+// in the order of their slots, to a 32-slot table (the smallest power of two
+// above 4*5), where the later keys join (no resize below 19 keys) and linear
+// probes of 9 apply where they fit.
+__device__ __forceinline__ int probe_bits(uint64_t key, uint32_t mask, uint32_t* occ) {
+    uint64_t perturb = key;
+    uint32_t i = (uint32_t)(key & mask);
+    for (;;) {
+        if (!((*occ >> i) & 1u)) {
+            *occ |= 1u << i;
+            return (int)i;
         }
+        if (i + 9 <= mask) {
+            for (uint32_t j = 1; j <= 9; ++j)
+                if (!((*occ >> (i + j)) & 1u)) {
+                    *occ |= 1u << (i + j);
+                    return (int)(i + j);
+                }
+        }
+        perturb >>= 5;
+        i = (uint32_t)((i * 5 + 1 + perturb) & mask);
+    }
+}
+
+__device__ __forceinline__ void set_order8(const int32_t (&rows)[8], int n, int64_t row0, int32_t* out) {
+    int slot[8];
+    uint32_t occ = 0;
+    const int na = n < 5 ? n : 5;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        slot[k] = 64;
+        if (k < na) slot[k] = probe_bits((uint64_t)(rows[k] - row0), 7u, &occ);
+    }
+    int tsize = 8;
+    if (n > 4) {  // the resize to 32 slots: the five keys in the order of their slots, then the rest
+        int s32[8];
+        uint32_t occ32 = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s32[k] = 64;
+        for (int sl = 0; sl < 8; ++sl) {
+#pragma unroll
+            for (int k = 0; k < 5; ++k)
+                if (slot[k] == sl) s32[k] = probe_bits((uint64_t)(rows[k] - row0), 31u, &occ32);
+        }
+#pragma unroll
+        for (int k = 5; k < 8; ++k)
+            if (k < n) s32[k] = probe_bits((uint64_t)(rows[k] - row0), 31u, &occ32);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) slot[k] = s32[k];
+        tsize = 32;
     }
     int o = 0;
-    for (int sl = 0; sl < 8; ++sl) {
+    for (int sl = 0; sl < tsize; ++sl) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
+        for (int k = 0; k < 8; ++k)
             if (slot[k] == sl) out[o++] = rows[k];
     }
 }
@@ -361,7 +398,7 @@ __device__ __forceinline__ int64_t find_cell(const uint32_t* cell_key, int64_t C
 // in ascending row order. seg_of(t) gives (first slot, members) of the cell
 // of global key t, or (0, 0).
 // Where the key-table path keeps a key's members (k_key_count): the first
-// four in key_small, the rest in a list. The cell pass empties both (the
+// eight in key_small, the rest in a list. The cell pass empties both (the
 // count returns to zero, the list head to -1) for the next pass.
 struct KeyLists {
     const int4* key_small;
@@ -385,19 +422,27 @@ __device__ __forceinline__ int32_t cell_work(int64_t c, int64_t r, bool sorted, 
     // neighbour lookups wait only for the key
     const SceneDev& sc = scenes[scene_of_key(scenes, S, key)];
     const int64_t lk = (int64_t)key - sc.kbase + sc.kmin;
-    int32_t v[4];
+    int32_t v[8];
     if (kl) {  // key-table path: the members from the key's entry and list
-        const int4 sm = kl->key_small[key];
+        const int4 sm = kl->key_small[2 * (int64_t)key];
         v[0] = sm.x;
         v[1] = n > 1 ? sm.y : 0;
         v[2] = n > 2 ? sm.z : 0;
         v[3] = n > 3 ? sm.w : 0;
+        v[4] = v[5] = v[6] = v[7] = 0;
+        if (n > 4) {
+            const int4 s2 = kl->key_small[2 * (int64_t)key + 1];
+            v[4] = s2.x;
+            v[5] = n > 5 ? s2.y : 0;
+            v[6] = n > 6 ? s2.z : 0;
+            v[7] = n > 7 ? s2.w : 0;
+        }
         kl->key_cnt[2 * (int64_t)key] = 0;
-        if (n > 4) {  // the cell's slots take all of them, the list after the four
+        if (n > KEY_SMALL) {  // the cell's slots take all of them: the entry's eight, then the list
 #pragma unroll
-            for (int k = 0; k < 4; ++k) m[k] = v[k];
+            for (int k = 0; k < KEY_SMALL; ++k) m[k] = v[k];
             int32_t row = kl->key_head[key];
-            for (int k = 4; k < n; ++k) {
+            for (int k = KEY_SMALL; k < n; ++k) {
                 m[k] = row;
                 row = kl->row_next[row];
             }
@@ -405,7 +450,7 @@ __device__ __forceinline__ int32_t cell_work(int64_t c, int64_t r, bool sorted, 
         }
     } else {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) v[k] = k < n ? m[k] : 0;
+        for (int k = 0; k < 8; ++k) v[k] = k < n ? m[k] : 0;
     }
     int2 seg[MWD_MAX_OFF];
 #pragma unroll
@@ -416,10 +461,10 @@ __device__ __forceinline__ int32_t cell_work(int64_t c, int64_t r, bool sorted, 
             if (t >= 0 && t < sc.kspan) seg[o] = seg_of((uint32_t)(sc.kbase + t));
         }
     }
-    if (n <= 4) {  // members in registers
+    if (n <= 8) {  // members in registers
         if (!sorted) {
 #pragma unroll
-            for (int k = 1; k < 4; ++k) {
+            for (int k = 1; k < 8; ++k) {
                 if (k < n) {
                     const int32_t x = v[k];
                     int j = k - 1;
@@ -432,9 +477,9 @@ __device__ __forceinline__ int32_t cell_work(int64_t c, int64_t r, bool sorted, 
             }
         }
         if (n == 1) perm[start] = v[0];
-        else if (n > 1) set_order4(v, n, sc.row0, perm + start);
+        else if (n > 1) set_order8(v, n, sc.row0, perm + start);
     } else {
-        if (!sorted) {  // insertion sort in place; cells hold a few members
+        if (!sorted) {  // insertion sort in place
             for (int k = 1; k < n; ++k) {
                 const int32_t x = m[k];
                 int j = k - 1;
