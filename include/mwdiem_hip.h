@@ -327,7 +327,8 @@ int mwd_step(mwd_workspace* ws, mwd_state* state);
  * stream between the stages of mwd_step; mwd_stage_times synchronises and
  * returns the accumulated milliseconds of the MWD_N_STAGES stages and the
  * number of timed steps, then resets the accumulators. The stages: wall slab
- * test; cell keys and their sort; cell tables; pair emission (one kernel);
+ * test (where it does not ride on the key pass); cell keys (with the wall slab
+ * tests on the key-table path) and their sort; cell tables; pair emission (one kernel);
  * pair count; bounding-sphere test (one kernel); GJK (one kernel); contact
  * compaction; EPA and contact law (one kernel); force assembly and Verlet
  * (one kernel). */
