@@ -353,6 +353,7 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     }
     const int64_t M = (int64_t)plist.size();
     ws->n_moving = M;
+    ws->row_base = n_scenes == 1 ? scenes[0].row0 + scenes[0].n_walls : -1;
     // wall-word segments: per scene and wall, the words covering the scene's
     // moving-particle list entries [plist_off, plist_off + np)
     std::vector<int64_t> wseg_off(1, 0);

@@ -163,15 +163,16 @@ __global__ void k_key_count(const double* __restrict__ com, const int64_t* __res
                             int32_t* __restrict__ key_cnt, int32_t* __restrict__ key_min, uint32_t* flags,
                             const double* __restrict__ radius, double* __restrict__ sph, WallWordsIn ww,
                             int64_t* __restrict__ epoch, int32_t* __restrict__ key_small,
-                            int32_t* __restrict__ key_head, int32_t* __restrict__ row_next) {
+                            int32_t* __restrict__ key_head, int32_t* __restrict__ row_next, int64_t row_base) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) *epoch += 1;  // a new pass: key segments of earlier passes go stale
     const bool live = i < (*m_live < M_cap ? *m_live : M_cap);
     if (!ww.hit_words && !live) return;
     int32_t row = 0, s = -1;
     if (live) {
-        row = plist[i];
-        s = scene_of[row];
+        // one scene: its moving rows are consecutive, entry i is row row_base + i
+        row = row_base >= 0 ? (int32_t)(row_base + i) : plist[i];
+        s = row_base >= 0 ? 0 : scene_of[row];
         const SceneDev& sc = scenes[s];
         if (sph) pack_sphere(sph, com, radius, row);
         const uint32_t k = global_key(com, keys_in, row, sc, flags);
@@ -1078,7 +1079,8 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
             hipLaunchKernelGGL(k_key_count, dim3(blocks_for(M)), dim3(256), 0, s, st->com, keys_in, ws->plist, M,
                                ws->m_live, ws->scene_of, ws->scenes_dev, ws->row_key, ws->key_cnt, ws->key_min,
                                ws->flags, st->radius, st->radius ? ws->sph : nullptr, ww, ctr + 6,
-                               reinterpret_cast<int32_t*>(ws->key_small), ws->key_head, ws->row_next);
+                               reinterpret_cast<int32_t*>(ws->key_small), ws->key_head, ws->row_next,
+                               ws->row_base);
             MWD_LAUNCH_CHECK();
             mwd_mark(ws, 2);
             const CellTableIO ct{ws->row_key, ws->key_cnt, ws->key_min, ws->cell_start, ws->cell_key,

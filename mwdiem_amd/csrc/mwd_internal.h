@@ -231,6 +231,7 @@ struct mwd_workspace {
     mwd_scene* scenes_host;
     SceneDev* scenes_dev;
     int64_t n_moving;          // M: moving particles over all scenes
+    int64_t row_base;          // one scene: its first moving row (moving-list entry i is row row_base + i); else -1
     int64_t n_wall_tests;      // sum over scenes of W_s * (n_active_s - W_s)
     int64_t key_span_total;    // sum of kspan
     int key_bits, row_bits;    // significant bits of the sort keys

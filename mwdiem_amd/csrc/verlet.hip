@@ -292,8 +292,9 @@ __device__ __forceinline__ void reduce_verlet_row(const VState& a, const int32_t
                                                   int64_t ovf_cap, const ContactOut& co,
                                                   const double* __restrict__ fext, const double* __restrict__ text,
                                                   const int32_t* __restrict__ nverts, int vmax, double dt,
-                                                  VertJob* job, bool fp32 = false) {
-    const int64_t P = plist[i];
+                                                  VertJob* job, bool fp32 = false, int64_t row_base = -1) {
+    // one scene: its moving rows are consecutive, entry i is row row_base + i
+    const int64_t P = row_base >= 0 ? row_base + i : plist[i];
     V3<T> F = zero3<T>(), Tq = zero3<T>();
     // this step's records of the particle, summed in ascending contact order
     // (the order of the candidate pairs); the bucket is emptied for the next step
@@ -316,7 +317,7 @@ __device__ __forceinline__ void reduce_verlet_row(const VState& a, const int32_t
     }
     F = F + mk3<T>(T(fext[3 * P]), T(fext[3 * P + 1]), T(fext[3 * P + 2]));
     Tq = Tq + mk3<T>(T(text[3 * P]), T(text[3 * P + 1]), T(text[3 * P + 2]));
-    const SceneDev& sc = scenes[scene_of[P]];
+    const SceneDev& sc = scenes[row_base >= 0 ? 0 : scene_of[P]];
     verlet_row<T>(a, P, F, Tq, vmax, nverts[P], dt, sc.box, sc.law_p.mode == MWD_LAW_PHYSICAL ? 1 : 0, job);
 }
 
@@ -330,7 +331,7 @@ k_reduce_verlet_v(VState a, const int32_t* __restrict__ plist, int64_t M_cap, co
                   const SceneDev* __restrict__ scenes, int32_t* __restrict__ rec_cnt,
                   const int32_t* __restrict__ rec_bkt, const int32_t* __restrict__ rec_ovf, const int64_t* n_ovf,
                   int64_t ovf_cap, ContactOut co, const double* __restrict__ fext, const double* __restrict__ text,
-                  const int32_t* __restrict__ nverts, int vmax, double dt, int fp32) {
+                  const int32_t* __restrict__ nverts, int vmax, double dt, int fp32, int64_t row_base) {
     __shared__ VertJob job[VB];
     __shared__ int64_t rows[VB];
     __shared__ unsigned long long r2bits[VB];
@@ -341,9 +342,9 @@ k_reduce_verlet_v(VState a, const int32_t* __restrict__ plist, int64_t M_cap, co
     const int nb = (int)(M - first < VB ? M - first : VB);
     const int64_t i = first + threadIdx.x;
     if (i < M) {
-        rows[threadIdx.x] = plist[i];
+        rows[threadIdx.x] = row_base >= 0 ? row_base + i : plist[i];
         reduce_verlet_row<double>(a, plist, i, scene_of, scenes, rec_cnt, rec_bkt, rec_ovf, n_ovf, ovf_cap, co, fext,
-                                  text, nverts, vmax, dt, &job[threadIdx.x], fp32 != 0);
+                                  text, nverts, vmax, dt, &job[threadIdx.x], fp32 != 0, row_base);
     }
     vertex_pass<VB, PACK>(a.verts.v, rows, job, r2bits, vbase, nb, vmax, a.radius);
 }
@@ -446,7 +447,7 @@ static void launch_reduce_verlet_v(mwd_workspace* ws, const mwd_state* st, const
                   dim3((unsigned)((M + VB - 1) / VB)), dim3(VB), 0, ws->stream, a, ws->plist, M, ws->m_live,
                   ws->scene_of,
                   ws->scenes_dev, ws->rec_cnt, ws->rec_bkt, ws->rec_ovf, ws->counters + 11, 2 * ws->contact_cap, co,
-                  st->fext, st->text, st->nverts, (int)st->vmax, ws->dt, ws->reduce_fp32);
+                  st->fext, st->text, st->nverts, (int)st->vmax, ws->dt, ws->reduce_fp32, ws->row_base);
 }
 
 template <int VB, int WPE>
