@@ -431,7 +431,7 @@ MWD_API int mwd_workspace_create(mwd_workspace** out, int device, const mwd_scen
     ws->cell_table = ws->cell_table_ok;
     if (ws->cell_table_ok) {
         A(key_seg, ws->key_span_total);
-        A(key_small, 2 * ws->key_span_total);
+        A(key_small, 4 * ws->key_span_total);
         A(key_head, ws->key_span_total);
         A(row_next, n_rows);
         A(key_cnt, 2 * ws->key_span_total);  // (count, lowest row) interleaved: one cache line per key

@@ -20,7 +20,7 @@
 //   key table (default while the summed key span is at most 2^28): a
 //     counting sort whose single digit is the whole key. Per particle, count
 //     the key's members, keep its lowest row and file the row under the key
-//     (atomics on key-indexed tables: the first eight members side by side,
+//     (atomics on key-indexed tables: the first sixteen members side by side,
 //     more in a list); one scan over rows numbers the cells by rank and
 //     places their member ranges (a packed two-count scan), stamping each
 //     key's (first slot, members) with the pass's epoch; per cell (fused
@@ -132,8 +132,8 @@ __device__ __forceinline__ int64_t wall_pair_bits(d3 x, const double* __restrict
 }
 
 // Key-table path, one thread per moving particle: its key, the member count
-// of that key and the key's lowest row; the key's first eight members go to
-// its 32-byte entry of key_small (in arrival order), any more to a list
+// of that key and the key's lowest row; the key's first sixteen members go to
+// its 64-byte entry of key_small (in arrival order), any more to a list
 // (key_head, row_next), so that the cell pass finds a cell's members without
 // a scatter of rows into slots. With wall words, also the particle's
 // slab tests against every wall of its scene, gathered by ballots into the
@@ -141,7 +141,7 @@ __device__ __forceinline__ int64_t wall_pair_bits(d3 x, const double* __restrict
 // ballot for one scene and wall is that segment's word; a scan over the
 // words then places the pairs in the reference's order).
 constexpr int32_t KEY_MIN_EMPTY = 0x7F7F7F7F;  // above every row: the mark of an empty key
-constexpr int KEY_SMALL = 8;  // members of a key kept side by side (32 bytes); more go to a list
+constexpr int KEY_SMALL = 16;  // members of a key kept side by side (64 bytes); more go to a list
 
 struct WallWordsIn {  // what the slab tests read and write (hit_words NULL: no tests here)
     const double* wall_n;
@@ -179,7 +179,7 @@ __global__ void k_key_count(const double* __restrict__ com, const int64_t* __res
         row_key[row] = (int32_t)k;
         const int slot = atomicAdd(&key_cnt[2 * k], 1);
         atomicMin(&key_min[2 * k], row);
-        // the key's first eight members side by side, the others in a list
+        // the key's first sixteen members side by side, the others in a list
         if (slot < KEY_SMALL) key_small[KEY_SMALL * (int64_t)k + slot] = row;
         else row_next[row] = atomicExch(&key_head[k], row);
     }
@@ -231,84 +231,88 @@ __global__ void k_key_count(const double* __restrict__ com, const int64_t* __res
 // CPython set iteration order of non-negative ints inserted in the given
 // order (Objects/setobject.c: 9 linear probes, then perturbation; resize to
 // the smallest power of two above 4*used once fill*5 >= mask*3).
-__device__ void cpy_probe(int32_t* tab, uint32_t mask, int32_t key) {
-    uint64_t perturb = (uint64_t)key;
-    uint64_t i = (uint64_t)key & mask;
+constexpr int SET_TAB = 128;  // holds up to MWD_MAX_CELL members
+
+// The general case (any n up to MWD_MAX_CELL): the same table walk with the
+// table's occupancy as 128 bits in registers and its keys in a slot array
+// that is read only where a bit is set (nothing to clear), so a cell costs a
+// few accesses per member. Table sizes follow the rule above: 8 slots, 32
+// once 5 keys fill the first (5*5 >= 7*3), 128 once 19 fill the second
+// (19*5 >= 31*3; 128 is the smallest power of two above 4*19); the next
+// resize would come at 77 keys, past MWD_MAX_CELL.
+__device__ __forceinline__ bool occ_get(uint64_t lo, uint64_t hi, uint32_t i) {
+    return (((i < 64) ? lo : hi) >> (i & 63)) & 1ull;
+}
+__device__ __forceinline__ void occ_set(uint64_t& lo, uint64_t& hi, uint32_t i) {
+    if (i < 64) lo |= 1ull << i;
+    else hi |= 1ull << (i - 64);
+}
+__device__ __forceinline__ uint32_t probe_occ(uint64_t key, uint32_t mask, uint64_t& lo, uint64_t& hi) {
+    uint64_t perturb = key;
+    uint32_t i = (uint32_t)(key & mask);
     for (;;) {
-        if (tab[i] < 0) {
-            tab[i] = key;
-            return;
+        if (!occ_get(lo, hi, i)) {
+            occ_set(lo, hi, i);
+            return i;
         }
         if (i + 9 <= mask) {
-            for (uint64_t j = 1; j <= 9; ++j)
-                if (tab[i + j] < 0) {
-                    tab[i + j] = key;
-                    return;
+            for (uint32_t j = 1; j <= 9; ++j)
+                if (!occ_get(lo, hi, i + j)) {
+                    occ_set(lo, hi, i + j);
+                    return i + j;
                 }
         }
         perturb >>= 5;
-        i = (i * 5 + 1 + perturb) & mask;
+        i = (uint32_t)((i * 5 + 1 + perturb) & mask);
     }
 }
-
-constexpr int SET_TAB = 128;  // holds up to MWD_MAX_CELL members
 
 __device__ void set_order(const int32_t* rows, int n, int64_t row0, int32_t* out) {
     if (n <= 1) {
         if (n == 1) out[0] = rows[0];
         return;
     }
-    if (n <= 4) {
-        // 8-slot table, never resized at this size; with mask 7 no linear
-        // probe fits, so each collision perturbs. Slots live in registers.
-        uint32_t occ = 0;
-        int slot[4] = {8, 8, 8, 8};
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (k < n) {
-                const uint64_t key = (uint64_t)(rows[k] - row0);
-                uint64_t perturb = key;
-                uint32_t i = (uint32_t)(key & 7u);
-                while ((occ >> i) & 1u) {
-                    perturb >>= 5;
-                    i = (uint32_t)((i * 5u + 1u + perturb) & 7u);
-                }
-                occ |= 1u << i;
-                slot[k] = (int)i;
-            }
-        }
-        int o = 0;
-        for (int sl = 0; sl < 8; ++sl) {
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-                if (slot[k] == sl) out[o++] = rows[k];
-        }
-        return;
-    }
-    int32_t tab[SET_TAB], tmp[SET_TAB];
-    uint32_t size = 8;
-    for (int k = 0; k < SET_TAB; ++k) tab[k] = -1;
-    uint32_t fill = 0;
+    int32_t tab[2][SET_TAB];  // slot -> key (row - row0), valid where the occupancy bit is set
+    int cur = 0;
+    uint64_t lo = 0, hi = 0;
+    uint32_t size = 8, fill = 0;
     for (int k = 0; k < n; ++k) {
-        cpy_probe(tab, size - 1, (int32_t)(rows[k] - row0));
+        const int32_t key = (int32_t)(rows[k] - row0);
+        tab[cur][probe_occ((uint64_t)key, size - 1, lo, hi)] = key;
         ++fill;
-        if (fill * 5 >= (size - 1) * 3) {
+        if (fill * 5 >= (size - 1) * 3) {  // resize: the keys in the order of their slots
             const uint32_t minused = fill * 4;
             uint32_t ns = 8;
             while (ns <= minused) ns <<= 1;
-            for (uint32_t q = 0; q < ns; ++q) tmp[q] = -1;
-            for (uint32_t q = 0; q < size; ++q)
-                if (tab[q] >= 0) cpy_probe(tmp, ns - 1, tab[q]);
-            for (uint32_t q = 0; q < ns; ++q) tab[q] = tmp[q];
+            uint64_t nlo = 0, nhi = 0;
+            const int nxt = 1 - cur;
+            for (int half = 0; half < 2; ++half) {
+                uint64_t bits = half ? hi : lo;
+                while (bits) {
+                    const uint32_t q = (uint32_t)(__ffsll((long long)bits) - 1) + 64u * half;
+                    bits &= bits - 1;
+                    const int32_t key2 = tab[cur][q];
+                    tab[nxt][probe_occ((uint64_t)key2, ns - 1, nlo, nhi)] = key2;
+                }
+            }
+            lo = nlo;
+            hi = nhi;
+            cur = nxt;
             size = ns;
         }
     }
     int o = 0;
-    for (uint32_t q = 0; q < size; ++q)
-        if (tab[q] >= 0) out[o++] = (int32_t)(tab[q] + row0);
+    for (int half = 0; half < 2; ++half) {
+        uint64_t bits = half ? hi : lo;
+        while (bits) {
+            const uint32_t q = (uint32_t)(__ffsll((long long)bits) - 1) + 64u * half;
+            bits &= bits - 1;
+            out[o++] = (int32_t)(tab[cur][q] + row0);
+        }
+    }
 }
 
-// set_order of n <= 8 members given in registers: the same table walks
+// set_order of n <= 16 members given in registers: the same table walks
 // with the tables as occupancy bits. Up to four keys the 8-slot table is
 // never resized (perturbation only: with mask 7 no linear probe fits); the
 // fifth insertion fills it to 5 of 8 (5*5 >= 7*3), so the five keys move,
@@ -335,37 +339,53 @@ __device__ __forceinline__ int probe_bits(uint64_t key, uint32_t mask, uint32_t*
     }
 }
 
-__device__ __forceinline__ void set_order8(const int32_t (&rows)[8], int n, int64_t row0, int32_t* out) {
-    int slot[8];
+// Sort of the first n of 16 values in registers (compare-exchanges at fixed
+// indices; entries at or past n hold INT32_MAX and stay last): odd-even
+// transposition over 8 or 16 entries.
+template <int N>
+__device__ __forceinline__ void sort_net(int32_t (&v)[16]) {
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+#pragma unroll
+        for (int i = r & 1; i + 1 < N; i += 2) {
+            const int32_t a = v[i], b = v[i + 1];
+            v[i] = a < b ? a : b;
+            v[i + 1] = a < b ? b : a;
+        }
+    }
+}
+
+__device__ __forceinline__ void set_order16(const int32_t (&rows)[16], int n, int64_t row0, int32_t* out) {
+    int slot[16];
     uint32_t occ = 0;
     const int na = n < 5 ? n : 5;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
+    for (int k = 0; k < 16; ++k) {
         slot[k] = 64;
         if (k < na) slot[k] = probe_bits((uint64_t)(rows[k] - row0), 7u, &occ);
     }
     int tsize = 8;
     if (n > 4) {  // the resize to 32 slots: the five keys in the order of their slots, then the rest
-        int s32[8];
+        int s32[16];
         uint32_t occ32 = 0;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) s32[k] = 64;
+        for (int k = 0; k < 16; ++k) s32[k] = 64;
         for (int sl = 0; sl < 8; ++sl) {
 #pragma unroll
             for (int k = 0; k < 5; ++k)
                 if (slot[k] == sl) s32[k] = probe_bits((uint64_t)(rows[k] - row0), 31u, &occ32);
         }
 #pragma unroll
-        for (int k = 5; k < 8; ++k)
+        for (int k = 5; k < 16; ++k)
             if (k < n) s32[k] = probe_bits((uint64_t)(rows[k] - row0), 31u, &occ32);
 #pragma unroll
-        for (int k = 0; k < 8; ++k) slot[k] = s32[k];
+        for (int k = 0; k < 16; ++k) slot[k] = s32[k];
         tsize = 32;
     }
     int o = 0;
     for (int sl = 0; sl < tsize; ++sl) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
+        for (int k = 0; k < 16; ++k)
             if (slot[k] == sl) out[o++] = rows[k];
     }
 }
@@ -399,7 +419,7 @@ __device__ __forceinline__ int64_t find_cell(const uint32_t* cell_key, int64_t C
 // in ascending row order. seg_of(t) gives (first slot, members) of the cell
 // of global key t, or (0, 0).
 // Where the key-table path keeps a key's members (k_key_count): the first
-// eight in key_small, the rest in a list. The cell pass empties both (the
+// sixteen in key_small, the rest in a list. The cell pass empties both (the
 // count returns to zero, the list head to -1) for the next pass.
 struct KeyLists {
     const int4* key_small;
@@ -423,23 +443,17 @@ __device__ __forceinline__ int32_t cell_work(int64_t c, int64_t r, bool sorted, 
     // neighbour lookups wait only for the key
     const SceneDev& sc = scenes[scene_of_key(scenes, S, key)];
     const int64_t lk = (int64_t)key - sc.kbase + sc.kmin;
-    int32_t v[8];
-    if (kl) {  // key-table path: the members from the key's entry and list
-        const int4 sm = kl->key_small[2 * (int64_t)key];
-        v[0] = sm.x;
-        v[1] = n > 1 ? sm.y : 0;
-        v[2] = n > 2 ? sm.z : 0;
-        v[3] = n > 3 ? sm.w : 0;
-        v[4] = v[5] = v[6] = v[7] = 0;
-        if (n > 4) {
-            const int4 s2 = kl->key_small[2 * (int64_t)key + 1];
-            v[4] = s2.x;
-            v[5] = n > 5 ? s2.y : 0;
-            v[6] = n > 6 ? s2.z : 0;
-            v[7] = n > 7 ? s2.w : 0;
-        }
+    int32_t v[16];  // members in registers, INT32_MAX past n
+    if (kl) {  // key-table path: the members from the key's entry (one 64-byte line) and list
+        const int4* e = kl->key_small + 4 * (int64_t)key;
+        const int4 z = make_int4(INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX);
+        const int4 e0 = e[0], e1 = n > 4 ? e[1] : z, e2 = n > 8 ? e[2] : z, e3 = n > 12 ? e[3] : z;
+        const int32_t w[16] = {e0.x, e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, e1.w,
+                               e2.x, e2.y, e2.z, e2.w, e3.x, e3.y, e3.z, e3.w};
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = k < n ? w[k] : INT32_MAX;
         kl->key_cnt[2 * (int64_t)key] = 0;
-        if (n > KEY_SMALL) {  // the cell's slots take all of them: the entry's eight, then the list
+        if (n > KEY_SMALL) {  // the cell's slots take all of them: the entry's sixteen, then the list
 #pragma unroll
             for (int k = 0; k < KEY_SMALL; ++k) m[k] = v[k];
             int32_t row = kl->key_head[key];
@@ -451,7 +465,7 @@ __device__ __forceinline__ int32_t cell_work(int64_t c, int64_t r, bool sorted, 
         }
     } else {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) v[k] = k < n ? m[k] : 0;
+        for (int k = 0; k < 16; ++k) v[k] = k < n ? m[k] : INT32_MAX;
     }
     int2 seg[MWD_MAX_OFF];
 #pragma unroll
@@ -462,40 +476,28 @@ __device__ __forceinline__ int32_t cell_work(int64_t c, int64_t r, bool sorted, 
             if (t >= 0 && t < sc.kspan) seg[o] = seg_of((uint32_t)(sc.kbase + t));
         }
     }
-    if (n <= 8) {  // members in registers
+    if (n <= 16) {  // members in registers
         if (!sorted) {
-#pragma unroll
-            for (int k = 1; k < 8; ++k) {
-                if (k < n) {
-                    const int32_t x = v[k];
-                    int j = k - 1;
-                    while (j >= 0 && v[j] > x) {
-                        v[j + 1] = v[j];
-                        --j;
-                    }
-                    v[j + 1] = x;
-                }
-            }
+            if (n <= 8) sort_net<8>(v);
+            else sort_net<16>(v);
         }
         if (n == 1) perm[start] = v[0];
-        else if (n > 1) set_order8(v, n, sc.row0, perm + start);
+        else if (n > 1) set_order16(v, n, sc.row0, perm + start);
     } else {
-        if (!sorted) {  // insertion sort in place
-            for (int k = 1; k < n; ++k) {
-                const int32_t x = m[k];
-                int j = k - 1;
-                while (j >= 0 && m[j] > x) {
-                    m[j + 1] = m[j];
-                    --j;
-                }
-                m[j + 1] = x;
-            }
-        }
         if (n > MWD_MAX_CELL) {
             atomicOr(flags, MWD_F_CELL);
             for (int k = 0; k < n; ++k) perm[start + k] = m[k];
-        } else {
+        } else if (sorted) {
             set_order(m, n, sc.row0, perm + start);
+        } else {  // sorted by rank (the rows are distinct): no chain of dependent moves
+            int32_t srt[MWD_MAX_CELL];
+            for (int k = 0; k < n; ++k) {
+                const int32_t x = m[k];
+                int rk = 0;
+                for (int j = 0; j < n; ++j) rk += m[j] < x ? 1 : 0;
+                srt[rk] = x;
+            }
+            set_order(srt, n, sc.row0, perm + start);
         }
     }
     cell_self[r] = make_int2(start, n);

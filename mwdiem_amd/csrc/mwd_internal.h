@@ -264,7 +264,7 @@ struct mwd_workspace {
     PairTune tune;
     int4* key_seg;             // [key_span_total] per key this pass: (epoch, first member slot, members, 0);
                                // an entry whose epoch is not the pass's (counters[6]) has no cell
-    int4* key_small;           // [2 * key_span_total] per key its first eight member rows (in arrival order)
+    int4* key_small;           // [4 * key_span_total] per key its first sixteen member rows (in arrival order)
     int32_t* key_head;         // [key_span_total] per key the list of its further members (-1: none)
     int32_t* row_next;         // [n_rows] next row of a key's list
     int32_t* key_cnt;          // [2 * key_span_total] per key k: members at 2k (zero between steps) and
