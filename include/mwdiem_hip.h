@@ -55,6 +55,8 @@ extern "C" {
 #define MWD_F_ROW_CONTACTS 64u     /* the stepper's contact-record overflow list is full */
 #define MWD_F_EPA_UNRESOLVED 128u  /* some EPA ended on a placeholder face (epa_status bit 4);
                                       the physical law applies no force for such a contact */
+#define MWD_F_TAPE 256u            /* mwd_step_record found the tape full, or
+                                      mwd_step_adjoint_taped found it empty (nothing copied) */
 
 #define MWD_EPA_MAX_FACES 128
 #define MWD_MAX_CELL 76            /* largest cell occupancy whose set order is emulated */
@@ -430,6 +432,30 @@ int mwd_step_tangent(mwd_workspace* ws, mwd_state* state, double* const* tan);
  * friction) is held fixed at what the values give. Scenes on the physical
  * law only (MWD_E_INVALID otherwise). Walls receive no adjoint. */
 int mwd_step_adjoint(mwd_workspace* ws, mwd_state* state, double* const* lam, double* dtheta);
+
+/* The reverse sweep's tape (no counterpart in the reference). A sweep with
+ * checkpoints recomputes each segment's steps before reversing them; with the
+ * tape, the recomputed steps keep what the adjoint step reads of their
+ * contact stages (the contact counts, the per-particle contact records, the
+ * candidate pairs and their contact slots, and per contact its pair, depth,
+ * normal, contact points, force, torques and EPA's final face), so that the
+ * adjoint step takes them back instead of rerunning the contact stages.
+ *
+ * mwd_tape_init: room for n_slots steps (one slot: a fixed size from the
+ * workspace's capacities, returned in *slot_bytes when not NULL), allocated
+ * on the device with the adjoint step's buffers; n_slots = 0 frees it. The
+ * tape starts empty. Same conditions as mwd_step_adjoint. */
+int mwd_tape_init(mwd_workspace* ws, int64_t n_slots, int64_t* slot_bytes);
+/* Empty the tape (on the device, so it can be captured in a graph). */
+int mwd_tape_rewind(mwd_workspace* ws);
+/* mwd_step, its values bit for bit, also pushing the step's contact-stage
+ * outputs onto the tape (MWD_F_TAPE when it is full). */
+int mwd_step_record(mwd_workspace* ws, mwd_state* state);
+/* mwd_step_adjoint of the step last recorded and not yet taken back: pops
+ * its contact-stage outputs from the tape (MWD_F_TAPE when it is empty)
+ * instead of rerunning the contact stages. state must hold that step's input
+ * state. The same results as mwd_step_adjoint, bit for bit. */
+int mwd_step_adjoint_taped(mwd_workspace* ws, mwd_state* state, double* const* lam, double* dtheta);
 
 #ifdef __cplusplus
 }

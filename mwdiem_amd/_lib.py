@@ -26,6 +26,7 @@ F_EPA_CAP = 16
 F_EPA_FACES = 32
 F_ROW_CONTACTS = 64
 F_EPA_UNRESOLVED = 128
+F_TAPE = 256
 FATAL_FLAGS = F_PAIR_OVERFLOW | F_CONTACT_OVERFLOW | F_KEYRANGE | F_CELL | F_ROW_CONTACTS
 
 ROW_CONTACTS = 32
@@ -145,6 +146,10 @@ _SIGS = {
     "mwd_set_law_tangents": (C.c_int, [_P, C.POINTER(LawTan)]),
     "mwd_step_tangent": (C.c_int, [_P, C.POINTER(State), C.POINTER(_P)]),
     "mwd_step_adjoint": (C.c_int, [_P, C.POINTER(State), C.POINTER(_P), _P]),
+    "mwd_tape_init": (C.c_int, [_P, C.c_int64, C.POINTER(C.c_int64)]),
+    "mwd_tape_rewind": (C.c_int, [_P]),
+    "mwd_step_record": (C.c_int, [_P, C.POINTER(State)]),
+    "mwd_step_adjoint_taped": (C.c_int, [_P, C.POINTER(State), C.POINTER(_P), _P]),
 }
 
 N_KTIMED = 5

@@ -5,9 +5,12 @@ reverse sweep takes the segments last to first: it restores a segment's
 checkpoint, recomputes the segment's steps keeping each step's input state,
 then reverses those steps one at a time with mwd_step_adjoint -- the
 hand-written adjoint kernels of the Verlet step, the physical contact law
-and EPA's final-face geometry; the step's contact stages rerun on the
-restored state, so the topology is the forward's. Memory: the checkpoints
-plus one segment of states. The result equals the forward-mode tangents'
+and EPA's final-face geometry. The recomputed steps push their contact-stage
+outputs onto a device tape (mwd_step_record) and the adjoint steps pop them
+(mwd_step_adjoint_taped), so the topology is the forward's and the contact
+stages run once per recomputed step; without the tape they rerun on the
+restored state instead. Memory: the checkpoints plus one segment of states
+and tape slots. The result equals the forward-mode tangents'
 (mwdiem_amd.diff) to rounding; the adjoint costs one recomputed forward
 step and one reverse step per step, whatever the number of parameters.
 """
@@ -33,13 +36,17 @@ def _set_init(S: Stepper, init):
 
 
 def adjoint_run(sc, init, theta, steps: int, every: int = 100, top_frac: float = 0.1, seed=None,
-                steps_per_graph: int = 0, timings: dict | None = None, tuning: dict | None = None):
+                steps_per_graph: int = 0, timings: dict | None = None, tuning: dict | None = None,
+                tape: bool = True):
     """Run `steps` steps of the scene with law parameters theta = (Kn, en,
     mu) from `init` (None, initial velocities or a state dict) and return
     (outcome, d outcome/d theta) by the adjoint. With steps_per_graph the
-    forward and recomputed segments replay captured graphs. timings, if
-    given, receives the forward and reverse wall times (s); tuning: launch
-    choices."""
+    forward and recomputed segments replay captured graphs. With tape (the
+    default) the recomputed steps keep their contact-stage outputs on the
+    device (one slot per step of a segment) and the adjoint steps take them
+    back instead of rerunning the contact stages; the gradient is the same bit
+    for bit. timings, if given, receives the forward and reverse wall times
+    (s); tuning: launch choices."""
     import time
     S = Stepper(_with_theta(sc, theta))
     if tuning:
@@ -70,19 +77,25 @@ def adjoint_run(sc, init, theta, steps: int, every: int = 100, top_frac: float =
     # graphs of one step each (captured once, on the stepper's fixed buffers)
     live = [S.t[k] for k in SNAP_KEYS]
     slots = [[torch.empty_like(t) for t in live] for _ in range(min(every, steps))]
+    if tape:
+        S.tape_init(min(every, steps))
     stream = S.stream.cuda_stream
     g_step = g_adj = None
     for j in reversed(range(len(ckpt))):
         n0 = j * every
         n1 = min(n0 + every, steps)
         S.restore(ckpt[j])
+        if tape:
+            S.tape_rewind()
         for i in range(n1 - n0):
             _lib.copy_arrays(slots[i], live, stream)
             if steps_per_graph:
                 if g_step is None:
-                    g_step = S.capture(1)
+                    g_step = S.capture_record() if tape else S.capture(1)
                 with torch.cuda.stream(S.stream):
                     g_step.replay()
+            elif tape:
+                S.step_record(1)
             else:
                 S.step(1)
         for i in reversed(range(n1 - n0)):
@@ -91,9 +104,12 @@ def adjoint_run(sc, init, theta, steps: int, every: int = 100, top_frac: float =
                 with torch.cuda.stream(S.stream):
                     g_adj.replay()
             else:
-                S.step_adjoint(ptrs, dth)
+                if tape:
+                    S.step_adjoint_taped(ptrs, dth)
+                else:
+                    S.step_adjoint(ptrs, dth)
                 if steps_per_graph:
-                    g_adj = S.capture_adjoint(ptrs, dth)
+                    g_adj = S.capture_adjoint(ptrs, dth, taped=tape)
     del slots
     S.check()
     torch.cuda.synchronize(S.device)

@@ -12,7 +12,8 @@
 // lambda_n in the same arrays and adds d(lambda_{n+1} . s_{n+1})/d theta to
 // an accumulator, theta being the three law parameters whose coefficient
 // derivatives mwd_set_law_tangents gave. Kernels, after the forward contact
-// stages have rerun on s_n:
+// stages have rerun on s_n (or their outputs have come back from the tape,
+// below):
 //   k_force_sum    per moving row: the step's contact force and torque (its
 //                  records in ascending order, as the forward reduction sums
 //                  them) plus the external ones;
@@ -572,7 +573,163 @@ unsigned blocks_of(int64_t n, int64_t cap) {
     return (unsigned)b;
 }
 
+// ------------------------------------------------------------ the tape
+// The reverse sweep's tape holds, per recorded step, what the adjoint step
+// reads of the contact stages' outputs: the contact counts, the per-row
+// records, the candidate pairs and their contact slots, and per contact its
+// pair, depth, normal, contact points, force, torques and EPA's final face.
+// A slot is a header of the three counts (pairs, contacts, overflow records;
+// int64) and then each array at a 256-byte boundary; an array keeps the
+// prefix its count covers (all of it for the per-row arrays).
+constexpr int TAPE_MAX = 16;
+constexpr int TAPE_HEAD = 64;   // words (uint32) before the first array
+// the counters of the pairs, the contacts and the overflow records
+__device__ __forceinline__ int tape_ctr(int k) { return k == 0 ? 3 : (k == 1 ? 4 : 11); }
+
+struct TapeArr {
+    uint32_t* ws;   // the workspace array
+    int64_t off;    // its offset in a slot (words, a multiple of 64)
+    int64_t unit;   // words per element
+    int64_t cap;    // elements at most
+    int cnt;        // -1: all cap elements; else the count (0 pairs, 1 contacts, 2 overflow records)
+};
+
+struct TapeDesc {
+    TapeArr a[TAPE_MAX];
+    int n;
+    uint32_t* base;
+    int64_t slot_words, slots;
+    int64_t* ctr;      // workspace counters; [18] steps held, [19] blocks of this launch done
+    uint32_t* flags;
+};
+
+// The slot layout of a workspace (base and slot count left to the caller);
+// returns the words per slot.
+int64_t tape_layout(const mwd_workspace* ws, TapeDesc& d) {
+    d = TapeDesc{};
+    int64_t off = TAPE_HEAD;
+    auto add = [&](void* p, int64_t unit, int64_t cap, int cnt) {
+        TapeArr& t = d.a[d.n++];
+        t.ws = (uint32_t*)p;
+        t.off = off;
+        t.unit = unit;
+        t.cap = cap;
+        t.cnt = cnt;
+        off += (unit * cap + 63) / 64 * 64;
+    };
+    const int64_t C = ws->contact_cap, P = ws->pair_cap;
+    add(ws->rec_cnt, 1, ws->n_rows, -1);
+    add(ws->rec_bkt, MWD_ROW_CONTACTS, ws->n_rows, -1);
+    add(ws->rec_ovf, 2, 2 * C, 2);
+    add(ws->sidx, 1, P, 0);
+    add(ws->pairs, 2, P, 0);
+    add(ws->contact_pair, 1, C, 1);
+    add(ws->adj_face, 7, C, 1);
+    add(ws->depth, 2, C, 1);
+    add(ws->normal, 6, C, 1);
+    add(ws->cpA, 6, C, 1);
+    add(ws->cpB, 6, C, 1);
+    add(ws->Fn, 6, C, 1);
+    add(ws->tA, 6, C, 1);
+    add(ws->tB, 6, C, 1);
+    d.ctr = ws->counters;
+    d.flags = ws->flags;
+    return off;
+}
+
+// blockIdx.y: the array; the blocks along x split its words. PUSH copies the
+// workspace arrays into slot counters[18] and raises it; a pop lowers it and
+// copies that slot back (counts included). The last block to finish moves
+// the slot count, so every block has read it before it changes. A push onto
+// a full tape or a pop from an empty one copies nothing and raises
+// MWD_F_TAPE.
+template <bool PUSH>
+__global__ void __launch_bounds__(256) k_tape(TapeDesc d) {
+    __shared__ int64_t s_slot;
+    __shared__ int64_t s_cnt[3];
+    if (threadIdx.x == 0) {
+        const int64_t held = *(volatile int64_t*)&d.ctr[18];
+        const int64_t slot = PUSH ? held : held - 1;
+        s_slot = slot >= 0 && slot < d.slots ? slot : -1;
+        if (s_slot >= 0) {
+            const int64_t* head = (const int64_t*)(d.base + s_slot * d.slot_words);
+            for (int k = 0; k < 3; ++k) s_cnt[k] = PUSH ? d.ctr[tape_ctr(k)] : head[k];
+        }
+    }
+    __syncthreads();
+    const int64_t slot = s_slot;
+    if (slot >= 0) {
+        uint32_t* sw = d.base + slot * d.slot_words;
+        if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 3) {
+            int64_t* head = (int64_t*)sw;
+            if (PUSH) head[threadIdx.x] = s_cnt[threadIdx.x];
+            else d.ctr[tape_ctr(threadIdx.x)] = s_cnt[threadIdx.x];
+        }
+        const TapeArr t = d.a[blockIdx.y];
+        int64_t n = t.cap;
+        if (t.cnt >= 0) {
+            const int64_t c = s_cnt[t.cnt];
+            n = c < 0 ? 0 : (c < n ? c : n);
+        }
+        const int64_t words = n * t.unit;
+        uint32_t* dst = PUSH ? sw + t.off : t.ws;
+        const uint32_t* src = PUSH ? t.ws : sw + t.off;
+        // 16-byte moves (both ends 256-byte aligned), then the last words
+        const int64_t n4 = words >> 2;
+        const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride)
+            ((uint4*)dst)[i] = ((const uint4*)src)[i];
+        if (blockIdx.x == 0 && (int64_t)threadIdx.x < words - 4 * n4)
+            dst[4 * n4 + threadIdx.x] = src[4 * n4 + threadIdx.x];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned long long total = (unsigned long long)gridDim.x * gridDim.y;
+        const unsigned long long done = atomicAdd((unsigned long long*)&d.ctr[19], 1ull);
+        if (done == total - 1) {
+            d.ctr[19] = 0;
+            if (slot >= 0) d.ctr[18] += PUSH ? 1 : -1;
+            else atomicOr(d.flags, MWD_F_TAPE);
+        }
+    }
+}
+
+__global__ void k_tape_rewind(int64_t* ctr) {
+    if (threadIdx.x == 0) {
+        ctr[18] = 0;
+        ctr[19] = 0;
+    }
+}
+
 }  // namespace
+
+int64_t mwd_tape_words(const mwd_workspace* ws) {
+    TapeDesc d;
+    return tape_layout(ws, d);
+}
+
+int mwd_launch_tape(mwd_workspace* ws, bool push) {
+    TapeDesc d;
+    d.slot_words = tape_layout(ws, d);
+    d.base = ws->tape;
+    d.slots = ws->tape_slots;
+    if (!d.base || d.slot_words != ws->tape_words) {
+        mwd_set_error("the tape is not allocated for this workspace (mwd_tape_init)");
+        return MWD_E_INVALID;
+    }
+    const dim3 grid(64, d.n);
+    if (push) hipLaunchKernelGGL(k_tape<true>, grid, dim3(256), 0, ws->stream, d);
+    else hipLaunchKernelGGL(k_tape<false>, grid, dim3(256), 0, ws->stream, d);
+    MWD_LAUNCH_CHECK();
+    return MWD_OK;
+}
+
+int mwd_launch_tape_rewind(mwd_workspace* ws) {
+    hipLaunchKernelGGL(k_tape_rewind, dim3(1), dim3(64), 0, ws->stream, ws->counters);
+    MWD_LAUNCH_CHECK();
+    return MWD_OK;
+}
 
 int mwd_launch_step_adjoint(mwd_workspace* ws, const mwd_state* st, const BodyView& A, const BodyView& B,
                             double* const* lam_arr, double* dtheta) {

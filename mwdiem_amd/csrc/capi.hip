@@ -55,7 +55,7 @@ void free_ws(mwd_workspace* ws) {
                     ws->rec_bkt, ws->rec_ovf, ws->hist,
                     ws->hist_off, ws->scan_tmp, ws->scan_tmp_w, ws->wseg_off, ws->word_desc, ws->hit_words, ws->gjk_words, ws->counters, ws->flags, ws->laws,
                     ws->scene_wt_off, ws->ctan, ws->law_tans, ws->adj_F, ws->adj_T, ws->adj_lamF,
-                    ws->adj_lamT, ws->adj_face, ws->adj_c, ws->adj_cth, ws->adj_rowth};
+                    ws->adj_lamT, ws->adj_face, ws->adj_c, ws->adj_cth, ws->adj_rowth, ws->tape};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     destroy_events(ws);
@@ -984,51 +984,128 @@ MWD_API int mwd_step_tangent(mwd_workspace* ws, mwd_state* st, double* const* ta
     return MWD_OK;
 }
 
-MWD_API int mwd_step_adjoint(mwd_workspace* ws, mwd_state* st, double* const* lam, double* dtheta) {
-    if (!ws || !st || !lam || !dtheta || st->vmax != ws->vmax) {
-        mwd_set_error("mwd_step_adjoint: invalid argument");
-        return MWD_E_INVALID;
-    }
+// What the adjoint step and the tape need of a workspace (name: the entry point).
+static int adjoint_ok(mwd_workspace* ws, const char* name) {
     if (ws->spawn_on) {
-        mwd_set_error("mwd_step_adjoint: not with a spawn schedule");
-        return MWD_E_INVALID;
-    }
-    for (int i = 0; i < MWD_N_TAN_ARRAYS; ++i)
-        if (!lam[i]) {
-            mwd_set_error("mwd_step_adjoint: adjoint array " + std::to_string(i) + " is NULL");
-            return MWD_E_INVALID;
-        }
-    if (!ws->law_tans) {
-        mwd_set_error("mwd_step_adjoint: call mwd_set_law_tangents first (the parameter derivatives of the laws)");
+        mwd_set_error(std::string(name) + ": not with a spawn schedule");
         return MWD_E_INVALID;
     }
     if (ws->hist_on) {
-        mwd_set_error("mwd_step_adjoint: not with a persistent tangential history");
+        mwd_set_error(std::string(name) + ": not with a persistent tangential history");
         return MWD_E_INVALID;
     }
     for (int s = 0; s < ws->n_scenes; ++s)
         if (ws->scenes_host[s].law_p.mode != MWD_LAW_PHYSICAL || ws->scenes_host[s].law_w.mode != MWD_LAW_PHYSICAL) {
-            mwd_set_error("mwd_step_adjoint: scene " + std::to_string(s) + " is not on the physical law");
+            mwd_set_error(std::string(name) + ": scene " + std::to_string(s) + " is not on the physical law");
             return MWD_E_INVALID;
         }
-    if (!ws->adj_F) {  // the first call allocates (so later calls can be captured in a graph)
-        MWD_HIP_CHECK(hipSetDevice(ws->device));
-        const size_t r3 = sizeof(double) * 3 * (size_t)ws->n_rows;
-        MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_F, r3));
-        MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_T, r3));
-        MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_lamF, r3));
-        MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_lamT, r3));
-        MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_face, sizeof(int32_t) * 7 * (size_t)ws->contact_cap));
-        MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_c, sizeof(double) * 36 * (size_t)ws->contact_cap));
-        MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_cth, sizeof(double) * 3 * (size_t)ws->contact_cap));
-        MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_rowth, r3));
+    return MWD_OK;
+}
+
+// The adjoint step's buffers, at the first adjoint call or tape set-up (so
+// that later calls can be captured in a graph).
+static int alloc_adjoint(mwd_workspace* ws) {
+    if (ws->adj_F) return MWD_OK;
+    MWD_HIP_CHECK(hipSetDevice(ws->device));
+    const size_t r3 = sizeof(double) * 3 * (size_t)ws->n_rows;
+    MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_F, r3));
+    MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_T, r3));
+    MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_lamF, r3));
+    MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_lamT, r3));
+    MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_face, sizeof(int32_t) * 7 * (size_t)ws->contact_cap));
+    MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_c, sizeof(double) * 36 * (size_t)ws->contact_cap));
+    MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_cth, sizeof(double) * 3 * (size_t)ws->contact_cap));
+    MWD_HIP_CHECK(hipMalloc((void**)&ws->adj_rowth, r3));
+    return MWD_OK;
+}
+
+static int adjoint_args(mwd_workspace* ws, mwd_state* st, double* const* lam, double* dtheta, const char* name) {
+    if (!ws || !st || !lam || !dtheta || st->vmax != ws->vmax) {
+        mwd_set_error(std::string(name) + ": invalid argument");
+        return MWD_E_INVALID;
     }
-    int rc = mwd_candidates(ws, st, nullptr);
+    int rc = adjoint_ok(ws, name);
+    if (rc) return rc;
+    for (int i = 0; i < MWD_N_TAN_ARRAYS; ++i)
+        if (!lam[i]) {
+            mwd_set_error(std::string(name) + ": adjoint array " + std::to_string(i) + " is NULL");
+            return MWD_E_INVALID;
+        }
+    if (!ws->law_tans) {
+        mwd_set_error(std::string(name) +
+                      ": call mwd_set_law_tangents first (the parameter derivatives of the laws)");
+        return MWD_E_INVALID;
+    }
+    return alloc_adjoint(ws);
+}
+
+MWD_API int mwd_step_adjoint(mwd_workspace* ws, mwd_state* st, double* const* lam, double* dtheta) {
+    int rc = adjoint_args(ws, st, lam, dtheta, "mwd_step_adjoint");
+    if (rc) return rc;
+    rc = mwd_candidates(ws, st, nullptr);
     if (rc) return rc;
     ws->emit_faces = 1;
     rc = fine_contact(ws, st, nullptr);
     ws->emit_faces = 0;
     if (rc) return rc;
+    rc = mwd_launch_step_adjoint(ws, st, state_view(ws, st, 0), state_view(ws, st, 1), lam, dtheta);
+    ws->bkt_dirty = 0;  // the gather emptied the buckets
+    return rc;
+}
+
+MWD_API int mwd_tape_init(mwd_workspace* ws, int64_t n_slots, int64_t* slot_bytes) {
+    if (!ws || n_slots < 0) {
+        mwd_set_error("mwd_tape_init: null workspace or negative slot count");
+        return MWD_E_INVALID;
+    }
+    int rc = adjoint_ok(ws, "mwd_tape_init");
+    if (rc || (rc = alloc_adjoint(ws))) return rc;
+    const int64_t words = mwd_tape_words(ws);
+    if (slot_bytes) *slot_bytes = 4 * words;
+    MWD_HIP_CHECK(hipSetDevice(ws->device));
+    MWD_HIP_CHECK(hipStreamSynchronize(ws->stream));
+    if (ws->tape) MWD_HIP_CHECK(hipFree(ws->tape));
+    ws->tape = nullptr;
+    ws->tape_slots = ws->tape_words = 0;
+    if (n_slots > 0) {
+        MWD_HIP_CHECK(hipMalloc((void**)&ws->tape, sizeof(uint32_t) * (size_t)words * (size_t)n_slots));
+        ws->tape_slots = n_slots;
+        ws->tape_words = words;
+    }
+    if ((rc = mwd_launch_tape_rewind(ws))) return rc;
+    MWD_HIP_CHECK(hipStreamSynchronize(ws->stream));
+    return MWD_OK;
+}
+
+MWD_API int mwd_tape_rewind(mwd_workspace* ws) {
+    if (!ws) return MWD_E_INVALID;
+    return mwd_launch_tape_rewind(ws);
+}
+
+MWD_API int mwd_step_record(mwd_workspace* ws, mwd_state* st) {
+    if (!ws || !st || !ws->tape) {
+        mwd_set_error("mwd_step_record: invalid argument, or no tape (mwd_tape_init)");
+        return MWD_E_INVALID;
+    }
+    int rc = adjoint_ok(ws, "mwd_step_record");
+    if (rc || (rc = mwd_candidates(ws, st, nullptr))) return rc;
+    ws->emit_faces = 1;
+    rc = fine_contact(ws, st, nullptr);
+    ws->emit_faces = 0;
+    if (rc || (rc = mwd_launch_tape(ws, true)) || (rc = mwd_reduce_verlet(ws, st))) return rc;
+    if (ws->wall_motion && (rc = mwd_launch_move_walls(ws, st))) return rc;
+    mwd_mark_end(ws);
+    return MWD_OK;
+}
+
+MWD_API int mwd_step_adjoint_taped(mwd_workspace* ws, mwd_state* st, double* const* lam, double* dtheta) {
+    int rc = adjoint_args(ws, st, lam, dtheta, "mwd_step_adjoint_taped");
+    if (rc) return rc;
+    if (!ws->tape) {
+        mwd_set_error("mwd_step_adjoint_taped: no tape (mwd_tape_init)");
+        return MWD_E_INVALID;
+    }
+    if ((rc = mwd_launch_tape(ws, false))) return rc;
     rc = mwd_launch_step_adjoint(ws, st, state_view(ws, st, 0), state_view(ws, st, 1), lam, dtheta);
     ws->bkt_dirty = 0;  // the gather emptied the buckets
     return rc;

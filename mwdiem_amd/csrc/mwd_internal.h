@@ -295,6 +295,10 @@ struct mwd_workspace {
     double* adj_cth;               // [3*contact_cap] per contact: d/d theta
     double* adj_rowth;             // [3*n_rows] per moving row: its contacts' share of d/d theta
     int emit_faces;                // host: the next fine-contact pass writes adj_face
+    // the reverse sweep's tape (mwd_tape_init): per recorded step, the
+    // contact stages' outputs the adjoint step reads; counters[18] slots held
+    uint32_t* tape;                // [tape_slots * tape_words]
+    int64_t tape_slots, tape_words;
     // per moving row, the records (2c for side A of contact c, 2c+1 for side
     // B) of this step's contacts, in arrival order; the reduction sorts them
     int32_t* rec_cnt;          // [n_rows], zero between steps
@@ -330,7 +334,8 @@ struct mwd_workspace {
     // overflow records, 12-13 slots filled in the tangential history tables,
     // 14 moving rows active now (m_live), 15 wall slab tests now (wt_live),
     // 16 steps taken since the spawn schedule was set, 17 wall pairs listed
-    // for GJK (the rest are separated by the wall's slab)
+    // for GJK (the rest are separated by the wall's slab), 18 steps held on
+    // the reverse sweep's tape, 19 blocks of a tape launch done
     int64_t* counters;
     int64_t* m_live;           // counters + 14: every kernel over the moving rows reads its count here
     int64_t* wt_live;          // counters + 15
@@ -416,6 +421,12 @@ int mwd_launch_copy_arrays(int n, void* const* dst, const void* const* src, cons
 // state (adj_face written): lam in/out, dtheta (device, 3) accumulated.
 int mwd_launch_step_adjoint(mwd_workspace* ws, const mwd_state* st, const BodyView& A, const BodyView& B,
                             double* const* lam, double* dtheta);
+// The tape: words per slot for this workspace; push the contact stages'
+// outputs of the step just run (push = true) or pop the last recorded step's
+// back into the workspace arrays; empty it.
+int64_t mwd_tape_words(const mwd_workspace* ws);
+int mwd_launch_tape(mwd_workspace* ws, bool push);
+int mwd_launch_tape_rewind(mwd_workspace* ws);
 int mwd_launch_cell_keys(const double* com, int64_t n, const double* cell, int64_t ncx, int64_t ncy,
                          int64_t* keys, hipStream_t s);
 // GJK over pairs [0, n) (n from n_dev), in two launches: a bounding-sphere

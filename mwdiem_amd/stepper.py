@@ -288,19 +288,60 @@ class Stepper:
         _lib.check(_lib.load().mwd_step_adjoint(self.ws, C.byref(self.state), lam_ptrs, dtheta.data_ptr()),
                    "mwd_step_adjoint")
 
-    def capture_adjoint(self, lam_ptrs, dtheta: torch.Tensor):
-        """A HIP graph of one step_adjoint on these adjoint arrays (capture
-        runs nothing). step_adjoint must have run once before (its first call
-        allocates)."""
+    def capture_adjoint(self, lam_ptrs, dtheta: torch.Tensor, taped: bool = False):
+        """A HIP graph of one step_adjoint (step_adjoint_taped if taped) on
+        these adjoint arrays (capture runs nothing). step_adjoint or tape_init
+        must have run before (they allocate)."""
         self._enter()
         torch.cuda.synchronize(self.device)
         g = torch.cuda.CUDAGraph()
         lib = _lib.load()
+        fn = "mwd_step_adjoint_taped" if taped else "mwd_step_adjoint"
         with torch.cuda.graph(g, stream=self.stream):
-            _lib.check(lib.mwd_step_adjoint(self.ws, C.byref(self.state), lam_ptrs, dtheta.data_ptr()),
-                       "mwd_step_adjoint")
+            _lib.check(getattr(lib, fn)(self.ws, C.byref(self.state), lam_ptrs, dtheta.data_ptr()), fn)
         torch.cuda.synchronize(self.device)
         return g
+
+    # ------------------------------------------------------------ the tape
+    def tape_init(self, n_slots: int) -> int:
+        """Room for n_slots recorded steps on the reverse sweep's tape (0
+        frees it); returns the bytes of one slot. The tape starts empty."""
+        self._enter()
+        b = C.c_int64()
+        _lib.check(_lib.load().mwd_tape_init(self.ws, int(n_slots), C.byref(b)), "mwd_tape_init")
+        return int(b.value)
+
+    def tape_rewind(self):
+        """Empty the tape (stream-ordered, on the device)."""
+        self._enter()
+        _lib.check(_lib.load().mwd_tape_rewind(self.ws), "mwd_tape_rewind")
+
+    def step_record(self, n: int = 1):
+        """n steps (values as step()), each pushing its contact-stage outputs
+        onto the tape."""
+        lib = _lib.load()
+        self._enter()
+        for _ in range(n):
+            _lib.check(lib.mwd_step_record(self.ws, C.byref(self.state)), "mwd_step_record")
+        self.steps_done += n
+
+    def capture_record(self):
+        """A HIP graph of one step_record (capture runs nothing)."""
+        self._enter()
+        torch.cuda.synchronize(self.device)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=self.stream):
+            _lib.check(_lib.load().mwd_step_record(self.ws, C.byref(self.state)), "mwd_step_record")
+        torch.cuda.synchronize(self.device)
+        return g
+
+    def step_adjoint_taped(self, lam_ptrs, dtheta: torch.Tensor):
+        """step_adjoint of the step last recorded on the tape (its input state
+        restored in the stepper), its contact-stage outputs taken from the
+        tape."""
+        self._enter()
+        _lib.check(_lib.load().mwd_step_adjoint_taped(self.ws, C.byref(self.state), lam_ptrs, dtheta.data_ptr()),
+                   "mwd_step_adjoint_taped")
 
     def enable_tangents(self, seed=None):
         """Carry d(state)/d(theta) for three law parameters theta_k.
@@ -358,6 +399,9 @@ class Stepper:
         if f & _lib.F_ROW_CONTACTS:
             raise _lib.MwdError(_lib.MWD_E_CAPACITY, "step",
                                 "the contact-record overflow list is full; rerun with a larger contact_cap")
+        if f & _lib.F_TAPE:
+            raise _lib.MwdError(_lib.MWD_E_CAPACITY, "step",
+                                "a recorded step found the tape full, or a taped adjoint step found it empty")
         return s
 
     # ---------------------------------------------------------------- views

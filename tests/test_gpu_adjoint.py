@@ -115,6 +115,53 @@ def test_adjoint_gradient_matches_tangents(gpu):
     assert float(out) == L_t and np.allclose(th.grad.numpy(), g_a, rtol=1e-12, atol=0)
 
 
+def test_taped_sweep_equals_the_rerun_sweep(gpu):
+    """The reverse sweep whose recomputed steps keep their contact-stage
+    outputs on the tape gives the gradient of the sweep that reruns the
+    contact stages in each adjoint step, bit for bit: eager, and with every
+    step replayed from graphs; 200 steps of the 3 m/s pour from its first
+    impacts, checkpoints every 64 steps (a short last segment)."""
+    from mwdiem_amd.adjoint import adjoint_run
+    sc, st = _state_with_contacts()
+    theta = np.array([1e4, 0.5, 0.3])
+    L0, g0 = adjoint_run(sc, st, theta, 200, every=64, tape=False)
+    L1, g1 = adjoint_run(sc, st, theta, 200, every=64, tape=True)
+    print(f"outcome {L0}; rerun {g0} taped {g1}")
+    assert np.all(np.isfinite(g0)) and np.any(g0 != 0)
+    assert L1 == L0 and np.array_equal(g1, g0), (g1, g0)
+    L2, g2 = adjoint_run(sc, st, theta, 200, every=64, steps_per_graph=8, tape=True)
+    assert L2 == L0 and np.array_equal(g2, g0), (g2, g0)
+
+
+def test_tape_full_and_empty_raise(gpu):
+    """A recorded step on a full tape and a taped adjoint step on an empty one
+    copy nothing and raise the tape flag, which check() reports; the tape's
+    slot count follows pushes and pops."""
+    from mwdiem_amd import _lib
+    from mwdiem_amd.stepper import Stepper
+    sc, st = _state_with_contacts()
+    S = Stepper(sc)
+    S.set_state(st)
+    S.set_law_seeds()
+    assert S.tape_init(2) > 0
+    S.step_record(2)
+    S.check()
+    S.step_record(1)
+    with pytest.raises(_lib.MwdError, match="tape"):
+        S.check()
+    _lib.check(_lib.load().mwd_clear_flags(S.ws), "mwd_clear_flags")
+    lt = {k: torch.zeros_like(S.t[k]) for k in _lib.TAN_ARRAYS}
+    ptrs = (C.c_void_p * len(lt))(*[lt[k].data_ptr() for k in _lib.TAN_ARRAYS])
+    dth = torch.zeros(3, dtype=torch.float64, device=S.device)
+    S.step_adjoint_taped(ptrs, dth)
+    S.step_adjoint_taped(ptrs, dth)
+    S.check()
+    S.step_adjoint_taped(ptrs, dth)
+    with pytest.raises(_lib.MwdError, match="tape"):
+        S.check()
+    S.close()
+
+
 def test_adjoint_refuses_the_reference_law(gpu, dense_scene):
     from mwdiem_amd import _lib
     from mwdiem_amd.stepper import Stepper
