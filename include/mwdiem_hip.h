@@ -436,9 +436,9 @@ int mwd_step_adjoint(mwd_workspace* ws, mwd_state* state, double* const* lam, do
 /* The reverse sweep's tape (no counterpart in the reference). A sweep with
  * checkpoints recomputes each segment's steps before reversing them; with the
  * tape, the recomputed steps keep what the adjoint step reads of their
- * contact stages (the contact counts, the per-particle contact records, the
- * candidate pairs and their contact slots, and per contact its pair, depth,
- * normal, contact points, force, torques and EPA's final face), so that the
+ * contact stages (the contact counts, the per-particle contact records, and
+ * per contact its pair, the pair's rows, depth, normal, contact points,
+ * force, torques and EPA's final face), so that the
  * adjoint step takes them back instead of rerunning the contact stages.
  *
  * mwd_tape_init: room for n_slots steps (one slot: a fixed size from the

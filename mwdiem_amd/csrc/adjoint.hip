@@ -106,33 +106,88 @@ __device__ __forceinline__ void add3(double* p, d3 a) {
 }
 
 // ------------------------------------------------------------ force sums
-__global__ void __launch_bounds__(256) k_force_sum(const int32_t* __restrict__ plist, int64_t M,
-                                                   const int32_t* __restrict__ rec_cnt,
-                                                   const int32_t* __restrict__ rec_bkt,
-                                                   const int32_t* __restrict__ rec_ovf, const int64_t* n_ovf,
-                                                   int64_t ovf_cap, const int32_t* __restrict__ sidx,
-                                                   const double* __restrict__ Fn,
-                                                   const double* __restrict__ tA, const double* __restrict__ tB,
-                                                   const double* __restrict__ fext, const double* __restrict__ text,
-                                                   double* __restrict__ F, double* __restrict__ T) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M) return;
-    const int64_t P = plist[i];
-    d3 f = z3(), t = z3();
-    for_row_records(P, rec_cnt[P], rec_bkt, rec_ovf, n_ovf, ovf_cap, [&](int32_t r) {
-        const int64_t c = sidx[r >> 1];
-        if (r & 1) {
-            f = f + neg3(ld3(Fn + 3 * c));
-            t = t + ld3(tB + 3 * c);
-        } else {
-            f = f + ld3(Fn + 3 * c);
-            t = t + ld3(tA + 3 * c);
+// Per-row kernels over contact records give a row GL lanes: the lanes load
+// the row's records (its bucket, when it did not overflow) and what each
+// record brings, in parallel, and park them in LDS at the record's rank
+// (the records below it); then sums run over the ranks in ascending order,
+// one lane per summed component, adding in exactly the order of a walk over
+// the sorted records. A row whose bucket overflowed takes that walk on one
+// lane (for_row_records). GROWS rows per 64-lane block.
+constexpr int GL = 16;
+constexpr int GROWS = 4;
+constexpr int GB = GL * GROWS;
+constexpr int RC = MWD_ROW_CONTACTS;
+
+// The row's bucket into rec[0, n) and each lane's records' ranks (n <= RC);
+// call with every lane of the block (it holds a barrier).
+__device__ __forceinline__ void group_bucket(const int32_t* __restrict__ b, int n, bool grouped, int lane,
+                                             int32_t* rec) {
+    if (grouped)
+        for (int j = lane; j < n; j += GL) rec[j] = b[j];
+    __syncthreads();
+}
+__device__ __forceinline__ int rank_in(const int32_t* rec, int n, int32_t r) {
+    int k = 0;
+    for (int j = 0; j < n; ++j) k += rec[j] < r ? 1 : 0;
+    return k;
+}
+
+__global__ void __launch_bounds__(GB) k_force_sum(const int32_t* __restrict__ plist, int64_t M,
+                                                  const int32_t* __restrict__ rec_cnt,
+                                                  const int32_t* __restrict__ rec_bkt,
+                                                  const int32_t* __restrict__ rec_ovf, const int64_t* n_ovf,
+                                                  int64_t ovf_cap, const int32_t* __restrict__ sidx,
+                                                  const double* __restrict__ Fn,
+                                                  const double* __restrict__ tA, const double* __restrict__ tB,
+                                                  const double* __restrict__ fext, const double* __restrict__ text,
+                                                  double* __restrict__ F, double* __restrict__ T) {
+    __shared__ int32_t s_rec[GROWS][RC];
+    __shared__ double s_val[GROWS][RC][6];  // the record's force, then its torque
+    const int g = threadIdx.x / GL, lane = threadIdx.x % GL;
+    const int64_t i = (int64_t)blockIdx.x * GROWS + g;
+    const bool live = i < M;
+    const int64_t P = live ? plist[i] : 0;
+    const int n = live ? rec_cnt[P] : 0;
+    const bool grouped = n <= RC;
+    group_bucket(rec_bkt + P * RC, n, grouped, lane, s_rec[g]);
+    if (grouped)
+        for (int j = lane; j < n; j += GL) {
+            const int32_t r = s_rec[g][j];
+            const int k = rank_in(s_rec[g], n, r);
+            const int64_t c = sidx[r >> 1];
+            const d3 f = ld3(Fn + 3 * c), t = ld3((r & 1) ? tB + 3 * c : tA + 3 * c);
+            double* o = s_val[g][k];
+            o[0] = (r & 1) ? -f.x : f.x;
+            o[1] = (r & 1) ? -f.y : f.y;
+            o[2] = (r & 1) ? -f.z : f.z;
+            o[3] = t.x;
+            o[4] = t.y;
+            o[5] = t.z;
         }
-    });
-    f = f + ld3(fext + 3 * P);
-    t = t + ld3(text + 3 * P);
-    st3(F + 3 * P, f);
-    st3(T + 3 * P, t);
+    __syncthreads();
+    if (!live) return;
+    if (grouped) {
+        if (lane < 6) {
+            double acc = 0.0;
+            for (int k = 0; k < n; ++k) acc += s_val[g][k][lane];
+            if (lane < 3) F[3 * P + lane] = acc + fext[3 * P + lane];
+            else T[3 * P + lane - 3] = acc + text[3 * P + lane - 3];
+        }
+    } else if (lane == 0) {
+        d3 f = z3(), t = z3();
+        for_row_records(P, n, rec_bkt, rec_ovf, n_ovf, ovf_cap, [&](int32_t r) {
+            const int64_t c = sidx[r >> 1];
+            if (r & 1) {
+                f = f + neg3(ld3(Fn + 3 * c));
+                t = t + ld3(tB + 3 * c);
+            } else {
+                f = f + ld3(Fn + 3 * c);
+                t = t + ld3(tA + 3 * c);
+            }
+        });
+        st3(F + 3 * P, f + ld3(fext + 3 * P));
+        st3(T + 3 * P, t + ld3(text + 3 * P));
+    }
 }
 
 // ---------------------------------------------------------- Verlet adjoint
@@ -506,42 +561,103 @@ __global__ void __launch_bounds__(256) k_contact_adj(BodyView A, BodyView B, con
 }
 
 // ------------------------------------------------------------- gathering
-__global__ void __launch_bounds__(256) k_gather_adj(const int32_t* __restrict__ plist, int64_t M,
-                                                    int32_t* __restrict__ rec_cnt,
-                                                    const int32_t* __restrict__ rec_bkt,
-                                                    const int32_t* __restrict__ rec_ovf, const int64_t* n_ovf,
-                                                    int64_t ovf_cap, const int32_t* __restrict__ sidx,
-                                                    const double* __restrict__ cadj,
-                                                    const int32_t* __restrict__ face, Lam lam, int vmax,
-                                                    const int32_t* __restrict__ pairs,
-                                                    const uint8_t* __restrict__ is_wall,
-                                                    const double* __restrict__ cdth, double* __restrict__ rowth) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M) return;
-    const int64_t P = plist[i];
-    const int n = rec_cnt[P];
-    // the row's share of d/d theta: the contacts it holds the A side of (the
-    // B side when A is a wall), in the row's record order
-    double th[MWD_N_TAN] = {};
-    if (n <= 0) {
-        for (int k = 0; k < MWD_N_TAN; ++k) rowth[MWD_N_TAN * P + k] = 0.0;
-        return;
+// Per moving row (GL lanes, as k_force_sum): its contacts' contributions
+// added in the row's record order -- to its com, v and w adjoints (one lane
+// per component, from the value there), to its share of d/d theta (the
+// contacts it holds the A side of, the B side when A is a wall; from zero),
+// and to the adjoints of the face vertices (one lane per vertex the faces
+// name, at its first mention, adding every mention in order); the bucket
+// emptied. The same additions in the same order as a walk over the sorted
+// records, so the same bits.
+constexpr int GO = 21;  // per record in LDS: com, v, w, three face vertices, d/d theta
+__global__ void __launch_bounds__(GB) k_gather_adj(const int32_t* __restrict__ plist, int64_t M,
+                                                   int32_t* __restrict__ rec_cnt,
+                                                   const int32_t* __restrict__ rec_bkt,
+                                                   const int32_t* __restrict__ rec_ovf, const int64_t* n_ovf,
+                                                   int64_t ovf_cap, const int32_t* __restrict__ sidx,
+                                                   const double* __restrict__ cadj,
+                                                   const int32_t* __restrict__ face, Lam lam, int vmax,
+                                                   const int32_t* __restrict__ pairs,
+                                                   const uint8_t* __restrict__ is_wall,
+                                                   const double* __restrict__ cdth, double* __restrict__ rowth) {
+    __shared__ int32_t s_rec[GROWS][RC];
+    __shared__ double s_o[GROWS][RC][GO];
+    __shared__ int32_t s_f[GROWS][RC][4];  // the side's three face vertices; whether d/d theta counts
+    const int g = threadIdx.x / GL, lane = threadIdx.x % GL;
+    const int64_t i = (int64_t)blockIdx.x * GROWS + g;
+    const bool live = i < M;
+    const int64_t P = live ? plist[i] : 0;
+    const int n = live ? rec_cnt[P] : 0;
+    const bool grouped = n <= RC;
+    group_bucket(rec_bkt + P * RC, n, grouped, lane, s_rec[g]);
+    if (grouped)
+        for (int j = lane; j < n; j += GL) {
+            const int32_t r = s_rec[g][j];
+            const int k = rank_in(s_rec[g], n, r);
+            const int64_t c = sidx[r >> 1];
+            const int side = r & 1;
+            const double* o = cadj + (2 * c + side) * CADJ;
+            for (int t = 0; t < CADJ; ++t) s_o[g][k][t] = o[t];
+            for (int t = 0; t < MWD_N_TAN; ++t) s_o[g][k][CADJ + t] = cdth[MWD_N_TAN * c + t];
+            const int32_t* f = face + 7 * c + 3 * side;
+            for (int t = 0; t < 3; ++t) s_f[g][k][t] = f[t];
+            s_f[g][k][3] = side == (is_wall[pairs[2 * (int64_t)(r >> 1)]] ? 1 : 0);
+        }
+    __syncthreads();
+    if (!live) return;
+    if (grouped) {
+        if (lane < 9) {
+            double* dst = (lane < 3 ? lam.com : lane < 6 ? lam.v : lam.w) + 3 * P + lane % 3;
+            double acc = *dst;
+            for (int k = 0; k < n; ++k) acc += s_o[g][k][lane];
+            if (n > 0) *dst = acc;
+        } else if (lane < 9 + MWD_N_TAN) {
+            const int t = lane - 9;
+            double acc = 0.0;
+            for (int k = 0; k < n; ++k)
+                if (s_f[g][k][3]) acc += s_o[g][k][CADJ + t];
+            rowth[MWD_N_TAN * i + t] = acc;
+        }
+        // face vertices: mention e = 3k + s is vertex s of the face of the
+        // record ranked k
+        for (int e = lane; e < 3 * n; e += GL) {
+            const int v = s_f[g][e / 3][e % 3];
+            if (v < 0 || v >= vmax) continue;
+            bool first = true;
+            for (int e2 = 0; e2 < e && first; ++e2) first = s_f[g][e2 / 3][e2 % 3] != v;
+            if (!first) continue;
+            double* dst = lam.verts + 3 * ((int64_t)vmax * P + v);
+            double a0 = dst[0], a1 = dst[1], a2 = dst[2];
+            for (int e2 = e; e2 < 3 * n; ++e2)
+                if (s_f[g][e2 / 3][e2 % 3] == v) {
+                    const double* o = s_o[g][e2 / 3] + 9 + 3 * (e2 % 3);
+                    a0 += o[0];
+                    a1 += o[1];
+                    a2 += o[2];
+                }
+            dst[0] = a0;
+            dst[1] = a1;
+            dst[2] = a2;
+        }
+        if (lane == 0) rec_cnt[P] = 0;
+    } else if (lane == 0) {
+        double th[MWD_N_TAN] = {};
+        for_row_records(P, n, rec_bkt, rec_ovf, n_ovf, ovf_cap, [&](int32_t r) {
+            const int64_t c = sidx[r >> 1];
+            const int side = r & 1;
+            if (side == (is_wall[pairs[2 * (int64_t)(r >> 1)]] ? 1 : 0))
+                for (int k = 0; k < MWD_N_TAN; ++k) th[k] += cdth[MWD_N_TAN * c + k];
+            const double* o = cadj + (2 * c + side) * CADJ;
+            add3(lam.com + 3 * P, ld3(o));
+            add3(lam.v + 3 * P, ld3(o + 3));
+            add3(lam.w + 3 * P, ld3(o + 6));
+            const int32_t* f = face + 7 * c + 3 * side;
+            for (int k = 0; k < 3; ++k)
+                if (f[k] >= 0 && f[k] < vmax) add3(lam.verts + 3 * ((int64_t)vmax * P + f[k]), ld3(o + 9 + 3 * k));
+        });
+        rec_cnt[P] = 0;
+        for (int k = 0; k < MWD_N_TAN; ++k) rowth[MWD_N_TAN * i + k] = th[k];
     }
-    for_row_records(P, n, rec_bkt, rec_ovf, n_ovf, ovf_cap, [&](int32_t r) {
-        const int64_t c = sidx[r >> 1];
-        const int side = r & 1;
-        if (side == (is_wall[pairs[2 * (int64_t)(r >> 1)]] ? 1 : 0))
-            for (int k = 0; k < MWD_N_TAN; ++k) th[k] += cdth[MWD_N_TAN * c + k];
-        const double* o = cadj + (2 * c + side) * CADJ;
-        add3(lam.com + 3 * P, ld3(o));
-        add3(lam.v + 3 * P, ld3(o + 3));
-        add3(lam.w + 3 * P, ld3(o + 6));
-        const int32_t* f = face + 7 * c + 3 * side;
-        for (int k = 0; k < 3; ++k)
-            if (f[k] >= 0 && f[k] < vmax) add3(lam.verts + 3 * ((int64_t)vmax * P + f[k]), ld3(o + 9 + 3 * k));
-    });
-    rec_cnt[P] = 0;
-    for (int k = 0; k < MWD_N_TAN; ++k) rowth[MWD_N_TAN * P + k] = th[k];
 }
 
 // The rows' shares of d/d theta, summed by one block in a fixed order (thread
@@ -553,8 +669,9 @@ __global__ void __launch_bounds__(TSUM) k_theta_sum(const double* __restrict__ r
                                                     double* __restrict__ dtheta) {
     __shared__ double s[MWD_N_TAN][TSUM];
     double acc[MWD_N_TAN] = {};
+#pragma unroll 8
     for (int64_t i = threadIdx.x; i < M; i += TSUM)
-        for (int k = 0; k < MWD_N_TAN; ++k) acc[k] += rowth[MWD_N_TAN * (int64_t)plist[i] + k];
+        for (int k = 0; k < MWD_N_TAN; ++k) acc[k] += rowth[MWD_N_TAN * i + k];
     for (int k = 0; k < MWD_N_TAN; ++k) s[k][threadIdx.x] = acc[k];
     __syncthreads();
     for (int h = TSUM / 2; h > 0; h >>= 1) {
@@ -566,21 +683,16 @@ __global__ void __launch_bounds__(TSUM) k_theta_sum(const double* __restrict__ r
         for (int k = 0; k < MWD_N_TAN; ++k) dtheta[k] += s[k][0];
 }
 
-unsigned blocks_of(int64_t n, int64_t cap) {
-    int64_t b = (n + 255) / 256;
-    if (b < 1) b = 1;
-    if (b > cap) b = cap;
-    return (unsigned)b;
-}
-
 // ------------------------------------------------------------ the tape
 // The reverse sweep's tape holds, per recorded step, what the adjoint step
 // reads of the contact stages' outputs: the contact counts, the per-row
-// records, the candidate pairs and their contact slots, and per contact its
-// pair, depth, normal, contact points, force, torques and EPA's final face.
-// A slot is a header of the three counts (pairs, contacts, overflow records;
-// int64) and then each array at a 256-byte boundary; an array keeps the
-// prefix its count covers (all of it for the per-row arrays).
+// records, and per contact its pair, the pair's two rows, depth, normal,
+// contact points, force, torques and EPA's final face. A slot is a header of
+// the three counts (pairs, contacts, overflow records; int64) and then each
+// array at a 256-byte boundary; an array keeps the prefix its count covers
+// (all of it for the per-row arrays). The candidate pairs themselves are not
+// kept: a pop writes back, for each contact c of pair p, sidx[p] = c and the
+// pair's rows, which is all the adjoint step reads of them.
 constexpr int TAPE_MAX = 16;
 constexpr int TAPE_HEAD = 64;   // words (uint32) before the first array
 // the counters of the pairs, the contacts and the overflow records
@@ -592,6 +704,7 @@ struct TapeArr {
     int64_t unit;   // words per element
     int64_t cap;    // elements at most
     int cnt;        // -1: all cap elements; else the count (0 pairs, 1 contacts, 2 overflow records)
+    int rows;       // 1: per contact, the rows of its pair (ws: the pairs array)
 };
 
 struct TapeDesc {
@@ -601,6 +714,10 @@ struct TapeDesc {
     int64_t slot_words, slots;
     int64_t* ctr;      // workspace counters; [18] steps held, [19] blocks of this launch done
     uint32_t* flags;
+    int32_t* sidx;     // per candidate pair: its contact slot
+    const int32_t* cpair;  // per contact slot: its pair
+    int64_t cpair_off;     // the slot offset of the contacts' pairs
+    int64_t pair_cap;
 };
 
 // The slot layout of a workspace (base and slot count left to the caller);
@@ -608,22 +725,23 @@ struct TapeDesc {
 int64_t tape_layout(const mwd_workspace* ws, TapeDesc& d) {
     d = TapeDesc{};
     int64_t off = TAPE_HEAD;
-    auto add = [&](void* p, int64_t unit, int64_t cap, int cnt) {
+    auto add = [&](void* p, int64_t unit, int64_t cap, int cnt, int rows = 0) {
         TapeArr& t = d.a[d.n++];
         t.ws = (uint32_t*)p;
         t.off = off;
         t.unit = unit;
         t.cap = cap;
         t.cnt = cnt;
+        t.rows = rows;
         off += (unit * cap + 63) / 64 * 64;
     };
-    const int64_t C = ws->contact_cap, P = ws->pair_cap;
+    const int64_t C = ws->contact_cap;
     add(ws->rec_cnt, 1, ws->n_rows, -1);
     add(ws->rec_bkt, MWD_ROW_CONTACTS, ws->n_rows, -1);
     add(ws->rec_ovf, 2, 2 * C, 2);
-    add(ws->sidx, 1, P, 0);
-    add(ws->pairs, 2, P, 0);
+    d.cpair_off = off;
     add(ws->contact_pair, 1, C, 1);
+    add(ws->pairs, 2, C, 1, 1);
     add(ws->adj_face, 7, C, 1);
     add(ws->depth, 2, C, 1);
     add(ws->normal, 6, C, 1);
@@ -634,6 +752,9 @@ int64_t tape_layout(const mwd_workspace* ws, TapeDesc& d) {
     add(ws->tB, 6, C, 1);
     d.ctr = ws->counters;
     d.flags = ws->flags;
+    d.sidx = ws->sidx;
+    d.cpair = ws->contact_pair;
+    d.pair_cap = ws->pair_cap;
     return off;
 }
 
@@ -671,20 +792,37 @@ __global__ void __launch_bounds__(256) k_tape(TapeDesc d) {
             const int64_t c = s_cnt[t.cnt];
             n = c < 0 ? 0 : (c < n ? c : n);
         }
-        const int64_t words = n * t.unit;
-        uint32_t* dst = PUSH ? sw + t.off : t.ws;
-        const uint32_t* src = PUSH ? t.ws : sw + t.off;
-        // 16-byte moves (both ends 256-byte aligned), then the last words
-        const int64_t n4 = words >> 2;
+        const int64_t first = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
         const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride)
-            ((uint4*)dst)[i] = ((const uint4*)src)[i];
-        if (blockIdx.x == 0 && (int64_t)threadIdx.x < words - 4 * n4)
-            dst[4 * n4 + threadIdx.x] = src[4 * n4 + threadIdx.x];
+        if (t.rows) {  // the contacts' pair rows, gathered from or scattered to the pair list
+            int2* rw = (int2*)(sw + t.off);
+            int2* pr = (int2*)t.ws;
+            const int32_t* cp = PUSH ? d.cpair : (const int32_t*)(sw + d.cpair_off);
+            for (int64_t c = first; c < n; c += stride) {
+                const int32_t p = cp[c];
+                if (p < 0 || p >= d.pair_cap) continue;
+                if (PUSH) {
+                    rw[c] = pr[p];
+                } else {
+                    pr[p] = rw[c];
+                    d.sidx[p] = (int32_t)c;
+                }
+            }
+        } else {
+            const int64_t words = n * t.unit;
+            uint32_t* dst = PUSH ? sw + t.off : t.ws;
+            const uint32_t* src = PUSH ? t.ws : sw + t.off;
+            // 16-byte moves (both ends 256-byte aligned), then the last words
+            const int64_t n4 = words >> 2;
+            for (int64_t i = first; i < n4; i += stride) ((uint4*)dst)[i] = ((const uint4*)src)[i];
+            if (blockIdx.x == 0 && (int64_t)threadIdx.x < words - 4 * n4)
+                dst[4 * n4 + threadIdx.x] = src[4 * n4 + threadIdx.x];
+        }
     }
     __syncthreads();
+    // no fence: the slot count moves after every block has read it, and no
+    // block reads what another copies
     if (threadIdx.x == 0) {
-        __threadfence();
         const unsigned long long total = (unsigned long long)gridDim.x * gridDim.y;
         const unsigned long long done = atomicAdd((unsigned long long*)&d.ctr[19], 1ull);
         if (done == total - 1) {
@@ -718,7 +856,7 @@ int mwd_launch_tape(mwd_workspace* ws, bool push) {
         mwd_set_error("the tape is not allocated for this workspace (mwd_tape_init)");
         return MWD_E_INVALID;
     }
-    const dim3 grid(64, d.n);
+    const dim3 grid(16, d.n);
     if (push) hipLaunchKernelGGL(k_tape<true>, grid, dim3(256), 0, ws->stream, d);
     else hipLaunchKernelGGL(k_tape<false>, grid, dim3(256), 0, ws->stream, d);
     MWD_LAUNCH_CHECK();
@@ -741,21 +879,23 @@ int mwd_launch_step_adjoint(mwd_workspace* ws, const mwd_state* st, const BodyVi
             lam_arr[MWD_TAN_QALL], lam_arr[MWD_TAN_DXALL], lam_arr[MWD_TAN_VERTS]};
     const VState a = mwd_vstate(st, nullptr);
     if (M > 0) {
-        hipLaunchKernelGGL(k_force_sum, dim3(blocks_of(M, 1ll << 30)), dim3(256), 0, s, ws->plist, M, ws->rec_cnt,
+        hipLaunchKernelGGL(k_force_sum, dim3((unsigned)((M + GROWS - 1) / GROWS)), dim3(GB), 0, s, ws->plist, M,
+                           ws->rec_cnt,
                            ws->rec_bkt, ws->rec_ovf, ctr + 11, 2 * ws->contact_cap, ws->sidx, ws->Fn, ws->tA,
                            ws->tB, st->fext, st->text, ws->adj_F, ws->adj_T);
         MWD_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_verlet_adj, dim3(blocks_of(M, 1ll << 30)), dim3(256), 0, s, a, ws->adj_F, ws->adj_T, lam,
+        hipLaunchKernelGGL(k_verlet_adj, dim3((unsigned)((M + 63) / 64)), dim3(64), 0, s, a, ws->adj_F, ws->adj_T, lam,
                            ws->adj_lamF, ws->adj_lamT, ws->plist, M, ws->scene_of, ws->scenes_dev, st->nverts,
                            (int)st->vmax, ws->dt);
         MWD_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(k_contact_adj, dim3(blocks_of(ws->contact_cap, 2048)), dim3(256), 0, s, A, B, ws->laws,
+    hipLaunchKernelGGL(k_contact_adj, dim3((unsigned)std::min<int64_t>((ws->contact_cap + 63) / 64, 8192)), dim3(64), 0, s, A, B, ws->laws,
                        ws->law_tans, ws->contact_pair, ctr + 4, ws->contact_cap, ws->adj_face, ws->depth, ws->normal,
                        ws->cpA, ws->cpB, ws->adj_lamF, ws->adj_lamT, ws->adj_c, ws->adj_cth);
     MWD_LAUNCH_CHECK();
     if (M > 0) {
-        hipLaunchKernelGGL(k_gather_adj, dim3(blocks_of(M, 1ll << 30)), dim3(256), 0, s, ws->plist, M, ws->rec_cnt,
+        hipLaunchKernelGGL(k_gather_adj, dim3((unsigned)((M + GROWS - 1) / GROWS)), dim3(GB), 0, s, ws->plist, M,
+                           ws->rec_cnt,
                            ws->rec_bkt, ws->rec_ovf, ctr + 11, 2 * ws->contact_cap, ws->sidx, ws->adj_c,
                            ws->adj_face, lam, (int)st->vmax, ws->pairs, ws->is_wall, ws->adj_cth, ws->adj_rowth);
         MWD_LAUNCH_CHECK();

@@ -293,7 +293,7 @@ struct mwd_workspace {
     int32_t* adj_face;             // [7*contact_cap] EPA's final face per contact
     double* adj_c;                 // [2*18*contact_cap] per contact side: adjoint contributions
     double* adj_cth;               // [3*contact_cap] per contact: d/d theta
-    double* adj_rowth;             // [3*n_rows] per moving row: its contacts' share of d/d theta
+    double* adj_rowth;             // [3*n_rows] per moving-list entry: its row's contacts' share of d/d theta
     int emit_faces;                // host: the next fine-contact pass writes adj_face
     // the reverse sweep's tape (mwd_tape_init): per recorded step, the
     // contact stages' outputs the adjoint step reads; counters[18] slots held
