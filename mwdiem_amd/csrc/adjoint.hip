@@ -246,16 +246,28 @@ __global__ void __launch_bounds__(256) k_verlet_adj(VState a, const double* __re
     d3 lwbf = z3(), lw = z3();
     q4 lqn = qzero(), lq32 = qzero(), lq = qzero(), lq12 = qzero(), lqr = qzero();
     // vertices: nv = qr (verts - com) conj(qr) + com + dx (dx undone when out)
+    // (VCH vertices' loads issued together, ahead of their stores)
     const int nrot = nverts[P];
-    for (int k = 0; k < nrot; ++k) {
-        double* pl = lam.verts + 3 * ((int64_t)vmax * P + k);
-        const d3 L = ld3(pl);
-        const d3 r = ld3(a.verts.v + 3 * ((int64_t)vmax * P + k)) - com;
-        d3 lr = z3();
-        qrotc_adj(qr, r, L, lqr, lr);
-        st3(pl, lr);  // lambda of the vertex itself
-        lcom = lcom + (L - lr);
-        if (!out) ldx = ldx + L;
+    double* const plv = lam.verts + 3 * (int64_t)vmax * P;
+    const double* const pv = a.verts.v + 3 * (int64_t)vmax * P;
+    constexpr int VCH = 8;
+    for (int k0 = 0; k0 < nrot; k0 += VCH) {
+        d3 L[VCH], r[VCH];
+#pragma unroll
+        for (int j = 0; j < VCH; ++j)
+            if (k0 + j < nrot) {
+                L[j] = ld3(plv + 3 * (k0 + j));
+                r[j] = ld3(pv + 3 * (k0 + j)) - com;
+            }
+#pragma unroll
+        for (int j = 0; j < VCH; ++j)
+            if (k0 + j < nrot) {
+                d3 lr = z3();
+                qrotc_adj(qr, r[j], L[j], lqr, lr);
+                st3(plv + 3 * (k0 + j), lr);  // lambda of the vertex itself
+                lcom = lcom + (L[j] - lr);
+                if (!out) ldx = ldx + L[j];
+            }
     }
     lcom = lcom + Lcom;
     if (!out) ldx = ldx + Lcom;
@@ -345,6 +357,14 @@ __global__ void __launch_bounds__(256) k_contact_adj(BodyView A, BodyView B, con
         const int64_t ra = view_row(A, p), rb = view_row(B, p);
         const bool wallA = A.is_wall[ra], wallB = B.is_wall[rb];
         const int scene = A.scene ? A.scene[ra] : 0;
+        // the bodies' state, loaded with the face's vertices (used by the law below)
+        const d3 xA = ld3(A.com + 3 * ra), xB = ld3(B.com + 3 * rb);
+        const d3 vA = ld3(A.vel + 3 * ra), vB = ld3(B.vel + 3 * rb);
+        const d3 wA = ld3(A.angvel + 3 * ra), wB = ld3(B.angvel + 3 * rb);
+        const double imA = (A.mass && !wallA) ? 1.0 / A.mass[ra] : 0.0;
+        const double imB = (B.mass && !wallB) ? 1.0 / B.mass[rb] : 0.0;
+        const double iA = imA == 0.0 ? 0.0 : 1.0 / A.isize[ra];
+        const double iB = imB == 0.0 ? 0.0 : 1.0 / B.isize[rb];
         const int lw = (wallA || wallB) ? 1 : 0;
         const mwd_law& L = laws2[2 * scene + lw];
         const mwd_law_tan& LT = lawt2[2 * scene + lw];
@@ -396,13 +416,6 @@ __global__ void __launch_bounds__(256) k_contact_adj(BodyView A, BodyView B, con
             cpB = ld3(cpB_e + 3 * c);
         }
         // ---- law, forward (force_law<double>, physical branch, no tangential history)
-        const d3 xA = ld3(A.com + 3 * ra), xB = ld3(B.com + 3 * rb);
-        const d3 vA = ld3(A.vel + 3 * ra), vB = ld3(B.vel + 3 * rb);
-        const d3 wA = ld3(A.angvel + 3 * ra), wB = ld3(B.angvel + 3 * rb);
-        const double imA = (A.mass && !wallA) ? 1.0 / A.mass[ra] : 0.0;
-        const double imB = (B.mass && !wallB) ? 1.0 / B.mass[rb] : 0.0;
-        const double iA = imA == 0.0 ? 0.0 : 1.0 / A.isize[ra];
-        const double iB = imB == 0.0 ? 0.0 : 1.0 / B.isize[rb];
         const d3 cp = (cpA + cpB) / 2.0;
         const d3 dA = cp - xA, dB = cp - xB;
         const double imn = imA + imB;
