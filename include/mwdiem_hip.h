@@ -435,26 +435,30 @@ int mwd_step_adjoint(mwd_workspace* ws, mwd_state* state, double* const* lam, do
 
 /* The reverse sweep's tape (no counterpart in the reference). A sweep with
  * checkpoints recomputes each segment's steps before reversing them; with the
- * tape, the recomputed steps keep what the adjoint step reads of their
- * contact stages (the contact counts, the per-particle contact records, and
- * per contact its pair, the pair's rows, depth, normal, contact points,
- * force, torques and EPA's final face), so that the
- * adjoint step takes them back instead of rerunning the contact stages.
+ * tape, each recomputed step keeps its input state (the evolving arrays,
+ * radii when the state has them, wall centres) and what the adjoint step
+ * reads of its contact stages (the contact counts, the per-particle contact
+ * records, and per contact its pair, the pair's rows, depth, normal, contact
+ * points, force, torques and EPA's final face), so that the adjoint step puts
+ * the state back and takes the contacts back instead of rerunning the contact
+ * stages. The slot count lives on the device, so a whole segment of recorded
+ * steps, and of taped adjoint steps, can be captured in one graph each.
  *
  * mwd_tape_init: room for n_slots steps (one slot: a fixed size from the
- * workspace's capacities, returned in *slot_bytes when not NULL), allocated
- * on the device with the adjoint step's buffers; n_slots = 0 frees it. The
- * tape starts empty. Same conditions as mwd_step_adjoint. */
+ * workspace's rows, vmax and capacities, returned in *slot_bytes when not
+ * NULL), allocated on the device with the adjoint step's buffers; n_slots = 0
+ * frees it. The tape starts empty. Same conditions as mwd_step_adjoint. */
 int mwd_tape_init(mwd_workspace* ws, int64_t n_slots, int64_t* slot_bytes);
 /* Empty the tape (on the device, so it can be captured in a graph). */
 int mwd_tape_rewind(mwd_workspace* ws);
-/* mwd_step, its values bit for bit, also pushing the step's contact-stage
- * outputs onto the tape (MWD_F_TAPE when it is full). */
+/* mwd_step, its values bit for bit, also pushing the step's input state and
+ * contact-stage outputs onto the tape (MWD_F_TAPE when it is full). */
 int mwd_step_record(mwd_workspace* ws, mwd_state* state);
 /* mwd_step_adjoint of the step last recorded and not yet taken back: pops
- * its contact-stage outputs from the tape (MWD_F_TAPE when it is empty)
- * instead of rerunning the contact stages. state must hold that step's input
- * state. The same results as mwd_step_adjoint, bit for bit. */
+ * its input state into `state` and its contact-stage outputs into the
+ * workspace (MWD_F_TAPE when the tape is empty) instead of rerunning the
+ * contact stages. The same results as mwd_step_adjoint on that input state,
+ * bit for bit. */
 int mwd_step_adjoint_taped(mwd_workspace* ws, mwd_state* state, double* const* lam, double* dtheta);
 
 #ifdef __cplusplus

@@ -5,12 +5,12 @@ reverse sweep takes the segments last to first: it restores a segment's
 checkpoint, recomputes the segment's steps keeping each step's input state,
 then reverses those steps one at a time with mwd_step_adjoint -- the
 hand-written adjoint kernels of the Verlet step, the physical contact law
-and EPA's final-face geometry. The recomputed steps push their contact-stage
-outputs onto a device tape (mwd_step_record) and the adjoint steps pop them
-(mwd_step_adjoint_taped), so the topology is the forward's and the contact
-stages run once per recomputed step; without the tape they rerun on the
-restored state instead. Memory: the checkpoints plus one segment of states
-and tape slots. The result equals the forward-mode tangents'
+and EPA's final-face geometry. The recomputed steps push their input states
+and contact-stage outputs onto a device tape (mwd_step_record) and the
+adjoint steps pop them (mwd_step_adjoint_taped), so the topology is the
+forward's and the contact stages run once per recomputed step; without the
+tape the input states are copied aside and the contact stages rerun on the
+restored state. Memory: the checkpoints plus one segment of tape slots. The result equals the forward-mode tangents'
 (mwdiem_amd.diff) to rounding; the adjoint costs one recomputed forward
 step and one reverse step per step, whatever the number of parameters.
 """
@@ -42,11 +42,14 @@ def adjoint_run(sc, init, theta, steps: int, every: int = 100, top_frac: float =
     mu) from `init` (None, initial velocities or a state dict) and return
     (outcome, d outcome/d theta) by the adjoint. With steps_per_graph the
     forward and recomputed segments replay captured graphs. With tape (the
-    default) the recomputed steps keep their contact-stage outputs on the
-    device (one slot per step of a segment) and the adjoint steps take them
-    back instead of rerunning the contact stages; the gradient is the same bit
-    for bit. timings, if given, receives the forward and reverse wall times
-    (s); tuning: launch choices."""
+    default) the recomputed steps keep their input states and contact-stage
+    outputs on the device (one slot per step of a segment) and the adjoint
+    steps take them back instead of rerunning the contact stages; with
+    steps_per_graph a segment is then one graph of recorded steps and one of
+    adjoint steps. Without the tape, each step's input state is saved and put
+    back by copy launches and each adjoint step reruns the contact stages. The
+    gradient is the same bit for bit either way. timings, if given, receives
+    the forward and reverse wall times (s); tuning: launch choices."""
     import time
     S = Stepper(_with_theta(sc, theta))
     if tuning:
@@ -72,45 +75,65 @@ def adjoint_run(sc, init, theta, steps: int, every: int = 100, top_frac: float =
     lam["com"].copy_(g)
     ptrs = (C.c_void_p * len(_lib.TAN_ARRAYS))(*[lam[k].data_ptr() for k in _lib.TAN_ARRAYS])
     dth = torch.zeros(_lib.N_TAN, dtype=torch.float64, device=S.device)
-    # one segment's step inputs, saved and put back with one copy launch each;
-    # with steps_per_graph the recomputed step and the adjoint step replay
-    # graphs of one step each (captured once, on the stepper's fixed buffers)
-    live = [S.t[k] for k in SNAP_KEYS]
-    slots = [[torch.empty_like(t) for t in live] for _ in range(min(every, steps))]
-    if tape:
-        S.tape_init(min(every, steps))
     stream = S.stream.cuda_stream
-    g_step = g_adj = None
-    for j in reversed(range(len(ckpt))):
-        n0 = j * every
-        n1 = min(n0 + every, steps)
-        S.restore(ckpt[j])
-        if tape:
+    if tape:
+        # each recorded step pushes its input state and contact-stage outputs
+        # onto the device tape and each taped adjoint step pops them; with
+        # steps_per_graph both directions replay graphs of that many steps
+        # (captured once per length, on the stepper's fixed buffers)
+        S.tape_init(min(every, steps))
+        g_rec, g_adj = {}, {}
+
+        def replay(n, cache, make):
+            while n > 0:
+                k = min(steps_per_graph, n)
+                if k not in cache:
+                    cache[k] = make(k)
+                with torch.cuda.stream(S.stream):
+                    cache[k].replay()
+                n -= k
+        for j in reversed(range(len(ckpt))):
+            n = min(every, steps - j * every)
+            S.restore(ckpt[j])
             S.tape_rewind()
-        for i in range(n1 - n0):
-            _lib.copy_arrays(slots[i], live, stream)
             if steps_per_graph:
-                if g_step is None:
-                    g_step = S.capture_record() if tape else S.capture(1)
-                with torch.cuda.stream(S.stream):
-                    g_step.replay()
-            elif tape:
-                S.step_record(1)
+                replay(n, g_rec, S.capture_record)
+                replay(n, g_adj, lambda k: S.capture_adjoint(ptrs, dth, taped=True, steps=k))
             else:
-                S.step(1)
-        for i in reversed(range(n1 - n0)):
-            _lib.copy_arrays(live, slots[i], stream)
-            if g_adj is not None:
-                with torch.cuda.stream(S.stream):
-                    g_adj.replay()
-            else:
-                if tape:
+                S.step_record(n)
+                for _ in range(n):
                     S.step_adjoint_taped(ptrs, dth)
+        del g_rec, g_adj
+    else:
+        # one segment's step inputs, saved and put back with one copy launch
+        # each; with steps_per_graph the recomputed step and the adjoint step
+        # replay graphs of one step each
+        live = [S.t[k] for k in SNAP_KEYS]
+        slots = [[torch.empty_like(t) for t in live] for _ in range(min(every, steps))]
+        g_step = g_adj1 = None
+        for j in reversed(range(len(ckpt))):
+            n0 = j * every
+            n1 = min(n0 + every, steps)
+            S.restore(ckpt[j])
+            for i in range(n1 - n0):
+                _lib.copy_arrays(slots[i], live, stream)
+                if steps_per_graph:
+                    if g_step is None:
+                        g_step = S.capture(1)
+                    with torch.cuda.stream(S.stream):
+                        g_step.replay()
+                else:
+                    S.step(1)
+            for i in reversed(range(n1 - n0)):
+                _lib.copy_arrays(live, slots[i], stream)
+                if g_adj1 is not None:
+                    with torch.cuda.stream(S.stream):
+                        g_adj1.replay()
                 else:
                     S.step_adjoint(ptrs, dth)
-                if steps_per_graph:
-                    g_adj = S.capture_adjoint(ptrs, dth, taped=tape)
-    del slots
+                    if steps_per_graph:
+                        g_adj1 = S.capture_adjoint(ptrs, dth)
+        del slots
     S.check()
     torch.cuda.synchronize(S.device)
     t2 = time.perf_counter()

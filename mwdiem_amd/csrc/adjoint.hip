@@ -697,16 +697,26 @@ __global__ void __launch_bounds__(TSUM) k_theta_sum(const double* __restrict__ r
 }
 
 // ------------------------------------------------------------ the tape
-// The reverse sweep's tape holds, per recorded step, what the adjoint step
-// reads of the contact stages' outputs: the contact counts, the per-row
-// records, and per contact its pair, the pair's two rows, depth, normal,
-// contact points, force, torques and EPA's final face. A slot is a header of
+// The reverse sweep's tape holds, per recorded step, the step's input state
+// (the evolving state arrays, the bounding radii and the wall centres) and
+// what the adjoint step reads of the contact stages' outputs: the contact
+// counts, the per-row records, and per contact its pair, the pair's two rows,
+// depth, normal, contact points, force, torques and EPA's final face. A
+// recorded step pushes after its contact stages (which leave the state as it
+// came in); a taped adjoint step pops before its kernels. A slot is a header of
 // the three counts (pairs, contacts, overflow records; int64) and then each
 // array at a 256-byte boundary; an array keeps the prefix its count covers
 // (all of it for the per-row arrays). The candidate pairs themselves are not
 // kept: a pop writes back, for each contact c of pair p, sidx[p] = c and the
 // pair's rows, which is all the adjoint step reads of them.
-constexpr int TAPE_MAX = 16;
+constexpr int TAPE_MAX = 32;
+// Blocks finishing a tape launch count themselves in TAPE_SUB counters (by
+// block number), each on its own 128-byte line, and the last of each counter
+// in one more: same-address atomics serialise, so one counter for every
+// block would cost more than the copy.
+constexpr int TAPE_SUB = 16;
+constexpr int TAPE_SYNC_STRIDE = 32;
+static_assert((TAPE_SUB + 1) * TAPE_SYNC_STRIDE <= MWD_TAPE_SYNC_WORDS, "tape block counters");
 constexpr int TAPE_HEAD = 64;   // words (uint32) before the first array
 // the counters of the pairs, the contacts and the overflow records
 __device__ __forceinline__ int tape_ctr(int k) { return k == 0 ? 3 : (k == 1 ? 4 : 11); }
@@ -718,6 +728,7 @@ struct TapeArr {
     int64_t cap;    // elements at most
     int cnt;        // -1: all cap elements; else the count (0 pairs, 1 contacts, 2 overflow records)
     int rows;       // 1: per contact, the rows of its pair (ws: the pairs array)
+    int vec;        // the workspace array is 16-byte aligned (16-byte moves)
 };
 
 struct TapeDesc {
@@ -725,7 +736,8 @@ struct TapeDesc {
     int n;
     uint32_t* base;
     int64_t slot_words, slots;
-    int64_t* ctr;      // workspace counters; [18] steps held, [19] blocks of this launch done
+    int64_t* ctr;      // workspace counters; [18] steps held
+    uint32_t* sync;    // [TAPE_SUB + 1] * TAPE_SYNC_STRIDE: blocks of this launch done
     uint32_t* flags;
     int32_t* sidx;     // per candidate pair: its contact slot
     const int32_t* cpair;  // per contact slot: its pair
@@ -734,8 +746,9 @@ struct TapeDesc {
 };
 
 // The slot layout of a workspace (base and slot count left to the caller);
-// returns the words per slot.
-int64_t tape_layout(const mwd_workspace* ws, TapeDesc& d) {
+// returns the words per slot. st: the state whose arrays the slot holds
+// (NULL: the layout only).
+int64_t tape_layout(const mwd_workspace* ws, const mwd_state* st, TapeDesc& d) {
     d = TapeDesc{};
     int64_t off = TAPE_HEAD;
     auto add = [&](void* p, int64_t unit, int64_t cap, int cnt, int rows = 0) {
@@ -746,9 +759,25 @@ int64_t tape_layout(const mwd_workspace* ws, TapeDesc& d) {
         t.cap = cap;
         t.cnt = cnt;
         t.rows = rows;
+        t.vec = ((uintptr_t)p & 15) == 0;
         off += (unit * cap + 63) / 64 * 64;
     };
-    const int64_t C = ws->contact_cap;
+    const int64_t C = ws->contact_cap, N = ws->n_rows;
+    const mwd_state z{};
+    const mwd_state& a = st ? *st : z;
+    add(a.com, 6, N, -1);
+    add(a.v, 6, N, -1);
+    add(a.v12, 6, N, -1);
+    add(a.w, 6, N, -1);
+    add(a.wbf, 6, N, -1);
+    add(a.w12bf, 6, N, -1);
+    add(a.q, 8, N, -1);
+    add(a.q12, 8, N, -1);
+    add(a.qall, 8, N, -1);
+    add(a.dxall, 6, N, -1);
+    add(a.verts, 6 * (int64_t)ws->vmax, N, -1);
+    add(a.radius, 2, N, -1);  // (when the state has radii)
+    add(const_cast<double*>(a.wall_c), 6, N, -1);
     add(ws->rec_cnt, 1, ws->n_rows, -1);
     add(ws->rec_bkt, MWD_ROW_CONTACTS, ws->n_rows, -1);
     add(ws->rec_ovf, 2, 2 * C, 2);
@@ -771,113 +800,188 @@ int64_t tape_layout(const mwd_workspace* ws, TapeDesc& d) {
     return off;
 }
 
-// blockIdx.y: the array; the blocks along x split its words. PUSH copies the
-// workspace arrays into slot counters[18] and raises it; a pop lowers it and
-// copies that slot back (counts included). The last block to finish moves
-// the slot count, so every block has read it before it changes. A push onto
-// a full tape or a pop from an empty one copies nothing and raises
-// MWD_F_TAPE.
+// The live part of every array in quads (four 32-bit words; a contact pair's
+// two rows count as half a quad), cut into units of TAPE_UQ quads; the
+// blocks take the units in turn, each thread TAPE_QPT quads of a unit a block
+// apart, all its loads issued before its stores. PUSH copies the workspace
+// and state arrays into slot counters[18] and raises it; a pop lowers it and
+// copies that slot back (counts included). The last block to finish moves the
+// slot count, so every block has read it before it changes. A push onto a
+// full tape or a pop from an empty one copies nothing and raises MWD_F_TAPE.
+constexpr int TAPE_QPT = 4;
+constexpr int TAPE_UQ = 256 * TAPE_QPT;
+constexpr int TAPE_BLOCKS = 256;
+
 template <bool PUSH>
 __global__ void __launch_bounds__(256) k_tape(TapeDesc d) {
     __shared__ int64_t s_slot;
     __shared__ int64_t s_cnt[3];
-    if (threadIdx.x == 0) {
+    __shared__ int64_t s_u0[TAPE_MAX + 1];  // each array's first unit; [n]: all of them
+    __shared__ int64_t s_w[TAPE_MAX];       // its live words (pair rows: contacts)
+    __shared__ int64_t s_q[TAPE_MAX];       // its live quads
+    __shared__ TapeArr s_a[TAPE_MAX];
+#pragma unroll
+    for (int a = 0; a < TAPE_MAX; ++a)
+        if ((int)threadIdx.x == a && a < d.n) s_a[a] = d.a[a];
+    __syncthreads();
+    if (threadIdx.x < 64) {  // wave 0: the slot, the counts, and lane a sizes array a
+        const int lane = threadIdx.x;
         const int64_t held = *(volatile int64_t*)&d.ctr[18];
         const int64_t slot = PUSH ? held : held - 1;
-        s_slot = slot >= 0 && slot < d.slots ? slot : -1;
-        if (s_slot >= 0) {
-            const int64_t* head = (const int64_t*)(d.base + s_slot * d.slot_words);
-            for (int k = 0; k < 3; ++k) s_cnt[k] = PUSH ? d.ctr[tape_ctr(k)] : head[k];
+        const bool ok = slot >= 0 && slot < d.slots;
+        int64_t cnt[3] = {0, 0, 0};
+        if (ok) {
+            const int64_t* head = (const int64_t*)(d.base + slot * d.slot_words);
+            for (int k = 0; k < 3; ++k) cnt[k] = PUSH ? d.ctr[tape_ctr(k)] : head[k];
+        }
+        int64_t w = 0, q = 0;
+        if (lane < d.n && ok) {
+            const TapeArr& t = s_a[lane];
+            int64_t m = t.cap;
+            if (t.cnt >= 0) {
+                const int64_t c = t.cnt == 0 ? cnt[0] : (t.cnt == 1 ? cnt[1] : cnt[2]);
+                m = c < 0 ? 0 : (c < m ? c : m);
+            }
+            if (!t.ws) m = 0;  // a state without radii
+            w = t.rows ? m : m * t.unit;
+            q = t.rows ? (w + 1) / 2 : (w + 3) / 4;
+        }
+        const int64_t u = (q + TAPE_UQ - 1) / TAPE_UQ;
+        int64_t incl = u;  // inclusive scan of the units over the lanes
+        for (int o = 1; o < 64; o <<= 1) {
+            const int64_t v = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += v;
+        }
+        if (lane < d.n) {
+            s_w[lane] = w;
+            s_q[lane] = q;
+            s_u0[lane] = incl - u;
+        }
+        if (lane == d.n - 1) s_u0[d.n] = incl;
+        if (lane == 0) {
+            s_slot = ok ? slot : -1;
+            for (int k = 0; k < 3; ++k) s_cnt[k] = cnt[k];
         }
     }
     __syncthreads();
     const int64_t slot = s_slot;
     if (slot >= 0) {
         uint32_t* sw = d.base + slot * d.slot_words;
-        if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 3) {
+        if (blockIdx.x == 0 && threadIdx.x < 3) {
             int64_t* head = (int64_t*)sw;
             if (PUSH) head[threadIdx.x] = s_cnt[threadIdx.x];
             else d.ctr[tape_ctr(threadIdx.x)] = s_cnt[threadIdx.x];
         }
-        const TapeArr t = d.a[blockIdx.y];
-        int64_t n = t.cap;
-        if (t.cnt >= 0) {
-            const int64_t c = s_cnt[t.cnt];
-            n = c < 0 ? 0 : (c < n ? c : n);
-        }
-        const int64_t first = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-        const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-        if (t.rows) {  // the contacts' pair rows, gathered from or scattered to the pair list
-            int2* rw = (int2*)(sw + t.off);
-            int2* pr = (int2*)t.ws;
-            const int32_t* cp = PUSH ? d.cpair : (const int32_t*)(sw + d.cpair_off);
-            for (int64_t c = first; c < n; c += stride) {
-                const int32_t p = cp[c];
-                if (p < 0 || p >= d.pair_cap) continue;
-                if (PUSH) {
-                    rw[c] = pr[p];
-                } else {
-                    pr[p] = rw[c];
-                    d.sidx[p] = (int32_t)c;
-                }
+        const int n = d.n;
+        for (int64_t unit = blockIdx.x; unit < s_u0[n]; unit += gridDim.x) {
+            int lo = 0, hi = n - 1;  // the array holding this unit: the last with s_u0 <= unit
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_u0[mid] <= unit) lo = mid;
+                else hi = mid - 1;
             }
-        } else {
-            const int64_t words = n * t.unit;
+            const TapeArr& t = s_a[lo];
+            const int64_t w = s_w[lo], q = s_q[lo];
+            const int64_t q0 = (unit - s_u0[lo]) * TAPE_UQ + threadIdx.x;
+            if (t.rows) {  // contacts 2lq, 2lq+1: their pair rows, gathered from or scattered to the pair list
+                int2* rw = (int2*)(sw + t.off);
+                int2* pr = (int2*)t.ws;
+                const int32_t* cp = PUSH ? d.cpair : (const int32_t*)(sw + d.cpair_off);
+                for (int k = 0; k < TAPE_QPT; ++k)
+                    for (int64_t c = 2 * (q0 + k * 256); c < 2 * (q0 + k * 256) + 2 && c < w; ++c) {
+                        const int32_t p = cp[c];
+                        if (p >= 0 && p < d.pair_cap) {
+                            if (PUSH) {
+                                rw[c] = pr[p];
+                            } else {
+                                pr[p] = rw[c];
+                                d.sidx[p] = (int32_t)c;
+                            }
+                        }
+                    }
+                continue;
+            }
             uint32_t* dst = PUSH ? sw + t.off : t.ws;
             const uint32_t* src = PUSH ? t.ws : sw + t.off;
-            // 16-byte moves (both ends 256-byte aligned), then the last words
-            const int64_t n4 = words >> 2;
-            for (int64_t i = first; i < n4; i += stride) ((uint4*)dst)[i] = ((const uint4*)src)[i];
-            if (blockIdx.x == 0 && (int64_t)threadIdx.x < words - 4 * n4)
-                dst[4 * n4 + threadIdx.x] = src[4 * n4 + threadIdx.x];
+            uint4 val[TAPE_QPT];
+#pragma unroll
+            for (int k = 0; k < TAPE_QPT; ++k) {
+                const int64_t lq = q0 + k * 256;
+                val[k] = make_uint4(0, 0, 0, 0);
+                if (lq < q && t.vec && 4 * lq + 4 <= w) val[k] = ((const uint4*)src)[lq];
+            }
+#pragma unroll
+            for (int k = 0; k < TAPE_QPT; ++k) {
+                const int64_t lq = q0 + k * 256;
+                if (lq >= q) continue;
+                if (t.vec && 4 * lq + 4 <= w) {  // the slot's arrays start on 256-byte boundaries
+                    ((uint4*)dst)[lq] = val[k];
+                } else {
+                    for (int64_t j = 4 * lq; j < 4 * lq + 4 && j < w; ++j) dst[j] = src[j];
+                }
+            }
         }
     }
     __syncthreads();
     // no fence: the slot count moves after every block has read it, and no
     // block reads what another copies
     if (threadIdx.x == 0) {
-        const unsigned long long total = (unsigned long long)gridDim.x * gridDim.y;
-        const unsigned long long done = atomicAdd((unsigned long long*)&d.ctr[19], 1ull);
-        if (done == total - 1) {
-            d.ctr[19] = 0;
-            if (slot >= 0) d.ctr[18] += PUSH ? 1 : -1;
-            else atomicOr(d.flags, MWD_F_TAPE);
+        const unsigned nb = gridDim.x;
+        const unsigned sub = blockIdx.x % TAPE_SUB;
+        const unsigned in_sub = nb / TAPE_SUB + (sub < nb % TAPE_SUB ? 1u : 0u);
+        uint32_t* cs = d.sync + sub * TAPE_SYNC_STRIDE;
+        if (atomicAdd(cs, 1u) == in_sub - 1) {
+            *cs = 0;
+            uint32_t* ct = d.sync + TAPE_SUB * TAPE_SYNC_STRIDE;
+            const unsigned subs = nb < (unsigned)TAPE_SUB ? nb : (unsigned)TAPE_SUB;
+            if (atomicAdd(ct, 1u) == subs - 1) {
+                *ct = 0;
+                if (slot >= 0) d.ctr[18] += PUSH ? 1 : -1;
+                else atomicOr(d.flags, MWD_F_TAPE);
+            }
         }
     }
 }
 
-__global__ void k_tape_rewind(int64_t* ctr) {
-    if (threadIdx.x == 0) {
-        ctr[18] = 0;
-        ctr[19] = 0;
-    }
+__global__ void k_tape_rewind(int64_t* ctr, uint32_t* sync) {
+    if (threadIdx.x == 0) ctr[18] = 0;
+    for (int k = threadIdx.x; k <= TAPE_SUB; k += blockDim.x) sync[k * TAPE_SYNC_STRIDE] = 0;
 }
 
 }  // namespace
 
 int64_t mwd_tape_words(const mwd_workspace* ws) {
     TapeDesc d;
-    return tape_layout(ws, d);
+    return tape_layout(ws, nullptr, d);
 }
 
-int mwd_launch_tape(mwd_workspace* ws, bool push) {
+int mwd_launch_tape(mwd_workspace* ws, const mwd_state* st, bool push) {
     TapeDesc d;
-    d.slot_words = tape_layout(ws, d);
+    d.slot_words = tape_layout(ws, st, d);
     d.base = ws->tape;
     d.slots = ws->tape_slots;
-    if (!d.base || d.slot_words != ws->tape_words) {
+    d.sync = ws->tape_sync;
+    if (!d.base || !d.sync || d.slot_words != ws->tape_words) {
         mwd_set_error("the tape is not allocated for this workspace (mwd_tape_init)");
         return MWD_E_INVALID;
     }
-    const dim3 grid(16, d.n);
-    if (push) hipLaunchKernelGGL(k_tape<true>, grid, dim3(256), 0, ws->stream, d);
-    else hipLaunchKernelGGL(k_tape<false>, grid, dim3(256), 0, ws->stream, d);
+    // units at full capacity bound the grid (fixed, so it can be captured)
+    int64_t units = 0;
+    for (int a = 0; a < d.n; ++a)
+        units += ((d.a[a].rows ? (d.a[a].cap + 1) / 2 : (d.a[a].cap * d.a[a].unit + 3) / 4) + TAPE_UQ - 1) / TAPE_UQ;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(units, TAPE_BLOCKS));
+    if (push) hipLaunchKernelGGL(k_tape<true>, dim3(grid), dim3(256), 0, ws->stream, d);
+    else hipLaunchKernelGGL(k_tape<false>, dim3(grid), dim3(256), 0, ws->stream, d);
     MWD_LAUNCH_CHECK();
     return MWD_OK;
 }
 
 int mwd_launch_tape_rewind(mwd_workspace* ws) {
-    hipLaunchKernelGGL(k_tape_rewind, dim3(1), dim3(64), 0, ws->stream, ws->counters);
+    if (!ws->tape_sync) {
+        mwd_set_error("the tape is not allocated for this workspace (mwd_tape_init)");
+        return MWD_E_INVALID;
+    }
+    hipLaunchKernelGGL(k_tape_rewind, dim3(1), dim3(64), 0, ws->stream, ws->counters, ws->tape_sync);
     MWD_LAUNCH_CHECK();
     return MWD_OK;
 }

@@ -55,7 +55,7 @@ void free_ws(mwd_workspace* ws) {
                     ws->rec_bkt, ws->rec_ovf, ws->hist,
                     ws->hist_off, ws->scan_tmp, ws->scan_tmp_w, ws->wseg_off, ws->word_desc, ws->hit_words, ws->gjk_words, ws->counters, ws->flags, ws->laws,
                     ws->scene_wt_off, ws->ctan, ws->law_tans, ws->adj_F, ws->adj_T, ws->adj_lamF,
-                    ws->adj_lamT, ws->adj_face, ws->adj_c, ws->adj_cth, ws->adj_rowth, ws->tape};
+                    ws->adj_lamT, ws->adj_face, ws->adj_c, ws->adj_cth, ws->adj_rowth, ws->tape, ws->tape_sync};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     destroy_events(ws);
@@ -1072,6 +1072,10 @@ MWD_API int mwd_tape_init(mwd_workspace* ws, int64_t n_slots, int64_t* slot_byte
         ws->tape_slots = n_slots;
         ws->tape_words = words;
     }
+    if (!ws->tape_sync) {
+        MWD_HIP_CHECK(hipMalloc((void**)&ws->tape_sync, sizeof(uint32_t) * MWD_TAPE_SYNC_WORDS));
+        MWD_HIP_CHECK(hipMemsetAsync(ws->tape_sync, 0, sizeof(uint32_t) * MWD_TAPE_SYNC_WORDS, ws->stream));
+    }
     if ((rc = mwd_launch_tape_rewind(ws))) return rc;
     MWD_HIP_CHECK(hipStreamSynchronize(ws->stream));
     return MWD_OK;
@@ -1092,7 +1096,7 @@ MWD_API int mwd_step_record(mwd_workspace* ws, mwd_state* st) {
     ws->emit_faces = 1;
     rc = fine_contact(ws, st, nullptr);
     ws->emit_faces = 0;
-    if (rc || (rc = mwd_launch_tape(ws, true)) || (rc = mwd_reduce_verlet(ws, st))) return rc;
+    if (rc || (rc = mwd_launch_tape(ws, st, true)) || (rc = mwd_reduce_verlet(ws, st))) return rc;
     if (ws->wall_motion && (rc = mwd_launch_move_walls(ws, st))) return rc;
     mwd_mark_end(ws);
     return MWD_OK;
@@ -1105,7 +1109,7 @@ MWD_API int mwd_step_adjoint_taped(mwd_workspace* ws, mwd_state* st, double* con
         mwd_set_error("mwd_step_adjoint_taped: no tape (mwd_tape_init)");
         return MWD_E_INVALID;
     }
-    if ((rc = mwd_launch_tape(ws, false))) return rc;
+    if ((rc = mwd_launch_tape(ws, st, false))) return rc;
     rc = mwd_launch_step_adjoint(ws, st, state_view(ws, st, 0), state_view(ws, st, 1), lam, dtheta);
     ws->bkt_dirty = 0;  // the gather emptied the buckets
     return rc;

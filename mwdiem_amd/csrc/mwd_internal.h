@@ -298,6 +298,7 @@ struct mwd_workspace {
     // the reverse sweep's tape (mwd_tape_init): per recorded step, the
     // contact stages' outputs the adjoint step reads; counters[18] slots held
     uint32_t* tape;                // [tape_slots * tape_words]
+    uint32_t* tape_sync;           // [MWD_TAPE_SYNC_WORDS] block counters of the tape launches (zero between them)
     int64_t tape_slots, tape_words;
     // per moving row, the records (2c for side A of contact c, 2c+1 for side
     // B) of this step's contacts, in arrival order; the reduction sorts them
@@ -335,7 +336,7 @@ struct mwd_workspace {
     // 14 moving rows active now (m_live), 15 wall slab tests now (wt_live),
     // 16 steps taken since the spawn schedule was set, 17 wall pairs listed
     // for GJK (the rest are separated by the wall's slab), 18 steps held on
-    // the reverse sweep's tape, 19 blocks of a tape launch done
+    // the reverse sweep's tape
     int64_t* counters;
     int64_t* m_live;           // counters + 14: every kernel over the moving rows reads its count here
     int64_t* wt_live;          // counters + 15
@@ -421,11 +422,12 @@ int mwd_launch_copy_arrays(int n, void* const* dst, const void* const* src, cons
 // state (adj_face written): lam in/out, dtheta (device, 3) accumulated.
 int mwd_launch_step_adjoint(mwd_workspace* ws, const mwd_state* st, const BodyView& A, const BodyView& B,
                             double* const* lam, double* dtheta);
-// The tape: words per slot for this workspace; push the contact stages'
-// outputs of the step just run (push = true) or pop the last recorded step's
-// back into the workspace arrays; empty it.
+// The tape: words per slot for this workspace; push the input state and the
+// contact stages' outputs of the step running on st (push = true), or pop
+// the last recorded step's back into st and the workspace arrays; empty it.
+#define MWD_TAPE_SYNC_WORDS (17 * 32)
 int64_t mwd_tape_words(const mwd_workspace* ws);
-int mwd_launch_tape(mwd_workspace* ws, bool push);
+int mwd_launch_tape(mwd_workspace* ws, const mwd_state* st, bool push);
 int mwd_launch_tape_rewind(mwd_workspace* ws);
 int mwd_launch_cell_keys(const double* com, int64_t n, const double* cell, int64_t ncx, int64_t ncy,
                          int64_t* keys, hipStream_t s);

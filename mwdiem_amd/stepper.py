@@ -288,17 +288,18 @@ class Stepper:
         _lib.check(_lib.load().mwd_step_adjoint(self.ws, C.byref(self.state), lam_ptrs, dtheta.data_ptr()),
                    "mwd_step_adjoint")
 
-    def capture_adjoint(self, lam_ptrs, dtheta: torch.Tensor, taped: bool = False):
-        """A HIP graph of one step_adjoint (step_adjoint_taped if taped) on
-        these adjoint arrays (capture runs nothing). step_adjoint or tape_init
-        must have run before (they allocate)."""
+    def capture_adjoint(self, lam_ptrs, dtheta: torch.Tensor, taped: bool = False, steps: int = 1):
+        """A HIP graph of `steps` step_adjoint calls (step_adjoint_taped if
+        taped) on these adjoint arrays (capture runs nothing). step_adjoint or
+        tape_init must have run before (they allocate)."""
         self._enter()
         torch.cuda.synchronize(self.device)
         g = torch.cuda.CUDAGraph()
         lib = _lib.load()
         fn = "mwd_step_adjoint_taped" if taped else "mwd_step_adjoint"
         with torch.cuda.graph(g, stream=self.stream):
-            _lib.check(getattr(lib, fn)(self.ws, C.byref(self.state), lam_ptrs, dtheta.data_ptr()), fn)
+            for _ in range(steps):
+                _lib.check(getattr(lib, fn)(self.ws, C.byref(self.state), lam_ptrs, dtheta.data_ptr()), fn)
         torch.cuda.synchronize(self.device)
         return g
 
@@ -325,20 +326,22 @@ class Stepper:
             _lib.check(lib.mwd_step_record(self.ws, C.byref(self.state)), "mwd_step_record")
         self.steps_done += n
 
-    def capture_record(self):
-        """A HIP graph of one step_record (capture runs nothing)."""
+    def capture_record(self, steps: int = 1):
+        """A HIP graph of `steps` step_record calls (capture runs nothing)."""
         self._enter()
         torch.cuda.synchronize(self.device)
         g = torch.cuda.CUDAGraph()
+        lib = _lib.load()
         with torch.cuda.graph(g, stream=self.stream):
-            _lib.check(_lib.load().mwd_step_record(self.ws, C.byref(self.state)), "mwd_step_record")
+            for _ in range(steps):
+                _lib.check(lib.mwd_step_record(self.ws, C.byref(self.state)), "mwd_step_record")
         torch.cuda.synchronize(self.device)
         return g
 
     def step_adjoint_taped(self, lam_ptrs, dtheta: torch.Tensor):
-        """step_adjoint of the step last recorded on the tape (its input state
-        restored in the stepper), its contact-stage outputs taken from the
-        tape."""
+        """step_adjoint of the step last recorded on the tape: its input state
+        and contact-stage outputs are taken back from the tape (the stepper's
+        state becomes that input state)."""
         self._enter()
         _lib.check(_lib.load().mwd_step_adjoint_taped(self.ws, C.byref(self.state), lam_ptrs, dtheta.data_ptr()),
                    "mwd_step_adjoint_taped")
