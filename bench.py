@@ -283,6 +283,9 @@ def profile_leg(args):
 
     from mwdiem_amd import scene
     from mwdiem_amd.stepper import Stepper
+    if args.profile_leg == "adjoint":
+        profile_adjoint(args)
+        return
     kw = {}
     v = None
     if args.profile_leg == "poly":
@@ -307,6 +310,35 @@ def profile_leg(args):
     print(json.dumps({"profile_leg": args.profile_leg, "steps": args.steps, "warmup": args.warmup,
                       "status": S.check()}))
     S.close()
+
+
+def profile_adjoint(args):
+    """The differentiable form's reverse sweep for a rocprofv3 run: the
+    4,096-hull pile poured on its spawn schedule, then the bed outcome's
+    gradient over --steps steps by the adjoint with checkpoints every 100
+    steps and the tape, graphs of --graph-steps steps, fixed launch choices
+    for a packed bed (the sorted cell pass, 4 cells per fill block, 4 lanes
+    per GJK pair)."""
+    import numpy as np
+
+    from mwdiem_amd import scene
+    from mwdiem_amd.adjoint import adjoint_run
+    from mwdiem_amd.diff import form_pile
+    sc, v_pour, spawn = scene.config_pour(n=4096)
+    n_pour = spawn.every * ((sc.n_particles + spawn.per - 1) // spawn.per) + 40_000
+    tune = dict(fill_cells=4, gjk_group=4)
+    print(f"# profile adjoint: pouring {sc.n_particles} hulls over {n_pour} steps", file=sys.stderr, flush=True)
+    st0 = form_pile(sc, v_pour, steps=n_pour, spawn=spawn, steps_per_graph=args.graph_steps, tuning=tune)
+    tune["cell_pass"] = 1  # the sorted pass (the pour's spawn schedule needs the key tables)
+    theta = np.array([sc.law_p.Kn, sc.law_p.en, sc.law_p.mu])
+    print(f"# profile adjoint: {args.steps} steps forward, then the reverse sweep", file=sys.stderr, flush=True)
+    tm = {}
+    L, g = adjoint_run(sc, st0, theta, args.steps, every=100, steps_per_graph=args.graph_steps, timings=tm,
+                       tuning=tune)
+    print(json.dumps({"profile_leg": "adjoint", "steps": args.steps, "outcome": L, "grad": [float(x) for x in g],
+                      "forward_ms_per_step": 1e3 * tm["forward_s"] / args.steps,
+                      "reverse_ms_per_step": 1e3 * tm["reverse_s"] / args.steps,
+                      "outcome_and_seed_ms": 1e3 * tm["outcome_s"]}))
 
 
 def contacting_scene(sc, gap: float = -0.6):
@@ -520,8 +552,9 @@ def bench_diff(pour, steps: int, graph_steps: int, horizons=(10, 100, 1000, 2000
            "rel_to_tangents": [float(x) for x in np.abs(ga - gt) / np.maximum(np.abs(gt), 1e-300)],
            "forward_ms_per_step": 1e3 * tm["forward_s"] / steps,
            "reverse_ms_per_step": 1e3 * tm["reverse_s"] / steps,
+           "outcome_and_seed_ms": 1e3 * tm["outcome_s"],
            "adjoint_vs_forward": tm["reverse_s"] / steps / times["plain"],
-           "gradient_vs_forward": (tm["forward_s"] + tm["reverse_s"]) / steps / times["plain"]}
+           "gradient_vs_forward": (tm["forward_s"] + tm["outcome_s"] + tm["reverse_s"]) / steps / times["plain"]}
     print(f"# diff: adjoint {adj}", file=sys.stderr, flush=True)
     return {"particles": n, "steps": steps, "pile": pile, "ms_per_step_plain": 1e3 * times["plain"],
             "ms_per_step_tangent": 1e3 * times["tangent"], "tangent_vs_forward": times["tangent"] / times["plain"],
@@ -659,10 +692,11 @@ def main():
     ap.add_argument("--poly-steps", type=int, default=2000, help="steps of the polydisperse stress configuration")
     ap.add_argument("--no-poly", action="store_true")
     ap.add_argument("--no-contacting", action="store_true", help="skip the compacted (contact-bearing) C2 bed")
-    ap.add_argument("--profile-leg", choices=("c2", "contacting", "poly"), default=None,
+    ap.add_argument("--profile-leg", choices=("c2", "contacting", "poly", "adjoint"), default=None,
                     help="only step one workload eagerly (warm-up plus --steps steps) for a profiler run: the "
                          "large settle, its compacted bed, or the polydisperse configuration thrown about at "
-                         "1 m/s (--warmup steps first, then --steps steps)")
+                         "1 m/s (--warmup steps first, then --steps steps); or 'adjoint': the poured pile's "
+                         "reverse sweep over --steps steps")
     ap.add_argument("--copy-probe", type=int, default=0, help="only run a device copy of this many bytes (counter "
                                                                  "calibration)")
     args = ap.parse_args()
