@@ -365,8 +365,9 @@ def bench_batched(scs, steps: int, plain_steps: int, graph_steps: int, tune: boo
     own graph-captured Stepper with the default launch choices. Every scene
     starts with its hulls thrown about (v_rms mm/s per component, seeded per
     scene), so contacts form from the first steps in every scene and GJK/EPA
-    work is uneven across them; kernel durations of the batched step come
-    from the library's kernel timers over 20 eager steps after the timed run."""
+    work is uneven across them; the launch choices are timed 200 steps into
+    the throw, and kernel durations of the batched step come from the
+    library's kernel timers over 20 eager steps after the timed run."""
     import numpy as np
     import torch
 
@@ -392,7 +393,13 @@ def bench_batched(scs, steps: int, plain_steps: int, graph_steps: int, tune: boo
     B.step(5)
     B.check()
     if tune:
-        tuned = autotune(B, steps=5)
+        # launch choices timed on the state 200 steps on, once the throw has
+        # made contacts in every scene (GJK's choices matter only then); the
+        # timed run starts from step 5, as the plain runs do
+        snap = B.snapshot()
+        B.step(200)
+        tuned = autotune(B, steps=10)
+        B.restore(snap)
         tuned = {"cell_scale": scale, "cell_scale_ms_per_step": scale_ms, "tuning": tuned["tuning"],
                  "ms_per_step_eager": tuned["ms_per_step"], "default_ms_per_step_eager": tuned["default_ms_per_step"]}
         print(f"# batched autotune: {tuned['tuning']}", file=sys.stderr, flush=True)
