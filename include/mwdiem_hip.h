@@ -217,6 +217,9 @@ int mwd_set_reduction(mwd_workspace* ws, int32_t mode);
  *                separate EPA launch over all contacts.
  *   fill_cells   cells per block of the pair emission: 64 (default), 16 or
  *                4 (fewer for beds packed many hulls to a cell).
+ *   fill_split   blocks sharing one emission block's cells, each taking every
+ *                fill_split-th round of their pairs: 1 (default), 4 or 16
+ *                (more blocks for beds of few, crowded cells).
  *   gjk_buckets  1: the sphere-test survivors are first sorted by
  *                ceil(max(nA, nB) / gjk_group) and GJK runs one launch per
  *                class, each with the vertices-per-lane its class needs
@@ -239,6 +242,7 @@ typedef struct mwd_tuning {
     int32_t epa_fused;
     int32_t fill_cells;
     int32_t gjk_buckets;
+    int32_t fill_split;
 } mwd_tuning;
 int mwd_get_tuning(const mwd_workspace* ws, mwd_tuning* out);
 int mwd_set_tuning(mwd_workspace* ws, const mwd_tuning* t);

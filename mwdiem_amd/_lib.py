@@ -94,7 +94,7 @@ class Tuning(C.Structure):
                 ("sphere_block", C.c_int32), ("verlet_block", C.c_int32), ("cell_items", C.c_int32),
                 ("verlet_waves", C.c_int32), ("vertex_pack", C.c_int32), ("epa_blocks", C.c_int32),
                 ("epa_fused", C.c_int32), ("fill_cells", C.c_int32),
-                ("gjk_buckets", C.c_int32)]
+                ("gjk_buckets", C.c_int32), ("fill_split", C.c_int32)]
 
 
 HISTORY_RESET = 0

@@ -539,6 +539,7 @@ MWD_API int mwd_get_tuning(const mwd_workspace* ws, mwd_tuning* out) {
     out->epa_fused = ws->tune.epa_fused;
     out->fill_cells = ws->tune.fill_cells;
     out->gjk_buckets = ws->tune.gjk_buckets;
+    out->fill_split = ws->tune.fill_split;
     return MWD_OK;
 }
 
@@ -559,12 +560,14 @@ MWD_API int mwd_set_tuning(mwd_workspace* ws, const mwd_tuning* t) {
     const bool epa_ok = t->epa_blocks >= 16 && t->epa_blocks <= 65536 && (t->epa_fused == 0 || t->epa_fused == 1);
     const bool fill_ok = t->fill_cells == 4 || t->fill_cells == 16 || t->fill_cells == 64;
     const bool buckets_ok = t->gjk_buckets == 0 || t->gjk_buckets == 1;
+    const bool split_ok = t->fill_split == 1 || t->fill_split == 4 || t->fill_split == 16;
     if (!pass_ok || !group_ok || !blocks_ok || !sphere_ok || !verlet_ok || !items_ok || !waves_ok || !pack_ok ||
-        !epa_ok || !fill_ok || !buckets_ok) {
+        !epa_ok || !fill_ok || !buckets_ok || !split_ok) {
         mwd_set_error(!pass_ok ? "mwd_set_tuning: cell_pass unknown, MWD_CELLS_TABLE without key tables, or "
                                  "MWD_CELLS_SORTED with a spawn schedule"
                                : "mwd_set_tuning: gjk_group, gjk_blocks, sphere_block, verlet_block, cell_items, "
-                                 "verlet_waves, vertex_pack, epa_blocks, epa_fused, fill_cells or gjk_buckets "
+                                 "verlet_waves, vertex_pack, epa_blocks, epa_fused, fill_cells, gjk_buckets or "
+                                 "fill_split "
                                  "out of range");
         return MWD_E_INVALID;
     }
@@ -580,6 +583,7 @@ MWD_API int mwd_set_tuning(mwd_workspace* ws, const mwd_tuning* t) {
     ws->tune.epa_fused = t->epa_fused;
     ws->tune.fill_cells = t->fill_cells;
     ws->tune.gjk_buckets = t->gjk_buckets;
+    ws->tune.fill_split = t->fill_split;
     return MWD_OK;
 }
 

@@ -583,7 +583,10 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t live) {
 // The cells a block takes (FILL_CELLS, a launch choice: 64, 16 or 4) set the
 // balance: with about one hull per cell 64 cells give a block a few hundred
 // pairs; a packed bed holds many hulls per cell and thousands of pairs per
-// cell, and fewer cells per block spread them over more blocks.
+// cell, and fewer cells per block spread them over more blocks. With split
+// > 1 (a launch choice: 4 or 16) `split` blocks share one group of cells,
+// block s taking rounds s, s + split, ... of its pairs: for beds of few,
+// crowded cells, which give too few groups to fill the chip.
 constexpr int FILL_THREADS = 256;
 constexpr int FILL_SURV = 2048;   // survivors gathered in LDS before a global append
 
@@ -596,7 +599,7 @@ __global__ void __launch_bounds__(FILL_THREADS) k_cell_fill(const int32_t* __res
                                                           int32_t* __restrict__ pairs, int64_t cap,
                                                           const double* __restrict__ sph,
                                                           int32_t* __restrict__ surv, int64_t* n_surv,
-                                                          int32_t* __restrict__ collision) {
+                                                          int32_t* __restrict__ collision, int split) {
     __shared__ int64_t s_off[FILL_CELLS + 1];
     __shared__ int2 s_self[FILL_CELLS];
     __shared__ int2 s_seg[FILL_CELLS * MWD_MAX_OFF];
@@ -615,8 +618,10 @@ __global__ void __launch_bounds__(FILL_THREADS) k_cell_fill(const int32_t* __res
     __shared__ int64_t s_gbase;
     const int64_t C = counters[0];
     const int64_t live = (C + FILL_CELLS - 1) / FILL_CELLS;
-    if ((int64_t)blockIdx.x >= live) return;  // uniform per block
-    const int64_t r0 = xcd_block(blockIdx.x, live) * FILL_CELLS;
+    const int64_t groups = gridDim.x / split;  // the grid is `split` runs of `groups` blocks
+    const int64_t g = blockIdx.x % groups, part = blockIdx.x / groups;
+    if (g >= live) return;  // uniform per block
+    const int64_t r0 = xcd_block(g, live) * FILL_CELLS;
     const int nc = (int)(C - r0 < FILL_CELLS ? C - r0 : FILL_CELLS);
     const int tid = threadIdx.x;
     const int64_t base = cell_off[r0];
@@ -645,7 +650,8 @@ __global__ void __launch_bounds__(FILL_THREADS) k_cell_fill(const int32_t* __res
     // a uniform trip count over the block, so that the survivors' ballot
     // sees every lane of a wave
     int rounds = 0;
-    for (int64_t q0 = 0; q0 < total; q0 += FILL_THREADS) {
+    const int64_t qstep = (int64_t)FILL_THREADS * split;
+    for (int64_t q0 = part * FILL_THREADS; q0 < total; q0 += qstep) {
         const int64_t q = q0 + tid;
         bool have = q < total;
         int32_t a = 0, b = 0;
@@ -708,7 +714,7 @@ __global__ void __launch_bounds__(FILL_THREADS) k_cell_fill(const int32_t* __res
             // append after FILL_SURV / FILL_THREADS rounds (the buffer is then
             // at most full) and after the last; the round count is uniform,
             // so only the appending rounds pay the barriers
-            if (++rounds == FILL_SURV / FILL_THREADS || q0 + FILL_THREADS >= total) {
+            if (++rounds == FILL_SURV / FILL_THREADS || q0 + qstep >= total) {
                 rounds = 0;
                 __syncthreads();
                 const int cnt = s_nsurv;
@@ -1153,10 +1159,11 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
         mwd_mark(ws, 3);
         const hipEvent_t e0 = mwd_kev(ws, MWD_KT_EMIT, 0), e1 = mwd_kev(ws, MWD_KT_EMIT, 1);
         double* sph = st->radius ? ws->sph : nullptr;
+        const int split = ws->tune.fill_split;
 #define MWD_FILL(CELLS)                                                                                      \
-    MWD_LAUNCH_EV(e0, e1, k_cell_fill<CELLS>, dim3((unsigned)((M + (CELLS)-1) / (CELLS))), dim3(FILL_THREADS), \
-                  0, s, ws->perm, ws->cell_self, ws->cell_seg, ws->cell_off, ctr, ws->pairs, ws->pair_cap, sph, \
-                  ws->surv, ctr + 10, ws->collision)
+    MWD_LAUNCH_EV(e0, e1, k_cell_fill<CELLS>, dim3((unsigned)(((M + (CELLS)-1) / (CELLS)) * split)),             \
+                  dim3(FILL_THREADS), 0, s, ws->perm, ws->cell_self, ws->cell_seg, ws->cell_off, ctr, ws->pairs,  \
+                  ws->pair_cap, sph, ws->surv, ctr + 10, ws->collision, split)
         switch (ws->tune.fill_cells) {
             case 4:
                 MWD_FILL(4);

@@ -137,6 +137,7 @@ struct PairTune {
     int epa_blocks = 2048;   // grid cap of the EPA launch (blocks of 4 waves, one contact per wave)
     int epa_fused = 1;       // EPA and the law inside the GJK launch (1) or as their own launch (0)
     int fill_cells = 64;     // cells per block of the pair emission: 64, 16 or 4
+    int fill_split = 1;      // blocks per emission block's cells, splitting its rounds: 1, 4 or 16
     int gjk_buckets = 0;     // GJK launched per class of vertex count (survivors sorted first): 1, or one launch (0)
 };
 

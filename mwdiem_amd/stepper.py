@@ -229,7 +229,7 @@ class Stepper:
     def tuning(self) -> dict:
         """Launch choices of the workspace (cell_pass, gjk_group, gjk_blocks, sphere_block, verlet_block,
         cell_items, verlet_waves, vertex_pack, epa_blocks, epa_fused, fill_cells,
-        gjk_buckets)."""
+        gjk_buckets, fill_split)."""
         t = _lib.Tuning()
         _lib.check(_lib.load().mwd_get_tuning(self.ws, C.byref(t)), "mwd_get_tuning")
         return {name: int(getattr(t, name)) for name, _ in _lib.Tuning._fields_}

@@ -40,6 +40,7 @@ SPACE = {
     "verlet_waves": (1, 3, 4),
     "vertex_pack": (0, 1),
     "fill_cells": (64, 16, 4),
+    "fill_split": (1, 4, 16),
     "gjk_buckets": (0, 1),
 }
 # Not searched: epa_blocks (the EPA grid cap) and epa_fused (EPA inside the

@@ -201,7 +201,8 @@ def test_tuning_choices_change_nothing(gpu, dense_phys):
                 dict(epa_blocks=128), dict(epa_fused=0), dict(epa_fused=0, gjk_group=16),
                 dict(gjk_group=16, gjk_blocks=64), dict(fill_cells=16), dict(fill_cells=4),
                 dict(fill_cells=4, cell_pass=_lib.CELLS_SORTED), dict(gjk_buckets=1),
-                dict(gjk_buckets=1, gjk_group=4), dict(gjk_buckets=1, epa_fused=0)]
+                dict(gjk_buckets=1, gjk_group=4), dict(gjk_buckets=1, epa_fused=0), dict(fill_split=4),
+                dict(fill_split=16, fill_cells=4), dict(fill_split=4, cell_pass=_lib.CELLS_SORTED)]
     for v in variants:
         S = _stepper(sc)
         S.set_tuning(**v)
@@ -215,7 +216,8 @@ def test_tuning_choices_change_nothing(gpu, dense_phys):
     S = _stepper(sc)
     for bad in (dict(gjk_group=5), dict(gjk_blocks=8), dict(sphere_block=100), dict(cell_pass=7),
                 dict(verlet_block=96), dict(cell_items=3), dict(verlet_waves=2), dict(vertex_pack=2),
-                dict(epa_blocks=8), dict(epa_fused=2), dict(fill_cells=8), dict(gjk_buckets=2)):
+                dict(epa_blocks=8), dict(epa_fused=2), dict(fill_cells=8), dict(gjk_buckets=2),
+                dict(fill_split=2)):
         with pytest.raises(_lib.MwdError):
             S.set_tuning(**bad)
 
