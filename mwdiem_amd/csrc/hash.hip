@@ -468,12 +468,51 @@ __device__ __forceinline__ int32_t cell_work(int64_t c, int64_t r, bool sorted, 
         for (int k = 0; k < 16; ++k) v[k] = k < n ? m[k] : INT32_MAX;
     }
     int2 seg[MWD_MAX_OFF];
+    if constexpr (SegOf::kSearch) {
+        // sorted keys: the neighbours' lower-bound searches run in lockstep
+        // (the step count depends on C only), so each step's probes go out
+        // together instead of one dependent chain per neighbour
+        uint32_t kk[MWD_MAX_OFF];
+        bool ok[MWD_MAX_OFF];
+        int64_t b[MWD_MAX_OFF];
 #pragma unroll
-    for (int o = 0; o < MWD_MAX_OFF; ++o) {
-        seg[o] = make_int2(0, 0);
-        if (o < sc.n_off) {
-            const int64_t t = lk + sc.off[o] - sc.kmin;
-            if (t >= 0 && t < sc.kspan) seg[o] = seg_of((uint32_t)(sc.kbase + t));
+        for (int o = 0; o < MWD_MAX_OFF; ++o) {
+            seg[o] = make_int2(0, 0);
+            ok[o] = false;
+            kk[o] = 0;
+            b[o] = 0;
+            if (o < sc.n_off) {
+                const int64_t t = lk + sc.off[o] - sc.kmin;
+                if (t >= 0 && t < sc.kspan) {
+                    ok[o] = true;
+                    kk[o] = (uint32_t)(sc.kbase + t);
+                }
+            }
+        }
+        const uint32_t* __restrict__ ck = seg_of.cell_key;
+        const int64_t C = seg_of.C;  // at least 1: this cell
+        for (int64_t n = C; n > 1;) {  // b + n <= C throughout
+            const int64_t half = n >> 1;
+#pragma unroll
+            for (int o = 0; o < MWD_MAX_OFF; ++o) b[o] = ck[b[o] + half] < kk[o] ? b[o] + half : b[o];
+            n -= half;
+        }
+#pragma unroll
+        for (int o = 0; o < MWD_MAX_OFF; ++o) {
+            const int64_t lb = b[o] + (ck[b[o]] < kk[o] ? 1 : 0);  // the first key >= kk[o]
+            if (ok[o] && lb < C && ck[lb] == kk[o]) {
+                const int32_t a0 = seg_of.cell_start[lb];
+                seg[o] = make_int2(a0, seg_of.cell_start[lb + 1] - a0);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int o = 0; o < MWD_MAX_OFF; ++o) {
+            seg[o] = make_int2(0, 0);
+            if (o < sc.n_off) {
+                const int64_t t = lk + sc.off[o] - sc.kmin;
+                if (t >= 0 && t < sc.kspan) seg[o] = seg_of((uint32_t)(sc.kbase + t));
+            }
         }
     }
     if (n <= 16) {  // members in registers
@@ -511,6 +550,7 @@ __device__ __forceinline__ int32_t cell_work(int64_t c, int64_t r, bool sorted, 
 }
 
 struct SegSorted {  // binary search of the sorted cell keys
+    static constexpr bool kSearch = true;
     const uint32_t* cell_key;
     const int32_t* cell_start;
     int64_t C;
@@ -523,6 +563,7 @@ struct SegSorted {  // binary search of the sorted cell keys
 };
 
 struct SegTable {  // key -> (first slot, members) this pass; other epochs' entries are stale
+    static constexpr bool kSearch = false;
     const int4* key_seg;
     int32_t epoch;
     __device__ __forceinline__ int2 operator()(uint32_t key) const {
