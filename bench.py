@@ -317,8 +317,8 @@ def profile_adjoint(args):
     4,096-hull pile poured on its spawn schedule, then the bed outcome's
     gradient over --steps steps by the adjoint with checkpoints every 100
     steps and the tape, graphs of --graph-steps steps, fixed launch choices
-    for a packed bed (the sorted cell pass, 4 cells per fill block, 4 lanes
-    per GJK pair)."""
+    for a packed bed (the sorted cell pass, 4 cells per fill block shared by
+    16 blocks, 4 lanes per GJK pair)."""
     import numpy as np
 
     from mwdiem_amd import scene
@@ -329,7 +329,7 @@ def profile_adjoint(args):
     tune = dict(fill_cells=4, gjk_group=4)
     print(f"# profile adjoint: pouring {sc.n_particles} hulls over {n_pour} steps", file=sys.stderr, flush=True)
     st0 = form_pile(sc, v_pour, steps=n_pour, spawn=spawn, steps_per_graph=args.graph_steps, tuning=tune)
-    tune["cell_pass"] = 1  # the sorted pass (the pour's spawn schedule needs the key tables)
+    tune.update(cell_pass=1, fill_split=16)  # the sorted pass (the pour's spawn schedule needs the key tables)
     theta = np.array([sc.law_p.Kn, sc.law_p.en, sc.law_p.mu])
     print(f"# profile adjoint: {args.steps} steps forward, then the reverse sweep", file=sys.stderr, flush=True)
     tm = {}
@@ -526,8 +526,16 @@ def bench_diff(pour, steps: int, graph_steps: int, horizons=(10, 100, 1000, 2000
     # reverse mode: the gradient by the adjoint, checkpoints every 100 steps
     # (forward, then per segment a recomputation and the reverse sweep)
     from mwdiem_amd.adjoint import adjoint_run
+    from mwdiem_amd.diff import bed_outcome
     every = 100
     tm = {}
+    # torch loads the kernels of the outcome (top-k, reductions) and of its
+    # autograd at their first call in the process (about 0.5 s): one untimed
+    # call on a tensor of the same shape first
+    with torch.enable_grad():
+        x = torch.zeros((sc.n_rows, 3), dtype=torch.float64, device="cuda").requires_grad_(True)
+        torch.autograd.grad(bed_outcome(x, W), x)
+    torch.cuda.synchronize()
     La, ga = adjoint_run(sc, st0, theta, steps, every=every, steps_per_graph=graph_steps, timings=tm, tuning=tune)
     checks = {}
     longest = None
