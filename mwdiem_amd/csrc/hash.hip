@@ -267,12 +267,14 @@ __device__ __forceinline__ uint32_t probe_occ(uint64_t key, uint32_t mask, uint6
     }
 }
 
-__device__ void set_order(const int32_t* rows, int n, int64_t row0, int32_t* out) {
+// tab: two tables of SET_TAB slots (slot -> key, row - row0, valid where the
+// occupancy bit is set), in the caller's memory: a thread's own array, or
+// its share of LDS where the kernel has room for it.
+__device__ void set_order(const int32_t* rows, int n, int64_t row0, int32_t* out, int32_t (*tab)[SET_TAB]) {
     if (n <= 1) {
         if (n == 1) out[0] = rows[0];
         return;
     }
-    int32_t tab[2][SET_TAB];  // slot -> key (row - row0), valid where the occupancy bit is set
     int cur = 0;
     uint64_t lo = 0, hi = 0;
     uint32_t size = 8, fill = 0;
@@ -310,6 +312,10 @@ __device__ void set_order(const int32_t* rows, int n, int64_t row0, int32_t* out
             out[o++] = (int32_t)(tab[cur][q] + row0);
         }
     }
+}
+__device__ void set_order(const int32_t* rows, int n, int64_t row0, int32_t* out) {
+    int32_t tab[2][SET_TAB];
+    set_order(rows, n, row0, out, tab);
 }
 
 // set_order of n <= 16 members given in registers: the same table walks
@@ -434,7 +440,8 @@ __device__ __forceinline__ int32_t cell_work(int64_t c, int64_t r, bool sorted, 
                                              const uint32_t* __restrict__ cell_key,
                                              const SceneDev* __restrict__ scenes, int S,
                                              int32_t* __restrict__ perm, int2* __restrict__ cell_self,
-                                             int2* __restrict__ cell_seg, uint32_t* flags, const SegOf& seg_of) {
+                                             int2* __restrict__ cell_seg, uint32_t* flags, const SegOf& seg_of,
+                                             int32_t (*tab)[SET_TAB] = nullptr) {
     const int32_t start = cell_start[c], end = cell_start[c + 1];
     const uint32_t key = cell_key[c];
     const int n = end - start;
@@ -527,7 +534,8 @@ __device__ __forceinline__ int32_t cell_work(int64_t c, int64_t r, bool sorted, 
             atomicOr(flags, MWD_F_CELL);
             for (int k = 0; k < n; ++k) perm[start + k] = m[k];
         } else if (sorted) {
-            set_order(m, n, sc.row0, perm + start);
+            if (tab) set_order(m, n, sc.row0, perm + start, tab);
+            else set_order(m, n, sc.row0, perm + start);
         } else {  // sorted by rank (the rows are distinct): no chain of dependent moves
             int32_t srt[MWD_MAX_CELL];
             for (int k = 0; k < n; ++k) {
@@ -536,7 +544,8 @@ __device__ __forceinline__ int32_t cell_work(int64_t c, int64_t r, bool sorted, 
                 for (int j = 0; j < n; ++j) rk += m[j] < x ? 1 : 0;
                 srt[rk] = x;
             }
-            set_order(srt, n, sc.row0, perm + start);
+            if (tab) set_order(srt, n, sc.row0, perm + start, tab);
+            else set_order(srt, n, sc.row0, perm + start);
         }
     }
     cell_self[r] = make_int2(start, n);
@@ -572,19 +581,23 @@ struct SegTable {  // key -> (first slot, members) this pass; other epochs' entr
     }
 };
 
-// Sorted path: one thread per cell (cells in key order, members sorted).
-__global__ void k_cell_info(int32_t* __restrict__ vals, const int32_t* __restrict__ cell_start,
+// Sorted path: one thread per cell (cells in key order, members sorted),
+// 64 per block, each with its share of LDS for the set-order tables of a
+// crowded cell (scratch memory otherwise).
+constexpr int CELL_INFO_BLOCK = 64;
+__global__ void __launch_bounds__(CELL_INFO_BLOCK) k_cell_info(int32_t* __restrict__ vals, const int32_t* __restrict__ cell_start,
                             const uint32_t* __restrict__ cell_key, const int64_t* __restrict__ n_cells,
                             const SceneDev* __restrict__ scenes, int S,
                             const int64_t* __restrict__ row_rank, int32_t* __restrict__ perm,
                             int2* __restrict__ cell_self, int2* __restrict__ cell_seg,
                             int32_t* __restrict__ cell_cnt, uint32_t* flags) {
+    __shared__ int32_t s_tab[CELL_INFO_BLOCK][2][SET_TAB];
     const int64_t C = *n_cells;
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C) return;
     const int64_t r = row_rank[vals[cell_start[c]]];
     cell_cnt[r] = cell_work(c, r, true, vals, nullptr, cell_start, cell_key, scenes, S, perm, cell_self, cell_seg,
-                            flags, SegSorted{cell_key, cell_start, C});
+                            flags, SegSorted{cell_key, cell_start, C}, s_tab[threadIdx.x]);
 }
 
 // Key-table path, the per-cell work as a kernel of its own (one cell per
@@ -1187,7 +1200,8 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
             rc = mwd_scan_i32(ws->row_first, ws->row_rank, nullptr, ws->n_rows, nullptr, ws->scan_tmp,
                               ws->scan_tmp_len, s);
             if (rc) return rc;
-            hipLaunchKernelGGL(k_cell_info, dim3(blocks_for(M)), dim3(256), 0, s, vs, ws->cell_start,
+            hipLaunchKernelGGL(k_cell_info, dim3((unsigned)((M + CELL_INFO_BLOCK - 1) / CELL_INFO_BLOCK)),
+                               dim3(CELL_INFO_BLOCK), 0, s, vs, ws->cell_start,
                                ws->cell_key, ctr + 0, ws->scenes_dev, ws->n_scenes, ws->row_rank, ws->perm,
                                ws->cell_self, ws->cell_seg, ws->cell_cnt, ws->flags);
             MWD_LAUNCH_CHECK();

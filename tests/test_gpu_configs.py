@@ -207,6 +207,30 @@ def poured_pile():
     return sc, st0
 
 
+def test_c5_poured_pile_candidates_both_cell_passes(gpu, poured_pile):
+    """The settled pile packs cells of up to about 40 hulls (the set-order
+    walks past sixteen members, the key lists, the LDS and scratch tables):
+    the candidate list of one step, order included, from the key-table pass
+    and from the sorted pass (with the emission's block split) equals the
+    oracle's."""
+    from mwdiem_amd import _lib
+    from mwdiem_amd.stepper import Stepper
+    from oracle import oracle as O
+    sc, st0 = poured_pile
+    ref = O.candidates({k: v.copy() for k, v in st0.items()}, sc)
+    for choice in (dict(cell_pass=_lib.CELLS_TABLE), dict(cell_pass=_lib.CELLS_SORTED, fill_cells=4, fill_split=16),
+                   dict(cell_pass=_lib.CELLS_TABLE, cell_items=0, fill_cells=16, fill_split=4)):
+        S = Stepper(sc)
+        S.set_state(st0)
+        S.set_tuning(**choice)
+        S.step(1)
+        S.check()
+        got = S.pairs()
+        S.close()
+        print(f"C5 candidates {choice}: {len(got)} pairs (oracle {len(ref)})")
+        assert np.array_equal(got, ref), choice
+
+
 def test_c5_poured_pile_adjoint_against_torch_cpu_reference(gpu, poured_pile, steps=20):
     """The pile poured on the device (physical law, Kn=1e4, en=0.5, mu=0.3):
     d(bed outcome)/d(Kn, en, mu) over 20 steps of the settled pile (154,000
