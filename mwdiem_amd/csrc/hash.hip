@@ -434,6 +434,14 @@ struct KeyLists {
     int32_t* key_cnt;
 };
 
+// LDS a per-cell thread may be given for a crowded cell: the set-order
+// tables, and the members staged for their rank sort (key-table path).
+struct CellLds {
+    int32_t tab[2][SET_TAB];
+    int32_t m[MWD_MAX_CELL];
+    int32_t srt[MWD_MAX_CELL];
+};
+
 template <class SegOf>
 __device__ __forceinline__ int32_t cell_work(int64_t c, int64_t r, bool sorted, int32_t* __restrict__ members,
                                              const KeyLists* kl, const int32_t* __restrict__ cell_start,
@@ -441,7 +449,7 @@ __device__ __forceinline__ int32_t cell_work(int64_t c, int64_t r, bool sorted, 
                                              const SceneDev* __restrict__ scenes, int S,
                                              int32_t* __restrict__ perm, int2* __restrict__ cell_self,
                                              int2* __restrict__ cell_seg, uint32_t* flags, const SegOf& seg_of,
-                                             int32_t (*tab)[SET_TAB] = nullptr) {
+                                             CellLds* lds = nullptr) {
     const int32_t start = cell_start[c], end = cell_start[c + 1];
     const uint32_t key = cell_key[c];
     const int n = end - start;
@@ -460,12 +468,13 @@ __device__ __forceinline__ int32_t cell_work(int64_t c, int64_t r, bool sorted, 
 #pragma unroll
         for (int k = 0; k < 16; ++k) v[k] = k < n ? w[k] : INT32_MAX;
         kl->key_cnt[2 * (int64_t)key] = 0;
-        if (n > KEY_SMALL) {  // the cell's slots take all of them: the entry's sixteen, then the list
+        if (n > KEY_SMALL) {  // the cell's slots (or its LDS) take all of them: the entry's sixteen, then the list
+            int32_t* mm = lds && n <= MWD_MAX_CELL ? lds->m : m;
 #pragma unroll
-            for (int k = 0; k < KEY_SMALL; ++k) m[k] = v[k];
+            for (int k = 0; k < KEY_SMALL; ++k) mm[k] = v[k];
             int32_t row = kl->key_head[key];
             for (int k = KEY_SMALL; k < n; ++k) {
-                m[k] = row;
+                mm[k] = row;
                 row = kl->row_next[row];
             }
             kl->key_head[key] = -1;
@@ -534,9 +543,17 @@ __device__ __forceinline__ int32_t cell_work(int64_t c, int64_t r, bool sorted, 
             atomicOr(flags, MWD_F_CELL);
             for (int k = 0; k < n; ++k) perm[start + k] = m[k];
         } else if (sorted) {
-            if (tab) set_order(m, n, sc.row0, perm + start, tab);
+            if (lds) set_order(m, n, sc.row0, perm + start, lds->tab);
             else set_order(m, n, sc.row0, perm + start);
-        } else {  // sorted by rank (the rows are distinct): no chain of dependent moves
+        } else if (lds) {  // sorted by rank (the rows are distinct): no chain of dependent moves
+            for (int k = 0; k < n; ++k) {
+                const int32_t x = lds->m[k];
+                int rk = 0;
+                for (int j = 0; j < n; ++j) rk += lds->m[j] < x ? 1 : 0;
+                lds->srt[rk] = x;
+            }
+            set_order(lds->srt, n, sc.row0, perm + start, lds->tab);
+        } else {
             int32_t srt[MWD_MAX_CELL];
             for (int k = 0; k < n; ++k) {
                 const int32_t x = m[k];
@@ -544,8 +561,7 @@ __device__ __forceinline__ int32_t cell_work(int64_t c, int64_t r, bool sorted, 
                 for (int j = 0; j < n; ++j) rk += m[j] < x ? 1 : 0;
                 srt[rk] = x;
             }
-            if (tab) set_order(srt, n, sc.row0, perm + start, tab);
-            else set_order(srt, n, sc.row0, perm + start);
+            set_order(srt, n, sc.row0, perm + start);
         }
     }
     cell_self[r] = make_int2(start, n);
@@ -591,27 +607,30 @@ __global__ void __launch_bounds__(CELL_INFO_BLOCK) k_cell_info(int32_t* __restri
                             const int64_t* __restrict__ row_rank, int32_t* __restrict__ perm,
                             int2* __restrict__ cell_self, int2* __restrict__ cell_seg,
                             int32_t* __restrict__ cell_cnt, uint32_t* flags) {
-    __shared__ int32_t s_tab[CELL_INFO_BLOCK][2][SET_TAB];
+    __shared__ CellLds s_lds[CELL_INFO_BLOCK];
     const int64_t C = *n_cells;
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C) return;
     const int64_t r = row_rank[vals[cell_start[c]]];
     cell_cnt[r] = cell_work(c, r, true, vals, nullptr, cell_start, cell_key, scenes, S, perm, cell_self, cell_seg,
-                            flags, SegSorted{cell_key, cell_start, C}, s_tab[threadIdx.x]);
+                            flags, SegSorted{cell_key, cell_start, C}, &s_lds[threadIdx.x]);
 }
 
 // Key-table path, the per-cell work as a kernel of its own (one cell per
-// thread; the pair-count scan follows), instead of inside the scan's load.
-__global__ void k_cell_info_table(int32_t* __restrict__ members, KeyLists kl, const int32_t* __restrict__ cell_start,
-                                  const uint32_t* __restrict__ cell_key, const int4* __restrict__ key_seg,
-                                  const int64_t* __restrict__ counters, const SceneDev* __restrict__ scenes, int S,
-                                  int32_t* __restrict__ perm, int2* __restrict__ cell_self,
-                                  int2* __restrict__ cell_seg, int32_t* __restrict__ cell_cnt, uint32_t* flags) {
+// thread, 64 per block, each with its LDS for a crowded cell; the pair-count
+// scan follows), instead of inside the scan's load.
+__global__ void __launch_bounds__(CELL_INFO_BLOCK)
+k_cell_info_table(int32_t* __restrict__ members, KeyLists kl, const int32_t* __restrict__ cell_start,
+                  const uint32_t* __restrict__ cell_key, const int4* __restrict__ key_seg,
+                  const int64_t* __restrict__ counters, const SceneDev* __restrict__ scenes, int S,
+                  int32_t* __restrict__ perm, int2* __restrict__ cell_self, int2* __restrict__ cell_seg,
+                  int32_t* __restrict__ cell_cnt, uint32_t* flags) {
+    __shared__ CellLds s_lds[CELL_INFO_BLOCK];
     const int64_t C = counters[0];
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C) return;
     cell_cnt[c] = cell_work(c, c, false, members, &kl, cell_start, cell_key, scenes, S, perm, cell_self, cell_seg,
-                            flags, SegTable{key_seg, (int32_t)counters[6]});
+                            flags, SegTable{key_seg, (int32_t)counters[6]}, &s_lds[threadIdx.x]);
 }
 
 // Blocks are dispatched to the 8 XCDs in turn (block b to XCD b % 8), and
@@ -1161,7 +1180,8 @@ int mwd_launch_candidates(mwd_workspace* ws, const mwd_state* st, const int64_t*
                                 ctr,           ws->pair_cap,   ws->flags,     ws->sphere_folded != 0};
             switch (ws->tune.cell_items) {
                 case 0:  // the per-cell work as its own kernel, then the pair-count scan
-                    hipLaunchKernelGGL(k_cell_info_table, dim3(blocks_for(M)), dim3(256), 0, s, ws->vals, kl,
+                    hipLaunchKernelGGL(k_cell_info_table, dim3((unsigned)((M + CELL_INFO_BLOCK - 1) / CELL_INFO_BLOCK)),
+                                       dim3(CELL_INFO_BLOCK), 0, s, ws->vals, kl,
                                        ws->cell_start, ws->cell_key, ws->key_seg, ctr, ws->scenes_dev,
                                        ws->n_scenes, ws->perm, ws->cell_self, ws->cell_seg, ws->cell_cnt,
                                        ws->flags);
